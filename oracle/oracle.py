@@ -1,0 +1,171 @@
+"""ctypes driver for the FP64 CPU restatement (TEST INFRASTRUCTURE).
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg import
+this module, as the checker.  ``spectrum`` runs the reference algorithm
+(``cirsrtfg_wave.f:361-448`` looping ``cirsradg_wave``) over a request.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libradg_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "libradg_oracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        dp = C.POINTER(C.c_double)
+        _lib.orc_cirsradg_wave.restype = C.c_int
+        _lib.orc_spectrum.restype = C.c_int
+        _lib.orc_planck.restype = C.c_double
+        _lib.orc_planck.argtypes = [C.c_int, C.c_double, C.c_double]
+        _lib.orc_planckg.restype = C.c_double
+        _lib.orc_planckg.argtypes = [C.c_int, C.c_double, C.c_double]
+        _lib.orc_locate.restype = C.c_int
+        _lib.orc_locate.argtypes = [dp, C.c_int, C.c_double]
+    return _lib
+
+
+class _KT(C.Structure):
+    _fields_ = [("idgas", C.c_int), ("isogas", C.c_int), ("npoint", C.c_int),
+                ("np", C.c_int), ("nt", C.c_int), ("ng", C.c_int),
+                ("vmin", C.c_double), ("delv", C.c_double),
+                ("lnp", C.c_void_p), ("tk", C.c_void_p), ("g_ord", C.c_void_p),
+                ("delg", C.c_void_p), ("k", C.c_void_p)]
+
+
+class _CIA(C.Structure):
+    _fields_ = [("dnu", C.c_double), ("temps", C.c_void_p), ("kcia", C.c_void_p)]
+
+
+class _REQ(C.Structure):
+    _fields_ = [("nlayer", C.c_int), ("npath", C.c_int), ("ngas", C.c_int),
+                ("nsw", C.c_int), ("nem", C.c_int), ("inormal", C.c_int),
+                ("iray", C.c_int), ("ispace", C.c_int), ("itype", C.c_int),
+                ("flagh2p", C.c_int), ("amform", C.c_int),
+                ("dist", C.c_double), ("tsurf", C.c_double),
+                ("radius1", C.c_double), ("radextra", C.c_double),
+                ("press", C.c_void_p), ("temp", C.c_void_p), ("delh", C.c_void_p),
+                ("hfp", C.c_void_p), ("pp", C.c_void_p), ("amount", C.c_void_p),
+                ("idgas", C.c_void_p), ("isogas", C.c_void_p),
+                ("nlayin", C.c_void_p), ("imod", C.c_void_p), ("layinc", C.c_void_p),
+                ("scale", C.c_void_p), ("emtemp", C.c_void_p), ("isw", C.c_void_p),
+                ("vem", C.c_void_p), ("emissivity", C.c_void_p)]
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class _Pack:
+    """Keeps the contiguous copies alive while the C structs point at them."""
+
+    def __init__(self, tables, cia, req):
+        self.keep = []
+        kts = (_KT * max(1, len(tables)))()
+        for t, kt in enumerate(tables):
+            arrs = [_d(np.log(kt.press)), _d(kt.temp), _d(kt.g_ord), _d(kt.delg), _d(kt.k)]
+            self.keep += arrs
+            kts[t] = _KT(kt.idgas, kt.isogas, kt.npoint, kt.np_, kt.nt, kt.ng,
+                         kt.vmin, kt.delv, *[a.ctypes.data for a in arrs])
+        self.kts, self.ntab = kts, len(tables)
+        ct, kc = _d(cia.temps), _d(cia.kcia)
+        self.keep += [ct, kc]
+        self.cia = _CIA(cia.dnu, ct.ctypes.data, kc.ctypes.data)
+        L = req.nlayer
+        hfp = req.hfp if req.hfp is not None else np.zeros(L)
+        f = [_d(req.press), _d(req.temp), _d(req.delh), _d(hfp), _d(req.pp),
+             _d(req.amount), _i(req.idgas), _i(req.isogas), _i(req.nlayin),
+             _i(req.imod), _i(req.layinc), _d(req.scale), _d(req.emtemp),
+             _i(req.isw), _d(req.vem), _d(req.emissivity)]
+        self.keep += f
+        self.req = _REQ(L, req.npath, req.ngas, len(req.isw), len(req.vem),
+                        req.inormal, req.iray, req.ispace, req.itype, req.flagh2p,
+                        req.amform, req.dist, req.tsurf, req.radius1, req.radextra,
+                        *[a.ctypes.data for a in f])
+
+
+def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):
+    """Run the restatement over ``vwave`` (default: the request's grid).
+
+    Returns ``(output[nwave, npath], gtsurf[nwave, npath],
+    doutputdq[nwave, npath, nlayer, nparam])`` where ``doutputdq`` is
+    dR/d(amount) for parameters 1..ngas and dR/dT for ngas+1, indexed by the
+    path's layer position (``cirsradg_wave.f:1240-1286``).
+    """
+    lb = lib()
+    pk = _Pack(tables, cia, req)
+    vw = _d(req.vwave if vwave is None else vwave)
+    nw, npath, L, npar = vw.shape[0], req.npath, req.nlayer, req.ngas + 1
+    out = np.zeros((nw, npath))
+    gts = np.zeros((nw, npath))
+    dq = np.zeros((nw, npath, L, npar))
+    rc = lb.orc_spectrum(pk.kts, pk.ntab, C.byref(pk.cia), C.byref(pk.req),
+                         vw.ctypes.data_as(C.c_void_p), C.c_int(nw), C.c_int(jac_mode),
+                         C.c_int(nthreads), out.ctypes.data_as(C.c_void_p),
+                         gts.ctypes.data_as(C.c_void_p), dq.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"oracle returned status {rc}")
+    return out, gts, dq
+
+
+def noverlapg(delg, amount, k_gn, dkgndt):
+    """``noverlapg.f`` on one layer: returns ``k_g[ng]``, ``dkdq[ngas+1, ng]``."""
+    lb = lib()
+    k_gn = _d(k_gn)
+    dkgndt = _d(dkgndt)
+    ngas, ng = k_gn.shape
+    kg = np.zeros(ng)
+    dk = np.zeros((ngas + 1, ng))
+    lb.orc_noverlapg(_d(delg).ctypes.data_as(C.c_void_p), C.c_int(ng), C.c_int(ngas),
+                     _d(amount).ctypes.data_as(C.c_void_p), k_gn.ctypes.data_as(C.c_void_p),
+                     dkgndt.ctypes.data_as(C.c_void_p), kg.ctypes.data_as(C.c_void_p),
+                     dk.ctypes.data_as(C.c_void_p))
+    return kg, dk
+
+
+def planck(ispace, v, t):
+    return lib().orc_planck(int(ispace), float(v), float(t))
+
+
+def planckg(ispace, v, t):
+    return lib().orc_planckg(int(ispace), float(v), float(t))
+
+
+def sort2g(ra):
+    """Heapsort of ``ra`` (sort2g.f); returns sorted keys and 1-based order."""
+    n = len(ra)
+    a = np.zeros(n + 1)
+    a[1:] = ra
+    ib = np.zeros(n + 1, dtype=np.int32)
+    ib[1:] = np.arange(1, n + 1)
+    lib().orc_sort2g(C.c_int(n), a.ctypes.data_as(C.c_void_p), ib.ctypes.data_as(C.c_void_p))
+    return a[1:], ib[1:]
+
+
+def ciaread(cia, vv, temp):
+    lb = lib()
+    ct, kc = _d(cia.temps), _d(cia.kcia)
+    c = _CIA(cia.dnu, ct.ctypes.data, kc.ctypes.data)
+    a = np.zeros(9)
+    d = np.zeros(9)
+    lb.orc_ciaread(C.byref(c), C.c_double(vv), C.c_double(temp),
+                   a.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p))
+    return a, d

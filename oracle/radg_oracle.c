@@ -1,0 +1,1053 @@
+/*
+ * FP64 CPU restatement of CIRSRADG_WAVE -- TEST INFRASTRUCTURE (the oracle).
+ *
+ * Only tests/, the smoke check and the benchmark's CPU-baseline leg use this
+ * file, as the checker.  It restates, in plain C and in the reference's own
+ * loop order, the routines on the path:
+ *
+ *   locate      frecipes/locate.f:1-16
+ *   sort2g      radtran/cirsradg/sort2g.f:1-52
+ *   rankg       radtran/cirsradg/rankg.f:1-156
+ *   rankk       radtran/cirsradg/rankk.f:17-120
+ *   noverlapg   radtran/cirsradg/noverlapg.f:1-202
+ *   get_kg      radtran/cirsradg/get_kg.f:1-484 (+ read_klist.f:260-330 lookup)
+ *   pindex      radtran/ciatable/pindex.f:1-32
+ *   ciaread     radtran/radtran/ciaread.f:104-196
+ *   nciacon     radtran/radtran/nciacon.f:1-310
+ *   co2cia      radtran/radtran/co2cia.f
+ *   planck      radtran/radtran/planck_wave.f:1-42, planckg_wave.f:1-40
+ *   interpem    radtran/cirsrad/interpem.f:1-37
+ *   cirsradg    radtran/cirsradg/cirsradg_wave.f:333-1290 (imod 0, 1, 3)
+ *
+ * Everything that the reference declares REAL is double here, matching the
+ * reference compiled with REAL promoted to 8 bytes.
+ */
+#include "radg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXLAY 400
+#define MAXPAT 200
+#define MAXGAS 30
+#define MAXG 51
+#define MAXK 30
+#define CIA_NUMWN 1501
+#define CIA_NUMT 25
+#define CIA_NPAIRS 9
+
+/* constdef.f:7-11 */
+#define AMAGAT 2.68675e19
+#define KBOLTZMANN 1.37947e-23
+#define PI_REF 3.1415927
+
+/* ---------------------------------------------------------------- locate */
+/* frecipes/locate.f:1-16.  xx is 0-based here; the return value is the
+ * reference's 1-based JL in 0..n. */
+int orc_locate(const double *xx, int n, double x) {
+  int jl = 0, ju = n + 1;
+  int ascnd = xx[n - 1] > xx[0];
+  while (ju - jl > 1) {
+    int jm = (ju + jl) / 2;
+    if (ascnd == (x > xx[jm - 1]))
+      jl = jm;
+    else
+      ju = jm;
+  }
+  return jl;
+}
+
+/* ---------------------------------------------------------------- sort2g */
+/* sort2g.f:1-52: heapsort of ra(1:n) carrying ib.  ra/ib are 1-based views
+ * (element 0 unused). */
+void orc_sort2g(int n, double *ra, int *ib) {
+  int l, ir, i, j, irb;
+  double rra;
+  if (n < 2) return;
+  l = n / 2 + 1;
+  ir = n;
+  for (;;) {
+    if (l > 1) {
+      l = l - 1;
+      rra = ra[l];
+      irb = ib[l];
+    } else {
+      rra = ra[ir];
+      irb = ib[ir];
+      ra[ir] = ra[1];
+      ib[ir] = ib[1];
+      ir = ir - 1;
+      if (ir == 1) {
+        ra[1] = rra;
+        ib[1] = irb;
+        return;
+      }
+    }
+    i = l;
+    j = l + l;
+    while (j <= ir) {
+      if (j < ir) {
+        if (ra[j] < ra[j + 1]) j = j + 1;
+      }
+      if (rra < ra[j]) {
+        ra[i] = ra[j];
+        ib[i] = ib[j];
+        i = j;
+        j = j + j;
+      } else {
+        j = ir + 1;
+      }
+    }
+    ra[i] = rra;
+    ib[i] = irb;
+  }
+}
+
+/* ----------------------------------------------------------------- rankg */
+/* rankg.f:1-156.  weight/cont are 1-based (index 0 unused), length nloop;
+ * grad is 1-based in the element index with leading dimension grad_ld:
+ * grad[iparam*grad_ld + I], iparam 0-based.  k_g[ig] 1-based (size ng+2),
+ * dkdq[iparam*dkdq_ld + ig] 1-based in ig. */
+void orc_rankg(int nparam, const double *gw, int ng, int nloop,
+               double *weight, double *cont, double *grad, int grad_ld,
+               double *k_g, double *dkdq, int dkdq_ld) {
+  double g_ord[MAXG + 2];
+  double *gdist = (double *)malloc(sizeof(double) * (nloop + 1));
+  double *tmp = (double *)malloc(sizeof(double) * (nloop + 1));
+  int *ico = (int *)malloc(sizeof(int) * (nloop + 1));
+  int i, ig, ip;
+  double sum, frac;
+
+  g_ord[1] = 0.0;
+  for (i = 1; i <= ng; i++) g_ord[i + 1] = g_ord[i] + gw[i - 1];
+  if (g_ord[ng + 1] < 1.0) g_ord[ng + 1] = 1.0;
+  /* g_ord(ng+2) is never initialised by the reference; it is only read if an
+   * element lies wholly beyond the last boundary, which cannot happen when
+   * the total weight is (sum gw)^2 <= g_ord(ng+1). */
+  g_ord[ng + 2] = g_ord[ng + 1];
+
+  for (i = 1; i <= nloop; i++) ico[i] = i;
+  orc_sort2g(nloop, cont, ico);
+
+  for (i = 1; i <= nloop; i++) tmp[i] = weight[ico[i]];
+  for (i = 1; i <= nloop; i++) weight[i] = tmp[i];
+  for (ip = 0; ip < nparam; ip++) {
+    double *gp = grad + (size_t)ip * grad_ld;
+    for (i = 1; i <= nloop; i++) tmp[i] = gp[ico[i]];
+    for (i = 1; i <= nloop; i++) gp[i] = tmp[i];
+  }
+
+  gdist[0] = 0.0;
+  gdist[1] = weight[1];
+  for (i = 2; i <= nloop; i++) gdist[i] = weight[i] + gdist[i - 1];
+
+  for (i = 1; i <= ng + 1; i++) {
+    k_g[i] = 0.0;
+    for (ip = 0; ip < nparam; ip++) dkdq[ip * dkdq_ld + i] = 0.0;
+  }
+
+  ig = 1;
+  sum = 0.0;
+  for (i = 1; i <= nloop; i++) {
+    if (gdist[i] < g_ord[ig + 1]) {
+      k_g[ig] = k_g[ig] + cont[i] * weight[i];
+      for (ip = 0; ip < nparam; ip++)
+        dkdq[ip * dkdq_ld + ig] += grad[(size_t)ip * grad_ld + i] * weight[i];
+      sum = sum + weight[i];
+    } else {
+      frac = (g_ord[ig + 1] - gdist[i - 1]) / (gdist[i] - gdist[i - 1]);
+      k_g[ig] = k_g[ig] + frac * cont[i] * weight[i];
+      for (ip = 0; ip < nparam; ip++)
+        dkdq[ip * dkdq_ld + ig] += frac * grad[(size_t)ip * grad_ld + i] * weight[i];
+      sum = sum + frac * weight[i];
+      k_g[ig] = k_g[ig] / sum;
+      for (ip = 0; ip < nparam; ip++) dkdq[ip * dkdq_ld + ig] /= sum;
+      ig = ig + 1;
+      sum = (1.0 - frac) * weight[i];
+      if (ig <= ng + 1) {
+        k_g[ig] = (1.0 - frac) * cont[i] * weight[i];
+        for (ip = 0; ip < nparam; ip++)
+          dkdq[ip * dkdq_ld + ig] = (1.0 - frac) * grad[(size_t)ip * grad_ld + i] * weight[i];
+      }
+      if (ig > ng) {
+        /* The remaining elements would index g_ord beyond ng+1. */
+        break;
+      }
+    }
+  }
+  if (ig == ng) {
+    k_g[ig] = k_g[ig] / sum;
+    for (ip = 0; ip < nparam; ip++) dkdq[ip * dkdq_ld + ig] /= sum;
+  }
+  free(gdist);
+  free(tmp);
+  free(ico);
+}
+
+/* ----------------------------------------------------------------- rankk */
+/* rankk.f:17-120 (non-coincident wavenumber).  1-based arrays. */
+static void orc_rankk(const double *gw, int ng, double *cont, double *dcont,
+                      double *weight, int nloop, double *k_g, double *dkdt) {
+  double g_ord[MAXG + 2], gdist[2 * MAXG + 2], tmp[2 * MAXG + 2];
+  int ico[2 * MAXG + 2];
+  int i, ig;
+  double sum, frac;
+  g_ord[1] = 0.0;
+  for (i = 1; i <= ng; i++) g_ord[i + 1] = g_ord[i] + gw[i - 1];
+  if (g_ord[ng + 1] < 1.0) g_ord[ng + 1] = 1.0;
+  for (i = 1; i <= nloop; i++) ico[i] = i;
+  orc_sort2g(nloop, cont, ico);
+  for (i = 1; i <= nloop; i++) tmp[i] = weight[ico[i]];
+  for (i = 1; i <= nloop; i++) weight[i] = tmp[i];
+  for (i = 1; i <= nloop; i++) tmp[i] = dcont[ico[i]];
+  for (i = 1; i <= nloop; i++) dcont[i] = tmp[i];
+  gdist[0] = 0.0;
+  gdist[1] = weight[1];
+  for (i = 2; i <= nloop; i++) gdist[i] = weight[i] + gdist[i - 1];
+  for (i = 1; i <= ng + 1; i++) {
+    k_g[i] = 0.0;
+    dkdt[i] = 0.0;
+  }
+  ig = 1;
+  sum = 0.0;
+  for (i = 1; i <= nloop; i++) {
+    if (ig <= ng && gdist[i] <= g_ord[ig + 1]) {
+      k_g[ig] += cont[i] * weight[i];
+      dkdt[ig] += dcont[i] * weight[i];
+      sum += weight[i];
+    } else {
+      if (ig > ng) break; /* reference writes past ng only */
+      frac = (g_ord[ig + 1] - gdist[i - 1]) / (gdist[i] - gdist[i - 1]);
+      k_g[ig] += frac * cont[i] * weight[i];
+      dkdt[ig] += frac * dcont[i] * weight[i];
+      sum += frac * weight[i];
+      k_g[ig] /= sum;
+      dkdt[ig] /= sum;
+      ig = ig + 1;
+      if (ig <= ng) {
+        sum = (1.0 - frac) * weight[i];
+        k_g[ig] = (1.0 - frac) * cont[i] * weight[i];
+        dkdt[ig] = (1.0 - frac) * dcont[i] * weight[i];
+      }
+    }
+  }
+  if (ig == ng) {
+    k_g[ig] /= sum;
+    dkdt[ig] /= sum;
+  }
+}
+
+/* ------------------------------------------------------------- noverlapg */
+/* noverlapg.f:1-202.  k_gn/dkgndt are [gas][ng]; k_g is [ng]; dkdq is
+ * [param][ng] with ngas+1 params (the last one is T). */
+void orc_noverlapg(const double *delg, int ng, int ngas, const double *amount,
+                   const double *k_gn, const double *dkgndt, double *k_g_out,
+                   double *dkdq_out) {
+  const int nl = ng * ng;
+  const int ld = ng + 2; /* 1-based rows of k_g / dkdq */
+  double k_g[MAXG + 2], dkdq[(MAXGAS + 1) * (MAXG + 2)];
+  double k1[MAXG], k2[MAXG], d1[MAXG], d2[MAXG];
+  double *weight = (double *)malloc(sizeof(double) * (nl + 1));
+  double *contri = (double *)malloc(sizeof(double) * (nl + 1));
+  double *grad = (double *)malloc(sizeof(double) * (size_t)(nl + 1) * (ngas + 1));
+  int igas, i, j, jg, nloop;
+  double a1, a2;
+
+  memset(k_g, 0, sizeof(k_g));
+  memset(dkdq, 0, sizeof(dkdq));
+
+  if (ngas == 1) {
+    /* The reference leaves k_g/dkdq unwritten for one gas (the DO loop
+     * noverlapg.f:64 is empty) and cirsradg_wave then reads stale values.
+     * Defined here as the single scaled distribution. */
+    for (i = 1; i <= ng; i++) {
+      k_g[i] = k_gn[i - 1] * amount[0];
+      dkdq[0 * ld + i] = k_gn[i - 1];
+      dkdq[1 * ld + i] = dkgndt[i - 1] * amount[0];
+    }
+  }
+
+  for (igas = 1; igas <= ngas - 1; igas++) {
+    /* igas, jgas: 1-based gas numbers; dkdq column p (1-based) at (p-1)*ld */
+    if (igas == 1) {
+      a1 = amount[0];
+      a2 = amount[1];
+      for (i = 0; i < ng; i++) {
+        k1[i] = k_gn[0 * ng + i];
+        k2[i] = k_gn[1 * ng + i];
+        d1[i] = dkgndt[0 * ng + i];
+        d2[i] = dkgndt[1 * ng + i];
+      }
+      if (k1[ng - 1] == 0.0) {
+        for (i = 1; i <= ng; i++) {
+          k_g[i] = k2[i - 1] * a2;
+          dkdq[(igas - 1) * ld + i] = 0.0;
+          dkdq[(igas)*ld + i] = k2[i - 1];
+          dkdq[(igas + 1) * ld + i] = d2[i - 1] * a2;
+        }
+        continue;
+      }
+      if (k2[ng - 1] == 0.0) {
+        for (i = 1; i <= ng; i++) {
+          k_g[i] = k1[i - 1] * a1;
+          dkdq[(igas - 1) * ld + i] = k1[i - 1];
+          dkdq[(igas)*ld + i] = 0.0;
+          dkdq[(igas + 1) * ld + i] = d1[i - 1] * a1;
+        }
+        continue;
+      }
+      nloop = 0;
+      for (i = 0; i < ng; i++) {
+        for (j = 0; j < ng; j++) {
+          nloop++;
+          weight[nloop] = delg[i] * delg[j];
+          contri[nloop] = k1[i] * a1 + k2[j] * a2;
+          grad[(size_t)(igas - 1) * (nl + 1) + nloop] = k1[i];
+          grad[(size_t)(igas) * (nl + 1) + nloop] = k2[j];
+          grad[(size_t)(igas + 1) * (nl + 1) + nloop] = d1[i] * a1 + d2[j] * a2;
+        }
+      }
+    } else {
+      a2 = amount[igas];
+      for (i = 0; i < ng; i++) {
+        k1[i] = k_g[i + 1];
+        k2[i] = k_gn[igas * ng + i];
+        d1[i] = dkdq[(igas)*ld + i + 1]; /* dK/dT of the running sum */
+        d2[i] = dkgndt[igas * ng + i];
+      }
+      if (k1[ng - 1] == 0.0) {
+        for (i = 1; i <= ng; i++) {
+          k_g[i] = k2[i - 1] * a2;
+          for (jg = 1; jg <= igas; jg++) dkdq[(jg - 1) * ld + i] = 0.0;
+          dkdq[(igas)*ld + i] = k2[i - 1];
+          dkdq[(igas + 1) * ld + i] = d2[i - 1] * a2;
+        }
+        continue;
+      }
+      if (k2[ng - 1] == 0.0) {
+        for (i = 1; i <= ng; i++) {
+          k_g[i] = k1[i - 1];
+          dkdq[(igas)*ld + i] = 0.0;
+          dkdq[(igas + 1) * ld + i] = d1[i - 1];
+        }
+        continue;
+      }
+      nloop = 0;
+      for (i = 0; i < ng; i++) {
+        for (j = 0; j < ng; j++) {
+          nloop++;
+          weight[nloop] = delg[i] * delg[j];
+          contri[nloop] = k1[i] + k2[j] * a2;
+          for (jg = 1; jg <= igas; jg++)
+            grad[(size_t)(jg - 1) * (nl + 1) + nloop] = dkdq[(jg - 1) * ld + i + 1];
+          grad[(size_t)(igas) * (nl + 1) + nloop] = k2[j];
+          grad[(size_t)(igas + 1) * (nl + 1) + nloop] = d1[i] + d2[j] * a2;
+        }
+      }
+    }
+    orc_rankg(igas + 2, delg, ng, nloop, weight, contri, grad, nl + 1, k_g, dkdq, ld);
+  }
+
+  for (i = 0; i < ng; i++) {
+    k_g_out[i] = k_g[i + 1];
+    for (jg = 0; jg < ngas + 1; jg++) dkdq_out[jg * ng + i] = dkdq[jg * ld + i + 1];
+  }
+  free(weight);
+  free(contri);
+  free(grad);
+}
+
+/* ---------------------------------------------------------------- planck */
+/* planck_wave.f:1-42 (FP64 constants, as in the REAL(8) build). */
+double orc_planck(int ispace, double v, double t) {
+  const double c1 = 1.1911e-12, c2 = 1.439;
+  double y, a;
+  if (ispace == 0) {
+    y = v;
+    a = c1 * (y * y * y);
+  } else {
+    y = 1e4 / v;
+    a = c1 * (y * y * y * y * y) / 1e4;
+  }
+  if (y == 0.0) return 0.0;
+  return a / (exp(c2 * y / t) - 1.0);
+}
+
+/* planckg_wave.f:1-40 */
+double orc_planckg(int ispace, double v, double t) {
+  const double c1 = 1.1911e-12, c2 = 1.439;
+  double y, a, tmp, bot, top;
+  if (ispace == 0) {
+    y = v;
+    a = c1 * c2 * (y * y * y * y) / (t * t);
+  } else {
+    y = 1e4 / v;
+    a = c1 * c2 * (y * y * y * y * y * y) * 1e-4 / (t * t);
+  }
+  tmp = c2 * y / t;
+  bot = (exp(tmp) - 1.0) * (exp(tmp) - 1.0);
+  top = exp(tmp) * a;
+  return top / bot;
+}
+
+/* ----------------------------------------------------------------- pindex */
+/* pindex.f:1-32; x1 is clamped in place (the reference's side effect). */
+static int orc_pindex(const double *x, int nx, double *x1) {
+  int i, j = 0;
+  if (*x1 > x[nx - 1]) *x1 = x[nx - 1];
+  if (*x1 < x[0]) *x1 = x[0];
+  for (i = 1; i <= nx - 1; i++) {
+    if (*x1 >= x[i - 1])
+      j = i;
+    else
+      break;
+  }
+  return j;
+}
+
+/* ---------------------------------------------------------------- ciaread */
+/* ciaread.f:104-196.  alpha/dalphadt are [9]. */
+void orc_ciaread(const orc_cia *cia, double vv, double temp, double *alpha,
+                 double *dalphadt) {
+  double vvk1_lo = 0.0, vvk1_hi = (CIA_NUMWN - 1) * cia->dnu;
+  double temp1;
+  int cv, ct, i;
+  for (i = 0; i < CIA_NPAIRS; i++) {
+    alpha[i] = 0.0;
+    dalphadt[i] = 0.0;
+  }
+  if (vv > vvk1_hi || vv < vvk1_lo) return;
+  /* ciaread.f:133-157: the second test overwrites the first, so only the
+   * lower clamp survives here; pindex then clamps both ends in place. */
+  if (temp > cia->temps[CIA_NUMT - 1]) temp1 = cia->temps[CIA_NUMT - 1];
+  else temp1 = temp;
+  if (temp < cia->temps[0]) temp1 = cia->temps[0];
+  else temp1 = temp;
+  {
+    /* vvk1(i) = (i-1)*dnu1 (ciaread.f:96-98); pindex over that grid. */
+    double vtmp = vv;
+    cv = 0;
+    for (i = 1; i <= CIA_NUMWN - 1; i++) {
+      if (vtmp >= (i - 1) * cia->dnu) cv = i;
+      else break;
+    }
+  }
+  ct = orc_pindex(cia->temps, CIA_NUMT, &temp1);
+  for (i = 0; i < CIA_NPAIRS; i++) {
+    /* kcia(pair, T, wn) == kcia[(wn*25 + T)*9 + pair], 0-based */
+    const double *kc = cia->kcia;
+    double abs11 = kc[((size_t)(cv - 1) * CIA_NUMT + (ct - 1)) * CIA_NPAIRS + i];
+    double abs12 = kc[((size_t)(cv)*CIA_NUMT + (ct - 1)) * CIA_NPAIRS + i];
+    double abs22 = kc[((size_t)(cv)*CIA_NUMT + (ct)) * CIA_NPAIRS + i];
+    double abs21 = kc[((size_t)(cv - 1) * CIA_NUMT + (ct)) * CIA_NPAIRS + i];
+    double v_lo = (cv - 1) * cia->dnu, v_hi = cv * cia->dnu;
+    double t = (vv - v_lo) / (v_hi - v_lo);
+    double u = (temp1 - cia->temps[ct - 1]) / (cia->temps[ct] - cia->temps[ct - 1]);
+    double dudt = 1.0 / (cia->temps[ct] - cia->temps[ct - 1]);
+    alpha[i] = (1.0 - t) * (1.0 - u) * abs11 + t * (1.0 - u) * abs12 + t * u * abs22 +
+               (1.0 - t) * u * abs21;
+    dalphadt[i] = (-(1.0 - t) * abs11 - t * abs12 + t * abs22 + (1.0 - t) * abs21) * dudt;
+  }
+}
+
+/* co2cia.f */
+static double orc_co2cia(double v0) {
+  double xl = 1e4 / v0, a = 0.0;
+  if (xl >= 2.15 && xl <= 2.55) a = 4e-8;
+  if (xl >= 1.7 && xl <= 1.76) a = 6e-9;
+  if (xl >= 1.25 && xl < 1.35) a = 1.5e-9;
+  if (xl >= 1.125 && xl <= 1.225) a = 0.5 * (0.31 + 0.79) * 1e-9;
+  if (xl >= 1.06 && xl < 1.125) a = 0.5 * (0.29 + 0.67) * 1e-9;
+  return a;
+}
+
+/* ---------------------------------------------------------------- nciacon */
+/* nciacon.f:1-310.  iabs[5] are 1-based gas numbers (0 = absent),
+ * dabsorb[7].  N2 (id 22) is rejected by the caller, so the analytic N2-N2
+ * and N2-H2 terms, which carry a factor qn2, vanish. */
+void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
+                 int ngas, const int *idgas, const int *isogas,
+                 const double *amount, const double *pp, double xlen,
+                 double *absorb, int *iabs, double *dabsorb) {
+  const double P0 = 1.0, T0 = 273.15;
+  const double MODBOLTZA = 10.0 * KBOLTZMANN / 1.013;
+  double qh2 = 0, qhe = 0, qn2 = 0, qch4 = 0, qco2 = 0;
+  double alpha[CIA_NPAIRS], dalpha[CIA_NPAIRS];
+  double xp1, height = 0.0, tau, totam = 0.0, amag1, sum, aco2;
+  int i;
+  for (i = 0; i < 5; i++) {
+    iabs[i] = 0;
+    dabsorb[i] = 0.0;
+  }
+  dabsorb[5] = 0.0;
+  dabsorb[6] = 0.0;
+  for (i = 0; i < ngas; i++) {
+    if (idgas[i] == 39 && (isogas[i] == 0 || isogas[i] == 1)) {
+      qh2 = pp[i] / p;
+      iabs[0] = i + 1;
+    }
+    if (idgas[i] == 40) {
+      qhe = pp[i] / p;
+      iabs[1] = i + 1;
+    }
+    if (idgas[i] == 22) {
+      qn2 = pp[i] / p;
+      iabs[2] = i + 1;
+    }
+    if (idgas[i] == 6 && (isogas[i] == 0 || isogas[i] == 1)) {
+      qch4 = pp[i] / p;
+      iabs[3] = i + 1;
+    }
+    if (idgas[i] == 2) {
+      qco2 = pp[i] / p;
+      iabs[4] = i + 1;
+    }
+  }
+  if (xlen == 0.0) {
+    xp1 = 0.0;
+    for (i = 0; i < ngas; i++) {
+      if (pp[i] > xp1) {
+        height = amount[i] * MODBOLTZA * t / pp[i];
+        xp1 = pp[i];
+      }
+    }
+    tau = ((p / P0) * (p / P0)) * ((T0 / t) * (T0 / t)) * height;
+  } else {
+    height = xlen * 1e5;
+    xp1 = -1.0;
+    for (i = 0; i < ngas; i++) {
+      if (pp[i] > xp1 && pp[i] > 0.0) {
+        totam = amount[i] * p / pp[i];
+        xp1 = pp[i];
+      }
+    }
+    amag1 = (totam / height) / AMAGAT;
+    tau = height * amag1 * amag1;
+  }
+  sum = 0.0;
+  orc_ciaread(cia, v0, t, alpha, dalpha);
+  if (inormal == 0) {
+    sum = sum + (alpha[0] * qh2 * qh2) + (alpha[1] * qh2 * qhe);
+    dabsorb[0] += 2 * qh2 * alpha[0] + qhe * alpha[1];
+    dabsorb[1] += qh2 * alpha[1];
+    dabsorb[5] += qh2 * qh2 * dalpha[0] + qh2 * qhe * dalpha[1];
+  } else {
+    sum = sum + (alpha[2] * qh2 * qh2) + (alpha[3] * qh2 * qhe);
+    dabsorb[0] += 2 * qh2 * alpha[2] + qhe * alpha[3];
+    dabsorb[1] += qh2 * alpha[3];
+    dabsorb[5] += qh2 * qh2 * dalpha[2] + qh2 * qhe * dalpha[3];
+  }
+  sum = sum + (alpha[4] * qh2 * qn2);
+  dabsorb[0] += qn2 * alpha[4];
+  dabsorb[2] += qh2 * alpha[4];
+  dabsorb[5] += qh2 * qn2 * dalpha[4];
+  sum = sum + (alpha[5] * qn2 * qch4);
+  dabsorb[2] += qch4 * alpha[5];
+  dabsorb[3] += qn2 * alpha[5];
+  dabsorb[5] += qn2 * qch4 * dalpha[5];
+  sum = sum + (alpha[6] * qn2 * qn2);
+  dabsorb[2] += 2 * qn2 * alpha[6];
+  dabsorb[5] += qn2 * qn2 * dalpha[6];
+  sum = sum + (alpha[7] * qch4 * qch4);
+  dabsorb[3] += 2 * qch4 * alpha[7];
+  dabsorb[5] += qch4 * qch4 * dalpha[7];
+  sum = sum + (alpha[8] * qh2 * qch4);
+  dabsorb[0] += qch4 * alpha[8];
+  dabsorb[3] += qh2 * alpha[8];
+  dabsorb[5] += qh2 * qch4 * dalpha[8];
+  aco2 = orc_co2cia(v0);
+  sum = sum + aco2 * qco2 * qco2;
+  dabsorb[4] += 2 * qco2 * aco2;
+  *absorb = sum * tau;
+  for (i = 0; i < 7; i++) dabsorb[i] *= tau;
+}
+
+/* --------------------------------------------------------------- interpem */
+static int orc_interpem(int nem, const double *vem, const double *em, double x,
+                        double *out) {
+  int i;
+  double v = 0.0;
+  if (nem < 1 || x < vem[0] || x > vem[nem - 1]) return ORC_ERR_EMISSIVITY;
+  for (i = 0; i < nem - 1; i++) {
+    if (x >= vem[i] && x < vem[i + 1]) {
+      double f = (x - vem[i]) / (vem[i + 1] - vem[i]);
+      v = (1 - f) * em[i] + f * em[i + 1];
+    }
+  }
+  if (x == vem[nem - 1]) v = em[nem - 1];
+  *out = v;
+  return ORC_OK;
+}
+
+/* ----------------------------------------------------------------- get_kg */
+/* Table for gas igas at vwave: the first registered table with the gas's
+ * (id, iso) whose grid covers vwave (read_klist.f:260-290, findloc.f). */
+static const orc_ktable *find_table(const orc_ktable *tabs, int ntab, int id,
+                                    int iso, double vwave) {
+  int t;
+  for (t = 0; t < ntab; t++) {
+    const orc_ktable *kt = &tabs[t];
+    double xmax = kt->vmin + (kt->npoint - 1) * kt->delv;
+    if (kt->idgas != id || kt->isogas != iso || kt->delv <= 0.0) continue;
+    if (vwave >= kt->vmin - 1e-5 && vwave <= xmax + 1e-5) return kt;
+  }
+  return NULL;
+}
+
+/* get_kg.f:1-484 for one gas at all layers.  kout/dkoutdt are [L][ng]. */
+static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press,
+                           const double *temp, double vwave, double *kout,
+                           double *dkoutdt) {
+  const int np = kt->np, nt = kt->nt, ng = kt->ng;
+  const size_t ntab = (size_t)np * nt * ng;
+  const double pmin = kt->lnp[0], pmax = kt->lnp[np - 1];
+  const double tmin = kt->tk[0], tmax = kt->tk[nt - 1];
+  double eps, tmp, frac;
+  long n1;
+  int coinc, il, i;
+  const double *table, *table2;
+  double ktest = 0.0;
+
+  if (kt->delv >= 0.0001) eps = 0.02 * kt->delv;
+  else eps = 0.25 * kt->delv;
+  n1 = (long)((vwave + eps - kt->vmin) / kt->delv); /* IDINT truncates */
+  if (n1 < 0) n1 = 0;
+  if (n1 > kt->npoint - 1) n1 = kt->npoint - 1;
+  table = kt->k + (size_t)n1 * ntab;
+
+  for (i = 0; i < (int)ntab; i++) {
+    if (table[i] > 0.0) {
+      ktest = table[i];
+      break;
+    }
+  }
+  if (ktest == 0.0) {
+    memset(kout, 0, sizeof(double) * nlayer * ng);
+    memset(dkoutdt, 0, sizeof(double) * nlayer * ng);
+    return;
+  }
+  tmp = kt->vmin + kt->delv * n1;
+  frac = fabs(vwave - tmp);
+  coinc = frac < eps;
+  table2 = (n1 + 1 < kt->npoint) ? table + ntab : table;
+
+  for (il = 0; il < nlayer; il++) {
+    double p1 = log(press[il]), t1, v, u, dudt;
+    int cp, ct, i1, i2, i3, i4;
+    double kg[MAXG + 2], dkdt[MAXG + 2];
+    if (p1 < pmin) p1 = pmin;
+    if (p1 > pmax) p1 = pmax;
+    cp = orc_locate(kt->lnp, np, p1);
+    if (cp < 1) cp = 1;
+    if (cp >= np) cp = np - 1;
+    v = (p1 - kt->lnp[cp - 1]) / (kt->lnp[cp] - kt->lnp[cp - 1]);
+    t1 = temp[il];
+    if (t1 < tmin) t1 = tmin;
+    if (t1 > tmax) t1 = tmax;
+    ct = orc_locate(kt->tk, nt, t1);
+    if (ct < 1) ct = 1;
+    if (ct >= nt) ct = nt - 1;
+    u = (t1 - kt->tk[ct - 1]) / (kt->tk[ct] - kt->tk[ct - 1]);
+    dudt = 1.0 / (kt->tk[ct] - kt->tk[ct - 1]);
+    /* IOFF, get_kg.f:140-143 (0-based offsets) */
+    i1 = (nt * ng * (cp - 1)) + (ng * (ct - 1));
+    i2 = (nt * ng * cp) + (ng * (ct - 1));
+    i3 = (nt * ng * cp) + (ng * ct);
+    i4 = (nt * ng * (cp - 1)) + (ng * ct);
+
+    if (coinc) {
+      int klog = !(table[i1] <= 0.0 || table[i2] <= 0.0 || table[i3] <= 0.0 ||
+                   table[i4] <= 0.0);
+      for (i = 0; i < ng; i++) {
+        double y1 = table[i1 + i], y2 = table[i2 + i], y3 = table[i3 + i], y4 = table[i4 + i];
+        double x, dxdt;
+        if (klog) {
+          y1 = log(y1);
+          y2 = log(y2);
+          y3 = log(y3);
+          y4 = log(y4);
+        }
+        x = (1.0 - v) * (1.0 - u) * y1 + v * (1.0 - u) * y2 + v * u * y3 + (1.0 - v) * u * y4;
+        dxdt = (-(1.0 - v) * y1 - v * y2 + v * y3 + (1 - v) * y4) * dudt;
+        if (klog) {
+          kg[i + 1] = exp(x);
+          dkdt[i + 1] = exp(x) * dxdt;
+        } else {
+          kg[i + 1] = x;
+          dkdt[i + 1] = dxdt;
+        }
+      }
+    } else {
+      double weight2 = (vwave - tmp) / kt->delv, w[2];
+      double cont[2 * MAXG + 2], dcont[2 * MAXG + 2], fac[2 * MAXG + 2];
+      int loop, count = 0;
+      if (weight2 > 1.0) weight2 = 1.0;
+      if (weight2 < 0.0) weight2 = 0.0;
+      w[0] = 1.0 - weight2;
+      w[1] = weight2;
+      for (loop = 0; loop < 2; loop++) {
+        const double *tb = loop == 0 ? table : table2;
+        for (i = 0; i < ng; i++) {
+          double y1 = tb[i1 + i], y2 = tb[i2 + i], y3 = tb[i3 + i], y4 = tb[i4 + i];
+          double x = (1.0 - v) * (1.0 - u) * y1 + v * (1.0 - u) * y2 + v * u * y3 +
+                     (1.0 - v) * u * y4;
+          double dxdt = (-(1.0 - v) * y1 - v * y2 + v * y3 + (1 - v) * y4) * dudt;
+          count++;
+          cont[count] = x;
+          dcont[count] = dxdt;
+          fac[count] = kt->delg[i] * w[loop];
+        }
+      }
+      orc_rankk(kt->delg, ng, cont, dcont, fac, count, kg, dkdt);
+    }
+    for (i = 0; i < ng; i++) {
+      kout[il * ng + i] = isnan(kg[i + 1]) ? 1e-37 : kg[i + 1];
+      dkoutdt[il * ng + i] = isnan(dkdt[i + 1]) ? 1e-37 : dkdt[i + 1];
+    }
+  }
+}
+
+/* ------------------------------------------------------------ cirsradg */
+int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
+                      const orc_req *r, double vwave, int jac_mode,
+                      double *output, double *gtsurf, double *doutputdq) {
+  const int L = r->nlayer, ngas = r->ngas, npath = r->npath;
+  int ng = 0, nparam, ip, j, k, k1, l, ig, rc = ORC_OK;
+  const double *delg = NULL;
+  const orc_ktable *gtab[MAXGAS];
+  double esurf = 0.0, vv = vwave;
+  double *kout = NULL, *dkoutdt = NULL, *utotl = NULL, *taucon = NULL,
+         *dtaucondq = NULL, *kl_g = NULL, *dkdql = NULL, *bb = NULL, *dbdt = NULL,
+         *corkout = NULL, *tempgtsurf = NULL, *dcoutdq = NULL, *tautmp = NULL,
+         *dtautmpdq = NULL, *taus = NULL, *dtaudq = NULL, *dtolddq = NULL,
+         *dtrdq = NULL, *trv = NULL, *wv = NULL, *tlv = NULL;
+
+  /* check_limits.f:10-78 */
+  if (L > MAXLAY || L < 1 || npath > MAXPAT || npath < 1 || ngas > MAXGAS || ngas < 1)
+    return ORC_ERR_LIMITS;
+  for (ip = 0; ip < npath; ip++) {
+    if (r->nlayin[ip] > 2 * MAXPAT || r->nlayin[ip] < 1) return ORC_ERR_LIMITS;
+    for (j = 0; j < r->nlayin[ip]; j++)
+      if (r->layinc[ip * L + j] > L || r->layinc[ip * L + j] < 1) return ORC_ERR_LIMITS;
+    if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3) return ORC_ERR_UNSUPPORTED;
+  }
+  if (r->flagh2p == 1) return ORC_ERR_UNSUPPORTED;               /* nparacon_all */
+  if (r->iray > 0 && r->itype != 0) return ORC_ERR_UNSUPPORTED;  /* Rayleigh */
+  for (k = 0; k < ngas; k++)
+    if (r->idgas[k] == 22) return ORC_ERR_UNSUPPORTED;          /* N2 analytic CIA */
+
+  /* g-grid: all tables share it (read_klist.f:117-130). */
+  for (k = 0; k < ngas; k++) {
+    gtab[k] = find_table(tabs, ntab, r->idgas[k], r->isogas[k], vwave);
+    if (gtab[k]) {
+      if (ng == 0) {
+        ng = gtab[k]->ng;
+        delg = gtab[k]->delg;
+      } else if (gtab[k]->ng != ng) {
+        return ORC_ERR_LIMITS;
+      }
+    }
+  }
+  if (ng == 0 && ntab > 0) {
+    ng = tabs[0].ng;
+    delg = tabs[0].delg;
+  }
+  if (ng < 1 || ng > MAXG) return ORC_ERR_LIMITS;
+
+  nparam = ngas + 1;
+  rc = orc_interpem(r->nem, r->vem, r->emissivity, vwave, &esurf);
+  if (rc != ORC_OK && r->tsurf > 0.0) return rc;
+  rc = ORC_OK;
+  if (r->ispace == 1) vv = 1e4 / vwave;
+
+#define ALLOC(p, n)                                   \
+  do {                                                \
+    p = (double *)calloc((size_t)(n), sizeof(double)); \
+    if (!p) { rc = ORC_ERR_ALLOC; goto done; }        \
+  } while (0)
+  ALLOC(kout, (size_t)ngas * L * ng);
+  ALLOC(dkoutdt, (size_t)ngas * L * ng);
+  ALLOC(utotl, L);
+  ALLOC(taucon, L);
+  ALLOC(dtaucondq, (size_t)L * nparam);
+  ALLOC(kl_g, (size_t)L * ng);
+  ALLOC(dkdql, (size_t)L * ng * (ngas + 1));
+  ALLOC(bb, (size_t)npath * L);
+  ALLOC(dbdt, (size_t)npath * L);
+  ALLOC(corkout, (size_t)npath * ng);
+  ALLOC(tempgtsurf, (size_t)npath * ng);
+  ALLOC(dcoutdq, (size_t)npath * ng * L * nparam);
+  ALLOC(tautmp, L);
+  ALLOC(dtautmpdq, (size_t)L * nparam);
+  ALLOC(taus, L);
+  ALLOC(dtaudq, (size_t)L * nparam);
+  ALLOC(dtolddq, (size_t)L * nparam);
+  ALLOC(dtrdq, (size_t)L * nparam);
+  ALLOC(trv, L + 1);
+  ALLOC(wv, L + 1);
+  ALLOC(tlv, L + 1);
+
+  /* cirsradg_wave.f:418-426 */
+  for (j = 0; j < L; j++) {
+    double u = 0.0;
+    for (k = 0; k < ngas; k++) u += r->amount[j * ngas + k];
+    utotl[j] = u * 1e-20;
+  }
+
+  /* get_kg, cirsradg_wave.f:502-503 */
+  for (k = 0; k < ngas; k++) {
+    if (gtab[k])
+      orc_get_kg_gas(gtab[k], L, r->press, r->temp, vwave, kout + (size_t)k * L * ng,
+                     dkoutdt + (size_t)k * L * ng);
+  }
+
+  /* per-layer continuum and overlap, cirsradg_wave.f:510-760 */
+  for (j = 0; j < L; j++) {
+    double aam[MAXGAS], ppp[MAXGAS], absorb, dabsorb[7], amo[MAXGAS];
+    double k_gn[MAXGAS * MAXG], dkgndt[MAXGAS * MAXG];
+    double k_g[MAXG], dkdq[(MAXGAS + 1) * MAXG];
+    int iabs[5];
+    taucon[j] = 0.0;
+    /* ngascon returns 0 for every gas: all gas-continuum flags are off
+     * (ngascon.f:72-190 with /GASCONDEF/ zero). */
+    for (k = 0; k < ngas; k++) {
+      aam[k] = r->amount[j * ngas + k];
+      ppp[k] = r->pp[j * ngas + k];
+    }
+    orc_nciacon(cia, vv, r->press[j], r->temp[j], r->inormal, ngas, r->idgas, r->isogas,
+                aam, ppp, r->delh[j], &absorb, iabs, dabsorb);
+    taucon[j] += absorb;
+    for (l = 0; l < 5; l++) {
+      int kk = iabs[l];
+      if (kk > 0) dtaucondq[j * nparam + kk - 1] += dabsorb[l] / (utotl[j] * 1.0e20);
+    }
+    dtaucondq[j * nparam + ngas] += dabsorb[5];
+
+    for (k = 0; k < ngas; k++) {
+      amo[k] = r->amount[j * ngas + k] * 1.0e-20;
+      for (l = 0; l < ng; l++) {
+        if (!gtab[k]) {
+          k_gn[k * ng + l] = 0.0;
+          dkgndt[k * ng + l] = 0.0;
+        } else {
+          k_gn[k * ng + l] = kout[((size_t)k * L + j) * ng + l];
+          dkgndt[k * ng + l] = dkoutdt[((size_t)k * L + j) * ng + l];
+        }
+      }
+    }
+    orc_noverlapg(delg, ng, ngas, amo, k_gn, dkgndt, k_g, dkdq);
+    for (l = 0; l < ng; l++) {
+      kl_g[j * ng + l] = k_g[l];
+      for (k = 0; k < ngas; k++)
+        dkdql[((size_t)j * ng + l) * (ngas + 1) + k] = dkdq[k * ng + l] * 1.0e-20;
+      dkdql[((size_t)j * ng + l) * (ngas + 1) + ngas] = dkdq[ngas * ng + l];
+    }
+  }
+
+  /* g loop, cirsradg_wave.f:776-1232 */
+  for (ig = 0; ig < ng; ig++) {
+    for (j = 0; j < L; j++) {
+      tautmp[j] = taucon[j] + kl_g[j * ng + ig];
+      for (k = 0; k < nparam; k++) dtautmpdq[j * nparam + k] = dtaucondq[j * nparam + k];
+      for (k = 0; k < ngas + 1; k++)
+        dtautmpdq[j * nparam + k] += dkdql[((size_t)j * ng + ig) * (ngas + 1) + k];
+    }
+    for (ip = 0; ip < npath; ip++) {
+      const int nlays = r->nlayin[ip];
+      double *dco = dcoutdq + ((size_t)ip * ng + ig) * L * nparam;
+      double co = 0.0;
+      tempgtsurf[ip * ng + ig] = 0.0;
+      for (j = 0; j < nlays; j++)
+        for (k = 0; k < nparam; k++) dco[j * nparam + k] = 0.0;
+      for (j = 0; j < nlays; j++) {
+        int li = r->layinc[ip * L + j] - 1;
+        double sc = r->scale[ip * L + j];
+        taus[j] = tautmp[li] * sc;
+        for (k = 0; k < nparam; k++) dtaudq[j * nparam + k] = dtautmpdq[li * nparam + k] * sc;
+      }
+      if (r->imod[ip] == 0 || r->imod[ip] == 1) {
+        /* cirsradg_wave.f:854-895 (no solar file: xfac = 1) */
+        double tmp = 0.0;
+        for (j = 0; j < nlays; j++) tmp += taus[j];
+        if (r->imod[ip] == 0) {
+          co = exp(-tmp);
+          for (j = 0; j < nlays; j++)
+            for (k1 = 0; k1 < r->nsw; k1++) {
+              k = r->isw[k1] - 1;
+              dco[j * nparam + k] = -tmp * co * dtaudq[j * nparam + k];
+            }
+        } else {
+          co = 1.0 - exp(-tmp);
+          for (j = 0; j < nlays; j++)
+            for (k1 = 0; k1 < r->nsw; k1++) {
+              k = r->isw[k1] - 1;
+              dco[j * nparam + k] = tmp * (1 - co) * dtaudq[j * nparam + k];
+            }
+        }
+      } else {
+        /* imod = 3, cirsradg_wave.f:896-1035; iform = 0 so xfac = 1 */
+        const double xfac = 1.0;
+        double trold = 1.0, tr, tlayer, radground, gradground;
+        int jm = (int)(0.5 * nlays);
+        double p1, p2;
+        if (ig == 0) {
+          /* calc_nlte_t is the identity for inlte_flag = 0 */
+          for (j = 0; j < nlays; j++) {
+            bb[ip * L + j] = orc_planck(r->ispace, vwave, r->emtemp[ip * L + j]);
+            dbdt[ip * L + j] = orc_planckg(r->ispace, vwave, r->emtemp[ip * L + j]);
+          }
+        }
+        if (jac_mode == 0) {
+          for (j = 0; j < nlays; j++)
+            for (k = 0; k < nparam; k++) {
+              dtolddq[j * nparam + k] = 0.0;
+              dtrdq[j * nparam + k] = 0.0;
+            }
+          for (j = 0; j < nlays; j++) {
+            double bj = bb[ip * L + j];
+            tlayer = exp(-taus[j]);
+            tr = trold * tlayer;
+            co = co + xfac * (trold - tr) * bj;
+            for (k1 = 0; k1 < r->nsw; k1++) {
+              int j1;
+              k = r->isw[k1] - 1;
+              for (j1 = 0; j1 <= j; j1++) {
+                if (j1 < j)
+                  dtrdq[j1 * nparam + k] = dtolddq[j1 * nparam + k] * tlayer;
+                else
+                  dtrdq[j1 * nparam + k] = -dtaudq[j1 * nparam + k] * (tlayer * trold);
+                dco[j1 * nparam + k] +=
+                    xfac * bj * (dtolddq[j1 * nparam + k] - dtrdq[j1 * nparam + k]);
+              }
+              if (k == ngas) dco[j * nparam + k] += xfac * (trold - tr) * dbdt[ip * L + j];
+            }
+            trold = tr;
+            for (int j1 = 0; j1 <= j; j1++)
+              for (k1 = 0; k1 < r->nsw; k1++) {
+                k = r->isw[k1] - 1;
+                dtolddq[j1 * nparam + k] = dtrdq[j1 * nparam + k];
+              }
+          }
+        } else {
+          /* O(L) form: R = xfac*W_0 with W_{j-1} = B_j (1 - t_j) + t_j W_j and
+           * dR/dtau_j = xfac * tr_j * (B_j - W_j) (W_N = ground term). */
+          trv[0] = 1.0;
+          for (j = 0; j < nlays; j++) {
+            tlv[j] = exp(-taus[j]);
+            trv[j + 1] = trv[j] * tlv[j];
+            co = co + xfac * (trv[j] - trv[j + 1]) * bb[ip * L + j];
+          }
+          trold = trv[nlays];
+        }
+        /* cirsradg_wave.f:992-1024: limb test and surface term */
+        p1 = r->press[r->layinc[ip * L + (jm > 0 ? jm - 1 : 0)] - 1];
+        p2 = r->press[r->layinc[ip * L + nlays - 1] - 1];
+        radground = 0.0;
+        if (p2 > p1) {
+          if (r->tsurf <= 0.0) {
+            radground = bb[ip * L + nlays - 1];
+            gradground = dbdt[ip * L + nlays - 1];
+          } else {
+            radground = esurf * orc_planck(r->ispace, vwave, r->tsurf);
+            gradground = esurf * orc_planckg(r->ispace, vwave, r->tsurf);
+          }
+          co = co + xfac * trold * radground;
+          tempgtsurf[ip * ng + ig] = xfac * trold * gradground;
+          if (jac_mode == 0) {
+            for (j = 0; j < nlays; j++)
+              for (k1 = 0; k1 < r->nsw; k1++) {
+                k = r->isw[k1] - 1;
+                dco[j * nparam + k] += xfac * radground * dtolddq[j * nparam + k];
+              }
+          }
+        }
+        if (jac_mode != 0) {
+          wv[nlays] = radground;
+          for (j = nlays - 1; j >= 0; j--) {
+            /* wv[j] = upwelling radiance entering layer j from below */
+            if (j < nlays - 1)
+              wv[j] = bb[ip * L + j + 1] * (1.0 - tlv[j + 1]) + tlv[j + 1] * wv[j + 1];
+            else
+              wv[j] = radground;
+          }
+          for (j = 0; j < nlays; j++) {
+            double dr = xfac * trv[j + 1] * (bb[ip * L + j] - wv[j]);
+            for (k1 = 0; k1 < r->nsw; k1++) {
+              k = r->isw[k1] - 1;
+              dco[j * nparam + k] = dr * dtaudq[j * nparam + k];
+              if (k == ngas) dco[j * nparam + k] += xfac * (trv[j] - trv[j + 1]) * dbdt[ip * L + j];
+            }
+          }
+        }
+      }
+      corkout[ip * ng + ig] = co;
+    }
+  }
+
+  /* g-quadrature, cirsradg_wave.f:1240-1286 */
+  for (ip = 0; ip < npath; ip++) {
+    const int nlay1 = r->nlayin[ip];
+    output[ip] = 0.0;
+    gtsurf[ip] = 0.0;
+    for (j = 0; j < L; j++)
+      for (k = 0; k < nparam; k++) doutputdq[((size_t)ip * L + j) * nparam + k] = 0.0;
+    for (ig = 0; ig < ng; ig++) {
+      const double *dco = dcoutdq + ((size_t)ip * ng + ig) * L * nparam;
+      output[ip] += corkout[ip * ng + ig] * delg[ig];
+      gtsurf[ip] += tempgtsurf[ip * ng + ig] * delg[ig];
+      for (j = 0; j < nlay1; j++)
+        for (k1 = 0; k1 < r->nsw; k1++) {
+          k = r->isw[k1] - 1;
+          doutputdq[((size_t)ip * L + j) * nparam + k] += dco[j * nparam + k] * delg[ig];
+        }
+    }
+  }
+
+done:
+  free(kout);
+  free(dkoutdt);
+  free(utotl);
+  free(taucon);
+  free(dtaucondq);
+  free(kl_g);
+  free(dkdql);
+  free(bb);
+  free(dbdt);
+  free(corkout);
+  free(tempgtsurf);
+  free(dcoutdq);
+  free(tautmp);
+  free(dtautmpdq);
+  free(taus);
+  free(dtaudq);
+  free(dtolddq);
+  free(dtrdq);
+  free(trv);
+  free(wv);
+  free(tlv);
+  return rc;
+#undef ALLOC
+}
+
+int orc_spectrum(const orc_ktable *tabs, int ntab, const orc_cia *cia,
+                 const orc_req *r, const double *vwave, int nwave, int jac_mode,
+                 int nthreads, double *output, double *gtsurf, double *doutputdq) {
+  int iw, rc = ORC_OK;
+  const size_t per = (size_t)r->npath * r->nlayer * (r->ngas + 1);
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(max : rc)
+#endif
+  for (iw = 0; iw < nwave; iw++) {
+    int rci = orc_cirsradg_wave(tabs, ntab, cia, r, vwave[iw], jac_mode,
+                                output + (size_t)iw * r->npath, gtsurf + (size_t)iw * r->npath,
+                                doutputdq + (size_t)iw * per);
+    if (rci > rc) rc = rci;
+  }
+  (void)nthreads;
+  return rc;
+}

@@ -1,0 +1,98 @@
+/*
+ * FP64 CPU restatement of Radtrans' CIRSRADG_WAVE (correlated-k thermal
+ * emission with analytic Jacobian), used as TEST INFRASTRUCTURE only: the
+ * parity tests, the smoke check and the CPU-baseline leg of the benchmark call
+ * it as the checker.  The product library never links or calls it.
+ *
+ * Each function cites the reference file:line it restates.  All REAL
+ * quantities are double, as in the reference built with REAL promoted to
+ * 8 bytes.
+ */
+#ifndef RADG_ORACLE_H
+#define RADG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One gas's k-table (the content of a .kta file, read_khead.f:38-109). */
+typedef struct orc_ktable {
+  int idgas, isogas;
+  int npoint, np, nt, ng;
+  double vmin, delv;
+  const double *lnp;   /* [np]  ln(P / atm), as read_khead.f:80-85 stores it */
+  const double *tk;    /* [nt]  K */
+  const double *g_ord; /* [ng] */
+  const double *delg;  /* [ng] */
+  const double *k;     /* [npoint][np][nt][ng], k * 1e20 */
+} orc_ktable;
+
+/* The /CIATABLE/ block (ciacom.f:9-18): kcia(9,25,1501) == [1501][25][9]. */
+typedef struct orc_cia {
+  double dnu;
+  const double *temps; /* [25] */
+  const double *kcia;  /* [1501][25][9] */
+} orc_cia;
+
+/* The argument surface of cirsradg_wave (cirsradg_wave.f:1-5), compact C
+ * layout: per-layer arrays [L], per-(layer,gas) arrays [L][ngas],
+ * per-(path,layer) arrays [npath][L]. */
+typedef struct orc_req {
+  int nlayer, npath, ngas, nsw, nem;
+  int inormal, iray, ispace, itype, flagh2p, amform;
+  double dist, tsurf, radius1, radextra;
+  const double *press, *temp, *delh, *hfp;
+  const double *pp, *amount;
+  const int *idgas, *isogas;
+  const int *nlayin, *imod;
+  const int *layinc;             /* 1-based layer numbers */
+  const double *scale, *emtemp;
+  const int *isw;                /* 1-based parameter ids */
+  const double *vem, *emissivity;
+} orc_req;
+
+/* Status codes (the reference STOPs instead). */
+enum {
+  ORC_OK = 0,
+  ORC_ERR_LIMITS = 1,      /* check_limits.f:10-78 */
+  ORC_ERR_UNSUPPORTED = 2, /* branch outside this restatement */
+  ORC_ERR_EMISSIVITY = 3,  /* interpem.f:17-22 */
+  ORC_ERR_ALLOC = 4
+};
+
+/* Jacobian recurrence: 0 = the reference's O(L^2) triangular recurrence
+ * (cirsradg_wave.f:954-978); 1 = the O(L) adjoint form (same quantity). */
+int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
+                      const orc_req *r, double vwave, int jac_mode,
+                      double *output, double *gtsurf, double *doutputdq);
+
+/* cirsrtfg_wave.f:361-448 wavenumber loop: outputs [nwave][npath],
+ * [nwave][npath] and [nwave][npath][nlayer][nparam].  nthreads>1 splits the
+ * wavenumbers over OpenMP threads (the calls are independent). */
+int orc_spectrum(const orc_ktable *tabs, int ntab, const orc_cia *cia,
+                 const orc_req *r, const double *vwave, int nwave,
+                 int jac_mode, int nthreads,
+                 double *output, double *gtsurf, double *doutputdq);
+
+/* Pieces exposed for unit tests. */
+int orc_locate(const double *xx, int n, double x);                 /* locate.f */
+void orc_sort2g(int n, double *ra, int *ib);                        /* sort2g.f */
+void orc_rankg(int nparam, const double *gw, int ng, int nloop,
+               double *weight, double *cont, double *grad, int grad_ld,
+               double *k_g, double *dkdq, int dkdq_ld);             /* rankg.f */
+void orc_noverlapg(const double *delg, int ng, int ngas,
+                   const double *amount, const double *k_gn,
+                   const double *dkgndt, double *k_g, double *dkdq); /* noverlapg.f */
+double orc_planck(int ispace, double v, double t);                  /* planck_wave.f */
+double orc_planckg(int ispace, double v, double t);                 /* planckg_wave.f */
+void orc_ciaread(const orc_cia *cia, double vv, double temp,
+                 double *alpha, double *dalphadt);                  /* ciaread.f */
+void orc_nciacon(const orc_cia *cia, double v0, double p, double t,
+                 int inormal, int ngas, const int *idgas, const int *isogas,
+                 const double *amount, const double *pp, double xlen,
+                 double *absorb, int *iabs, double *dabsorb);       /* nciacon.f */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

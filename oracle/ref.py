@@ -1,0 +1,176 @@
+"""Driver for the reference's own CIRSRADG_WAVE (TEST INFRASTRUCTURE).
+
+``oracle/Makefile`` compiles the reference's Fortran sources, where they lie,
+into ``oracle/_ref/libradref64.so`` (REAL promoted to 8 bytes: the FP64 form)
+and ``libradref32.so`` (as written).  This module feeds them the same
+synthetic inputs as the restatement:
+
+* k-tables are written as ``.kta`` files and registered through the
+  reference's ``read_klist`` (``radtran/cirsrad/read_klist.f:30``);
+* the CIA table goes straight into ``/CIATABLE/`` (the harness's HSETCIA);
+* ``cirsradg_wave`` is called once per wavenumber with the reference's fixed
+  leading dimensions, as ``cirsrtfg_wave.f:397-401`` calls it.
+
+It is used to generate the golden fixtures and, on a machine where the
+library was built, as the ``reference`` CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(_HERE, "_ref")
+
+MAXLAY, MAXPAT, MAXGAS, MAXCON, MAXSEC = 400, 200, 30, 10, 16000
+NPARMAX = MAXGAS + 2 + MAXCON
+
+
+def lib_path(prec: int) -> str:
+    return os.path.join(REF_DIR, f"libradref{prec}.so")
+
+
+def available(prec: int = 64) -> bool:
+    return os.path.exists(lib_path(prec))
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+_libs = {}
+
+
+def _load(prec):
+    if prec not in _libs:
+        _libs[prec] = C.CDLL(lib_path(prec), mode=os.RTLD_LAZY)
+    return _libs[prec]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class RefRunner:
+    """One loaded reference library with tables registered for a request.
+
+    The reference keeps its state in COMMON blocks, so one runner per
+    precision is active at a time.
+    """
+
+    def __init__(self, tables, cia, req, prec: int = 64, workdir: str | None = None):
+        from radtrancode_amd.kta import write_kta
+
+        self.prec = prec
+        self.rdt = np.float64 if prec == 64 else np.float32
+        self.lib = _load(prec)
+        self.req = req
+        self._tmp = None
+        if workdir is None:
+            self._tmp = tempfile.TemporaryDirectory(prefix="radref")
+            workdir = self._tmp.name
+        names = []
+        for t, kt in enumerate(tables):
+            name = os.path.join(workdir, f"g{t}.kta")
+            write_kta(name, kt, recl=8 if prec == 64 else 4)
+            names.append(name)
+        klist = os.path.join(workdir, "tables.kls")
+        with open(klist, "w") as f:
+            for n in names:
+                f.write(n + "\n")
+        if len(klist) > 100:
+            raise ValueError("k-list path longer than CHARACTER*100")
+        rdt = self.rdt
+        self.lib.hradinit_()
+        dnu = np.array([cia.dnu], dtype=rdt)
+        temps = np.ascontiguousarray(cia.temps, dtype=rdt)
+        kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
+        self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        ngas = np.array([req.ngas], dtype=np.int32)
+        idg = np.ascontiguousarray(req.idgas, dtype=np.int32)
+        iso = np.ascontiguousarray(req.isogas, dtype=np.int32)
+        self.vwave = np.ascontiguousarray(req.vwave, dtype=rdt)
+        nw = np.array([self.vwave.shape[0]], dtype=np.int32)
+        nkl = np.zeros(1, dtype=np.int32)
+        kb = klist.encode()
+        self.lib.hreadk_(C.c_char_p(kb), _p(ngas), _p(idg), _p(iso), _p(nw),
+                         _p(self.vwave), _p(nkl), C.c_size_t(len(kb)))
+        self.nkl = int(nkl[0])
+        self._pack_args()
+
+    def _pack_args(self):
+        r, rdt = self.req, self.rdt
+        L, ng, npath = r.nlayer, r.ngas, r.npath
+        F = dict(order="F")
+        self.delh = np.ascontiguousarray(r.delh, dtype=rdt)
+        self.press = np.zeros(MAXLAY, rdt); self.press[:L] = r.press
+        self.temp = np.zeros(MAXLAY, rdt); self.temp[:L] = r.temp
+        self.hfp = np.zeros(MAXLAY, rdt)
+        if r.hfp is not None:
+            self.hfp[:L] = r.hfp
+        self.pp = np.zeros((MAXLAY, MAXGAS), rdt, **F); self.pp[:L, :ng] = r.pp
+        self.amount = np.zeros((MAXLAY, MAXGAS), rdt, **F); self.amount[:L, :ng] = r.amount
+        self.nlayin = np.zeros(MAXPAT, np.int32); self.nlayin[:npath] = r.nlayin
+        self.layinc = np.zeros((MAXLAY, MAXPAT), np.int32, **F)
+        self.layinc[:L, :npath] = r.layinc.T
+        self.cont = np.zeros((MAXCON, MAXLAY), rdt, **F)
+        self.scale = np.zeros((MAXLAY, MAXPAT), rdt, **F); self.scale[:L, :npath] = r.scale.T
+        self.emtemp = np.zeros((MAXLAY, MAXPAT), rdt, **F); self.emtemp[:L, :npath] = r.emtemp.T
+        self.imod = np.zeros(MAXPAT, np.int32); self.imod[:npath] = r.imod
+        self.idgas = np.zeros(MAXGAS, np.int32); self.idgas[:ng] = r.idgas
+        self.isogas = np.zeros(MAXGAS, np.int32); self.isogas[:ng] = r.isogas
+        self.vem = np.zeros(MAXSEC, rdt); self.vem[:len(r.vem)] = r.vem
+        self.emis = np.zeros(MAXSEC, rdt); self.emis[:len(r.emissivity)] = r.emissivity
+        self.isw = np.zeros(NPARMAX, np.int32); self.isw[:len(r.isw)] = r.isw
+        self.gtsurf = np.zeros(MAXPAT, rdt)
+        self.output = np.zeros(MAXPAT, rdt)
+        self.dq = np.zeros((MAXPAT, MAXLAY, NPARMAX), rdt, **F)
+        sc = lambda v, t=rdt: np.array([v], dtype=t)
+        ic = lambda v: np.array([v], dtype=np.int32)
+        self.s = dict(dist=sc(r.dist), inormal=ic(r.inormal), iray=ic(r.iray),
+                      nlayer=ic(L), npath=ic(npath), ngas=ic(ng), ispace=ic(r.ispace),
+                      amform=ic(r.amform), itype=ic(r.itype), nem=ic(len(r.vem)),
+                      tsurf=sc(r.tsurf), radius1=sc(r.radius1), flagh2p=ic(r.flagh2p),
+                      radextra=sc(r.radextra), nsw=ic(len(r.isw)))
+
+    def call(self, iwave: int):
+        """``cirsradg_wave`` for the 0-based wavenumber ``iwave``."""
+        s = self.s
+        iw = np.array([iwave + 1], dtype=np.int32)
+        vv = np.array([self.vwave[iwave]], dtype=self.rdt)
+        self.lib.cirsradg_wave_(
+            _p(s["dist"]), _p(s["inormal"]), _p(s["iray"]), _p(self.delh),
+            _p(s["nlayer"]), _p(s["npath"]), _p(s["ngas"]), _p(self.press),
+            _p(self.temp), _p(self.pp), _p(self.amount), _p(iw), _p(s["ispace"]),
+            _p(s["amform"]), _p(vv), _p(self.nlayin), _p(self.layinc), _p(self.cont),
+            _p(self.scale), _p(self.imod), _p(self.idgas), _p(self.isogas),
+            _p(self.emtemp), _p(s["itype"]), _p(s["nem"]), _p(self.vem), _p(self.emis),
+            _p(s["tsurf"]), _p(self.gtsurf), _p(s["radius1"]), _p(s["flagh2p"]),
+            _p(self.hfp), _p(s["radextra"]), _p(s["nsw"]), _p(self.isw),
+            _p(self.output), _p(self.dq))
+
+    def spectrum(self, iwaves=None):
+        """Same return layout as ``oracle.spectrum``."""
+        r = self.req
+        L, npath, npar = r.nlayer, r.npath, r.ngas + 1
+        iwaves = range(self.vwave.shape[0]) if iwaves is None else iwaves
+        iwaves = list(iwaves)
+        out = np.zeros((len(iwaves), npath))
+        gts = np.zeros((len(iwaves), npath))
+        dq = np.zeros((len(iwaves), npath, L, npar))
+        for n, iw in enumerate(iwaves):
+            self.call(iw)
+            out[n] = self.output[:npath]
+            gts[n] = self.gtsurf[:npath]
+            dq[n] = self.dq[:npath, :L, :npar]
+        return out, gts, dq
+
+    def close(self):
+        self.lib.hclosek_(_p(np.array([self.nkl], dtype=np.int32)))
+        if self._tmp is not None:
+            self._tmp.cleanup()
+            self._tmp = None

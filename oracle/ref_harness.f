@@ -1,0 +1,93 @@
+C Test harness for the reference's own CIRSRADG_WAVE, built by
+C oracle/Makefile into oracle/_ref/ (test infrastructure only).
+C
+C The reference routine takes most of its inputs from COMMON blocks that
+C NEMESIS fills from run files.  These entry points fill the same blocks
+C from arrays instead, so a caller can drive cirsradg_wave directly:
+C
+C   HRADINIT  switches off every optional physics flag
+C             (/diagnostic/, /lbltable/, /nlte_flags/, /dust/, /solardat/,
+C             /GASCONDEF/), as a NEMESIS run with zero flags would.
+C   HSETCIA   loads /CIATABLE/ as ciaread.f:63-103 would from a .tab file.
+C   HREADK    registers .kta files through the reference's read_klist.
+C   HCLOSEK   closes the units read_klist opened.
+C
+C Compiled twice: as written (REAL = 4 bytes) and with REAL promoted to
+C 8 bytes, which is the FP64 form of the reference.
+
+      SUBROUTINE HRADINIT()
+      IMPLICIT NONE
+      INCLUDE 'arrdef.f'
+      INCLUDE 'gascom.f'
+      INTEGER IDIAG, IQUIET, ILBL, INLTE, NSEC, NCONT
+      INTEGER IREAD, IFORM, NSPT
+      REAL NLTEK1, NLTEA, VSEC(MAXSEC), XSEC(2,MAXCON,MAXSEC)
+      REAL SOLRAD, SWAVE(MAXBIN), SRAD(MAXBIN)
+      COMMON /DIAGNOSTIC/ IDIAG, IQUIET
+      COMMON /LBLTABLE/ ILBL
+      COMMON /NLTE_FLAGS/ INLTE, NLTEK1, NLTEA
+      COMMON /DUST/ VSEC, XSEC, NSEC, NCONT
+      COMMON /SOLARDAT/ IREAD, IFORM, SOLRAD, SWAVE, SRAD, NSPT
+      IDIAG = 0
+      IQUIET = 1
+      ILBL = 0
+      INLTE = 0
+      NLTEK1 = 0.0
+      NLTEA = 0.0
+      NSEC = 0
+      NCONT = 0
+      IREAD = 0
+      IFORM = 0
+      ICH4 = 0
+      IH2O = 0
+      IO3 = 0
+      INH3 = 0
+      IUVSCAT = 0
+      RETURN
+      END
+
+      SUBROUTINE HSETCIA(DNU, TEMPS, KC)
+      IMPLICIT NONE
+      INCLUDE 'ciacom.f'
+      REAL DNU, TEMPS(NUMT), KC(NUMPAIRS,NUMT,NUMWN)
+      INTEGER I, J, K
+      IREAD1 = 0
+      DNU1 = DNU
+      DO I = 1, NUMT
+        TEMPK1(I) = TEMPS(I)
+      ENDDO
+      DO I = 1, NUMWN
+        VVK1(I) = (I-1)*DNU1
+      ENDDO
+      DO K = 1, NUMWN
+        DO J = 1, NUMT
+          DO I = 1, NUMPAIRS
+            KCIA(I,J,K) = KC(I,J,K)
+          ENDDO
+        ENDDO
+      ENDDO
+      IT1C = 0
+      IT2C = 0
+      IVC = 0
+      RETURN
+      END
+
+      SUBROUTINE HREADK(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
+      IMPLICIT NONE
+      CHARACTER*(*) KL
+      INTEGER NGAS, IDGAS(NGAS), ISOGAS(NGAS), NWAVE, NKL
+      REAL VWAVE(NWAVE)
+      CHARACTER*100 KLIST
+      KLIST = KL
+      CALL READ_KLIST(KLIST, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
+      RETURN
+      END
+
+      SUBROUTINE HCLOSEK(NKL)
+      IMPLICIT NONE
+      INTEGER NKL, I
+      DO I = 1, NKL
+        CLOSE(100 + I)
+      ENDDO
+      RETURN
+      END

@@ -1,0 +1,102 @@
+"""Reader and writer for the reference's ``.kta`` k-table files.
+
+A ``.kta`` file is a Fortran direct-access file of fixed-width records
+(``ISYS()`` bytes each: 4 for REAL(4) builds, ``radtran/rtm_util/isys.f:28``).
+Layout, from ``radtran/cirsrad/read_khead.f:38-109`` and the writer
+``radtran/radtran/calc_fnktablec.f:308-367,593-609``:
+
+====================  =====================================================
+record                content
+====================  =====================================================
+1                     IREC0, first data record (integer)
+2                     NPOINT (integer)
+3, 4, 5               VMIN, DELV, FWHM (real)
+6, 7, 8               NP, NT, NG (integer)
+9, 10                 IDGAS, ISOGAS (integer)
+11 .. 10+NG           G_ORD
+11+NG .. 10+2NG       DEL_G
+11+2NG, 12+2NG        unused
+13+2NG ..             PRESS(NP) in atm, then TEMP(NT) (or TEMP2(NP,|NT|))
+then (DELV <= 0)      VCEN(NPOINT)
+IREC0 ..              k * 1e20 for each wavenumber: NP*|NT|*NG values,
+                      g fastest, then T, then P
+====================  =====================================================
+
+Integers occupy the first 4 bytes of a record.  ``recl=8`` gives the
+layout the REAL(8) build of the reference reads.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .synth import KTable
+
+
+def _header_records(kt: KTable, vcen=None):
+    ng, np_, nt = kt.ng, kt.np_, kt.nt
+    nrec_head = 10 + 2 * ng + 2 + np_ + nt + (0 if kt.delv > 0 else kt.npoint)
+    irec0 = nrec_head + 1
+    ints = {1: irec0, 2: kt.npoint, 6: np_, 7: nt, 8: ng, 9: kt.idgas, 10: kt.isogas}
+    reals = {3: kt.vmin, 4: kt.delv, 5: kt.fwhm}
+    for j in range(ng):
+        reals[11 + j] = kt.g_ord[j]
+        reals[11 + ng + j] = kt.delg[j]
+    r = 11 + 2 * ng + 2
+    for j in range(np_):
+        reals[r + j] = kt.press[j]
+    r += np_
+    for j in range(nt):
+        reals[r + j] = kt.temp[j]
+    r += nt
+    if kt.delv <= 0:
+        for j in range(kt.npoint):
+            reals[r + j] = vcen[j]
+    return irec0, nrec_head, ints, reals
+
+
+def write_kta(path: str, kt: KTable, recl: int = 4, vcen=None) -> None:
+    """Write ``kt`` as a ``.kta`` file with ``recl``-byte records (4 or 8)."""
+    if recl not in (4, 8):
+        raise ValueError("recl must be 4 or 8")
+    rdt = np.float32 if recl == 4 else np.float64
+    irec0, nhead, ints, reals = _header_records(kt, vcen)
+    head = np.zeros((nhead, recl), dtype=np.uint8)
+    for r, v in ints.items():
+        head[r - 1, :4] = np.frombuffer(np.int32(v).tobytes(), dtype=np.uint8)
+    for r, v in reals.items():
+        head[r - 1, :] = np.frombuffer(rdt(v).tobytes(), dtype=np.uint8)
+    with open(path, "wb") as f:
+        f.write(head.tobytes())
+        f.write(np.ascontiguousarray(kt.k, dtype=rdt).tobytes())
+
+
+def read_kta(path: str, recl: int = 4) -> KTable:
+    """Read a ``.kta`` file (regular grid, ``DELV > 0``) into a :class:`KTable`."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    if raw.size % recl:
+        raise ValueError(f"{path}: size is not a multiple of {recl}")
+    rec = raw.reshape(-1, recl)
+    rdt = np.float32 if recl == 4 else np.float64
+
+    def gi(r):
+        return int(rec[r - 1, :4].view(np.int32)[0])
+
+    def gr(r):
+        return float(rec[r - 1].view(rdt)[0])
+
+    irec0, npoint, np_, nt, ng = gi(1), gi(2), gi(6), gi(7), gi(8)
+    if nt < 0:
+        raise ValueError("per-pressure temperature grids (NT < 0) are not supported")
+    vmin, delv, fwhm = gr(3), gr(4), gr(5)
+    if delv <= 0:
+        raise ValueError("irregular wavenumber grids (DELV <= 0) are not supported")
+    g_ord = np.array([gr(11 + j) for j in range(ng)])
+    delg = np.array([gr(11 + ng + j) for j in range(ng)])
+    r = 11 + 2 * ng + 2
+    press = np.array([gr(r + j) for j in range(np_)])
+    temp = np.array([gr(r + np_ + j) for j in range(nt)])
+    ntab = np_ * nt * ng
+    data = rec[irec0 - 1: irec0 - 1 + npoint * ntab].copy().view(rdt).reshape(
+        npoint, np_, nt, ng).astype(np.float64)
+    return KTable(idgas=gi(9), isogas=gi(10), vmin=vmin, delv=delv, fwhm=fwhm,
+                  press=press, temp=temp, g_ord=g_ord, delg=delg, k=data)

@@ -1,0 +1,196 @@
+#!/usr/bin/env python
+"""Jacobian-spectra/s of the CIRSRADG_WAVE path on one MI355X.
+
+Workload (default): the mixed-precision form at 120 layers x 1000
+wavenumbers (600-1400 cm-1) x 20 g-ordinates, 3 k-table gases + H2/He CIA,
+5 emission-angle paths.  One step = one device-resident batch of ``--batch``
+independent spectra (seeded atmospheres, shared resident k-tables), i.e. the
+layer/overlap kernel, the emission kernel and the angle quadrature over every
+wavenumber of every spectrum.  The plain FP64 form is timed on the same batch.
+
+Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
+this build runs on one GPU.
+"""
+import argparse
+import copy
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_batch_inputs(nspec, nwave=1000, nlayer=120, nkgas=3, ng=20):
+    from radtrancode_amd import synth
+
+    tables, cia, req0 = synth.config1(nwave=nwave, nlayer=nlayer, nkgas=nkgas, ng=ng)
+    reqs = [req0] + [synth.make_request(nlayer=nlayer, nkgas=nkgas, nwave=nwave, seed=100 + i)
+                     for i in range(1, nspec)]
+    return tables, cia, reqs
+
+
+def time_batch(rad, reqs, mode, steps, warmup, torch):
+    b = rad.Batch(reqs, mode=mode)
+    st = torch.cuda.current_stream()
+    for _ in range(warmup):
+        b.launch(st)
+    torch.cuda.synchronize()
+    n0, s0 = b.timing_stats()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(st)
+        b.launch(st)
+        ev[i][1].record(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n1, s1 = b.timing_stats()
+    lat = np.array([a.elapsed_time(z) for a, z in ev])
+    kern = (s1 - s0) / max(1, n1 - n0) if n1 - n0 == steps else s1 / max(1, n1)
+    return b, t1 - t0, lat, kern
+
+
+def cpu_baseline(tables, cia, req, seconds):
+    """The reference's own cirsradg_wave (REAL(8) build) if it was built,
+    else the oracle restatement, on one host core over the first wavenumbers
+    of one spectrum until about ``seconds`` of CPU time."""
+    from oracle import ref
+
+    n = 8
+    kind = "reference" if ref.available(64) else "port"
+    total_t, total_n = 0.0, 0
+    while total_t < seconds and total_n < req.nwave:
+        n = min(n, req.nwave - total_n)
+        ts = [copy.copy(t) for t in tables]
+        for t, t0 in zip(ts, tables):
+            t.k = t0.k[total_n:total_n + n].copy()
+            t.vmin = t0.vmin + total_n * t0.delv
+        r2 = copy.copy(req)
+        r2.vwave = req.vwave[total_n:total_n + n].copy()
+        if kind == "reference":
+            rr = ref.RefRunner(ts, cia, r2, prec=64)
+            t0 = time.perf_counter()
+            rr.spectrum()
+            dt = time.perf_counter() - t0
+            rr.close()
+        else:
+            from oracle import oracle
+            t0 = time.perf_counter()
+            oracle.spectrum(ts, cia, r2, nthreads=1)
+            dt = time.perf_counter() - t0
+        total_t += dt
+        total_n += n
+        n = min(4 * n, 256)
+    value = (total_n / req.nwave) / total_t
+    what = ("reference cirsradg_wave, REAL(8) build, called once per wavenumber with .kta "
+            "files on the host page cache" if kind == "reference" else "FP64 C restatement")
+    return dict(value=value, unit="Jacobian-spectra/s", cores=1, kind=kind,
+                sample=f"{total_n} of {req.nwave} wavenumbers of one 120-layer x 20 g spectrum "
+                       f"({what}) in {total_t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by wavenumber or by request with no exchange)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from oracle import oracle
+    from radtrancode_amd import cirsradg as rad
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    tables, cia, reqs = make_batch_inputs(args.batch)
+    rad.register_ktables(tables)
+    rad.register_cia(cia)
+
+    # ---- mixed-precision form: the workload
+    b, wall, lat, kern = time_batch(rad, reqs, rad.MODE_MIXED, args.steps, args.warmup, torch)
+    info = b.info()
+    nspec = args.batch * args.steps
+    value = nspec / wall
+    # accuracy of this run against the FP64 oracle on a sample of wavenumbers
+    sample = np.arange(0, 1000, 50)
+    res = b.read(0, weighted=False)
+    o_ref, g_ref, d_ref = oracle.spectrum(tables, cia, reqs[0], vwave=reqs[0].vwave[sample],
+                                          nthreads=os.cpu_count() or 1)
+    o_m = res["output"][sample]
+    d_m = res["doutputdq"][sample]
+    rad_err = float(np.max(np.abs(o_m - o_ref) / np.abs(o_ref)))
+    sc = np.max(np.abs(d_ref), axis=2, keepdims=True)
+    jac_err = float(np.max(np.abs(d_m - d_ref) / np.where(sc > 0, sc, 1.0)))
+    b.close()
+
+    # ---- plain FP64 form, same batch
+    b64, wall64, lat64, kern64 = time_batch(rad, reqs, rad.MODE_FP64, args.fp64_steps, 1, torch)
+    b64.close()
+    value64 = args.batch * args.fp64_steps / wall64
+
+    # ---- roofline of the dominant kernel (layer/overlap kernel): it reads
+    # the table nodes and the layer inputs of every spectrum in the batch
+    t_layer_s = kern[0] / 1e3
+    alg_read = info["table_bytes"] + info["input_bytes"]
+    achieved = alg_read / t_layer_s / 1e9
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=achieved / HBM_PEAK_GBS, traffic=None,
+                kernel="layer_kernel<float,8>", kernel_ms=float(kern[0]),
+                alg_bytes_per_launch=alg_read)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(tables, cia, reqs[0], args.cpu_seconds)
+
+    out = {
+        "metric": "Jacobian spectra/s (120 lyr,1k wn,20 g) + p50/p99 latency vs plain FP64",
+        "value": value,
+        "unit": "Jacobian-spectra/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "mixed-precision CIRSRADG_WAVE forward + analytic Jacobian, "
+                               "120 layers x 1000 wavenumbers x 20 g, 3 k-gases + H2/He CIA, "
+                               "5 emission angles",
+                   "spectra_per_step": args.batch, "parallelism": "single GPU"},
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "latency_unit": "one step (batch of spectra), HIP events",
+        "kernel_ms": {"layer": float(kern[0]), "emission": float(kern[1]),
+                      "quadrature": float(kern[2])},
+        "fp64_form": {"value": value64, "ms_per_step": wall64 / args.fp64_steps * 1e3,
+                      "kernel_ms": {"layer": float(kern64[0]), "emission": float(kern64[1]),
+                                    "quadrature": float(kern64[2])}},
+        "speedup_vs_fp64": value / value64,
+        "error_vs_fp64_oracle": {"radiance_rel": rad_err, "jacobian_col_scaled": jac_err,
+                                 "sample": f"{len(sample)} wavenumbers of spectrum 0"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * radg: MI355X (gfx950) implementation of Radtrans' CIRSRADG_WAVE --
+ * correlated-k thermal emission with its analytic Jacobian.
+ *
+ * C ABI.  Every entry point takes plain pointers and sizes, returns an int
+ * status (0 = success, see RADG_E*), and never aborts the host: the
+ * reference STOPs the process on the same conditions.  radg_last_error()
+ * returns the message of the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   cirsradg_wave_           radtran/cirsradg/cirsradg_wave.f:1-5 (types :167-178)
+ *   radg_spectrum_r4/_r8     the wavenumber loop radtran/cirsradg/cirsrtfg_wave.f:361-448
+ *   radg_read_klist          /INTERPK/ filled by radtran/cirsrad/read_klist.f:30-368
+ *                            from .kta headers (radtran/cirsrad/read_khead.f:18-120)
+ *   radg_register_ktables    the same registry from arrays already in memory
+ *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
+ *   radg_set_mode            (none: the reference's precision is fixed at compile time)
+ *   radg_batch_*             (none: device-resident batches for the serving loop that
+ *                            replaces nemesisPY/nemesis.py:4586-4664's process pool)
+ */
+#ifndef RADG_H
+#define RADG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Array limits, radtran/includes/arrdef.f:14-23.  They fix the leading
+ * dimensions of every array of the Fortran argument surface. */
+#define RADG_MAXLAY 400
+#define RADG_MAXPAT 200
+#define RADG_MAXGAS 30
+#define RADG_MAXCON 10
+#define RADG_MAXSEC 16000
+#define RADG_MAXG 51
+#define RADG_MAXK 30
+#define RADG_NPARMAX (RADG_MAXGAS + 2 + RADG_MAXCON)
+/* The GPU overlap sorts NG*NG keys in one wavefront's registers. */
+#define RADG_GPU_MAXG 32
+
+/* Status codes. */
+#define RADG_OK 0
+#define RADG_ELIMITS 1      /* check_limits.f:10-78 and the other size checks */
+#define RADG_EUNSUPPORTED 2 /* a branch this implementation does not provide */
+#define RADG_ENOTABLES 3    /* no k-tables or CIA table registered */
+#define RADG_EINPUT 4       /* invalid argument value (NaN, non-positive P, ...) */
+#define RADG_EHIP 5         /* a HIP runtime call failed */
+#define RADG_EBUDGET 6      /* a batch does not fit the device-memory budget */
+#define RADG_EIO 7          /* a .kta file could not be read */
+
+/* Arithmetic forms. */
+#define RADG_MODE_FP64 0   /* every quantity in FP64 */
+#define RADG_MODE_MIXED 1  /* FP32 interpolation/exp/overlap; FP64 Planck, gdist and
+                              transmission product; compensated g-sums */
+
+int radg_set_mode(int mode);
+int radg_get_mode(void);
+const char *radg_last_error(void);
+
+/* ---------------------------------------------------------------- tables */
+
+/* Register the k-tables of one list (replaces any earlier registration).
+ * All tables share one P/T/g grid, as read_klist.f:117-160 requires.
+ *   press_atm[np], temp[nt], g_ord[ng], delg[ng]
+ *   per table t: idgas[t], isogas[t], npoint[t], vmin[t], delv[t] (> 0) and
+ *   k[t] -> npoint[t]*np*nt*ng values, k*1e20, g fastest then T then P
+ *   (the .kta record order, calc_fnktablec.f:593-609). */
+int radg_register_ktables(int ntab, const int *idgas, const int *isogas,
+                          const int *npoint, const double *vmin, const double *delv,
+                          int np, int nt, int ng, const double *press_atm,
+                          const double *temp, const double *g_ord, const double *delg,
+                          const double *const *k);
+
+/* Read the .kta files named in a .kls list (one path per line) and register
+ * them (read_klist.f:30-110, read_khead.f:18-120).  recl is the record
+ * length in bytes (4 for the REAL(4) build, 8 for REAL(8)). */
+int radg_read_klist(const char *klist_path, int recl);
+
+/* /CIATABLE/: temps[25] (K), kcia[1501][25][9] in C order, which is the
+ * Fortran kcia(9,25,1501) (ciacom.f:9); wavenumber grid (i-1)*dnu. */
+int radg_register_cia(double dnu, const double *temps, const double *kcia);
+
+/* ------------------------------------------------ Fortran argument surface */
+
+/* Drop-in for CALL cirsradg_wave(...) from a REAL(4) host: same 37
+ * arguments, same fixed leading dimensions (arrdef.f), same outputs.  The
+ * hidden COMMON inputs come from radg_register_ktables/_cia; ncont = 0.
+ * iwave is accepted and ignored: the table slice is found from vwave. */
+void cirsradg_wave_(const float *dist, const int *inormal, const int *iray,
+                    const float *delh, const int *nlayer, const int *npath,
+                    const int *ngas, const float *press, const float *temp,
+                    const float *pp, const float *amount, const int *iwave,
+                    const int *ispace, const int *amform, const float *vwave,
+                    const int *nlayin, const int *layinc, const float *cont,
+                    const float *scale, const int *imod, const int *idgas,
+                    const int *isogas, const float *emtemp, const int *itype,
+                    const int *nem, const float *vem, const float *emissivity,
+                    const float *tsurf, float *gtsurf, const float *radius1,
+                    const int *flagh2p, const float *hfp, const float *radextra,
+                    const int *nsw, const int *isw, float *output, float *doutputdq);
+
+/* Status of the last cirsradg_wave_ call (the Fortran routine has none). */
+int radg_cirsradg_status(void);
+
+/* The whole wavenumber loop in one call.  Arguments as cirsradg_wave_ (fixed
+ * leading dimensions), plus nwave and vwave[nwave].  Outputs, C order:
+ *   output[nwave][npath], gtsurf[nwave][npath],
+ *   doutputdq[nwave][npath][nlayer][ngas+1]   (params: gas amounts 1..ngas,
+ *   then temperature; per molecule cm-2 and per K; rows past nlayin and
+ *   params not in isw are zero). */
+int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray,
+                     const float *delh, const int *nlayer, const int *npath,
+                     const int *ngas, const float *press, const float *temp,
+                     const float *pp, const float *amount, const int *ispace,
+                     const int *nlayin, const int *layinc, const float *scale,
+                     const int *imod, const int *idgas, const int *isogas,
+                     const float *emtemp, const int *itype, const int *nem,
+                     const float *vem, const float *emissivity, const float *tsurf,
+                     const int *flagh2p, const int *nsw, const int *isw,
+                     int nwave, const float *vwave, float *output, float *gtsurf,
+                     float *doutputdq);
+int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
+                     const double *delh, const int *nlayer, const int *npath,
+                     const int *ngas, const double *press, const double *temp,
+                     const double *pp, const double *amount, const int *ispace,
+                     const int *nlayin, const int *layinc, const double *scale,
+                     const int *imod, const int *idgas, const int *isogas,
+                     const double *emtemp, const int *itype, const int *nem,
+                     const double *vem, const double *emissivity, const double *tsurf,
+                     const int *flagh2p, const int *nsw, const int *isw,
+                     int nwave, const double *vwave, double *output, double *gtsurf,
+                     double *doutputdq);
+
+/* --------------------------------------------- device-resident batches */
+
+/* One spectrum request in compact C layout (0-based axes):
+ *   press/temp/delh[L]; pp/amount[L][ngas]; idgas/isogas[ngas];
+ *   nlayin/imod[npath]; layinc (1-based layer numbers), scale, emtemp
+ *   [npath][L]; isw[nsw] (1-based parameter ids); vwave[nwave];
+ *   vem/emissivity[nem]; path_weight[npath] (angle quadrature weights used
+ *   for the weighted spectrum; may be NULL). */
+typedef struct radg_request {
+  int nlayer, npath, ngas, nsw, nwave, nem;
+  int inormal, iray, ispace, itype, flagh2p;
+  double tsurf;
+  const double *press, *temp, *delh, *pp, *amount;
+  const int *idgas, *isogas, *nlayin, *imod, *layinc;
+  const double *scale, *emtemp;
+  const int *isw;
+  const double *vwave, *vem, *emissivity, *path_weight;
+} radg_request;
+
+typedef struct radg_batch radg_batch;
+
+/* Pack and upload nreq requests; outputs stay on the device.  Fails with
+ * RADG_EBUDGET if the batch needs more than budget_bytes of device memory
+ * (0 = no limit). */
+int radg_batch_create(int nreq, const radg_request *reqs, int mode,
+                      size_t budget_bytes, radg_batch **out);
+/* Device bytes a batch of these requests would allocate (for admission). */
+int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes);
+/* Launch every kernel of the batch on a HIP stream (NULL = default stream). */
+int radg_batch_launch(radg_batch *b, void *hip_stream);
+/* Wait for the batch's stream. */
+int radg_batch_sync(radg_batch *b);
+/* Copy request r's results to host memory (FP64 regardless of mode).  Any
+ * pointer may be NULL.  Shapes as radg_spectrum_r8; the weighted arrays are
+ * wout[nwave], wgts[nwave], wjac[nwave][nlayer][ngas+1] (sum over paths of
+ * path_weight * result). */
+int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf,
+                    double *doutputdq, double *wout, double *wgts, double *wjac);
+/* Kernel timings of the last launch in milliseconds (HIP events on the
+ * batch's stream): [0] layer/overlap kernel, [1] emission kernel,
+ * [2] path quadrature. */
+int radg_batch_timings(radg_batch *b, float *ms3);
+/* Sum of the kernel timings over the last (at most 128) launches. */
+int radg_batch_timing_stats(radg_batch *b, int *nlaunch, float *sum_ms3);
+/* Algorithmic bytes of the batch: alg_bytes3[0] = k-table nodes touched
+ * (distinct (P,T) nodes bracketing the layers x NG x bytes per value, per
+ * wavenumber and gas), [1] = outputs written, [2] = layer inputs and CIA
+ * nodes; plus the number of spectra and of layer-kernel work items. */
+int radg_batch_info(radg_batch *b, double *alg_bytes3, int *nspectra, int *nitems_a);
+int radg_batch_destroy(radg_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,87 @@
+"""Loader for the gfx950 library ``lib/libradg.so``.
+
+The library is the product: there is no CPU fallback.  ``lib()`` raises if
+the shared object is missing or does not load, and every wrapped call raises
+:class:`RadgError` with the library's own message on a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libradg.so")
+
+RADG_OK = 0
+MODE_FP64 = 0
+MODE_MIXED = 1
+
+# Every symbol the C header (include/radg.h) declares.
+EXPORTS = [
+    "radg_set_mode", "radg_get_mode", "radg_last_error", "radg_register_ktables",
+    "radg_read_klist", "radg_register_cia", "cirsradg_wave_", "radg_cirsradg_status",
+    "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
+    "radg_batch_launch", "radg_batch_sync", "radg_batch_read", "radg_batch_timings",
+    "radg_batch_info", "radg_batch_destroy", "radg_batch_timing_stats",
+]
+
+
+class RadgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"radg status {code}: {msg}")
+        self.code = code
+
+
+class Request(C.Structure):
+    """``radg_request`` (include/radg.h)."""
+    _fields_ = [("nlayer", C.c_int), ("npath", C.c_int), ("ngas", C.c_int),
+                ("nsw", C.c_int), ("nwave", C.c_int), ("nem", C.c_int),
+                ("inormal", C.c_int), ("iray", C.c_int), ("ispace", C.c_int),
+                ("itype", C.c_int), ("flagh2p", C.c_int), ("tsurf", C.c_double),
+                ("press", C.c_void_p), ("temp", C.c_void_p), ("delh", C.c_void_p),
+                ("pp", C.c_void_p), ("amount", C.c_void_p), ("idgas", C.c_void_p),
+                ("isogas", C.c_void_p), ("nlayin", C.c_void_p), ("imod", C.c_void_p),
+                ("layinc", C.c_void_p), ("scale", C.c_void_p), ("emtemp", C.c_void_p),
+                ("isw", C.c_void_p), ("vwave", C.c_void_p), ("vem", C.c_void_p),
+                ("emissivity", C.c_void_p), ("path_weight", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load ``libradg.so`` (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m radtrancode_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, ip = C.c_void_p, C.c_int
+    L.radg_last_error.restype = C.c_char_p
+    for name in ["radg_set_mode", "radg_get_mode", "radg_register_ktables", "radg_read_klist",
+                 "radg_register_cia", "radg_cirsradg_status", "radg_spectrum_r4",
+                 "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
+                 "radg_batch_launch", "radg_batch_sync", "radg_batch_read",
+                 "radg_batch_timings", "radg_batch_info", "radg_batch_destroy",
+                 "radg_batch_timing_stats"]:
+        getattr(L, name).restype = ip
+    L.radg_register_cia.argtypes = [C.c_double, vp, vp]
+    L.radg_read_klist.argtypes = [C.c_char_p, ip]
+    L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]
+    L.radg_batch_bytes.argtypes = [ip, vp, ip, C.POINTER(C.c_size_t)]
+    L.radg_batch_launch.argtypes = [vp, vp]
+    L.radg_batch_sync.argtypes = [vp]
+    L.radg_batch_read.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp]
+    L.radg_batch_timings.argtypes = [vp, vp]
+    L.radg_batch_info.argtypes = [vp, vp, C.POINTER(ip), C.POINTER(ip)]
+    L.radg_batch_destroy.argtypes = [vp]
+    L.radg_batch_timing_stats.argtypes = [vp, C.POINTER(ip), vp]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != RADG_OK:
+        raise RadgError(rc, lib().radg_last_error().decode())
+    return rc

@@ -1,0 +1,256 @@
+"""Python mirror of the CIRSRADG_WAVE interface on the gfx950 library.
+
+Reference correspondence:
+
+* :func:`cirsradg_wave` -- ``CALL cirsradg_wave(...)``
+  (``radtran/cirsradg/cirsradg_wave.f:1-5``): the same 37 arguments, as
+  Fortran-ordered arrays with the fixed leading dimensions of
+  ``radtran/includes/arrdef.f``; outputs are written in place.
+* :func:`spectrum` -- the wavenumber loop of ``CIRSRTFG_WAVE``
+  (``radtran/cirsradg/cirsrtfg_wave.f:361-448``) for one request.
+* :func:`register_ktables` / :func:`read_klist` -- the ``/INTERPK/`` registry
+  that ``read_klist.f`` fills; :func:`register_cia` -- ``/CIATABLE/``.
+* :class:`Batch` -- many requests resident on the device, launched together.
+
+Errors raise :class:`RadgError` where the reference would STOP.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._native import MODE_FP64, MODE_MIXED, RadgError, Request, check, lib  # noqa: F401
+
+MAXLAY, MAXPAT, MAXGAS, MAXCON, MAXSEC = 400, 200, 30, 10, 16000
+NPARMAX = MAXGAS + 2 + MAXCON
+
+
+def _vp(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def set_mode(mode: int) -> None:
+    check(lib().radg_set_mode(C.c_int(mode)))
+
+
+def get_mode() -> int:
+    return lib().radg_get_mode()
+
+
+def register_ktables(tables) -> None:
+    """Register ``synth.KTable`` objects sharing one P/T/g grid."""
+    t0 = tables[0]
+    n = len(tables)
+    ids = np.array([t.idgas for t in tables], dtype=np.int32)
+    isos = np.array([t.isogas for t in tables], dtype=np.int32)
+    npts = np.array([t.npoint for t in tables], dtype=np.int32)
+    vmin = np.array([t.vmin for t in tables], dtype=np.float64)
+    delv = np.array([t.delv for t in tables], dtype=np.float64)
+    ks = [np.ascontiguousarray(t.k, dtype=np.float64) for t in tables]
+    for t in tables[1:]:
+        if (t.np_, t.nt, t.ng) != (t0.np_, t0.nt, t0.ng):
+            raise ValueError("all tables must share the P/T/g grid")
+    ptrs = (C.c_void_p * n)(*[k.ctypes.data for k in ks])
+    press = np.ascontiguousarray(t0.press, dtype=np.float64)
+    temp = np.ascontiguousarray(t0.temp, dtype=np.float64)
+    gord = np.ascontiguousarray(t0.g_ord, dtype=np.float64)
+    delg = np.ascontiguousarray(t0.delg, dtype=np.float64)
+    check(lib().radg_register_ktables(
+        C.c_int(n), _vp(ids), _vp(isos), _vp(npts), _vp(vmin), _vp(delv),
+        C.c_int(t0.np_), C.c_int(t0.nt), C.c_int(t0.ng), _vp(press), _vp(temp),
+        _vp(gord), _vp(delg), ptrs))
+
+
+def read_klist(path: str, recl: int = 4) -> None:
+    """Register the ``.kta`` files named in a ``.kls`` list."""
+    check(lib().radg_read_klist(path.encode(), int(recl)))
+
+
+def register_cia(cia) -> None:
+    temps = np.ascontiguousarray(cia.temps, dtype=np.float64)
+    kcia = np.ascontiguousarray(cia.kcia, dtype=np.float64)
+    if kcia.shape != (1501, 25, 9):
+        raise ValueError("kcia must be [1501][25][9]")
+    check(lib().radg_register_cia(C.c_double(cia.dnu), _vp(temps), _vp(kcia)))
+
+
+class _ReqArrays:
+    """Contiguous copies of one request, kept alive while the batch exists."""
+
+    def __init__(self, req):
+        d = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        i = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+        self.press, self.temp, self.delh = d(req.press), d(req.temp), d(req.delh)
+        self.pp, self.amount = d(req.pp), d(req.amount)
+        self.idgas, self.isogas = i(req.idgas), i(req.isogas)
+        self.nlayin, self.imod, self.layinc = i(req.nlayin), i(req.imod), i(req.layinc)
+        self.scale, self.emtemp = d(req.scale), d(req.emtemp)
+        self.isw, self.vwave = i(req.isw), d(req.vwave)
+        self.vem, self.emis = d(req.vem), d(req.emissivity)
+        self.pw = d(req.path_weight)
+        self.s = Request(
+            req.nlayer, req.npath, req.ngas, len(self.isw), len(self.vwave), len(self.vem),
+            req.inormal, req.iray, req.ispace, req.itype, req.flagh2p, float(req.tsurf),
+            *[a.ctypes.data for a in (self.press, self.temp, self.delh, self.pp, self.amount,
+                                      self.idgas, self.isogas, self.nlayin, self.imod,
+                                      self.layinc, self.scale, self.emtemp, self.isw,
+                                      self.vwave, self.vem, self.emis, self.pw)])
+
+
+def _requests(reqs):
+    keep = [_ReqArrays(r) for r in reqs]
+    arr = (Request * len(keep))(*[k.s for k in keep])
+    return keep, arr
+
+
+def batch_bytes(reqs, mode=MODE_MIXED) -> int:
+    """Device bytes a batch of these requests would allocate."""
+    keep, arr = _requests(reqs)
+    out = C.c_size_t(0)
+    check(lib().radg_batch_bytes(len(reqs), C.cast(arr, C.c_void_p), int(mode), C.byref(out)))
+    return int(out.value)
+
+
+class Batch:
+    """A set of requests packed into device memory and launched together."""
+
+    def __init__(self, reqs, mode=MODE_MIXED, budget_bytes=0):
+        self.reqs = list(reqs)
+        self.mode = mode
+        self._keep, arr = _requests(self.reqs)
+        h = C.c_void_p()
+        check(lib().radg_batch_create(len(self.reqs), C.cast(arr, C.c_void_p), int(mode),
+                                      C.c_size_t(budget_bytes), C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        """Launch on a HIP stream: ``None``, an int handle or a torch stream."""
+        sh = None
+        if stream is not None:
+            sh = stream if isinstance(stream, int) else stream.cuda_stream
+        check(lib().radg_batch_launch(self.h, C.c_void_p(sh)))
+
+    def sync(self):
+        check(lib().radg_batch_sync(self.h))
+
+    def read(self, r: int, weighted=True):
+        q = self.reqs[r]
+        nw, npath, L, npar = q.nwave, q.npath, q.nlayer, q.ngas + 1
+        out = np.zeros((nw, npath))
+        gts = np.zeros((nw, npath))
+        jac = np.zeros((nw, npath, L, npar))
+        res = dict(output=out, gtsurf=gts, doutputdq=jac)
+        wo = wg = wj = None
+        if weighted:
+            wo, wg, wj = np.zeros(nw), np.zeros(nw), np.zeros((nw, L, npar))
+            res.update(wout=wo, wgts=wg, wjac=wj)
+        p = lambda a: None if a is None else _vp(a)
+        check(lib().radg_batch_read(self.h, int(r), p(out), p(gts), p(jac), p(wo), p(wg), p(wj)))
+        return res
+
+    def timings(self):
+        ms = np.zeros(3, dtype=np.float32)
+        check(lib().radg_batch_timings(self.h, _vp(ms)))
+        return ms
+
+    def timing_stats(self):
+        """(launches, summed ms per kernel) over the last <=128 launches."""
+        n = C.c_int()
+        ms = np.zeros(3, dtype=np.float32)
+        check(lib().radg_batch_timing_stats(self.h, C.byref(n), _vp(ms)))
+        return n.value, ms
+
+    def info(self):
+        ab = np.zeros(3)
+        ns, na = C.c_int(), C.c_int()
+        check(lib().radg_batch_info(self.h, _vp(ab), C.byref(ns), C.byref(na)))
+        return dict(table_bytes=ab[0], output_bytes=ab[1], input_bytes=ab[2],
+                    alg_bytes=float(ab.sum()), nspectra=ns.value, items_a=na.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().radg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spectrum(req, mode=MODE_FP64):
+    """Run one request; returns ``(output, gtsurf, doutputdq)`` as the oracle."""
+    b = Batch([req], mode=mode)
+    try:
+        b.launch()
+        b.sync()
+        r = b.read(0, weighted=False)
+    finally:
+        b.close()
+    return r["output"], r["gtsurf"], r["doutputdq"]
+
+
+def fortran_arrays(req, real=np.float32):
+    """The request laid out as the Fortran caller holds it (arrdef.f limits)."""
+    L, ng, npath = req.nlayer, req.ngas, req.npath
+    F = dict(order="F")
+    a = dict(
+        delh=np.asarray(req.delh, dtype=real),
+        press=np.zeros(MAXLAY, real), temp=np.zeros(MAXLAY, real), hfp=np.zeros(MAXLAY, real),
+        pp=np.zeros((MAXLAY, MAXGAS), real, **F), amount=np.zeros((MAXLAY, MAXGAS), real, **F),
+        nlayin=np.zeros(MAXPAT, np.int32), layinc=np.zeros((MAXLAY, MAXPAT), np.int32, **F),
+        cont=np.zeros((MAXCON, MAXLAY), real, **F), scale=np.zeros((MAXLAY, MAXPAT), real, **F),
+        emtemp=np.zeros((MAXLAY, MAXPAT), real, **F), imod=np.zeros(MAXPAT, np.int32),
+        idgas=np.zeros(MAXGAS, np.int32), isogas=np.zeros(MAXGAS, np.int32),
+        vem=np.zeros(MAXSEC, real), emissivity=np.zeros(MAXSEC, real),
+        isw=np.zeros(NPARMAX, np.int32))
+    a["press"][:L] = req.press
+    a["temp"][:L] = req.temp
+    a["pp"][:L, :ng] = req.pp
+    a["amount"][:L, :ng] = req.amount
+    a["nlayin"][:npath] = req.nlayin
+    a["layinc"][:L, :npath] = req.layinc.T
+    a["scale"][:L, :npath] = req.scale.T
+    a["emtemp"][:L, :npath] = req.emtemp.T
+    a["imod"][:npath] = req.imod
+    a["idgas"][:ng] = req.idgas
+    a["isogas"][:ng] = req.isogas
+    a["vem"][:len(req.vem)] = req.vem
+    a["emissivity"][:len(req.emissivity)] = req.emissivity
+    a["isw"][:len(req.isw)] = req.isw
+    return a
+
+
+def cirsradg_wave(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp, amount,
+                  iwave, ispace, amform, vwave, nlayin, layinc, cont, scale, imod, idgas,
+                  isogas, emtemp, itype, nem, vem, emissivity, tsurf, gtsurf, radius1,
+                  flagh2p, hfp, radextra, nsw, isw, output, doutputdq):
+    """``CALL cirsradg_wave(...)`` with REAL(4) arrays (arrdef.f leading dims).
+
+    ``gtsurf``, ``output`` (``[MAXPAT]`` float32) and ``doutputdq``
+    (``[MAXPAT, MAXLAY, NPARMAX]`` float32, Fortran order) are written in place.
+    """
+    f4 = lambda x: C.byref(C.c_float(float(x)))
+    i4 = lambda x: C.byref(C.c_int(int(x)))
+    arr = lambda a, dt: _vp(_checked(a, dt))
+    lib().cirsradg_wave_(
+        f4(dist), i4(inormal), i4(iray), arr(delh, np.float32), i4(nlayer), i4(npath), i4(ngas),
+        arr(press, np.float32), arr(temp, np.float32), arr(pp, np.float32),
+        arr(amount, np.float32), i4(iwave), i4(ispace), i4(amform), f4(vwave),
+        arr(nlayin, np.int32), arr(layinc, np.int32), arr(cont, np.float32),
+        arr(scale, np.float32), arr(imod, np.int32), arr(idgas, np.int32),
+        arr(isogas, np.int32), arr(emtemp, np.float32), i4(itype), i4(nem),
+        arr(vem, np.float32), arr(emissivity, np.float32), f4(tsurf), arr(gtsurf, np.float32),
+        f4(radius1), i4(flagh2p), arr(hfp, np.float32), f4(radextra), i4(nsw),
+        arr(isw, np.int32), arr(output, np.float32), arr(doutputdq, np.float32))
+    check(lib().radg_cirsradg_status())
+
+
+def _checked(a, dt):
+    if not isinstance(a, np.ndarray) or a.dtype != dt:
+        raise TypeError(f"expected a numpy array of {np.dtype(dt).name}")
+    if not (a.flags.f_contiguous or a.ndim == 1):
+        raise TypeError("arrays must be Fortran-ordered")
+    return a

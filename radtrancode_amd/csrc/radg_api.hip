@@ -1,0 +1,1022 @@
+// Host side of the radg C ABI: table/CIA registries, request validation and
+// packing, device batches and the Fortran-surface entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/radg.h"
+#include "radg_dev.h"
+
+// launchers (radg_kernels.hip)
+size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64);
+size_t radg_lds_emission_bytes(int RL, int nsw, int fp64);
+int radg_layer_E(int ng);
+int radg_emission_RL(int maxlays);
+hipError_t radg_launch_layer_lds(const KernelArgs &a, int fp64, int E, size_t lds, hipStream_t st);
+hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, size_t lds, hipStream_t st);
+hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st);
+
+namespace {
+
+// ------------------------------------------------------------- errors
+thread_local std::string g_err;
+thread_local int g_cirs_status = 0;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess)                                                      \
+      return fail(RADG_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                   \
+  } while (0)
+
+int g_mode = RADG_MODE_FP64;
+std::mutex g_mu;
+
+// constdef.f:7-11
+constexpr double AMAGAT = 2.68675e19;
+constexpr double KBOLTZMANN = 1.37947e-23;
+
+// ------------------------------------------------------------- tables
+struct Table {
+  int idgas, isogas, npoint;
+  double vmin, delv;
+  long long off;              // element offset in the pool
+  std::vector<uint8_t> zero;  // per slice: every value <= 0 (get_kg.f:224-250)
+};
+
+struct TableSet {
+  int np = 0, nt = 0, ng = 0;
+  std::vector<double> lnp, tk, g_ord, delg, G;
+  std::vector<Table> tabs;
+  long long total = 0;
+  double *d_k64 = nullptr;
+  float *d_k32 = nullptr;
+  double *d_G = nullptr, *d_delg64 = nullptr;
+  float *d_delg32 = nullptr;
+  ~TableSet() {
+    if (d_k64) (void)hipFree(d_k64);
+    if (d_k32) (void)hipFree(d_k32);
+    if (d_G) (void)hipFree(d_G);
+    if (d_delg64) (void)hipFree(d_delg64);
+    if (d_delg32) (void)hipFree(d_delg32);
+  }
+};
+
+struct CIA {
+  double dnu = 0;
+  double *d_kcia = nullptr, *d_temps = nullptr;
+  ~CIA() {
+    if (d_kcia) (void)hipFree(d_kcia);
+    if (d_temps) (void)hipFree(d_temps);
+  }
+};
+
+std::shared_ptr<TableSet> g_tables;
+std::shared_ptr<CIA> g_cia;
+
+// frecipes/locate.f: 1-based JL in 0..n
+int locate(const double *xx, int n, double x) {
+  int jl = 0, ju = n + 1;
+  const bool asc = xx[n - 1] > xx[0];
+  while (ju - jl > 1) {
+    const int jm = (ju + jl) / 2;
+    if (asc == (x > xx[jm - 1])) jl = jm;
+    else ju = jm;
+  }
+  return jl;
+}
+
+// ------------------------------------------------------------- packing
+struct Packed {
+  DevReq d{};
+  std::vector<LayerConst> lay;
+  std::vector<double> amo, press, scale, emtemp, vwave, esurf, pweight;
+  std::vector<PathInfo> paths;
+  std::vector<int> layinc;
+  std::vector<WaveGas> wg;
+  double alg_bytes[3] = {0, 0, 0};  // table nodes, outputs, inputs
+};
+
+bool finite_pos(double x) { return std::isfinite(x) && x > 0; }
+
+int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P) {
+  const int L = q.nlayer, ngas = q.ngas, npath = q.npath, nwave = q.nwave;
+  if (L < 1 || L > RADG_MAXLAY) return fail(RADG_ELIMITS, "nlayer=%d outside 1..%d", L, RADG_MAXLAY);
+  if (npath < 1 || npath > RADG_MAXPAT) return fail(RADG_ELIMITS, "npath=%d outside 1..%d", npath, RADG_MAXPAT);
+  if (ngas < 1 || ngas > RADG_MAXGAS) return fail(RADG_ELIMITS, "ngas=%d outside 1..%d", ngas, RADG_MAXGAS);
+  if (nwave < 1) return fail(RADG_ELIMITS, "nwave=%d", nwave);
+  if (q.flagh2p == 1) return fail(RADG_EUNSUPPORTED, "flagh2p=1 (para-H2 CIA, nparacon_all) is not provided");
+  if (q.iray > 0 && q.itype != 0) return fail(RADG_EUNSUPPORTED, "Rayleigh scattering (iray>0, itype!=0) is not provided");
+  if (q.ispace != 0 && q.ispace != 1) return fail(RADG_EINPUT, "ispace=%d", q.ispace);
+  const int nparam = ngas + 1;
+  if (q.nsw < 0 || q.nsw > nparam) return fail(RADG_ELIMITS, "nsw=%d > nparam=%d", q.nsw, nparam);
+  for (int k = 0; k < q.nsw; k++)
+    if (q.isw[k] < 1 || q.isw[k] > nparam) return fail(RADG_EINPUT, "isw(%d)=%d outside 1..%d", k + 1, q.isw[k], nparam);
+  for (int k = 0; k < ngas; k++)
+    if (q.idgas[k] == 22) return fail(RADG_EUNSUPPORTED, "N2 (id 22) needs the analytic N2 CIA terms, not provided");
+  for (int j = 0; j < L; j++) {
+    if (!finite_pos(q.press[j]) || !finite_pos(q.temp[j]) || !std::isfinite(q.delh[j]))
+      return fail(RADG_EINPUT, "layer %d: press=%g temp=%g delh=%g", j + 1, q.press[j], q.temp[j], q.delh[j]);
+    for (int k = 0; k < ngas; k++)
+      if (!std::isfinite(q.amount[j * ngas + k]) || !std::isfinite(q.pp[j * ngas + k]) ||
+          q.amount[j * ngas + k] < 0 || q.pp[j * ngas + k] < 0)
+        return fail(RADG_EINPUT, "layer %d gas %d: amount/pp invalid", j + 1, k + 1);
+  }
+  int maxlays = 0;
+  for (int p = 0; p < npath; p++) {
+    const int nl = q.nlayin[p];
+    if (nl < 1 || nl > L) return fail(RADG_ELIMITS, "nlayin(%d)=%d outside 1..nlayer", p + 1, nl);
+    if (q.imod[p] != 0 && q.imod[p] != 1 && q.imod[p] != 3)
+      return fail(RADG_EUNSUPPORTED, "imod(%d)=%d: only 0, 1 and 3 are provided", p + 1, q.imod[p]);
+    for (int m = 0; m < nl; m++) {
+      const int li = q.layinc[(long long)p * L + m];
+      if (li < 1 || li > L) return fail(RADG_ELIMITS, "layinc(%d,%d)=%d exceeds nlayer", m + 1, p + 1, li);
+      if (!std::isfinite(q.scale[(long long)p * L + m]))
+        return fail(RADG_EINPUT, "scale(%d,%d) not finite", m + 1, p + 1);
+      if (q.imod[p] == 3 && !finite_pos(q.emtemp[(long long)p * L + m]))
+        return fail(RADG_EINPUT, "emtemp(%d,%d)=%g", m + 1, p + 1, q.emtemp[(long long)p * L + m]);
+    }
+    maxlays = std::max(maxlays, nl);
+  }
+
+  DevReq &d = P.d;
+  d.L = L;
+  d.ngas = ngas;
+  d.npath = npath;
+  d.nparam = nparam;
+  d.nsw = q.nsw;
+  d.nwave = nwave;
+  d.ispace = q.ispace;
+  d.inormal = q.inormal;
+  d.tsurf = q.tsurf;
+  d.tsurf_le0 = q.tsurf <= 0.0;
+  for (int k = 0; k < RADG_MAXPARAM; k++) d.isw[k] = k < q.nsw ? q.isw[k] - 1 : -1;
+  for (int l = 0; l < 5; l++) d.iabs[l] = -1;
+
+  // ---- per layer
+  const int np = ts.np, nt = ts.nt, NG = ts.ng;
+  P.lay.resize(L);
+  P.amo.resize((size_t)L * ngas);
+  P.press.assign(q.press, q.press + L);
+  const double MODBOLTZA = 10. * KBOLTZMANN / 1.013;
+  std::set<long long> nodes;  // distinct (cp,ct) corners touched
+  for (int j = 0; j < L; j++) {
+    LayerConst &c = P.lay[j];
+    // get_kg.f:113-144
+    double p1 = std::log(q.press[j]);
+    p1 = std::min(std::max(p1, ts.lnp[0]), ts.lnp[np - 1]);
+    int cp = locate(ts.lnp.data(), np, p1);
+    cp = std::min(std::max(cp, 1), np - 1);
+    c.cp = cp;
+    c.v = (p1 - ts.lnp[cp - 1]) / (ts.lnp[cp] - ts.lnp[cp - 1]);
+    double t1 = std::min(std::max(q.temp[j], ts.tk[0]), ts.tk[nt - 1]);
+    int ct = locate(ts.tk.data(), nt, t1);
+    ct = std::min(std::max(ct, 1), nt - 1);
+    c.ct = ct;
+    c.u = (t1 - ts.tk[ct - 1]) / (ts.tk[ct] - ts.tk[ct - 1]);
+    c.dudt = 1.0 / (ts.tk[ct] - ts.tk[ct - 1]);
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) nodes.insert((long long)(cp - 1 + a) * nt + (ct - 1 + b));
+    double ut = 0;
+    for (int k = 0; k < ngas; k++) {
+      ut += q.amount[j * ngas + k];
+      P.amo[(size_t)j * ngas + k] = q.amount[j * ngas + k] * 1.0e-20;
+    }
+    c.utotl = ut * 1.e-20;
+    c.temp = q.temp[j];
+    // nciacon.f:65-176
+    const double p = q.press[j], t = q.temp[j];
+    const double *pp = q.pp + (size_t)j * ngas;
+    const double *am = q.amount + (size_t)j * ngas;
+    c.qh2 = c.qhe = c.qn2 = c.qch4 = c.qco2 = 0;
+    for (int i = 0; i < ngas; i++) {
+      const int id = q.idgas[i], iso = q.isogas[i];
+      if (id == 39 && (iso == 0 || iso == 1)) { c.qh2 = pp[i] / p; d.iabs[0] = i; }
+      if (id == 40) { c.qhe = pp[i] / p; d.iabs[1] = i; }
+      if (id == 22) { c.qn2 = pp[i] / p; d.iabs[2] = i; }
+      if (id == 6 && (iso == 0 || iso == 1)) { c.qch4 = pp[i] / p; d.iabs[3] = i; }
+      if (id == 2) { c.qco2 = pp[i] / p; d.iabs[4] = i; }
+    }
+    const double xlen = q.delh[j];
+    if (xlen == 0.0) {
+      double xp1 = 0.0, height = 0.0;
+      for (int i = 0; i < ngas; i++)
+        if (pp[i] > xp1) {
+          height = am[i] * MODBOLTZA * t / pp[i];
+          xp1 = pp[i];
+        }
+      c.tau = ((p / 1.0) * (p / 1.0)) * ((273.15 / t) * (273.15 / t)) * height;
+    } else {
+      const double height = xlen * 1e5;
+      double xp1 = -1.0, totam = 0.0;
+      for (int i = 0; i < ngas; i++)
+        if (pp[i] > xp1 && pp[i] > 0.0) {
+          totam = am[i] * p / pp[i];
+          xp1 = pp[i];
+        }
+      const double amag1 = (totam / height) / AMAGAT;
+      c.tau = height * amag1 * amag1;
+    }
+    if (c.utotl <= 0.0 && (c.qh2 > 0 || c.qhe > 0))
+      return fail(RADG_EINPUT, "layer %d: zero total amount with CIA gases present", j + 1);
+  }
+
+  // ---- per path
+  P.paths.resize(npath);
+  P.layinc.assign((size_t)npath * L, 0);
+  P.scale.assign((size_t)npath * L, 0.0);
+  P.emtemp.assign((size_t)npath * L, 0.0);
+  P.pweight.assign(npath, 0.0);
+  for (int p = 0; p < npath; p++) {
+    const int nl = q.nlayin[p];
+    PathInfo &pi = P.paths[p];
+    pi.nlays = nl;
+    pi.imod = q.imod[p];
+    int j1 = (int)(0.5 * nl);
+    if (j1 < 1) j1 = 1;
+    const double pa = q.press[q.layinc[(long long)p * L + j1 - 1] - 1];
+    const double pb = q.press[q.layinc[(long long)p * L + nl - 1] - 1];
+    pi.surf = pb > pa;  // cirsradg_wave.f:992-999
+    for (int m = 0; m < nl; m++) {
+      P.layinc[(size_t)p * L + m] = q.layinc[(long long)p * L + m] - 1;
+      P.scale[(size_t)p * L + m] = q.scale[(long long)p * L + m];
+      P.emtemp[(size_t)p * L + m] = q.emtemp[(long long)p * L + m];
+    }
+    P.pweight[p] = q.path_weight ? q.path_weight[p] : 0.0;
+  }
+
+  // ---- per wavenumber
+  P.vwave.assign(q.vwave, q.vwave + nwave);
+  P.esurf.resize(nwave);
+  P.wg.resize((size_t)nwave * ngas);
+  const long long ntab = (long long)np * nt * NG;
+  const size_t kbytes = (mode == RADG_MODE_FP64) ? 8 : 4;
+  double tbytes = 0;
+  for (int iw = 0; iw < nwave; iw++) {
+    const double v = q.vwave[iw];
+    if (!std::isfinite(v) || v <= 0) return fail(RADG_EINPUT, "vwave(%d)=%g", iw + 1, v);
+    // interpem.f:17-30 (called unconditionally by cirsradg_wave.f:381)
+    const int nem = q.nem;
+    if (nem < 1 || v < q.vem[0] || v > q.vem[nem - 1])
+      return fail(RADG_EINPUT, "wavenumber %g outside the emissivity grid", v);
+    double es = 0.0;
+    for (int i = 0; i < nem - 1; i++)
+      if (v >= q.vem[i] && v < q.vem[i + 1]) {
+        const double f = (v - q.vem[i]) / (q.vem[i + 1] - q.vem[i]);
+        es = (1 - f) * q.emissivity[i] + f * q.emissivity[i + 1];
+      }
+    if (v == q.vem[nem - 1]) es = q.emissivity[nem - 1];
+    P.esurf[iw] = es;
+    for (int k = 0; k < ngas; k++) {
+      WaveGas &w = P.wg[(size_t)iw * ngas + k];
+      w.base = -1;
+      w.coinc = 1;
+      const Table *tb = nullptr;
+      for (const Table &t : ts.tabs) {  // read_klist.f:260-290: first covering match
+        const double xmax = t.vmin + (t.npoint - 1) * t.delv;
+        if (t.idgas == q.idgas[k] && t.isogas == q.isogas[k] && v >= t.vmin - 1e-5 && v <= xmax + 1e-5) {
+          tb = &t;
+          break;
+        }
+      }
+      if (!tb) continue;
+      const double eps = (tb->delv >= 0.0001) ? 0.02 * tb->delv : 0.25 * tb->delv;
+      long long n1 = (long long)((v + eps - tb->vmin) / tb->delv);
+      n1 = std::min<long long>(std::max<long long>(n1, 0), tb->npoint - 1);
+      const double tmp = tb->vmin + tb->delv * n1;
+      if (!(std::fabs(v - tmp) < eps))
+        return fail(RADG_EUNSUPPORTED,
+                    "wavenumber %g lies between table points of gas %d (non-coincident rankk branch "
+                    "is provided by the CPU oracle only)", v, q.idgas[k]);
+      if (tb->zero[n1]) continue;  // all-zero slice: k = 0
+      w.base = tb->off + n1 * ntab;
+      tbytes += (double)nodes.size() * NG * kbytes;
+    }
+  }
+  // algorithmic bytes: table nodes + layer inputs + CIA nodes + outputs
+  const double obytes = (double)nwave * ((double)npath * (2.0 + (double)L * nparam) * kbytes +
+                                         (2.0 + (double)L * nparam) * 8.0);
+  const double ibytes = (double)L * (4 + 2 * ngas) * 8.0 + (double)nwave * 4 * 9 * 8.0;
+  P.alg_bytes[0] = tbytes;
+  P.alg_bytes[1] = obytes;
+  P.alg_bytes[2] = ibytes;
+  (void)maxlays;
+  return RADG_OK;
+}
+
+}  // namespace
+
+// ============================================================ batches
+struct radg_batch {
+  int mode = 0, nreq = 0;
+  std::shared_ptr<TableSet> ts;
+  std::shared_ptr<CIA> cia;
+  std::vector<DevReq> reqs;
+  char *dmem = nullptr;
+  size_t bytes = 0;
+  KernelArgs args{};
+  int E = 1, RL = 1, maxngas = 1, maxnsw = 1;
+  size_t ldsA = 0, ldsB = 0;
+  double alg_bytes[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;
+  // event ring: 4 events per launch, the last RING launches are kept
+  static constexpr int RING = 128;
+  std::vector<hipEvent_t> evs;
+  long long launches = 0;
+  hipEvent_t *ev = nullptr;  // events of the last launch
+  ~radg_batch() {
+    if (dmem) (void)hipFree(dmem);
+    for (auto &e : evs)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+struct Layout {
+  size_t off = 0;
+  template <typename X>
+  size_t take(size_t n) {
+    off = (off + 255) & ~size_t(255);
+    const size_t o = off;
+    off += n * sizeof(X);
+    return o;
+  }
+};
+
+struct Plan {
+  std::vector<Packed> packs;
+  size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
+      o_vw, o_es, o_wg, o_pw, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac, o_err;
+  size_t total = 0;
+  long long nA = 0, nB = 0, nC = 0;
+  long long nlay = 0, nlg = 0, npath = 0, npl = 0, nwv = 0, nwg = 0, ninter = 0, nout = 0, njac = 0,
+            nwout = 0, nwjac = 0;
+  std::vector<long long> sA, sB, sC;
+  int maxngas = 1, maxnsw = 1, maxlays = 1;
+};
+
+int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts, Plan &pl) {
+  if (nreq < 1) return fail(RADG_EINPUT, "nreq=%d", nreq);
+  pl.packs.resize(nreq);
+  pl.sA.assign(nreq + 1, 0);
+  pl.sB.assign(nreq + 1, 0);
+  pl.sC.assign(nreq + 1, 0);
+  for (int r = 0; r < nreq; r++) {
+    int rc = pack_request(reqs[r], ts, mode, pl.packs[r]);
+    if (rc) return rc;
+    DevReq &d = pl.packs[r].d;
+    const radg_request &q = reqs[r];
+    d.lay0 = pl.nlay;
+    d.lg0 = pl.nlg;
+    d.path0 = pl.npath;
+    d.pl0 = pl.npl;
+    d.wave0 = pl.nwv;
+    d.wg0 = pl.nwg;
+    d.itemA0 = pl.nA;
+    d.itemB0 = pl.nB;
+    d.itemC0 = pl.nC;
+    d.inter0 = pl.ninter;
+    d.out0 = pl.nout;
+    d.jac0 = pl.njac;
+    d.wout0 = pl.nwout;
+    d.wjac0 = pl.nwjac;
+    pl.nlay += q.nlayer;
+    pl.nlg += (long long)q.nlayer * q.ngas;
+    pl.npath += q.npath;
+    pl.npl += (long long)q.npath * q.nlayer;
+    pl.nwv += q.nwave;
+    pl.nwg += (long long)q.nwave * q.ngas;
+    pl.nA += (long long)q.nwave * q.nlayer;
+    pl.nB += (long long)q.nwave * q.npath;
+    pl.nC += q.nwave;
+    pl.ninter += (long long)q.nwave * inter_block(ts.ng, q.nlayer, q.ngas);
+    pl.nout += (long long)q.nwave * q.npath;
+    pl.njac += (long long)q.nwave * q.npath * q.nlayer * (q.ngas + 1);
+    pl.nwout += q.nwave;
+    pl.nwjac += (long long)q.nwave * q.nlayer * (q.ngas + 1);
+    pl.sA[r + 1] = pl.nA;
+    pl.sB[r + 1] = pl.nB;
+    pl.sC[r + 1] = pl.nC;
+    pl.maxngas = std::max(pl.maxngas, q.ngas);
+    pl.maxnsw = std::max(pl.maxnsw, std::max(q.nsw, 1));
+    pl.maxlays = std::max(pl.maxlays, q.nlayer);
+  }
+  const size_t ts_ = (mode == RADG_MODE_FP64) ? 8 : 4;
+  Layout lo;
+  pl.o_reqs = lo.take<DevReq>(nreq);
+  pl.o_sA = lo.take<long long>(nreq + 1);
+  pl.o_sB = lo.take<long long>(nreq + 1);
+  pl.o_sC = lo.take<long long>(nreq + 1);
+  pl.o_lay = lo.take<LayerConst>(pl.nlay);
+  pl.o_amo = lo.take<double>(pl.nlg);
+  pl.o_press = lo.take<double>(pl.nlay);
+  pl.o_paths = lo.take<PathInfo>(pl.npath);
+  pl.o_layinc = lo.take<int>(pl.npl);
+  pl.o_scale = lo.take<double>(pl.npl);
+  pl.o_emt = lo.take<double>(pl.npl);
+  pl.o_vw = lo.take<double>(pl.nwv);
+  pl.o_es = lo.take<double>(pl.nwv);
+  pl.o_wg = lo.take<WaveGas>(pl.nwg);
+  pl.o_pw = lo.take<double>(pl.npath);
+  pl.o_inter = lo.take<char>((size_t)pl.ninter * ts_);
+  pl.o_out = lo.take<char>((size_t)pl.nout * ts_);
+  pl.o_gts = lo.take<char>((size_t)pl.nout * ts_);
+  pl.o_jac = lo.take<char>((size_t)pl.njac * ts_);
+  pl.o_wout = lo.take<double>(pl.nwout);
+  pl.o_wgts = lo.take<double>(pl.nwout);
+  pl.o_wjac = lo.take<double>(pl.nwjac);
+  pl.o_err = lo.take<int>(4);
+  pl.total = lo.off;
+  return RADG_OK;
+}
+
+template <typename X>
+void put(std::vector<char> &h, size_t off, const X *src, size_t n) {
+  if (n) std::memcpy(h.data() + off, src, n * sizeof(X));
+}
+
+}  // namespace
+
+extern "C" {
+
+int radg_set_mode(int mode) {
+  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED) return fail(RADG_EINPUT, "mode=%d", mode);
+  g_mode = mode;
+  return RADG_OK;
+}
+int radg_get_mode(void) { return g_mode; }
+const char *radg_last_error(void) { return g_err.c_str(); }
+
+int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const int *npoint,
+                          const double *vmin, const double *delv, int np, int nt, int ng,
+                          const double *press_atm, const double *temp, const double *g_ord,
+                          const double *delg, const double *const *k) {
+  if (ntab < 1) return fail(RADG_EINPUT, "ntab=%d", ntab);
+  if (np < 2 || np > RADG_MAXK || nt < 2 || nt > RADG_MAXK)
+    return fail(RADG_ELIMITS, "np=%d nt=%d outside 2..%d", np, nt, RADG_MAXK);
+  if (ng < 1 || ng > RADG_GPU_MAXG)
+    return fail(RADG_ELIMITS, "ng=%d outside 1..%d (the overlap sort keeps NG*NG keys in one wavefront)",
+                ng, RADG_GPU_MAXG);
+  auto ts = std::make_shared<TableSet>();
+  ts->np = np;
+  ts->nt = nt;
+  ts->ng = ng;
+  for (int i = 0; i < np; i++) {
+    if (!finite_pos(press_atm[i])) return fail(RADG_EINPUT, "table pressure %d = %g", i + 1, press_atm[i]);
+    ts->lnp.push_back(std::log(press_atm[i]));  // read_khead.f:80-85
+  }
+  ts->tk.assign(temp, temp + nt);
+  ts->g_ord.assign(g_ord, g_ord + ng);
+  ts->delg.assign(delg, delg + ng);
+  double wmin = 1e300, wmax = 0;
+  for (int i = 0; i < ng; i++) {
+    if (!(delg[i] > 0)) return fail(RADG_EINPUT, "delg(%d)=%g must be positive", i + 1, delg[i]);
+    wmin = std::min(wmin, delg[i]);
+    wmax = std::max(wmax, delg[i]);
+  }
+  // Every mixed weight delg_i*delg_j must be narrower than every g-bin, so
+  // that one element never spans two bin edges; rankg.f advances one bin
+  // per element and the parallel rebin relies on that never mattering.
+  if (ng > 1 && wmax * wmax >= wmin)
+    return fail(RADG_EUNSUPPORTED, "g-weights with max(delg)^2 >= min(delg) are not supported");
+  ts->G.assign(ng + 1, 0.0);
+  for (int i = 1; i <= ng; i++) ts->G[i] = ts->G[i - 1] + delg[i - 1];
+  if (ts->G[ng] < 1.0) ts->G[ng] = 1.0;  // rankg.f:65
+  const long long per = (long long)np * nt * ng;
+  long long total = 0;
+  for (int t = 0; t < ntab; t++) {
+    if (npoint[t] < 1 || !(delv[t] > 0)) return fail(RADG_EUNSUPPORTED, "table %d: npoint=%d delv=%g (regular grids with DELV>0 only)", t + 1, npoint[t], delv[t]);
+    Table tb;
+    tb.idgas = idgas[t];
+    tb.isogas = isogas[t];
+    tb.npoint = npoint[t];
+    tb.vmin = vmin[t];
+    tb.delv = delv[t];
+    tb.off = total;
+    tb.zero.assign(npoint[t], 1);
+    for (long long s = 0; s < npoint[t]; s++) {
+      const double *sl = k[t] + s * per;
+      for (long long i = 0; i < per; i++) {
+        if (!std::isfinite(sl[i]) || std::fabs(sl[i]) > 1e30)
+          return fail(RADG_EINPUT, "table %d: value %g at wavenumber %lld is not finite or > 1e30",
+                      t + 1, sl[i], s + 1);
+        if (sl[i] > 0.0) tb.zero[s] = 0;
+      }
+    }
+    total += (long long)npoint[t] * per;
+    ts->tabs.push_back(std::move(tb));
+  }
+  ts->total = total;
+  HIPCHK(hipMalloc(&ts->d_k64, total * sizeof(double)));
+  HIPCHK(hipMalloc(&ts->d_k32, total * sizeof(float)));
+  std::vector<float> tmp;
+  for (int t = 0; t < ntab; t++) {
+    const long long n = (long long)npoint[t] * per;
+    HIPCHK(hipMemcpy(ts->d_k64 + ts->tabs[t].off, k[t], n * sizeof(double), hipMemcpyHostToDevice));
+    tmp.resize(n);
+    for (long long i = 0; i < n; i++) tmp[i] = (float)k[t][i];
+    HIPCHK(hipMemcpy(ts->d_k32 + ts->tabs[t].off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  std::vector<float> dg32(delg, delg + ng);
+  HIPCHK(hipMalloc(&ts->d_G, (ng + 1) * sizeof(double)));
+  HIPCHK(hipMalloc(&ts->d_delg64, ng * sizeof(double)));
+  HIPCHK(hipMalloc(&ts->d_delg32, ng * sizeof(float)));
+  HIPCHK(hipMemcpy(ts->d_G, ts->G.data(), (ng + 1) * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ts->d_delg64, delg, ng * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ts->d_delg32, dg32.data(), ng * sizeof(float), hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tables = ts;
+  return RADG_OK;
+}
+
+int radg_register_cia(double dnu, const double *temps, const double *kcia) {
+  if (!(dnu > 0)) return fail(RADG_EINPUT, "dnu=%g", dnu);
+  for (int i = 1; i < 25; i++)
+    if (!(temps[i] > temps[i - 1])) return fail(RADG_EINPUT, "CIA temperatures must increase");
+  auto c = std::make_shared<CIA>();
+  c->dnu = dnu;
+  const size_t n = (size_t)1501 * 25 * 9;
+  HIPCHK(hipMalloc(&c->d_kcia, n * sizeof(double)));
+  HIPCHK(hipMalloc(&c->d_temps, 25 * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_kcia, kcia, n * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_temps, temps, 25 * sizeof(double), hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_cia = c;
+  return RADG_OK;
+}
+
+int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes) {
+  std::shared_ptr<TableSet> ts;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ts = g_tables;
+  }
+  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
+  Plan pl;
+  int rc = plan_batch(nreq, reqs, mode, *ts, pl);
+  if (rc) return rc;
+  *bytes = pl.total;
+  return RADG_OK;
+}
+
+int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, radg_batch **out) {
+  *out = nullptr;
+  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED) return fail(RADG_EINPUT, "mode=%d", mode);
+  std::shared_ptr<TableSet> ts;
+  std::shared_ptr<CIA> cia;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ts = g_tables;
+    cia = g_cia;
+  }
+  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
+  if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
+  Plan pl;
+  int rc = plan_batch(nreq, reqs, mode, *ts, pl);
+  if (rc) return rc;
+  if (budget && pl.total > budget)
+    return fail(RADG_EBUDGET, "batch needs %zu bytes, budget %zu", pl.total, budget);
+  std::unique_ptr<radg_batch> b(new radg_batch);
+  b->mode = mode;
+  b->nreq = nreq;
+  b->ts = ts;
+  b->cia = cia;
+  b->bytes = pl.total;
+  HIPCHK(hipMalloc(&b->dmem, pl.total));
+  // host staging of everything but the intermediates and outputs
+  std::vector<char> h(pl.o_inter, 0);
+  std::vector<DevReq> dr(nreq);
+  for (int r = 0; r < nreq; r++) dr[r] = pl.packs[r].d;
+  put(h, pl.o_reqs, dr.data(), nreq);
+  put(h, pl.o_sA, pl.sA.data(), nreq + 1);
+  put(h, pl.o_sB, pl.sB.data(), nreq + 1);
+  put(h, pl.o_sC, pl.sC.data(), nreq + 1);
+  for (int r = 0; r < nreq; r++) {
+    const Packed &P = pl.packs[r];
+    const DevReq &d = P.d;
+    put(h, pl.o_lay + d.lay0 * sizeof(LayerConst), P.lay.data(), P.lay.size());
+    put(h, pl.o_amo + d.lg0 * sizeof(double), P.amo.data(), P.amo.size());
+    put(h, pl.o_press + d.lay0 * sizeof(double), P.press.data(), P.press.size());
+    put(h, pl.o_paths + d.path0 * sizeof(PathInfo), P.paths.data(), P.paths.size());
+    put(h, pl.o_layinc + d.pl0 * sizeof(int), P.layinc.data(), P.layinc.size());
+    put(h, pl.o_scale + d.pl0 * sizeof(double), P.scale.data(), P.scale.size());
+    put(h, pl.o_emt + d.pl0 * sizeof(double), P.emtemp.data(), P.emtemp.size());
+    put(h, pl.o_vw + d.wave0 * sizeof(double), P.vwave.data(), P.vwave.size());
+    put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
+    put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
+    put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
+    for (int i = 0; i < 3; i++) b->alg_bytes[i] += P.alg_bytes[i];
+  }
+  HIPCHK(hipMemcpy(b->dmem, h.data(), h.size(), hipMemcpyHostToDevice));
+  b->reqs = dr;
+  KernelArgs &a = b->args;
+  char *m = b->dmem;
+  a.reqs = reinterpret_cast<const DevReq *>(m + pl.o_reqs);
+  a.nreq = nreq;
+  a.startA = reinterpret_cast<const long long *>(m + pl.o_sA);
+  a.startB = reinterpret_cast<const long long *>(m + pl.o_sB);
+  a.startC = reinterpret_cast<const long long *>(m + pl.o_sC);
+  a.nA = pl.nA;
+  a.nB = pl.nB;
+  a.nC = pl.nC;
+  a.ng = ts->ng;
+  a.ktab = (mode == RADG_MODE_FP64) ? (const void *)ts->d_k64 : (const void *)ts->d_k32;
+  a.np = ts->np;
+  a.nt = ts->nt;
+  a.G = ts->d_G;
+  a.delg64 = ts->d_delg64;
+  a.delg32 = ts->d_delg32;
+  a.kcia = cia->d_kcia;
+  a.cia_temps = cia->d_temps;
+  a.cia_dnu = cia->dnu;
+  a.lay = reinterpret_cast<const LayerConst *>(m + pl.o_lay);
+  a.amo = reinterpret_cast<const double *>(m + pl.o_amo);
+  a.press = reinterpret_cast<const double *>(m + pl.o_press);
+  a.paths = reinterpret_cast<const PathInfo *>(m + pl.o_paths);
+  a.layinc = reinterpret_cast<const int *>(m + pl.o_layinc);
+  a.scale = reinterpret_cast<const double *>(m + pl.o_scale);
+  a.emtemp = reinterpret_cast<const double *>(m + pl.o_emt);
+  a.vwave = reinterpret_cast<const double *>(m + pl.o_vw);
+  a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
+  a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
+  a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
+  a.inter = m + pl.o_inter;
+  a.out = m + pl.o_out;
+  a.gts = m + pl.o_gts;
+  a.jac = m + pl.o_jac;
+  a.wout = reinterpret_cast<double *>(m + pl.o_wout);
+  a.wgts = reinterpret_cast<double *>(m + pl.o_wgts);
+  a.wjac = reinterpret_cast<double *>(m + pl.o_wjac);
+  a.err = reinterpret_cast<int *>(m + pl.o_err);
+  const int fp64 = mode == RADG_MODE_FP64;
+  b->E = radg_layer_E(ts->ng);
+  b->maxngas = pl.maxngas;
+  b->maxnsw = pl.maxnsw;
+  b->RL = radg_emission_RL(pl.maxlays);
+  b->ldsA = radg_lds_layer_bytes(ts->ng, pl.maxngas, b->E, fp64);
+  b->ldsB = radg_lds_emission_bytes(b->RL, pl.maxnsw, fp64);
+  if (b->ldsA > 160 * 1024 || b->ldsB > 160 * 1024)
+    return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", b->ldsA, b->ldsB);
+  b->evs.assign(4 * radg_batch::RING, nullptr);
+  for (auto &e : b->evs) HIPCHK(hipEventCreate(&e));
+  b->ev = &b->evs[0];
+  *out = b.release();
+  return RADG_OK;
+}
+
+int radg_batch_launch(radg_batch *b, void *stream) {
+  if (!b) return fail(RADG_EINPUT, "null batch");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  b->stream = st;
+  const int fp64 = b->mode == RADG_MODE_FP64;
+  b->ev = &b->evs[4 * (b->launches % radg_batch::RING)];
+  b->launches++;
+  HIPCHK(hipEventRecord(b->ev[0], st));
+  HIPCHK(radg_launch_layer_lds(b->args, fp64, b->E, b->ldsA, st));
+  HIPCHK(hipEventRecord(b->ev[1], st));
+  HIPCHK(radg_launch_emission(b->args, fp64, b->RL, b->ldsB, st));
+  HIPCHK(hipEventRecord(b->ev[2], st));
+  HIPCHK(radg_launch_quadrature(b->args, fp64, st));
+  HIPCHK(hipEventRecord(b->ev[3], st));
+  return RADG_OK;
+}
+
+int radg_batch_sync(radg_batch *b) {
+  if (!b) return fail(RADG_EINPUT, "null batch");
+  if (b->launches == 0) return RADG_OK;
+  HIPCHK(hipEventSynchronize(b->ev[3]));
+  return RADG_OK;
+}
+
+int radg_batch_timings(radg_batch *b, float *ms3) {
+  if (!b) return fail(RADG_EINPUT, "null batch");
+  if (b->launches == 0) return fail(RADG_EINPUT, "batch was never launched");
+  HIPCHK(hipEventSynchronize(b->ev[3]));
+  for (int i = 0; i < 3; i++) HIPCHK(hipEventElapsedTime(&ms3[i], b->ev[i], b->ev[i + 1]));
+  return RADG_OK;
+}
+
+int radg_batch_timing_stats(radg_batch *b, int *nlaunch, float *sum_ms3) {
+  if (!b) return fail(RADG_EINPUT, "null batch");
+  const long long n = std::min<long long>(b->launches, radg_batch::RING);
+  for (int i = 0; i < 3; i++) sum_ms3[i] = 0.f;
+  for (long long l = b->launches - n; l < b->launches; l++) {
+    hipEvent_t *e = &b->evs[4 * (l % radg_batch::RING)];
+    HIPCHK(hipEventSynchronize(e[3]));
+    for (int i = 0; i < 3; i++) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+      sum_ms3[i] += ms;
+    }
+  }
+  *nlaunch = (int)n;
+  return RADG_OK;
+}
+
+int radg_batch_info(radg_batch *b, double *alg_bytes3, int *nspectra, int *nitems_a) {
+  if (!b) return fail(RADG_EINPUT, "null batch");
+  if (alg_bytes3)
+    for (int i = 0; i < 3; i++) alg_bytes3[i] = b->alg_bytes[i];
+  if (nspectra) *nspectra = b->nreq;
+  if (nitems_a) *nitems_a = (int)b->args.nA;
+  return RADG_OK;
+}
+
+int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf, double *doutputdq,
+                    double *wout, double *wgts, double *wjac) {
+  if (!b || r < 0 || r >= b->nreq) return fail(RADG_EINPUT, "bad batch/request index");
+  if (b->launches == 0) return fail(RADG_EINPUT, "batch was never launched");
+  HIPCHK(hipEventSynchronize(b->ev[3]));
+  const DevReq &d = b->reqs[r];
+  const KernelArgs &a = b->args;
+  const bool fp64 = b->mode == RADG_MODE_FP64;
+  auto fetch = [&](const void *base, long long off, long long n, double *dst) -> int {
+    if (!dst) return RADG_OK;
+    if (fp64) {
+      HIPCHK(hipMemcpy(dst, reinterpret_cast<const double *>(base) + off, n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> t(n);
+      HIPCHK(hipMemcpy(t.data(), reinterpret_cast<const float *>(base) + off, n * sizeof(float),
+                       hipMemcpyDeviceToHost));
+      for (long long i = 0; i < n; i++) dst[i] = t[i];
+    }
+    return RADG_OK;
+  };
+  const long long nout = (long long)d.nwave * d.npath;
+  int rc;
+  if ((rc = fetch(a.out, d.out0, nout, output))) return rc;
+  if ((rc = fetch(a.gts, d.out0, nout, gtsurf))) return rc;
+  if ((rc = fetch(a.jac, d.jac0, nout * d.L * d.nparam, doutputdq))) return rc;
+  if (wout) HIPCHK(hipMemcpy(wout, a.wout + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
+  if (wgts) HIPCHK(hipMemcpy(wgts, a.wgts + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
+  if (wjac)
+    HIPCHK(hipMemcpy(wjac, a.wjac + d.wjac0, (size_t)d.nwave * d.L * d.nparam * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return RADG_OK;
+}
+
+int radg_batch_destroy(radg_batch *b) {
+  delete b;
+  return RADG_OK;
+}
+
+}  // extern "C"
+
+// ====================================================== Fortran surface
+namespace {
+
+template <typename R>
+int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R *temp,
+              const R *pp, const R *amount, int ispace, int inormal, int iray, int itype,
+              int flagh2p, const int *nlayin, const int *layinc, const R *scale, const int *imod,
+              const int *idgas, const int *isogas, const R *emtemp, int nem, const R *vem,
+              const R *emis, R tsurf, int nsw, const int *isw, int nwave, const R *vwave,
+              std::vector<double> &out, std::vector<double> &gts, std::vector<double> &jac) {
+  if (L < 1 || L > RADG_MAXLAY || npath < 1 || npath > RADG_MAXPAT || ngas < 1 || ngas > RADG_MAXGAS)
+    return fail(RADG_ELIMITS, "nlayer=%d npath=%d ngas=%d exceed the array limits", L, npath, ngas);
+  if (nem < 1 || nem > RADG_MAXSEC) return fail(RADG_ELIMITS, "nem=%d", nem);
+  // fixed leading dimensions (arrdef.f) -> compact
+  std::vector<double> pr(press, press + L), tp(temp, temp + L), dh(delh, delh + L);
+  std::vector<double> ppc((size_t)L * ngas), amc((size_t)L * ngas);
+  for (int j = 0; j < L; j++)
+    for (int k = 0; k < ngas; k++) {
+      ppc[(size_t)j * ngas + k] = pp[(size_t)k * RADG_MAXLAY + j];
+      amc[(size_t)j * ngas + k] = amount[(size_t)k * RADG_MAXLAY + j];
+    }
+  std::vector<int> li((size_t)npath * L, 1);
+  std::vector<double> sc((size_t)npath * L, 0.0), et((size_t)npath * L, 0.0);
+  for (int p = 0; p < npath; p++)
+    for (int m = 0; m < L; m++) {
+      li[(size_t)p * L + m] = layinc[(size_t)p * RADG_MAXLAY + m];
+      sc[(size_t)p * L + m] = scale[(size_t)p * RADG_MAXLAY + m];
+      et[(size_t)p * L + m] = emtemp[(size_t)p * RADG_MAXLAY + m];
+    }
+  std::vector<double> vw(vwave, vwave + nwave), ve(vem, vem + nem), em(emis, emis + nem);
+  radg_request q{};
+  q.nlayer = L;
+  q.npath = npath;
+  q.ngas = ngas;
+  q.nsw = nsw;
+  q.nwave = nwave;
+  q.nem = nem;
+  q.inormal = inormal;
+  q.iray = iray;
+  q.ispace = ispace;
+  q.itype = itype;
+  q.flagh2p = flagh2p;
+  q.tsurf = tsurf;
+  q.press = pr.data();
+  q.temp = tp.data();
+  q.delh = dh.data();
+  q.pp = ppc.data();
+  q.amount = amc.data();
+  q.idgas = idgas;
+  q.isogas = isogas;
+  q.nlayin = nlayin;
+  q.imod = imod;
+  q.layinc = li.data();
+  q.scale = sc.data();
+  q.emtemp = et.data();
+  q.isw = isw;
+  q.vwave = vw.data();
+  q.vem = ve.data();
+  q.emissivity = em.data();
+  q.path_weight = nullptr;
+  radg_batch *b = nullptr;
+  int rc = radg_batch_create(1, &q, g_mode, 0, &b);
+  if (rc) return rc;
+  std::unique_ptr<radg_batch> guard(b);
+  if ((rc = radg_batch_launch(b, nullptr))) return rc;
+  if ((rc = radg_batch_sync(b))) return rc;
+  out.assign((size_t)nwave * npath, 0.0);
+  gts.assign((size_t)nwave * npath, 0.0);
+  jac.assign((size_t)nwave * npath * L * (ngas + 1), 0.0);
+  return radg_batch_read(b, 0, out.data(), gts.data(), jac.data(), nullptr, nullptr, nullptr);
+}
+
+template <typename R>
+int spectrum_fixed(const R *dist, const int *inormal, const int *iray, const R *delh,
+                   const int *nlayer, const int *npath, const int *ngas, const R *press,
+                   const R *temp, const R *pp, const R *amount, const int *ispace,
+                   const int *nlayin, const int *layinc, const R *scale, const int *imod,
+                   const int *idgas, const int *isogas, const R *emtemp, const int *itype,
+                   const int *nem, const R *vem, const R *emissivity, const R *tsurf,
+                   const int *flagh2p, const int *nsw, const int *isw, int nwave, const R *vwave,
+                   R *output, R *gtsurf, R *doutputdq) {
+  (void)dist;
+  std::vector<double> o, g, j;
+  int rc = run_fixed<R>(delh, *nlayer, *npath, *ngas, press, temp, pp, amount, *ispace, *inormal,
+                        *iray, *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas,
+                        emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, o, g, j);
+  if (rc) return rc;
+  for (size_t i = 0; i < o.size(); i++) {
+    output[i] = (R)o[i];
+    gtsurf[i] = (R)g[i];
+  }
+  for (size_t i = 0; i < j.size(); i++) doutputdq[i] = (R)j[i];
+  return RADG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cirsradg_wave_(const float *dist, const int *inormal, const int *iray, const float *delh,
+                    const int *nlayer, const int *npath, const int *ngas, const float *press,
+                    const float *temp, const float *pp, const float *amount, const int *iwave,
+                    const int *ispace, const int *amform, const float *vwave, const int *nlayin,
+                    const int *layinc, const float *cont, const float *scale, const int *imod,
+                    const int *idgas, const int *isogas, const float *emtemp, const int *itype,
+                    const int *nem, const float *vem, const float *emissivity, const float *tsurf,
+                    float *gtsurf, const float *radius1, const int *flagh2p, const float *hfp,
+                    const float *radextra, const int *nsw, const int *isw, float *output,
+                    float *doutputdq) {
+  (void)dist; (void)iwave; (void)amform; (void)cont; (void)radius1; (void)hfp; (void)radextra;
+  std::vector<double> o, g, j;
+  const int L = *nlayer, np_ = *npath, ng = *ngas;
+  int rc = run_fixed<float>(delh, L, np_, ng, press, temp, pp, amount, *ispace, *inormal, *iray,
+                            *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
+                            *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, o, g, j);
+  g_cirs_status = rc;
+  if (rc) return;
+  const int nparam = ng + 1;
+  for (int p = 0; p < np_; p++) {
+    output[p] = (float)o[p];
+    gtsurf[p] = (float)g[p];
+    // doutputdq(maxpat,maxlay,maxgas+2+maxcon): zero every row, then fill
+    for (int m = 0; m < RADG_MAXLAY; m++)
+      for (int k = 0; k < nparam; k++)
+        doutputdq[p + (size_t)RADG_MAXPAT * (m + (size_t)RADG_MAXLAY * k)] =
+            m < L ? (float)j[((size_t)p * L + m) * nparam + k] : 0.0f;
+  }
+}
+
+int radg_cirsradg_status(void) { return g_cirs_status; }
+
+int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray, const float *delh,
+                     const int *nlayer, const int *npath, const int *ngas, const float *press,
+                     const float *temp, const float *pp, const float *amount, const int *ispace,
+                     const int *nlayin, const int *layinc, const float *scale, const int *imod,
+                     const int *idgas, const int *isogas, const float *emtemp, const int *itype,
+                     const int *nem, const float *vem, const float *emissivity, const float *tsurf,
+                     const int *flagh2p, const int *nsw, const int *isw, int nwave,
+                     const float *vwave, float *output, float *gtsurf, float *doutputdq) {
+  return spectrum_fixed<float>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
+                               amount, ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
+                               itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave,
+                               output, gtsurf, doutputdq);
+}
+
+int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, const double *delh,
+                     const int *nlayer, const int *npath, const int *ngas, const double *press,
+                     const double *temp, const double *pp, const double *amount, const int *ispace,
+                     const int *nlayin, const int *layinc, const double *scale, const int *imod,
+                     const int *idgas, const int *isogas, const double *emtemp, const int *itype,
+                     const int *nem, const double *vem, const double *emissivity,
+                     const double *tsurf, const int *flagh2p, const int *nsw, const int *isw,
+                     int nwave, const double *vwave, double *output, double *gtsurf,
+                     double *doutputdq) {
+  return spectrum_fixed<double>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
+                                amount, ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
+                                itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave,
+                                output, gtsurf, doutputdq);
+}
+
+// .kta reader (read_khead.f:18-120; regular grids, NT > 0)
+int radg_read_klist(const char *klist_path, int recl) {
+  if (recl != 4 && recl != 8) return fail(RADG_EINPUT, "recl=%d (4 or 8)", recl);
+  std::ifstream kl(klist_path);
+  if (!kl) return fail(RADG_EIO, "cannot open %s", klist_path);
+  std::vector<std::string> names;
+  std::string line;
+  while (std::getline(kl, line)) {
+    const auto a = line.find_first_not_of(" \t\r");
+    if (a == std::string::npos) break;  // read_klist.f:80: a blank line ends the list
+    const auto z = line.find_last_not_of(" \t\r");
+    std::string nm = line.substr(a, z - a + 1);
+    if (nm.size() < 4 || nm.substr(nm.size() - 4) != ".kta") nm += ".kta";
+    names.push_back(nm);
+  }
+  if (names.empty()) return fail(RADG_EIO, "%s lists no tables", klist_path);
+  std::vector<int> ids, isos, npts;
+  std::vector<double> vmins, delvs, press, temps, gord, delg;
+  std::vector<std::vector<double>> data;
+  int np0 = 0, nt0 = 0, ng0 = 0;
+  for (const auto &nm : names) {
+    std::ifstream f(nm, std::ios::binary);
+    if (!f) return fail(RADG_EIO, "cannot open %s", nm.c_str());
+    std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const size_t nrec = raw.size() / recl;
+    auto gi = [&](size_t r) {
+      int32_t v;
+      std::memcpy(&v, raw.data() + (r - 1) * recl, 4);
+      return (int)v;
+    };
+    auto gr = [&](size_t r) {
+      if (recl == 4) {
+        float v;
+        std::memcpy(&v, raw.data() + (r - 1) * recl, 4);
+        return (double)v;
+      }
+      double v;
+      std::memcpy(&v, raw.data() + (r - 1) * recl, 8);
+      return v;
+    };
+    if (nrec < 10) return fail(RADG_EIO, "%s: truncated header", nm.c_str());
+    const int irec0 = gi(1), npoint = gi(2), np = gi(6), nt = gi(7), ng = gi(8);
+    const double vmin = gr(3), delv = gr(4);
+    if (nt <= 0 || delv <= 0) return fail(RADG_EUNSUPPORTED, "%s: NT<=0 or DELV<=0 tables are not supported", nm.c_str());
+    if (np < 2 || np > RADG_MAXK || nt < 2 || nt > RADG_MAXK || ng < 1 || ng > RADG_MAXG - 1)
+      return fail(RADG_ELIMITS, "%s: np=%d nt=%d ng=%d", nm.c_str(), np, nt, ng);
+    const size_t per = (size_t)np * nt * ng;
+    if ((size_t)irec0 - 1 + (size_t)npoint * per > nrec) return fail(RADG_EIO, "%s: truncated data", nm.c_str());
+    if (ids.empty()) {
+      np0 = np; nt0 = nt; ng0 = ng;
+      for (int j = 0; j < ng; j++) gord.push_back(gr(11 + j));
+      for (int j = 0; j < ng; j++) delg.push_back(gr(11 + ng + j));
+      const size_t r0 = 11 + 2 * ng + 2;
+      for (int j = 0; j < np; j++) press.push_back(gr(r0 + j));
+      for (int j = 0; j < nt; j++) temps.push_back(gr(r0 + np + j));
+    } else if (np != np0 || nt != nt0 || ng != ng0) {
+      return fail(RADG_ELIMITS, "%s: grid differs from the first table (read_klist.f:117-130)", nm.c_str());
+    }
+    ids.push_back(gi(9));
+    isos.push_back(gi(10));
+    npts.push_back(npoint);
+    vmins.push_back(vmin);
+    delvs.push_back(delv);
+    std::vector<double> d(per * npoint);
+    for (size_t i = 0; i < d.size(); i++) d[i] = gr(irec0 + i);
+    data.push_back(std::move(d));
+  }
+  std::vector<const double *> ptrs;
+  for (auto &d : data) ptrs.push_back(d.data());
+  return radg_register_ktables((int)ids.size(), ids.data(), isos.data(), npts.data(), vmins.data(),
+                               delvs.data(), np0, nt0, ng0, press.data(), temps.data(), gord.data(),
+                               delg.data(), ptrs.data());
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// Device-side data layout shared by the kernels and the host packer.
+//
+// A batch holds any number of spectrum requests.  Per-request arrays are
+// concatenated into pooled device buffers; DevReq records the offsets.
+// Work items:
+//   layer kernel     one wavefront per (request, wavenumber, layer)
+//   emission kernel  one wavefront per (request, wavenumber, path)
+//   quadrature       one wavefront per (request, wavenumber)
+#pragma once
+#include <stdint.h>
+
+#define RADG_WAVE 64
+#define RADG_MAXPARAM 31  // ngas + 1 with ngas <= 30 (no dust, no para-H2)
+
+struct DevReq {
+  int L, ngas, npath, nparam, nsw, nwave;
+  int ispace, tsurf_le0, inormal, pad0;
+  double tsurf;
+  int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
+  int isw[RADG_MAXPARAM];      // 0-based parameter ids
+  long long lay0;              // LayerConst / per-layer arrays
+  long long lg0;               // per-(layer, gas) arrays: amo
+  long long path0;             // per-path arrays
+  long long pl0;               // per-(path, layer) arrays
+  long long wave0;             // per-wavenumber arrays
+  long long wg0;               // per-(wavenumber, gas) slice records
+  long long itemA0, itemB0, itemC0;
+  long long inter0;            // intermediate buffer, elements
+  long long out0;              // output[nwave][npath]
+  long long jac0;              // jac[nwave][npath][L][nparam]
+  long long wout0;             // wout[nwave]
+  long long wjac0;             // wjac[nwave][L][nparam]
+};
+
+// Per-layer constants, computed on the host in FP64 (get_kg.f:113-174 for
+// the table bracket, cirsradg_wave.f:418-426 for utotl, nciacon.f:65-176
+// for the CIA mixing ratios and path factor).
+struct LayerConst {
+  int cp, ct;            // 1-based lower bracket indices in P and T
+  double v, u, dudt;     // P weight, T weight, dU/dT
+  double utotl;          // sum of amounts * 1e-20
+  double temp;           // K
+  double qh2, qhe, qn2, qch4, qco2, tau;
+};
+
+// Table slice used by one gas at one wavenumber.
+struct WaveGas {
+  long long base;        // element offset of the slice in the table pool; -1: k = 0
+  int coinc;             // 1: the wavenumber is on the table grid (get_kg.f:253-264)
+  int pad;
+};
+
+struct PathInfo {
+  int nlays, imod, surf, pad;
+};
+
+// Intermediate block per (request, wavenumber), in elements of the compute
+// type:  ktot[NG][L] | dk[NG][ngas+1][L] | tcon[L] | dtcon[ngas+1][L]
+// dk and dtcon hold derivatives with respect to amount*1e-20 for gas slots
+// (the reference's dkdq before its 1e-20 factor, cirsradg_wave.f:749) and
+// with respect to T for slot ngas.
+__host__ __device__ inline long long inter_block(int ng, int L, int ngas) {
+  return (long long)ng * L * (ngas + 2) + (long long)L * (ngas + 2);
+}
+
+struct KernelArgs {
+  const DevReq *reqs;
+  int nreq;
+  const long long *startA, *startB, *startC;  // [nreq+1] item prefix sums
+  long long nA, nB, nC;
+  int ng;
+  // tables
+  const void *ktab;            // double* (FP64 form) or float* (mixed form)
+  int np, nt;
+  const double *G;             // g boundaries [ng+1], G[ng] forced >= 1
+  const double *delg64;
+  const float *delg32;
+  // CIA
+  const double *kcia;          // [1501][25][9]
+  const double *cia_temps;     // [25]
+  double cia_dnu;
+  // requests
+  const LayerConst *lay;
+  const double *amo;           // [layer][gas] amount*1e-20
+  const double *press;         // [layer]
+  const PathInfo *paths;
+  const int *layinc;           // [path][L] 0-based
+  const double *scale, *emtemp;
+  const double *vwave, *esurf;
+  const WaveGas *wg;
+  const double *pweight;       // [path]
+  // outputs
+  void *inter;                 // intermediates, compute type
+  void *out, *gts, *jac;       // compute type
+  double *wout, *wgts, *wjac;
+  int *err;                    // device error flag
+};

@@ -1,0 +1,884 @@
+// gfx950 kernels for the CIRSRADG_WAVE path.
+//
+//   layer_kernel     k-table interpolation (get_kg.f:308-476), CIA continuum
+//                    (nciacon.f, ciaread.f) and random-overlap g-mixing
+//                    (noverlapg.f, rankg.f) for one (wavenumber, layer) per
+//                    wavefront.  Each overlap round sorts the NG*NG mixed
+//                    keys with a register bitonic network, takes an FP64
+//                    prefix of the weights and lets NG "bin lanes" rebin in
+//                    the reference's sequential order.
+//   emission_kernel  per (wavenumber, path): layer optical depth, FP64
+//                    transmission prefix product, Planck source and an
+//                    affine suffix scan that gives dR/dtau for every layer
+//                    in O(L) (cirsradg_wave.f:896-1035 computes the same
+//                    derivative with an O(L^2) recurrence), then the
+//                    g-quadrature (cirsradg_wave.f:1240-1286).
+//   quadrature_kernel  angle-weighted sum over paths (forwardavfovX.f:330,356).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "radg_dev.h"
+
+namespace {
+
+constexpr int W = RADG_WAVE;
+
+// --------------------------------------------------------------- helpers
+__device__ __forceinline__ long long xcd_item(long long nblocks) {
+  // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+  // range of items so that the layers of one wavenumber (which read the
+  // same table slice) share an L2.
+  const long long b = blockIdx.x;
+  const long long chunk = nblocks / 8;
+  return (b % 8) * chunk + b / 8;
+}
+
+__device__ __forceinline__ int find_req(const long long *start, int nreq, long long item) {
+  int lo = 0, hi = nreq;  // start[lo] <= item < start[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (start[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename T> __device__ __forceinline__ T dexp(T x);
+template <> __device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <typename T> __device__ __forceinline__ T dlog(T x);
+template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
+template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
+template <typename T> __device__ __forceinline__ T dexpm1(T x);
+template <> __device__ __forceinline__ float dexpm1<float>(float x) { return expm1f(x); }
+template <> __device__ __forceinline__ double dexpm1<double>(double x) { return expm1(x); }
+template <typename T> __device__ __forceinline__ T dinf();
+template <> __device__ __forceinline__ float dinf<float>() { return __builtin_huge_valf(); }
+template <> __device__ __forceinline__ double dinf<double>() { return __builtin_huge_val(); }
+
+template <typename K>
+__device__ __forceinline__ bool key_lt(K ka, int ia, K kb, int ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+// Ascending bitonic sort of 64*E (key, index) pairs held E per lane, lane l
+// owning elements l*E .. l*E+E-1.  Ties are broken by index, so the order is
+// a strict total order and every lane agrees on it.
+template <typename K, int E>
+__device__ __forceinline__ void bitonic_sort(K (&key)[E], int (&idx)[E], int lane) {
+  constexpr int LOGP = (E == 1) ? 6 : (E == 2) ? 7 : (E == 4) ? 8 : (E == 8) ? 9 : 10;
+  static_assert(W * E == (1 << LOGP), "E must be 1, 2, 4, 8 or 16");
+  // linear induction variables so that both loops unroll completely and every
+  // register-array index is a compile-time constant
+#pragma unroll
+  for (int ks = 1; ks <= LOGP; ks++) {
+    const int k = 1 << ks;
+#pragma unroll
+    for (int js = ks - 1; js >= 0; js--) {
+      const int j = 1 << js;
+      if (j >= E) {
+        const int lm = j / E;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int e = lane * E + q;
+          const K ok = __shfl_xor(key[q], lm);
+          const int oi = __shfl_xor(idx[q], lm);
+          const bool up = (e & k) == 0;
+          const bool lower = (e & j) == 0;
+          const bool other_less = key_lt(ok, oi, key[q], idx[q]);
+          const bool take = (lower == up) ? other_less : !other_less;
+          if (take) {
+            key[q] = ok;
+            idx[q] = oi;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int q2 = q ^ j;
+          if (q2 > q) {
+            const int e = lane * E + q;
+            const bool up = (e & k) == 0;
+            const bool sw = up ? key_lt(key[q2], idx[q2], key[q], idx[q])
+                               : key_lt(key[q], idx[q], key[q2], idx[q2]);
+            if (sw) {
+              K tk = key[q]; key[q] = key[q2]; key[q2] = tk;
+              int ti = idx[q]; idx[q] = idx[q2]; idx[q2] = ti;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------- CIA, FP64
+// ciaread.f:104-196 + nciacon.f:209-303 for one layer.  dab[0..6] as
+// DABSORB (already multiplied by the path factor tau).
+__device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int inormal, const LayerConst &lc,
+                          double *absorb, double *dab) {
+  double alpha[9], dal[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    alpha[i] = 0.0;
+    dal[i] = 0.0;
+  }
+  const double vhi = 1500.0 * a.cia_dnu;
+  if (!(vv > vhi || vv < 0.0)) {
+    const double *tt = a.cia_temps;
+    double temp1 = lc.temp;
+    if (temp1 > tt[24]) temp1 = tt[24];
+    if (temp1 < tt[0]) temp1 = tt[0];
+    // pindex over (i-1)*dnu and over the temperature grid
+    int cv = (int)floor(vv / a.cia_dnu) + 1;
+    if (cv > 1500) cv = 1500;
+    if (cv < 1) cv = 1;
+    // pindex keeps the last i with x1 >= x(i): correct a floor that landed
+    // one cell high through rounding
+    while (cv > 1 && vv < (cv - 1) * a.cia_dnu) cv--;
+    while (cv < 1500 && vv >= cv * a.cia_dnu) cv++;
+    int ct = 1;
+    for (int i = 1; i <= 24; i++) {
+      if (temp1 >= tt[i - 1]) ct = i;
+      else break;
+    }
+    const double v_lo = (cv - 1) * a.cia_dnu, v_hi = cv * a.cia_dnu;
+    const double t = (vv - v_lo) / (v_hi - v_lo);
+    const double u = (temp1 - tt[ct - 1]) / (tt[ct] - tt[ct - 1]);
+    const double dudt = 1.0 / (tt[ct] - tt[ct - 1]);
+    const double *k11 = a.kcia + ((long long)(cv - 1) * 25 + (ct - 1)) * 9;
+    const double *k12 = a.kcia + ((long long)(cv)*25 + (ct - 1)) * 9;
+    const double *k22 = a.kcia + ((long long)(cv)*25 + (ct)) * 9;
+    const double *k21 = a.kcia + ((long long)(cv - 1) * 25 + (ct)) * 9;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const double a11 = k11[i], a12 = k12[i], a22 = k22[i], a21 = k21[i];
+      alpha[i] = (1.0 - t) * (1.0 - u) * a11 + t * (1.0 - u) * a12 + t * u * a22 +
+                 (1.0 - t) * u * a21;
+      dal[i] = (-(1.0 - t) * a11 - t * a12 + t * a22 + (1.0 - t) * a21) * dudt;
+    }
+  }
+  const double qh2 = lc.qh2, qhe = lc.qhe, qn2 = lc.qn2, qch4 = lc.qch4, qco2 = lc.qco2;
+  double sum = 0.0;
+  for (int i = 0; i < 7; i++) dab[i] = 0.0;
+  // equilibrium (inormal = 0) or normal (3:1) ortho:para H2 pairs
+  const int p0 = (inormal == 0) ? 0 : 2;
+  sum += alpha[p0] * qh2 * qh2 + alpha[p0 + 1] * qh2 * qhe;
+  dab[0] += 2 * qh2 * alpha[p0] + qhe * alpha[p0 + 1];
+  dab[1] += qh2 * alpha[p0 + 1];
+  dab[5] += qh2 * qh2 * dal[p0] + qh2 * qhe * dal[p0 + 1];
+  sum += alpha[4] * qh2 * qn2;
+  dab[0] += qn2 * alpha[4];
+  dab[2] += qh2 * alpha[4];
+  dab[5] += qh2 * qn2 * dal[4];
+  sum += alpha[5] * qn2 * qch4;
+  dab[2] += qch4 * alpha[5];
+  dab[3] += qn2 * alpha[5];
+  dab[5] += qn2 * qch4 * dal[5];
+  sum += alpha[6] * qn2 * qn2;
+  dab[2] += 2 * qn2 * alpha[6];
+  dab[5] += qn2 * qn2 * dal[6];
+  sum += alpha[7] * qch4 * qch4;
+  dab[3] += 2 * qch4 * alpha[7];
+  dab[5] += qch4 * qch4 * dal[7];
+  sum += alpha[8] * qh2 * qch4;
+  dab[0] += qch4 * alpha[8];
+  dab[3] += qh2 * alpha[8];
+  dab[5] += qh2 * qch4 * dal[8];
+  // co2cia.f: non-zero only between 1.06 and 2.55 micron
+  {
+    const double xl = 1e4 / vv;
+    double aco2 = 0.0;
+    if (xl >= 2.15 && xl <= 2.55) aco2 = 4e-8;
+    if (xl >= 1.7 && xl <= 1.76) aco2 = 6e-9;
+    if (xl >= 1.25 && xl < 1.35) aco2 = 1.5e-9;
+    if (xl >= 1.125 && xl <= 1.225) aco2 = 0.5 * (0.31 + 0.79) * 1e-9;
+    if (xl >= 1.06 && xl < 1.125) aco2 = 0.5 * (0.29 + 0.67) * 1e-9;
+    sum += aco2 * qco2 * qco2;
+    dab[4] += 2 * qco2 * aco2;
+  }
+  *absorb = sum * lc.tau;
+  for (int i = 0; i < 7; i++) dab[i] *= lc.tau;
+}
+
+}  // namespace
+
+// ============================================================ layer kernel
+// LDS layout (dynamic): see lds_layer_bytes() on the host.
+template <typename T, int E>
+__global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nblocks) {
+  const long long item = xcd_item(nblocks);
+  if (item >= a.nA) return;
+  const int lane = threadIdx.x;
+  const int rq = find_req(a.startA, a.nreq, item);
+  const DevReq &R = a.reqs[rq];
+  const int L = R.L, ngas = R.ngas, NG = a.ng;
+  const long long loc = item - R.itemA0;
+  const int iw = (int)(loc / L);
+  const int j = (int)(loc - (long long)iw * L);
+  const LayerConst lc = a.lay[R.lay0 + j];
+  const double vwave = a.vwave[R.wave0 + iw];
+  const int n = NG * NG;
+  const int NS = ngas + 2;  // row buffer slots: K, gas 0..ngas-1, T
+
+  extern __shared__ double smem_d[];
+  double *G = smem_d;                      // [NG+1]
+  double *sgd = G + (NG + 1);              // [64E]
+  T *kgn = reinterpret_cast<T *>(sgd + W * E);  // [ngas][NG]
+  T *dkgn = kgn + ngas * NG;               // [ngas][NG]
+  T *bufA = dkgn + ngas * NG;              // [NS][NG]
+  T *bufB = bufA + NS * NG;                // [NS][NG]
+  T *skey = bufB + NS * NG;                // [64E]
+  T *dlg = skey + W * E;                   // [NG]
+  int *sidx = reinterpret_cast<int *>(dlg + NG);  // [64E]
+
+  const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
+                                   : reinterpret_cast<const T *>(a.delg32);
+  for (int i = lane; i <= NG; i += W) G[i] = a.G[i];
+  for (int i = lane; i < NG; i += W) dlg[i] = delg[i];
+
+  // ---- k-table interpolation, get_kg.f:308-476 (coincident branch) -------
+  {
+    const T *tab = reinterpret_cast<const T *>(a.ktab);
+    const int nt = a.nt;
+    const int i1 = (nt * NG * (lc.cp - 1)) + (NG * (lc.ct - 1));
+    const int i2 = (nt * NG * lc.cp) + (NG * (lc.ct - 1));
+    const int i3 = (nt * NG * lc.cp) + (NG * lc.ct);
+    const int i4 = (nt * NG * (lc.cp - 1)) + (NG * lc.ct);
+    const T v = (T)lc.v, u = (T)lc.u, dudt = (T)lc.dudt;
+    for (int k = 0; k < ngas; k++) {
+      const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
+      if (lane < NG) {
+        T kk = 0, dk = 0;
+        if (wgk.base >= 0) {
+          const T *sl = tab + wgk.base;
+          const bool klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
+          T y1 = sl[i1 + lane], y2 = sl[i2 + lane], y3 = sl[i3 + lane], y4 = sl[i4 + lane];
+          if (klog) {
+            y1 = dlog(y1);
+            y2 = dlog(y2);
+            y3 = dlog(y3);
+            y4 = dlog(y4);
+          }
+          const T x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 +
+                      ((T)1 - v) * u * y4;
+          const T dxdt = (-((T)1 - v) * y1 - v * y2 + v * y3 + ((T)1 - v) * y4) * dudt;
+          if (klog) {
+            const T ex = dexp(x);
+            kk = ex;
+            dk = ex * dxdt;
+          } else {
+            kk = x;
+            dk = dxdt;
+          }
+          if (kk != kk) kk = (T)1e-37;
+          if (dk != dk) dk = (T)1e-37;
+        }
+        kgn[k * NG + lane] = kk;
+        dkgn[k * NG + lane] = dk;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- random overlap, noverlapg.f:64-200 --------------------------------
+  // Row state (running sum): bufA[0] = K (key part), bufA[1+s] = gas slot s,
+  // bufA[1+ngas] = dK/dT.  Round 1 starts from gas 0 scaled by its amount.
+  const double *amo = a.amo + R.lg0 + (long long)j * ngas;
+  {
+    const T a1 = (T)amo[0];
+    for (int i = lane; i < NG; i += W) {
+      const T k1 = kgn[i], d1 = dkgn[i];
+      bufA[i] = k1 * a1;
+      for (int s = 0; s < ngas; s++) bufA[(1 + s) * NG + i] = (s == 0) ? k1 : (T)0;
+      bufA[(1 + ngas) * NG + i] = d1 * a1;
+    }
+  }
+  __syncthreads();
+
+  T *rows = bufA, *nxt = bufB;
+  for (int r = 1; r < ngas; r++) {
+    const T a2 = (T)amo[r];
+    const T *k2 = kgn + r * NG;
+    const T *d2 = dkgn + r * NG;
+    // zero-distribution shortcuts test the last ordinate only
+    // (noverlapg.f:89,100,153,166); round 1 tests the raw gas-0 table values
+    const bool z1 = (r == 1) ? (kgn[NG - 1] == (T)0) : (rows[NG - 1] == (T)0);
+    const bool z2 = (k2[NG - 1] == (T)0);
+    if (z1) {
+      for (int i = lane; i < NG; i += W) {
+        nxt[i] = k2[i] * a2;
+        for (int s = 0; s < r; s++) nxt[(1 + s) * NG + i] = (T)0;
+        nxt[(1 + r) * NG + i] = k2[i];
+        for (int s = r + 1; s < ngas; s++) nxt[(1 + s) * NG + i] = (T)0;
+        nxt[(1 + ngas) * NG + i] = d2[i] * a2;
+      }
+    } else if (z2) {
+      for (int i = lane; i < NG; i += W) {
+        for (int s = 0; s <= ngas; s++) nxt[s * NG + i] = rows[s * NG + i];
+        nxt[(1 + r) * NG + i] = (T)0;
+      }
+    } else {
+      // keys K_i + k2_j*a2, element e = i*NG + j (noverlapg.f:117-128,175-187)
+      T key[E];
+      int idx[E];
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int e = lane * E + q;
+        if (e < n) {
+          const int i = e / NG, jj = e - (e / NG) * NG;
+          key[q] = rows[i] + k2[jj] * a2;
+        } else {
+          key[q] = dinf<T>();
+        }
+        idx[q] = e;
+      }
+      bitonic_sort<T, E>(key, idx, lane);
+      // FP64 cumulative weights (rankg.f:104-108)
+      double incl[E];
+      double run = 0.0;
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        double w = 0.0;
+        if (idx[q] < n) {
+          const int i = idx[q] / NG, jj = idx[q] - (idx[q] / NG) * NG;
+          w = (double)(dlg[i] * dlg[jj]);
+        }
+        run += w;
+        incl[q] = run;
+      }
+      double s = run;
+#pragma unroll
+      for (int ds = 0; ds < 6; ds++) {
+        const int d = 1 << ds;
+        const double o = __shfl_up(s, d);
+        if (lane >= d) s += o;
+      }
+      double ex = __shfl_up(s, 1);
+      if (lane == 0) ex = 0.0;
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int e = lane * E + q;
+        skey[e] = key[q];
+        sidx[e] = idx[q];
+        sgd[e] = ex + incl[q];
+      }
+      __syncthreads();
+      // bin lanes: bin b gathers, in sorted order, the carry of the element
+      // straddling G[b], the elements starting in [G[b], G[b+1]) and the
+      // straddling share of the last one (rankg.f:117-152)
+      if (lane < NG) {
+        const int b = lane;
+        auto lb = [&](double g) {  // first e in [0,n] with p0(e) >= g
+          int lo = 0, hi = n;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const double p0 = (mid == 0) ? 0.0 : sgd[mid - 1];
+            if (p0 >= g) hi = mid;
+            else lo = mid + 1;
+          }
+          return lo;
+        };
+        const int sb = (b == 0) ? 0 : lb(G[b]);
+        const int se = lb(G[b + 1]);
+        const int nslot = r + 2;  // gas slots 0..r, then T
+        T kacc = 0;
+        double sum = 0.0;
+        for (int s2 = 0; s2 < nslot; s2++) nxt[(1 + (s2 == r + 1 ? ngas : s2)) * NG + b] = (T)0;
+        auto payload = [&](int e, int s2) -> T {
+          const int id = sidx[e];
+          const int i = id / NG, jj = id - (id / NG) * NG;
+          if (s2 < r) return rows[(1 + s2) * NG + i];
+          if (s2 == r) return k2[jj];
+          return rows[(1 + ngas) * NG + i] + d2[jj] * a2;
+        };
+        auto wgt = [&](int e) -> T {
+          const int id = sidx[e];
+          const int i = id / NG, jj = id - (id / NG) * NG;
+          return dlg[i] * dlg[jj];
+        };
+        bool reached = sb < se;
+        // carry-in from the element that straddled G[b]
+        if (b > 0 && sb > 0 && sb <= n) {
+          const int c = sb - 1;
+          const double p0 = (c == 0) ? 0.0 : sgd[c - 1];
+          const double p1 = sgd[c];
+          if (p1 >= G[b]) {
+            const double frac = (G[b] - p0) / (p1 - p0);
+            const T w = wgt(c);
+            const T f1 = (T)(1.0 - frac);
+            sum = (1.0 - frac) * (double)w;
+            kacc = f1 * skey[c] * w;
+            for (int s2 = 0; s2 < nslot; s2++) {
+              const int sl = 1 + (s2 == r + 1 ? ngas : s2);
+              nxt[sl * NG + b] = f1 * payload(c, s2) * w;
+            }
+            reached = true;
+          }
+        }
+        bool normalised = false;
+        for (int e = sb; e < se; e++) {
+          const double p1 = sgd[e];
+          const T w = wgt(e);
+          const T ck = skey[e];
+          if (!(p1 >= G[b + 1])) {
+            kacc = kacc + ck * w;
+            for (int s2 = 0; s2 < nslot; s2++) {
+              const int sl = 1 + (s2 == r + 1 ? ngas : s2);
+              nxt[sl * NG + b] = nxt[sl * NG + b] + payload(e, s2) * w;
+            }
+            sum = sum + (double)w;
+          } else {
+            const double p0 = (e == 0) ? 0.0 : sgd[e - 1];
+            const double frac = (G[b + 1] - p0) / (p1 - p0);
+            const T ff = (T)frac;
+            kacc = kacc + ff * ck * w;
+            for (int s2 = 0; s2 < nslot; s2++) {
+              const int sl = 1 + (s2 == r + 1 ? ngas : s2);
+              nxt[sl * NG + b] = nxt[sl * NG + b] + ff * payload(e, s2) * w;
+            }
+            sum = sum + frac * (double)w;
+            normalised = true;
+            break;
+          }
+        }
+        if (normalised || (reached && b == NG - 1)) {
+          const T sm = (T)sum;
+          kacc = kacc / sm;
+          for (int s2 = 0; s2 < nslot; s2++) {
+            const int sl = 1 + (s2 == r + 1 ? ngas : s2);
+            nxt[sl * NG + b] = nxt[sl * NG + b] / sm;
+          }
+        }
+        nxt[b] = kacc;
+        for (int s2 = r + 1; s2 < ngas; s2++) nxt[(1 + s2) * NG + b] = (T)0;
+      }
+    }
+    __syncthreads();
+    T *t = rows;
+    rows = nxt;
+    nxt = t;
+  }
+
+  // ---- write the layer's k-distribution and derivatives ------------------
+  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas);
+  T *ktot = inter;                                  // [NG][L]
+  T *dkq = inter + (long long)NG * L;               // [NG][ngas+1][L]
+  T *tcon = dkq + (long long)NG * (ngas + 1) * L;   // [L]
+  T *dtcon = tcon + L;                              // [ngas+1][L]
+  for (int i = lane; i < NG; i += W) {
+    ktot[(long long)i * L + j] = rows[i];
+    for (int s = 0; s <= ngas; s++)
+      dkq[((long long)i * (ngas + 1) + s) * L + j] = rows[(1 + s) * NG + i];
+  }
+  // ---- CIA continuum (ngascon is zero: no gas-continuum flags) -----------
+  {
+    double absorb, dab[7];
+    const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
+    cia_layer(a, vv, R.inormal, lc, &absorb, dab);
+    for (int s = lane; s <= ngas; s += W) {
+      double d = 0.0;
+      for (int l = 0; l < 5; l++)
+        if (R.iabs[l] == s) d += dab[l] / lc.utotl;
+      if (s == ngas) d += dab[5];
+      dtcon[(long long)s * L + j] = (T)d;
+    }
+    if (lane == 0) tcon[j] = (T)absorb;
+  }
+}
+
+// ========================================================= emission kernel
+namespace {
+__device__ __forceinline__ double planck_d(int ispace, double v, double t) {
+  const double c1 = 1.1911e-12, c2 = 1.439;
+  double y, a;
+  if (ispace == 0) {
+    y = v;
+    a = c1 * (y * y * y);
+  } else {
+    y = 1e4 / v;
+    a = c1 * (y * y * y * y * y) / 1e4;
+  }
+  if (y == 0.0) return 0.0;
+  return a / (exp(c2 * y / t) - 1.0);
+}
+__device__ __forceinline__ double planckg_d(int ispace, double v, double t) {
+  const double c1 = 1.1911e-12, c2 = 1.439;
+  double y, a;
+  if (ispace == 0) {
+    y = v;
+    a = c1 * c2 * (y * y * y * y) / (t * t);
+  } else {
+    y = 1e4 / v;
+    a = c1 * c2 * (y * y * y * y * y * y) * 1e-4 / (t * t);
+  }
+  const double ex = exp(c2 * y / t);
+  return ex * a / ((ex - 1.0) * (ex - 1.0));
+}
+
+template <typename T>
+struct Kahan {
+  T s, c;
+  __device__ void init() { s = 0; c = 0; }
+  __device__ void add(T x) {
+    const T y = x - c;
+    const T t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int ds = 5; ds >= 0; ds--) x += __shfl_xor(x, 1 << ds);
+  return x;
+}
+}  // namespace
+
+// LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.
+template <typename T, int RL>
+__global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
+  const long long item = xcd_item(nblocks);
+  if (item >= a.nB) return;
+  const int lane = threadIdx.x;
+  const int rq = find_req(a.startB, a.nreq, item);
+  const DevReq &R = a.reqs[rq];
+  const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath, nsw = R.nsw,
+            nparam = R.nparam;
+  const long long loc = item - R.itemB0;
+  const int iw = (int)(loc / npath);
+  const int p = (int)(loc - (long long)iw * npath);
+  const PathInfo pi = a.paths[R.path0 + p];
+  const int nlays = pi.nlays;
+  const double v = a.vwave[R.wave0 + iw];
+  const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
+                   (long long)iw * inter_block(NG, L, ngas);
+  const T *ktot = inter;
+  const T *dkq = inter + (long long)NG * L;
+  const T *tcon = dkq + (long long)NG * (ngas + 1) * L;
+  const T *dtcon = tcon + L;
+  const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
+                                   : reinterpret_cast<const T *>(a.delg32);
+
+  extern __shared__ double smem_d[];
+  T *acc = reinterpret_cast<T *>(smem_d);  // [RL*64][nsw]
+  T *cmp = acc + (long long)RL * W * nsw;
+
+  int Lm[RL];
+  bool act[RL];
+  T sc[RL], tc[RL], bb[RL], db[RL];
+  const double xfac = 1.0;
+#pragma unroll
+  for (int q = 0; q < RL; q++) {
+    const int m = lane * RL + q;
+    act[q] = m < nlays;
+    Lm[q] = act[q] ? a.layinc[R.pl0 + (long long)p * L + m] : 0;
+    sc[q] = act[q] ? (T)a.scale[R.pl0 + (long long)p * L + m] : (T)0;
+    tc[q] = act[q] ? tcon[Lm[q]] : (T)0;
+    bb[q] = 0;
+    db[q] = 0;
+    if (act[q] && pi.imod == 3) {
+      const double em = a.emtemp[R.pl0 + (long long)p * L + m];
+      bb[q] = (T)planck_d(R.ispace, v, em);
+      db[q] = (T)planckg_d(R.ispace, v, em);
+    }
+    for (int k1 = 0; k1 < nsw; k1++) {
+      acc[(lane * RL + q) * nsw + k1] = 0;
+      cmp[(lane * RL + q) * nsw + k1] = 0;
+    }
+  }
+  // ground term (cirsradg_wave.f:992-1024)
+  double radground = 0.0, gradground = 0.0;
+  if (pi.imod == 3 && pi.surf) {
+    if (R.tsurf_le0) {
+      const int ob = (nlays - 1) / RL, oq = (nlays - 1) - ob * RL;
+      double b0 = 0.0, g0 = 0.0;
+#pragma unroll
+      for (int q = 0; q < RL; q++)
+        if (q == oq) {
+          b0 = (double)bb[q];
+          g0 = (double)db[q];
+        }
+      radground = __shfl(b0, ob);
+      gradground = __shfl(g0, ob);
+    } else {
+      const double es = a.esurf[R.wave0 + iw];
+      radground = es * (double)(T)planck_d(R.ispace, v, R.tsurf);
+      gradground = es * (double)(T)planckg_d(R.ispace, v, R.tsurf);
+    }
+  }
+
+  Kahan<T> rad, gts;
+  rad.init();
+  gts.init();
+  for (int g = 0; g < NG; g++) {
+    const T dg = delg[g];
+    T tau[RL];
+#pragma unroll
+    for (int q = 0; q < RL; q++)
+      tau[q] = act[q] ? (tc[q] + ktot[(long long)g * L + Lm[q]]) * sc[q] : (T)0;
+    T coR, coG = 0;
+    T dRdtau[RL], dT[RL];
+    if (pi.imod == 3) {
+      T tl[RL], om[RL];
+      double incl[RL];
+      double run = 1.0;
+#pragma unroll
+      for (int q = 0; q < RL; q++) {
+        tl[q] = act[q] ? dexp(-tau[q]) : (T)1;
+        om[q] = act[q] ? -dexpm1(-tau[q]) : (T)0;
+        run *= (double)tl[q];
+        incl[q] = run;
+      }
+      // exclusive prefix product over lanes (FP64)
+      double s = run;
+#pragma unroll
+      for (int ds = 0; ds < 6; ds++) {
+        const int d = 1 << ds;
+        const double o = __shfl_up(s, d);
+        if (lane >= d) s *= o;
+      }
+      double ex = __shfl_up(s, 1);
+      if (lane == 0) ex = 1.0;
+      const double trN = __shfl(s, W - 1);
+      // radiance, sum_j (tr_{j-1} - tr_j) B_j (cirsradg_wave.f:946-951)
+      double part = 0.0;
+      double trp[RL], trx[RL];
+#pragma unroll
+      for (int q = 0; q < RL; q++) {
+        trp[q] = (q == 0) ? ex : ex * incl[q - 1];
+        trx[q] = ex * incl[q];
+        if (act[q]) part += trp[q] * (double)om[q] * (double)bb[q];
+      }
+      double rg = xfac * wave_sum(part);
+      if (pi.surf) {
+        rg += xfac * trN * radground;
+        coG = (T)(xfac * trN * gradground);
+      }
+      coR = (T)rg;
+      // W_m: radiance entering layer m from below.  Suffix scan of the maps
+      // x -> B_m (1 - t_m) + t_m x over the deeper layers.
+      T ca = 0, cb = 1;  // lane composite, applied to the value from below
+#pragma unroll
+      for (int q = RL - 1; q >= 0; q--) {
+        if (act[q]) {
+          const T am = bb[q] * om[q], bm = tl[q];
+          ca = am + bm * ca;
+          cb = bm * cb;
+        }
+      }
+      T ia = ca, ib = cb;  // inclusive suffix composite
+#pragma unroll
+      for (int ds = 0; ds < 6; ds++) {
+        const int d = 1 << ds;
+        const T oa = __shfl_down(ia, d), ob = __shfl_down(ib, d);
+        if (lane + d < W) {
+          ia = ia + ib * oa;
+          ib = ib * ob;
+        }
+      }
+      T xa = __shfl_down(ia, 1), xb = __shfl_down(ib, 1);
+      if (lane == W - 1) {
+        xa = 0;
+        xb = 1;
+      }
+      T wv = xa + xb * (T)radground;  // W below this lane's last layer
+      T Wm[RL];
+#pragma unroll
+      for (int q = RL - 1; q >= 0; q--) {
+        Wm[q] = wv;
+        if (act[q]) wv = bb[q] * om[q] + tl[q] * wv;
+      }
+#pragma unroll
+      for (int q = 0; q < RL; q++) {
+        dRdtau[q] = act[q] ? (T)(xfac * trx[q] * (double)(bb[q] - Wm[q])) : (T)0;
+        dT[q] = act[q] ? (T)(xfac * trp[q] * (double)om[q]) * db[q] : (T)0;
+      }
+    } else {
+      double part = 0.0;
+#pragma unroll
+      for (int q = 0; q < RL; q++) part += (double)tau[q];
+      const T tmp = (T)wave_sum(part);
+      if (pi.imod == 0) {
+        coR = dexp(-tmp);
+#pragma unroll
+        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? -tmp * coR : (T)0;
+      } else {
+        coR = (T)1 - dexp(-tmp);
+#pragma unroll
+        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? tmp * ((T)1 - coR) : (T)0;
+      }
+#pragma unroll
+      for (int q = 0; q < RL; q++) dT[q] = 0;
+    }
+    rad.add(coR * dg);
+    gts.add(coG * dg);
+    // Jacobian: dR/dq = dR/dtau * scale * (dtaucon/dq + dk/dq) (+ Planck term)
+#pragma unroll
+    for (int q = 0; q < RL; q++) {
+      if (!act[q]) continue;
+      const int m = lane * RL + q;
+      for (int k1 = 0; k1 < nsw; k1++) {
+        const int k = R.isw[k1];
+        const T dq = (dtcon[(long long)k * L + Lm[q]] +
+                      dkq[((long long)g * (ngas + 1) + k) * L + Lm[q]]) * sc[q];
+        T c = dRdtau[q] * dq;
+        if (k == ngas) c = c + dT[q];
+        const T x = c * dg;
+        T &s0 = acc[m * nsw + k1];
+        T &c0 = cmp[m * nsw + k1];
+        const T y = x - c0;
+        const T t = s0 + y;
+        c0 = (t - s0) - y;
+        s0 = t;
+      }
+    }
+  }
+  // ---- outputs ----
+  const long long ob = R.out0 + (long long)iw * npath + p;
+  if (lane == 0) {
+    reinterpret_cast<T *>(a.out)[ob] = rad.s;
+    reinterpret_cast<T *>(a.gts)[ob] = gts.s;
+  }
+  T *jac = reinterpret_cast<T *>(a.jac) + R.jac0 + ((long long)iw * npath + p) * L * nparam;
+  // each lane writes whole rows: zeros, then its isw parameters
+  // (cirsradg_wave.f:1243-1247 zeroes every row before the g-sum)
+#pragma unroll
+  for (int q = 0; q < RL; q++) {
+    const int m = lane * RL + q;
+    if (m >= L) continue;
+    for (int k = 0; k < nparam; k++) jac[(long long)m * nparam + k] = (T)0;
+    if (!act[q]) continue;
+    for (int k1 = 0; k1 < nsw; k1++) {
+      const int k = R.isw[k1];
+      T val = acc[m * nsw + k1];
+      if (k < ngas) val = (T)((double)val * 1e-20);
+      jac[(long long)m * nparam + k] = val;
+    }
+  }
+}
+
+// ======================================================= path quadrature
+template <typename T>
+__global__ __launch_bounds__(64) void quadrature_kernel(KernelArgs a) {
+  const long long item = blockIdx.x;
+  if (item >= a.nC) return;
+  const int lane = threadIdx.x;
+  const int rq = find_req(a.startC, a.nreq, item);
+  const DevReq &R = a.reqs[rq];
+  const int iw = (int)(item - R.itemC0);
+  const int L = R.L, npath = R.npath, nparam = R.nparam;
+  const double *pw = a.pweight + R.path0;
+  const T *out = reinterpret_cast<const T *>(a.out) + R.out0 + (long long)iw * npath;
+  const T *gt = reinterpret_cast<const T *>(a.gts) + R.out0 + (long long)iw * npath;
+  if (lane == 0) {
+    double so = 0.0, sg = 0.0;
+    for (int p = 0; p < npath; p++) {
+      so += pw[p] * (double)out[p];
+      sg += pw[p] * (double)gt[p];
+    }
+    a.wout[R.wout0 + iw] = so;
+    a.wgts[R.wout0 + iw] = sg;
+  }
+  const T *jac = reinterpret_cast<const T *>(a.jac) + R.jac0 + (long long)iw * npath * L * nparam;
+  double *wj = a.wjac + R.wjac0 + (long long)iw * L * nparam;
+  const long long per = (long long)L * nparam;
+  for (long long x = lane; x < per; x += W) {
+    double s = 0.0;
+    for (int p = 0; p < npath; p++) s += pw[p] * (double)jac[p * per + x];
+    wj[x] = s;
+  }
+}
+
+// ============================================================ launchers
+namespace {
+template <typename T, int E>
+hipError_t launch_layer_E(const KernelArgs &a, size_t lds, hipStream_t st) {
+  long long nb = ((a.nA + 7) / 8) * 8;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL((layer_kernel<T, E>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
+  return hipGetLastError();
+}
+template <typename T, int RL>
+hipError_t launch_em_RL(const KernelArgs &a, size_t lds, hipStream_t st) {
+  long long nb = ((a.nB + 7) / 8) * 8;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL((emission_kernel<T, RL>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
+  return hipGetLastError();
+}
+}  // namespace
+
+size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
+  const size_t ts = fp64 ? 8 : 4;
+  size_t b = (size_t)(ng + 1) * 8 + (size_t)W * E * 8;
+  b += ts * ((size_t)2 * ngas * ng + (size_t)2 * (ngas + 2) * ng + (size_t)W * E + ng);
+  b += (size_t)W * E * 4;
+  return b;
+}
+
+size_t radg_lds_emission_bytes(int RL, int nsw, int fp64) {
+  const size_t ts = fp64 ? 8 : 4;
+  return (size_t)2 * RL * W * nsw * ts + 16;
+}
+
+int radg_layer_E(int ng) {
+  const int n = ng * ng;
+  int E = 1;
+  while (W * E < n) E <<= 1;
+  return E;
+}
+
+int radg_emission_RL(int maxlays) {
+  int RL = 1;
+  while (W * RL < maxlays) RL <<= 1;
+  return RL;
+}
+
+hipError_t radg_launch_layer_lds(const KernelArgs &a, int fp64, int E, size_t lds, hipStream_t st) {
+  if (fp64) {
+    switch (E) {
+      case 1: return launch_layer_E<double, 1>(a, lds, st);
+      case 2: return launch_layer_E<double, 2>(a, lds, st);
+      case 4: return launch_layer_E<double, 4>(a, lds, st);
+      case 8: return launch_layer_E<double, 8>(a, lds, st);
+      case 16: return launch_layer_E<double, 16>(a, lds, st);
+    }
+  } else {
+    switch (E) {
+      case 1: return launch_layer_E<float, 1>(a, lds, st);
+      case 2: return launch_layer_E<float, 2>(a, lds, st);
+      case 4: return launch_layer_E<float, 4>(a, lds, st);
+      case 8: return launch_layer_E<float, 8>(a, lds, st);
+      case 16: return launch_layer_E<float, 16>(a, lds, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, size_t lds, hipStream_t st) {
+  if (fp64) {
+    switch (RL) {
+      case 1: return launch_em_RL<double, 1>(a, lds, st);
+      case 2: return launch_em_RL<double, 2>(a, lds, st);
+      case 4: return launch_em_RL<double, 4>(a, lds, st);
+      case 8: return launch_em_RL<double, 8>(a, lds, st);
+    }
+  } else {
+    switch (RL) {
+      case 1: return launch_em_RL<float, 1>(a, lds, st);
+      case 2: return launch_em_RL<float, 2>(a, lds, st);
+      case 4: return launch_em_RL<float, 4>(a, lds, st);
+      case 8: return launch_em_RL<float, 8>(a, lds, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st) {
+  if (a.nC == 0) return hipSuccess;
+  if (fp64)
+    hipLaunchKernelGGL((quadrature_kernel<double>), dim3((unsigned)a.nC), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((quadrature_kernel<float>), dim3((unsigned)a.nC), dim3(64), 0, st, a);
+  return hipGetLastError();
+}

@@ -1,0 +1,70 @@
+"""Save and load golden cases: inputs plus the reference's outputs.
+
+A case file holds a request (the ``cirsradg_wave`` argument arrays), its
+k-tables and the outputs of the reference's own ``cirsradg_wave`` built with
+REAL promoted to 8 bytes (``ref64_*``) and as written (``ref32_*``).  The CIA
+table is shared by every case and stored once in ``cia.npz``.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from radtrancode_amd.synth import CIATable, KTable, Request
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_REQ_ARRAYS = ["press", "temp", "pp", "amount", "delh", "idgas", "isogas", "vwave",
+               "nlayin", "layinc", "scale", "emtemp", "imod", "isw", "path_weight",
+               "hfp", "vem", "emissivity"]
+_REQ_SCALARS = ["dist", "inormal", "iray", "ispace", "amform", "itype", "tsurf",
+                "radius1", "flagh2p", "radextra"]
+_KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
+_KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
+
+
+def save_case(path, tables, req, outputs: dict, note: str = ""):
+    d = {"note": np.array(note)}
+    for a in _REQ_ARRAYS:
+        d["req_" + a] = np.asarray(getattr(req, a))
+    for s in _REQ_SCALARS:
+        d["req_" + s] = np.asarray(getattr(req, s))
+    d["ntab"] = np.array(len(tables))
+    for t, kt in enumerate(tables):
+        for a in _KT_ARRAYS + _KT_SCALARS:
+            d[f"kt{t}_{a}"] = np.asarray(getattr(kt, a))
+    for k, v in outputs.items():
+        d[k] = np.asarray(v)
+    np.savez_compressed(path, **d)
+
+
+def load_case(path):
+    z = np.load(path, allow_pickle=False)
+    kw = {a: z["req_" + a] for a in _REQ_ARRAYS}
+    for s in _REQ_SCALARS:
+        v = z["req_" + s]
+        kw[s] = v.item()
+    req = Request(**kw)
+    tables = []
+    for t in range(int(z["ntab"])):
+        tk = {a: z[f"kt{t}_{a}"] for a in _KT_ARRAYS}
+        for s in _KT_SCALARS:
+            tk[s] = z[f"kt{t}_{s}"].item()
+        tables.append(KTable(**tk))
+    outs = {k: z[k] for k in z.files if k.startswith("ref")}
+    return tables, req, outs, str(z["note"])
+
+
+def save_cia(path, cia: CIATable):
+    np.savez_compressed(path, dnu=np.array(cia.dnu), temps=cia.temps, kcia=cia.kcia)
+
+
+def load_cia(path=None) -> CIATable:
+    z = np.load(path or os.path.join(HERE, "cia.npz"), allow_pickle=False)
+    return CIATable(dnu=float(z["dnu"]), temps=z["temps"], kcia=z["kcia"])
+
+
+def case_paths():
+    return sorted(os.path.join(HERE, f) for f in os.listdir(HERE)
+                  if f.startswith("case_") and f.endswith(".npz"))

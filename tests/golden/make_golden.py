@@ -1,0 +1,122 @@
+"""Generate the golden cases from the reference's own CIRSRADG_WAVE.
+
+Run in a checkout where ``/root/reference`` is present::
+
+    make -C oracle ref
+    python tests/golden/make_golden.py
+
+Each case stores its inputs (request arrays and k-tables) and the outputs of
+the reference compiled with REAL promoted to 8 bytes (``ref64_*``: the FP64
+oracle) and as written (``ref32_*``).  The inputs come from
+``radtrancode_amd.synth`` with fixed seeds; edits for the edge cases are made
+here and saved with the case, so the fixtures do not depend on the
+generator staying unchanged.
+"""
+from __future__ import annotations
+
+import copy
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from radtrancode_amd import synth  # noqa: E402
+from oracle import ref  # noqa: E402
+from tests.golden.golden_io import save_case, save_cia  # noqa: E402
+
+
+def slice_tables(tables, i0, n):
+    out = []
+    for kt in tables:
+        k2 = copy.copy(kt)
+        k2.k = kt.k[i0:i0 + n].copy()
+        k2.vmin = kt.vmin + i0 * kt.delv
+        out.append(k2)
+    return out
+
+
+def run_ref(tables, cia, req):
+    outs = {}
+    for prec in (64, 32):
+        rr = ref.RefRunner(tables, cia, req, prec=prec)
+        o, g, d = rr.spectrum()
+        rr.close()
+        outs[f"ref{prec}_output"] = o
+        outs[f"ref{prec}_gtsurf"] = g
+        outs[f"ref{prec}_doutputdq"] = d
+    return outs
+
+
+def cases():
+    # tiny: 4 layers, 4 g, 2 k-gases + H2/He, 3 wavenumbers
+    t, c, r = synth.config1(nwave=3, nlayer=4, nkgas=2, ng=4)
+    yield "tiny", t, r, "L=4 ng=4 2 k-gases + H2/He, 3 wavenumbers"
+
+    t, c, r = synth.config1(nwave=16, nlayer=20, nkgas=3, ng=8)
+    yield "medium", t, r, "L=20 ng=8 3 k-gases + H2/He, 16 wavenumbers"
+
+    t, c, r = synth.config1(nwave=1000, nlayer=120, nkgas=3, ng=20)
+    i0, n = 496, 8
+    ts = slice_tables(t, i0, n)
+    r.vwave = r.vwave[i0:i0 + n].copy()
+    yield "cfg1_slice", ts, r, "120 layers x 20 g x 3+2 gases, 5 paths, wavenumbers 496..503 of 1000"
+
+    t, c, r = synth.config1(nwave=12, nlayer=30, nkgas=2, ng=10)
+    r.tsurf = 150.0
+    r.vem = np.array([500.0, 2000.0])
+    r.emissivity = np.array([0.9, 0.95])
+    r.amount = r.amount * 1e-4
+    r.pp = r.pp * 1e-4
+    r.imod = np.array([0, 1, 3, 3, 0], dtype=np.int32)
+    yield "surface_imod", t, r, "tsurf>0 with emissivity, optically thin, imod 0/1/3 paths"
+
+    t, c, r = synth.config1(nwave=12, nlayer=30, nkgas=3, ng=10)
+    r.vwave = r.vwave[:-1] + 0.37 * t[0].delv
+    yield "noncoincident", t, r, "wavenumbers between table points (rankk branch)"
+
+    t, c, r = synth.config1(nwave=6, nlayer=25, nkgas=6, ng=12)
+    yield "six_gases", t, r, "6 k-gases + H2/He"
+
+    t, c, r = synth.config1(nwave=6, nlayer=25, nkgas=3, ng=8)
+    r.temp[:5] = 300.0
+    r.temp[-3:] = 20.0
+    r.emtemp = r.temp[r.layinc - 1]
+    r.press[:2] = 40.0
+    yield "clamps", t, r, "layer T and P outside the table grid"
+
+    t, c, r = synth.config1(nwave=6, nlayer=25, nkgas=3, ng=8)
+    t[1].k[2] = 0.0
+    t[0].k[3, :, :, 0] = 0.0
+    t[2].k[4, :, :, :3] = 0.0
+    yield "zeros", t, r, "all-zero slice, zero g=1 corners (linear interpolation), zero low-g"
+
+    t, c, r = synth.config1(nwave=4, nlayer=200, nkgas=4, ng=16)
+    yield "deep200", t, r, "200 layers, ng=16, 4 k-gases"
+
+    t, c, r = synth.config1(nwave=5, nlayer=40, nkgas=1, ng=20)
+    yield "one_kgas", t, r, "one k-gas + H2/He"
+
+    t, c, r = synth.config1(nwave=5, nlayer=36, nkgas=3, ng=20)
+    # a path that stops half way down (no surface term) and one at a
+    # different scale with every layer counted twice
+    r.nlayin = np.array([36, 18, 36, 36, 36], dtype=np.int32)
+    r.scale[3] = 1.7
+    yield "partial_paths", t, r, "a half-depth path (surface test fails) and an odd scale"
+
+
+def main():
+    cia = synth.make_cia()
+    save_cia(os.path.join(HERE, "cia.npz"), cia)
+    for name, t, r, note in cases():
+        outs = run_ref(t, cia, r)
+        path = os.path.join(HERE, f"case_{name}.npz")
+        save_case(path, t, r, outs, note)
+        print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
+              f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+"""The oracle (FP64 C restatement) pinned against the reference and maths.
+
+* every golden case: the restatement equals the reference's own
+  ``cirsradg_wave`` built with REAL promoted to 8 bytes (ref64 fixtures);
+* closed forms and invariants that catch plausible mistakes: Planck and its
+  T-derivative, bisection, the heapsort, the overlap's mean-k conservation,
+  monotonicity and shortcuts, a 2x2 hand case, the O(L) and O(L^2)
+  Jacobian recurrences, and central finite differences of the whole model.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from radtrancode_amd import synth
+from tests.golden.golden_io import case_paths, load_case, load_cia
+from tests.tolerance import jac_err, rad_err
+
+CASES = case_paths()
+
+
+@pytest.mark.parametrize("path", CASES, ids=[p.split("case_")[-1][:-4] for p in CASES])
+def test_oracle_matches_reference_fp64(path):
+    tables, req, outs, note = load_case(path)
+    cia = load_cia()
+    o, g, d = oracle.spectrum(tables, cia, req)
+    assert rad_err(o, outs["ref64_output"]) < 1e-12, note
+    assert rad_err(g, outs["ref64_gtsurf"]) < 1e-12, note
+    assert jac_err(d, outs["ref64_doutputdq"]) < 1e-12, note
+
+
+def test_reference_mixed_precision_build_is_close():
+    """The reference as written (REAL = 4 bytes) sits ~1e-6 from FP64."""
+    for path in CASES:
+        _, _, outs, note = load_case(path)
+        assert rad_err(outs["ref32_output"], outs["ref64_output"]) < 2e-5, note
+
+
+def test_planck_closed_form():
+    c1, c2 = 1.1911e-12, 1.439
+    for v, t in [(600.0, 110.0), (1400.0, 50.0), (900.0, 300.0)]:
+        want = c1 * v ** 3 / np.expm1(c2 * v / t)
+        assert abs(oracle.planck(0, v, t) / want - 1) < 1e-13
+        # wavelength form: y = 1e4/v micron
+        y = 1e4 / v
+        want_l = c1 * y ** 5 / 1e4 / np.expm1(c2 * y / t)
+        assert abs(oracle.planck(1, v, t) / want_l - 1) < 1e-13
+
+
+def test_planckg_is_the_temperature_derivative():
+    for v, t in [(600.0, 110.0), (1400.0, 60.0), (1000.0, 170.0)]:
+        # Richardson-extrapolated central difference (fourth order)
+        d = lambda h: (oracle.planck(0, v, t + h) - oracle.planck(0, v, t - h)) / (2 * h)
+        h = 1e-3 * t
+        fd = (4 * d(h / 2) - d(h)) / 3
+        assert abs(oracle.planckg(0, v, t) / fd - 1) < 1e-8
+
+
+def test_locate_is_bisection():
+    from oracle.oracle import lib
+    import ctypes as C
+    xx = np.sort(np.random.default_rng(0).uniform(0, 10, 17))
+    for x in [-1.0, 0.0, xx[0], xx[5], 5.0, xx[-1], 11.0]:
+        j = lib().orc_locate(xx.ctypes.data_as(C.POINTER(C.c_double)), len(xx), x)
+        # NR locate: xx(j) < x <= xx(j+1), j in 0..n (1-based)
+        assert j == int(np.searchsorted(xx, x, side="left"))
+
+
+def test_heapsort_sorts_and_permutes():
+    rng = np.random.default_rng(4)
+    ra = rng.normal(size=400)
+    s, ib = oracle.sort2g(ra)
+    assert np.all(np.diff(s) >= 0)
+    assert np.array_equal(np.sort(ib), np.arange(1, 401))
+    assert np.array_equal(ra[ib - 1], s)
+
+
+def _gl(ng):
+    return synth.gauss_legendre_g(ng)[1]
+
+
+def test_overlap_conserves_mean_k_and_is_monotone():
+    rng = np.random.default_rng(7)
+    ng = 20
+    delg = _gl(ng)
+    for trial in range(5):
+        ngas = 2 + trial % 3
+        k = np.sort(10 ** rng.normal(-2, 2, size=(ngas, ng)), axis=1)
+        dk = rng.normal(size=(ngas, ng)) * k
+        amt = 10 ** rng.uniform(1, 6, size=ngas)
+        kg, dkdq = oracle.noverlapg(delg, amt, k, dk)
+        # mean of the mixed distribution = sum of the scaled means
+        assert abs(np.dot(delg, kg) / np.dot(amt, k @ delg) - 1) < 1e-12
+        assert np.all(np.diff(kg) >= -1e-12 * kg.max())
+        # d(mean k)/d(amount_i) = mean k_i, d(mean k)/dT = sum a_i mean dk_i
+        for i in range(ngas):
+            assert abs(np.dot(delg, dkdq[i]) / np.dot(delg, k[i]) - 1) < 1e-12
+        assert abs(np.dot(delg, dkdq[ngas]) / np.dot(amt, dk @ delg) - 1) < 1e-10
+
+
+def test_overlap_two_by_two_by_hand():
+    """ng=2, equal weights: keys a+c, a+d, b+c, b+d with a<b, c<d."""
+    delg = np.array([0.5, 0.5])
+    k = np.array([[1.0, 4.0], [2.0, 3.0]])  # gas 1: 1,4; gas 2: 2,3 (amounts 1)
+    kg, dkdq = oracle.noverlapg(delg, np.array([1.0, 1.0]), k, np.zeros_like(k))
+    # keys 3 (1+2), 4 (1+3), 6 (4+2), 7 (4+3), weights 1/4 each -> bins {3,4}, {6,7}
+    assert np.allclose(kg, [3.5, 6.5], rtol=0, atol=1e-15)
+    # d/da1: payload k1 of each key: bin 1 -> (1+1)/2, bin 2 -> (4+4)/2
+    assert np.allclose(dkdq[0], [1.0, 4.0])
+    assert np.allclose(dkdq[1], [2.5, 2.5])
+
+
+def test_overlap_shortcuts_pass_one_gas_through():
+    delg = _gl(10)
+    k1 = np.sort(10 ** np.random.default_rng(1).normal(-2, 1, 10))
+    for zero in (0, 1):
+        k = np.stack([k1, k1 * 0]) if zero == 1 else np.stack([k1 * 0, k1])
+        dk = k * 0.1
+        kg, dkdq = oracle.noverlapg(delg, np.array([3.0, 3.0]), k, dk)
+        assert np.allclose(kg, 3.0 * k1, rtol=1e-15)
+        assert np.allclose(dkdq[1 - zero], k1)
+        assert np.all(dkdq[zero] == 0)
+        assert np.allclose(dkdq[2], 0.3 * k1)
+
+
+def test_linear_and_quadratic_jacobian_recurrences_agree():
+    tables, cia, req = synth.config1(nwave=6, nlayer=40, nkgas=3, ng=10)
+    o0, g0, d0 = oracle.spectrum(tables, cia, req, jac_mode=0)
+    o1, g1, d1 = oracle.spectrum(tables, cia, req, jac_mode=1)
+    assert rad_err(o1, o0) < 1e-13
+    assert jac_err(d1, d0) < 1e-11
+
+
+def test_jacobian_matches_finite_differences():
+    """Central differences of the whole model in amount and in T.
+
+    Thin, warm setup so every layer is visible; the layers stay inside one
+    table cell for the +-h steps.  The CIA gradient in the reference ignores
+    the dependence of its path factor on the gas amounts, so only k-table
+    gases are differenced for amounts.
+    """
+    tables, cia, req = synth.config1(nwave=3, nlayer=12, nkgas=3, ng=8)
+    req.amount = req.amount * 1e-3
+    req.pp = req.pp * 1e-3
+    o, g, d = oracle.spectrum(tables, cia, req)
+    ip = 2  # a path
+    for j in [0, 4, 9]:
+        lay = req.layinc[ip, j] - 1
+        for k in range(3):
+            h = 1e-4 * req.amount[lay, k]
+            rp, rm = _copy(req), _copy(req)
+            rp.amount[lay, k] += h
+            rm.amount[lay, k] -= h
+            fd = (oracle.spectrum(tables, cia, rp)[0] - oracle.spectrum(tables, cia, rm)[0]) / (2 * h)
+            scale = np.max(np.abs(d[:, ip, :, k]))
+            assert np.max(np.abs(fd[:, ip] - d[:, ip, j, k])) < 2e-4 * scale
+        # temperature: layer temperature and the path's emission temperature move together
+        h = 1e-3
+        rp, rm = _copy(req), _copy(req)
+        for rr, s in ((rp, h), (rm, -h)):
+            rr.temp[lay] += s
+            rr.emtemp = rr.temp[rr.layinc - 1]
+        fd = (oracle.spectrum(tables, cia, rp)[0] - oracle.spectrum(tables, cia, rm)[0]) / (2 * h)
+        ngas = req.ngas
+        scale = np.max(np.abs(d[:, ip, :, ngas]))
+        # every path sees the layer's emission temperature change
+        assert np.max(np.abs(fd[:, ip] - d[:, ip, j, ngas])) < 2e-3 * scale
+
+
+def _copy(req):
+    import copy
+    r = copy.copy(req)
+    r.amount = req.amount.copy()
+    r.temp = req.temp.copy()
+    r.pp = req.pp.copy()
+    r.emtemp = req.emtemp.copy()
+    return r
+
+
+def test_isothermal_single_layer_closed_form():
+    """One layer, T everywhere equal: R = B (1 - e^-tau) + e^-tau B_ground = B."""
+    tables, cia, req = synth.config1(nwave=3, nlayer=2, nkgas=2, ng=6)
+    req.temp[:] = 150.0
+    req.emtemp[:] = 150.0
+    o, g, d = oracle.spectrum(tables, cia, req)
+    for iw, v in enumerate(req.vwave):
+        b = oracle.planck(0, v, 150.0)
+        assert np.allclose(o[iw], b, rtol=1e-12)
+    # d R / d amount vanishes for an isothermal column with a same-T ground
+    assert np.max(np.abs(d[..., :req.ngas])) < 1e-12 * np.max(np.abs(o)) / req.amount.max()

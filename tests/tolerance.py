@@ -1,0 +1,43 @@
+"""Error measures shared by the parity tests.
+
+* radiance (and gtsurf): relative error per value, ``|a - b| / |b|``, where
+  ``b`` is the oracle; values the oracle gives as exactly 0 must come out
+  within ``1e-30`` absolute.
+* Jacobians: for each (wavenumber, path, parameter) column the error is
+  ``max_layer |a - b| / max_layer |b|``.  Per-element relative error is
+  meaningless here: single entries can cancel to near zero (an isothermal
+  stretch has dR/dtau = 0 exactly), so each column is scaled by its own
+  largest entry.
+"""
+import numpy as np
+
+# FP64 form against the FP64 oracle
+TOL_RAD_FP64 = 1e-5
+TOL_JAC_FP64 = 1e-4
+# mixed-precision form against the FP64 oracle
+TOL_RAD_MIXED = 1e-4
+TOL_JAC_MIXED = 1e-3
+
+
+def rad_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nz = b != 0
+    err = 0.0
+    if nz.any():
+        err = float(np.max(np.abs(a[nz] - b[nz]) / np.abs(b[nz])))
+    if (~nz).any():
+        za = float(np.max(np.abs(a[~nz])))
+        if za > 1e-30:
+            err = max(err, np.inf)
+    return err
+
+
+def jac_err(a, b, layer_axis=-2):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.max(np.abs(b), axis=layer_axis, keepdims=True)
+    diff = np.max(np.abs(a - b), axis=layer_axis, keepdims=True)
+    ok0 = scale == 0
+    err = np.where(ok0, np.where(diff > 1e-300, np.inf, 0.0), diff / np.where(ok0, 1.0, scale))
+    return float(np.max(err))

@@ -592,8 +592,10 @@ static const orc_ktable *find_table(const orc_ktable *tabs, int ntab, int id,
   for (t = 0; t < ntab; t++) {
     const orc_ktable *kt = &tabs[t];
     double xmax = kt->vmin + (kt->npoint - 1) * kt->delv;
+    double tol = (kt->delv >= 0.0001) ? 0.02 * kt->delv : 0.25 * kt->delv;
     if (kt->idgas != id || kt->isogas != iso || kt->delv <= 0.0) continue;
-    if (vwave >= kt->vmin - 1e-5 && vwave <= xmax + 1e-5) return kt;
+    /* grid ends widened by get_kg's coincidence tolerance (get_kg.f:199-206) */
+    if (vwave >= kt->vmin - tol && vwave <= xmax + tol) return kt;
   }
   return NULL;
 }

@@ -39,7 +39,30 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_HOST = os.path.join(LIBDIR, "radg_host")
+
+
+def build_fortran(verbose: bool = False) -> None:
+    """The ISO_C_BINDING module and a Fortran host linked to libradg.so."""
+    if not os.path.exists(FLANG):
+        return
+    fdir = os.path.join(HERE, "fortran")
+    cmds = [
+        [FLANG, "-O2", "-fPIC", "-J", LIBDIR, "-c", os.path.join(fdir, "radg_mod.f90"),
+         "-o", os.path.join(LIBDIR, "radg_mod.o")],
+        [FLANG, "-O2", "-J", LIBDIR, os.path.join(fdir, "radg_host.f90"),
+         os.path.join(LIBDIR, "radg_mod.o"), "-L", LIBDIR, "-lradg",
+         "-Wl,-rpath,$ORIGIN", "-o", FORTRAN_HOST],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

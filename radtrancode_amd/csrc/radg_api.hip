@@ -294,8 +294,11 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
       w.coinc = 1;
       const Table *tb = nullptr;
       for (const Table &t : ts.tabs) {  // read_klist.f:260-290: first covering match
+        // the grid ends are widened by get_kg's coincidence tolerance so that
+        // a REAL(4) wavenumber rounded past the last point still matches
         const double xmax = t.vmin + (t.npoint - 1) * t.delv;
-        if (t.idgas == q.idgas[k] && t.isogas == q.isogas[k] && v >= t.vmin - 1e-5 && v <= xmax + 1e-5) {
+        const double tol = (t.delv >= 0.0001) ? 0.02 * t.delv : 0.25 * t.delv;
+        if (t.idgas == q.idgas[k] && t.isogas == q.isogas[k] && v >= t.vmin - tol && v <= xmax + tol) {
           tb = &t;
           break;
         }

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nbloc
       }
     } else if (z2) {
       for (int i = lane; i < NG; i += W) {
-        for (int s = 0; s <= ngas; s++) nxt[s * NG + i] = rows[s * NG + i];
+        for (int s = 0; s < NS; s++) nxt[s * NG + i] = rows[s * NG + i];
         nxt[(1 + r) * NG + i] = (T)0;
       }
     } else {

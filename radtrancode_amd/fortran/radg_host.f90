@@ -1,0 +1,78 @@
+! A Fortran host for the drop-in: it calls cirsradg_wave once per
+! wavenumber with the reference's argument list and fixed leading dimensions,
+! exactly as radtran/cirsradg/cirsrtfg_wave.f:397-401 does, then the
+! whole-spectrum entry radg_spectrum_r4 once.
+!
+! Usage: radg_host <input.bin> <output.bin>
+! The input file (stream access) holds the k-list path, the CIA table and
+! the argument arrays; the output file holds output/gtsurf/doutputdq per
+! wavenumber from both calls.
+program radg_host
+  use radg
+  use iso_c_binding
+  implicit none
+  integer, parameter :: maxlay = 400, maxpat = 200, maxgas = 30, maxcon = 10
+  integer, parameter :: maxsec = 16000, nparmax = maxgas + 2 + maxcon
+  character(len=512) :: fin, fout, klist
+  integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
+  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode
+  real(c_double) :: dnu, ctemps(25)
+  real(c_double), allocatable :: kcia(:, :, :)
+  real(c_float) :: dist, tsurf, radius1, radextra, vv
+  real(c_float), allocatable :: delh(:), vwave(:)
+  real(c_float) :: press(maxlay), temp(maxlay), hfp(maxlay)
+  real(c_float) :: pp(maxlay, maxgas), amount(maxlay, maxgas), cont(maxcon, maxlay)
+  real(c_float) :: scale(maxlay, maxpat), emtemp(maxlay, maxpat)
+  real(c_float) :: vem(maxsec), emissivity(maxsec)
+  real(c_float) :: output(maxpat), gtsurf(maxpat)
+  real(c_float), allocatable :: doutputdq(:, :, :), sout(:), sgts(:), sdq(:)
+  integer(c_int) :: nlayin(maxpat), layinc(maxlay, maxpat), imod(maxpat)
+  integer(c_int) :: idgas(maxgas), isogas(maxgas), isw(nparmax)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open (10, file=trim(fin), access='stream', form='unformatted', status='old')
+  read (10) klen
+  klist = ' '
+  read (10) klist(1:klen)
+  read (10) recl, mode
+  allocate (kcia(9, 25, 1501))
+  read (10) dnu, ctemps, kcia
+  read (10) nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace, itype, flagh2p, amform
+  read (10) dist, tsurf, radius1, radextra
+  allocate (delh(nlayer), vwave(nwave))
+  read (10) delh, press, temp, pp, amount, nlayin, layinc, cont, scale, imod, idgas, isogas
+  read (10) emtemp, vem, emissivity, hfp, isw, vwave
+  close (10)
+
+  if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
+  if (radg_register_cia(dnu, ctemps, kcia) /= RADG_OK) call die()
+  if (radg_set_mode(mode) /= RADG_OK) call die()
+
+  nparam = ngas + 1
+  allocate (doutputdq(maxpat, maxlay, nparmax))
+  open (11, file=trim(fout), access='stream', form='unformatted', status='replace')
+  do iwave = 1, nwave
+    vv = vwave(iwave)
+    call cirsradg_wave(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp, amount, &
+                       iwave, ispace, amform, vv, nlayin, layinc, cont, scale, imod, idgas, &
+                       isogas, emtemp, itype, nem, vem, emissivity, tsurf, gtsurf, radius1, &
+                       flagh2p, hfp, radextra, nsw, isw, output, doutputdq)
+    if (radg_cirsradg_status() /= RADG_OK) call die()
+    write (11) output(1:npath), gtsurf(1:npath), doutputdq(1:npath, 1:nlayer, 1:nparam)
+  end do
+  allocate (sout(nwave * npath), sgts(nwave * npath), sdq(nwave * npath * nlayer * nparam))
+  rc = radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp, amount, &
+                        ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp, itype, nem, &
+                        vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave, sout, sgts, sdq)
+  if (rc /= RADG_OK) call die()
+  write (11) sout, sgts, sdq
+  close (11)
+
+contains
+
+  subroutine die()
+    write (0, '(a)') 'radg error: ' // trim(radg_error_message())
+    stop 1
+  end subroutine
+end program radg_host

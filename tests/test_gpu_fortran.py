@@ -1,0 +1,116 @@
+"""The drop-in from Fortran: a flang-built host calls ``cirsradg_wave``
+unchanged (implicit interface, REAL(4) arrays with the arrdef.f leading
+dimensions, once per wavenumber as cirsrtfg_wave.f:397-401 does) and the
+whole-spectrum entry once, with tables registered from ``.kta`` files through
+the ISO_C_BINDING module.  Both must match the reference's FP64 outputs.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from radtrancode_amd import build as rbuild
+from radtrancode_amd.cirsradg import MAXCON, MAXGAS, MAXLAY, MAXPAT, NPARMAX, fortran_arrays
+from radtrancode_amd.kta import write_kta
+from tests.golden.golden_io import case_paths, load_case, load_cia
+from tests.tolerance import jac_err, rad_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _write_input(path, klist, req, cia, mode):
+    a = fortran_arrays(req)
+    f4 = lambda x: np.asarray(x, dtype=np.float32).tobytes(order="F")
+    i4 = lambda x: np.asarray(x, dtype=np.int32).tobytes(order="F")
+    with open(path, "wb") as f:
+        kb = klist.encode()
+        f.write(i4(len(kb)) + kb + i4([4, mode]))
+        f.write(np.float64(cia.dnu).tobytes() + np.asarray(cia.temps, np.float64).tobytes()
+                + np.ascontiguousarray(cia.kcia, np.float64).tobytes())
+        f.write(i4([req.nlayer, req.npath, req.ngas, req.nwave, len(req.vem), len(req.isw),
+                    req.inormal, req.iray, req.ispace, req.itype, req.flagh2p, req.amform]))
+        f.write(f4([req.dist, req.tsurf, req.radius1, req.radextra]))
+        for k in ("delh", "press", "temp", "pp", "amount"):
+            f.write(f4(a[k]))
+        f.write(i4(a["nlayin"]) + i4(a["layinc"]) + f4(a["cont"]) + f4(a["scale"]))
+        f.write(i4(a["imod"]) + i4(a["idgas"]) + i4(a["isogas"]))
+        f.write(f4(a["emtemp"]) + f4(a["vem"]) + f4(a["emissivity"]) + f4(a["hfp"]))
+        f.write(i4(a["isw"]) + f4(req.vwave))
+
+
+def _read_output(path, req):
+    nw, npath, L, npar = req.nwave, req.npath, req.nlayer, req.ngas + 1
+    raw = np.fromfile(path, dtype=np.float32)
+    per = 2 * npath + npath * L * npar
+    o = np.zeros((nw, npath))
+    g = np.zeros((nw, npath))
+    d = np.zeros((nw, npath, L, npar))
+    for iw in range(nw):
+        blk = raw[iw * per:(iw + 1) * per]
+        o[iw] = blk[:npath]
+        g[iw] = blk[npath:2 * npath]
+        d[iw] = blk[2 * npath:].reshape((npath, L, npar), order="F")
+    rest = raw[nw * per:]
+    n1 = nw * npath
+    so = rest[:n1].reshape(nw, npath)
+    sg = rest[n1:2 * n1].reshape(nw, npath)
+    sd = rest[2 * n1:].reshape(nw, npath, L, npar)
+    return (o, g, d), (so, sg, sd)
+
+
+@pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths"])
+def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
+    host = rbuild.FORTRAN_HOST
+    assert os.path.exists(host), "Fortran host not built (flang missing at build time?)"
+    path = [p for p in case_paths() if p.endswith(f"case_{name}.npz")][0]
+    tables, req, outs, note = load_case(path)
+    cia = load_cia()
+    names = []
+    for t, kt in enumerate(tables):
+        kp = str(tmp_path / f"g{t}.kta")
+        write_kta(kp, kt, recl=4)
+        names.append(kp)
+    klist = str(tmp_path / "k.kls")
+    with open(klist, "w") as f:
+        f.write("\n".join(names) + "\n")
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, klist, req, cia, mode=0)
+    print(f"fortran host case {name}", flush=True)
+    p = subprocess.run([host, fin, fout], capture_output=True, text=True, timeout=110)
+    assert p.returncode == 0, p.stderr
+    (o, g, d), (so, sg, sd) = _read_output(fout, req)
+    assert np.array_equal(o, so) and np.array_equal(d, sd)
+    # The host hands over REAL(4) arrays and reads REAL(4) .kta files, so the
+    # FP64 oracle is run on the same rounded inputs: the plumbing must then
+    # agree to the FP64-form tolerances.
+    from oracle import oracle
+    r32 = _rounded(req)
+    t32 = [_rounded_table(kt) for kt in tables]
+    o_ref, g_ref, d_ref = oracle.spectrum(t32, cia, r32)
+    assert rad_err(o, o_ref) < 1e-5, note
+    assert rad_err(g, g_ref) < 1e-5, note
+    assert jac_err(d, d_ref) < 1e-4, note
+    # and it stays close to what the reference's own REAL(4) build returns
+    assert rad_err(o, outs["ref32_output"]) < 1e-5, note
+    assert jac_err(d, outs["ref32_doutputdq"]) < 1e-3, note
+
+
+def _rounded(req):
+    import copy
+    r = copy.copy(req)
+    for k in ("press", "temp", "pp", "amount", "delh", "vwave", "scale", "emtemp", "vem",
+              "emissivity"):
+        setattr(r, k, np.asarray(getattr(req, k), dtype=np.float32).astype(np.float64))
+    r.tsurf = float(np.float32(req.tsurf))
+    return r
+
+
+def _rounded_table(kt):
+    import copy
+    t = copy.copy(kt)
+    for k in ("press", "temp", "g_ord", "delg", "k"):
+        setattr(t, k, np.asarray(getattr(kt, k), dtype=np.float32).astype(np.float64))
+    t.vmin = float(np.float32(kt.vmin))
+    t.delv = float(np.float32(kt.delv))
+    return t

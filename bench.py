@@ -106,6 +106,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true", help="time only the mixed form")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a kernel of known "
+                         "traffic, to scale rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -118,6 +122,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     tables, cia, reqs = make_batch_inputs(args.batch)
     rad.register_ktables(tables)
     rad.register_cia(cia)
@@ -140,9 +151,13 @@ def main():
     b.close()
 
     # ---- plain FP64 form, same batch
-    b64, wall64, lat64, kern64 = time_batch(rad, reqs, rad.MODE_FP64, args.fp64_steps, 1, torch)
-    b64.close()
-    value64 = args.batch * args.fp64_steps / wall64
+    if args.skip_fp64:
+        wall64, kern64, value64 = float("nan"), np.full(3, np.nan), float("nan")
+    else:
+        b64, wall64, lat64, kern64 = time_batch(rad, reqs, rad.MODE_FP64, args.fp64_steps, 1,
+                                                torch)
+        b64.close()
+        value64 = args.batch * args.fp64_steps / wall64
 
     # ---- roofline of the dominant kernel (layer/overlap kernel): it reads
     # the table nodes and the layer inputs of every spectrum in the batch

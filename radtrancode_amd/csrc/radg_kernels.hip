@@ -56,61 +56,186 @@ template <typename T> __device__ __forceinline__ T dinf();
 template <> __device__ __forceinline__ float dinf<float>() { return __builtin_huge_valf(); }
 template <> __device__ __forceinline__ double dinf<double>() { return __builtin_huge_val(); }
 
-template <typename K>
-__device__ __forceinline__ bool key_lt(K ka, int ia, K kb, int ib) {
-  return ka < kb || (ka == kb && ia < ib);
-}
+// Ascending bitonic networks over 64*E slots held E per lane (lane l owns
+// slots l*E .. l*E+E-1), in the form where the first stage of each merge
+// compares slot e with its mirror e ^ (k-1) and every later stage compares
+// e with e ^ j; every exchange keeps the smaller value at the lower slot, so
+// the only lane-dependent quantities are the six lane bits.
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
 
-// Ascending bitonic sort of 64*E (key, index) pairs held E per lane, lane l
-// owning elements l*E .. l*E+E-1.  Ties are broken by index, so the order is
-// a strict total order and every lane agrees on it.
-template <typename K, int E>
-__device__ __forceinline__ void bitonic_sort(K (&key)[E], int (&idx)[E], int lane) {
-  constexpr int LOGP = (E == 1) ? 6 : (E == 2) ? 7 : (E == 4) ? 8 : (E == 8) ? 9 : 10;
-  static_assert(W * E == (1 << LOGP), "E must be 1, 2, 4, 8 or 16");
-  // linear induction variables so that both loops unroll completely and every
-  // register-array index is a compile-time constant
+template <typename V>
+struct SortOps;
+
+// FP32 keys are >= 0, so their IEEE bit patterns order like their values:
+// one u64 holds (key bits << 32 | element index); one 64-bit compare orders
+// (key, index) pairs, ties broken by index.
+template <>
+struct SortOps<unsigned long long> {
+  static __device__ __forceinline__ bool lt(unsigned long long a, unsigned long long b) { return a < b; }
+  static __device__ __forceinline__ unsigned long long shfl(unsigned long long a, int m) {
+    return __shfl_xor(a, m);
+  }
+};
+
+template <typename V, int E>
+__device__ __forceinline__ void bitonic_sort_v(V (&v)[E], int lane) {
+  using O = SortOps<V>;
+  constexpr int LOGE = ilog2c(E);
+  constexpr int LOGP = 6 + LOGE;
 #pragma unroll
   for (int ks = 1; ks <= LOGP; ks++) {
-    const int k = 1 << ks;
+    // mirror stage: partner e ^ (2^ks - 1)
+    if (ks - 1 >= LOGE) {
+      const int lm = ((1 << ks) - 1) >> LOGE;
+      const bool lower = ((lane >> (ks - 1 - LOGE)) & 1) == 0;
+      V o[E];
 #pragma unroll
-    for (int js = ks - 1; js >= 0; js--) {
-      const int j = 1 << js;
-      if (j >= E) {
-        const int lm = j / E;
+      for (int q = 0; q < E; q++) o[q] = O::shfl(v[E - 1 - q], lm);
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const bool olt = O::lt(o[q], v[q]);
+        v[q] = (olt == lower) ? o[q] : v[q];
+      }
+    } else {
+      const int km = (1 << ks) - 1;
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int q2 = q ^ km;
+        if (q2 > q) {
+          const V x = v[q], y = v[q2];
+          const bool sw = O::lt(y, x);
+          v[q] = sw ? y : x;
+          v[q2] = sw ? x : y;
+        }
+      }
+    }
+    // half-cleaners
+#pragma unroll
+    for (int js = ks - 2; js >= 0; js--) {
+      if (js >= LOGE) {
+        const int lm = 1 << (js - LOGE);
+        const bool lower = ((lane >> (js - LOGE)) & 1) == 0;
 #pragma unroll
         for (int q = 0; q < E; q++) {
-          const int e = lane * E + q;
-          const K ok = __shfl_xor(key[q], lm);
-          const int oi = __shfl_xor(idx[q], lm);
-          const bool up = (e & k) == 0;
-          const bool lower = (e & j) == 0;
-          const bool other_less = key_lt(ok, oi, key[q], idx[q]);
-          const bool take = (lower == up) ? other_less : !other_less;
-          if (take) {
-            key[q] = ok;
-            idx[q] = oi;
-          }
+          const V o = O::shfl(v[q], lm);
+          const bool olt = O::lt(o, v[q]);
+          v[q] = (olt == lower) ? o : v[q];
         }
       } else {
+        const int j = 1 << js;
 #pragma unroll
         for (int q = 0; q < E; q++) {
           const int q2 = q ^ j;
           if (q2 > q) {
-            const int e = lane * E + q;
-            const bool up = (e & k) == 0;
-            const bool sw = up ? key_lt(key[q2], idx[q2], key[q], idx[q])
-                               : key_lt(key[q], idx[q], key[q2], idx[q2]);
-            if (sw) {
-              K tk = key[q]; key[q] = key[q2]; key[q2] = tk;
-              int ti = idx[q]; idx[q] = idx[q2]; idx[q2] = ti;
-            }
+            const V x = v[q], y = v[q2];
+            const bool sw = O::lt(y, x);
+            v[q] = sw ? y : x;
+            v[q2] = sw ? x : y;
           }
         }
       }
     }
   }
 }
+
+template <typename T, int E> struct Sorter;
+template <int E> struct Sorter<float, E> {
+  static __device__ __forceinline__ void sort(float (&key)[E], int (&idx)[E], int lane) {
+    unsigned long long v[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const float k = (key[q] > 0.0f) ? key[q] : 0.0f;  // -0 and 0 order alike
+      v[q] = ((unsigned long long)__float_as_uint(k) << 32) | (unsigned int)idx[q];
+    }
+    bitonic_sort_v<unsigned long long, E>(v, lane);
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      key[q] = __uint_as_float((unsigned int)(v[q] >> 32));
+      idx[q] = (int)(unsigned int)(v[q] & 0xffffffffull);
+    }
+  }
+};
+// FP64 form: the same network on separate key and index arrays.
+__device__ __forceinline__ bool kd_lt(double ka, int ia, double kb, int ib) {
+  return (ka < kb) | ((ka == kb) & (ia < ib));
+}
+
+template <int E>
+__device__ __forceinline__ void bitonic_sort_kd(double (&k)[E], int (&x)[E], int lane) {
+  constexpr int LOGE = ilog2c(E);
+  constexpr int LOGP = 6 + LOGE;
+#pragma unroll
+  for (int ks = 1; ks <= LOGP; ks++) {
+    if (ks - 1 >= LOGE) {
+      const int lm = ((1 << ks) - 1) >> LOGE;
+      const bool lower = ((lane >> (ks - 1 - LOGE)) & 1) == 0;
+      double ok[E];
+      int ox[E];
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        ok[q] = __shfl_xor(k[E - 1 - q], lm);
+        ox[q] = __shfl_xor(x[E - 1 - q], lm);
+      }
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const bool take = kd_lt(ok[q], ox[q], k[q], x[q]) == lower;
+        k[q] = take ? ok[q] : k[q];
+        x[q] = take ? ox[q] : x[q];
+      }
+    } else {
+      const int km = (1 << ks) - 1;
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int q2 = q ^ km;
+        if (q2 > q) {
+          const bool sw = kd_lt(k[q2], x[q2], k[q], x[q]);
+          const double a0 = k[q], a1 = k[q2];
+          const int b0 = x[q], b1 = x[q2];
+          k[q] = sw ? a1 : a0;
+          k[q2] = sw ? a0 : a1;
+          x[q] = sw ? b1 : b0;
+          x[q2] = sw ? b0 : b1;
+        }
+      }
+    }
+#pragma unroll
+    for (int js = ks - 2; js >= 0; js--) {
+      if (js >= LOGE) {
+        const int lm = 1 << (js - LOGE);
+        const bool lower = ((lane >> (js - LOGE)) & 1) == 0;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const double ok = __shfl_xor(k[q], lm);
+          const int ox = __shfl_xor(x[q], lm);
+          const bool take = kd_lt(ok, ox, k[q], x[q]) == lower;
+          k[q] = take ? ok : k[q];
+          x[q] = take ? ox : x[q];
+        }
+      } else {
+        const int j = 1 << js;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int q2 = q ^ j;
+          if (q2 > q) {
+            const bool sw = kd_lt(k[q2], x[q2], k[q], x[q]);
+            const double a0 = k[q], a1 = k[q2];
+            const int b0 = x[q], b1 = x[q2];
+            k[q] = sw ? a1 : a0;
+            k[q2] = sw ? a0 : a1;
+            x[q] = sw ? b1 : b0;
+            x[q2] = sw ? b0 : b1;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int E> struct Sorter<double, E> {
+  static __device__ __forceinline__ void sort(double (&key)[E], int (&idx)[E], int lane) {
+    bitonic_sort_kd<E>(key, idx, lane);
+  }
+};
 
 // ----------------------------------------------------------- CIA, FP64
 // ciaread.f:104-196 + nciacon.f:209-303 for one layer.  dab[0..6] as
@@ -204,9 +329,14 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
 }  // namespace
 
 // ============================================================ layer kernel
-// LDS layout (dynamic): see lds_layer_bytes() on the host.
+// LDS layout (dynamic), sized by radg_lds_layer_bytes():
+//   G[NG+1] | recS[NG][NQ] | recC[NG][NQ] | recG[NG] | recW[NG]      (FP64)
+//   kgn[ngas][NG] | dkgn[ngas][NG] | bufA[NS][NG] | bufB[NS][NG] | dlg[NG]  (T)
+//   rtype[NG]                                                      (int)
+// with NS = ngas + 2 row slots (K, gas 0..ngas-1, T) and NQ = ngas + 2 rebinned
+// quantities (key, gas slots, T).
 template <typename T, int E>
-__global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nblocks) {
+__global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
   if (item >= a.nA) return;
   const int lane = threadIdx.x;
@@ -219,23 +349,29 @@ __global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nbloc
   const LayerConst lc = a.lay[R.lay0 + j];
   const double vwave = a.vwave[R.wave0 + iw];
   const int n = NG * NG;
-  const int NS = ngas + 2;  // row buffer slots: K, gas 0..ngas-1, T
+  const int NS = ngas + 2;
+  const int NQ = ngas + 2;
 
   extern __shared__ double smem_d[];
-  double *G = smem_d;                      // [NG+1]
-  double *sgd = G + (NG + 1);              // [64E]
-  T *kgn = reinterpret_cast<T *>(sgd + W * E);  // [ngas][NG]
-  T *dkgn = kgn + ngas * NG;               // [ngas][NG]
-  T *bufA = dkgn + ngas * NG;              // [NS][NG]
-  T *bufB = bufA + NS * NG;                // [NS][NG]
-  T *skey = bufB + NS * NG;                // [64E]
-  T *dlg = skey + W * E;                   // [NG]
-  int *sidx = reinterpret_cast<int *>(dlg + NG);  // [64E]
+  double *G = smem_d;
+  double *recS = G + (NG + 1);
+  double *recC = recS + NG * NQ;
+  double *recG = recC + NG * NQ;
+  double *recW = recG + NG;
+  T *kgn = reinterpret_cast<T *>(recW + NG);
+  T *dkgn = kgn + ngas * NG;
+  T *bufA = dkgn + ngas * NG;
+  T *bufB = bufA + NS * NG;
+  T *dlg = bufB + NS * NG;
+  int *rtype = reinterpret_cast<int *>(dlg + NG);
 
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
                                    : reinterpret_cast<const T *>(a.delg32);
   for (int i = lane; i <= NG; i += W) G[i] = a.G[i];
-  for (int i = lane; i < NG; i += W) dlg[i] = delg[i];
+  for (int i = lane; i < NG; i += W) {
+    dlg[i] = delg[i];
+    rtype[i] = 0;
+  }
 
   // ---- k-table interpolation, get_kg.f:308-476 (coincident branch) -------
   {
@@ -296,6 +432,9 @@ __global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nbloc
   }
   __syncthreads();
 
+  // first (i, j) of this lane's keys, e = lane*E = i*NG + j
+  const int i00 = (lane * E) / NG, j00 = lane * E - i00 * NG;
+
   T *rows = bufA, *nxt = bufB;
   for (int r = 1; r < ngas; r++) {
     const T a2 = (T)amo[r];
@@ -319,33 +458,40 @@ __global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nbloc
         nxt[(1 + r) * NG + i] = (T)0;
       }
     } else {
-      // keys K_i + k2_j*a2, element e = i*NG + j (noverlapg.f:117-128,175-187)
+      // keys K_i + k2_j*a2 (noverlapg.f:117-128,175-187); the index packs
+      // (i, j) as i*32 + j, padding slots get unique indices >= 1024
       T key[E];
       int idx[E];
+      {
+        int ii = i00, jj = j00;
 #pragma unroll
-      for (int q = 0; q < E; q++) {
-        const int e = lane * E + q;
-        if (e < n) {
-          const int i = e / NG, jj = e - (e / NG) * NG;
-          key[q] = rows[i] + k2[jj] * a2;
-        } else {
-          key[q] = dinf<T>();
+        for (int q = 0; q < E; q++) {
+          const int e = lane * E + q;
+          if (e < n) {
+            key[q] = rows[ii] + k2[jj] * a2;
+            idx[q] = (ii << 5) | jj;
+          } else {
+            key[q] = dinf<T>();
+            idx[q] = 1024 + e;
+          }
+          if (++jj == NG) {
+            jj = 0;
+            ++ii;
+          }
         }
-        idx[q] = e;
       }
-      bitonic_sort<T, E>(key, idx, lane);
-      // FP64 cumulative weights (rankg.f:104-108)
-      double incl[E];
+      Sorter<T, E>::sort(key, idx, lane);
+
+      // weights and their FP64 cumulative sum gdist (rankg.f:104-108)
+      T wq[E];
+      double gd[E];
       double run = 0.0;
 #pragma unroll
       for (int q = 0; q < E; q++) {
-        double w = 0.0;
-        if (idx[q] < n) {
-          const int i = idx[q] / NG, jj = idx[q] - (idx[q] / NG) * NG;
-          w = (double)(dlg[i] * dlg[jj]);
-        }
-        run += w;
-        incl[q] = run;
+        const int id = idx[q];
+        wq[q] = (id < 1024) ? dlg[id >> 5] * dlg[id & 31] : (T)0;
+        run += (double)wq[q];
+        gd[q] = run;
       }
       double s = run;
 #pragma unroll
@@ -357,101 +503,135 @@ __global__ __launch_bounds__(64) void layer_kernel(KernelArgs a, long long nbloc
       double ex = __shfl_up(s, 1);
       if (lane == 0) ex = 0.0;
 #pragma unroll
+      for (int q = 0; q < E; q++) gd[q] += ex;
+      double p0[E];
+      p0[0] = __shfl_up(gd[E - 1], 1);
+      if (lane == 0) p0[0] = 0.0;
+#pragma unroll
+      for (int q = 1; q < E; q++) p0[q] = gd[q - 1];
+
+      // bin of each element = number of edges G[1..NG] <= p0; an element
+      // straddles when its end reaches the next edge (rankg.f:120)
+      // bq: bin | straddle flag << 8; omf: 1 - frac of a straddling element
+      int bq[E];
+      T omf[E];
+      {
+        int lo = 0, hi = NG;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (G[mid] <= p0[0]) lo = mid;
+          else hi = mid - 1;
+        }
+        int b = lo;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          if (q > 0)
+            while (b < NG && G[b + 1] <= p0[q]) b++;
+          const bool stq = (b < NG) && !(gd[q] < G[b + 1]);
+          bq[q] = b | (stq ? 256 : 0);
+          omf[q] = stq ? (T)(1.0 - (G[b + 1] - p0[q]) / (gd[q] - p0[q])) : (T)0;
+        }
+      }
+      // bin records: a straddling element closes its bin; the last slot
+      // closes the bin it lies in (or, if it straddles, opens the next one)
+#pragma unroll
       for (int q = 0; q < E; q++) {
         const int e = lane * E + q;
-        skey[e] = key[q];
-        sidx[e] = idx[q];
-        sgd[e] = ex + incl[q];
+        const int bi = bq[q] & 255;
+        const bool stq = bq[q] > 255;
+        if (stq) {
+          recG[bi] = gd[q];
+          recW[bi] = (double)omf[q] * (double)wq[q];
+          rtype[bi] = 1;
+        }
+        if (e == W * E - 1) {
+          const int bb = stq ? bi + 1 : bi;
+          if (bb < NG) {
+            recG[bb] = gd[q];
+            recW[bb] = 0.0;
+            rtype[bb] = 2;
+          }
+        }
+      }
+      // weighted key and payloads: FP64 prefix sums along the sorted order
+      const int nslot = r + 2, nq = nslot + 1;
+      for (int vq = 0; vq < nq; vq++) {
+        const int s2 = vq - 1;
+        double pre[E];
+        T cq[E];
+        double run2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int id = idx[q];
+          T c = (T)0;
+          if (id < 1024) {
+            const int i = id >> 5, jj = id & 31;
+            T val;
+            if (vq == 0) val = key[q];
+            else if (s2 < r) val = rows[(1 + s2) * NG + i];
+            else if (s2 == r) val = k2[jj];
+            else val = rows[(1 + ngas) * NG + i] + d2[jj] * a2;
+            c = val * wq[q];
+          }
+          cq[q] = c;
+          run2 += (double)c;
+          pre[q] = run2;
+        }
+        double s3 = run2;
+#pragma unroll
+        for (int ds = 0; ds < 6; ds++) {
+          const int d = 1 << ds;
+          const double o = __shfl_up(s3, d);
+          if (lane >= d) s3 += o;
+        }
+        double ex3 = __shfl_up(s3, 1);
+        if (lane == 0) ex3 = 0.0;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int e = lane * E + q;
+          const double pq = pre[q] + ex3;
+          const int bi = bq[q] & 255;
+          const bool stq = bq[q] > 255;
+          if (stq) {
+            recS[bi * NQ + vq] = pq;
+            recC[bi * NQ + vq] = (double)omf[q] * (double)cq[q];
+          }
+          if (e == W * E - 1) {
+            const int bb = stq ? bi + 1 : bi;
+            if (bb < NG) {
+              recS[bb * NQ + vq] = pq;
+              recC[bb * NQ + vq] = 0.0;
+            }
+          }
+        }
       }
       __syncthreads();
-      // bin lanes: bin b gathers, in sorted order, the carry of the element
-      // straddling G[b], the elements starting in [G[b], G[b+1]) and the
-      // straddling share of the last one (rankg.f:117-152)
+      // bin b = its records minus the previous bin's, plus the share carried
+      // over from the element straddling its lower edge; normalised by the
+      // accumulated weight when a straddle closed it, or when it is the last
+      // bin (rankg.f:129-152)
       if (lane < NG) {
         const int b = lane;
-        auto lb = [&](double g) {  // first e in [0,n] with p0(e) >= g
-          int lo = 0, hi = n;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            const double p0 = (mid == 0) ? 0.0 : sgd[mid - 1];
-            if (p0 >= g) hi = mid;
-            else lo = mid + 1;
+        const int t = rtype[b];
+        if (t == 0) {
+          for (int sl = 0; sl < NS; sl++) nxt[sl * NG + b] = (T)0;
+        } else {
+          const bool hp = b > 0;
+          const double sw =
+              recG[b] - (hp ? recG[b - 1] : 0.0) + (hp ? recW[b - 1] : 0.0) - recW[b];
+          const bool norm = (t == 1) || (b == NG - 1);
+          const T smT = (T)sw;
+          for (int vq = 0; vq < nq; vq++) {
+            const double Sv = recS[b * NQ + vq] - (hp ? recS[(b - 1) * NQ + vq] : 0.0) +
+                              (hp ? recC[(b - 1) * NQ + vq] : 0.0) - recC[b * NQ + vq];
+            T val = (T)Sv;
+            if (norm) val = val / smT;
+            const int sl = (vq == 0) ? 0 : ((vq - 1 == r + 1) ? 1 + ngas : vq);
+            nxt[sl * NG + b] = val;
           }
-          return lo;
-        };
-        const int sb = (b == 0) ? 0 : lb(G[b]);
-        const int se = lb(G[b + 1]);
-        const int nslot = r + 2;  // gas slots 0..r, then T
-        T kacc = 0;
-        double sum = 0.0;
-        for (int s2 = 0; s2 < nslot; s2++) nxt[(1 + (s2 == r + 1 ? ngas : s2)) * NG + b] = (T)0;
-        auto payload = [&](int e, int s2) -> T {
-          const int id = sidx[e];
-          const int i = id / NG, jj = id - (id / NG) * NG;
-          if (s2 < r) return rows[(1 + s2) * NG + i];
-          if (s2 == r) return k2[jj];
-          return rows[(1 + ngas) * NG + i] + d2[jj] * a2;
-        };
-        auto wgt = [&](int e) -> T {
-          const int id = sidx[e];
-          const int i = id / NG, jj = id - (id / NG) * NG;
-          return dlg[i] * dlg[jj];
-        };
-        bool reached = sb < se;
-        // carry-in from the element that straddled G[b]
-        if (b > 0 && sb > 0 && sb <= n) {
-          const int c = sb - 1;
-          const double p0 = (c == 0) ? 0.0 : sgd[c - 1];
-          const double p1 = sgd[c];
-          if (p1 >= G[b]) {
-            const double frac = (G[b] - p0) / (p1 - p0);
-            const T w = wgt(c);
-            const T f1 = (T)(1.0 - frac);
-            sum = (1.0 - frac) * (double)w;
-            kacc = f1 * skey[c] * w;
-            for (int s2 = 0; s2 < nslot; s2++) {
-              const int sl = 1 + (s2 == r + 1 ? ngas : s2);
-              nxt[sl * NG + b] = f1 * payload(c, s2) * w;
-            }
-            reached = true;
-          }
+          for (int s4 = r + 1; s4 < ngas; s4++) nxt[(1 + s4) * NG + b] = (T)0;
         }
-        bool normalised = false;
-        for (int e = sb; e < se; e++) {
-          const double p1 = sgd[e];
-          const T w = wgt(e);
-          const T ck = skey[e];
-          if (!(p1 >= G[b + 1])) {
-            kacc = kacc + ck * w;
-            for (int s2 = 0; s2 < nslot; s2++) {
-              const int sl = 1 + (s2 == r + 1 ? ngas : s2);
-              nxt[sl * NG + b] = nxt[sl * NG + b] + payload(e, s2) * w;
-            }
-            sum = sum + (double)w;
-          } else {
-            const double p0 = (e == 0) ? 0.0 : sgd[e - 1];
-            const double frac = (G[b + 1] - p0) / (p1 - p0);
-            const T ff = (T)frac;
-            kacc = kacc + ff * ck * w;
-            for (int s2 = 0; s2 < nslot; s2++) {
-              const int sl = 1 + (s2 == r + 1 ? ngas : s2);
-              nxt[sl * NG + b] = nxt[sl * NG + b] + ff * payload(e, s2) * w;
-            }
-            sum = sum + frac * (double)w;
-            normalised = true;
-            break;
-          }
-        }
-        if (normalised || (reached && b == NG - 1)) {
-          const T sm = (T)sum;
-          kacc = kacc / sm;
-          for (int s2 = 0; s2 < nslot; s2++) {
-            const int sl = 1 + (s2 == r + 1 ? ngas : s2);
-            nxt[sl * NG + b] = nxt[sl * NG + b] / sm;
-          }
-        }
-        nxt[b] = kacc;
-        for (int s2 = r + 1; s2 < ngas; s2++) nxt[(1 + s2) * NG + b] = (T)0;
+        rtype[b] = 0;
       }
     }
     __syncthreads();
@@ -809,10 +989,12 @@ hipError_t launch_em_RL(const KernelArgs &a, size_t lds, hipStream_t st) {
 }  // namespace
 
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
+  (void)E;
   const size_t ts = fp64 ? 8 : 4;
-  size_t b = (size_t)(ng + 1) * 8 + (size_t)W * E * 8;
-  b += ts * ((size_t)2 * ngas * ng + (size_t)2 * (ngas + 2) * ng + (size_t)W * E + ng);
-  b += (size_t)W * E * 4;
+  const size_t nq = ngas + 2, ns = ngas + 2;
+  size_t b = 8 * ((size_t)(ng + 1) + 2 * (size_t)ng * nq + 2 * (size_t)ng);
+  b += ts * ((size_t)2 * ngas * ng + (size_t)2 * ns * ng + ng);
+  b += 4 * (size_t)ng;
   return b;
 }
 

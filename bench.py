@@ -97,6 +97,40 @@ def cpu_baseline(tables, cia, req, seconds):
                        f"({what}) in {total_t:.1f} s")
 
 
+def serving_run(rad, n, rate):
+    """The serving mix: ``n`` requests (layers 40-200, wavenumber windows of
+    200-5000 points on a shared 5000-point grid, 1-6 k-gases + H2/He)
+    arriving Poisson at ``rate``/s, mixed form, admitted by device-memory
+    budget into continuously formed batches on 2 streams."""
+    from radtrancode_amd import serving
+
+    tabs = serving.serving_tables()
+    rad.register_ktables(tabs)
+    arrivals, reqs = serving.poisson_workload(n=n, rate=rate)
+    budget = 16 << 30
+    srv = serving.Server(mode=rad.MODE_MIXED, budget_bytes=budget, max_batch=32, max_inflight=2,
+                         per_path_outputs=False)
+    r = serving.run_poisson(srv, arrivals, reqs)
+    st = srv.stats()
+    srv.close()
+    lat = r["latency"] * 1e3
+    ok = int(np.sum(r["status"] == 0))
+    return {
+        "requests": n, "completed_ok": ok, "arrival_rate_per_s": rate,
+        "wall_s": r["wall"], "requests_per_s": n / r["wall"],
+        "config1_equivalent_spectra_per_s": r["work"] / r["wall"],
+        "latency_ms_p50": float(np.percentile(lat, 50)),
+        "latency_ms_p99": float(np.percentile(lat, 99)),
+        "latency_ms_max": float(lat.max()),
+        "batches": st["steps"], "largest_batch": st["max_batch"],
+        "peak_inflight_bytes": st["peak_bytes"], "budget_bytes": budget,
+        "worker_s": {"pack_upload_launch": st["create_s"], "retire": st["retire_s"],
+                     "kernels": st["gpu_s"]},
+        "mix": "layers U[40,200], wavenumbers U[200,5000] (window of a 5000-point grid), "
+               "k-gases U[1,6] + H2/He, 20 g, 5 angles; mixed form",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +141,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true", help="time only the mixed form")
+    ap.add_argument("--serving-requests", type=int, default=4096,
+                    help="requests in the Poisson serving run (0: skip it)")
+    ap.add_argument("--serving-rate", type=float, default=60.0,
+                    help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -173,6 +211,10 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(tables, cia, reqs[0], args.cpu_seconds)
 
+    serve = None
+    if args.serving_requests > 0:
+        serve = serving_run(rad, args.serving_requests, args.serving_rate)
+
     out = {
         "metric": "Jacobian spectra/s (120 lyr,1k wn,20 g) + p50/p99 latency vs plain FP64",
         "value": value,
@@ -203,6 +245,7 @@ def main():
                                  "sample": f"{len(sample)} wavenumbers of spectrum 0"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

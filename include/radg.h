@@ -15,8 +15,8 @@
  *   radg_register_ktables    the same registry from arrays already in memory
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
  *   radg_set_mode            (none: the reference's precision is fixed at compile time)
- *   radg_batch_*             (none: device-resident batches for the serving loop that
- *                            replaces nemesisPY/nemesis.py:4586-4664's process pool)
+ *   radg_batch_*, radg_server_*  (none: device-resident batches and the serving loop
+ *                            that replace nemesisPY/nemesis.py:4586-4664's process pool)
  */
 #ifndef RADG_H
 #define RADG_H
@@ -50,6 +50,8 @@ extern "C" {
 #define RADG_EHIP 5         /* a HIP runtime call failed */
 #define RADG_EBUDGET 6      /* a batch does not fit the device-memory budget */
 #define RADG_EIO 7          /* a .kta file could not be read */
+#define RADG_ECANCELLED 8   /* a serving request was cancelled before admission */
+#define RADG_ETIMEOUT 9     /* radg_server_wait timed out */
 
 /* Arithmetic forms. */
 #define RADG_MODE_FP64 0   /* every quantity in FP64 */
@@ -164,6 +166,10 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode,
 int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes);
 /* Launch every kernel of the batch on a HIP stream (NULL = default stream). */
 int radg_batch_launch(radg_batch *b, void *hip_stream);
+/* Also queue, behind each launch's kernels, copies of the outputs to pinned
+ * host memory: bit 0 the angle-weighted arrays, bit 1 the per-path arrays
+ * (0 = none).  radg_batch_read then serves from that copy. */
+int radg_batch_set_fetch(radg_batch *b, int what);
 /* Wait for the batch's stream. */
 int radg_batch_sync(radg_batch *b);
 /* Copy request r's results to host memory (FP64 regardless of mode).  Any
@@ -184,6 +190,38 @@ int radg_batch_timing_stats(radg_batch *b, int *nlaunch, float *sum_ms3);
  * nodes; plus the number of spectra and of layer-kernel work items. */
 int radg_batch_info(radg_batch *b, double *alg_bytes3, int *nspectra, int *nitems_a);
 int radg_batch_destroy(radg_batch *b);
+
+/* ------------------------------------------------------- serving loop */
+
+/* A persistent loop on a worker thread: requests are deep-copied, validated
+ * and sized at submit, queued, and admitted in arrival order into device
+ * batches while the in-flight batches' memory stays within budget_bytes
+ * (0 = unlimited) and a stream slot is free (max_inflight streams, each
+ * batch at most max_batch requests).  Requests that arrive while others run
+ * are admitted as slots and memory free up; their kernels overlap the running
+ * batches.  Results stay with the ticket until radg_server_wait collects them:
+ * always the angle-weighted arrays, and the per-path arrays too when
+ * per_path_outputs is non-zero. */
+typedef struct radg_server radg_server;
+int radg_server_create(int mode, size_t budget_bytes, int max_batch, int max_inflight,
+                       int per_path_outputs, radg_server **out);
+/* Returns the validation status of the request (its ticket then reports the
+ * same status); *ticket is always set. */
+int radg_server_submit(radg_server *s, const radg_request *req, long long *ticket);
+/* Cancel a request still waiting for admission (the reference's .abo file,
+ * nemesis/coreret.f:1090-1100); its status becomes RADG_ECANCELLED. */
+int radg_server_cancel(radg_server *s, long long ticket);
+/* Wait up to timeout_s seconds (< 0: forever) for a ticket; on completion the
+ * ticket is released, *status holds the request's status and *latency_s the
+ * time from submit to results on the host.  Outputs as radg_batch_read. */
+int radg_server_wait(radg_server *s, long long ticket, double timeout_s, double *output,
+                     double *gtsurf, double *doutputdq, double *wout, double *wgts, double *wjac,
+                     int *status, double *latency_s);
+/* steps (batches launched), requests admitted, requests completed, largest
+ * batch, peak in-flight bytes, queue length, then worker microseconds spent
+ * packing/uploading/launching, retiring results, and in kernels. */
+int radg_server_stats(radg_server *s, long long *stats9);
+int radg_server_destroy(radg_server *s);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libradg.so")
 
 RADG_OK = 0
+RADG_ECANCELLED = 8
+RADG_ETIMEOUT = 9
 MODE_FP64 = 0
 MODE_MIXED = 1
 
@@ -23,6 +25,8 @@ EXPORTS = [
     "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
     "radg_batch_launch", "radg_batch_sync", "radg_batch_read", "radg_batch_timings",
     "radg_batch_info", "radg_batch_destroy", "radg_batch_timing_stats",
+    "radg_server_create", "radg_server_submit", "radg_server_cancel", "radg_server_wait",
+    "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch",
 ]
 
 
@@ -64,7 +68,9 @@ def lib():
                  "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
                  "radg_batch_launch", "radg_batch_sync", "radg_batch_read",
                  "radg_batch_timings", "radg_batch_info", "radg_batch_destroy",
-                 "radg_batch_timing_stats"]:
+                 "radg_batch_timing_stats", "radg_server_create", "radg_server_submit",
+                 "radg_server_cancel", "radg_server_wait", "radg_server_stats",
+                 "radg_server_destroy"]:
         getattr(L, name).restype = ip
     L.radg_register_cia.argtypes = [C.c_double, vp, vp]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
@@ -77,6 +83,16 @@ def lib():
     L.radg_batch_info.argtypes = [vp, vp, C.POINTER(ip), C.POINTER(ip)]
     L.radg_batch_destroy.argtypes = [vp]
     L.radg_batch_timing_stats.argtypes = [vp, C.POINTER(ip), vp]
+    ll = C.c_longlong
+    L.radg_server_create.argtypes = [ip, C.c_size_t, ip, ip, ip, C.POINTER(vp)]
+    L.radg_batch_set_fetch.restype = ip
+    L.radg_batch_set_fetch.argtypes = [vp, ip]
+    L.radg_server_submit.argtypes = [vp, vp, C.POINTER(ll)]
+    L.radg_server_cancel.argtypes = [vp, ll]
+    L.radg_server_wait.argtypes = [vp, ll, C.c_double, vp, vp, vp, vp, vp, vp,
+                                   C.POINTER(ip), C.POINTER(C.c_double)]
+    L.radg_server_stats.argtypes = [vp, vp]
+    L.radg_server_destroy.argtypes = [vp]
     _lib = L
     return L
 

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <set>
 #include <string>
@@ -341,6 +342,17 @@ struct radg_batch {
   KernelArgs args{};
   int E = 1, RL = 1, maxngas = 1, maxnsw = 1;
   size_t ldsA = 0, ldsB = 0;
+  // emission launches, one per layers-per-lane class (chosen per request from
+  // its own layer count, so a request's arithmetic never depends on the
+  // other requests of its batch)
+  struct EmClass {
+    int RL, nreq;
+    long long nB;
+    const int *reqmap;
+    const long long *start;
+    size_t lds;
+  };
+  std::vector<EmClass> em;
   double alg_bytes[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
   // event ring: 4 events per launch, the last RING launches are kept
@@ -348,12 +360,96 @@ struct radg_batch {
   std::vector<hipEvent_t> evs;
   long long launches = 0;
   hipEvent_t *ev = nullptr;  // events of the last launch
-  ~radg_batch() {
-    if (dmem) (void)hipFree(dmem);
-    for (auto &e : evs)
-      if (e) (void)hipEventDestroy(e);
+  // output regions (byte offsets in dmem) and their pinned host copies
+  size_t o_out = 0, o_gts = 0, o_jac = 0, o_wout = 0, o_wgts = 0, o_wjac = 0;
+  size_t span_p0 = 0, span_p1 = 0, span_w0 = 0, span_w1 = 0;
+  int fetch = 0;           // bit 0: weighted outputs, bit 1: per-path outputs
+  char *hst = nullptr;     // pinned: [weighted span][per-path span]
+  ~radg_batch();
+};
+
+namespace {
+
+// Caching allocators.  Batches come and go at serving rates; a fresh
+// hipMalloc / hipHostMalloc of hundreds of MB per batch would dominate, so
+// freed blocks are kept (up to a cap) and handed out again for requests of
+// between half and all of their size.  The caches live until the process
+// ends.
+struct BlockCache {
+  std::mutex mu;
+  std::multimap<size_t, void *> freed;
+  std::map<void *, size_t> live;
+  size_t cached = 0, cap = 0;
+  bool pinned = false;
+  BlockCache(size_t c, bool p) : cap(c), pinned(p) {}
+  void *get(size_t n) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = freed.lower_bound(n);
+      if (it != freed.end() && it->first <= 2 * n + ((size_t)64 << 20)) {
+        void *p = it->second;
+        const size_t sz = it->first;
+        freed.erase(it);
+        cached -= sz;
+        live[p] = sz;
+        return p;
+      }
+    }
+    void *p = nullptr;
+    hipError_t e = pinned ? hipHostMalloc(&p, n, hipHostMallocDefault) : hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      // drop the cache and retry once
+      release_all();
+      e = pinned ? hipHostMalloc(&p, n, hipHostMallocDefault) : hipMalloc(&p, n);
+      if (e != hipSuccess) return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    live[p] = n;
+    return p;
+  }
+  void put(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = live.find(p);
+    if (it == live.end()) return;
+    const size_t sz = it->second;
+    live.erase(it);
+    if (cached + sz <= cap) {
+      freed.insert({sz, p});
+      cached += sz;
+    } else {
+      if (pinned) (void)hipHostFree(p);
+      else (void)hipFree(p);
+    }
+  }
+  void release_all() {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &kv : freed) {
+      if (pinned) (void)hipHostFree(kv.second);
+      else (void)hipFree(kv.second);
+    }
+    freed.clear();
+    cached = 0;
   }
 };
+
+BlockCache &dev_cache() {
+  static BlockCache *c = new BlockCache((size_t)48 << 30, false);
+  return *c;
+}
+BlockCache &host_cache() {
+  static BlockCache *c = new BlockCache((size_t)8 << 30, true);
+  return *c;
+}
+
+}  // namespace
+
+radg_batch::~radg_batch() {
+  dev_cache().put(dmem);
+  host_cache().put(hst);
+  for (auto &e : evs)
+    if (e) (void)hipEventDestroy(e);
+}
 
 namespace {
 
@@ -378,6 +474,9 @@ struct Plan {
             nwout = 0, nwjac = 0;
   std::vector<long long> sA, sB, sC;
   int maxngas = 1, maxnsw = 1, maxlays = 1;
+  std::vector<int> cls_rl;
+  std::vector<std::vector<int>> cls_members;
+  size_t o_clsmap = 0, o_clsstart = 0;
 };
 
 int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts, Plan &pl) {
@@ -426,6 +525,19 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     pl.maxnsw = std::max(pl.maxnsw, std::max(q.nsw, 1));
     pl.maxlays = std::max(pl.maxlays, q.nlayer);
   }
+  // emission classes: requests grouped by their own layers-per-lane
+  pl.cls_rl.clear();
+  pl.cls_members.clear();
+  for (int r = 0; r < nreq; r++) {
+    const int rl = radg_emission_RL(reqs[r].nlayer);
+    size_t c = 0;
+    while (c < pl.cls_rl.size() && pl.cls_rl[c] != rl) c++;
+    if (c == pl.cls_rl.size()) {
+      pl.cls_rl.push_back(rl);
+      pl.cls_members.emplace_back();
+    }
+    pl.cls_members[c].push_back(r);
+  }
   const size_t ts_ = (mode == RADG_MODE_FP64) ? 8 : 4;
   Layout lo;
   pl.o_reqs = lo.take<DevReq>(nreq);
@@ -443,6 +555,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_es = lo.take<double>(pl.nwv);
   pl.o_wg = lo.take<WaveGas>(pl.nwg);
   pl.o_pw = lo.take<double>(pl.npath);
+  pl.o_clsmap = lo.take<int>(nreq);
+  pl.o_clsstart = lo.take<long long>(nreq + pl.cls_rl.size());
   pl.o_inter = lo.take<char>((size_t)pl.ninter * ts_);
   pl.o_out = lo.take<char>((size_t)pl.nout * ts_);
   pl.o_gts = lo.take<char>((size_t)pl.nout * ts_);
@@ -607,7 +721,19 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   b->ts = ts;
   b->cia = cia;
   b->bytes = pl.total;
-  HIPCHK(hipMalloc(&b->dmem, pl.total));
+  b->dmem = static_cast<char *>(dev_cache().get(pl.total));
+  if (!b->dmem) return fail(RADG_EHIP, "device allocation of %zu bytes failed", pl.total);
+  b->o_out = pl.o_out;
+  b->o_gts = pl.o_gts;
+  b->o_jac = pl.o_jac;
+  b->o_wout = pl.o_wout;
+  b->o_wgts = pl.o_wgts;
+  b->o_wjac = pl.o_wjac;
+  const size_t tsz = (mode == RADG_MODE_FP64) ? 8 : 4;
+  b->span_p0 = pl.o_out;
+  b->span_p1 = pl.o_jac + (size_t)pl.njac * tsz;
+  b->span_w0 = pl.o_wout;
+  b->span_w1 = pl.o_wjac + (size_t)pl.nwjac * 8;
   // host staging of everything but the intermediates and outputs
   std::vector<char> h(pl.o_inter, 0);
   std::vector<DevReq> dr(nreq);
@@ -616,6 +742,21 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   put(h, pl.o_sA, pl.sA.data(), nreq + 1);
   put(h, pl.o_sB, pl.sB.data(), nreq + 1);
   put(h, pl.o_sC, pl.sC.data(), nreq + 1);
+  {
+    std::vector<int> cmap;
+    std::vector<long long> cstart;
+    for (const auto &mem : pl.cls_members) {
+      long long acc = 0;
+      cstart.push_back(0);
+      for (int r : mem) {
+        cmap.push_back(r);
+        acc += (long long)reqs[r].nwave * reqs[r].npath;
+        cstart.push_back(acc);
+      }
+    }
+    put(h, pl.o_clsmap, cmap.data(), cmap.size());
+    put(h, pl.o_clsstart, cstart.data(), cstart.size());
+  }
   for (int r = 0; r < nreq; r++) {
     const Packed &P = pl.packs[r];
     const DevReq &d = P.d;
@@ -682,8 +823,28 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   b->ldsB = radg_lds_emission_bytes(b->RL, pl.maxnsw, fp64);
   if (b->ldsA > 160 * 1024 || b->ldsB > 160 * 1024)
     return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", b->ldsA, b->ldsB);
+  {
+    size_t off = 0, soff = 0;
+    for (size_t c = 0; c < pl.cls_rl.size(); c++) {
+      radg_batch::EmClass ec;
+      ec.RL = pl.cls_rl[c];
+      ec.nreq = (int)pl.cls_members[c].size();
+      ec.nB = 0;
+      int maxnsw = 1;
+      for (int r : pl.cls_members[c]) {
+        ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
+        maxnsw = std::max(maxnsw, reqs[r].nsw);
+      }
+      ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
+      ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
+      ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
+      off += ec.nreq;
+      soff += ec.nreq + 1;
+      b->em.push_back(ec);
+    }
+  }
+  // events are created on first use of each ring slot
   b->evs.assign(4 * radg_batch::RING, nullptr);
-  for (auto &e : b->evs) HIPCHK(hipEventCreate(&e));
   b->ev = &b->evs[0];
   *out = b.release();
   return RADG_OK;
@@ -695,14 +856,44 @@ int radg_batch_launch(radg_batch *b, void *stream) {
   b->stream = st;
   const int fp64 = b->mode == RADG_MODE_FP64;
   b->ev = &b->evs[4 * (b->launches % radg_batch::RING)];
+  for (int i = 0; i < 4; i++)
+    if (!b->ev[i]) HIPCHK(hipEventCreate(&b->ev[i]));
   b->launches++;
   HIPCHK(hipEventRecord(b->ev[0], st));
   HIPCHK(radg_launch_layer_lds(b->args, fp64, b->E, b->ldsA, st));
   HIPCHK(hipEventRecord(b->ev[1], st));
-  HIPCHK(radg_launch_emission(b->args, fp64, b->RL, b->ldsB, st));
+  for (const auto &ec : b->em) {
+    KernelArgs a = b->args;
+    a.startB = ec.start;
+    a.nB = ec.nB;
+    a.reqmapB = ec.reqmap;
+    a.nreqB = ec.nreq;
+    HIPCHK(radg_launch_emission(a, fp64, ec.RL, ec.lds, st));
+  }
   HIPCHK(hipEventRecord(b->ev[2], st));
   HIPCHK(radg_launch_quadrature(b->args, fp64, st));
   HIPCHK(hipEventRecord(b->ev[3], st));
+  // queue the copies of the requested outputs to pinned host memory behind
+  // the kernels, so they overlap the next batch's work
+  const size_t wlen = b->span_w1 - b->span_w0;
+  if (b->fetch & 1)
+    HIPCHK(hipMemcpyAsync(b->hst, b->dmem + b->span_w0, wlen, hipMemcpyDeviceToHost, st));
+  if (b->fetch & 2)
+    HIPCHK(hipMemcpyAsync(b->hst + wlen, b->dmem + b->span_p0, b->span_p1 - b->span_p0,
+                          hipMemcpyDeviceToHost, st));
+  return RADG_OK;
+}
+
+int radg_batch_set_fetch(radg_batch *b, int what) {
+  if (!b || what < 0 || what > 3) return fail(RADG_EINPUT, "bad batch or fetch flags");
+  host_cache().put(b->hst);
+  b->hst = nullptr;
+  b->fetch = 0;
+  if (what == 0) return RADG_OK;
+  const size_t need = (b->span_w1 - b->span_w0) + (b->span_p1 - b->span_p0);
+  b->hst = static_cast<char *>(host_cache().get(need));
+  if (!b->hst) return fail(RADG_EHIP, "pinned allocation of %zu bytes failed", need);
+  b->fetch = what;
   return RADG_OK;
 }
 
@@ -769,15 +960,47 @@ int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf, double
     return RADG_OK;
   };
   const long long nout = (long long)d.nwave * d.npath;
+  const size_t nwj = (size_t)d.nwave * d.L * d.nparam;
+  if (b->fetch) {
+    // results were copied to pinned memory behind the kernels
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t wlen = b->span_w1 - b->span_w0;
+    const size_t ts_ = fp64 ? 8 : 4;
+    auto from_pinned = [&](size_t region, long long off, long long n, double *dst) {
+      if (!dst) return;
+      const char *src = b->hst + wlen + (region - b->span_p0) + (size_t)off * ts_;
+      if (fp64) std::memcpy(dst, src, n * sizeof(double));
+      else {
+        const float *f = reinterpret_cast<const float *>(src);
+        for (long long i = 0; i < n; i++) dst[i] = f[i];
+      }
+    };
+    auto wfrom = [&](size_t region, long long off, size_t n, double *dst) {
+      if (dst) std::memcpy(dst, b->hst + (region - b->span_w0) + off * 8, n * 8);
+    };
+    const bool need_p = output || gtsurf || doutputdq;
+    const bool need_w = wout || wgts || wjac;
+    if ((!need_p || (b->fetch & 2)) && (!need_w || (b->fetch & 1))) {
+      if (b->fetch & 2) {
+        from_pinned(b->o_out, d.out0, nout, output);
+        from_pinned(b->o_gts, d.out0, nout, gtsurf);
+        from_pinned(b->o_jac, d.jac0, nout * d.L * d.nparam, doutputdq);
+      }
+      if (b->fetch & 1) {
+        wfrom(b->o_wout, d.wout0, d.nwave, wout);
+        wfrom(b->o_wgts, d.wout0, d.nwave, wgts);
+        wfrom(b->o_wjac, d.wjac0, nwj, wjac);
+      }
+      return RADG_OK;
+    }
+  }
   int rc;
   if ((rc = fetch(a.out, d.out0, nout, output))) return rc;
   if ((rc = fetch(a.gts, d.out0, nout, gtsurf))) return rc;
   if ((rc = fetch(a.jac, d.jac0, nout * d.L * d.nparam, doutputdq))) return rc;
   if (wout) HIPCHK(hipMemcpy(wout, a.wout + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
   if (wgts) HIPCHK(hipMemcpy(wgts, a.wgts + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
-  if (wjac)
-    HIPCHK(hipMemcpy(wjac, a.wjac + d.wjac0, (size_t)d.nwave * d.L * d.nparam * sizeof(double),
-                     hipMemcpyDeviceToHost));
+  if (wjac) HIPCHK(hipMemcpy(wjac, a.wjac + d.wjac0, nwj * sizeof(double), hipMemcpyDeviceToHost));
   return RADG_OK;
 }
 

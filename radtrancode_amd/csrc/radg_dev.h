@@ -68,6 +68,10 @@ struct KernelArgs {
   int nreq;
   const long long *startA, *startB, *startC;  // [nreq+1] item prefix sums
   long long nA, nB, nC;
+  // emission launches run one layers-per-lane class at a time: startB and
+  // nB then cover the class's requests, listed in reqmapB[nreqB]
+  const int *reqmapB;
+  int nreqB;
   int ng;
   // tables
   const void *ktab;            // double* (FP64 form) or float* (mixed form)

@@ -721,11 +721,11 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const long long item = xcd_item(nblocks);
   if (item >= a.nB) return;
   const int lane = threadIdx.x;
-  const int rq = find_req(a.startB, a.nreq, item);
-  const DevReq &R = a.reqs[rq];
+  const int rqc = find_req(a.startB, a.nreqB, item);
+  const DevReq &R = a.reqs[a.reqmapB[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath, nsw = R.nsw,
             nparam = R.nparam;
-  const long long loc = item - R.itemB0;
+  const long long loc = item - a.startB[rqc];
   const int iw = (int)(loc / npath);
   const int p = (int)(loc - (long long)iw * npath);
   const PathInfo pi = a.paths[R.path0 + p];

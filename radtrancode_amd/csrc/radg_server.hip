@@ -1,0 +1,412 @@
+// Persistent serving loop for forward + Jacobian requests.
+//
+// A worker thread owns up to `max_inflight` device batches, each on its own
+// non-blocking HIP stream.  Submitted requests are deep-copied, validated
+// and sized at submit time, then queued.  Every pass of the worker retires
+// the batches whose completion event has fired (results are copied into
+// each request's ticket), then admits queued requests in arrival order into
+// a new batch while the sum of in-flight batch bytes stays within the HBM
+// budget.  Requests that arrive while earlier batches run are therefore
+// admitted as soon as a stream slot and memory free up, and their kernels
+// overlap the running ones on the device.  One launch per batch per step.
+//
+// This replaces the reference's process-pool pattern
+// (nemesisPY/nemesis.py:4586-4664: independent forward-model subprocesses that
+// communicate through files) for many independent solves.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/radg.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+// Deep copy of one radg_request.
+struct OwnedRequest {
+  radg_request q{};
+  std::vector<double> press, temp, delh, pp, amount, scale, emtemp, vwave, vem, emis, pw;
+  std::vector<int> idgas, isogas, nlayin, imod, layinc, isw;
+
+  explicit OwnedRequest(const radg_request &s) {
+    q = s;
+    const size_t L = s.nlayer, G = s.ngas, P = s.npath;
+    press.assign(s.press, s.press + L);
+    temp.assign(s.temp, s.temp + L);
+    delh.assign(s.delh, s.delh + L);
+    pp.assign(s.pp, s.pp + L * G);
+    amount.assign(s.amount, s.amount + L * G);
+    idgas.assign(s.idgas, s.idgas + G);
+    isogas.assign(s.isogas, s.isogas + G);
+    nlayin.assign(s.nlayin, s.nlayin + P);
+    imod.assign(s.imod, s.imod + P);
+    layinc.assign(s.layinc, s.layinc + P * L);
+    scale.assign(s.scale, s.scale + P * L);
+    emtemp.assign(s.emtemp, s.emtemp + P * L);
+    isw.assign(s.isw, s.isw + s.nsw);
+    vwave.assign(s.vwave, s.vwave + s.nwave);
+    vem.assign(s.vem, s.vem + s.nem);
+    emis.assign(s.emissivity, s.emissivity + s.nem);
+    if (s.path_weight) pw.assign(s.path_weight, s.path_weight + P);
+    else pw.assign(P, 0.0);
+    q.press = press.data();
+    q.temp = temp.data();
+    q.delh = delh.data();
+    q.pp = pp.data();
+    q.amount = amount.data();
+    q.idgas = idgas.data();
+    q.isogas = isogas.data();
+    q.nlayin = nlayin.data();
+    q.imod = imod.data();
+    q.layinc = layinc.data();
+    q.scale = scale.data();
+    q.emtemp = emtemp.data();
+    q.isw = isw.data();
+    q.vwave = vwave.data();
+    q.vem = vem.data();
+    q.emissivity = emis.data();
+    q.path_weight = pw.data();
+  }
+};
+
+// Uninitialised result buffer (results are written over it in full).
+struct Buf {
+  std::unique_ptr<double[]> p;
+  size_t n = 0;
+  void reset(size_t m) {
+    p.reset(new double[m]);
+    n = m;
+  }
+};
+
+struct Ticket {
+  long long id = 0;
+  std::unique_ptr<OwnedRequest> req;
+  size_t bytes = 0;
+  int status = -1;  // -1 pending, else a RADG status
+  bool cancelled = false;
+  std::string err;
+  Buf out, gts, jac, wout, wgts, wjac;
+  clk::time_point t_submit, t_admit, t_done;
+};
+
+struct Inflight {
+  radg_batch *b = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<long long> ids;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct radg_server {
+  int mode = RADG_MODE_MIXED;
+  size_t budget = 0;
+  int max_batch = 64;
+  int max_inflight = 2;
+  int fetch = 1;  // bit 1: also keep the per-path outputs
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<long long> queue;
+  std::map<long long, std::shared_ptr<Ticket>> tickets;
+  std::vector<Inflight> slots;
+  std::atomic<bool> running{true};
+  std::thread worker;
+  long long next_id = 1;
+  size_t inflight_bytes = 0;
+  long long steps = 0, admitted = 0, completed = 0, max_batch_seen = 0, peak_bytes = 0;
+  // worker time (microseconds): packing + upload + launch, retiring, kernels
+  long long us_create = 0, us_retire = 0, us_gpu = 0;
+  int device = 0;
+};
+
+namespace {
+
+void retire(radg_server *s, Inflight &f) {
+  const auto t0 = clk::now();
+  float ms3[3] = {0, 0, 0};
+  if (radg_batch_timings(f.b, ms3) == RADG_OK) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->us_gpu += (long long)(1e3 * (ms3[0] + ms3[1] + ms3[2]));
+  }
+  // results for every request of a finished batch
+  for (size_t r = 0; r < f.ids.size(); r++) {
+    std::shared_ptr<Ticket> t;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->tickets.find(f.ids[r]);
+      if (it == s->tickets.end()) continue;
+      t = it->second;
+    }
+    const radg_request &q = t->req->q;
+    const size_t nw = q.nwave, P = q.npath, L = q.nlayer, np = q.ngas + 1;
+    double *po = nullptr, *pg = nullptr, *pj = nullptr;
+    if (s->fetch & 2) {
+      t->out.reset(nw * P);
+      t->gts.reset(nw * P);
+      t->jac.reset(nw * P * L * np);
+      po = t->out.p.get();
+      pg = t->gts.p.get();
+      pj = t->jac.p.get();
+    }
+    t->wout.reset(nw);
+    t->wgts.reset(nw);
+    t->wjac.reset(nw * L * np);
+    const int rc = radg_batch_read(f.b, (int)r, po, pg, pj, t->wout.p.get(), t->wgts.p.get(),
+                                   t->wjac.p.get());
+    std::lock_guard<std::mutex> lk(s->mu);
+    t->status = rc;
+    if (rc) t->err = radg_last_error();
+    t->t_done = clk::now();
+    t->req.reset();  // inputs no longer needed
+    s->completed++;
+  }
+  radg_batch_destroy(f.b);
+  f.b = nullptr;
+  f.ids.clear();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->inflight_bytes -= f.bytes;
+    s->us_retire += std::chrono::duration_cast<std::chrono::microseconds>(clk::now() - t0).count();
+  }
+  f.bytes = 0;
+  s->cv_done.notify_all();
+}
+
+void worker_loop(radg_server *s) {
+  (void)hipSetDevice(s->device);
+  while (s->running.load()) {
+    bool progressed = false;
+    // retire finished batches
+    for (auto &f : s->slots) {
+      if (f.b && hipStreamQuery(f.stream) == hipSuccess) {
+        retire(s, f);
+        progressed = true;
+      }
+    }
+    // admit queued requests into a free slot
+    Inflight *slot = nullptr;
+    for (auto &f : s->slots)
+      if (!f.b) {
+        slot = &f;
+        break;
+      }
+    if (slot) {
+      std::vector<std::shared_ptr<Ticket>> take;
+      size_t bytes = 0;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        while (!s->queue.empty() && (int)take.size() < s->max_batch) {
+          auto it = s->tickets.find(s->queue.front());
+          if (it == s->tickets.end() || it->second->cancelled) {
+            s->queue.pop_front();
+            continue;
+          }
+          const size_t need = it->second->bytes;
+          if (s->budget && s->inflight_bytes + bytes + need > s->budget) break;
+          take.push_back(it->second);
+          bytes += need;
+          s->queue.pop_front();
+        }
+        s->inflight_bytes += bytes;
+        s->peak_bytes = std::max<long long>(s->peak_bytes, (long long)s->inflight_bytes);
+      }
+      if (!take.empty()) {
+        std::vector<radg_request> qs;
+        for (auto &t : take) qs.push_back(t->req->q);
+        radg_batch *b = nullptr;
+        const auto tc0 = clk::now();
+        int rc = radg_batch_create((int)qs.size(), qs.data(), s->mode, 0, &b);
+        if (rc == RADG_OK) rc = radg_batch_set_fetch(b, 1 | (s->fetch & 2));
+        if (rc == RADG_OK) rc = radg_batch_launch(b, slot->stream);
+        const auto now = clk::now();
+        if (rc != RADG_OK) {
+          const std::string msg = radg_last_error();
+          if (b) radg_batch_destroy(b);
+          std::lock_guard<std::mutex> lk(s->mu);
+          for (auto &t : take) {
+            t->status = rc;
+            t->err = msg;
+            t->t_done = now;
+          }
+          s->inflight_bytes -= bytes;
+          s->cv_done.notify_all();
+        } else {
+          slot->b = b;
+          slot->bytes = bytes;
+          for (auto &t : take) {
+            slot->ids.push_back(t->id);
+            t->t_admit = now;
+          }
+          std::lock_guard<std::mutex> lk(s->mu);
+          s->us_create += std::chrono::duration_cast<std::chrono::microseconds>(now - tc0).count();
+          s->steps++;
+          s->admitted += (long long)take.size();
+          s->max_batch_seen = std::max<long long>(s->max_batch_seen, (long long)take.size());
+        }
+        progressed = true;
+      }
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> lk(s->mu);
+      bool busy = false;
+      for (auto &f : s->slots) busy |= (f.b != nullptr);
+      if (busy || !s->queue.empty())
+        s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+      else
+        s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+    }
+  }
+  for (auto &f : s->slots)
+    if (f.b) {
+      (void)hipStreamSynchronize(f.stream);
+      retire(s, f);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int radg_server_create(int mode, size_t budget_bytes, int max_batch, int max_inflight,
+                       int per_path_outputs, radg_server **out) {
+  *out = nullptr;
+  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED) return RADG_EINPUT;
+  if (max_batch < 1 || max_inflight < 1 || max_inflight > 16) return RADG_EINPUT;
+  auto *s = new radg_server;
+  s->mode = mode;
+  s->fetch = per_path_outputs ? 3 : 1;
+  s->budget = budget_bytes;
+  s->max_batch = max_batch;
+  s->max_inflight = max_inflight;
+  (void)hipGetDevice(&s->device);
+  s->slots.resize(max_inflight);
+  for (auto &f : s->slots) {
+    if (hipStreamCreateWithFlags(&f.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete s;
+      return RADG_EHIP;
+    }
+  }
+  s->worker = std::thread(worker_loop, s);
+  *out = s;
+  return RADG_OK;
+}
+
+int radg_server_submit(radg_server *s, const radg_request *req, long long *ticket) {
+  if (!s || !req || !ticket) return RADG_EINPUT;
+  auto t = std::make_shared<Ticket>();
+  t->req.reset(new OwnedRequest(*req));
+  t->t_submit = clk::now();
+  size_t bytes = 0;
+  const int rc = radg_batch_bytes(1, &t->req->q, s->mode, &bytes);
+  std::lock_guard<std::mutex> lk(s->mu);
+  t->id = s->next_id++;
+  *ticket = t->id;
+  s->tickets[t->id] = t;
+  if (rc != RADG_OK) {
+    t->status = rc;
+    t->err = radg_last_error();
+    t->t_done = t->t_submit;
+    return rc;
+  }
+  if (s->budget && bytes > s->budget) {
+    t->status = RADG_EBUDGET;
+    t->err = "request needs more device memory than the whole budget";
+    t->t_done = t->t_submit;
+    return RADG_EBUDGET;
+  }
+  t->bytes = bytes;
+  s->queue.push_back(t->id);
+  s->cv_work.notify_one();
+  return RADG_OK;
+}
+
+int radg_server_cancel(radg_server *s, long long ticket) {
+  if (!s) return RADG_EINPUT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->tickets.find(ticket);
+  if (it == s->tickets.end()) return RADG_EINPUT;
+  bool queued = false;
+  for (long long id : s->queue) queued |= (id == ticket);
+  if (queued && it->second->status == -1) {
+    it->second->cancelled = true;
+    it->second->status = RADG_ECANCELLED;
+    it->second->err = "cancelled before admission";
+    it->second->t_done = clk::now();
+    s->cv_done.notify_all();
+  }
+  return RADG_OK;
+}
+
+int radg_server_wait(radg_server *s, long long ticket, double timeout_s, double *output,
+                     double *gtsurf, double *doutputdq, double *wout, double *wgts, double *wjac,
+                     int *status, double *latency_s) {
+  if (!s) return RADG_EINPUT;
+  std::shared_ptr<Ticket> t;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return RADG_EINPUT;
+    t = it->second;
+    const auto deadline = clk::now() + std::chrono::duration_cast<clk::duration>(
+                                           std::chrono::duration<double>(timeout_s));
+    while (t->status == -1) {
+      if (timeout_s >= 0 && clk::now() >= deadline) return RADG_ETIMEOUT;
+      s->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+    }
+    s->tickets.erase(it);
+  }
+  if (status) *status = t->status;
+  if (latency_s) *latency_s = std::chrono::duration<double>(t->t_done - t->t_submit).count();
+  if (t->status != RADG_OK) return RADG_OK;
+  auto cp = [](double *dst, const Buf &v) {
+    if (dst && v.n) std::memcpy(dst, v.p.get(), v.n * sizeof(double));
+  };
+  cp(output, t->out);
+  cp(gtsurf, t->gts);
+  cp(doutputdq, t->jac);
+  cp(wout, t->wout);
+  cp(wgts, t->wgts);
+  cp(wjac, t->wjac);
+  return RADG_OK;
+}
+
+int radg_server_stats(radg_server *s, long long *stats9) {
+  if (!s || !stats9) return RADG_EINPUT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  stats9[0] = s->steps;
+  stats9[1] = s->admitted;
+  stats9[2] = s->completed;
+  stats9[3] = s->max_batch_seen;
+  stats9[4] = s->peak_bytes;
+  stats9[5] = (long long)s->queue.size();
+  stats9[6] = s->us_create;
+  stats9[7] = s->us_retire;
+  stats9[8] = s->us_gpu;
+  return RADG_OK;
+}
+
+int radg_server_destroy(radg_server *s) {
+  if (!s) return RADG_OK;
+  s->running = false;
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (auto &f : s->slots)
+    if (f.stream) (void)hipStreamDestroy(f.stream);
+  delete s;
+  return RADG_OK;
+}
+
+}  // extern "C"

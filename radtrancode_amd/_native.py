@@ -17,6 +17,7 @@ RADG_ECANCELLED = 8
 RADG_ETIMEOUT = 9
 MODE_FP64 = 0
 MODE_MIXED = 1
+MODE_BF16 = 2
 
 # Every symbol the C header (include/radg.h) declares.
 EXPORTS = [
@@ -26,7 +27,7 @@ EXPORTS = [
     "radg_batch_launch", "radg_batch_sync", "radg_batch_read", "radg_batch_timings",
     "radg_batch_info", "radg_batch_destroy", "radg_batch_timing_stats",
     "radg_server_create", "radg_server_submit", "radg_server_cancel", "radg_server_wait",
-    "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch",
+    "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch", "radg_ktable_bf16",
 ]
 
 
@@ -70,8 +71,9 @@ def lib():
                  "radg_batch_timings", "radg_batch_info", "radg_batch_destroy",
                  "radg_batch_timing_stats", "radg_server_create", "radg_server_submit",
                  "radg_server_cancel", "radg_server_wait", "radg_server_stats",
-                 "radg_server_destroy"]:
+                 "radg_server_destroy", "radg_ktable_bf16"]:
         getattr(L, name).restype = ip
+    L.radg_ktable_bf16.argtypes = [ip, vp, vp]
     L.radg_register_cia.argtypes = [C.c_double, vp, vp]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
     L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]

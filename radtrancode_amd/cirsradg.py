@@ -20,7 +20,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._native import MODE_FP64, MODE_MIXED, RadgError, Request, check, lib  # noqa: F401
+from ._native import MODE_BF16, MODE_FP64, MODE_MIXED, RadgError, Request, check, lib  # noqa: F401
 
 MAXLAY, MAXPAT, MAXGAS, MAXCON, MAXSEC = 400, 200, 30, 10, 16000
 NPARMAX = MAXGAS + 2 + MAXCON
@@ -60,6 +60,15 @@ def register_ktables(tables) -> None:
         C.c_int(n), _vp(ids), _vp(isos), _vp(npts), _vp(vmin), _vp(delv),
         C.c_int(t0.np_), C.c_int(t0.nt), C.c_int(t0.ng), _vp(press), _vp(temp),
         _vp(gord), _vp(delg), ptrs))
+
+
+def ktable_bf16(itab: int, shape):
+    """BF16 codes of registered table ``itab`` as made on the device, with
+    the table's (offset, scale): log10(k) = code * scale + offset."""
+    codes = np.empty(shape, dtype=np.uint16)
+    off_sc = np.zeros(2, dtype=np.float64)
+    check(lib().radg_ktable_bf16(C.c_int(itab), _vp(codes), _vp(off_sc)))
+    return codes, float(off_sc[0]), float(off_sc[1])
 
 
 def read_klist(path: str, recl: int = 4) -> None:

@@ -23,7 +23,10 @@ size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64);
 size_t radg_lds_emission_bytes(int RL, int nsw, int fp64);
 int radg_layer_E(int ng);
 int radg_emission_RL(int maxlays);
-hipError_t radg_launch_layer_lds(const KernelArgs &a, int fp64, int E, size_t lds, hipStream_t st);
+hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t lds, hipStream_t st);
+hipError_t radg_convert_f32(const double *src, float *dst, long long n, hipStream_t st);
+hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n, double off,
+                             double sc, hipStream_t st);
 hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, size_t lds, hipStream_t st);
 hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st);
 
@@ -64,6 +67,7 @@ struct Table {
   double vmin, delv;
   long long off;              // element offset in the pool
   std::vector<uint8_t> zero;  // per slice: every value <= 0 (get_kg.f:224-250)
+  double bf_off = 0, bf_sc = 1;  // BF16 form: log10(k) = code * bf_sc + bf_off
 };
 
 struct TableSet {
@@ -72,12 +76,15 @@ struct TableSet {
   std::vector<Table> tabs;
   long long total = 0;
   double *d_k64 = nullptr;
-  float *d_k32 = nullptr;
+  float *d_k32 = nullptr;            // made on first use by the mixed form
+  unsigned short *d_k16 = nullptr;   // made on first use by the BF16 form
+  std::mutex form_mu;
   double *d_G = nullptr, *d_delg64 = nullptr;
   float *d_delg32 = nullptr;
   ~TableSet() {
     if (d_k64) (void)hipFree(d_k64);
     if (d_k32) (void)hipFree(d_k32);
+    if (d_k16) (void)hipFree(d_k16);
     if (d_G) (void)hipFree(d_G);
     if (d_delg64) (void)hipFree(d_delg64);
     if (d_delg32) (void)hipFree(d_delg32);
@@ -95,6 +102,45 @@ struct CIA {
 
 std::shared_ptr<TableSet> g_tables;
 std::shared_ptr<CIA> g_cia;
+
+bool valid_mode(int mode) {
+  return mode == RADG_MODE_FP64 || mode == RADG_MODE_MIXED || mode == RADG_MODE_BF16;
+}
+
+// Make the table form a mode reads (FP32 copy or BF16 log10 codes) from the
+// FP64 master on the device, once per registered table set.
+int ensure_form(TableSet &ts, int mode) {
+  if (mode == RADG_MODE_FP64) return RADG_OK;
+  std::lock_guard<std::mutex> lk(ts.form_mu);
+  if (mode == RADG_MODE_MIXED && !ts.d_k32) {
+    float *d = nullptr;
+    HIPCHK(hipMalloc(&d, ts.total * sizeof(float)));
+    hipError_t e = radg_convert_f32(ts.d_k64, d, ts.total, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(RADG_EHIP, "FP32 table conversion: %s", hipGetErrorString(e));
+    }
+    ts.d_k32 = d;
+  }
+  if (mode == RADG_MODE_BF16 && !ts.d_k16) {
+    unsigned short *d = nullptr;
+    HIPCHK(hipMalloc(&d, ts.total * sizeof(unsigned short)));
+    hipError_t e = hipSuccess;
+    for (size_t t = 0; t < ts.tabs.size() && e == hipSuccess; t++) {
+      const Table &tb = ts.tabs[t];
+      const long long n = (long long)tb.npoint * ts.np * ts.nt * ts.ng;
+      e = radg_convert_bf16(ts.d_k64 + tb.off, d + tb.off, n, tb.bf_off, tb.bf_sc, nullptr);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(RADG_EHIP, "BF16 table conversion: %s", hipGetErrorString(e));
+    }
+    ts.d_k16 = d;
+  }
+  return RADG_OK;
+}
 
 // frecipes/locate.f: 1-based JL in 0..n
 int locate(const double *xx, int n, double x) {
@@ -272,7 +318,8 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   P.esurf.resize(nwave);
   P.wg.resize((size_t)nwave * ngas);
   const long long ntab = (long long)np * nt * NG;
-  const size_t kbytes = (mode == RADG_MODE_FP64) ? 8 : 4;
+  const size_t kbytes = (mode == RADG_MODE_FP64) ? 8 : (mode == RADG_MODE_BF16) ? 2 : 4;
+  const size_t obytes_el = (mode == RADG_MODE_FP64) ? 8 : 4;
   double tbytes = 0;
   for (int iw = 0; iw < nwave; iw++) {
     const double v = q.vwave[iw];
@@ -293,6 +340,9 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
       WaveGas &w = P.wg[(size_t)iw * ngas + k];
       w.base = -1;
       w.coinc = 1;
+      w.sc = 1.0f;
+      w.off = 0.0f;
+      w.pad = 0;
       const Table *tb = nullptr;
       for (const Table &t : ts.tabs) {  // read_klist.f:260-290: first covering match
         // the grid ends are widened by get_kg's coincidence tolerance so that
@@ -315,11 +365,13 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
                     "is provided by the CPU oracle only)", v, q.idgas[k]);
       if (tb->zero[n1]) continue;  // all-zero slice: k = 0
       w.base = tb->off + n1 * ntab;
+      w.sc = (float)tb->bf_sc;
+      w.off = (float)tb->bf_off;
       tbytes += (double)nodes.size() * NG * kbytes;
     }
   }
   // algorithmic bytes: table nodes + layer inputs + CIA nodes + outputs
-  const double obytes = (double)nwave * ((double)npath * (2.0 + (double)L * nparam) * kbytes +
+  const double obytes = (double)nwave * ((double)npath * (2.0 + (double)L * nparam) * obytes_el +
                                          (2.0 + (double)L * nparam) * 8.0);
   const double ibytes = (double)L * (4 + 2 * ngas) * 8.0 + (double)nwave * 4 * 9 * 8.0;
   P.alg_bytes[0] = tbytes;
@@ -579,7 +631,7 @@ void put(std::vector<char> &h, size_t off, const X *src, size_t n) {
 extern "C" {
 
 int radg_set_mode(int mode) {
-  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED) return fail(RADG_EINPUT, "mode=%d", mode);
+  if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   g_mode = mode;
   return RADG_OK;
 }
@@ -633,28 +685,35 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
     tb.delv = delv[t];
     tb.off = total;
     tb.zero.assign(npoint[t], 1);
+    double lmin = 1e300, lmax = 0;
     for (long long s = 0; s < npoint[t]; s++) {
       const double *sl = k[t] + s * per;
       for (long long i = 0; i < per; i++) {
         if (!std::isfinite(sl[i]) || std::fabs(sl[i]) > 1e30)
           return fail(RADG_EINPUT, "table %d: value %g at wavenumber %lld is not finite or > 1e30",
                       t + 1, sl[i], s + 1);
-        if (sl[i] > 0.0) tb.zero[s] = 0;
+        if (sl[i] > 0.0) {
+          tb.zero[s] = 0;
+          lmin = std::min(lmin, sl[i]);
+          lmax = std::max(lmax, sl[i]);
+        }
       }
+    }
+    // BF16 form: codes span [-1, 1] over the table's positive range, so the
+    // rounding error of log10(k) is at most |log10(k) - bf_off| * 2^-8
+    if (lmax > 0) {
+      const double a = std::log10(lmin), z = std::log10(lmax);
+      tb.bf_off = 0.5 * (a + z);
+      tb.bf_sc = std::max(0.5 * (z - a), 1e-3);
     }
     total += (long long)npoint[t] * per;
     ts->tabs.push_back(std::move(tb));
   }
   ts->total = total;
   HIPCHK(hipMalloc(&ts->d_k64, total * sizeof(double)));
-  HIPCHK(hipMalloc(&ts->d_k32, total * sizeof(float)));
-  std::vector<float> tmp;
   for (int t = 0; t < ntab; t++) {
     const long long n = (long long)npoint[t] * per;
     HIPCHK(hipMemcpy(ts->d_k64 + ts->tabs[t].off, k[t], n * sizeof(double), hipMemcpyHostToDevice));
-    tmp.resize(n);
-    for (long long i = 0; i < n; i++) tmp[i] = (float)k[t][i];
-    HIPCHK(hipMemcpy(ts->d_k32 + ts->tabs[t].off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
   }
   std::vector<float> dg32(delg, delg + ng);
   HIPCHK(hipMalloc(&ts->d_G, (ng + 1) * sizeof(double)));
@@ -665,6 +724,24 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
   HIPCHK(hipMemcpy(ts->d_delg32, dg32.data(), ng * sizeof(float), hipMemcpyHostToDevice));
   std::lock_guard<std::mutex> lk(g_mu);
   g_tables = ts;
+  return RADG_OK;
+}
+
+int radg_ktable_bf16(int itab, unsigned short *codes, double *off_sc) {
+  std::shared_ptr<TableSet> ts;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ts = g_tables;
+  }
+  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
+  if (itab < 0 || itab >= (int)ts->tabs.size()) return fail(RADG_EINPUT, "itab=%d", itab);
+  int rc = ensure_form(*ts, RADG_MODE_BF16);
+  if (rc) return rc;
+  const Table &tb = ts->tabs[itab];
+  const long long n = (long long)tb.npoint * ts->np * ts->nt * ts->ng;
+  HIPCHK(hipMemcpy(codes, ts->d_k16 + tb.off, n * sizeof(unsigned short), hipMemcpyDeviceToHost));
+  off_sc[0] = tb.bf_off;
+  off_sc[1] = tb.bf_sc;
   return RADG_OK;
 }
 
@@ -700,7 +777,7 @@ int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes
 
 int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, radg_batch **out) {
   *out = nullptr;
-  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED) return fail(RADG_EINPUT, "mode=%d", mode);
+  if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   std::shared_ptr<TableSet> ts;
   std::shared_ptr<CIA> cia;
   {
@@ -712,6 +789,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
   Plan pl;
   int rc = plan_batch(nreq, reqs, mode, *ts, pl);
+  if (rc) return rc;
+  rc = ensure_form(*ts, mode);
   if (rc) return rc;
   if (budget && pl.total > budget)
     return fail(RADG_EBUDGET, "batch needs %zu bytes, budget %zu", pl.total, budget);
@@ -786,7 +865,9 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.nB = pl.nB;
   a.nC = pl.nC;
   a.ng = ts->ng;
-  a.ktab = (mode == RADG_MODE_FP64) ? (const void *)ts->d_k64 : (const void *)ts->d_k32;
+  a.ktab = (mode == RADG_MODE_FP64)   ? (const void *)ts->d_k64
+           : (mode == RADG_MODE_BF16) ? (const void *)ts->d_k16
+                                      : (const void *)ts->d_k32;
   a.np = ts->np;
   a.nt = ts->nt;
   a.G = ts->d_G;
@@ -860,7 +941,7 @@ int radg_batch_launch(radg_batch *b, void *stream) {
     if (!b->ev[i]) HIPCHK(hipEventCreate(&b->ev[i]));
   b->launches++;
   HIPCHK(hipEventRecord(b->ev[0], st));
-  HIPCHK(radg_launch_layer_lds(b->args, fp64, b->E, b->ldsA, st));
+  HIPCHK(radg_launch_layer_lds(b->args, b->mode, b->E, b->ldsA, st));
   HIPCHK(hipEventRecord(b->ev[1], st));
   for (const auto &ec : b->em) {
     KernelArgs a = b->args;

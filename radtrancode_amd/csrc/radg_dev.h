@@ -47,8 +47,12 @@ struct LayerConst {
 struct WaveGas {
   long long base;        // element offset of the slice in the table pool; -1: k = 0
   int coinc;             // 1: the wavenumber is on the table grid (get_kg.f:253-264)
+  float sc, off;         // BF16 storage: log10(k) = code * sc + off
   int pad;
 };
+
+// BF16 code of k <= 0 (bf16 -infinity)
+#define RADG_BF16_NONPOS 0xFF80u
 
 struct PathInfo {
   int nlays, imod, surf, pad;

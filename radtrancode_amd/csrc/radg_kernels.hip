@@ -330,12 +330,12 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
 
 // ============================================================ layer kernel
 // LDS layout (dynamic), sized by radg_lds_layer_bytes():
-//   G[NG+1] | recS[NG][NQ] | recC[NG][NQ] | recG[NG] | recW[NG]      (FP64)
+//   G[NG+1] | recS[NG][NQ] | recC[NG][NQ] | recG[NG] | recW[NG] | dl64[NG] (FP64)
 //   kgn[ngas][NG] | dkgn[ngas][NG] | bufA[NS][NG] | bufB[NS][NG] | dlg[NG]  (T)
 //   rtype[NG]                                                      (int)
 // with NS = ngas + 2 row slots (K, gas 0..ngas-1, T) and NQ = ngas + 2 rebinned
 // quantities (key, gas slots, T).
-template <typename T, int E>
+template <typename T, int E, int BF>
 __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
   if (item >= a.nA) return;
@@ -358,7 +358,8 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
   double *recC = recS + NG * NQ;
   double *recG = recC + NG * NQ;
   double *recW = recG + NG;
-  T *kgn = reinterpret_cast<T *>(recW + NG);
+  double *dl64 = recW + NG;
+  T *kgn = reinterpret_cast<T *>(dl64 + NG);
   T *dkgn = kgn + ngas * NG;
   T *bufA = dkgn + ngas * NG;
   T *bufB = bufA + NS * NG;
@@ -370,6 +371,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
   for (int i = lane; i <= NG; i += W) G[i] = a.G[i];
   for (int i = lane; i < NG; i += W) {
     dlg[i] = delg[i];
+    dl64[i] = a.delg64[i];
     rtype[i] = 0;
   }
 
@@ -387,14 +389,36 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
       if (lane < NG) {
         T kk = 0, dk = 0;
         if (wgk.base >= 0) {
-          const T *sl = tab + wgk.base;
-          const bool klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
-          T y1 = sl[i1 + lane], y2 = sl[i2 + lane], y3 = sl[i3 + lane], y4 = sl[i4 + lane];
-          if (klog) {
-            y1 = dlog(y1);
-            y2 = dlog(y2);
-            y3 = dlog(y3);
-            y4 = dlog(y4);
+          bool klog;
+          T y1, y2, y3, y4;
+          if (BF) {
+            // BF16 log10(k) codes: ln k comes straight from the code
+            const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + wgk.base;
+            klog = !(sl[i1] == RADG_BF16_NONPOS || sl[i2] == RADG_BF16_NONPOS ||
+                     sl[i3] == RADG_BF16_NONPOS || sl[i4] == RADG_BF16_NONPOS);
+            const float ln10 = 2.302585092994046f;
+            auto dec = [&](unsigned short c) -> float {
+              if (c == RADG_BF16_NONPOS) return klog ? -__builtin_huge_valf() : 0.0f;
+              const float l = (__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10;
+              return klog ? l : expf(l);
+            };
+            y1 = (T)dec(sl[i1 + lane]);
+            y2 = (T)dec(sl[i2 + lane]);
+            y3 = (T)dec(sl[i3 + lane]);
+            y4 = (T)dec(sl[i4 + lane]);
+          } else {
+            const T *sl = tab + wgk.base;
+            klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
+            y1 = sl[i1 + lane];
+            y2 = sl[i2 + lane];
+            y3 = sl[i3 + lane];
+            y4 = sl[i4 + lane];
+            if (klog) {
+              y1 = dlog(y1);
+              y2 = dlog(y2);
+              y3 = dlog(y3);
+              y4 = dlog(y4);
+            }
           }
           const T x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 +
                       ((T)1 - v) * u * y4;
@@ -482,15 +506,19 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
       }
       Sorter<T, E>::sort(key, idx, lane);
 
-      // weights and their FP64 cumulative sum gdist (rankg.f:104-108)
+      // weights and their FP64 cumulative sum gdist (rankg.f:104-108).  The
+      // sum is taken over FP64 products: where a run of elements exactly
+      // fills a bin, FP32 products would spill a ~1e-8 share of the next
+      // (possibly far larger) key across the edge.
       T wq[E];
       double gd[E];
       double run = 0.0;
 #pragma unroll
       for (int q = 0; q < E; q++) {
         const int id = idx[q];
-        wq[q] = (id < 1024) ? dlg[id >> 5] * dlg[id & 31] : (T)0;
-        run += (double)wq[q];
+        const bool real = id < 1024;
+        wq[q] = real ? dlg[id >> 5] * dlg[id & 31] : (T)0;
+        run += real ? dl64[id >> 5] * dl64[id & 31] : 0.0;
         gd[q] = run;
       }
       double s = run;
@@ -746,7 +774,8 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
 
   int Lm[RL];
   bool act[RL];
-  T sc[RL], tc[RL], bb[RL], db[RL];
+  T sc[RL], tc[RL], db[RL];
+  double bb[RL];  // Planck source of each layer, FP64
   const double xfac = 1.0;
 #pragma unroll
   for (int q = 0; q < RL; q++) {
@@ -759,7 +788,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     db[q] = 0;
     if (act[q] && pi.imod == 3) {
       const double em = a.emtemp[R.pl0 + (long long)p * L + m];
-      bb[q] = (T)planck_d(R.ispace, v, em);
+      bb[q] = planck_d(R.ispace, v, em);
       db[q] = (T)planckg_d(R.ispace, v, em);
     }
     for (int k1 = 0; k1 < nsw; k1++) {
@@ -776,15 +805,15 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
 #pragma unroll
       for (int q = 0; q < RL; q++)
         if (q == oq) {
-          b0 = (double)bb[q];
+          b0 = bb[q];
           g0 = (double)db[q];
         }
       radground = __shfl(b0, ob);
       gradground = __shfl(g0, ob);
     } else {
       const double es = a.esurf[R.wave0 + iw];
-      radground = es * (double)(T)planck_d(R.ispace, v, R.tsurf);
-      gradground = es * (double)(T)planckg_d(R.ispace, v, R.tsurf);
+      radground = es * planck_d(R.ispace, v, R.tsurf);
+      gradground = es * planckg_d(R.ispace, v, R.tsurf);
     }
   }
 
@@ -828,7 +857,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       for (int q = 0; q < RL; q++) {
         trp[q] = (q == 0) ? ex : ex * incl[q - 1];
         trx[q] = ex * incl[q];
-        if (act[q]) part += trp[q] * (double)om[q] * (double)bb[q];
+        if (act[q]) part += trp[q] * (double)om[q] * bb[q];
       }
       double rg = xfac * wave_sum(part);
       if (pi.surf) {
@@ -836,42 +865,61 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
         coG = (T)(xfac * trN * gradground);
       }
       coR = (T)rg;
-      // W_m: radiance entering layer m from below.  Suffix scan of the maps
-      // x -> B_m (1 - t_m) + t_m x over the deeper layers.
-      T ca = 0, cb = 1;  // lane composite, applied to the value from below
+      // dR/dtau_m = tr_m (B_m - W_m), W_m the radiance entering layer m from
+      // below.  D_m = B_m - W_m is carried by its own recurrence
+      //   D_{m-1} = (B_{m-1} - B_m) + t_m D_m,   D_{N-1} = B_{N-1} - R_ground
+      // (W_{m-1} = B_m (1 - t_m) + t_m W_m), so it keeps its relative accuracy
+      // where B and W nearly cancel.  Affine suffix scan over lanes, FP64.
+      const double bup0 = __shfl_up(bb[RL - 1], 1);  // layer above this lane's first
+      double dN;
+      {
+        const int ob = (nlays - 1) / RL, oq = (nlays - 1) - ob * RL;
+        double b0 = 0.0;
+#pragma unroll
+        for (int q = 0; q < RL; q++)
+          if (q == oq) b0 = bb[q];
+        dN = __shfl(b0, ob) - radground;
+      }
+      double cm[RL];
+#pragma unroll
+      for (int q = 0; q < RL; q++) {
+        const double bu = (q == 0) ? bup0 : bb[q - 1];
+        cm[q] = (act[q] && lane * RL + q > 0) ? bu - bb[q] : 0.0;
+      }
+      double ca = 0.0, cb = 1.0;  // lane composite, applied to D from below
 #pragma unroll
       for (int q = RL - 1; q >= 0; q--) {
         if (act[q]) {
-          const T am = bb[q] * om[q], bm = tl[q];
-          ca = am + bm * ca;
-          cb = bm * cb;
+          const double tm = (double)tl[q];
+          ca = cm[q] + tm * ca;
+          cb = tm * cb;
         }
       }
-      T ia = ca, ib = cb;  // inclusive suffix composite
+      double ia = ca, ib = cb;  // inclusive suffix composite
 #pragma unroll
       for (int ds = 0; ds < 6; ds++) {
         const int d = 1 << ds;
-        const T oa = __shfl_down(ia, d), ob = __shfl_down(ib, d);
+        const double oa = __shfl_down(ia, d), ob = __shfl_down(ib, d);
         if (lane + d < W) {
           ia = ia + ib * oa;
           ib = ib * ob;
         }
       }
-      T xa = __shfl_down(ia, 1), xb = __shfl_down(ib, 1);
+      double xa = __shfl_down(ia, 1), xb = __shfl_down(ib, 1);
       if (lane == W - 1) {
-        xa = 0;
-        xb = 1;
+        xa = 0.0;
+        xb = 1.0;
       }
-      T wv = xa + xb * (T)radground;  // W below this lane's last layer
-      T Wm[RL];
+      double dv = xa + xb * dN;  // D of this lane's deepest active layer
+      double Dm[RL];
 #pragma unroll
       for (int q = RL - 1; q >= 0; q--) {
-        Wm[q] = wv;
-        if (act[q]) wv = bb[q] * om[q] + tl[q] * wv;
+        Dm[q] = dv;
+        if (act[q]) dv = cm[q] + (double)tl[q] * dv;
       }
 #pragma unroll
       for (int q = 0; q < RL; q++) {
-        dRdtau[q] = act[q] ? (T)(xfac * trx[q] * (double)(bb[q] - Wm[q])) : (T)0;
+        dRdtau[q] = act[q] ? (T)(xfac * trx[q] * Dm[q]) : (T)0;
         dT[q] = act[q] ? (T)(xfac * trp[q] * (double)om[q]) * db[q] : (T)0;
       }
     } else {
@@ -972,11 +1020,11 @@ __global__ __launch_bounds__(64) void quadrature_kernel(KernelArgs a) {
 
 // ============================================================ launchers
 namespace {
-template <typename T, int E>
+template <typename T, int E, int BF = 0>
 hipError_t launch_layer_E(const KernelArgs &a, size_t lds, hipStream_t st) {
   long long nb = ((a.nA + 7) / 8) * 8;
   if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL((layer_kernel<T, E>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
+  hipLaunchKernelGGL((layer_kernel<T, E, BF>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
   return hipGetLastError();
 }
 template <typename T, int RL>
@@ -992,7 +1040,7 @@ size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
   (void)E;
   const size_t ts = fp64 ? 8 : 4;
   const size_t nq = ngas + 2, ns = ngas + 2;
-  size_t b = 8 * ((size_t)(ng + 1) + 2 * (size_t)ng * nq + 2 * (size_t)ng);
+  size_t b = 8 * ((size_t)(ng + 1) + 2 * (size_t)ng * nq + 3 * (size_t)ng);
   b += ts * ((size_t)2 * ngas * ng + (size_t)2 * ns * ng + ng);
   b += 4 * (size_t)ng;
   return b;
@@ -1016,14 +1064,23 @@ int radg_emission_RL(int maxlays) {
   return RL;
 }
 
-hipError_t radg_launch_layer_lds(const KernelArgs &a, int fp64, int E, size_t lds, hipStream_t st) {
-  if (fp64) {
+// mode: 0 FP64, 1 mixed (FP32 table), 2 mixed with BF16 log10(k) table
+hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t lds, hipStream_t st) {
+  if (mode == 0) {
     switch (E) {
       case 1: return launch_layer_E<double, 1>(a, lds, st);
       case 2: return launch_layer_E<double, 2>(a, lds, st);
       case 4: return launch_layer_E<double, 4>(a, lds, st);
       case 8: return launch_layer_E<double, 8>(a, lds, st);
       case 16: return launch_layer_E<double, 16>(a, lds, st);
+    }
+  } else if (mode == 2) {
+    switch (E) {
+      case 1: return launch_layer_E<float, 1, 1>(a, lds, st);
+      case 2: return launch_layer_E<float, 2, 1>(a, lds, st);
+      case 4: return launch_layer_E<float, 4, 1>(a, lds, st);
+      case 8: return launch_layer_E<float, 8, 1>(a, lds, st);
+      case 16: return launch_layer_E<float, 16, 1>(a, lds, st);
     }
   } else {
     switch (E) {
@@ -1062,5 +1119,45 @@ hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st)
     hipLaunchKernelGGL((quadrature_kernel<double>), dim3((unsigned)a.nC), dim3(64), 0, st, a);
   else
     hipLaunchKernelGGL((quadrature_kernel<float>), dim3((unsigned)a.nC), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+// ======================================================= table storage
+// The FP32 and BF16 copies of a registered k-table pool are made on the
+// device from its FP64 master, once, on first use.
+namespace {
+__global__ void to_f32_kernel(const double *src, float *dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+// code = bf16((log10 k - off) / sc), round to nearest even; k <= 0 -> -inf
+__global__ void to_bf16_kernel(const double *src, unsigned short *dst, long long n, double off,
+                               double sc) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double k = src[i];
+  if (!(k > 0.0)) {
+    dst[i] = (unsigned short)RADG_BF16_NONPOS;
+    return;
+  }
+  const float x = (float)((log10(k) - off) / sc);
+  unsigned int b = __float_as_uint(x);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  dst[i] = (unsigned short)(b >> 16);
+}
+}  // namespace
+
+hipError_t radg_convert_f32(const double *src, float *dst, long long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n, double off,
+                             double sc, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src,
+                     dst, n, off, sc);
   return hipGetLastError();
 }

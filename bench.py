@@ -97,6 +97,66 @@ def cpu_baseline(tables, cia, req, seconds):
                        f"({what}) in {total_t:.1f} s")
 
 
+def weighted(out, dq, pw):
+    """Angle-weighted radiance and Jacobian from per-path results."""
+    return np.einsum("wp,p->w", out, pw), np.einsum("wplq,p->wlq", dq, pw)
+
+
+def config5_run(rad, n, cia, nthreads):
+    """The large-spectrum stress: ``n`` requests of 200 layers x 10000
+    wavenumbers (0.1 cm-1) with 8 k-gases + H2/He, k-tables on a 30 P x 15 T
+    x 20 g grid stored as BF16 log10(k), all submitted at once to the serving
+    loop (device-memory budget admission).  The BF16-storage error of request
+    0 is measured against the FP64 oracle on a sample of wavenumbers."""
+    from oracle import oracle
+    from radtrancode_amd import serving, synth
+
+    t0 = time.perf_counter()
+    tabs = synth.config5_tables()
+    rad.register_ktables(tabs)
+    t_reg = time.perf_counter() - t0
+    reqs = [synth.config5_request(seed=5000 + i) for i in range(n)]
+    budget = 96 << 30
+    srv = serving.Server(mode=rad.MODE_BF16, budget_bytes=budget, max_batch=16, max_inflight=2,
+                         per_path_outputs=False)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(r) for r in reqs]
+    lat, res0 = [], None
+    for i, t in enumerate(tickets):
+        r = srv.wait(t, outputs=(i == 0))
+        lat.append(r["latency_s"])
+        if i == 0:
+            res0 = r
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    sample = np.arange(0, reqs[0].nwave, 500)
+    o_ref, _, d_ref = oracle.spectrum(tabs, cia, reqs[0], vwave=reqs[0].vwave[sample],
+                                      nthreads=nthreads)
+    wo_ref, wj_ref = weighted(o_ref, d_ref, reqs[0].path_weight)
+    wo, wj = res0["wout"][sample], res0["wjac"][sample]
+    rad_e = float(np.max(np.abs(wo - wo_ref) / np.abs(wo_ref)))
+    sc = np.max(np.abs(wj_ref), axis=1, keepdims=True)
+    jac_e = float(np.max(np.abs(wj - wj_ref) / np.where(sc > 0, sc, 1.0)))
+    lat_ms = np.array(lat) * 1e3
+    del tabs
+    return {
+        "requests": n, "wall_s": wall, "requests_per_s": n / wall,
+        "latency_ms_p50": float(np.percentile(lat_ms, 50)),
+        "latency_ms_p99": float(np.percentile(lat_ms, 99)),
+        "batches": st["steps"], "largest_batch": st["max_batch"],
+        "peak_inflight_bytes": st["peak_bytes"], "budget_bytes": budget,
+        "worker_s": {"pack_upload_launch": st["create_s"], "retire": st["retire_s"],
+                     "kernels": st["gpu_s"]},
+        "table_setup_s": t_reg,
+        "bf16_storage_error_vs_fp64_oracle": {
+            "radiance_rel": rad_e, "jacobian_col_scaled": jac_e,
+            "sample": f"{len(sample)} wavenumbers of request 0, angle-weighted"},
+        "shape": "200 layers x 10000 wavenumbers (0.1 cm-1), 8 k-gases + H2/He, 30 P x 15 T "
+                 "x 20 g, BF16 log10(k) tables (1.44 GB), 5 angles, weighted outputs",
+    }
+
+
 def serving_run(rad, n, rate):
     """The serving mix: ``n`` requests (layers 40-200, wavenumber windows of
     200-5000 points on a shared 5000-point grid, 1-6 k-gases + H2/He)
@@ -145,6 +205,9 @@ def main():
                     help="requests in the Poisson serving run (0: skip it)")
     ap.add_argument("--serving-rate", type=float, default=60.0,
                     help="Poisson arrival rate of the serving run, requests/s")
+    ap.add_argument("--bf16-steps", type=int, default=3)
+    ap.add_argument("--config5-requests", type=int, default=64,
+                    help="requests in the large-spectrum BF16 stress (0: skip it)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -197,6 +260,17 @@ def main():
         b64.close()
         value64 = args.batch * args.fp64_steps / wall64
 
+    # ---- BF16 log10(k) table form, same batch
+    b16, wall16, lat16, kern16 = time_batch(rad, reqs, rad.MODE_BF16, args.bf16_steps, 1, torch)
+    info16 = b16.info()
+    res16 = b16.read(0, weighted=False)
+    b16.close()
+    value16 = args.batch * args.bf16_steps / wall16
+    o16, d16 = res16["output"][sample], res16["doutputdq"][sample]
+    rad_e16 = float(np.max(np.abs(o16 - o_ref) / np.abs(o_ref)))
+    jac_e16 = float(np.max(np.abs(d16 - d_ref) / np.where(sc > 0, sc, 1.0)))
+    alg16 = info16["table_bytes"] + info16["input_bytes"]
+
     # ---- roofline of the dominant kernel (layer/overlap kernel): it reads
     # the table nodes and the layer inputs of every spectrum in the batch
     t_layer_s = kern[0] / 1e3
@@ -210,6 +284,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(tables, cia, reqs[0], args.cpu_seconds)
+
+    c5 = None
+    if args.config5_requests > 0:
+        c5 = config5_run(rad, args.config5_requests, cia, os.cpu_count() or 1)
 
     serve = None
     if args.serving_requests > 0:
@@ -240,12 +318,19 @@ def main():
         "fp64_form": {"value": value64, "ms_per_step": wall64 / args.fp64_steps * 1e3,
                       "kernel_ms": {"layer": float(kern64[0]), "emission": float(kern64[1]),
                                     "quadrature": float(kern64[2])}},
+        "bf16_form": {"value": value16, "ms_per_step": wall16 / args.bf16_steps * 1e3,
+                      "kernel_ms": {"layer": float(kern16[0]), "emission": float(kern16[1]),
+                                    "quadrature": float(kern16[2])},
+                      "layer_kernel_alg_GBps": alg16 / (kern16[0] / 1e3) / 1e9,
+                      "error_vs_fp64_oracle": {"radiance_rel": rad_e16,
+                                               "jacobian_col_scaled": jac_e16}},
         "speedup_vs_fp64": value / value64,
         "error_vs_fp64_oracle": {"radiance_rel": rad_err, "jacobian_col_scaled": jac_err,
                                  "sample": f"{len(sample)} wavenumbers of spectrum 0"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": serve,
+        "config5_bf16_stress": c5,
     }
     print(json.dumps(out), flush=True)
 

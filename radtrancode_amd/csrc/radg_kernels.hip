@@ -52,6 +52,12 @@ template <> __device__ __forceinline__ double dlog<double>(double x) { return lo
 template <typename T> __device__ __forceinline__ T dexpm1(T x);
 template <> __device__ __forceinline__ float dexpm1<float>(float x) { return expm1f(x); }
 template <> __device__ __forceinline__ double dexpm1<double>(double x) { return expm1(x); }
+// a / b: IEEE division in FP64, one reciprocal (1 ulp) in FP32
+template <typename T> __device__ __forceinline__ T ratio(T a, T b);
+template <> __device__ __forceinline__ float ratio<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+template <> __device__ __forceinline__ double ratio<double>(double a, double b) { return a / b; }
 template <typename T> __device__ __forceinline__ T dinf();
 template <> __device__ __forceinline__ float dinf<float>() { return __builtin_huge_valf(); }
 template <> __device__ __forceinline__ double dinf<double>() { return __builtin_huge_val(); }
@@ -347,7 +353,6 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
   const int iw = (int)(loc / L);
   const int j = (int)(loc - (long long)iw * L);
   const LayerConst lc = a.lay[R.lay0 + j];
-  const double vwave = a.vwave[R.wave0 + iw];
   const int n = NG * NG;
   const int NS = ngas + 2;
   const int NQ = ngas + 2;
@@ -557,7 +562,8 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
             while (b < NG && G[b + 1] <= p0[q]) b++;
           const bool stq = (b < NG) && !(gd[q] < G[b + 1]);
           bq[q] = b | (stq ? 256 : 0);
-          omf[q] = stq ? (T)(1.0 - (G[b + 1] - p0[q]) / (gd[q] - p0[q])) : (T)0;
+          // 1 - frac of rankg.f:121, formed as a ratio of two FP64 differences
+          omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - p0[q])) : (T)0;
         }
       }
       // bin records: a straddling element closes its bin; the last slot
@@ -672,27 +678,44 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas);
   T *ktot = inter;                                  // [NG][L]
   T *dkq = inter + (long long)NG * L;               // [NG][ngas+1][L]
-  T *tcon = dkq + (long long)NG * (ngas + 1) * L;   // [L]
-  T *dtcon = tcon + L;                              // [ngas+1][L]
   for (int i = lane; i < NG; i += W) {
     ktot[(long long)i * L + j] = rows[i];
     for (int s = 0; s <= ngas; s++)
       dkq[((long long)i * (ngas + 1) + s) * L + j] = rows[(1 + s) * NG + i];
   }
-  // ---- CIA continuum (ngascon is zero: no gas-continuum flags) -----------
-  {
-    double absorb, dab[7];
-    const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
-    cia_layer(a, vv, R.inormal, lc, &absorb, dab);
-    for (int s = lane; s <= ngas; s += W) {
-      double d = 0.0;
-      for (int l = 0; l < 5; l++)
-        if (R.iabs[l] == s) d += dab[l] / lc.utotl;
-      if (s == ngas) d += dab[5];
-      dtcon[(long long)s * L + j] = (T)d;
-    }
-    if (lane == 0) tcon[j] = (T)absorb;
+}
+
+// ============================================================= CIA kernel
+// CIA continuum (nciacon.f, ciaread.f; ngascon is zero: no gas-continuum
+// flags) for one (request, wavenumber, layer) per thread, FP64, written to
+// the tcon/dtcon part of the intermediate block.  Scalar work per layer, so
+// it runs one item per lane rather than inside the per-layer wavefront.
+template <typename T>
+__global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= a.nA) return;
+  const int rq = find_req(a.startA, a.nreq, item);
+  const DevReq &R = a.reqs[rq];
+  const int L = R.L, ngas = R.ngas, NG = a.ng;
+  const long long loc = item - R.itemA0;
+  const int iw = (int)(loc / L);
+  const int j = (int)(loc - (long long)iw * L);
+  const LayerConst lc = a.lay[R.lay0 + j];
+  const double vwave = a.vwave[R.wave0 + iw];
+  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas);
+  T *tcon = inter + (long long)NG * L + (long long)NG * (ngas + 1) * L;
+  T *dtcon = tcon + L;
+  double absorb, dab[7];
+  const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
+  cia_layer(a, vv, R.inormal, lc, &absorb, dab);
+  for (int s = 0; s <= ngas; s++) {
+    double d = 0.0;
+    for (int l = 0; l < 5; l++)
+      if (R.iabs[l] == s) d += dab[l] / lc.utotl;
+    if (s == ngas) d += dab[5];
+    dtcon[(long long)s * L + j] = (T)d;
   }
+  tcon[j] = (T)absorb;
 }
 
 // ========================================================= emission kernel
@@ -1066,6 +1089,13 @@ int radg_emission_RL(int maxlays) {
 
 // mode: 0 FP64, 1 mixed (FP32 table), 2 mixed with BF16 log10(k) table
 hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t lds, hipStream_t st) {
+  if (a.nA > 0) {
+    const unsigned nb = (unsigned)((a.nA + 255) / 256);
+    if (mode == 0) hipLaunchKernelGGL(cia_kernel<double>, dim3(nb), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(cia_kernel<float>, dim3(nb), dim3(256), 0, st, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (mode == 0) {
     switch (E) {
       case 1: return launch_layer_E<double, 1>(a, lds, st);

@@ -153,11 +153,14 @@ def gauss_legendre_g(ng: int):
 
 def make_ktables(ids, vmin, delv, npoint, ng=20, np_=20, nt=10,
                  pmin_bar=5e-5, pmax_bar=20.0, tmin=40.0, tmax=220.0,
-                 seed=2, mu=-22.0, sigma=2.0) -> List[KTable]:
+                 seed=2, mu=-22.0, sigma=2.0, distinct=None) -> List[KTable]:
     """k-tables with log10(k_phys) ~ N(mu, sigma), sorted ascending in g.
 
     One independent draw per (gas, wavenumber, P, T); the table stores
-    ``k_phys * 1e20``.
+    ``k_phys * 1e20``.  With ``distinct`` set, only that many wavenumber
+    slices are drawn per gas and repeated along the grid (each wavenumber
+    still owns its own copy in memory): large tables without the cost of
+    drawing every value.
     """
     rng = np.random.default_rng(seed)
     g_ord, delg = gauss_legendre_g(ng)
@@ -165,9 +168,12 @@ def make_ktables(ids, vmin, delv, npoint, ng=20, np_=20, nt=10,
     temp = np.linspace(tmin, tmax, nt)
     out = []
     for gid in ids:
-        lk = rng.normal(mu, sigma, size=(npoint, np_, nt, ng))
+        nd = npoint if distinct is None else min(int(distinct), npoint)
+        lk = rng.normal(mu, sigma, size=(nd, np_, nt, ng))
         lk.sort(axis=-1)
         k = np.power(10.0, lk + 20.0)
+        if nd < npoint:
+            k = k[np.arange(npoint) % nd]
         out.append(KTable(idgas=int(gid), isogas=0, vmin=float(vmin),
                           delv=float(delv), fwhm=0.0, press=press,
                           temp=temp, g_ord=g_ord, delg=delg, k=k))
@@ -255,6 +261,21 @@ def make_request(nlayer=120, nkgas=3, nwave=1000, v0=600.0, v1=1400.0,
                    layinc=layinc, scale=scale, emtemp=emtemp, imod=imod,
                    isw=np.asarray(isw, dtype=np.int32), path_weight=pw,
                    hfp=np.zeros(nlayer))
+
+
+def config5_tables(nwave=10000, nkgas=8, ng=20, np_=30, nt=15, distinct=64):
+    """Tables of the large-spectrum stress: 8 k-gases on 10000 wavenumbers at
+    0.1 cm-1 (600-1599.9 cm-1), a 30 P x 15 T grid and 20 g-ordinates
+    (8 x 10000 x 9000 values, 5.76 GB in FP64, 1.44 GB as BF16)."""
+    return make_ktables(KGAS_IDS[:nkgas], 600.0, 0.1, nwave, ng=ng, np_=np_, nt=nt,
+                        distinct=distinct)
+
+
+def config5_request(seed, nwave=10000, nlayer=200, nkgas=8):
+    """One request of the large-spectrum stress: 200 layers, the 8 k-gases +
+    H2/He, every wavenumber of the 0.1 cm-1 table grid."""
+    return make_request(nlayer=nlayer, nkgas=nkgas, nwave=nwave, seed=seed,
+                        table_vmin=600.0, table_delv=0.1)
 
 
 def config1(nwave=1000, nlayer=120, nkgas=3, ng=20, np_=20, nt=10):

@@ -243,6 +243,186 @@ template <int E> struct Sorter<double, E> {
   }
 };
 
+// ------------------------------------------- 32-bit sort for FP32 keys
+// Value of lane (lane ^ m) for a lane mask m known after unrolling: DPP
+// quad permutes and row mirrors where the pattern exists on gfx9, two DPP
+// moves for 4 and 8, a swizzle within 32 lanes for 16 and 31, and
+// ds_bpermute only across the two wave halves.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int x) {
+  // every lane has a source lane in these patterns, so no lane keeps an old value
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
+}
+template <int M>
+__device__ __forceinline__ unsigned xlane(unsigned v) {
+  const int x = (int)v;
+  if constexpr (M == 1) return (unsigned)dpp_mov<0xB1>(x);                  // quad_perm 1,0,3,2
+  else if constexpr (M == 2) return (unsigned)dpp_mov<0x4E>(x);             // quad_perm 2,3,0,1
+  else if constexpr (M == 3) return (unsigned)dpp_mov<0x1B>(x);             // quad_perm 3,2,1,0
+  else if constexpr (M == 4) return (unsigned)dpp_mov<0x141>(dpp_mov<0x1B>(x));   // ^3, ^7
+  else if constexpr (M == 7) return (unsigned)dpp_mov<0x141>(x);            // row_half_mirror
+  else if constexpr (M == 8) return (unsigned)dpp_mov<0x140>(dpp_mov<0x141>(x));  // ^7, ^15
+  else if constexpr (M == 15) return (unsigned)dpp_mov<0x140>(x);           // row_mirror
+  else if constexpr (M == 16 || M == 31)
+    return (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x1F | (M << 10));
+  else return (unsigned)__shfl_xor(x, M);
+}
+
+// Exchange across lanes: the lower lane of a pair keeps the smaller value,
+// the upper lane the larger, i.e. med3(v, partner, c) with c = 0 in lower
+// lanes and c = 0xffffffff in upper lanes.
+__device__ __forceinline__ unsigned med3_u32(unsigned a, unsigned b, unsigned c) {
+  unsigned r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+// 0 where bit b of the lane number is 0, 0xffffffff where it is 1
+__device__ __forceinline__ unsigned lane_bit_mask(int lane, int b) {
+  return (unsigned)(((int)((unsigned)lane << (31 - b))) >> 31);
+}
+
+// Ascending bitonic network (same mirror form as above) on 32-bit unsigned
+// values, unrolled by template recursion so that every lane pattern is a
+// constant: an in-lane compare-exchange is one min and one max, a cross-lane
+// one a lane move and a med3.
+template <int E, int JS>
+__device__ __forceinline__ void u32_half_cleaners(unsigned (&v)[E], int lane) {
+  constexpr int LOGE = ilog2c(E);
+  if constexpr (JS >= LOGE) {
+    const unsigned c = lane_bit_mask(lane, JS - LOGE);
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = med3_u32(v[q], xlane<(1 << (JS - LOGE))>(v[q]), c);
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int q2 = q ^ (1 << JS);
+      if (q2 > q) {
+        const unsigned x = v[q], y = v[q2];
+        v[q] = min(x, y);
+        v[q2] = max(x, y);
+      }
+    }
+  }
+  if constexpr (JS > 0) u32_half_cleaners<E, JS - 1>(v, lane);
+}
+
+template <int E, int KS>
+__device__ __forceinline__ void u32_merge_levels(unsigned (&v)[E], int lane) {
+  constexpr int LOGE = ilog2c(E);
+  constexpr int LOGP = 6 + LOGE;
+  // mirror stage: partner e ^ (2^KS - 1)
+  if constexpr (KS - 1 >= LOGE) {
+    const unsigned c = lane_bit_mask(lane, KS - 1 - LOGE);
+    unsigned o[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) o[q] = xlane<(((1 << KS) - 1) >> LOGE)>(v[E - 1 - q]);
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = med3_u32(v[q], o[q], c);
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int q2 = q ^ ((1 << KS) - 1);
+      if (q2 > q) {
+        const unsigned x = v[q], y = v[q2];
+        v[q] = min(x, y);
+        v[q2] = max(x, y);
+      }
+    }
+  }
+  if constexpr (KS >= 2) u32_half_cleaners<E, KS - 2>(v, lane);
+  if constexpr (KS < LOGP) u32_merge_levels<E, KS + 1>(v, lane);
+}
+
+template <int E>
+__device__ __forceinline__ void bitonic_sort_u32(unsigned (&v)[E], int lane) {
+  u32_merge_levels<E, 1>(v, lane);
+}
+
+__device__ __forceinline__ bool kf_lt(float ka, int ia, float kb, int ib) {
+  return (ka < kb) | ((ka == kb) & (ia < ib));
+}
+
+// After the network has ordered the truncated words (key bits with the low
+// log2(64E) mantissa bits replaced by the element number), slots can only be
+// out of exact (key, element) order inside a run of equal truncated keys.
+// Odd-even transposition on the exact keys, until no adjacent pair is out of
+// order (a run of m slots needs at most m phases; the bound W*E is never
+// reached on a sorted input).
+template <int E>
+__device__ __forceinline__ void exact_fixup(float (&kx)[E], int (&ex)[E], int lane) {
+  auto unsorted = [&]() {
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q + 1 < E; q++) bad |= kf_lt(kx[q + 1], ex[q + 1], kx[q], ex[q]);
+    const float nk = __shfl_down(kx[0], 1);
+    const int ne = __shfl_down(ex[0], 1);
+    bad |= (lane + 1 < W) && kf_lt(nk, ne, kx[E - 1], ex[E - 1]);
+    return __any(bad);
+  };
+  auto in_lane = [&](int q) {
+    if (kf_lt(kx[q + 1], ex[q + 1], kx[q], ex[q])) {
+      const float tk = kx[q];
+      const int te = ex[q];
+      kx[q] = kx[q + 1];
+      ex[q] = ex[q + 1];
+      kx[q + 1] = tk;
+      ex[q + 1] = te;
+    }
+  };
+  // pair (slot E-1 of lane l, slot 0 of lane l+1) for every l with en(l)
+  auto cross = [&](auto en) {
+    const float nk = __shfl_down(kx[0], 1);
+    const int ne = __shfl_down(ex[0], 1);
+    const float pk = __shfl_up(kx[E - 1], 1);
+    const int pe = __shfl_up(ex[E - 1], 1);
+    const bool lo = (lane + 1 < W) && en(lane) && kf_lt(nk, ne, kx[E - 1], ex[E - 1]);
+    const bool hi = (lane > 0) && en(lane - 1) && kf_lt(kx[0], ex[0], pk, pe);
+    if (lo) {
+      kx[E - 1] = nk;
+      ex[E - 1] = ne;
+    }
+    if (hi) {
+      kx[0] = pk;
+      ex[0] = pe;
+    }
+  };
+  for (int it = 0; it < W * E && unsorted(); it++) {
+    if constexpr (E == 1) {
+      cross([](int l) { return (l & 1) == 0; });
+      cross([](int l) { return (l & 1) == 1; });
+    } else {
+#pragma unroll
+      for (int q = 0; q + 1 < E; q += 2) in_lane(q);
+#pragma unroll
+      for (int q = 1; q + 1 < E; q += 2) in_lane(q);
+      cross([](int) { return true; });
+    }
+  }
+}
+
+// ------------------------------------------------ FP64 wave scans by DPP
+template <int CTRL, int RM, bool BC>
+__device__ __forceinline__ double dpp_d(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, RM, 0xF, BC);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, RM, 0xF, BC);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+// inclusive prefix sum over the 64 lanes: row shifts 1, 2, 4, 8 (zero fill),
+// then lane 15 of each row into the next row and lane 31 into rows 2 and 3
+__device__ __forceinline__ double wave_incl_sum(double s) {
+  s += dpp_d<0x111, 0xF, true>(s);
+  s += dpp_d<0x112, 0xF, true>(s);
+  s += dpp_d<0x114, 0xF, true>(s);
+  s += dpp_d<0x118, 0xF, true>(s);
+  s += dpp_d<0x142, 0xA, false>(s);
+  s += dpp_d<0x143, 0xC, false>(s);
+  return s;
+}
+// value of lane - 1 (0 in lane 0)
+__device__ __forceinline__ double wave_shr1(double s) { return dpp_d<0x138, 0xF, true>(s); }
+
 // ----------------------------------------------------------- CIA, FP64
 // ciaread.f:104-196 + nciacon.f:209-303 for one layer.  dab[0..6] as
 // DABSORB (already multiplied by the path factor tau).
@@ -339,10 +519,11 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
 //   G[NG+1] | recS[NG][NQ] | recC[NG][NQ] | recG[NG] | recW[NG] | dl64[NG] (FP64)
 //   kgn[ngas][NG] | dkgn[ngas][NG] | bufA[NS][NG] | bufB[NS][NG] | dlg[NG]  (T)
 //   rtype[NG]                                                      (int)
+//   keyE[64*E]  exact FP32 key of each element (FP32 form)       (float)
 // with NS = ngas + 2 row slots (K, gas 0..ngas-1, T) and NQ = ngas + 2 rebinned
 // quantities (key, gas slots, T).
 template <typename T, int E, int BF>
-__global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs a, long long nblocks) {
+__global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
   if (item >= a.nA) return;
   const int lane = threadIdx.x;
@@ -370,6 +551,8 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
   T *bufB = bufA + NS * NG;
   T *dlg = bufB + NS * NG;
   int *rtype = reinterpret_cast<int *>(dlg + NG);
+  float *keyE = reinterpret_cast<float *>(rtype + NG);  // [W*E], FP32 form only
+  const float rng = 1.0f / (float)NG;  // element e -> i = floor((e + 0.5) / NG)
 
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
                                    : reinterpret_cast<const T *>(a.delg32);
@@ -487,29 +670,68 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
         nxt[(1 + r) * NG + i] = (T)0;
       }
     } else {
-      // keys K_i + k2_j*a2 (noverlapg.f:117-128,175-187); the index packs
-      // (i, j) as i*32 + j, padding slots get unique indices >= 1024
+      // keys K_i + k2_j*a2 (noverlapg.f:117-128,175-187) of element
+      // e = i*NG + j; padding slots e >= n hold +inf.  Ties are ordered by e.
       T key[E];
       int idx[E];
-      {
+      int ij[E];  // (i << 8) | j of each sorted slot
+      if constexpr (sizeof(T) == 4) {
+        // FP32 keys are >= 0, so their bit patterns order like their
+        // values.  Sort one 32-bit word per slot, (key bits with the low
+        // log2(64E) mantissa bits cleared) | e, with min/max exchanges; then
+        // restore the exact (key, e) order inside runs of equal truncated
+        // keys from the exact keys kept in LDS.
+        constexpr unsigned IM = (unsigned)(W * E - 1);
+        unsigned pk[E];
+        {
+          int ii = i00, jj = j00;
+#pragma unroll
+          for (int q = 0; q < E; q++) {
+            const int e = lane * E + q;
+            float kq = __builtin_huge_valf();
+            if (e < n) {
+              kq = rows[ii] + k2[jj] * a2;
+              kq = (kq > 0.0f) ? kq : 0.0f;  // -0 and 0 order alike
+            }
+            keyE[e] = kq;
+            pk[q] = (__float_as_uint(kq) & ~IM) | (unsigned)e;
+            if (++jj == NG) {
+              jj = 0;
+              ++ii;
+            }
+          }
+        }
+        bitonic_sort_u32<E>(pk, lane);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          idx[q] = (int)(pk[q] & IM);
+          key[q] = keyE[idx[q]];
+        }
+        exact_fixup<E>(key, idx, lane);
+      } else {
         int ii = i00, jj = j00;
 #pragma unroll
         for (int q = 0; q < E; q++) {
           const int e = lane * E + q;
           if (e < n) {
             key[q] = rows[ii] + k2[jj] * a2;
-            idx[q] = (ii << 5) | jj;
           } else {
             key[q] = dinf<T>();
-            idx[q] = 1024 + e;
           }
+          idx[q] = e;
           if (++jj == NG) {
             jj = 0;
             ++ii;
           }
         }
+        Sorter<T, E>::sort(key, idx, lane);
       }
-      Sorter<T, E>::sort(key, idx, lane);
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int i = (int)(((float)idx[q] + 0.5f) * rng);
+        ij[q] = (i << 8) | (idx[q] - i * NG);
+      }
 
       // weights and their FP64 cumulative sum gdist (rankg.f:104-108).  The
       // sum is taken over FP64 products: where a run of elements exactly
@@ -520,50 +742,42 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
       double run = 0.0;
 #pragma unroll
       for (int q = 0; q < E; q++) {
-        const int id = idx[q];
-        const bool real = id < 1024;
-        wq[q] = real ? dlg[id >> 5] * dlg[id & 31] : (T)0;
-        run += real ? dl64[id >> 5] * dl64[id & 31] : 0.0;
+        const bool real = ij[q] < (NG << 8);
+        const int i = ij[q] >> 8, jj = ij[q] & 255;
+        wq[q] = real ? dlg[i] * dlg[jj] : (T)0;
+        run += real ? dl64[i] * dl64[jj] : 0.0;
         gd[q] = run;
       }
-      double s = run;
-#pragma unroll
-      for (int ds = 0; ds < 6; ds++) {
-        const int d = 1 << ds;
-        const double o = __shfl_up(s, d);
-        if (lane >= d) s += o;
-      }
-      double ex = __shfl_up(s, 1);
-      if (lane == 0) ex = 0.0;
+      const double s = wave_incl_sum(run);
+      const double ex = wave_shr1(s);
 #pragma unroll
       for (int q = 0; q < E; q++) gd[q] += ex;
-      double p0[E];
-      p0[0] = __shfl_up(gd[E - 1], 1);
-      if (lane == 0) p0[0] = 0.0;
-#pragma unroll
-      for (int q = 1; q < E; q++) p0[q] = gd[q - 1];
+      // cumulative weight before slot q: gd[q-1], or the previous lane's
+      // last gd for q = 0
+      const double gprev = wave_shr1(gd[E - 1]);
 
-      // bin of each element = number of edges G[1..NG] <= p0; an element
-      // straddles when its end reaches the next edge (rankg.f:120)
-      // bq: bin | straddle flag << 8; omf: 1 - frac of a straddling element
-      int bq[E];
+      // bin of each element = number of edges G[1..NG] <= its start; an
+      // element straddles when its end reaches the next edge (rankg.f:120)
+      // ij[q] |= bin << 16 | straddle flag << 24; omf: 1 - frac of a
+      // straddling element
       T omf[E];
       {
         int lo = 0, hi = NG;
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
-          if (G[mid] <= p0[0]) lo = mid;
+          if (G[mid] <= gprev) lo = mid;
           else hi = mid - 1;
         }
         int b = lo;
 #pragma unroll
         for (int q = 0; q < E; q++) {
+          const double st = (q == 0) ? gprev : gd[q - 1];
           if (q > 0)
-            while (b < NG && G[b + 1] <= p0[q]) b++;
+            while (b < NG && G[b + 1] <= st) b++;
           const bool stq = (b < NG) && !(gd[q] < G[b + 1]);
-          bq[q] = b | (stq ? 256 : 0);
+          ij[q] |= (b << 16) | (stq ? (1 << 24) : 0);
           // 1 - frac of rankg.f:121, formed as a ratio of two FP64 differences
-          omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - p0[q])) : (T)0;
+          omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - st)) : (T)0;
         }
       }
       // bin records: a straddling element closes its bin; the last slot
@@ -571,8 +785,8 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
 #pragma unroll
       for (int q = 0; q < E; q++) {
         const int e = lane * E + q;
-        const int bi = bq[q] & 255;
-        const bool stq = bq[q] > 255;
+        const int bi = (ij[q] >> 16) & 255;
+        const bool stq = (ij[q] >> 24) != 0;
         if (stq) {
           recG[bi] = gd[q];
           recW[bi] = (double)omf[q] * (double)wq[q];
@@ -587,45 +801,25 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
           }
         }
       }
-      // weighted key and payloads: FP64 prefix sums along the sorted order
-      const int nslot = r + 2, nq = nslot + 1;
-      for (int vq = 0; vq < nq; vq++) {
-        const int s2 = vq - 1;
-        double pre[E];
+      // weighted key and payloads: FP64 prefix sums along the sorted order,
+      // recorded at the straddling slots and the last slot
+      const int NGH = NG << 8;  // (ij & 0xffff) < NGH: a real element
+      auto scan_quantity = [&](int vq, auto value) {
         T cq[E];
         double run2 = 0.0;
 #pragma unroll
         for (int q = 0; q < E; q++) {
-          const int id = idx[q];
-          T c = (T)0;
-          if (id < 1024) {
-            const int i = id >> 5, jj = id & 31;
-            T val;
-            if (vq == 0) val = key[q];
-            else if (s2 < r) val = rows[(1 + s2) * NG + i];
-            else if (s2 == r) val = k2[jj];
-            else val = rows[(1 + ngas) * NG + i] + d2[jj] * a2;
-            c = val * wq[q];
-          }
+          const T c = ((ij[q] & 0xffff) < NGH) ? value(q) * wq[q] : (T)0;
           cq[q] = c;
           run2 += (double)c;
-          pre[q] = run2;
         }
-        double s3 = run2;
-#pragma unroll
-        for (int ds = 0; ds < 6; ds++) {
-          const int d = 1 << ds;
-          const double o = __shfl_up(s3, d);
-          if (lane >= d) s3 += o;
-        }
-        double ex3 = __shfl_up(s3, 1);
-        if (lane == 0) ex3 = 0.0;
+        double pq = wave_shr1(wave_incl_sum(run2));
 #pragma unroll
         for (int q = 0; q < E; q++) {
           const int e = lane * E + q;
-          const double pq = pre[q] + ex3;
-          const int bi = bq[q] & 255;
-          const bool stq = bq[q] > 255;
+          pq += (double)cq[q];
+          const int bi = (ij[q] >> 16) & 255;
+          const bool stq = (ij[q] >> 24) != 0;
           if (stq) {
             recS[bi * NQ + vq] = pq;
             recC[bi * NQ + vq] = (double)omf[q] * (double)cq[q];
@@ -637,6 +831,23 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 3)) void layer_kernel(KernelArgs
               recC[bb * NQ + vq] = 0.0;
             }
           }
+        }
+      };
+      const int nq = r + 3;
+      if constexpr (sizeof(T) == 4)
+        scan_quantity(0, [&](int q) { return keyE[((ij[q] >> 8) & 255) * NG + (ij[q] & 255)]; });
+      else
+        scan_quantity(0, [&](int q) { return key[q]; });
+      for (int vq = 1; vq < nq; vq++) {
+        const int s2 = vq - 1;
+        if (s2 < r) {
+          const T *src = rows + (1 + s2) * NG;
+          scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255]; });
+        } else if (s2 == r) {
+          scan_quantity(vq, [&](int q) { return k2[ij[q] & 255]; });
+        } else {
+          const T *src = rows + (1 + ngas) * NG;
+          scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255] + d2[ij[q] & 255] * a2; });
         }
       }
       __syncthreads();
@@ -1060,12 +1271,12 @@ hipError_t launch_em_RL(const KernelArgs &a, size_t lds, hipStream_t st) {
 }  // namespace
 
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
-  (void)E;
   const size_t ts = fp64 ? 8 : 4;
   const size_t nq = ngas + 2, ns = ngas + 2;
   size_t b = 8 * ((size_t)(ng + 1) + 2 * (size_t)ng * nq + 3 * (size_t)ng);
   b += ts * ((size_t)2 * ngas * ng + (size_t)2 * ns * ng + ng);
   b += 4 * (size_t)ng;
+  if (!fp64) b += 4 * (size_t)W * E;
   return b;
 }
 

@@ -97,6 +97,27 @@ def cpu_baseline(tables, cia, req, seconds):
                        f"({what}) in {total_t:.1f} s")
 
 
+def pmc_figures(batch, t_layer_s):
+    """HBM bytes per launch and VALU issue of the layer kernel from the
+    committed rocprofv3 PMC passes (profiles/layer_kernel_traffic.json), when
+    they were taken on this batch size; the VALU rate uses this run's
+    kernel time."""
+    path = os.path.join(ROOT, "profiles", "layer_kernel_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        p = json.load(f)
+    if p.get("spectra_per_launch") != batch:
+        return {}
+    traffic = (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]) * 1024
+    rate = p["valu_wave_instructions"] / t_layer_s
+    return {"traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (" + p["source"] + ")",
+            "valu_issue": {"wave_instructions_per_launch": p["valu_wave_instructions"],
+                           "achieved_per_s": rate, "peak_per_s": p["valu_issue_peak_per_s"],
+                           "frac": rate / p["valu_issue_peak_per_s"]}}
+
+
 def weighted(out, dq, pw):
     """Angle-weighted radiance and Jacobian from per-path results."""
     return np.einsum("wp,p->w", out, pw), np.einsum("wplq,p->wlq", dq, pw)
@@ -203,7 +224,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true", help="time only the mixed form")
     ap.add_argument("--serving-requests", type=int, default=4096,
                     help="requests in the Poisson serving run (0: skip it)")
-    ap.add_argument("--serving-rate", type=float, default=60.0,
+    ap.add_argument("--serving-rate", type=float, default=240.0,
                     help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--bf16-steps", type=int, default=3)
     ap.add_argument("--config5-requests", type=int, default=64,
@@ -280,6 +301,7 @@ def main():
                 frac=achieved / HBM_PEAK_GBS, traffic=None,
                 kernel="layer_kernel<float,8>", kernel_ms=float(kern[0]),
                 alg_bytes_per_launch=alg_read)
+    roof.update(pmc_figures(args.batch, t_layer_s))
 
     cpu = None
     if not args.no_cpu:

@@ -339,6 +339,8 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
     for (int k = 0; k < ngas; k++) {
       WaveGas &w = P.wg[(size_t)iw * ngas + k];
       w.base = -1;
+      w.base2 = -1;
+      w.w2 = 0.0;
       w.coinc = 1;
       w.sc = 1.0f;
       w.off = 0.0f;
@@ -359,14 +361,20 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
       long long n1 = (long long)((v + eps - tb->vmin) / tb->delv);
       n1 = std::min<long long>(std::max<long long>(n1, 0), tb->npoint - 1);
       const double tmp = tb->vmin + tb->delv * n1;
-      if (!(std::fabs(v - tmp) < eps))
-        return fail(RADG_EUNSUPPORTED,
-                    "wavenumber %g lies between table points of gas %d (non-coincident rankk branch "
-                    "is provided by the CPU oracle only)", v, q.idgas[k]);
+      // get_kg.f:224-250 tests the slice at or below the wavenumber only
       if (tb->zero[n1]) continue;  // all-zero slice: k = 0
       w.base = tb->off + n1 * ntab;
       w.sc = (float)tb->bf_sc;
       w.off = (float)tb->bf_off;
+      if (!(std::fabs(v - tmp) < eps)) {
+        // between table points: interpolate the two slices and merge them
+        // with rankk (get_kg.f:379-440); the last point has no slice above
+        // and reuses its own
+        w.coinc = 0;
+        w.base2 = (n1 + 1 < tb->npoint) ? w.base + ntab : w.base;
+        w.w2 = std::min(1.0, std::max(0.0, (v - tmp) / tb->delv));
+        tbytes += (double)nodes.size() * NG * kbytes;
+      }
       tbytes += (double)nodes.size() * NG * kbytes;
     }
   }

@@ -43,9 +43,11 @@ struct LayerConst {
   double qh2, qhe, qn2, qch4, qco2, tau;
 };
 
-// Table slice used by one gas at one wavenumber.
+// Table slice(s) used by one gas at one wavenumber.
 struct WaveGas {
   long long base;        // element offset of the slice in the table pool; -1: k = 0
+  long long base2;       // next slice up in wavenumber (non-coincident only)
+  double w2;             // weight of the upper slice, get_kg.f:381-384
   int coinc;             // 1: the wavenumber is on the table grid (get_kg.f:253-264)
   float sc, off;         // BF16 storage: log10(k) = code * sc + off
   int pad;

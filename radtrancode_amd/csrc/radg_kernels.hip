@@ -519,6 +519,7 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
 //   G[NG+1] | recS[NG][NQ] | recC[NG][NQ] | recG[NG] | recW[NG] | dl64[NG] (FP64)
 //   kgn[ngas][NG] | dkgn[ngas][NG] | bufA[NS][NG] | bufB[NS][NG] | dlg[NG]  (T)
 //   rtype[NG]                                                      (int)
+//   rkK | rkD | rkF [64 each]  rankk merge of two slices, 8-byte aligned (T)
 //   keyE[64*E]  exact FP32 key of each element (FP32 form)       (float)
 // with NS = ngas + 2 row slots (K, gas 0..ngas-1, T) and NQ = ngas + 2 rebinned
 // quantities (key, gas slots, T).
@@ -551,7 +552,10 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
   T *bufB = bufA + NS * NG;
   T *dlg = bufB + NS * NG;
   int *rtype = reinterpret_cast<int *>(dlg + NG);
-  float *keyE = reinterpret_cast<float *>(rtype + NG);  // [W*E], FP32 form only
+  T *rkK = reinterpret_cast<T *>(smem_d + ((reinterpret_cast<char *>(rtype + NG) -
+                                            reinterpret_cast<char *>(smem_d) + 7) / 8));
+  T *rkD = rkK + W, *rkF = rkD + W;                      // [W] each: rankk merge
+  float *keyE = reinterpret_cast<float *>(rkF + W);      // [W*E], FP32 form only
   const float rng = 1.0f / (float)NG;  // element e -> i = floor((e + 0.5) / NG)
 
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
@@ -563,7 +567,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
     rtype[i] = 0;
   }
 
-  // ---- k-table interpolation, get_kg.f:308-476 (coincident branch) -------
+  // ---- k-table interpolation, get_kg.f:308-476 -----------------------------
   {
     const T *tab = reinterpret_cast<const T *>(a.ktab);
     const int nt = a.nt;
@@ -572,8 +576,122 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
     const int i3 = (nt * NG * lc.cp) + (NG * lc.ct);
     const int i4 = (nt * NG * (lc.cp - 1)) + (NG * lc.ct);
     const T v = (T)lc.v, u = (T)lc.u, dudt = (T)lc.dudt;
+    const float ln10 = 2.302585092994046f;
+
+    // Wavenumber between two table points (get_kg.f:379-440): interpolate
+    // linear k on the slice at or below it (lanes 0..NG-1) and on the slice
+    // above (lanes NG..2NG-1), weight the ordinates delg_i (1 - w2) and
+    // delg_i w2, sort the 2NG points and rebin them in the order of rankk.f
+    // (whole points while their cumulative weight is <= the next edge, the
+    // straddling point split by the fraction below the edge, each closed bin
+    // divided by its accumulated weight).  FP64 accumulation in both forms.
+    auto merge_slices = [&](int k, const WaveGas &wgk) {
+      const int cnt = 2 * NG;
+      const bool act = lane < cnt;
+      const int gi = act ? (lane < NG ? lane : lane - NG) : 0;
+      T x = (T)0, dxdt = (T)0, fw = (T)0;
+      if (act) {
+        const long long sb = (lane < NG) ? wgk.base : wgk.base2;
+        T y1, y2, y3, y4;
+        if (BF) {
+          const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + sb;
+          auto dec = [&](unsigned short c) -> T {
+            if (c == RADG_BF16_NONPOS) return (T)0;
+            return (T)expf((__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10);
+          };
+          y1 = dec(sl[i1 + gi]);
+          y2 = dec(sl[i2 + gi]);
+          y3 = dec(sl[i3 + gi]);
+          y4 = dec(sl[i4 + gi]);
+        } else {
+          const T *sl = tab + sb;
+          y1 = sl[i1 + gi];
+          y2 = sl[i2 + gi];
+          y3 = sl[i3 + gi];
+          y4 = sl[i4 + gi];
+        }
+        x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 +
+            ((T)1 - v) * u * y4;
+        dxdt = (-((T)1 - v) * y1 - v * y2 + v * y3 + ((T)1 - v) * y4) * dudt;
+        const T wl = (lane < NG) ? (T)(1.0 - wgk.w2) : (T)wgk.w2;
+        fw = dlg[gi] * wl;
+      }
+      rkD[lane] = dxdt;
+      rkF[lane] = fw;
+      T key[1];
+      int idx[1];
+      if constexpr (sizeof(T) == 4) {
+        const float kq = act ? ((x > 0.0f) ? x : 0.0f) : __builtin_huge_valf();
+        keyE[lane] = kq;
+        unsigned pk[1] = {(__float_as_uint(kq) & ~(unsigned)(W - 1)) | (unsigned)lane};
+        bitonic_sort_u32<1>(pk, lane);
+        __syncthreads();
+        idx[0] = (int)(pk[0] & (unsigned)(W - 1));
+        key[0] = keyE[idx[0]];
+        exact_fixup<1>(key, idx, lane);
+      } else {
+        key[0] = act ? x : dinf<T>();
+        idx[0] = lane;
+        __syncthreads();
+        Sorter<T, 1>::sort(key, idx, lane);
+      }
+      const T sd = rkD[idx[0]], sf = rkF[idx[0]];
+      __syncthreads();
+      rkK[lane] = key[0];
+      rkD[lane] = sd;
+      rkF[lane] = sf;
+      for (int i = lane; i < NG; i += W) {
+        kgn[k * NG + i] = (T)0;
+        dkgn[k * NG + i] = (T)0;
+      }
+      __syncthreads();
+      double gprev = 0.0, sum = 0.0, ak = 0.0, ad = 0.0;
+      int ig = 0;
+      for (int I = 0; I < cnt; I++) {
+        const double c = (double)rkK[I], dc = (double)rkD[I], wgt = (double)rkF[I];
+        const double gcur = gprev + wgt;
+        if (ig < NG) {  // once the last bin is closed the remaining points are dropped
+          if (gcur <= G[ig + 1]) {
+            ak += c * wgt;
+            ad += dc * wgt;
+            sum += wgt;
+          } else {
+            const double fr = (G[ig + 1] - gprev) / (gcur - gprev);
+            ak += fr * c * wgt;
+            ad += fr * dc * wgt;
+            sum += fr * wgt;
+            if (lane == 0) {
+              kgn[k * NG + ig] = (T)(ak / sum);
+              dkgn[k * NG + ig] = (T)(ad / sum);
+            }
+            ig++;
+            if (ig < NG) {
+              sum = (1.0 - fr) * wgt;
+              ak = (1.0 - fr) * c * wgt;
+              ad = (1.0 - fr) * dc * wgt;
+            }
+          }
+        }
+        gprev = gcur;
+      }
+      if (ig == NG - 1 && lane == 0) {
+        kgn[k * NG + ig] = (T)(ak / sum);
+        dkgn[k * NG + ig] = (T)(ad / sum);
+      }
+      __syncthreads();
+      for (int i = lane; i < NG; i += W) {  // get_kg.f:443-471
+        const T kk = kgn[k * NG + i], dk = dkgn[k * NG + i];
+        if (kk != kk) kgn[k * NG + i] = (T)1e-37;
+        if (dk != dk) dkgn[k * NG + i] = (T)1e-37;
+      }
+    };
+
     for (int k = 0; k < ngas; k++) {
       const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
+      if (wgk.base >= 0 && !wgk.coinc) {
+        merge_slices(k, wgk);
+        continue;
+      }
       if (lane < NG) {
         T kk = 0, dk = 0;
         if (wgk.base >= 0) {
@@ -584,7 +702,6 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
             const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + wgk.base;
             klog = !(sl[i1] == RADG_BF16_NONPOS || sl[i2] == RADG_BF16_NONPOS ||
                      sl[i3] == RADG_BF16_NONPOS || sl[i4] == RADG_BF16_NONPOS);
-            const float ln10 = 2.302585092994046f;
             auto dec = [&](unsigned short c) -> float {
               if (c == RADG_BF16_NONPOS) return klog ? -__builtin_huge_valf() : 0.0f;
               const float l = (__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10;
@@ -1276,6 +1393,7 @@ size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
   size_t b = 8 * ((size_t)(ng + 1) + 2 * (size_t)ng * nq + 3 * (size_t)ng);
   b += ts * ((size_t)2 * ngas * ng + (size_t)2 * ns * ng + ng);
   b += 4 * (size_t)ng;
+  b += 8 + 3 * (size_t)W * ts;  // rankk merge buffers after 8-byte alignment
   if (!fp64) b += 4 * (size_t)W * E;
   return b;
 }

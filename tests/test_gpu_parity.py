@@ -28,11 +28,6 @@ def _run(rad, tables, req, mode):
 def test_golden_case(gpu_lib, path, mode):
     tables, req, outs, note = load_case(path)
     print(f"case {path.split('/')[-1]} mode {mode}", flush=True)
-    if "noncoincident" in path:
-        with pytest.raises(gpu_lib.RadgError) as ei:
-            _run(gpu_lib, tables, req, mode)
-        assert ei.value.code == 2  # RADG_EUNSUPPORTED: rankk branch is CPU-oracle only
-        return
     o, g, d = _run(gpu_lib, tables, req, mode)
     tr, tj = (TOL_RAD_FP64, TOL_JAC_FP64) if mode == 0 else (TOL_RAD_MIXED, TOL_JAC_MIXED)
     er, eg, ej = (rad_err(o, outs["ref64_output"]), rad_err(g, outs["ref64_gtsurf"]),
@@ -54,6 +49,30 @@ def test_seeded_against_oracle(gpu_lib, shape):
     for mode in (0, 1):
         o, g, d = _run(gpu_lib, tables, req, mode)
         tr, tj = (TOL_RAD_FP64, TOL_JAC_FP64) if mode == 0 else (TOL_RAD_MIXED, TOL_JAC_MIXED)
+        er, ej = rad_err(o, o_ref), jac_err(d, d_ref)
+        print(f"  mode {mode}: radiance {er:.2e} jacobian {ej:.2e}", flush=True)
+        assert er < tr and ej < tj
+
+
+@pytest.mark.parametrize("frac", [0.37, 0.5, 0.93, 0.01], ids=lambda f: f"w2_{f}")
+def test_between_table_points_against_oracle(gpu_lib, frac):
+    """Wavenumbers between table points take get_kg's two-slice rankk merge
+    (get_kg.f:379-440, rankk.f); w2 = 0.01 lies just outside the 0.02 delv
+    coincidence tolerance.  The last wavenumber sits past the last table
+    point (within the tolerance), where the slice above is the slice itself."""
+    from oracle import oracle
+    tables, cia, req = synth.config1(nwave=10, nlayer=60, nkgas=3, ng=20)
+    delv = tables[0].delv
+    req.vwave = req.vwave + frac * delv
+    req.vwave[-1] = tables[0].vmin + (tables[0].npoint - 1) * delv + 0.015 * delv
+    print(f"between table points frac {frac}", flush=True)
+    o_ref, g_ref, d_ref = oracle.spectrum(tables, load_cia(), req, nthreads=8)
+    for mode in (0, 1, 2):
+        o, g, d = _run(gpu_lib, tables, req, mode)
+        if mode == 2:  # BF16 storage: error reported, bounded loosely
+            tr, tj = 2e-3, 0.5
+        else:
+            tr, tj = (TOL_RAD_FP64, TOL_JAC_FP64) if mode == 0 else (TOL_RAD_MIXED, TOL_JAC_MIXED)
         er, ej = rad_err(o, o_ref), jac_err(d, d_ref)
         print(f"  mode {mode}: radiance {er:.2e} jacobian {ej:.2e}", flush=True)
         assert er < tr and ej < tj

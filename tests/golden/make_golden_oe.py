@@ -1,0 +1,77 @@
+"""Write the golden vectors of the optimal-estimation step and the layer ->
+state maps from the reference's own routines.
+
+Run in a container where ``/root/reference`` is present and
+``make -C oracle ref`` has built ``oracle/_ref/liboeref64.so``:
+
+    python tests/golden/make_golden_oe.py
+
+Cases:
+* ``oe_small``: nx = 12, ny = 30 (calc_gain_matrix's nx < ny branch);
+* ``oe_wide``:  nx = 40, ny = 25 (its nx >= ny branch);
+* ``oe_medium``: nx = 121, ny = 300;
+* ``oe_cfg4``: nx = 363, ny = 1000, the batched-retrieval shape.  Its inputs
+  are regenerated from the seed by :func:`oe_inputs`; only the outputs, the
+  row sums of the gain matrix and the diagonal of the averaging kernels are
+  stored;
+* ``map_small``: map2pro + map2xvec for 3 paths x 20 layers x 2 gases,
+  21 levels, 30 state elements.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+
+def oe_inputs(nx, ny, seed):
+    """Seeded OE inputs: K ~ N(0, 1), Sa = A A^T / nx + 0.5 I, diagonal
+    Se in [0.5, 2], xa ~ N(0, 1), xn = xa + 0.1 N(0, 1), y ~ N(0, 1),
+    yn = y + 0.2 N(0, 1)."""
+    rng = np.random.default_rng(seed)
+    kk = rng.normal(size=(ny, nx))
+    a = rng.normal(size=(nx, nx))
+    sa = a @ a.T / nx + 0.5 * np.eye(nx)
+    se = rng.uniform(0.5, 2.0, ny)
+    xa = rng.normal(size=nx)
+    xn = xa + 0.1 * rng.normal(size=nx)
+    y = rng.normal(size=ny)
+    yn = y + 0.2 * rng.normal(size=ny)
+    return dict(kk=kk, sa=sa, se_var=se, xa=xa, xn=xn, y=y, yn=yn)
+
+
+def map_inputs(seed=11, npath=3, L=20, ngas=2, npro=21, nv=30):
+    rng = np.random.default_rng(seed)
+    return dict(dq=rng.normal(size=(npath, L, ngas + 1)), nlayin=np.array([20, 18, 20]),
+                layinc=np.array([rng.permutation(L) + 1 for _ in range(npath)]),
+                dam=rng.normal(size=(L, npro)), dte=rng.normal(size=(L, npro)),
+                xmap=rng.normal(size=(nv, ngas + 1, npro)))
+
+
+def main():
+    from oracle import oe_ref
+    assert oe_ref.available(), "build oracle/_ref/liboeref64.so first (make -C oracle ref)"
+    for name, nx, ny, seed, full in [("oe_small", 12, 30, 1, True), ("oe_wide", 40, 25, 2, True),
+                                     ("oe_medium", 121, 300, 3, True),
+                                     ("oe_cfg4", 363, 1000, 4, False)]:
+        inp = oe_inputs(nx, ny, seed)
+        r = oe_ref.oe_step(**inp)
+        out = dict(nx=nx, ny=ny, seed=seed, x_out=r["x_out"], phi=r["phi"], chisq=r["chisq"],
+                   dd_rowsum=r["dd"].sum(axis=1), aa_diag=np.diag(r["aa"]).copy())
+        if full:
+            out.update({k: v for k, v in inp.items()})
+            out.update(dd=r["dd"], aa=r["aa"], sai=r["sai"])
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print(name, nx, ny, "phi", r["phi"])
+    m = map_inputs()
+    dx = oe_ref.map_state(**m)
+    np.savez_compressed(os.path.join(HERE, "map_small.npz"), doutdx=dx, **m)
+    print("map_small", dx.shape)
+
+
+if __name__ == "__main__":
+    main()

@@ -17,6 +17,12 @@
  *   radg_set_mode            (none: the reference's precision is fixed at compile time)
  *   radg_batch_*, radg_server_*  (none: device-resident batches and the serving loop
  *                            that replace nemesisPY/nemesis.py:4586-4664's process pool)
+ *   radg_batch_set_statemaps / _read_mapped  the caller's tail of the wavenumber loop,
+ *                            map2pro + map2xvec (radtran/cirsradg/cirsrtfg_wave.f:403-425,
+ *                            map2pro.f:49-84, map2xvec.f:41-54)
+ *   radg_oe_*                one Levenberg-Marquardt proposal and the cost function of
+ *                            nemesis/coreret.f:554-1068 (calc_gain_matrix.f, calcnextxn.f,
+ *                            calc_phiret.f, dinvertm.f) for many retrievals at once
  */
 #ifndef RADG_H
 #define RADG_H
@@ -202,6 +208,61 @@ int radg_batch_timing_stats(radg_batch *b, int *nlaunch, float *sum_ms3);
  * nodes; plus the number of spectra and of layer-kernel work items. */
 int radg_batch_info(radg_batch *b, double *alg_bytes3, int *nspectra, int *nitems_a);
 int radg_batch_destroy(radg_batch *b);
+/* ------------------------------------- layer -> profile -> state maps */
+/* Per request: the matrices cirsrtfg_wave.f:403-425 applies after each
+ * cirsradg_wave call (0-based axes, C order):
+ *   dam[nlayer][npro]  d(layer amount)/d(level vmr), the /LGRAD/ DAM of
+ *                      radtran/includes/laygrad.f (used for every gas)
+ *   dte[nlayer][npro]  d(layer T)/d(level T), /LGRAD/ DTE
+ *   xmap[nv][ngas+1][npro]  d(profile quantity)/d(state element), map2xvec.f's
+ *                      xmap(maxv, maxgas+2+maxcon, maxpro) without the dust and
+ *                      para-H2 rows
+ * nv = 0: no map for that request. */
+typedef struct radg_statemap {
+  int nv, npro;
+  const double *dam, *dte, *xmap;
+} radg_statemap;
+/* Attach maps[nreq] to a batch (copied to the device).  Every later launch
+ * then also forms, on the device and in FP64,
+ *   kk[nwave][nv] = sum_p path_weight[p] doutdx[.][p][.]
+ * and, when per_path is non-zero or the paths of a request do not share
+ * their layers, doutdx[nwave][npath][nv] -- the reference's gradients per
+ * path, doutdx = map2xvec(map2pro(doutputdq)) (cirsrtfg_wave.f:436-444). */
+int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_path);
+/* Copy request r's mapped Jacobians to the host (either pointer may be NULL). */
+int radg_batch_read_mapped(radg_batch *b, int r, double *doutdx, double *kk);
+/* Device addresses of request r's angle-weighted radiance wout[nwave] and
+ * mapped Jacobian kk[nwave][nv] (NULL without a map), valid until the batch
+ * is destroyed, for consumers that stay on the device (radg_oe_*). */
+int radg_batch_device_outputs(radg_batch *b, int r, const double **wout, const double **kk);
+
+/* ------------------------------------------------ optimal estimation */
+/* nret independent retrievals of nx state elements from ny measurements,
+ * everything in FP64.  Constants are copied to the device at creation:
+ *   y[nret][ny] measurements, se[nret][ny] their (diagonal) variances
+ *   (coreret.f:255-285), xa[nret][nx] a-priori states, sa the a-priori
+ *   covariances, [nret][nx][nx] or, with sa_shared, one [nx][nx] for all.
+ * Sa is inverted on the device by Cholesky (dinvertm JMOD=2 in
+ * coreret.f:225-250).  Needs nx <= 4096; state elements not retrieved are
+ * the caller's business (coreret.f:613-619 resets fixed ones). */
+typedef struct radg_oe radg_oe;
+int radg_oe_create(int nret, int nx, int ny, const double *y, const double *se,
+                   const double *xa, const double *sa, int sa_shared, radg_oe **out);
+/* calc_phiret.f for every retrieval at states xn[nret][nx] (host) with
+ * forward radiances yn_dev[r] -> [ny] (device): chisq = (yn - y)^T Se^-1
+ * (yn - y) and phi = chisq + (xn - xa)^T Sa^-1 (xn - xa).  chisq may be NULL. */
+int radg_oe_cost(radg_oe *h, const double *xn, const double *const *yn_dev, double *phi,
+                 double *chisq);
+/* One damped step per retrieval (coreret.f:609-632): from xn (host), the
+ * forward radiances yn_dev[r] and Jacobians kk_dev[r] -> [ny][nx] (device),
+ *   x_out = xa + dd (y - yn) - aa (xa - xn)          (calcnextxn.f)
+ * with dd = (Sa^-1 + K^T Se^-1 K)^-1 K^T Se^-1 and aa = dd K
+ * (calc_gain_matrix.f), computed as xa + H^-1 K^T Se^-1 [(y - yn) + K (xn - xa)]
+ * by one Cholesky factorisation and solve of H = Sa^-1 + K^T Se^-1 K, then
+ *   xn1 = xn + (x_out - xn) / (1 + lambda[r])   -> xn1[nret][nx] (host). */
+int radg_oe_propose(radg_oe *h, const double *xn, const double *const *yn_dev,
+                    const double *const *kk_dev, const double *lambda, double *xn1);
+int radg_oe_destroy(radg_oe *h);
 
 /* ------------------------------------------------------- serving loop */
 

@@ -28,6 +28,8 @@ EXPORTS = [
     "radg_batch_info", "radg_batch_destroy", "radg_batch_timing_stats",
     "radg_server_create", "radg_server_submit", "radg_server_cancel", "radg_server_wait",
     "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch", "radg_ktable_bf16",
+    "radg_batch_set_statemaps", "radg_batch_read_mapped", "radg_batch_device_outputs",
+    "radg_oe_create", "radg_oe_cost", "radg_oe_propose", "radg_oe_destroy",
 ]
 
 
@@ -95,6 +97,17 @@ def lib():
                                    C.POINTER(ip), C.POINTER(C.c_double)]
     L.radg_server_stats.argtypes = [vp, vp]
     L.radg_server_destroy.argtypes = [vp]
+    for name in ["radg_batch_set_statemaps", "radg_batch_read_mapped",
+                 "radg_batch_device_outputs", "radg_oe_create", "radg_oe_cost",
+                 "radg_oe_propose", "radg_oe_destroy"]:
+        getattr(L, name).restype = ip
+    L.radg_batch_set_statemaps.argtypes = [vp, vp, ip]
+    L.radg_batch_read_mapped.argtypes = [vp, ip, vp, vp]
+    L.radg_batch_device_outputs.argtypes = [vp, ip, C.POINTER(vp), C.POINTER(vp)]
+    L.radg_oe_create.argtypes = [ip, ip, ip, vp, vp, vp, vp, ip, C.POINTER(vp)]
+    L.radg_oe_cost.argtypes = [vp, vp, vp, vp, vp]
+    L.radg_oe_propose.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.radg_oe_destroy.argtypes = [vp]
     _lib = L
     return L
 

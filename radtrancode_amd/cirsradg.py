@@ -107,6 +107,12 @@ class _ReqArrays:
                                       self.vwave, self.vem, self.emis, self.pw)])
 
 
+class StateMap(C.Structure):
+    """``radg_statemap`` (include/radg.h)."""
+    _fields_ = [("nv", C.c_int), ("npro", C.c_int), ("dam", C.c_void_p), ("dte", C.c_void_p),
+                ("xmap", C.c_void_p)]
+
+
 def _requests(reqs):
     keep = [_ReqArrays(r) for r in reqs]
     arr = (Request * len(keep))(*[k.s for k in keep])
@@ -157,6 +163,43 @@ class Batch:
         p = lambda a: None if a is None else _vp(a)
         check(lib().radg_batch_read(self.h, int(r), p(out), p(gts), p(jac), p(wo), p(wg), p(wj)))
         return res
+
+    def set_statemaps(self, maps, per_path=False):
+        """Attach layer -> profile -> state maps (map2pro.f + map2xvec.f) to
+        the requests: ``maps[r]`` is None or a dict with ``dam``, ``dte``
+        [nlayer, npro] and ``xmap`` [nv, ngas+1, npro] (FP64)."""
+        n = len(self.reqs)
+        arr = (StateMap * n)()
+        keep = []
+        for r, m in enumerate(maps):
+            if m is None:
+                continue
+            dam, dte, xm = (np.ascontiguousarray(m[k], dtype=np.float64)
+                            for k in ("dam", "dte", "xmap"))
+            q = self.reqs[r]
+            if dam.shape != dte.shape or dam.shape[0] != q.nlayer or \
+                    xm.shape[1:] != (q.ngas + 1, dam.shape[1]):
+                raise ValueError(f"request {r}: statemap shapes do not match the request")
+            keep += [dam, dte, xm]
+            arr[r] = StateMap(xm.shape[0], dam.shape[1], dam.ctypes.data, dte.ctypes.data,
+                              xm.ctypes.data)
+        self._maps = [None if m is None else m["xmap"].shape[0] for m in maps]
+        check(lib().radg_batch_set_statemaps(self.h, C.cast(arr, C.c_void_p), int(per_path)))
+
+    def read_mapped(self, r: int, per_path=False):
+        """Request r's mapped Jacobian: ``kk`` [nwave, nv] (path-weighted) and,
+        if asked for, ``doutdx`` [nwave, npath, nv]."""
+        q, nv = self.reqs[r], self._maps[r]
+        kk = np.zeros((q.nwave, nv))
+        dx = np.zeros((q.nwave, q.npath, nv)) if per_path else None
+        check(lib().radg_batch_read_mapped(self.h, int(r), None if dx is None else _vp(dx), _vp(kk)))
+        return dict(kk=kk, doutdx=dx)
+
+    def device_outputs(self, r: int):
+        """Device addresses (ints) of request r's wout[nwave] and kk[nwave, nv]."""
+        w, k = C.c_void_p(), C.c_void_p()
+        check(lib().radg_batch_device_outputs(self.h, int(r), C.byref(w), C.byref(k)))
+        return w.value, k.value
 
     def timings(self):
         ms = np.zeros(3, dtype=np.float32)

@@ -29,6 +29,12 @@ hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n
                              double sc, hipStream_t st);
 hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, size_t lds, hipStream_t st);
 hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st);
+hipError_t radg_launch_gemm(const GemmProblem *probs_dev, int nprob, int maxM, int maxN,
+                            int a_fp32, hipStream_t st);
+hipError_t radg_launch_pathsum(const void *ps_dev, int nps, long long maxn, hipStream_t st);
+size_t radg_pathsum_bytes();
+void radg_pathsum_fill(void *dst, const double *in, const double *pw, double *out, int nwave,
+                       int npath, int n);
 
 namespace {
 
@@ -391,6 +397,10 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
 
 }  // namespace
 
+// Failures of the other sources (radg_oe.hip) land in the same per-thread
+// message radg_last_error() returns.
+int radg_fail_msg(int code, const char *msg) { return fail(code, "%s", msg); }
+
 // ============================================================ batches
 struct radg_batch {
   int mode = 0, nreq = 0;
@@ -425,6 +435,31 @@ struct radg_batch {
   size_t span_p0 = 0, span_p1 = 0, span_w0 = 0, span_w1 = 0;
   int fetch = 0;           // bit 0: weighted outputs, bit 1: per-path outputs
   char *hst = nullptr;     // pinned: [weighted span][per-path span]
+  // host copy of each request's path layers (0-based layinc, zero past nlays)
+  struct HostReq {
+    int L = 0, npath = 0, ngas = 0, nwave = 0;
+    std::vector<int> layinc, nlays;
+  };
+  std::vector<HostReq> hreq;
+  // layer -> profile -> state maps (map2pro.f:49-84, map2xvec.f:41-54), set
+  // by radg_batch_set_statemaps and applied after the quadrature of every
+  // launch: one GEMM per (request, path class, parameter) forms the combined
+  // matrix C[(j, k)][iv] = sum_level D_k[layinc(j)][level] xmap[iv][k][level],
+  // then kk = wjac C when all paths of a request share their layers (else
+  // the per-path products summed with the path weights)
+  struct SMap {
+    int nv = 0, npro = 0, ncls = 0, per_path = 0;
+    std::vector<int> pcls;           // path -> class
+    size_t o_cm = 0, o_kk = 0, o_dx = 0;
+  };
+  std::vector<SMap> sm;
+  char *smem = nullptr;
+  size_t smem_bytes = 0;
+  GemmProblem *d_pc = nullptr, *d_pk64 = nullptr, *d_pkT = nullptr;
+  void *d_ps = nullptr;
+  int n_pc = 0, n_pk64 = 0, n_pkT = 0, n_ps = 0;
+  int mx_pc[2] = {0, 0}, mx_pk64[2] = {0, 0}, mx_pkT[2] = {0, 0};
+  long long mx_ps = 0;
   ~radg_batch();
 };
 
@@ -505,6 +540,7 @@ BlockCache &host_cache() {
 }  // namespace
 
 radg_batch::~radg_batch() {
+  dev_cache().put(smem);
   dev_cache().put(dmem);
   host_cache().put(hst);
   for (auto &e : evs)
@@ -860,6 +896,18 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
     for (int i = 0; i < 3; i++) b->alg_bytes[i] += P.alg_bytes[i];
   }
+  b->hreq.resize(nreq);
+  for (int r = 0; r < nreq; r++) {
+    const Packed &P = pl.packs[r];
+    radg_batch::HostReq &hq = b->hreq[r];
+    hq.L = reqs[r].nlayer;
+    hq.npath = reqs[r].npath;
+    hq.ngas = reqs[r].ngas;
+    hq.nwave = reqs[r].nwave;
+    hq.layinc = P.layinc;
+    hq.nlays.resize(hq.npath);
+    for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
+  }
   HIPCHK(hipMemcpy(b->dmem, h.data(), h.size(), hipMemcpyHostToDevice));
   b->reqs = dr;
   KernelArgs &a = b->args;
@@ -961,6 +1009,12 @@ int radg_batch_launch(radg_batch *b, void *stream) {
   }
   HIPCHK(hipEventRecord(b->ev[2], st));
   HIPCHK(radg_launch_quadrature(b->args, fp64, st));
+  if (b->smem) {
+    HIPCHK(radg_launch_gemm(b->d_pc, b->n_pc, b->mx_pc[0], b->mx_pc[1], 0, st));
+    HIPCHK(radg_launch_gemm(b->d_pk64, b->n_pk64, b->mx_pk64[0], b->mx_pk64[1], 0, st));
+    HIPCHK(radg_launch_gemm(b->d_pkT, b->n_pkT, b->mx_pkT[0], b->mx_pkT[1], fp64 ? 0 : 1, st));
+    HIPCHK(radg_launch_pathsum(b->d_ps, b->n_ps, b->mx_ps, st));
+  }
   HIPCHK(hipEventRecord(b->ev[3], st));
   // queue the copies of the requested outputs to pinned host memory behind
   // the kernels, so they overlap the next batch's work
@@ -1090,6 +1144,215 @@ int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf, double
   if (wout) HIPCHK(hipMemcpy(wout, a.wout + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
   if (wgts) HIPCHK(hipMemcpy(wgts, a.wgts + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
   if (wjac) HIPCHK(hipMemcpy(wjac, a.wjac + d.wjac0, nwj * sizeof(double), hipMemcpyDeviceToHost));
+  return RADG_OK;
+}
+
+int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_path) {
+  if (!b || !maps) return fail(RADG_EINPUT, "null batch or maps");
+  if (b->launches > 0) HIPCHK(hipEventSynchronize(b->ev[3]));
+  dev_cache().put(b->smem);
+  b->smem = nullptr;
+  b->smem_bytes = 0;
+  b->sm.clear();
+  b->n_pc = b->n_pk64 = b->n_pkT = b->n_ps = 0;
+  const int nreq = b->nreq;
+  const int fp64 = b->mode == RADG_MODE_FP64;
+  const size_t tsz = fp64 ? 8 : 4;
+  std::vector<radg_batch::SMap> sm(nreq);
+  // ---- sizes and path classes
+  Layout lo;
+  std::vector<size_t> o_dam(nreq), o_dte(nreq), o_xm(nreq), o_rm(nreq);
+  int n_pc = 0, n_pk64 = 0, n_pkT = 0, n_ps = 0;
+  for (int r = 0; r < nreq; r++) {
+    const radg_statemap &mp = maps[r];
+    if (mp.nv == 0) continue;
+    const radg_batch::HostReq &hq = b->hreq[r];
+    const int L = hq.L, npar = hq.ngas + 1;
+    if (mp.nv < 0 || mp.npro < 1 || !mp.dam || !mp.dte || !mp.xmap)
+      return fail(RADG_EINPUT, "request %d: statemap needs nv >= 0, npro >= 1 and dam, dte, xmap", r);
+    radg_batch::SMap &S = sm[r];
+    S.nv = mp.nv;
+    S.npro = mp.npro;
+    S.pcls.assign(hq.npath, -1);
+    std::vector<int> rep;  // representative path of each class
+    for (int p = 0; p < hq.npath; p++) {
+      for (size_t c = 0; c < rep.size(); c++) {
+        const int q = rep[c];
+        if (hq.nlays[q] == hq.nlays[p] &&
+            std::equal(hq.layinc.begin() + (size_t)p * L, hq.layinc.begin() + (size_t)p * L + hq.nlays[p],
+                       hq.layinc.begin() + (size_t)q * L)) {
+          S.pcls[p] = (int)c;
+          break;
+        }
+      }
+      if (S.pcls[p] < 0) {
+        S.pcls[p] = (int)rep.size();
+        rep.push_back(p);
+      }
+    }
+    S.ncls = (int)rep.size();
+    S.per_path = (per_path || S.ncls > 1) ? 1 : 0;
+    o_dam[r] = lo.take<double>((size_t)L * mp.npro);
+    o_dte[r] = lo.take<double>((size_t)L * mp.npro);
+    o_xm[r] = lo.take<double>((size_t)mp.nv * npar * mp.npro);
+    o_rm[r] = lo.take<int>((size_t)S.ncls * L);
+    S.o_cm = lo.take<double>((size_t)S.ncls * L * npar * mp.nv);
+    S.o_kk = lo.take<double>((size_t)hq.nwave * mp.nv);
+    if (S.per_path) S.o_dx = lo.take<double>((size_t)hq.nwave * hq.npath * mp.nv);
+    n_pc += S.ncls * npar;
+    if (S.ncls == 1) n_pk64 += 1;
+    else n_ps += 1;
+    if (S.per_path) n_pkT += hq.npath;
+  }
+  const size_t o_pc = lo.take<GemmProblem>(n_pc), o_pk64 = lo.take<GemmProblem>(n_pk64),
+               o_pkT = lo.take<GemmProblem>(n_pkT);
+  const size_t o_ps = lo.take<char>((size_t)n_ps * radg_pathsum_bytes());
+  const size_t total = lo.off;
+  if (n_pc == 0) {
+    b->sm = sm;
+    return RADG_OK;
+  }
+  char *dm = static_cast<char *>(dev_cache().get(total));
+  if (!dm) return fail(RADG_EHIP, "device allocation of %zu bytes failed", total);
+  // ---- inputs and problem descriptors, staged on the host
+  std::vector<char> h(total, 0);
+  std::vector<GemmProblem> pc, pk64, pkT;
+  std::vector<char> ps((size_t)n_ps * radg_pathsum_bytes());
+  int mpc[2] = {0, 0}, mk64[2] = {0, 0}, mkT[2] = {0, 0};
+  long long mps = 0;
+  int ips = 0;
+  const KernelArgs &a = b->args;
+  auto gp = [](const void *A, const int *rm, const double *B, double *C, long long sAm,
+               long long sAk, long long sBk, long long sBn, long long sCm, long long sCn, int M,
+               int N, int K) {
+    GemmProblem P{};
+    P.A = A;
+    P.rowmapA = rm;
+    P.B = B;
+    P.scaleK = nullptr;
+    P.C = C;
+    P.sAm = sAm;
+    P.sAk = sAk;
+    P.sBk = sBk;
+    P.sBn = sBn;
+    P.sCm = sCm;
+    P.sCn = sCn;
+    P.M = M;
+    P.N = N;
+    P.K = K;
+    P.accumulate = 0;
+    P.lower_only = 0;
+    P.alpha = 1.0;
+    return P;
+  };
+  for (int r = 0; r < nreq; r++) {
+    const radg_batch::SMap &S = sm[r];
+    if (S.nv == 0) continue;
+    const radg_statemap &mp = maps[r];
+    const radg_batch::HostReq &hq = b->hreq[r];
+    const DevReq &d = b->reqs[r];
+    const int L = hq.L, npar = hq.ngas + 1, nv = S.nv, npro = S.npro;
+    std::memcpy(h.data() + o_dam[r], mp.dam, (size_t)L * npro * 8);
+    std::memcpy(h.data() + o_dte[r], mp.dte, (size_t)L * npro * 8);
+    std::memcpy(h.data() + o_xm[r], mp.xmap, (size_t)nv * npar * npro * 8);
+    int *rm = reinterpret_cast<int *>(h.data() + o_rm[r]);
+    std::vector<int> rep(S.ncls, -1);
+    for (int p = 0; p < hq.npath; p++)
+      if (rep[S.pcls[p]] < 0) rep[S.pcls[p]] = p;
+    for (int c = 0; c < S.ncls; c++)
+      for (int j = 0; j < L; j++)  // rows past the path's layers read level row 0; their Jacobian rows are 0
+        rm[(size_t)c * L + j] = j < hq.nlays[rep[c]] ? hq.layinc[(size_t)rep[c] * L + j] : 0;
+    const double *ddam = reinterpret_cast<const double *>(dm + o_dam[r]);
+    const double *ddte = reinterpret_cast<const double *>(dm + o_dte[r]);
+    const double *dxm = reinterpret_cast<const double *>(dm + o_xm[r]);
+    const int *drm = reinterpret_cast<const int *>(dm + o_rm[r]);
+    double *dcm = reinterpret_cast<double *>(dm + S.o_cm);
+    double *dkk = reinterpret_cast<double *>(dm + S.o_kk);
+    const long long csz = (long long)L * npar * nv;
+    for (int c = 0; c < S.ncls; c++)
+      for (int k = 0; k < npar; k++) {
+        pc.push_back(gp(k < hq.ngas ? ddam : ddte, drm + (size_t)c * L, dxm + (size_t)k * npro,
+                        dcm + c * csz + (long long)k * nv, npro, 1, 1, (long long)npar * npro,
+                        (long long)npar * nv, 1, L, nv, npro));
+        mpc[0] = std::max(mpc[0], L);
+        mpc[1] = std::max(mpc[1], nv);
+      }
+    if (S.ncls == 1) {
+      pk64.push_back(gp(a.wjac + d.wjac0, nullptr, dcm, dkk, (long long)L * npar, 1, nv, 1, nv, 1,
+                        hq.nwave, nv, L * npar));
+      mk64[0] = std::max(mk64[0], hq.nwave);
+      mk64[1] = std::max(mk64[1], nv);
+    }
+    if (S.per_path) {
+      double *ddx = reinterpret_cast<double *>(dm + S.o_dx);
+      for (int p = 0; p < hq.npath; p++) {
+        const char *ja = reinterpret_cast<const char *>(a.jac) +
+                         (size_t)(d.jac0 + (long long)p * L * npar) * tsz;
+        pkT.push_back(gp(ja, nullptr, dcm + S.pcls[p] * csz, ddx + (long long)p * nv,
+                         (long long)hq.npath * L * npar, 1, nv, 1, (long long)hq.npath * nv, 1,
+                         hq.nwave, nv, L * npar));
+      }
+      mkT[0] = std::max(mkT[0], hq.nwave);
+      mkT[1] = std::max(mkT[1], nv);
+      if (S.ncls > 1) {
+        radg_pathsum_fill(ps.data() + (size_t)ips * radg_pathsum_bytes(), ddx, a.pweight + d.path0,
+                          dkk, hq.nwave, hq.npath, nv);
+        ips++;
+        mps = std::max(mps, (long long)hq.nwave * nv);
+      }
+    }
+  }
+  std::memcpy(h.data() + o_pc, pc.data(), pc.size() * sizeof(GemmProblem));
+  if (!pk64.empty()) std::memcpy(h.data() + o_pk64, pk64.data(), pk64.size() * sizeof(GemmProblem));
+  if (!pkT.empty()) std::memcpy(h.data() + o_pkT, pkT.data(), pkT.size() * sizeof(GemmProblem));
+  if (!ps.empty()) std::memcpy(h.data() + o_ps, ps.data(), ps.size());
+  const hipError_t e = hipMemcpy(dm, h.data(), total, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dev_cache().put(dm);
+    return fail(RADG_EHIP, "statemap upload: %s", hipGetErrorString(e));
+  }
+  b->smem = dm;
+  b->smem_bytes = total;
+  b->sm = sm;
+  b->d_pc = reinterpret_cast<GemmProblem *>(dm + o_pc);
+  b->d_pk64 = reinterpret_cast<GemmProblem *>(dm + o_pk64);
+  b->d_pkT = reinterpret_cast<GemmProblem *>(dm + o_pkT);
+  b->d_ps = dm + o_ps;
+  b->n_pc = n_pc;
+  b->n_pk64 = n_pk64;
+  b->n_pkT = n_pkT;
+  b->n_ps = n_ps;
+  std::copy(mpc, mpc + 2, b->mx_pc);
+  std::copy(mk64, mk64 + 2, b->mx_pk64);
+  std::copy(mkT, mkT + 2, b->mx_pkT);
+  b->mx_ps = mps;
+  return RADG_OK;
+}
+
+int radg_batch_read_mapped(radg_batch *b, int r, double *doutdx, double *kk) {
+  if (!b || r < 0 || r >= b->nreq) return fail(RADG_EINPUT, "bad batch or request index");
+  if ((int)b->sm.size() != b->nreq || b->sm[r].nv == 0)
+    return fail(RADG_EINPUT, "request %d has no state map", r);
+  if (b->launches == 0) return fail(RADG_EINPUT, "batch was never launched");
+  const radg_batch::SMap &S = b->sm[r];
+  const radg_batch::HostReq &hq = b->hreq[r];
+  if (doutdx && !S.per_path)
+    return fail(RADG_EINPUT, "request %d: per-path mapped output was not requested", r);
+  HIPCHK(hipEventSynchronize(b->ev[3]));
+  if (kk) HIPCHK(hipMemcpy(kk, b->smem + S.o_kk, (size_t)hq.nwave * S.nv * 8, hipMemcpyDeviceToHost));
+  if (doutdx)
+    HIPCHK(hipMemcpy(doutdx, b->smem + S.o_dx, (size_t)hq.nwave * hq.npath * S.nv * 8,
+                     hipMemcpyDeviceToHost));
+  return RADG_OK;
+}
+
+int radg_batch_device_outputs(radg_batch *b, int r, const double **wout, const double **kk) {
+  if (!b || r < 0 || r >= b->nreq) return fail(RADG_EINPUT, "bad batch or request index");
+  if (wout) *wout = b->args.wout + b->reqs[r].wout0;
+  if (kk) {
+    const bool has = (int)b->sm.size() == b->nreq && b->sm[r].nv > 0;
+    *kk = has ? reinterpret_cast<const double *>(b->smem + b->sm[r].o_kk) : nullptr;
+  }
   return RADG_OK;
 }
 

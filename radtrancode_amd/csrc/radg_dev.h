@@ -105,3 +105,20 @@ struct KernelArgs {
   double *wout, *wgts, *wjac;
   int *err;                    // device error flag
 };
+
+// One product of a batched FP64 GEMM (radg_oe.hip):
+//   C(m, n) (+)= alpha * sum_k A(m, k) * B(k, n) * (scaleK ? scaleK[k] : 1)
+// with A(m, k) = A[(rowmapA ? rowmapA[m] : m) * sAm + k * sAk] (float or
+// double, fixed per launch), B(k, n) = B[k * sBk + n * sBn] and
+// C(m, n) = C[m * sCm + n * sCn].  lower_only skips output tiles that lie
+// wholly above the diagonal.
+struct GemmProblem {
+  const void *A;
+  const int *rowmapA;
+  const double *B;
+  const double *scaleK;
+  double *C;
+  long long sAm, sAk, sBk, sBn, sCm, sCn;
+  int M, N, K, accumulate, lower_only, pad;
+  double alpha;
+};

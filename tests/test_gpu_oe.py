@@ -1,0 +1,140 @@
+"""GPU parity of the Jacobian's consumers: the layer -> profile -> state maps
+(map2pro.f + map2xvec.f) against the oracle on the batch's own per-path
+Jacobians, and the optimal-estimation step (radg_oe_*) against the
+reference's own calc_gain_matrix / calcnextxn / calc_phiret (golden
+vectors).  FP64 throughout: tolerance 1e-9 relative."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oe as oracle_oe
+from radtrancode_amd import synth
+from tests.golden.make_golden_oe import oe_inputs
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
+
+
+def _dev(a):
+    import torch
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def _case(name):
+    z = np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False)
+    if "kk" in z.files:
+        inp = {k: z[k] for k in ("kk", "sa", "se_var", "xa", "xn", "y", "yn")}
+    else:
+        inp = oe_inputs(int(z["nx"]), int(z["ny"]), int(z["seed"]))
+    return inp, z
+
+
+@pytest.mark.parametrize("name", ["oe_small", "oe_wide", "oe_medium", "oe_cfg4"])
+def test_oe_step_against_reference(gpu_lib, name):
+    from radtrancode_amd.retrieval import OE
+    inp, z = _case(name)
+    print(f"oe case {name}", flush=True)
+    # three retrievals of the same problem with different brakes; the first
+    # (lambda = 0) is calcnextxn's x_out itself
+    lam = np.array([0.0, 1.0, 10.0])
+    n = len(lam)
+    kk = [_dev(inp["kk"]) for _ in range(n)]
+    yn = [_dev(inp["yn"]) for _ in range(n)]
+    oe = OE(np.tile(inp["y"], (n, 1)), np.tile(inp["se_var"], (n, 1)), np.tile(inp["xa"], (n, 1)),
+            inp["sa"], sa_shared=True)
+    xn = np.tile(inp["xn"], (n, 1))
+    phi, chi = oe.cost(xn, [t.data_ptr() for t in yn])
+    xn1 = oe.propose(xn, [t.data_ptr() for t in yn], [t.data_ptr() for t in kk], lam)
+    oe.close()
+    x_out = z["x_out"]
+    e = _rel(xn1[0], x_out)
+    print(f"  x_out rel err {e:.2e}, phi {phi[0]:.6g} vs {float(z['phi']):.6g}", flush=True)
+    assert e < 1e-9
+    for i in (1, 2):
+        assert _rel(xn1[i], inp["xn"] + (x_out - inp["xn"]) / (1 + lam[i])) < 1e-9
+    assert np.all(np.abs(phi / float(z["phi"]) - 1) < 1e-11)
+    assert np.all(np.abs(chi / float(z["chisq"]) - 1) < 1e-11)
+
+
+def test_oe_per_retrieval_apriori(gpu_lib):
+    """Unshared Sa: two retrievals with their own a-priori covariances."""
+    from radtrancode_amd.retrieval import OE
+    a, b = oe_inputs(24, 80, 21), oe_inputs(24, 80, 22)
+    oe = OE(np.stack([a["y"], b["y"]]), np.stack([a["se_var"], b["se_var"]]),
+            np.stack([a["xa"], b["xa"]]), np.stack([a["sa"], b["sa"]]), sa_shared=False)
+    kk = [_dev(a["kk"]), _dev(b["kk"])]
+    yn = [_dev(a["yn"]), _dev(b["yn"])]
+    xn = np.stack([a["xn"], b["xn"]])
+    xn1 = oe.propose(xn, [t.data_ptr() for t in yn], [t.data_ptr() for t in kk], np.zeros(2))
+    phi, _ = oe.cost(xn, [t.data_ptr() for t in yn])
+    oe.close()
+    for i, c in enumerate((a, b)):
+        r = oracle_oe.oe_step(**c)
+        assert _rel(xn1[i], r["x_out"]) < 1e-9
+        assert abs(phi[i] / r["phi"] - 1) < 1e-11
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["fp64", "mixed"])
+def test_statemaps_against_oracle(gpu_lib, mode):
+    """kk and the per-path doutdx of a batch equal map2xvec(map2pro(.)) of
+    the batch's own per-path Jacobians, wavenumber by wavenumber; request 1
+    has paths of different layers (separate path classes)."""
+    rng = np.random.default_rng(5)
+    tables, _, req0 = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    gpu_lib.register_ktables(tables)
+    req1 = synth.make_request(nlayer=30, nkgas=3, nwave=6, seed=9)
+    req1.nlayin = req1.nlayin.copy()
+    req1.nlayin[1] = 20
+    req1.layinc = req1.layinc.copy()
+    req1.layinc[1, :20] = req1.layinc[1, 10:]
+    reqs = [req0, req1]
+    maps = []
+    for q in reqs:
+        npro, nv = 31, 17
+        maps.append(dict(dam=rng.normal(size=(q.nlayer, npro)) * 1e20,
+                         dte=rng.normal(size=(q.nlayer, npro)),
+                         xmap=rng.normal(size=(nv, q.ngas + 1, npro))))
+    b = gpu_lib.Batch(reqs, mode=mode)
+    b.set_statemaps(maps, per_path=True)
+    b.launch()
+    b.sync()
+    for r, q in enumerate(reqs):
+        res = b.read(r, weighted=False)
+        mp = b.read_mapped(r, per_path=True)
+        for w in range(q.nwave):
+            dmod = oracle_oe.map2pro(res["doutputdq"][w], q.nlayin, q.layinc, maps[r]["dam"],
+                                     maps[r]["dte"], q.ngas)
+            dx = oracle_oe.map2xvec(dmod, maps[r]["xmap"])
+            assert _rel(mp["doutdx"][w], dx) < 1e-9, (r, w)
+            assert _rel(mp["kk"][w], q.path_weight @ dx) < 1e-9, (r, w)
+    b.close()
+
+
+def test_small_retrieval_reduces_cost_and_error(gpu_lib):
+    """Four retrievals on 60 wavenumbers: every cost falls from the a priori
+    and the retrieved temperatures move toward the truth."""
+    from radtrancode_amd import retrieval
+    tables, _, _ = synth.config1(nwave=60, nlayer=120, nkgas=3, ng=20)
+    gpu_lib.register_ktables(tables)
+    prob = retrieval.retrieval_problem(nret=4, nwave=60, seed=44)
+    y, se = retrieval.measure(prob)
+    print("retrieval: measured", flush=True)
+    out = retrieval.retrieve(prob, y, se, kiter=6, log=lambda s: print(s, flush=True))
+    oe = retrieval.OE(y, se, prob.xa, prob.sa, sa_shared=True)
+    b = retrieval.forward(prob, prob.xa, range(4))
+    b.sync()
+    phi0, _ = oe.cost(prob.xa, [b.device_outputs(i)[0] for i in range(4)])
+    oe.close()
+    b.close()
+    print("phi a priori", phi0, "final", out["phi"], flush=True)
+    assert np.all(out["phi"] < phi0)
+    nlev = prob.nlev
+    e0 = np.abs(prob.xa[:, :nlev] - prob.x_true[:, :nlev]).mean()
+    e1 = np.abs(out["x"][:, :nlev] - prob.x_true[:, :nlev]).mean()
+    print(f"mean |T - T_true|: a priori {e0:.3f} K, retrieved {e1:.3f} K", flush=True)
+    assert e1 < e0

@@ -1196,9 +1196,6 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     o_dte[r] = lo.take<double>((size_t)L * mp.npro);
     o_xm[r] = lo.take<double>((size_t)mp.nv * npar * mp.npro);
     o_rm[r] = lo.take<int>((size_t)S.ncls * L);
-    S.o_cm = lo.take<double>((size_t)S.ncls * L * npar * mp.nv);
-    S.o_kk = lo.take<double>((size_t)hq.nwave * mp.nv);
-    if (S.per_path) S.o_dx = lo.take<double>((size_t)hq.nwave * hq.npath * mp.nv);
     n_pc += S.ncls * npar;
     if (S.ncls == 1) n_pk64 += 1;
     else n_ps += 1;
@@ -1207,6 +1204,18 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
   const size_t o_pc = lo.take<GemmProblem>(n_pc), o_pk64 = lo.take<GemmProblem>(n_pk64),
                o_pkT = lo.take<GemmProblem>(n_pkT);
   const size_t o_ps = lo.take<char>((size_t)n_ps * radg_pathsum_bytes());
+  // the inputs and descriptors above are uploaded; the products below are
+  // written on the device
+  const size_t staged = lo.off;
+  for (int r = 0; r < nreq; r++) {
+    radg_batch::SMap &S = sm[r];
+    if (S.nv == 0) continue;
+    const radg_batch::HostReq &hq = b->hreq[r];
+    const int L = hq.L, npar = hq.ngas + 1;
+    S.o_cm = lo.take<double>((size_t)S.ncls * L * npar * S.nv);
+    S.o_kk = lo.take<double>((size_t)hq.nwave * S.nv);
+    if (S.per_path) S.o_dx = lo.take<double>((size_t)hq.nwave * hq.npath * S.nv);
+  }
   const size_t total = lo.off;
   if (n_pc == 0) {
     b->sm = sm;
@@ -1215,7 +1224,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
   char *dm = static_cast<char *>(dev_cache().get(total));
   if (!dm) return fail(RADG_EHIP, "device allocation of %zu bytes failed", total);
   // ---- inputs and problem descriptors, staged on the host
-  std::vector<char> h(total, 0);
+  std::vector<char> h(staged, 0);
   std::vector<GemmProblem> pc, pk64, pkT;
   std::vector<char> ps((size_t)n_ps * radg_pathsum_bytes());
   int mpc[2] = {0, 0}, mk64[2] = {0, 0}, mkT[2] = {0, 0};
@@ -1306,7 +1315,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
   if (!pk64.empty()) std::memcpy(h.data() + o_pk64, pk64.data(), pk64.size() * sizeof(GemmProblem));
   if (!pkT.empty()) std::memcpy(h.data() + o_pkT, pkT.data(), pkT.size() * sizeof(GemmProblem));
   if (!ps.empty()) std::memcpy(h.data() + o_ps, ps.data(), ps.size());
-  const hipError_t e = hipMemcpy(dm, h.data(), total, hipMemcpyHostToDevice);
+  const hipError_t e = hipMemcpy(dm, h.data(), staged, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     dev_cache().put(dm);
     return fail(RADG_EHIP, "statemap upload: %s", hipGetErrorString(e));

@@ -142,10 +142,10 @@ def state_request(prob: Problem, r: int, x):
 
 
 def retrieval_problem(nret=512, nwave=1000, seed=40, t_sigma=5.0, lnvmr_sigma=0.5,
-                      corr_scale_heights=1.5, tables_nwave=None):
+                      corr_scale_heights=1.5):
     """The batched-retrieval workload on the 120-layer, 3 k-gas + H2/He,
-    600-1400 cm-1 atmosphere of :func:`synth.config1` (tables from
-    ``synth.config1`` with ``tables_nwave`` points):
+    600-1400 cm-1 atmosphere of :func:`synth.config1` (register the tables of
+    ``synth.config1(nwave=nwave)`` first):
 
     * state: T and ln VMR of k-gases 1 and 2 on the 121 levels (nx = 363);
     * truth: a seeded perturbation of the a priori drawn from Sa;
@@ -185,9 +185,7 @@ def retrieval_problem(nret=512, nwave=1000, seed=40, t_sigma=5.0, lnvmr_sigma=0.
                    xa=np.array(xa), sa=sa, x_true=np.array(xt), rgas=(0, 1))
 
 
-def forward(prob: Problem, xs, idx, mode=rad.MODE_MIXED):
-    """One device batch: the forward models and state maps of retrievals
-    ``idx`` at states ``xs[idx]``, launched; returns the batch."""
+def _chunk_batch(prob, xs, idx, mode):
     reqs, maps = [], []
     for r in idx:
         q, m = state_request(prob, r, xs[r])
@@ -199,18 +197,36 @@ def forward(prob: Problem, xs, idx, mode=rad.MODE_MIXED):
     return b
 
 
+_POOL = None
+
+
+def forward(prob: Problem, xs, idx, mode=rad.MODE_MIXED, chunk=16, workers=8):
+    """The forward models and state maps of retrievals ``idx`` at states
+    ``xs[idx]``: device batches of ``chunk`` requests, packed by a pool of
+    host threads and each launched as soon as it is packed, so packing
+    overlaps the kernels of the batches before it.  Returns the list of
+    (batch, index in batch) for each entry of ``idx``."""
+    global _POOL
+    from concurrent.futures import ThreadPoolExecutor
+    if _POOL is None:
+        _POOL = ThreadPoolExecutor(max_workers=workers)
+    idx = list(idx)
+    parts = [idx[i:i + chunk] for i in range(0, len(idx), chunk)]
+    futs = [_POOL.submit(_chunk_batch, prob, xs, p, mode) for p in parts]
+    out = []
+    for p, f in zip(parts, futs):
+        b = f.result()
+        out += [(b, i) for i in range(len(p))]
+    return out
+
+
 def measure(prob: Problem, mode=rad.MODE_MIXED, noise=0.01, seed=41):
     """y = F(x_true) (1 + noise N(0, 1)), se = (noise F(x_true))^2."""
     rng = np.random.default_rng(seed)
     nret = prob.xa.shape[0]
-    ys = []
-    for c0 in range(0, nret, 64):
-        idx = list(range(c0, min(nret, c0 + 64)))
-        b = forward(prob, prob.x_true, idx, mode)
-        b.sync()
-        ys += [b.read(i)["wout"] for i in range(len(idx))]
-        b.close()
-    y = np.array(ys)
+    held = forward(prob, prob.x_true, range(nret), mode)
+    y = np.array([b.read(i)["wout"] for b, i in held])
+    del held
     se = (noise * y) ** 2
     return y * (1.0 + noise * rng.normal(size=y.shape)), se
 
@@ -225,9 +241,13 @@ def retrieve(prob: Problem, y, se, kiter=10, phlimit=0.1, mode=rad.MODE_MIXED, l
     oe = OE(y, se, prob.xa, prob.sa, sa_shared=True)
     xn = prob.xa.copy()
     t = dict(forward=0.0, propose=0.0, cost=0.0)
+    def settle(held):
+        for bb in {id(x[0]): x[0] for x in held}.values():
+            bb.sync()
+
     t0 = time.perf_counter()
-    b = forward(prob, xn, range(nret), mode)
-    cur = [(b, i) for i in range(nret)]         # batch holding retrieval r's yn, kk
+    cur = forward(prob, xn, range(nret), mode)  # batch holding retrieval r's yn, kk
+    settle(cur)
     t["forward"] += time.perf_counter() - t0
 
     def ptrs(state):
@@ -259,10 +279,11 @@ def retrieve(prob: Problem, y, se, kiter=10, phlimit=0.1, mode=rad.MODE_MIXED, l
         if not run:
             continue
         t0 = time.perf_counter()
-        bn = forward(prob, xn1, run, mode)
+        held = forward(prob, xn1, run, mode)
+        settle(held)
         trial = list(cur)
-        for i, r in enumerate(run):
-            trial[r] = (bn, i)
+        for r, h in zip(run, held):
+            trial[r] = h
         t["forward"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         wp, _ = ptrs(trial)
@@ -294,7 +315,7 @@ def retrieve(prob: Problem, y, se, kiter=10, phlimit=0.1, mode=rad.MODE_MIXED, l
         history.append(float(np.median(ophi)))
         if log:
             log(f"iteration {it}: running {len(run)}, median phi {history[-1]:.4g}, "
-                f"accepted {int(sum(cur[r][0] is bn for r in run))}")
+                f"accepted {int(sum(cur[r] is trial[r] for r in run))}")
     out = dict(x=xn, phi=ophi, iters=iters, converged=done, history=history, timings=t)
     oe.close()
     return out

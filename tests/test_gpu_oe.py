@@ -126,11 +126,10 @@ def test_small_retrieval_reduces_cost_and_error(gpu_lib):
     print("retrieval: measured", flush=True)
     out = retrieval.retrieve(prob, y, se, kiter=6, log=lambda s: print(s, flush=True))
     oe = retrieval.OE(y, se, prob.xa, prob.sa, sa_shared=True)
-    b = retrieval.forward(prob, prob.xa, range(4))
-    b.sync()
-    phi0, _ = oe.cost(prob.xa, [b.device_outputs(i)[0] for i in range(4)])
+    held = retrieval.forward(prob, prob.xa, range(4))
+    phi0, _ = oe.cost(prob.xa, [b.device_outputs(i)[0] for b, i in held])
     oe.close()
-    b.close()
+    del held
     print("phi a priori", phi0, "final", out["phi"], flush=True)
     assert np.all(out["phi"] < phi0)
     nlev = prob.nlev

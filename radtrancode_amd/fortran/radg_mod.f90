@@ -16,6 +16,31 @@ module radg
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
+  public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
+  public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
+  public :: radg_batch_device_outputs
+  public :: radg_oe_create, radg_oe_cost, radg_oe_propose, radg_oe_destroy
+
+  ! radg_request of include/radg.h: sizes, then C addresses (c_loc) of the
+  ! caller's arrays in the compact 0-based layout the header documents.
+  type, bind(C), public :: radg_request
+    integer(c_int) :: nlayer, npath, ngas, nsw, nwave, nem
+    integer(c_int) :: inormal, iray, ispace, itype, flagh2p
+    real(c_double) :: tsurf
+    type(c_ptr) :: press, temp, delh, pp, amount
+    type(c_ptr) :: idgas, isogas, nlayin, imod, layinc
+    type(c_ptr) :: scale, emtemp
+    type(c_ptr) :: isw
+    type(c_ptr) :: vwave, vem, emissivity, path_weight
+  end type
+
+  ! radg_statemap: the /LGRAD/ DAM and DTE and map2xvec's xmap of one request
+  ! (DAM(nlayer,npro) in C order is dam(npro,nlayer) in Fortran order, and
+  ! xmap(nv,ngas+1,npro) in C order is xmap(npro,ngas+1,nv)).
+  type, bind(C), public :: radg_statemap
+    integer(c_int) :: nv, npro
+    type(c_ptr) :: dam, dte, xmap
+  end type
 
   interface
     integer(c_int) function radg_set_mode(mode) bind(C, name="radg_set_mode")
@@ -56,6 +81,94 @@ module radg
       integer(c_int) :: inormal, iray, nlayer, npath, ngas, ispace, nlayin(*), layinc(*)
       integer(c_int) :: imod(*), idgas(*), isogas(*), itype, nem, flagh2p, nsw, isw(*)
       integer(c_int), value :: nwave
+    end function
+
+    ! ---- device-resident batches (include/radg.h)
+    integer(c_int) function radg_batch_create(nreq, reqs, mode, budget, b) &
+        bind(C, name="radg_batch_create")
+      import :: c_int, c_size_t, c_ptr, radg_request
+      integer(c_int), value :: nreq, mode
+      type(radg_request) :: reqs(*)
+      integer(c_size_t), value :: budget
+      type(c_ptr) :: b
+    end function
+
+    integer(c_int) function radg_batch_launch(b, stream) bind(C, name="radg_batch_launch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, stream
+    end function
+
+    integer(c_int) function radg_batch_sync(b) bind(C, name="radg_batch_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    integer(c_int) function radg_batch_read(b, r, output, gtsurf, doutputdq, wout, wgts, wjac) &
+        bind(C, name="radg_batch_read")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, output, gtsurf, doutputdq, wout, wgts, wjac
+      integer(c_int), value :: r
+    end function
+
+    integer(c_int) function radg_batch_destroy(b) bind(C, name="radg_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    ! ---- map2pro + map2xvec on the device (cirsrtfg_wave.f:403-425)
+    integer(c_int) function radg_batch_set_statemaps(b, maps, per_path) &
+        bind(C, name="radg_batch_set_statemaps")
+      import :: c_int, c_ptr, radg_statemap
+      type(c_ptr), value :: b
+      type(radg_statemap) :: maps(*)
+      integer(c_int), value :: per_path
+    end function
+
+    integer(c_int) function radg_batch_read_mapped(b, r, doutdx, kk) &
+        bind(C, name="radg_batch_read_mapped")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, doutdx, kk
+      integer(c_int), value :: r
+    end function
+
+    integer(c_int) function radg_batch_device_outputs(b, r, wout, kk) &
+        bind(C, name="radg_batch_device_outputs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: r
+      type(c_ptr) :: wout, kk
+    end function
+
+    ! ---- optimal estimation for many retrievals (coreret.f:554-1068).
+    ! Fortran order: y(ny,nret), se(ny,nret), xa(nx,nret), sa(nx,nx[,nret]),
+    ! xn(nx,nret); yn_dev and kk_dev hold device addresses per retrieval
+    ! (radg_batch_device_outputs).
+    integer(c_int) function radg_oe_create(nret, nx, ny, y, se, xa, sa, sa_shared, h) &
+        bind(C, name="radg_oe_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nret, nx, ny, sa_shared
+      real(c_double) :: y(*), se(*), xa(*), sa(*)
+      type(c_ptr) :: h
+    end function
+
+    integer(c_int) function radg_oe_cost(h, xn, yn_dev, phi, chisq) bind(C, name="radg_oe_cost")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double) :: xn(*), phi(*), chisq(*)
+      type(c_ptr) :: yn_dev(*)
+    end function
+
+    integer(c_int) function radg_oe_propose(h, xn, yn_dev, kk_dev, lambda, xn1) &
+        bind(C, name="radg_oe_propose")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double) :: xn(*), lambda(*), xn1(*)
+      type(c_ptr) :: yn_dev(*), kk_dev(*)
+    end function
+
+    integer(c_int) function radg_oe_destroy(h) bind(C, name="radg_oe_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
     end function
   end interface
 

@@ -17,6 +17,7 @@
 
 #include "../../include/radg.h"
 #include "radg_dev.h"
+#include "radg_internal.h"
 
 // launchers (radg_kernels.hip)
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64);
@@ -425,6 +426,7 @@ struct radg_batch {
   std::vector<EmClass> em;
   double alg_bytes[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
+  bool stream_used = false;
   // event ring: 4 events per launch, the last RING launches are kept
   static constexpr int RING = 128;
   std::vector<hipEvent_t> evs;
@@ -441,6 +443,11 @@ struct radg_batch {
     std::vector<int> layinc, nlays;
   };
   std::vector<HostReq> hreq;
+  // pinned staging of the inputs (and of the state maps), uploaded by the
+  // next launch on its stream
+  char *hup = nullptr, *sup = nullptr;
+  size_t hup_bytes = 0, sup_bytes = 0;
+  bool up_pending = false, sup_pending = false;
   // layer -> profile -> state maps (map2pro.f:49-84, map2xvec.f:41-54), set
   // by radg_batch_set_statemaps and applied after the quadrature of every
   // launch: one GEMM per (request, path class, parameter) forms the combined
@@ -477,7 +484,20 @@ struct BlockCache {
   size_t cached = 0, cap = 0;
   bool pinned = false;
   BlockCache(size_t c, bool p) : cap(c), pinned(p) {}
-  void *get(size_t n) {
+  // Sizes are rounded up to classes (powers of two to 256 MiB, then steps of
+  // 128 MiB) so that batches of different sizes reuse each other's blocks.
+  static size_t size_class(size_t n) {
+    const size_t big = (size_t)256 << 20;
+    if (n <= big) {
+      size_t c = 4096;
+      while (c < n) c <<= 1;
+      return c;
+    }
+    const size_t step = (size_t)128 << 20;
+    return (n + step - 1) / step * step;
+  }
+  void *get(size_t n0) {
+    const size_t n = size_class(n0);
     {
       std::lock_guard<std::mutex> lk(mu);
       auto it = freed.lower_bound(n);
@@ -540,6 +560,9 @@ BlockCache &host_cache() {
 }  // namespace
 
 radg_batch::~radg_batch() {
+  if (launches > 0 && stream_used) (void)hipStreamSynchronize(stream);
+  host_cache().put(hup);
+  host_cache().put(sup);
   dev_cache().put(smem);
   dev_cache().put(dmem);
   host_cache().put(hst);
@@ -666,8 +689,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
 }
 
 template <typename X>
-void put(std::vector<char> &h, size_t off, const X *src, size_t n) {
-  if (n) std::memcpy(h.data() + off, src, n * sizeof(X));
+void put(char *h, size_t off, const X *src, size_t n) {
+  if (n) std::memcpy(h + off, src, n * sizeof(X));
 }
 
 }  // namespace
@@ -857,8 +880,16 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   b->span_p1 = pl.o_jac + (size_t)pl.njac * tsz;
   b->span_w0 = pl.o_wout;
   b->span_w1 = pl.o_wjac + (size_t)pl.nwjac * 8;
-  // host staging of everything but the intermediates and outputs
-  std::vector<char> h(pl.o_inter, 0);
+  // Host staging of everything but the intermediates and outputs, in pinned
+  // memory.  Nothing here touches the device: the staged inputs go up with
+  // the first launch, on the launch's stream, so that packing a batch never
+  // waits for the kernels of batches already running.
+  b->hup = static_cast<char *>(host_cache().get(pl.o_inter));
+  if (!b->hup) return fail(RADG_EHIP, "pinned allocation of %zu bytes failed", pl.o_inter);
+  b->hup_bytes = pl.o_inter;
+  b->up_pending = true;
+  char *h = b->hup;
+  std::memset(h, 0, pl.o_inter);
   std::vector<DevReq> dr(nreq);
   for (int r = 0; r < nreq; r++) dr[r] = pl.packs[r].d;
   put(h, pl.o_reqs, dr.data(), nreq);
@@ -908,7 +939,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     hq.nlays.resize(hq.npath);
     for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
   }
-  HIPCHK(hipMemcpy(b->dmem, h.data(), h.size(), hipMemcpyHostToDevice));
+
   b->reqs = dr;
   KernelArgs &a = b->args;
   char *m = b->dmem;
@@ -996,6 +1027,15 @@ int radg_batch_launch(radg_batch *b, void *stream) {
   for (int i = 0; i < 4; i++)
     if (!b->ev[i]) HIPCHK(hipEventCreate(&b->ev[i]));
   b->launches++;
+  b->stream_used = true;
+  if (b->up_pending) {
+    HIPCHK(hipMemcpyAsync(b->dmem, b->hup, b->hup_bytes, hipMemcpyHostToDevice, st));
+    b->up_pending = false;
+  }
+  if (b->sup_pending) {
+    HIPCHK(hipMemcpyAsync(b->smem, b->sup, b->sup_bytes, hipMemcpyHostToDevice, st));
+    b->sup_pending = false;
+  }
   HIPCHK(hipEventRecord(b->ev[0], st));
   HIPCHK(radg_launch_layer_lds(b->args, b->mode, b->E, b->ldsA, st));
   HIPCHK(hipEventRecord(b->ev[1], st));
@@ -1147,6 +1187,46 @@ int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf, double
   return RADG_OK;
 }
 
+}  // extern "C"
+
+int radg_batch_pinned_view(radg_batch *b, int r, PinnedView *v) {
+  if (!b || !v || r < 0 || r >= b->nreq) return fail(RADG_EINPUT, "bad batch/request index");
+  if (!b->hst || b->launches == 0) return fail(RADG_EINPUT, "batch has no fetched results");
+  const DevReq &d = b->reqs[r];
+  const bool fp64 = b->mode == RADG_MODE_FP64;
+  const size_t ts = fp64 ? 8 : 4;
+  const size_t wlen = b->span_w1 - b->span_w0;
+  *v = PinnedView{};
+  v->fp64 = fp64 ? 1 : 0;
+  v->nwave = d.nwave;
+  v->npath = d.npath;
+  v->L = d.L;
+  v->nparam = d.nparam;
+  if (b->fetch & 1) {
+    v->wout = reinterpret_cast<const double *>(b->hst + (b->o_wout - b->span_w0)) + d.wout0;
+    v->wgts = reinterpret_cast<const double *>(b->hst + (b->o_wgts - b->span_w0)) + d.wout0;
+    v->wjac = reinterpret_cast<const double *>(b->hst + (b->o_wjac - b->span_w0)) + d.wjac0;
+  }
+  if (b->fetch & 2) {
+    const char *pb = b->hst + wlen;
+    v->out = pb + (b->o_out - b->span_p0) + (size_t)d.out0 * ts;
+    v->gts = pb + (b->o_gts - b->span_p0) + (size_t)d.out0 * ts;
+    v->jac = pb + (b->o_jac - b->span_p0) + (size_t)d.jac0 * ts;
+  }
+  return RADG_OK;
+}
+
+std::shared_ptr<char> radg_batch_take_pinned(radg_batch *b) {
+  char *p = b ? b->hst : nullptr;
+  if (b) {
+    b->hst = nullptr;
+    b->fetch = 0;
+  }
+  return std::shared_ptr<char>(p, [](char *q) { host_cache().put(q); });
+}
+
+extern "C" {
+
 int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_path) {
   if (!b || !maps) return fail(RADG_EINPUT, "null batch or maps");
   if (b->launches > 0) HIPCHK(hipEventSynchronize(b->ev[3]));
@@ -1223,8 +1303,17 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
   }
   char *dm = static_cast<char *>(dev_cache().get(total));
   if (!dm) return fail(RADG_EHIP, "device allocation of %zu bytes failed", total);
-  // ---- inputs and problem descriptors, staged on the host
-  std::vector<char> h(staged, 0);
+  // ---- inputs and problem descriptors, staged in pinned memory and
+  // uploaded by the next launch
+  host_cache().put(b->sup);
+  b->sup = static_cast<char *>(host_cache().get(staged));
+  if (!b->sup) {
+    dev_cache().put(dm);
+    return fail(RADG_EHIP, "pinned allocation of %zu bytes failed", staged);
+  }
+  b->sup_bytes = staged;
+  char *hs = b->sup;
+  std::memset(hs, 0, staged);
   std::vector<GemmProblem> pc, pk64, pkT;
   std::vector<char> ps((size_t)n_ps * radg_pathsum_bytes());
   int mpc[2] = {0, 0}, mk64[2] = {0, 0}, mkT[2] = {0, 0};
@@ -1261,10 +1350,10 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     const radg_batch::HostReq &hq = b->hreq[r];
     const DevReq &d = b->reqs[r];
     const int L = hq.L, npar = hq.ngas + 1, nv = S.nv, npro = S.npro;
-    std::memcpy(h.data() + o_dam[r], mp.dam, (size_t)L * npro * 8);
-    std::memcpy(h.data() + o_dte[r], mp.dte, (size_t)L * npro * 8);
-    std::memcpy(h.data() + o_xm[r], mp.xmap, (size_t)nv * npar * npro * 8);
-    int *rm = reinterpret_cast<int *>(h.data() + o_rm[r]);
+    std::memcpy(hs + o_dam[r], mp.dam, (size_t)L * npro * 8);
+    std::memcpy(hs + o_dte[r], mp.dte, (size_t)L * npro * 8);
+    std::memcpy(hs + o_xm[r], mp.xmap, (size_t)nv * npar * npro * 8);
+    int *rm = reinterpret_cast<int *>(hs + o_rm[r]);
     std::vector<int> rep(S.ncls, -1);
     for (int p = 0; p < hq.npath; p++)
       if (rep[S.pcls[p]] < 0) rep[S.pcls[p]] = p;
@@ -1311,15 +1400,11 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
       }
     }
   }
-  std::memcpy(h.data() + o_pc, pc.data(), pc.size() * sizeof(GemmProblem));
-  if (!pk64.empty()) std::memcpy(h.data() + o_pk64, pk64.data(), pk64.size() * sizeof(GemmProblem));
-  if (!pkT.empty()) std::memcpy(h.data() + o_pkT, pkT.data(), pkT.size() * sizeof(GemmProblem));
-  if (!ps.empty()) std::memcpy(h.data() + o_ps, ps.data(), ps.size());
-  const hipError_t e = hipMemcpy(dm, h.data(), staged, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    dev_cache().put(dm);
-    return fail(RADG_EHIP, "statemap upload: %s", hipGetErrorString(e));
-  }
+  std::memcpy(hs + o_pc, pc.data(), pc.size() * sizeof(GemmProblem));
+  if (!pk64.empty()) std::memcpy(hs + o_pk64, pk64.data(), pk64.size() * sizeof(GemmProblem));
+  if (!pkT.empty()) std::memcpy(hs + o_pkT, pkT.data(), pkT.size() * sizeof(GemmProblem));
+  if (!ps.empty()) std::memcpy(hs + o_ps, ps.data(), ps.size());
+  b->sup_pending = true;
   b->smem = dm;
   b->smem_bytes = total;
   b->sm = sm;

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/radg.h"
+#include "radg_internal.h"
 
 namespace {
 
@@ -81,16 +82,6 @@ struct OwnedRequest {
   }
 };
 
-// Uninitialised result buffer (results are written over it in full).
-struct Buf {
-  std::unique_ptr<double[]> p;
-  size_t n = 0;
-  void reset(size_t m) {
-    p.reset(new double[m]);
-    n = m;
-  }
-};
-
 struct Ticket {
   long long id = 0;
   std::unique_ptr<OwnedRequest> req;
@@ -98,7 +89,10 @@ struct Ticket {
   int status = -1;  // -1 pending, else a RADG status
   bool cancelled = false;
   std::string err;
-  Buf out, gts, jac, wout, wgts, wjac;
+  // results: a view into the pinned copy of the finished batch, which the
+  // tickets of that batch share until the last of them is collected
+  std::shared_ptr<char> block;
+  PinnedView view;
   clk::time_point t_submit, t_admit, t_done;
 };
 
@@ -141,36 +135,29 @@ void retire(radg_server *s, Inflight &f) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->us_gpu += (long long)(1e3 * (ms3[0] + ms3[1] + ms3[2]));
   }
-  // results for every request of a finished batch
+  // results for every request of a finished batch: views into the batch's
+  // pinned copy, which outlives the batch's device memory
+  std::vector<PinnedView> views(f.ids.size());
+  std::vector<int> rcs(f.ids.size());
+  std::vector<std::string> errs(f.ids.size());
   for (size_t r = 0; r < f.ids.size(); r++) {
-    std::shared_ptr<Ticket> t;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      auto it = s->tickets.find(f.ids[r]);
-      if (it == s->tickets.end()) continue;
-      t = it->second;
-    }
-    const radg_request &q = t->req->q;
-    const size_t nw = q.nwave, P = q.npath, L = q.nlayer, np = q.ngas + 1;
-    double *po = nullptr, *pg = nullptr, *pj = nullptr;
-    if (s->fetch & 2) {
-      t->out.reset(nw * P);
-      t->gts.reset(nw * P);
-      t->jac.reset(nw * P * L * np);
-      po = t->out.p.get();
-      pg = t->gts.p.get();
-      pj = t->jac.p.get();
-    }
-    t->wout.reset(nw);
-    t->wgts.reset(nw);
-    t->wjac.reset(nw * L * np);
-    const int rc = radg_batch_read(f.b, (int)r, po, pg, pj, t->wout.p.get(), t->wgts.p.get(),
-                                   t->wjac.p.get());
+    rcs[r] = radg_batch_pinned_view(f.b, (int)r, &views[r]);
+    if (rcs[r]) errs[r] = radg_last_error();
+  }
+  std::shared_ptr<char> block = radg_batch_take_pinned(f.b);
+  for (size_t r = 0; r < f.ids.size(); r++) {
     std::lock_guard<std::mutex> lk(s->mu);
-    t->status = rc;
-    if (rc) t->err = radg_last_error();
-    t->t_done = clk::now();
-    t->req.reset();  // inputs no longer needed
+    auto it = s->tickets.find(f.ids[r]);
+    if (it == s->tickets.end()) continue;
+    Ticket &t = *it->second;
+    t.status = rcs[r];
+    if (rcs[r]) t.err = errs[r];
+    else {
+      t.block = block;
+      t.view = views[r];
+    }
+    t.t_done = clk::now();
+    t.req.reset();  // inputs no longer needed
     s->completed++;
   }
   radg_batch_destroy(f.b);
@@ -187,6 +174,7 @@ void retire(radg_server *s, Inflight &f) {
 
 void worker_loop(radg_server *s) {
   (void)hipSetDevice(s->device);
+  Inflight ready;  // a batch packed ahead of a free slot (no stream yet)
   while (s->running.load()) {
     bool progressed = false;
     // retire finished batches
@@ -196,14 +184,9 @@ void worker_loop(radg_server *s) {
         progressed = true;
       }
     }
-    // admit queued requests into a free slot
-    Inflight *slot = nullptr;
-    for (auto &f : s->slots)
-      if (!f.b) {
-        slot = &f;
-        break;
-      }
-    if (slot) {
+    // pack the next batch from the queue (also while every slot is busy, so
+    // that a freed slot is refilled at once)
+    if (!ready.b) {
       std::vector<std::shared_ptr<Ticket>> take;
       size_t bytes = 0;
       {
@@ -226,15 +209,14 @@ void worker_loop(radg_server *s) {
       if (!take.empty()) {
         std::vector<radg_request> qs;
         for (auto &t : take) qs.push_back(t->req->q);
-        radg_batch *b = nullptr;
+        radg_batch *nb = nullptr;
         const auto tc0 = clk::now();
-        int rc = radg_batch_create((int)qs.size(), qs.data(), s->mode, 0, &b);
-        if (rc == RADG_OK) rc = radg_batch_set_fetch(b, 1 | (s->fetch & 2));
-        if (rc == RADG_OK) rc = radg_batch_launch(b, slot->stream);
+        int rc = radg_batch_create((int)qs.size(), qs.data(), s->mode, 0, &nb);
+        if (rc == RADG_OK) rc = radg_batch_set_fetch(nb, 1 | (s->fetch & 2));
         const auto now = clk::now();
         if (rc != RADG_OK) {
           const std::string msg = radg_last_error();
-          if (b) radg_batch_destroy(b);
+          if (nb) radg_batch_destroy(nb);
           std::lock_guard<std::mutex> lk(s->mu);
           for (auto &t : take) {
             t->status = rc;
@@ -244,26 +226,60 @@ void worker_loop(radg_server *s) {
           s->inflight_bytes -= bytes;
           s->cv_done.notify_all();
         } else {
-          slot->b = b;
-          slot->bytes = bytes;
-          for (auto &t : take) {
-            slot->ids.push_back(t->id);
-            t->t_admit = now;
-          }
+          ready.b = nb;
+          ready.bytes = bytes;
+          for (auto &t : take) ready.ids.push_back(t->id);
           std::lock_guard<std::mutex> lk(s->mu);
           s->us_create += std::chrono::duration_cast<std::chrono::microseconds>(now - tc0).count();
-          s->steps++;
-          s->admitted += (long long)take.size();
-          s->max_batch_seen = std::max<long long>(s->max_batch_seen, (long long)take.size());
         }
         progressed = true;
       }
+    }
+    // launch the packed batch into a free slot
+    Inflight *slot = nullptr;
+    for (auto &f : s->slots)
+      if (!f.b) {
+        slot = &f;
+        break;
+      }
+    if (slot && ready.b) {
+      const int rc = radg_batch_launch(ready.b, slot->stream);
+      const auto now = clk::now();
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (rc != RADG_OK) {
+        const std::string msg = radg_last_error();
+        for (long long id : ready.ids) {
+          auto it = s->tickets.find(id);
+          if (it == s->tickets.end()) continue;
+          it->second->status = rc;
+          it->second->err = msg;
+          it->second->t_done = now;
+        }
+        radg_batch_destroy(ready.b);
+        s->inflight_bytes -= ready.bytes;
+        s->cv_done.notify_all();
+      } else {
+        slot->b = ready.b;
+        slot->bytes = ready.bytes;
+        slot->ids = ready.ids;
+        for (long long id : ready.ids) {
+          auto it = s->tickets.find(id);
+          if (it != s->tickets.end()) it->second->t_admit = now;
+        }
+        s->steps++;
+        s->admitted += (long long)ready.ids.size();
+        s->max_batch_seen = std::max<long long>(s->max_batch_seen, (long long)ready.ids.size());
+      }
+      ready.b = nullptr;
+      ready.ids.clear();
+      ready.bytes = 0;
+      progressed = true;
     }
     if (!progressed) {
       std::unique_lock<std::mutex> lk(s->mu);
       bool busy = false;
       for (auto &f : s->slots) busy |= (f.b != nullptr);
-      if (busy || !s->queue.empty())
+      if (busy || ready.b || !s->queue.empty())
         s->cv_work.wait_for(lk, std::chrono::microseconds(50));
       else
         s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
@@ -274,6 +290,18 @@ void worker_loop(radg_server *s) {
       (void)hipStreamSynchronize(f.stream);
       retire(s, f);
     }
+  if (ready.b) {  // packed but never launched: fail its requests
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (long long id : ready.ids) {
+      auto it = s->tickets.find(id);
+      if (it == s->tickets.end()) continue;
+      it->second->status = RADG_ECANCELLED;
+      it->second->err = "server destroyed before launch";
+      it->second->t_done = clk::now();
+    }
+    radg_batch_destroy(ready.b);
+    s->cv_done.notify_all();
+  }
 }
 
 }  // namespace
@@ -372,15 +400,26 @@ int radg_server_wait(radg_server *s, long long ticket, double timeout_s, double 
   if (status) *status = t->status;
   if (latency_s) *latency_s = std::chrono::duration<double>(t->t_done - t->t_submit).count();
   if (t->status != RADG_OK) return RADG_OK;
-  auto cp = [](double *dst, const Buf &v) {
-    if (dst && v.n) std::memcpy(dst, v.p.get(), v.n * sizeof(double));
+  const PinnedView &v = t->view;
+  const size_t nw = (size_t)v.nwave, nout = nw * v.npath;
+  const size_t nwj = nw * v.L * v.nparam, nj = nout * v.L * v.nparam;
+  auto cpw = [](double *dst, const double *src, size_t n) {
+    if (dst && src) std::memcpy(dst, src, n * sizeof(double));
   };
-  cp(output, t->out);
-  cp(gtsurf, t->gts);
-  cp(doutputdq, t->jac);
-  cp(wout, t->wout);
-  cp(wgts, t->wgts);
-  cp(wjac, t->wjac);
+  auto cpp = [&](double *dst, const void *src, size_t n) {
+    if (!dst || !src) return;
+    if (v.fp64) std::memcpy(dst, src, n * sizeof(double));
+    else {
+      const float *f = static_cast<const float *>(src);
+      for (size_t i = 0; i < n; i++) dst[i] = f[i];
+    }
+  };
+  cpw(wout, v.wout, nw);
+  cpw(wgts, v.wgts, nw);
+  cpw(wjac, v.wjac, nwj);
+  cpp(output, v.out, nout);
+  cpp(gtsurf, v.gts, nout);
+  cpp(doutputdq, v.jac, nj);
   return RADG_OK;
 }
 

@@ -34,6 +34,8 @@ def lib():
         _lib.orc_planck.argtypes = [C.c_int, C.c_double, C.c_double]
         _lib.orc_planckg.restype = C.c_double
         _lib.orc_planckg.argtypes = [C.c_int, C.c_double, C.c_double]
+        _lib.orc_rayleigh.restype = C.c_double
+        _lib.orc_rayleigh.argtypes = [C.c_int] + [C.c_double] * 6
         _lib.orc_locate.restype = C.c_int
         _lib.orc_locate.argtypes = [dp, C.c_int, C.c_double]
     return _lib
@@ -143,6 +145,12 @@ def noverlapg(delg, amount, k_gn, dkgndt):
 
 def planck(ispace, v, t):
     return lib().orc_planck(int(ispace), float(v), float(t))
+
+
+def rayleigh(iray, v, p=1.0, t=273.15, fheh2=0.0, fch4h2=0.0, fnh3=0.0):
+    """Rayleigh cross-section (cm2) of rayleighj/v/a/ls.f for iray 1..4."""
+    return lib().orc_rayleigh(int(iray), float(v), float(p), float(t), float(fheh2),
+                              float(fch4h2), float(fnh3))
 
 
 def planckg(ispace, v, t):

@@ -15,6 +15,8 @@
  *   ciaread     radtran/radtran/ciaread.f:104-196
  *   nciacon     radtran/radtran/nciacon.f:1-310
  *   co2cia      radtran/radtran/co2cia.f
+ *   rayleigh*   radtran/scatter/rayleighj.f, rayleighv.f, rayleigha.f,
+ *               rayleighls.f (the iray branch, cirsradg_wave.f:663-691)
  *   planck      radtran/radtran/planck_wave.f:1-42, planckg_wave.f:1-40
  *   interpem    radtran/cirsrad/interpem.f:1-37
  *   cirsradg    radtran/cirsradg/cirsradg_wave.f:333-1290 (imod 0, 1, 3)
@@ -465,6 +467,80 @@ static double orc_co2cia(double v0) {
   return a;
 }
 
+/* ---------------------------------------------------------------- Rayleigh */
+/* Rayleigh cross-section per molecule (cm2) at wavenumber v (cm-1), with the
+ * reference's pi literal (PI_REF: rayleigha.f:21, rayleighj.f:22 and
+ * constdef.f:11 all write 3.1415927).
+ *   iray 1  rayleighj.f:1-52   H2/He mixture (fH2 = 0.864), no anisotropy
+ *   iray 2  rayleighv.f:1-29   C / lambda^4, C = 8.8e-28
+ *   iray 3  rayleigha.f:1-52   Earth air with the (P, T) refractivity
+ *                              correction and faniso(0.035) (rayleigha.f:55-66)
+ *   iray 4  rayleighls.f:1-71  H2, He, CH4, NH3 mixture from the ratios
+ *                              He/H2, CH4/H2 and the NH3 fraction */
+static double orc_faniso(double delta) { return (6.0 + 3.0 * delta) / (6.0 - 7.0 * delta); }
+
+double orc_rayleigh(int iray, double v, double p, double t, double fheh2, double fch4h2,
+                    double fnh3) {
+  const double pi = PI_REF;
+  if (iray == 1) {
+    const double ah2 = 13.58e-5, bh2 = 7.52e-3, ahe = 3.48e-5, bhe = 2.30e-3, fh2 = 0.864;
+    const double k = 1.37971e-23, p0 = 1.013e5, t0 = 273.15;
+    double lambda = 1e-2 / v;           /* m */
+    double x = 1.0 / (lambda * 1e6);    /* micron^-1 */
+    double nh2 = ah2 * (1.0 + bh2 * x * x);
+    double nhe = ahe * (1.0 + bhe * x * x);
+    double nair = fh2 * nh2 + (1 - fh2) * nhe;
+    double temp = 32 * (pi * pi * pi) * nair * nair;
+    double n0 = p0 / (k * t0);
+    x = n0 * lambda * lambda;
+    return temp * 1e4 * orc_faniso(0.0) / (3 * (x * x));
+  }
+  if (iray == 2) {
+    double lambda = 10000 / v;          /* micron */
+    lambda = lambda * lambda * lambda * lambda;
+    return 8.8e-28 / lambda;
+  }
+  if (iray == 3) {
+    const double c1 = 8342.12, c2 = 2406030.0, c3 = 15997.0, delta = 0.035, k = 1.37971e-23;
+    double lambda = 1e4 / v;            /* micron */
+    double nu2 = 1.0 / (lambda * lambda);
+    double m1 = c1 + c2 / (130.0 - nu2) + c3 / (38.9 - nu2);
+    double p1, t1, xcorr, ns, x, temp;
+    m1 = m1 * 1e-8;
+    p1 = p * 1.013e5;                   /* Pa */
+    t1 = t - 273.15;                    /* C */
+    xcorr = p1 * (1 + p1 * (61.3 - t1) * 1e-10) / (96095.4 * (1 + 0.003661 * t1));
+    m1 = m1 * xcorr;
+    ns = p1 / (k * t);
+    lambda = lambda * 1e-6;             /* m */
+    x = ns * lambda * lambda;
+    temp = 8 * (pi * pi * pi) * ((m1 * (m1 + 2.0)) * (m1 * (m1 + 2.0)));
+    return temp * 1e4 * orc_faniso(delta) / (3 * (x * x));
+  }
+  {
+    /* rayleighls.f: A, B, D for H2, He, CH4, NH3 */
+    static const double a[4] = {13.58e-5, 3.48e-5, 37.0e-5, 37.0e-5};
+    static const double b[4] = {7.52e-3, 2.3e-3, 12.0e-3, 12.0e-3};
+    static const double d[4] = {0.0221, 0.025, .0922, .0922};
+    double comp[4], h2overtot, loschpm3, wl, xc1 = 0.0, sumwt = 0.0, fact;
+    int j;
+    h2overtot = (1.0 - fnh3) / (1 + fheh2 + fch4h2);
+    comp[0] = h2overtot;
+    comp[1] = fheh2 * h2overtot;
+    comp[2] = fch4h2 * h2overtot;
+    comp[3] = fnh3;
+    loschpm3 = 2.687e19 * 1e-12;
+    wl = 1e4 / v;
+    for (j = 0; j < 4; j++) {
+      double nr = 1.0 + a[j] * (1.0 + b[j] / (wl * wl));
+      xc1 = xc1 + (nr * nr - 1.0) * (nr * nr - 1.0) * comp[j] * (6 + 3 * d[j]) / (6 - 7 * d[j]);
+      sumwt = sumwt + comp[j];
+    }
+    fact = 8.0 * (pi * pi * pi) / (3.0 * (wl * wl * wl * wl) * (loschpm3 * loschpm3));
+    return fact * 1e-8 * xc1 / sumwt;
+  }
+}
+
 /* ---------------------------------------------------------------- nciacon */
 /* nciacon.f:1-310.  iabs[5] are 1-based gas numbers (0 = absent),
  * dabsorb[7].  N2 (id 22) is rejected by the caller, so the analytic N2-N2
@@ -719,6 +795,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
                       double *output, double *gtsurf, double *doutputdq) {
   const int L = r->nlayer, ngas = r->ngas, npath = r->npath;
   int ng = 0, nparam, ip, j, k, k1, l, ig, rc = ORC_OK;
+  int jh2 = -1, jhe = -1, jch4 = -1, jnh3 = -1;
   const double *delg = NULL;
   const orc_ktable *gtab[MAXGAS];
   double esurf = 0.0, vv = vwave;
@@ -738,7 +815,6 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
     if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3) return ORC_ERR_UNSUPPORTED;
   }
   if (r->flagh2p == 1) return ORC_ERR_UNSUPPORTED;               /* nparacon_all */
-  if (r->iray > 0 && r->itype != 0) return ORC_ERR_UNSUPPORTED;  /* Rayleigh */
   for (k = 0; k < ngas; k++)
     if (r->idgas[k] == 22) return ORC_ERR_UNSUPPORTED;          /* N2 analytic CIA */
 
@@ -761,6 +837,14 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
   if (ng < 1 || ng > MAXG) return ORC_ERR_LIMITS;
 
   nparam = ngas + 1;
+  /* cirsradg_wave.f:393-414: the last H2, He, CH4, NH3 (isotope 0 or 1) */
+  for (k = 0; k < ngas; k++) {
+    if (r->isogas[k] != 0 && r->isogas[k] != 1) continue;
+    if (r->idgas[k] == 39) jh2 = k;
+    if (r->idgas[k] == 40) jhe = k;
+    if (r->idgas[k] == 6) jch4 = k;
+    if (r->idgas[k] == 11) jnh3 = k;
+  }
   rc = orc_interpem(r->nem, r->vem, r->emissivity, vwave, &esurf);
   if (rc != ORC_OK && r->tsurf > 0.0) return rc;
   rc = ORC_OK;
@@ -828,6 +912,29 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       if (kk > 0) dtaucondq[j * nparam + kk - 1] += dabsorb[l] / (utotl[j] * 1.0e20);
     }
     dtaucondq[j * nparam + ngas] += dabsorb[5];
+
+    /* Rayleigh, cirsradg_wave.f:663-691: an extinction utotl * xray in the
+     * continuum and xray * 1e-20 in every gas's d(tau)/d(amount); its P, T
+     * and composition derivatives are neglected there and here. */
+    if (r->iray > 0 && r->itype != 0) {
+      double xray;
+      if (r->iray <= 3) {
+        xray = orc_rayleigh(r->iray, vv, r->press[j], r->temp[j], 0.0, 0.0, 0.0) * 1e20;
+      } else {
+        /* FRAC = pp / press (:418-426) of the gases picked at :393-414; a
+         * gas that is absent leaves the reference's FH2/FHE/FCH4/FNH3 unset,
+         * and the restatement takes 0 for it. */
+        const double fh2 = jh2 >= 0 ? r->pp[j * ngas + jh2] / r->press[j] : 0.0;
+        const double fhe = jhe >= 0 ? r->pp[j * ngas + jhe] / r->press[j] : 0.0;
+        const double fch4 = jch4 >= 0 ? r->pp[j * ngas + jch4] / r->press[j] : 0.0;
+        const double fnh3 = jnh3 >= 0 ? r->pp[j * ngas + jnh3] / r->press[j] : 0.0;
+        const double fheh2 = fh2 > 0.0 ? fhe / fh2 : 0.0;
+        const double fch4h2 = fh2 > 0.0 ? fch4 / fh2 : 0.0;
+        xray = orc_rayleigh(4, vv, r->press[j], r->temp[j], fheh2, fch4h2, fnh3) * 1e20;
+      }
+      taucon[j] += utotl[j] * xray;
+      for (k = 0; k < ngas; k++) dtaucondq[j * nparam + k] += xray * 1.0e-20;
+    }
 
     for (k = 0; k < ngas; k++) {
       amo[k] = r->amount[j * ngas + k] * 1.0e-20;

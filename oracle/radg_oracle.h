@@ -91,6 +91,8 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t,
                  int inormal, int ngas, const int *idgas, const int *isogas,
                  const double *amount, const double *pp, double xlen,
                  double *absorb, int *iabs, double *dabsorb);       /* nciacon.f */
+double orc_rayleigh(int iray, double v, double p, double t, double fheh2,
+                    double fch4h2, double fnh3);                    /* rayleigh*.f */
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,19 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def rayleigh(prec, iray, v, p=1.0, t=273.15, fheh2=0.0, fch4h2=0.0, fnh3=0.0):
+    """The reference's own RAYLEIGHJ/V/A (v, P, T) or RAYLEIGHLS (v, He/H2,
+    CH4/H2, NH3 fraction) from the built library (``radtran/scatter/``)."""
+    lb = _load(prec)
+    ct = C.c_double if prec == 64 else C.c_float
+    name = {1: "rayleighj_", 2: "rayleighv_", 3: "rayleigha_"}.get(iray, "rayleighls_")
+    fn = getattr(lb, name)
+    fn.restype = ct
+    args = (v, p, t) if iray <= 3 else (v, fheh2, fch4h2, fnh3)
+    held = [ct(x) for x in args]
+    return float(fn(*[C.byref(h) for h in held]))
+
+
 class RefRunner:
     """One loaded reference library with tables registered for a request.
 

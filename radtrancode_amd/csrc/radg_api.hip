@@ -181,7 +181,6 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   if (ngas < 1 || ngas > RADG_MAXGAS) return fail(RADG_ELIMITS, "ngas=%d outside 1..%d", ngas, RADG_MAXGAS);
   if (nwave < 1) return fail(RADG_ELIMITS, "nwave=%d", nwave);
   if (q.flagh2p == 1) return fail(RADG_EUNSUPPORTED, "flagh2p=1 (para-H2 CIA, nparacon_all) is not provided");
-  if (q.iray > 0 && q.itype != 0) return fail(RADG_EUNSUPPORTED, "Rayleigh scattering (iray>0, itype!=0) is not provided");
   if (q.ispace != 0 && q.ispace != 1) return fail(RADG_EINPUT, "ispace=%d", q.ispace);
   const int nparam = ngas + 1;
   if (q.nsw < 0 || q.nsw > nparam) return fail(RADG_ELIMITS, "nsw=%d > nparam=%d", q.nsw, nparam);
@@ -223,10 +222,24 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   d.nwave = nwave;
   d.ispace = q.ispace;
   d.inormal = q.inormal;
+  // cirsradg_wave.f:663: the Rayleigh branch runs for iray > 0 and itype != 0;
+  // iray 1, 2, 3 pick rayleighj/v/a, any larger value rayleighls.
+  d.iray = (q.iray > 0 && q.itype != 0) ? std::min(q.iray, 4) : 0;
   d.tsurf = q.tsurf;
   d.tsurf_le0 = q.tsurf <= 0.0;
   for (int k = 0; k < RADG_MAXPARAM; k++) d.isw[k] = k < q.nsw ? q.isw[k] - 1 : -1;
   for (int l = 0; l < 5; l++) d.iabs[l] = -1;
+
+  // cirsradg_wave.f:393-414: the gases whose fractions feed rayleighls (the
+  // last H2, He, CH4, NH3 of isotope 0 or 1); an absent one contributes 0.
+  int jh2 = -1, jhe = -1, jch4 = -1, jnh3 = -1;
+  for (int k = 0; k < ngas; k++) {
+    if (q.isogas[k] != 0 && q.isogas[k] != 1) continue;
+    if (q.idgas[k] == 39) jh2 = k;
+    if (q.idgas[k] == 40) jhe = k;
+    if (q.idgas[k] == 6) jch4 = k;
+    if (q.idgas[k] == 11) jnh3 = k;
+  }
 
   // ---- per layer
   const int np = ts.np, nt = ts.nt, NG = ts.ng;
@@ -259,6 +272,16 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
     }
     c.utotl = ut * 1.e-20;
     c.temp = q.temp[j];
+    c.press = q.press[j];
+    {
+      const double *ppj = q.pp + (size_t)j * ngas;
+      const double fh2 = jh2 >= 0 ? ppj[jh2] / q.press[j] : 0.0;
+      const double fhe = jhe >= 0 ? ppj[jhe] / q.press[j] : 0.0;
+      const double fch4 = jch4 >= 0 ? ppj[jch4] / q.press[j] : 0.0;
+      c.fnh3 = jnh3 >= 0 ? ppj[jnh3] / q.press[j] : 0.0;
+      c.fheh2 = fh2 > 0.0 ? fhe / fh2 : 0.0;
+      c.fch4h2 = fh2 > 0.0 ? fch4 / fh2 : 0.0;
+    }
     // nciacon.f:65-176
     const double p = q.press[j], t = q.temp[j];
     const double *pp = q.pp + (size_t)j * ngas;

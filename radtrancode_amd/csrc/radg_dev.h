@@ -14,7 +14,8 @@
 
 struct DevReq {
   int L, ngas, npath, nparam, nsw, nwave;
-  int ispace, tsurf_le0, inormal, pad0;
+  int ispace, tsurf_le0, inormal;
+  int iray;                    // Rayleigh form 1..4, 0 = off (iray > 0 and itype != 0)
   double tsurf;
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
@@ -41,6 +42,8 @@ struct LayerConst {
   double utotl;          // sum of amounts * 1e-20
   double temp;           // K
   double qh2, qhe, qn2, qch4, qco2, tau;
+  double press;          // atm
+  double fheh2, fch4h2, fnh3;  // rayleighls mixture, cirsradg_wave.f:520-523,671-677
 };
 
 // Table slice(s) used by one gas at one wavenumber.

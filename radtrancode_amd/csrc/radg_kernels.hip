@@ -512,6 +512,57 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
   for (int i = 0; i < 7; i++) dab[i] *= lc.tau;
 }
 
+// Rayleigh cross-section per molecule (cm2) at wavenumber vv (cm-1), in FP64
+// for every form; pi is the reference's literal 3.1415927.
+//   1: rayleighj.f (H2/He mixture, fH2 = 0.864, isotropic)
+//   2: rayleighv.f (8.8e-28 / lambda_um^4)
+//   3: rayleigha.f (air, refractivity corrected to the layer's P and T,
+//      depolarisation factor faniso(0.035))
+//   4: rayleighls.f (H2, He, CH4, NH3 in the layer's proportions)
+__device__ __forceinline__ double rayleigh_xs(int iray, double vv, const LayerConst &lc) {
+  const double pi3 = 3.1415927 * 3.1415927 * 3.1415927;
+  if (iray == 2) {
+    const double lum = 10000 / vv;
+    return 8.8e-28 / ((lum * lum) * (lum * lum));
+  }
+  if (iray == 1) {
+    const double lm = 1e-2 / vv;         // m
+    const double x2 = (vv * 1e-4) * (vv * 1e-4);  // micron^-2
+    const double n = 0.864 * (13.58e-5 * (1.0 + 7.52e-3 * x2)) +
+                     (1 - 0.864) * (3.48e-5 * (1.0 + 2.30e-3 * x2));
+    const double nl = (1.013e5 / (1.37971e-23 * 273.15)) * lm * lm;
+    return 32 * pi3 * n * n * 1e4 / (3 * (nl * nl));
+  }
+  if (iray == 3) {
+    const double lum = 1e4 / vv;
+    const double nu2 = 1.0 / (lum * lum);
+    const double p1 = lc.press * 1.013e5, t1 = lc.temp - 273.15;
+    const double xcorr = p1 * (1 + p1 * (61.3 - t1) * 1e-10) / (96095.4 * (1 + 0.003661 * t1));
+    const double m = (8342.12 + 2406030.0 / (130.0 - nu2) + 15997.0 / (38.9 - nu2)) * 1e-8 * xcorr;
+    const double lm = lum * 1e-6;
+    const double nl = p1 / (1.37971e-23 * lc.temp) * lm * lm;
+    const double mm = m * (m + 2.0);
+    const double fan = (6.0 + 3.0 * 0.035) / (6.0 - 7.0 * 0.035);
+    return 8 * pi3 * (mm * mm) * 1e4 * fan / (3 * (nl * nl));
+  }
+  const double wl = 1e4 / vv, iwl2 = 1.0 / (wl * wl);
+  const double h2 = (1.0 - lc.fnh3) / (1 + lc.fheh2 + lc.fch4h2);
+  const double comp[4] = {h2, lc.fheh2 * h2, lc.fch4h2 * h2, lc.fnh3};
+  const double A[4] = {13.58e-5, 3.48e-5, 37.0e-5, 37.0e-5};
+  const double B[4] = {7.52e-3, 2.3e-3, 12.0e-3, 12.0e-3};
+  const double D[4] = {0.0221, 0.025, .0922, .0922};
+  double s = 0.0, w = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double nr = 1.0 + A[i] * (1.0 + B[i] * iwl2);
+    const double e = nr * nr - 1.0;
+    s += e * e * comp[i] * (6 + 3 * D[i]) / (6 - 7 * D[i]);
+    w += comp[i];
+  }
+  const double lo = 2.687e19 * 1e-12;
+  return 8.0 * pi3 / (3.0 * ((wl * wl) * (wl * wl)) * (lo * lo)) * 1e-8 * s / w;
+}
+
 }  // namespace
 
 // ============================================================ layer kernel
@@ -1036,11 +1087,20 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   double absorb, dab[7];
   const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
   cia_layer(a, vv, R.inormal, lc, &absorb, dab);
+  // Rayleigh (cirsradg_wave.f:663-691): utotl * xray more extinction and
+  // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.
+  double dray = 0.0;
+  if (R.iray > 0) {
+    const double xray = rayleigh_xs(R.iray, vv, lc) * 1e20;
+    absorb += lc.utotl * xray;
+    dray = xray * 1.0e-20;
+  }
   for (int s = 0; s <= ngas; s++) {
     double d = 0.0;
     for (int l = 0; l < 5; l++)
       if (R.iabs[l] == s) d += dab[l] / lc.utotl;
     if (s == ngas) d += dab[5];
+    else d += dray;
     dtcon[(long long)s * L + j] = (T)d;
   }
   tcon[j] = (T)absorb;

@@ -3,7 +3,8 @@
 Run in a checkout where ``/root/reference`` is present::
 
     make -C oracle ref
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py rayleigh_j # only the named ones
 
 Each case stores its inputs (request arrays and k-tables) and the outputs of
 the reference compiled with REAL promoted to 8 bytes (``ref64_*``: the FP64
@@ -106,11 +107,67 @@ def cases():
     r.scale[3] = 1.7
     yield "partial_paths", t, r, "a half-depth path (surface test fails) and an odd scale"
 
+    # Rayleigh (iray > 0, itype = 11 as forwardavfovX sets it).  Above the
+    # CIA table's 1500 cm-1 the H2 and He Jacobian columns are Rayleigh alone.
+    # At 1450 cm-1 (rayleigh_cia) the CIA terms dominate and Rayleigh adds
+    # to their derivatives.
+    for iray, tag, seed, v0 in ((1, "j", 11, 5000.0), (2, "v", 12, 5200.0),
+                                (3, "a", 13, 4800.0), (4, "ls", 14, 5400.0),
+                                (4, "cia", 15, 1450.0)):
+        t, r = rayleigh_inputs(iray, seed, v0)
+        yield f"rayleigh_{tag}", t, r, (f"iray={iray} itype=11 at {v0:.0f} cm-1"
+                                        + (", CH4 and NH3 present" if iray == 4 else "")
+                                        + (", CIA overlapping" if v0 < 1500 else ", no CIA"))
 
-def main():
+
+def rayleigh_inputs(iray, seed, v0, nkgas=3, nlayer=30, ng=10, nwave=6, delv=2.0,
+                    np_=10, nt=6):
+    """A 30-layer request at ``v0`` with Rayleigh on.  For iray=4 the second
+    k-gas is CH4 (id 6), so H2, He, CH4 and NH3 (id 11) all set the mixture
+    of rayleighls.f."""
+    ids = list(synth.KGAS_IDS[:nkgas])
+    if iray == 4:
+        ids[1] = 6
+    t = synth.make_ktables(ids, v0, delv, nwave, ng=ng, np_=np_, nt=nt, seed=seed)
+    r = synth.make_request(nlayer=nlayer, nkgas=nkgas, nwave=nwave, seed=seed,
+                           table_vmin=v0, table_delv=delv)
+    r.idgas = np.array(ids + [synth.H2_ID, synth.HE_ID], dtype=np.int32)
+    r.iray = iray
+    r.itype = 11
+    return t, r
+
+
+RAYLEIGH_FN = {1: "rayleighj", 2: "rayleighv", 3: "rayleigha", 4: "rayleighls"}
+
+
+def rayleigh_vectors(n=64, seed=7):
+    """The reference's own RAYLEIGHJ/V/A/LS on log-spaced wavenumbers
+    (100..30000 cm-1), pressures (1e-4..100 atm), temperatures (50..800 K)
+    and mixtures (He/H2, CH4/H2, NH3 fraction), REAL(8) and REAL(4) builds."""
+    rng = np.random.default_rng(seed)
+    v = np.exp(rng.uniform(np.log(100.0), np.log(30000.0), n))
+    p = np.exp(rng.uniform(np.log(1e-4), np.log(100.0), n))
+    t = rng.uniform(50.0, 800.0, n)
+    mix = rng.uniform(0.0, 0.3, (n, 3))
+    out = dict(v=v, p=p, t=t, mix=mix)
+    for iray in (1, 2, 3, 4):
+        for prec in (64, 32):
+            out[f"ref{prec}_iray{iray}"] = np.array(
+                [ref.rayleigh(prec, iray, v[i], p[i], t[i], *mix[i]) for i in range(n)])
+    return out
+
+
+def main(names=()):
+    """Regenerate every fixture, or only the named cases (``rayleigh`` is the
+    function-level Rayleigh vectors)."""
     cia = synth.make_cia()
-    save_cia(os.path.join(HERE, "cia.npz"), cia)
+    if not names:
+        save_cia(os.path.join(HERE, "cia.npz"), cia)
+    if not names or "rayleigh" in names:
+        np.savez_compressed(os.path.join(HERE, "rayleigh.npz"), **rayleigh_vectors())
     for name, t, r, note in cases():
+        if names and name not in names:
+            continue
         outs = run_ref(t, cia, r)
         path = os.path.join(HERE, f"case_{name}.npz")
         save_case(path, t, r, outs, note)
@@ -119,4 +176,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

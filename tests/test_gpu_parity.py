@@ -76,3 +76,29 @@ def test_between_table_points_against_oracle(gpu_lib, frac):
         er, ej = rad_err(o, o_ref), jac_err(d, d_ref)
         print(f"  mode {mode}: radiance {er:.2e} jacobian {ej:.2e}", flush=True)
         assert er < tr and ej < tj
+
+
+RAYLEIGH = [p for p in CASES if "case_rayleigh_" in p]
+
+
+@pytest.mark.parametrize("path", RAYLEIGH, ids=[p.split("case_")[-1][:-4] for p in RAYLEIGH])
+def test_rayleigh_fp64_tight(gpu_lib, path):
+    """Rayleigh extinction (iray 1..4, itype 11) in the FP64 form, held to
+    1e-9 (radiance) and 1e-8 (Jacobian) against the reference: at 1450 cm-1
+    (rayleigh_cia) the term is ~1e-6 of the radiance, under the general
+    tolerance.  With itype = 0 the branch is off (cirsradg_wave.f:663) and
+    the result equals iray = 0."""
+    import copy
+    tables, req, outs, note = load_case(path)
+    print(f"rayleigh case {path.split('/')[-1]}", flush=True)
+    o, g, d = _run(gpu_lib, tables, req, 0)
+    er, ej = rad_err(o, outs["ref64_output"]), jac_err(d, outs["ref64_doutputdq"])
+    print(f"  radiance {er:.2e} jacobian {ej:.2e}", flush=True)
+    assert er < 1e-9 and ej < 1e-8, note
+    gate, off = copy.copy(req), copy.copy(req)
+    gate.itype = 0
+    off.iray = 0
+    o1, g1, d1 = gpu_lib.spectrum(gate, mode=0)
+    o0, g0, d0 = gpu_lib.spectrum(off, mode=0)
+    assert np.array_equal(o1, o0) and np.array_equal(d1, d0)
+    assert rad_err(o0, outs["ref64_output"]) > 1e-7

@@ -1190,7 +1190,9 @@ int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf, double
       if (b->fetch & 2) {
         from_pinned(b->o_out, d.out0, nout, output);
         from_pinned(b->o_gts, d.out0, nout, gtsurf);
-        from_pinned(b->o_jac, d.jac0, nout * d.L * d.nparam, doutputdq);
+        if (doutputdq)
+          radg_jac_to_host(b->hst + wlen + (b->o_jac - b->span_p0) + (size_t)d.jac0 * ts_, fp64,
+                           nout * d.L * d.nparam, d.nparam, d.ngas, doutputdq);
       }
       if (b->fetch & 1) {
         wfrom(b->o_wout, d.wout0, d.nwave, wout);
@@ -1203,7 +1205,13 @@ int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf, double
   int rc;
   if ((rc = fetch(a.out, d.out0, nout, output))) return rc;
   if ((rc = fetch(a.gts, d.out0, nout, gtsurf))) return rc;
-  if ((rc = fetch(a.jac, d.jac0, nout * d.L * d.nparam, doutputdq))) return rc;
+  if (doutputdq) {
+    const size_t nj = (size_t)nout * d.L * d.nparam, ts_ = fp64 ? 8 : 4;
+    std::vector<char> t(nj * ts_);
+    HIPCHK(hipMemcpy(t.data(), static_cast<const char *>(a.jac) + (size_t)d.jac0 * ts_, nj * ts_,
+                     hipMemcpyDeviceToHost));
+    radg_jac_to_host(t.data(), fp64, nj, d.nparam, d.ngas, doutputdq);
+  }
   if (wout) HIPCHK(hipMemcpy(wout, a.wout + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
   if (wgts) HIPCHK(hipMemcpy(wgts, a.wgts + d.wout0, d.nwave * sizeof(double), hipMemcpyDeviceToHost));
   if (wjac) HIPCHK(hipMemcpy(wjac, a.wjac + d.wjac0, nwj * sizeof(double), hipMemcpyDeviceToHost));
@@ -1225,6 +1233,7 @@ int radg_batch_pinned_view(radg_batch *b, int r, PinnedView *v) {
   v->npath = d.npath;
   v->L = d.L;
   v->nparam = d.nparam;
+  v->ngas = d.ngas;
   if (b->fetch & 1) {
     v->wout = reinterpret_cast<const double *>(b->hst + (b->o_wout - b->span_w0)) + d.wout0;
     v->wgts = reinterpret_cast<const double *>(b->hst + (b->o_wgts - b->span_w0)) + d.wout0;
@@ -1409,9 +1418,12 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
       for (int p = 0; p < hq.npath; p++) {
         const char *ja = reinterpret_cast<const char *>(a.jac) +
                          (size_t)(d.jac0 + (long long)p * L * npar) * tsz;
-        pkT.push_back(gp(ja, nullptr, dcm + S.pcls[p] * csz, ddx + (long long)p * nv,
-                         (long long)hq.npath * L * npar, 1, nv, 1, (long long)hq.npath * nv, 1,
-                         hq.nwave, nv, L * npar));
+        GemmProblem q = gp(ja, nullptr, dcm + S.pcls[p] * csz, ddx + (long long)p * nv,
+                           (long long)hq.npath * L * npar, 1, nv, 1, (long long)hq.npath * nv, 1,
+                           hq.nwave, nv, L * npar);
+        q.kper = npar;  // the stored Jacobian's gas columns are per 1e20 molecules
+        q.kgas = hq.ngas;
+        pkT.push_back(q);
       }
       mkT[0] = std::max(mkT[0], hq.nwave);
       mkT[1] = std::max(mkT[1], nv);

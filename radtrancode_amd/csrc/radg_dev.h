@@ -104,7 +104,7 @@ struct KernelArgs {
   const double *pweight;       // [path]
   // outputs
   void *inter;                 // intermediates, compute type
-  void *out, *gts, *jac;       // compute type
+  void *out, *gts, *jac;       // compute type; jac gas columns per 1e20 molecules
   double *wout, *wgts, *wjac;
   int *err;                    // device error flag
 };
@@ -123,5 +123,9 @@ struct GemmProblem {
   double *C;
   long long sAm, sAk, sBk, sBn, sCm, sCn;
   int M, N, K, accumulate, lower_only, pad;
+  // A holds a stored per-path Jacobian: along k (period kper = nparam), the
+  // first kgas entries are gas columns per 1e20 molecules and are scaled by
+  // 1e-20 on load.  kper = 0: no scaling.
+  int kper, kgas;
   double alpha;
 };

@@ -1088,12 +1088,14 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
   cia_layer(a, vv, R.inormal, lc, &absorb, dab);
   // Rayleigh (cirsradg_wave.f:663-691): utotl * xray more extinction and
-  // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.
+  // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.  The gas
+  // rows of dtcon, like dk/dq, are per 1e20 molecules (the 1e-20 is applied
+  // once when the Jacobian is written), so the row gets xray itself.
   double dray = 0.0;
   if (R.iray > 0) {
     const double xray = rayleigh_xs(R.iray, vv, lc) * 1e20;
     absorb += lc.utotl * xray;
-    dray = xray * 1.0e-20;
+    dray = xray;
   }
   for (int s = 0; s <= ngas; s++) {
     double d = 0.0;
@@ -1381,7 +1383,10 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   }
   T *jac = reinterpret_cast<T *>(a.jac) + R.jac0 + ((long long)iw * npath + p) * L * nparam;
   // each lane writes whole rows: zeros, then its isw parameters
-  // (cirsradg_wave.f:1243-1247 zeroes every row before the g-sum)
+  // (cirsradg_wave.f:1243-1247 zeroes every row before the g-sum).  The gas
+  // columns stay per 1e20 molecules, as accumulated: per molecule they reach
+  // 1e-50 and below in the near infrared, under FP32's range.  Every reader
+  // (path quadrature, host copies, state-map GEMM) applies the 1e-20 in FP64.
 #pragma unroll
   for (int q = 0; q < RL; q++) {
     const int m = lane * RL + q;
@@ -1390,9 +1395,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     if (!act[q]) continue;
     for (int k1 = 0; k1 < nsw; k1++) {
       const int k = R.isw[k1];
-      T val = acc[m * nsw + k1];
-      if (k < ngas) val = (T)((double)val * 1e-20);
-      jac[(long long)m * nparam + k] = val;
+      jac[(long long)m * nparam + k] = acc[m * nsw + k1];
     }
   }
 }
@@ -1425,7 +1428,8 @@ __global__ __launch_bounds__(64) void quadrature_kernel(KernelArgs a) {
   for (long long x = lane; x < per; x += W) {
     double s = 0.0;
     for (int p = 0; p < npath; p++) s += pw[p] * (double)jac[p * per + x];
-    wj[x] = s;
+    // gas columns are stored per 1e20 molecules
+    wj[x] = (x % nparam < R.ngas) ? s * 1e-20 : s;
   }
 }
 

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmProblem *probs, int
       if (gm < P.M && gk < P.K) {
         const long long row = P.rowmapA ? P.rowmapA[gm] : gm;
         v = (double)A[row * P.sAm + (long long)gk * P.sAk];
+        if (P.kper > 0 && gk % P.kper < P.kgas) v *= 1e-20;
       }
       As[kk][m] = v;
     }

@@ -419,7 +419,8 @@ int radg_server_wait(radg_server *s, long long ticket, double timeout_s, double 
   cpw(wjac, v.wjac, nwj);
   cpp(output, v.out, nout);
   cpp(gtsurf, v.gts, nout);
-  cpp(doutputdq, v.jac, nj);
+  if (doutputdq && v.jac)
+    radg_jac_to_host(v.jac, v.fp64, nj, (int)v.nparam, (int)v.ngas, doutputdq);
   return RADG_OK;
 }
 

@@ -14,6 +14,8 @@
  *                            from .kta headers (radtran/cirsrad/read_khead.f:18-120)
  *   radg_register_ktables    the same registry from arrays already in memory
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
+ *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
+ *                            225,312) that the caller fills from its .xsc file
  *   radg_set_mode            (none: the reference's precision is fixed at compile time)
  *   radg_batch_*, radg_server_*  (none: device-resident batches and the serving loop
  *                            that replace nemesisPY/nemesis.py:4586-4664's process pool)
@@ -103,6 +105,14 @@ int radg_ktable_bf16(int itab, unsigned short *codes, double *off_sc);
  * Fortran kcia(9,25,1501) (ciacom.f:9); wavenumber grid (i-1)*dnu. */
 int radg_register_cia(double dnu, const double *temps, const double *kcia);
 
+/* /DUST/ for the fixed-argument entry points (cirsradg_wave_,
+ * radg_spectrum_r4/_r8), whose cont(maxcon,maxlay) argument then holds
+ * ncont particle columns per layer: wavenumbers vsec[nsec] (ascending) and
+ * extinction cross-sections xsec[ncont][nsec] (cm2 per particle; the
+ * reference's xsec(1,k,l)).  ncont = 0 clears it.  nsec <= 16000,
+ * ncont <= 10 (check_limits.f:24-33).  Batched requests carry their own. */
+int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xsec);
+
 /* ------------------------------------------------ Fortran argument surface */
 
 /* Drop-in for CALL cirsradg_wave(...) from a REAL(4) host: same 37
@@ -126,17 +136,20 @@ void cirsradg_wave_(const float *dist, const int *inormal, const int *iray,
 int radg_cirsradg_status(void);
 
 /* The whole wavenumber loop in one call.  Arguments as cirsradg_wave_ (fixed
- * leading dimensions), plus nwave and vwave[nwave].  Outputs, C order:
+ * leading dimensions; cont(maxcon,maxlay) is read for the registered dust
+ * types), plus nwave and vwave[nwave].  Outputs, C order:
  *   output[nwave][npath], gtsurf[nwave][npath],
- *   doutputdq[nwave][npath][nlayer][ngas+1]   (params: gas amounts 1..ngas,
- *   then temperature; per molecule cm-2 and per K; rows past nlayin and
- *   params not in isw are zero). */
+ *   doutputdq[nwave][npath][nlayer][ngas+1+ncont]   (params: gas amounts
+ *   1..ngas, then temperature, then the registered dust types; per molecule
+ *   cm-2, per K and per particle cm-2; rows past nlayin and params not in
+ *   isw are zero). */
 int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray,
                      const float *delh, const int *nlayer, const int *npath,
                      const int *ngas, const float *press, const float *temp,
                      const float *pp, const float *amount, const int *ispace,
-                     const int *nlayin, const int *layinc, const float *scale,
-                     const int *imod, const int *idgas, const int *isogas,
+                     const int *nlayin, const int *layinc, const float *cont,
+                     const float *scale, const int *imod, const int *idgas,
+                     const int *isogas,
                      const float *emtemp, const int *itype, const int *nem,
                      const float *vem, const float *emissivity, const float *tsurf,
                      const int *flagh2p, const int *nsw, const int *isw,
@@ -146,8 +159,9 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
                      const double *delh, const int *nlayer, const int *npath,
                      const int *ngas, const double *press, const double *temp,
                      const double *pp, const double *amount, const int *ispace,
-                     const int *nlayin, const int *layinc, const double *scale,
-                     const int *imod, const int *idgas, const int *isogas,
+                     const int *nlayin, const int *layinc, const double *cont,
+                     const double *scale, const int *imod, const int *idgas,
+                     const int *isogas,
                      const double *emtemp, const int *itype, const int *nem,
                      const double *vem, const double *emissivity, const double *tsurf,
                      const int *flagh2p, const int *nsw, const int *isw,
@@ -161,7 +175,12 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
  *   nlayin/imod[npath]; layinc (1-based layer numbers), scale, emtemp
  *   [npath][L]; isw[nsw] (1-based parameter ids); vwave[nwave];
  *   vem/emissivity[nem]; path_weight[npath] (angle quadrature weights used
- *   for the weighted spectrum; may be NULL). */
+ *   for the weighted spectrum; may be NULL).
+ * Dust (cirsradg_wave.f:550-577), ncont = 0 for none: cont[L][ncont]
+ *   particles/cm2, cross-section grid vsec[nsec] (ascending cm-1) and
+ *   extinction xsec[ncont][nsec] (cm2 per particle; nsec = 1: constant).
+ *   The Jacobian then has nparam = ngas + 1 + ncont columns, the dust ones
+ *   last (d/d(particles/cm2)). */
 typedef struct radg_request {
   int nlayer, npath, ngas, nsw, nwave, nem;
   int inormal, iray, ispace, itype, flagh2p;
@@ -171,6 +190,8 @@ typedef struct radg_request {
   const double *scale, *emtemp;
   const int *isw;
   const double *vwave, *vem, *emissivity, *path_weight;
+  int ncont, nsec;
+  const double *cont, *vsec, *xsec;
 } radg_request;
 
 typedef struct radg_batch radg_batch;
@@ -192,7 +213,7 @@ int radg_batch_set_fetch(radg_batch *b, int what);
 int radg_batch_sync(radg_batch *b);
 /* Copy request r's results to host memory (FP64 regardless of mode).  Any
  * pointer may be NULL.  Shapes as radg_spectrum_r8; the weighted arrays are
- * wout[nwave], wgts[nwave], wjac[nwave][nlayer][ngas+1] (sum over paths of
+ * wout[nwave], wgts[nwave], wjac[nwave][nlayer][nparam] (sum over paths of
  * path_weight * result). */
 int radg_batch_read(radg_batch *b, int r, double *output, double *gtsurf,
                     double *doutputdq, double *wout, double *wgts, double *wjac);
@@ -214,13 +235,16 @@ int radg_batch_destroy(radg_batch *b);
  *   dam[nlayer][npro]  d(layer amount)/d(level vmr), the /LGRAD/ DAM of
  *                      radtran/includes/laygrad.f (used for every gas)
  *   dte[nlayer][npro]  d(layer T)/d(level T), /LGRAD/ DTE
- *   xmap[nv][ngas+1][npro]  d(profile quantity)/d(state element), map2xvec.f's
- *                      xmap(maxv, maxgas+2+maxcon, maxpro) without the dust and
- *                      para-H2 rows
+ *   dco[nlayer][npro]  d(layer dust)/d(level dust), /LGRAD/ DCO (used for every
+ *                      dust type, map2pro.f:72-76); may be NULL without dust
+ *   xmap[nv][nparam][npro]  d(profile quantity)/d(state element), map2xvec.f's
+ *                      xmap(maxv, maxgas+2+maxcon, maxpro) for the request's
+ *                      nparam = ngas + 1 + ncont parameters
  * nv = 0: no map for that request. */
 typedef struct radg_statemap {
   int nv, npro;
   const double *dam, *dte, *xmap;
+  const double *dco;
 } radg_statemap;
 /* Attach maps[nreq] to a batch (copied to the device).  Every later launch
  * then also forms, on the device and in FP64,

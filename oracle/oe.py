@@ -118,17 +118,18 @@ def oe_step(kk, sa, se_var, xa, xn, y, yn):
                 chisq=chisq)
 
 
-def map2pro(dq, nlayin, layinc, dam, dte, ngas):
-    """map2pro.f:49-84 for imod != 8.  ``dq`` [npath, L, ngas+1] (layer
-    Jacobian, the cirsradg_wave doutputdq order), ``layinc`` [npath, L]
-    1-based, ``dam``/``dte`` [L, npro].  Returns doutmoddq [npath, ngas+1, npro]."""
-    npath = dq.shape[0]
-    npar = ngas + 1
+def map2pro(dq, nlayin, layinc, dam, dte, ngas, dco=None):
+    """map2pro.f:49-84 for imod != 8.  ``dq`` [npath, L, nparam] (layer
+    Jacobian, the cirsradg_wave doutputdq order: ngas gases, T, then dust
+    types), ``layinc`` [npath, L] 1-based, ``dam``/``dte``/``dco`` [L, npro]:
+    DAM for gases, DTE for T, DCO for every dust type (map2pro.f:62-76).
+    Returns doutmoddq [npath, nparam, npro]."""
+    npath, _, npar = dq.shape
     out = np.zeros((npath, npar, dam.shape[1]))
     for p in range(npath):
         idx = np.asarray(layinc[p][:nlayin[p]]) - 1
         for k in range(npar):
-            d = dam if k < ngas else dte
+            d = dam if k < ngas else dte if k == ngas else dco
             out[p, k] = dq[p, :nlayin[p], k] @ d[idx]
     return out
 

@@ -82,23 +82,25 @@ C ICHK (dinvertm's check flag for Sa).
       END
 
 C HMAP fills the COMMON blocks map2pro.f reads (/PATHVA/ npath, nlayin,
-C imod, layinc, ngas, ncont; /MODCOM/ npro; /LGRAD/ DAM, DTE) and runs
-C map2pro then map2xvec as cirsrtfg_wave.f:403-425 does for one
+C imod, layinc, ngas, ncont; /MODCOM/ npro; /LGRAD/ DAM, DTE, DCO) and
+C runs map2pro then map2xvec as cirsrtfg_wave.f:403-425 does for one
 C wavenumber.  Compact arguments (Fortran order): NLAYC(NPTH),
 C LAYC(NLYR,NPTH) (1-based layer numbers), DAMC(NLYR,NPR), DTEC(NLYR,NPR),
-C XMAPC(NVC,NGS+1,NPR), DQC(NPTH,NLYR,NGS+1); output DXC(NPTH,NVC).
+C DCOC(NLYR,NPR), XMAPC(NVC,NGS+1+NCT,NPR), DQC(NPTH,NLYR,NGS+1+NCT) for
+C NCT dust types; output DXC(NPTH,NVC).
 
-      SUBROUTINE HMAP(NPTH, NLYR, NGS, NPR, NVC, NLAYC, LAYC, DAMC,
-     1  DTEC, XMAPC, DQC, DXC)
+      SUBROUTINE HMAP(NPTH, NLYR, NGS, NCT, NPR, NVC, NLAYC, LAYC, DAMC,
+     1  DTEC, DCOC, XMAPC, DQC, DXC)
       IMPLICIT NONE
       INCLUDE 'arrdef.f'
       INCLUDE 'pathcom.f'
       INCLUDE 'laycom.f'
       INCLUDE 'laygrad.f'
-      INTEGER NPTH, NLYR, NGS, NPR, NVC, I, J, K, NPAR
+      INTEGER NPTH, NLYR, NGS, NCT, NPR, NVC, I, J, K, NPAR
       INTEGER NLAYC(NPTH), LAYC(NLYR,NPTH)
-      REAL DAMC(NLYR,NPR), DTEC(NLYR,NPR), XMAPC(NVC,NGS+1,NPR)
-      REAL DQC(NPTH,NLYR,NGS+1), DXC(NPTH,NVC)
+      REAL DAMC(NLYR,NPR), DTEC(NLYR,NPR), DCOC(NLYR,NPR)
+      REAL XMAPC(NVC,NGS+1+NCT,NPR)
+      REAL DQC(NPTH,NLYR,NGS+1+NCT), DXC(NPTH,NVC)
       REAL DQL(MAXPAT,MAXLAY,MAXGAS+2+MAXCON)
       REAL DPRF(MAXPAT,MAXGAS+2+MAXCON,MAXPRO)
       REAL XMV(MAXV,MAXGAS+2+MAXCON,MAXPRO), DXV(MAXPAT,MAXV)
@@ -106,9 +108,9 @@ C XMAPC(NVC,NGS+1,NPR), DQC(NPTH,NLYR,NGS+1); output DXC(NPTH,NVC).
 
       NPATH = NPTH
       NGAS = NGS
-      NCONT = 0
+      NCONT = NCT
       NPRO = NPR
-      NPAR = NGS + 1
+      NPAR = NGS + 1 + NCT
       DO J = 1, NPTH
         NLAYIN(J) = NLAYC(J)
         IMOD(J) = 3
@@ -120,6 +122,7 @@ C XMAPC(NVC,NGS+1,NPR), DQC(NPTH,NLYR,NGS+1); output DXC(NPTH,NVC).
         DO I = 1, NLYR
           DAM(I,J) = DAMC(I,J)
           DTE(I,J) = DTEC(I,J)
+          DCO(I,J) = DCOC(I,J)
         ENDDO
       ENDDO
       DO K = 1, NPAR

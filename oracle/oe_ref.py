@@ -65,17 +65,18 @@ def oe_step(kk, sa, se_var, xa, xn, y, yn):
                 phi=phi.value, chisq=chisq.value, icheck=ichk.value)
 
 
-def map_state(dq, nlayin, layinc, dam, dte, xmap):
+def map_state(dq, nlayin, layinc, dam, dte, xmap, dco=None, ncont=0):
     """HMAP: map2pro then map2xvec for one wavenumber.  ``dq`` [npath, L,
-    ngas+1], ``layinc`` [npath, L] 1-based, ``dam``/``dte`` [L, npro],
-    ``xmap`` [nv, ngas+1, npro].  Returns doutdx [npath, nv]."""
+    ngas+1+ncont], ``layinc`` [npath, L] 1-based, ``dam``/``dte``/``dco``
+    [L, npro], ``xmap`` [nv, ngas+1+ncont, npro].  Returns doutdx [npath, nv]."""
     npath, L, npar = dq.shape
     npro = dam.shape[1]
     nv = xmap.shape[0]
     nl = _f(nlayin, np.int32)
     lay = _f(np.asarray(layinc).T, np.int32)          # LAYC(NLAY, NPTH)
-    damf, dtef, xmf, dqf = _f(dam), _f(dte), _f(xmap), _f(dq)
+    dco = np.zeros_like(dam) if dco is None else dco
+    damf, dtef, dcof, xmf, dqf = _f(dam), _f(dte), _f(dco), _f(xmap), _f(dq)
     dx = np.zeros((npath, nv), order="F")
-    _load().hmap_(_i(npath), _i(L), _i(npar - 1), _i(npro), _i(nv), _p(nl), _p(lay),
-                  _p(damf), _p(dtef), _p(xmf), _p(dqf), _p(dx))
+    _load().hmap_(_i(npath), _i(L), _i(npar - 1 - ncont), _i(ncont), _i(npro), _i(nv), _p(nl),
+                  _p(lay), _p(damf), _p(dtef), _p(dcof), _p(xmf), _p(dqf), _p(dx))
     return np.array(dx)

@@ -65,7 +65,9 @@ class _REQ(C.Structure):
                 ("idgas", C.c_void_p), ("isogas", C.c_void_p),
                 ("nlayin", C.c_void_p), ("imod", C.c_void_p), ("layinc", C.c_void_p),
                 ("scale", C.c_void_p), ("emtemp", C.c_void_p), ("isw", C.c_void_p),
-                ("vem", C.c_void_p), ("emissivity", C.c_void_p)]
+                ("vem", C.c_void_p), ("emissivity", C.c_void_p),
+                ("ncont", C.c_int), ("nsec", C.c_int),
+                ("cont", C.c_void_p), ("vsec", C.c_void_p), ("xsec", C.c_void_p)]
 
 
 def _d(a):
@@ -98,10 +100,15 @@ class _Pack:
              _i(req.imod), _i(req.layinc), _d(req.scale), _d(req.emtemp),
              _i(req.isw), _d(req.vem), _d(req.emissivity)]
         self.keep += f
+        nc = req.ncont
+        dust = [_d(req.cont), _d(req.vsec), _d(req.xsec)] if nc else []
+        self.keep += dust
         self.req = _REQ(L, req.npath, req.ngas, len(req.isw), len(req.vem),
                         req.inormal, req.iray, req.ispace, req.itype, req.flagh2p,
                         req.amform, req.dist, req.tsurf, req.radius1, req.radextra,
-                        *[a.ctypes.data for a in f])
+                        *[a.ctypes.data for a in f],
+                        nc, len(req.vsec) if nc else 0,
+                        *([a.ctypes.data for a in dust] if nc else [None, None, None]))
 
 
 def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):
@@ -109,13 +116,14 @@ def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):
 
     Returns ``(output[nwave, npath], gtsurf[nwave, npath],
     doutputdq[nwave, npath, nlayer, nparam])`` where ``doutputdq`` is
-    dR/d(amount) for parameters 1..ngas and dR/dT for ngas+1, indexed by the
-    path's layer position (``cirsradg_wave.f:1240-1286``).
+    dR/d(amount) for parameters 1..ngas, dR/dT for ngas+1 and dR/d(cont) for
+    the ncont dust types after it, indexed by the path's layer position
+    (``cirsradg_wave.f:1240-1286``).
     """
     lb = lib()
     pk = _Pack(tables, cia, req)
     vw = _d(req.vwave if vwave is None else vwave)
-    nw, npath, L, npar = vw.shape[0], req.npath, req.nlayer, req.ngas + 1
+    nw, npath, L, npar = vw.shape[0], req.npath, req.nlayer, req.nparam
     out = np.zeros((nw, npath))
     gts = np.zeros((nw, npath))
     dq = np.zeros((nw, npath, L, npar))

@@ -19,6 +19,7 @@
  *               rayleighls.f (the iray branch, cirsradg_wave.f:663-691)
  *   planck      radtran/radtran/planck_wave.f:1-42, planckg_wave.f:1-40
  *   interpem    radtran/cirsrad/interpem.f:1-37
+ *   interp      radtran/rtm_util/interp.f:1-26 (dust cross-sections)
  *   cirsradg    radtran/cirsradg/cirsradg_wave.f:333-1290 (imod 0, 1, 3)
  *
  * Everything that the reference declares REAL is double here, matching the
@@ -38,6 +39,8 @@
 #define MAXGAS 30
 #define MAXG 51
 #define MAXK 30
+#define MAXCON 10
+#define MAXSEC 16000
 #define CIA_NUMWN 1501
 #define CIA_NUMT 25
 #define CIA_NPAIRS 9
@@ -467,6 +470,19 @@ static double orc_co2cia(double v0) {
   return a;
 }
 
+/* ---------------------------------------------------------------- interp */
+/* rtm_util/interp.f:1-26: linear interpolation (and extrapolation from the
+ * end intervals) on a monotonic grid; x, y 0-based here. */
+double orc_interp(const double *x, const double *y, int n, double xin) {
+  int i;  /* 1-based */
+  for (i = 1; i <= n; i++) {
+    if (x[0] < x[n - 1] ? x[i - 1] > xin : x[i - 1] < xin) break;
+  }
+  if (i > n) i = n;
+  if (i == 1) i = 2;
+  return y[i - 2] + (y[i - 1] - y[i - 2]) * (xin - x[i - 2]) / (x[i - 1] - x[i - 2]);
+}
+
 /* ---------------------------------------------------------------- Rayleigh */
 /* Rayleigh cross-section per molecule (cm2) at wavenumber v (cm-1), with the
  * reference's pi literal (PI_REF: rayleigha.f:21, rayleighj.f:22 and
@@ -814,6 +830,12 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       if (r->layinc[ip * L + j] > L || r->layinc[ip * L + j] < 1) return ORC_ERR_LIMITS;
     if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3) return ORC_ERR_UNSUPPORTED;
   }
+  if (r->ncont < 0 || r->ncont > MAXCON || (r->ncont > 0 && (r->nsec < 1 || r->nsec > MAXSEC)))
+    return ORC_ERR_LIMITS;                                        /* check_limits.f:24-33 */
+  /* cirsradg_wave.f:439-453: the wavenumber must lie on the dust grid */
+  if (r->ncont > 0 && r->nsec > 1 &&
+      (vwave < r->vsec[0] || vwave > r->vsec[r->nsec - 1]))
+    return ORC_ERR_LIMITS;
   if (r->flagh2p == 1) return ORC_ERR_UNSUPPORTED;               /* nparacon_all */
   for (k = 0; k < ngas; k++)
     if (r->idgas[k] == 22) return ORC_ERR_UNSUPPORTED;          /* N2 analytic CIA */
@@ -836,7 +858,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
   }
   if (ng < 1 || ng > MAXG) return ORC_ERR_LIMITS;
 
-  nparam = ngas + 1;
+  nparam = ngas + 1 + r->ncont;  /* cirsradg_wave.f:375 */
   /* cirsradg_wave.f:393-414: the last H2, He, CH4, NH3 (isotope 0 or 1) */
   for (k = 0; k < ngas; k++) {
     if (r->isogas[k] != 0 && r->isogas[k] != 1) continue;
@@ -898,6 +920,15 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
     double k_g[MAXG], dkdq[(MAXGAS + 1) * MAXG];
     int iabs[5];
     taucon[j] = 0.0;
+    /* dust, cirsradg_wave.f:550-577: extinction cross-section at vwave
+     * (constant for nsec = 1, else interp.f on the /DUST/ grid) times the
+     * particles/cm2; d(tau)/d(cont) is the cross-section. */
+    for (k = 0; k < r->ncont; k++) {
+      const double *xk = r->xsec + (size_t)k * r->nsec;
+      const double tau = r->nsec == 1 ? xk[0] : orc_interp(r->vsec, xk, r->nsec, vwave);
+      taucon[j] += tau * r->cont[(size_t)j * r->ncont + k];
+      dtaucondq[j * nparam + ngas + 1 + k] = tau;
+    }
     /* ngascon returns 0 for every gas: all gas-continuum flags are off
      * (ngascon.f:72-190 with /GASCONDEF/ zero). */
     for (k = 0; k < ngas; k++) {
@@ -1146,7 +1177,7 @@ int orc_spectrum(const orc_ktable *tabs, int ntab, const orc_cia *cia,
                  const orc_req *r, const double *vwave, int nwave, int jac_mode,
                  int nthreads, double *output, double *gtsurf, double *doutputdq) {
   int iw, rc = ORC_OK;
-  const size_t per = (size_t)r->npath * r->nlayer * (r->ngas + 1);
+  const size_t per = (size_t)r->npath * r->nlayer * (r->ngas + 1 + r->ncont);
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(max : rc)

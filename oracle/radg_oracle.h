@@ -49,6 +49,11 @@ typedef struct orc_req {
   const double *scale, *emtemp;
   const int *isw;                /* 1-based parameter ids */
   const double *vem, *emissivity;
+  /* dust: cont(k,j) particles/cm2 as [L][ncont]; the /DUST/ block's
+   * wavenumbers vsec[nsec] and extinction cross-sections xsec(1,k,l) as
+   * [ncont][nsec] (cirsradg_wave.f:225,312). ncont = 0: no dust. */
+  int ncont, nsec;
+  const double *cont, *vsec, *xsec;
 } orc_req;
 
 /* Status codes (the reference STOPs instead). */
@@ -91,6 +96,7 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t,
                  int inormal, int ngas, const int *idgas, const int *isogas,
                  const double *amount, const double *pp, double xlen,
                  double *absorb, int *iabs, double *dabsorb);       /* nciacon.f */
+double orc_interp(const double *x, const double *y, int n, double xin); /* interp.f */
 double orc_rayleigh(int iray, double v, double p, double t, double fheh2,
                     double fch4h2, double fnh3);                    /* rayleigh*.f */
 

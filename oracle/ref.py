@@ -103,6 +103,12 @@ class RefRunner:
         temps = np.ascontiguousarray(cia.temps, dtype=rdt)
         kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
         self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        if req.ncont:
+            nsec = np.array([len(req.vsec)], dtype=np.int32)
+            ncont = np.array([req.ncont], dtype=np.int32)
+            self._vsec = np.ascontiguousarray(req.vsec, dtype=rdt)
+            self._xsec = np.ascontiguousarray(req.xsec, dtype=rdt)  # [ncont][nsec]
+            self.lib.hsetdust_(_p(nsec), _p(ncont), _p(self._vsec), _p(self._xsec))
         ngas = np.array([req.ngas], dtype=np.int32)
         idg = np.ascontiguousarray(req.idgas, dtype=np.int32)
         iso = np.ascontiguousarray(req.isogas, dtype=np.int32)
@@ -131,6 +137,8 @@ class RefRunner:
         self.layinc = np.zeros((MAXLAY, MAXPAT), np.int32, **F)
         self.layinc[:L, :npath] = r.layinc.T
         self.cont = np.zeros((MAXCON, MAXLAY), rdt, **F)
+        if r.ncont:
+            self.cont[:r.ncont, :L] = np.asarray(r.cont).T
         self.scale = np.zeros((MAXLAY, MAXPAT), rdt, **F); self.scale[:L, :npath] = r.scale.T
         self.emtemp = np.zeros((MAXLAY, MAXPAT), rdt, **F); self.emtemp[:L, :npath] = r.emtemp.T
         self.imod = np.zeros(MAXPAT, np.int32); self.imod[:npath] = r.imod
@@ -169,7 +177,7 @@ class RefRunner:
     def spectrum(self, iwaves=None):
         """Same return layout as ``oracle.spectrum``."""
         r = self.req
-        L, npath, npar = r.nlayer, r.npath, r.ngas + 1
+        L, npath, npar = r.nlayer, r.npath, r.nparam
         iwaves = range(self.vwave.shape[0]) if iwaves is None else iwaves
         iwaves = list(iwaves)
         out = np.zeros((len(iwaves), npath))

@@ -72,6 +72,28 @@ C 8 bytes, which is the FP64 form of the reference.
       RETURN
       END
 
+C     /DUST/ (cirsradg_wave.f:225,312): NSEC wavenumbers, NCONT particle
+C     types, extinction cross-sections XEXT1(L,K); the second row of XSEC
+C     (albedo times extinction) is not read by the emission branches.
+      SUBROUTINE HSETDUST(NSEC1, NCONT1, VSEC1, XEXT1)
+      IMPLICIT NONE
+      INCLUDE 'arrdef.f'
+      INTEGER NSEC1, NCONT1, NSEC, NCONT, K, L
+      REAL VSEC1(NSEC1), XEXT1(NSEC1, NCONT1)
+      REAL VSEC(MAXSEC), XSEC(2,MAXCON,MAXSEC)
+      COMMON /DUST/ VSEC, XSEC, NSEC, NCONT
+      NSEC = NSEC1
+      NCONT = NCONT1
+      DO L = 1, NSEC1
+        VSEC(L) = VSEC1(L)
+        DO K = 1, NCONT1
+          XSEC(1,K,L) = XEXT1(L,K)
+          XSEC(2,K,L) = 0.5*XEXT1(L,K)
+        ENDDO
+      ENDDO
+      RETURN
+      END
+
       SUBROUTINE HREADK(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
       IMPLICIT NONE
       CHARACTER*(*) KL

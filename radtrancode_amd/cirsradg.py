@@ -9,7 +9,9 @@ Reference correspondence:
 * :func:`spectrum` -- the wavenumber loop of ``CIRSRTFG_WAVE``
   (``radtran/cirsradg/cirsrtfg_wave.f:361-448``) for one request.
 * :func:`register_ktables` / :func:`read_klist` -- the ``/INTERPK/`` registry
-  that ``read_klist.f`` fills; :func:`register_cia` -- ``/CIATABLE/``.
+  that ``read_klist.f`` fills; :func:`register_cia` -- ``/CIATABLE/``;
+  :func:`register_dust` -- ``/DUST/`` for :func:`cirsradg_wave` (batched
+  requests carry their own dust arrays).
 * :class:`Batch` -- many requests resident on the device, launched together.
 
 Errors raise :class:`RadgError` where the reference would STOP.
@@ -84,6 +86,19 @@ def register_cia(cia) -> None:
     check(lib().radg_register_cia(C.c_double(cia.dnu), _vp(temps), _vp(kcia)))
 
 
+def register_dust(vsec=None, xsec=None) -> None:
+    """``/DUST/``: extinction cross-sections ``xsec`` [ncont, nsec] (cm2) on
+    ``vsec`` [nsec] for the fixed-argument entry point; None clears it."""
+    if vsec is None:
+        check(lib().radg_register_dust(0, 0, None, None))
+        return
+    vs = np.ascontiguousarray(vsec, dtype=np.float64)
+    xs = np.ascontiguousarray(xsec, dtype=np.float64)
+    if xs.ndim != 2 or xs.shape[1] != vs.shape[0]:
+        raise ValueError("xsec must be [ncont, nsec]")
+    check(lib().radg_register_dust(int(vs.shape[0]), int(xs.shape[0]), _vp(vs), _vp(xs)))
+
+
 class _ReqArrays:
     """Contiguous copies of one request, kept alive while the batch exists."""
 
@@ -98,19 +113,23 @@ class _ReqArrays:
         self.isw, self.vwave = i(req.isw), d(req.vwave)
         self.vem, self.emis = d(req.vem), d(req.emissivity)
         self.pw = d(req.path_weight)
+        nc = req.ncont
+        dust = (d(req.cont), d(req.vsec), d(req.xsec)) if nc else ()
+        self.dust = dust
         self.s = Request(
             req.nlayer, req.npath, req.ngas, len(self.isw), len(self.vwave), len(self.vem),
             req.inormal, req.iray, req.ispace, req.itype, req.flagh2p, float(req.tsurf),
             *[a.ctypes.data for a in (self.press, self.temp, self.delh, self.pp, self.amount,
                                       self.idgas, self.isogas, self.nlayin, self.imod,
                                       self.layinc, self.scale, self.emtemp, self.isw,
-                                      self.vwave, self.vem, self.emis, self.pw)])
+                                      self.vwave, self.vem, self.emis, self.pw)],
+            nc, len(dust[1]) if nc else 0, *([a.ctypes.data for a in dust] if nc else [None] * 3))
 
 
 class StateMap(C.Structure):
     """``radg_statemap`` (include/radg.h)."""
     _fields_ = [("nv", C.c_int), ("npro", C.c_int), ("dam", C.c_void_p), ("dte", C.c_void_p),
-                ("xmap", C.c_void_p)]
+                ("xmap", C.c_void_p), ("dco", C.c_void_p)]
 
 
 def _requests(reqs):
@@ -151,7 +170,7 @@ class Batch:
 
     def read(self, r: int, weighted=True):
         q = self.reqs[r]
-        nw, npath, L, npar = q.nwave, q.npath, q.nlayer, q.ngas + 1
+        nw, npath, L, npar = q.nwave, q.npath, q.nlayer, q.nparam
         out = np.zeros((nw, npath))
         gts = np.zeros((nw, npath))
         jac = np.zeros((nw, npath, L, npar))
@@ -167,7 +186,8 @@ class Batch:
     def set_statemaps(self, maps, per_path=False):
         """Attach layer -> profile -> state maps (map2pro.f + map2xvec.f) to
         the requests: ``maps[r]`` is None or a dict with ``dam``, ``dte``
-        [nlayer, npro] and ``xmap`` [nv, ngas+1, npro] (FP64)."""
+        [nlayer, npro], ``xmap`` [nv, nparam, npro] and, for a request with
+        dust, ``dco`` [nlayer, npro] (FP64)."""
         n = len(self.reqs)
         arr = (StateMap * n)()
         keep = []
@@ -178,11 +198,16 @@ class Batch:
                             for k in ("dam", "dte", "xmap"))
             q = self.reqs[r]
             if dam.shape != dte.shape or dam.shape[0] != q.nlayer or \
-                    xm.shape[1:] != (q.ngas + 1, dam.shape[1]):
+                    xm.shape[1:] != (q.nparam, dam.shape[1]):
                 raise ValueError(f"request {r}: statemap shapes do not match the request")
-            keep += [dam, dte, xm]
+            dco = None
+            if m.get("dco") is not None:
+                dco = np.ascontiguousarray(m["dco"], dtype=np.float64)
+                if dco.shape != dam.shape:
+                    raise ValueError(f"request {r}: dco must be [nlayer, npro]")
+            keep += [dam, dte, xm, dco]
             arr[r] = StateMap(xm.shape[0], dam.shape[1], dam.ctypes.data, dte.ctypes.data,
-                              xm.ctypes.data)
+                              xm.ctypes.data, None if dco is None else dco.ctypes.data)
         self._maps = [None if m is None else m["xmap"].shape[0] for m in maps]
         check(lib().radg_batch_set_statemaps(self.h, C.cast(arr, C.c_void_p), int(per_path)))
 
@@ -272,6 +297,8 @@ def fortran_arrays(req, real=np.float32):
     a["vem"][:len(req.vem)] = req.vem
     a["emissivity"][:len(req.emissivity)] = req.emissivity
     a["isw"][:len(req.isw)] = req.isw
+    if req.ncont:
+        a["cont"][:req.ncont, :L] = np.asarray(req.cont).T
     return a
 
 

@@ -110,6 +110,13 @@ struct CIA {
 std::shared_ptr<TableSet> g_tables;
 std::shared_ptr<CIA> g_cia;
 
+// /DUST/ for the fixed-argument entry points (cirsradg_wave.f:225,312)
+struct Dust {
+  int nsec = 0, ncont = 0;
+  std::vector<double> vsec, xsec;  // [nsec], [ncont][nsec]
+};
+std::shared_ptr<Dust> g_dust;
+
 bool valid_mode(int mode) {
   return mode == RADG_MODE_FP64 || mode == RADG_MODE_MIXED || mode == RADG_MODE_BF16;
 }
@@ -166,6 +173,7 @@ struct Packed {
   DevReq d{};
   std::vector<LayerConst> lay;
   std::vector<double> amo, press, scale, emtemp, vwave, esurf, pweight;
+  std::vector<double> dext, cont;  // dust: [nwave][ncont] extinction, [L][ncont] columns
   std::vector<PathInfo> paths;
   std::vector<int> layinc;
   std::vector<WaveGas> wg;
@@ -182,7 +190,21 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   if (nwave < 1) return fail(RADG_ELIMITS, "nwave=%d", nwave);
   if (q.flagh2p == 1) return fail(RADG_EUNSUPPORTED, "flagh2p=1 (para-H2 CIA, nparacon_all) is not provided");
   if (q.ispace != 0 && q.ispace != 1) return fail(RADG_EINPUT, "ispace=%d", q.ispace);
-  const int nparam = ngas + 1;
+  // dust, check_limits.f:24-33 and cirsradg_wave.f:439-453
+  const int ncont = q.ncont;
+  if (ncont < 0 || ncont > RADG_MAXCON) return fail(RADG_ELIMITS, "ncont=%d outside 0..%d", ncont, RADG_MAXCON);
+  if (ncont > 0) {
+    if (q.nsec < 1 || q.nsec > RADG_MAXSEC) return fail(RADG_ELIMITS, "nsec=%d outside 1..%d", q.nsec, RADG_MAXSEC);
+    if (!q.cont || !q.vsec || !q.xsec) return fail(RADG_EINPUT, "ncont=%d without cont/vsec/xsec", ncont);
+    for (int i = 0; i < q.nsec; i++)
+      if (!std::isfinite(q.vsec[i]) || (i > 0 && !(q.vsec[i] > q.vsec[i - 1])))
+        return fail(RADG_EINPUT, "dust wavenumbers vsec must be finite and increasing");
+    for (long long i = 0; i < (long long)ncont * q.nsec; i++)
+      if (!std::isfinite(q.xsec[i])) return fail(RADG_EINPUT, "dust cross-section not finite");
+    for (long long i = 0; i < (long long)L * ncont; i++)
+      if (!std::isfinite(q.cont[i])) return fail(RADG_EINPUT, "dust column cont not finite");
+  }
+  const int nparam = ngas + 1 + ncont;  // cirsradg_wave.f:375
   if (q.nsw < 0 || q.nsw > nparam) return fail(RADG_ELIMITS, "nsw=%d > nparam=%d", q.nsw, nparam);
   for (int k = 0; k < q.nsw; k++)
     if (q.isw[k] < 1 || q.isw[k] > nparam) return fail(RADG_EINPUT, "isw(%d)=%d outside 1..%d", k + 1, q.isw[k], nparam);
@@ -218,6 +240,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   d.ngas = ngas;
   d.npath = npath;
   d.nparam = nparam;
+  d.ncont = ncont;
   d.nsw = q.nsw;
   d.nwave = nwave;
   d.ispace = q.ispace;
@@ -346,6 +369,8 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   // ---- per wavenumber
   P.vwave.assign(q.vwave, q.vwave + nwave);
   P.esurf.resize(nwave);
+  P.dext.assign((size_t)nwave * ncont, 0.0);
+  P.cont.assign(q.cont, q.cont + (size_t)L * ncont);
   P.wg.resize((size_t)nwave * ngas);
   const long long ntab = (long long)np * nt * NG;
   const size_t kbytes = (mode == RADG_MODE_FP64) ? 8 : (mode == RADG_MODE_BF16) ? 2 : 4;
@@ -366,6 +391,26 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
       }
     if (v == q.vem[nem - 1]) es = q.emissivity[nem - 1];
     P.esurf[iw] = es;
+    // dust extinction at this wavenumber (cirsradg_wave.f:559-569): the one
+    // value for nsec = 1, else interp.f on the cross-section grid, which
+    // must cover the wavenumber (:446-452)
+    if (ncont > 0) {
+      const int ns = q.nsec;
+      if (ns > 1 && (v < q.vsec[0] || v > q.vsec[ns - 1]))
+        return fail(RADG_EINPUT, "wavenumber %g outside the dust cross-section grid [%g, %g]", v,
+                    q.vsec[0], q.vsec[ns - 1]);
+      int i = 1;  // interp.f: first grid point above v (1-based), at least 2
+      if (ns > 1) {
+        while (i <= ns && !(q.vsec[i - 1] > v)) i++;
+        i = std::max(2, std::min(i, ns));
+      }
+      for (int k = 0; k < ncont; k++) {
+        const double *y = q.xsec + (size_t)k * ns;
+        P.dext[(size_t)iw * ncont + k] =
+            ns == 1 ? y[0]
+                    : y[i - 2] + (y[i - 1] - y[i - 2]) * (v - q.vsec[i - 2]) / (q.vsec[i - 1] - q.vsec[i - 2]);
+      }
+    }
     for (int k = 0; k < ngas; k++) {
       WaveGas &w = P.wg[(size_t)iw * ngas + k];
       w.base = -1;
@@ -411,7 +456,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   // algorithmic bytes: table nodes + layer inputs + CIA nodes + outputs
   const double obytes = (double)nwave * ((double)npath * (2.0 + (double)L * nparam) * obytes_el +
                                          (2.0 + (double)L * nparam) * 8.0);
-  const double ibytes = (double)L * (4 + 2 * ngas) * 8.0 + (double)nwave * 4 * 9 * 8.0;
+  const double ibytes = (double)L * (4 + 2 * ngas + ncont) * 8.0 + (double)nwave * (4 * 9 + ncont) * 8.0;
   P.alg_bytes[0] = tbytes;
   P.alg_bytes[1] = obytes;
   P.alg_bytes[2] = ibytes;
@@ -462,7 +507,7 @@ struct radg_batch {
   char *hst = nullptr;     // pinned: [weighted span][per-path span]
   // host copy of each request's path layers (0-based layinc, zero past nlays)
   struct HostReq {
-    int L = 0, npath = 0, ngas = 0, nwave = 0;
+    int L = 0, npath = 0, ngas = 0, nwave = 0, ncont = 0;
     std::vector<int> layinc, nlays;
   };
   std::vector<HostReq> hreq;
@@ -609,11 +654,12 @@ struct Layout {
 struct Plan {
   std::vector<Packed> packs;
   size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
-      o_vw, o_es, o_wg, o_pw, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac, o_err;
+      o_vw, o_es, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
+      o_err;
   size_t total = 0;
   long long nA = 0, nB = 0, nC = 0;
   long long nlay = 0, nlg = 0, npath = 0, npl = 0, nwv = 0, nwg = 0, ninter = 0, nout = 0, njac = 0,
-            nwout = 0, nwjac = 0;
+            nwout = 0, nwjac = 0, ndx = 0, nct = 0;
   std::vector<long long> sA, sB, sC;
   int maxngas = 1, maxnsw = 1, maxlays = 1;
   std::vector<int> cls_rl;
@@ -646,6 +692,10 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     d.jac0 = pl.njac;
     d.wout0 = pl.nwout;
     d.wjac0 = pl.nwjac;
+    d.dx0 = pl.ndx;
+    d.ct0 = pl.nct;
+    pl.ndx += (long long)q.nwave * q.ncont;
+    pl.nct += (long long)q.nlayer * q.ncont;
     pl.nlay += q.nlayer;
     pl.nlg += (long long)q.nlayer * q.ngas;
     pl.npath += q.npath;
@@ -657,9 +707,9 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     pl.nC += q.nwave;
     pl.ninter += (long long)q.nwave * inter_block(ts.ng, q.nlayer, q.ngas);
     pl.nout += (long long)q.nwave * q.npath;
-    pl.njac += (long long)q.nwave * q.npath * q.nlayer * (q.ngas + 1);
+    pl.njac += (long long)q.nwave * q.npath * q.nlayer * d.nparam;
     pl.nwout += q.nwave;
-    pl.nwjac += (long long)q.nwave * q.nlayer * (q.ngas + 1);
+    pl.nwjac += (long long)q.nwave * q.nlayer * d.nparam;
     pl.sA[r + 1] = pl.nA;
     pl.sB[r + 1] = pl.nB;
     pl.sC[r + 1] = pl.nC;
@@ -697,6 +747,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_es = lo.take<double>(pl.nwv);
   pl.o_wg = lo.take<WaveGas>(pl.nwg);
   pl.o_pw = lo.take<double>(pl.npath);
+  pl.o_dx = lo.take<double>(pl.ndx);
+  pl.o_ct = lo.take<double>(pl.nct);
   pl.o_clsmap = lo.take<int>(nreq);
   pl.o_clsstart = lo.take<long long>(nreq + pl.cls_rl.size());
   pl.o_inter = lo.take<char>((size_t)pl.ninter * ts_);
@@ -851,6 +903,26 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia) {
   return RADG_OK;
 }
 
+int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xsec) {
+  if (ncont == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_dust.reset();
+    return RADG_OK;
+  }
+  if (ncont < 0 || ncont > RADG_MAXCON || nsec < 1 || nsec > RADG_MAXSEC)
+    return fail(RADG_ELIMITS, "dust: nsec=%d ncont=%d outside 1..%d, 0..%d", nsec, ncont, RADG_MAXSEC,
+                RADG_MAXCON);
+  if (!vsec || !xsec) return fail(RADG_EINPUT, "dust: null vsec or xsec");
+  auto d = std::make_shared<Dust>();
+  d->nsec = nsec;
+  d->ncont = ncont;
+  d->vsec.assign(vsec, vsec + nsec);
+  d->xsec.assign(xsec, xsec + (size_t)ncont * nsec);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_dust = d;
+  return RADG_OK;
+}
+
 int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes) {
   std::shared_ptr<TableSet> ts;
   {
@@ -948,6 +1020,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
     put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
     put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
+    put(h, pl.o_dx + d.dx0 * sizeof(double), P.dext.data(), P.dext.size());
+    put(h, pl.o_ct + d.ct0 * sizeof(double), P.cont.data(), P.cont.size());
     for (int i = 0; i < 3; i++) b->alg_bytes[i] += P.alg_bytes[i];
   }
   b->hreq.resize(nreq);
@@ -958,6 +1032,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     hq.npath = reqs[r].npath;
     hq.ngas = reqs[r].ngas;
     hq.nwave = reqs[r].nwave;
+    hq.ncont = reqs[r].ncont;
     hq.layinc = P.layinc;
     hq.nlays.resize(hq.npath);
     for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
@@ -997,6 +1072,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
   a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
+  a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
+  a.cont = reinterpret_cast<const double *>(m + pl.o_ct);
   a.inter = m + pl.o_inter;
   a.out = m + pl.o_out;
   a.gts = m + pl.o_gts;
@@ -1273,15 +1350,17 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
   std::vector<radg_batch::SMap> sm(nreq);
   // ---- sizes and path classes
   Layout lo;
-  std::vector<size_t> o_dam(nreq), o_dte(nreq), o_xm(nreq), o_rm(nreq);
+  std::vector<size_t> o_dam(nreq), o_dte(nreq), o_dco(nreq), o_xm(nreq), o_rm(nreq);
   int n_pc = 0, n_pk64 = 0, n_pkT = 0, n_ps = 0;
   for (int r = 0; r < nreq; r++) {
     const radg_statemap &mp = maps[r];
     if (mp.nv == 0) continue;
     const radg_batch::HostReq &hq = b->hreq[r];
-    const int L = hq.L, npar = hq.ngas + 1;
+    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont;
     if (mp.nv < 0 || mp.npro < 1 || !mp.dam || !mp.dte || !mp.xmap)
       return fail(RADG_EINPUT, "request %d: statemap needs nv >= 0, npro >= 1 and dam, dte, xmap", r);
+    if (hq.ncont > 0 && !mp.dco)
+      return fail(RADG_EINPUT, "request %d: statemap of a request with dust needs dco", r);
     radg_batch::SMap &S = sm[r];
     S.nv = mp.nv;
     S.npro = mp.npro;
@@ -1306,6 +1385,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     S.per_path = (per_path || S.ncls > 1) ? 1 : 0;
     o_dam[r] = lo.take<double>((size_t)L * mp.npro);
     o_dte[r] = lo.take<double>((size_t)L * mp.npro);
+    o_dco[r] = lo.take<double>(hq.ncont > 0 ? (size_t)L * mp.npro : 0);
     o_xm[r] = lo.take<double>((size_t)mp.nv * npar * mp.npro);
     o_rm[r] = lo.take<int>((size_t)S.ncls * L);
     n_pc += S.ncls * npar;
@@ -1323,7 +1403,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     radg_batch::SMap &S = sm[r];
     if (S.nv == 0) continue;
     const radg_batch::HostReq &hq = b->hreq[r];
-    const int L = hq.L, npar = hq.ngas + 1;
+    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont;
     S.o_cm = lo.take<double>((size_t)S.ncls * L * npar * S.nv);
     S.o_kk = lo.take<double>((size_t)hq.nwave * S.nv);
     if (S.per_path) S.o_dx = lo.take<double>((size_t)hq.nwave * hq.npath * S.nv);
@@ -1381,9 +1461,10 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     const radg_statemap &mp = maps[r];
     const radg_batch::HostReq &hq = b->hreq[r];
     const DevReq &d = b->reqs[r];
-    const int L = hq.L, npar = hq.ngas + 1, nv = S.nv, npro = S.npro;
+    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont, nv = S.nv, npro = S.npro;
     std::memcpy(hs + o_dam[r], mp.dam, (size_t)L * npro * 8);
     std::memcpy(hs + o_dte[r], mp.dte, (size_t)L * npro * 8);
+    if (hq.ncont > 0) std::memcpy(hs + o_dco[r], mp.dco, (size_t)L * npro * 8);
     std::memcpy(hs + o_xm[r], mp.xmap, (size_t)nv * npar * npro * 8);
     int *rm = reinterpret_cast<int *>(hs + o_rm[r]);
     std::vector<int> rep(S.ncls, -1);
@@ -1394,6 +1475,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
         rm[(size_t)c * L + j] = j < hq.nlays[rep[c]] ? hq.layinc[(size_t)rep[c] * L + j] : 0;
     const double *ddam = reinterpret_cast<const double *>(dm + o_dam[r]);
     const double *ddte = reinterpret_cast<const double *>(dm + o_dte[r]);
+    const double *ddco = reinterpret_cast<const double *>(dm + o_dco[r]);
     const double *dxm = reinterpret_cast<const double *>(dm + o_xm[r]);
     const int *drm = reinterpret_cast<const int *>(dm + o_rm[r]);
     double *dcm = reinterpret_cast<double *>(dm + S.o_cm);
@@ -1401,7 +1483,9 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     const long long csz = (long long)L * npar * nv;
     for (int c = 0; c < S.ncls; c++)
       for (int k = 0; k < npar; k++) {
-        pc.push_back(gp(k < hq.ngas ? ddam : ddte, drm + (size_t)c * L, dxm + (size_t)k * npro,
+        // map2pro.f:62-76: DAM for the gases, DTE for T, DCO for every dust type
+        const double *D = k < hq.ngas ? ddam : k == hq.ngas ? ddte : ddco;
+        pc.push_back(gp(D, drm + (size_t)c * L, dxm + (size_t)k * npro,
                         dcm + c * csz + (long long)k * nv, npro, 1, 1, (long long)npar * npro,
                         (long long)npar * nv, 1, L, nv, npro));
         mpc[0] = std::max(mpc[0], L);
@@ -1501,7 +1585,8 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
               int flagh2p, const int *nlayin, const int *layinc, const R *scale, const int *imod,
               const int *idgas, const int *isogas, const R *emtemp, int nem, const R *vem,
               const R *emis, R tsurf, int nsw, const int *isw, int nwave, const R *vwave,
-              std::vector<double> &out, std::vector<double> &gts, std::vector<double> &jac) {
+              const R *cont, std::vector<double> &out, std::vector<double> &gts,
+              std::vector<double> &jac) {
   if (L < 1 || L > RADG_MAXLAY || npath < 1 || npath > RADG_MAXPAT || ngas < 1 || ngas > RADG_MAXGAS)
     return fail(RADG_ELIMITS, "nlayer=%d npath=%d ngas=%d exceed the array limits", L, npath, ngas);
   if (nem < 1 || nem > RADG_MAXSEC) return fail(RADG_ELIMITS, "nem=%d", nem);
@@ -1522,6 +1607,16 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
       et[(size_t)p * L + m] = emtemp[(size_t)p * RADG_MAXLAY + m];
     }
   std::vector<double> vw(vwave, vwave + nwave), ve(vem, vem + nem), em(emis, emis + nem);
+  // /DUST/ as registered, cont(maxcon,maxlay) -> [L][ncont]
+  std::shared_ptr<Dust> dust;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    dust = g_dust;
+  }
+  const int ncont = dust ? dust->ncont : 0;
+  std::vector<double> ct((size_t)L * ncont);
+  for (int j = 0; j < L; j++)
+    for (int k = 0; k < ncont; k++) ct[(size_t)j * ncont + k] = cont[(size_t)j * RADG_MAXCON + k];
   radg_request q{};
   q.nlayer = L;
   q.npath = npath;
@@ -1552,6 +1647,13 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   q.vem = ve.data();
   q.emissivity = em.data();
   q.path_weight = nullptr;
+  q.ncont = ncont;
+  if (ncont > 0) {
+    q.nsec = dust->nsec;
+    q.cont = ct.data();
+    q.vsec = dust->vsec.data();
+    q.xsec = dust->xsec.data();
+  }
   radg_batch *b = nullptr;
   int rc = radg_batch_create(1, &q, g_mode, 0, &b);
   if (rc) return rc;
@@ -1560,7 +1662,7 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   if ((rc = radg_batch_sync(b))) return rc;
   out.assign((size_t)nwave * npath, 0.0);
   gts.assign((size_t)nwave * npath, 0.0);
-  jac.assign((size_t)nwave * npath * L * (ngas + 1), 0.0);
+  jac.assign((size_t)nwave * npath * L * (ngas + 1 + ncont), 0.0);
   return radg_batch_read(b, 0, out.data(), gts.data(), jac.data(), nullptr, nullptr, nullptr);
 }
 
@@ -1568,16 +1670,17 @@ template <typename R>
 int spectrum_fixed(const R *dist, const int *inormal, const int *iray, const R *delh,
                    const int *nlayer, const int *npath, const int *ngas, const R *press,
                    const R *temp, const R *pp, const R *amount, const int *ispace,
-                   const int *nlayin, const int *layinc, const R *scale, const int *imod,
-                   const int *idgas, const int *isogas, const R *emtemp, const int *itype,
-                   const int *nem, const R *vem, const R *emissivity, const R *tsurf,
-                   const int *flagh2p, const int *nsw, const int *isw, int nwave, const R *vwave,
-                   R *output, R *gtsurf, R *doutputdq) {
+                   const int *nlayin, const int *layinc, const R *cont, const R *scale,
+                   const int *imod, const int *idgas, const int *isogas, const R *emtemp,
+                   const int *itype, const int *nem, const R *vem, const R *emissivity,
+                   const R *tsurf, const int *flagh2p, const int *nsw, const int *isw, int nwave,
+                   const R *vwave, R *output, R *gtsurf, R *doutputdq) {
   (void)dist;
   std::vector<double> o, g, j;
   int rc = run_fixed<R>(delh, *nlayer, *npath, *ngas, press, temp, pp, amount, *ispace, *inormal,
                         *iray, *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas,
-                        emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, o, g, j);
+                        emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, cont, o,
+                        g, j);
   if (rc) return rc;
   for (size_t i = 0; i < o.size(); i++) {
     output[i] = (R)o[i];
@@ -1601,15 +1704,15 @@ void cirsradg_wave_(const float *dist, const int *inormal, const int *iray, cons
                     float *gtsurf, const float *radius1, const int *flagh2p, const float *hfp,
                     const float *radextra, const int *nsw, const int *isw, float *output,
                     float *doutputdq) {
-  (void)dist; (void)iwave; (void)amform; (void)cont; (void)radius1; (void)hfp; (void)radextra;
+  (void)dist; (void)iwave; (void)amform; (void)radius1; (void)hfp; (void)radextra;
   std::vector<double> o, g, j;
   const int L = *nlayer, np_ = *npath, ng = *ngas;
   int rc = run_fixed<float>(delh, L, np_, ng, press, temp, pp, amount, *ispace, *inormal, *iray,
                             *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
-                            *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, o, g, j);
+                            *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, cont, o, g, j);
   g_cirs_status = rc;
   if (rc) return;
-  const int nparam = ng + 1;
+  const int nparam = (int)(j.size() / ((size_t)np_ * L));  // ngas + 1 + ncont
   for (int p = 0; p < np_; p++) {
     output[p] = (float)o[p];
     gtsurf[p] = (float)g[p];
@@ -1626,13 +1729,15 @@ int radg_cirsradg_status(void) { return g_cirs_status; }
 int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray, const float *delh,
                      const int *nlayer, const int *npath, const int *ngas, const float *press,
                      const float *temp, const float *pp, const float *amount, const int *ispace,
-                     const int *nlayin, const int *layinc, const float *scale, const int *imod,
-                     const int *idgas, const int *isogas, const float *emtemp, const int *itype,
-                     const int *nem, const float *vem, const float *emissivity, const float *tsurf,
-                     const int *flagh2p, const int *nsw, const int *isw, int nwave,
-                     const float *vwave, float *output, float *gtsurf, float *doutputdq) {
+                     const int *nlayin, const int *layinc, const float *cont, const float *scale,
+                     const int *imod, const int *idgas, const int *isogas, const float *emtemp,
+                     const int *itype, const int *nem, const float *vem, const float *emissivity,
+                     const float *tsurf, const int *flagh2p, const int *nsw, const int *isw,
+                     int nwave, const float *vwave, float *output, float *gtsurf,
+                     float *doutputdq) {
   return spectrum_fixed<float>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
-                               amount, ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
+                               amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas,
+                               emtemp,
                                itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave,
                                output, gtsurf, doutputdq);
 }
@@ -1640,14 +1745,15 @@ int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray, con
 int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, const double *delh,
                      const int *nlayer, const int *npath, const int *ngas, const double *press,
                      const double *temp, const double *pp, const double *amount, const int *ispace,
-                     const int *nlayin, const int *layinc, const double *scale, const int *imod,
-                     const int *idgas, const int *isogas, const double *emtemp, const int *itype,
-                     const int *nem, const double *vem, const double *emissivity,
-                     const double *tsurf, const int *flagh2p, const int *nsw, const int *isw,
-                     int nwave, const double *vwave, double *output, double *gtsurf,
-                     double *doutputdq) {
+                     const int *nlayin, const int *layinc, const double *cont,
+                     const double *scale, const int *imod, const int *idgas, const int *isogas,
+                     const double *emtemp, const int *itype, const int *nem, const double *vem,
+                     const double *emissivity, const double *tsurf, const int *flagh2p,
+                     const int *nsw, const int *isw, int nwave, const double *vwave,
+                     double *output, double *gtsurf, double *doutputdq) {
   return spectrum_fixed<double>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
-                                amount, ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
+                                amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas,
+                                emtemp,
                                 itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave,
                                 output, gtsurf, doutputdq);
 }

@@ -10,12 +10,15 @@
 #include <stdint.h>
 
 #define RADG_WAVE 64
-#define RADG_MAXPARAM 31  // ngas + 1 with ngas <= 30 (no dust, no para-H2)
+#define RADG_MAXPARAM 41  // ngas + 1 + ncont with ngas <= 30, ncont <= 10 (no para-H2)
 
 struct DevReq {
   int L, ngas, npath, nparam, nsw, nwave;
   int ispace, tsurf_le0, inormal;
   int iray;                    // Rayleigh form 1..4, 0 = off (iray > 0 and itype != 0)
+  int ncont, pad1;             // dust types; nparam = ngas + 1 + ncont
+  long long dx0;               // per-(wavenumber, dust type) extinction
+  long long ct0;               // per-(layer, dust type) particle columns
   double tsurf;
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
@@ -102,6 +105,8 @@ struct KernelArgs {
   const double *vwave, *esurf;
   const WaveGas *wg;
   const double *pweight;       // [path]
+  const double *dext;          // [wavenumber][ncont] dust extinction (cm2) at vwave
+  const double *cont;          // [layer][ncont] dust particles/cm2
   // outputs
   void *inter;                 // intermediates, compute type
   void *out, *gts, *jac;       // compute type; jac gas columns per 1e20 molecules

@@ -1086,7 +1086,13 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   T *dtcon = tcon + L;
   double absorb, dab[7];
   const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
+  // dust (cirsradg_wave.f:550-577), first as in the reference: extinction at
+  // vwave (interpolated on the host) times particles/cm2
+  double tdust = 0.0;
+  for (int k = 0; k < R.ncont; k++)
+    tdust += a.dext[R.dx0 + (long long)iw * R.ncont + k] * a.cont[R.ct0 + (long long)j * R.ncont + k];
   cia_layer(a, vv, R.inormal, lc, &absorb, dab);
+  absorb = tdust + absorb;
   // Rayleigh (cirsradg_wave.f:663-691): utotl * xray more extinction and
   // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.  The gas
   // rows of dtcon, like dk/dq, are per 1e20 molecules (the 1e-20 is applied
@@ -1361,8 +1367,12 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       const int m = lane * RL + q;
       for (int k1 = 0; k1 < nsw; k1++) {
         const int k = R.isw[k1];
-        const T dq = (dtcon[(long long)k * L + Lm[q]] +
-                      dkq[((long long)g * (ngas + 1) + k) * L + Lm[q]]) * sc[q];
+        // gases and T: continuum + overlap derivatives; dust type k-ngas-1:
+        // d(tau)/d(particles/cm2) is its extinction, the same in every layer
+        const T dq = (k <= ngas ? dtcon[(long long)k * L + Lm[q]] +
+                                      dkq[((long long)g * (ngas + 1) + k) * L + Lm[q]]
+                                : (T)a.dext[R.dx0 + (long long)iw * R.ncont + (k - ngas - 1)]) *
+                     sc[q];
         T c = dRdtau[q] * dq;
         if (k == ngas) c = c + dT[q];
         const T x = c * dg;

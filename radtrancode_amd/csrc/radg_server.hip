@@ -39,6 +39,7 @@ using clk = std::chrono::steady_clock;
 struct OwnedRequest {
   radg_request q{};
   std::vector<double> press, temp, delh, pp, amount, scale, emtemp, vwave, vem, emis, pw;
+  std::vector<double> cont, vsec, xsec;
   std::vector<int> idgas, isogas, nlayin, imod, layinc, isw;
 
   explicit OwnedRequest(const radg_request &s) {
@@ -79,6 +80,19 @@ struct OwnedRequest {
     q.vem = vem.data();
     q.emissivity = emis.data();
     q.path_weight = pw.data();
+    // dust arrays are copied only when their sizes are in range; otherwise the
+    // pointers stay as given and validation at submit rejects the request
+    if (s.ncont > 0 && s.ncont <= RADG_MAXCON && s.nsec > 0 && s.nsec <= RADG_MAXSEC && s.cont &&
+        s.vsec && s.xsec) {
+      cont.assign(s.cont, s.cont + L * s.ncont);
+      vsec.assign(s.vsec, s.vsec + s.nsec);
+      xsec.assign(s.xsec, s.xsec + (size_t)s.ncont * s.nsec);
+      q.cont = cont.data();
+      q.vsec = vsec.data();
+      q.xsec = xsec.data();
+    } else {
+      q.cont = q.vsec = q.xsec = nullptr;
+    }
   }
 };
 

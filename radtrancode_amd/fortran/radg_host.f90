@@ -15,7 +15,8 @@ program radg_host
   integer, parameter :: maxsec = 16000, nparmax = maxgas + 2 + maxcon
   character(len=512) :: fin, fout, klist
   integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
-  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode
+  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont
+  real(c_double), allocatable :: vsec(:), xsec(:)
   real(c_double) :: dnu, ctemps(25)
   real(c_double), allocatable :: kcia(:, :, :)
   real(c_float) :: dist, tsurf, radius1, radextra, vv
@@ -43,13 +44,18 @@ program radg_host
   allocate (delh(nlayer), vwave(nwave))
   read (10) delh, press, temp, pp, amount, nlayin, layinc, cont, scale, imod, idgas, isogas
   read (10) emtemp, vem, emissivity, hfp, isw, vwave
+  ! /DUST/: nsec, ncont, then vsec(nsec) and xsec(nsec,ncont) when ncont > 0
+  read (10) nsec, ncont
+  allocate (vsec(max(nsec, 1)), xsec(max(nsec * ncont, 1)))
+  if (ncont > 0) read (10) vsec(1:nsec), xsec(1:nsec * ncont)
   close (10)
 
   if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
   if (radg_register_cia(dnu, ctemps, kcia) /= RADG_OK) call die()
   if (radg_set_mode(mode) /= RADG_OK) call die()
+  if (radg_register_dust(nsec, ncont, vsec, xsec) /= RADG_OK) call die()
 
-  nparam = ngas + 1
+  nparam = ngas + 1 + ncont
   allocate (doutputdq(maxpat, maxlay, nparmax))
   open (11, file=trim(fout), access='stream', form='unformatted', status='replace')
   do iwave = 1, nwave
@@ -63,7 +69,7 @@ program radg_host
   end do
   allocate (sout(nwave * npath), sgts(nwave * npath), sdq(nwave * npath * nlayer * nparam))
   rc = radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp, amount, &
-                        ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp, itype, nem, &
+                        ispace, nlayin, layinc, cont, scale, imod, idgas, isogas, emtemp, itype, nem, &
                         vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave, sout, sgts, sdq)
   if (rc /= RADG_OK) call die()
   write (11) sout, sgts, sdq

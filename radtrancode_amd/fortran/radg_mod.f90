@@ -4,9 +4,10 @@
 !     CALL cirsradg_wave(dist,inormal,iray,delh,nlayer,npath,ngas,press,...)
 ! through its implicit interface: the library exports the link name
 ! cirsradg_wave_ with the same 37 by-reference arguments.  This module adds
-! the entry points with no Fortran counterpart: registering the k-tables and
-! CIA table that the reference keeps in /INTERPK/ and /CIATABLE/, choosing the
-! arithmetic form, the whole-spectrum call and the status query.
+! the entry points with no Fortran counterpart: registering the k-tables, CIA
+! table and dust cross-sections that the reference keeps in /INTERPK/,
+! /CIATABLE/ and /DUST/, choosing the arithmetic form, the whole-spectrum call
+! and the status query.
 module radg
   use iso_c_binding
   implicit none
@@ -15,6 +16,7 @@ module radg
   integer(c_int), parameter, public :: RADG_OK = 0
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
+  public :: radg_register_dust
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -32,14 +34,18 @@ module radg
     type(c_ptr) :: scale, emtemp
     type(c_ptr) :: isw
     type(c_ptr) :: vwave, vem, emissivity, path_weight
+    integer(c_int) :: ncont, nsec            ! dust: 0 = none
+    type(c_ptr) :: cont, vsec, xsec          ! cont(ncont,nlayer), vsec(nsec), xsec(nsec,ncont)
   end type
 
-  ! radg_statemap: the /LGRAD/ DAM and DTE and map2xvec's xmap of one request
-  ! (DAM(nlayer,npro) in C order is dam(npro,nlayer) in Fortran order, and
-  ! xmap(nv,ngas+1,npro) in C order is xmap(npro,ngas+1,nv)).
+  ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
+  ! request (DAM(nlayer,npro) in C order is dam(npro,nlayer) in Fortran order,
+  ! and xmap(nv,nparam,npro) in C order is xmap(npro,nparam,nv)); dco may be
+  ! c_null_ptr for a request without dust.
   type, bind(C), public :: radg_statemap
     integer(c_int) :: nv, npro
     type(c_ptr) :: dam, dte, xmap
+    type(c_ptr) :: dco
   end type
 
   interface
@@ -60,6 +66,15 @@ module radg
       real(c_double) :: temps(25), kcia(9, 25, 1501)
     end function
 
+    ! /DUST/: xsec(nsec,ncont) extinction cross-sections (cm2) on vsec(nsec);
+    ! ncont = 0 clears it
+    integer(c_int) function radg_register_dust(nsec, ncont, vsec, xsec) &
+        bind(C, name="radg_register_dust")
+      import :: c_int, c_double
+      integer(c_int), value :: nsec, ncont
+      real(c_double) :: vsec(*), xsec(*)
+    end function
+
     integer(c_int) function radg_cirsradg_status() bind(C, name="radg_cirsradg_status")
       import :: c_int
     end function
@@ -71,11 +86,11 @@ module radg
     ! The wavenumber loop of cirsrtfg_wave.f:361-448 in one call; outputs
     ! output(npath,nwave) ... in Fortran order: see include/radg.h.
     integer(c_int) function radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, &
-        press, temp, pp, amount, ispace, nlayin, layinc, scale, imod, idgas, isogas, emtemp, &
-        itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave, output, gtsurf, &
-        doutputdq) bind(C, name="radg_spectrum_r4")
+        press, temp, pp, amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas, &
+        emtemp, itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave, output, &
+        gtsurf, doutputdq) bind(C, name="radg_spectrum_r4")
       import :: c_int, c_float
-      real(c_float) :: dist, delh(*), press(*), temp(*), pp(*), amount(*), scale(*)
+      real(c_float) :: dist, delh(*), press(*), temp(*), pp(*), amount(*), scale(*), cont(*)
       real(c_float) :: emtemp(*), vem(*), emissivity(*), tsurf, vwave(*)
       real(c_float) :: output(*), gtsurf(*), doutputdq(*)
       integer(c_int) :: inormal, iray, nlayer, npath, ngas, ispace, nlayin(*), layinc(*)

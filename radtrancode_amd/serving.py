@@ -39,7 +39,7 @@ class Server:
         t = C.c_longlong()
         rc = lib().radg_server_submit(self.h, C.cast(arr, C.c_void_p), C.byref(t))
         with self._lock:
-            self._meta[t.value] = (req.nwave, req.npath, req.nlayer, req.ngas + 1)
+            self._meta[t.value] = (req.nwave, req.npath, req.nlayer, req.nparam)
         if rc != 0:
             raise RadgError(rc, lib().radg_last_error().decode())
         return t.value

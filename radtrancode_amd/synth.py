@@ -122,6 +122,21 @@ class Request:
     radius1: float = 71492.0
     flagh2p: int = 0
     radextra: float = 0.0
+    # dust (cirsradg_wave.f:550-577): cont [L, ncont] particles cm-2; the
+    # cross-section grid vsec [nsec] (cm-1) and extinction xsec [ncont, nsec]
+    # (cm2 per particle) that the reference reads from /DUST/
+    cont: Optional[np.ndarray] = None
+    vsec: Optional[np.ndarray] = None
+    xsec: Optional[np.ndarray] = None
+
+    @property
+    def ncont(self) -> int:
+        return 0 if self.cont is None else int(self.cont.shape[1])
+
+    @property
+    def nparam(self) -> int:
+        # cirsradg_wave.f:375-376
+        return self.ngas + 1 + self.ncont + (1 if self.flagh2p == 1 else 0)
 
     @property
     def nlayer(self) -> int:
@@ -139,10 +154,6 @@ class Request:
     def nwave(self) -> int:
         return int(self.vwave.shape[0])
 
-    @property
-    def nparam(self) -> int:
-        # cirsradg_wave.f:375-376 with ncont = 0
-        return self.ngas + 1 + (1 if self.flagh2p == 1 else 0)
 
 
 def gauss_legendre_g(ng: int):

@@ -20,6 +20,7 @@ _REQ_ARRAYS = ["press", "temp", "pp", "amount", "delh", "idgas", "isogas", "vwav
                "hfp", "vem", "emissivity"]
 _REQ_SCALARS = ["dist", "inormal", "iray", "ispace", "amform", "itype", "tsurf",
                 "radius1", "flagh2p", "radextra"]
+_REQ_OPTIONAL = ["cont", "vsec", "xsec"]  # dust, present only when ncont > 0
 _KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
 _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 
@@ -30,6 +31,9 @@ def save_case(path, tables, req, outputs: dict, note: str = ""):
         d["req_" + a] = np.asarray(getattr(req, a))
     for s in _REQ_SCALARS:
         d["req_" + s] = np.asarray(getattr(req, s))
+    for a in _REQ_OPTIONAL:
+        if getattr(req, a) is not None:
+            d["req_" + a] = np.asarray(getattr(req, a))
     d["ntab"] = np.array(len(tables))
     for t, kt in enumerate(tables):
         for a in _KT_ARRAYS + _KT_SCALARS:
@@ -45,6 +49,9 @@ def load_case(path):
     for s in _REQ_SCALARS:
         v = z["req_" + s]
         kw[s] = v.item()
+    for a in _REQ_OPTIONAL:
+        if "req_" + a in z.files:
+            kw[a] = z["req_" + a]
     req = Request(**kw)
     tables = []
     for t in range(int(z["ntab"])):

@@ -107,6 +107,16 @@ def cases():
     r.scale[3] = 1.7
     yield "partial_paths", t, r, "a half-depth path (surface test fails) and an odd scale"
 
+    t, c, r = synth.config1(nwave=8, nlayer=30, nkgas=2, ng=10)
+    r = add_dust(r, 2, 40, seed=21)
+    yield "dust", t, r, "two dust types (cloud deck, haze) on a 40-point cross-section grid"
+
+    t, c, r = synth.config1(nwave=5, nlayer=24, nkgas=3, ng=8)
+    r = add_dust(r, 1, 1, seed=22, peak_tau=(3.0,))
+    r.isw = np.array([1, 3, 4, 6, 7], dtype=np.int32)
+    yield "dust_const", t, r, ("one dust type with one cross-section (nsec = 1), optically "
+                               "thick; isw skips gases 2 and 5")
+
     # Rayleigh (iray > 0, itype = 11 as forwardavfovX sets it).  Above the
     # CIA table's 1500 cm-1 the H2 and He Jacobian columns are Rayleigh alone.
     # At 1450 cm-1 (rayleigh_cia) the CIA terms dominate and Rayleigh adds
@@ -118,6 +128,35 @@ def cases():
         yield f"rayleigh_{tag}", t, r, (f"iray={iray} itype=11 at {v0:.0f} cm-1"
                                         + (", CH4 and NH3 present" if iray == 4 else "")
                                         + (", CIA overlapping" if v0 < 1500 else ", no CIA"))
+
+
+def add_dust(r, ncont, nsec, seed, peak_tau=(1.5, 0.4)):
+    """Dust on a request (cirsradg_wave.f:550-577): particle type 0 is a
+    cloud deck near 0.7 bar, type 1 a haze spread over the upper layers;
+    extinction cross-sections ~3e-8 cm2 varying smoothly over a ``nsec``-point
+    grid that spans the wavenumbers (``nsec = 1``: one constant value)."""
+    rng = np.random.default_rng(seed)
+    L = r.nlayer
+    lp = np.log(r.press * 1.01325)  # ln(bar)
+    prof = [np.exp(-0.5 * ((lp - np.log(0.7)) / 0.35) ** 2),
+            np.exp(-0.5 * ((lp - np.log(0.01)) / 1.5) ** 2)]
+    if nsec == 1:
+        vsec = np.array([0.5 * (r.vwave[0] + r.vwave[-1])])
+    else:
+        vsec = np.linspace(r.vwave[0] - 7.3, r.vwave[-1] + 11.9, nsec)
+    xsec = np.empty((ncont, nsec))
+    cont = np.empty((L, ncont))
+    for k in range(ncont):
+        a, b = rng.uniform(0.2, 0.5, 2)
+        ph = rng.uniform(0, 2 * np.pi)
+        x = (vsec - vsec[0]) / max(vsec[-1] - vsec[0], 1.0)
+        xsec[k] = 3e-8 * (1 + a * np.sin(2 * np.pi * x + ph) + b * x)
+        c = prof[k % 2] * rng.uniform(0.8, 1.2, L)
+        # scale so that the column optical depth is peak_tau at the grid centre
+        cont[:, k] = c * peak_tau[k % 2] / (c.sum() * np.median(xsec[k]))
+    r.cont, r.vsec, r.xsec = cont, vsec, xsec
+    r.isw = np.arange(1, r.ngas + 2 + ncont, dtype=np.int32)
+    return r
 
 
 def rayleigh_inputs(iray, seed, v0, nkgas=3, nlayer=30, ng=10, nwave=6, delv=2.0,

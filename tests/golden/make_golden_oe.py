@@ -15,7 +15,10 @@ Cases:
   row sums of the gain matrix and the diagonal of the averaging kernels are
   stored;
 * ``map_small``: map2pro + map2xvec for 3 paths x 20 layers x 2 gases,
-  21 levels, 30 state elements.
+  21 levels, 30 state elements;
+* ``map_dust``: the same with 2 dust types mapped through DCO.
+
+``python tests/golden/make_golden_oe.py maps`` writes only the two map cases.
 """
 from __future__ import annotations
 
@@ -44,18 +47,24 @@ def oe_inputs(nx, ny, seed):
     return dict(kk=kk, sa=sa, se_var=se, xa=xa, xn=xn, y=y, yn=yn)
 
 
-def map_inputs(seed=11, npath=3, L=20, ngas=2, npro=21, nv=30):
+def map_inputs(seed=11, npath=3, L=20, ngas=2, npro=21, nv=30, ncont=0):
+    """Layer Jacobians and maps; with ``ncont`` dust types the Jacobian and
+    xmap gain their columns and a DCO matrix is added."""
     rng = np.random.default_rng(seed)
-    return dict(dq=rng.normal(size=(npath, L, ngas + 1)), nlayin=np.array([20, 18, 20]),
-                layinc=np.array([rng.permutation(L) + 1 for _ in range(npath)]),
-                dam=rng.normal(size=(L, npro)), dte=rng.normal(size=(L, npro)),
-                xmap=rng.normal(size=(nv, ngas + 1, npro)))
+    npar = ngas + 1 + ncont
+    m = dict(dq=rng.normal(size=(npath, L, npar)), nlayin=np.array([20, 18, 20]),
+             layinc=np.array([rng.permutation(L) + 1 for _ in range(npath)]),
+             dam=rng.normal(size=(L, npro)), dte=rng.normal(size=(L, npro)),
+             xmap=rng.normal(size=(nv, npar, npro)))
+    if ncont:
+        m.update(dco=rng.normal(size=(L, npro)), ncont=ncont)
+    return m
 
 
-def main():
+def main(only_maps=False):
     from oracle import oe_ref
     assert oe_ref.available(), "build oracle/_ref/liboeref64.so first (make -C oracle ref)"
-    for name, nx, ny, seed, full in [("oe_small", 12, 30, 1, True), ("oe_wide", 40, 25, 2, True),
+    for name, nx, ny, seed, full in [] if only_maps else [("oe_small", 12, 30, 1, True), ("oe_wide", 40, 25, 2, True),
                                      ("oe_medium", 121, 300, 3, True),
                                      ("oe_cfg4", 363, 1000, 4, False)]:
         inp = oe_inputs(nx, ny, seed)
@@ -67,11 +76,12 @@ def main():
             out.update(dd=r["dd"], aa=r["aa"], sai=r["sai"])
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print(name, nx, ny, "phi", r["phi"])
-    m = map_inputs()
-    dx = oe_ref.map_state(**m)
-    np.savez_compressed(os.path.join(HERE, "map_small.npz"), doutdx=dx, **m)
-    print("map_small", dx.shape)
+    for name, kw in (("map_small", {}), ("map_dust", dict(seed=12, ncont=2))):
+        m = map_inputs(**kw)
+        dx = oe_ref.map_state(**m)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), doutdx=dx, **m)
+        print(name, dx.shape)
 
 
 if __name__ == "__main__":
-    main()
+    main(only_maps=sys.argv[1:] == ["maps"])

@@ -37,10 +37,14 @@ def _write_input(path, klist, req, cia, mode):
         f.write(i4(a["imod"]) + i4(a["idgas"]) + i4(a["isogas"]))
         f.write(f4(a["emtemp"]) + f4(a["vem"]) + f4(a["emissivity"]) + f4(a["hfp"]))
         f.write(i4(a["isw"]) + f4(req.vwave))
+        f.write(i4([len(req.vsec) if req.ncont else 0, req.ncont]))
+        if req.ncont:
+            f.write(np.asarray(req.vsec, np.float64).tobytes()
+                    + np.ascontiguousarray(req.xsec, np.float64).tobytes())
 
 
 def _read_output(path, req):
-    nw, npath, L, npar = req.nwave, req.npath, req.nlayer, req.ngas + 1
+    nw, npath, L, npar = req.nwave, req.npath, req.nlayer, req.nparam
     raw = np.fromfile(path, dtype=np.float32)
     per = 2 * npath + npath * L * npar
     o = np.zeros((nw, npath))
@@ -59,7 +63,7 @@ def _read_output(path, req):
     return (o, g, d), (so, sg, sd)
 
 
-@pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths"])
+@pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths", "dust"])
 def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     host = rbuild.FORTRAN_HOST
     assert os.path.exists(host), "Fortran host not built (flang missing at build time?)"
@@ -100,8 +104,9 @@ def _rounded(req):
     import copy
     r = copy.copy(req)
     for k in ("press", "temp", "pp", "amount", "delh", "vwave", "scale", "emtemp", "vem",
-              "emissivity"):
-        setattr(r, k, np.asarray(getattr(req, k), dtype=np.float32).astype(np.float64))
+              "emissivity", "cont"):
+        if getattr(req, k) is not None:
+            setattr(r, k, np.asarray(getattr(req, k), dtype=np.float32).astype(np.float64))
     r.tsurf = float(np.float32(req.tsurf))
     return r
 

@@ -137,3 +137,33 @@ def test_small_retrieval_reduces_cost_and_error(gpu_lib):
     e1 = np.abs(out["x"][:, :nlev] - prob.x_true[:, :nlev]).mean()
     print(f"mean |T - T_true|: a priori {e0:.3f} K, retrieved {e1:.3f} K", flush=True)
     assert e1 < e0
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["fp64", "mixed"])
+def test_statemaps_with_dust(gpu_lib, mode):
+    """Dust columns map through DCO (map2pro.f:62-76): kk and doutdx equal
+    map2xvec(map2pro(.)) of the batch's own per-path Jacobians, and the
+    request's own k-tables and dust arrays ride along in the batch."""
+    from tests.golden.golden_io import case_paths, load_case
+    path = [p for p in case_paths() if p.endswith("case_dust.npz")][0]
+    tables, q, _, _ = load_case(path)
+    gpu_lib.register_ktables(tables)
+    rng = np.random.default_rng(8)
+    npro, nv = 25, 11
+    m = dict(dam=rng.normal(size=(q.nlayer, npro)) * 1e20, dte=rng.normal(size=(q.nlayer, npro)),
+             dco=rng.normal(size=(q.nlayer, npro)) * 1e7,
+             xmap=rng.normal(size=(nv, q.nparam, npro)))
+    print(f"statemaps with dust, mode {mode}", flush=True)
+    b = gpu_lib.Batch([q], mode=mode)
+    b.set_statemaps([m], per_path=True)
+    b.launch()
+    b.sync()
+    res = b.read(0, weighted=False)
+    mp = b.read_mapped(0, per_path=True)
+    for w in range(q.nwave):
+        dmod = oracle_oe.map2pro(res["doutputdq"][w], q.nlayin, q.layinc, m["dam"], m["dte"],
+                                 q.ngas, dco=m["dco"])
+        dx = oracle_oe.map2xvec(dmod, m["xmap"])
+        assert _rel(mp["doutdx"][w], dx) < 1e-9, w
+        assert _rel(mp["kk"][w], q.path_weight @ dx) < 1e-9, w
+    b.close()

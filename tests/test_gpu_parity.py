@@ -102,3 +102,18 @@ def test_rayleigh_fp64_tight(gpu_lib, path):
     o0, g0, d0 = gpu_lib.spectrum(off, mode=0)
     assert np.array_equal(o1, o0) and np.array_equal(d1, d0)
     assert rad_err(o0, outs["ref64_output"]) > 1e-7
+
+
+def test_dust_grid_must_cover_the_wavenumbers(gpu_lib):
+    """cirsradg_wave.f:446-452 stops for a wavenumber off the dust grid; here
+    the batch is refused with RADG_EINPUT before anything runs."""
+    import copy
+    from radtrancode_amd._native import RadgError
+    path = [p for p in CASES if p.endswith("case_dust.npz")][0]
+    tables, req, _, _ = load_case(path)
+    gpu_lib.register_ktables(tables)
+    bad = copy.copy(req)
+    bad.vsec = req.vsec + (req.vwave[-1] - req.vsec[0]) + 1.0
+    with pytest.raises(RadgError) as e:
+        gpu_lib.spectrum(bad, mode=0)
+    assert e.value.code == 4 and "dust" in str(e.value)  # RADG_EINPUT

@@ -559,8 +559,8 @@ double orc_rayleigh(int iray, double v, double p, double t, double fheh2, double
 
 /* ---------------------------------------------------------------- nciacon */
 /* nciacon.f:1-310.  iabs[5] are 1-based gas numbers (0 = absent),
- * dabsorb[7].  N2 (id 22) is rejected by the caller, so the analytic N2-N2
- * and N2-H2 terms, which carry a factor qn2, vanish. */
+ * dabsorb[7].  The near-infrared N2-N2 and N2-H2 terms (nciacon.f:287-296)
+ * vanish outside 3995-4995 cm-1, where the caller refuses N2. */
 void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
                  int ngas, const int *idgas, const int *isogas,
                  const double *amount, const double *pp, double xlen,
@@ -837,8 +837,12 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       (vwave < r->vsec[0] || vwave > r->vsec[r->nsec - 1]))
     return ORC_ERR_LIMITS;
   if (r->flagh2p == 1) return ORC_ERR_UNSUPPORTED;               /* nparacon_all */
-  for (k = 0; k < ngas; k++)
-    if (r->idgas[k] == 22) return ORC_ERR_UNSUPPORTED;          /* N2 analytic CIA */
+  /* N2's near-infrared CIA (n2n2cia.f, n2h2cia.f: tabulated, non-zero only
+   * for 3995 <= v <= 4995 cm-1) is not restated: N2 is refused there */
+  for (k = 0; k < ngas; k++) {
+    const double vn = r->ispace == 1 ? 1e4 / vwave : vwave;
+    if (r->idgas[k] == 22 && vn >= 3995.0 && vn <= 4995.0) return ORC_ERR_UNSUPPORTED;
+  }
 
   /* g-grid: all tables share it (read_klist.f:117-130). */
   for (k = 0; k < ngas; k++) {

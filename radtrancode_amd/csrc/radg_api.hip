@@ -208,8 +208,8 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   if (q.nsw < 0 || q.nsw > nparam) return fail(RADG_ELIMITS, "nsw=%d > nparam=%d", q.nsw, nparam);
   for (int k = 0; k < q.nsw; k++)
     if (q.isw[k] < 1 || q.isw[k] > nparam) return fail(RADG_EINPUT, "isw(%d)=%d outside 1..%d", k + 1, q.isw[k], nparam);
-  for (int k = 0; k < ngas; k++)
-    if (q.idgas[k] == 22) return fail(RADG_EUNSUPPORTED, "N2 (id 22) needs the analytic N2 CIA terms, not provided");
+  bool has_n2 = false;
+  for (int k = 0; k < ngas; k++) has_n2 = has_n2 || q.idgas[k] == 22;
   for (int j = 0; j < L; j++) {
     if (!finite_pos(q.press[j]) || !finite_pos(q.temp[j]) || !std::isfinite(q.delh[j]))
       return fail(RADG_EINPUT, "layer %d: press=%g temp=%g delh=%g", j + 1, q.press[j], q.temp[j], q.delh[j]);
@@ -379,6 +379,13 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   for (int iw = 0; iw < nwave; iw++) {
     const double v = q.vwave[iw];
     if (!std::isfinite(v) || v <= 0) return fail(RADG_EINPUT, "vwave(%d)=%g", iw + 1, v);
+    // N2's near-infrared CIA (n2n2cia.f, n2h2cia.f; nciacon.f:287-296) is
+    // tabulated for 3995-4995 cm-1 and zero elsewhere; it is not provided
+    {
+      const double vn = q.ispace == 1 ? 1e4 / v : v;
+      if (has_n2 && vn >= 3995.0 && vn <= 4995.0)
+        return fail(RADG_EUNSUPPORTED, "N2 near-infrared CIA (3995-4995 cm-1) is not provided");
+    }
     // interpem.f:17-30 (called unconditionally by cirsradg_wave.f:381)
     const int nem = q.nem;
     if (nem < 1 || v < q.vem[0] || v > q.vem[nem - 1])

@@ -25,8 +25,12 @@ _KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
 _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 
 
-def save_case(path, tables, req, outputs: dict, note: str = ""):
+def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = None):
+    """``cia``: a CIA table of the case's own, stored with it (else the
+    shared ``cia.npz`` applies)."""
     d = {"note": np.array(note)}
+    if cia is not None:
+        d.update(cia_dnu=np.array(cia.dnu), cia_temps=cia.temps, cia_kcia=cia.kcia)
     for a in _REQ_ARRAYS:
         d["req_" + a] = np.asarray(getattr(req, a))
     for s in _REQ_SCALARS:
@@ -65,6 +69,15 @@ def load_case(path):
 
 def save_cia(path, cia: CIATable):
     np.savez_compressed(path, dnu=np.array(cia.dnu), temps=cia.temps, kcia=cia.kcia)
+
+
+def case_cia(path) -> CIATable:
+    """The CIA table a case runs with: its own if it holds one, else the
+    shared one."""
+    z = np.load(path, allow_pickle=False)
+    if "cia_kcia" in z.files:
+        return CIATable(dnu=float(z["cia_dnu"]), temps=z["cia_temps"], kcia=z["cia_kcia"])
+    return load_cia()
 
 
 def load_cia(path=None) -> CIATable:

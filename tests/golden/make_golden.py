@@ -117,6 +117,20 @@ def cases():
     yield "dust_const", t, r, ("one dust type with one cross-section (nsec = 1), optically "
                                "thick; isw skips gases 2 and 5")
 
+    # Titan-like: N2 and CH4 with H2, the N2 and CH4 CIA pairs from a CIA
+    # table of the case's own (CASE_CIA).  Amounts are scaled by 0.05 and the
+    # k-tables centred on 1e-25 so the column stays semi-transparent: the
+    # N2 pairs then set 64% of the radiance and every output is within FP32
+    # range
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=2, ng=10)
+    ids = [6, 27]
+    t = synth.make_ktables(ids, t[0].vmin, t[0].delv, t[0].npoint, ng=10, seed=31, mu=-25.0)
+    r.idgas = np.array(ids + [synth.H2_ID, 22], dtype=np.int32)
+    q = np.array([0.015, 3e-6, 0.001, 0.984])
+    r.pp = r.press[:, None] * q[None, :] * 0.05
+    r.amount = r.amount.sum(axis=1)[:, None] * q[None, :] * 0.05
+    yield "titan_n2", t, r, "N2 + CH4 + H2 with the N2 and CH4 CIA pairs (own CIA table)"
+
     # Rayleigh (iray > 0, itype = 11 as forwardavfovX sets it).  Above the
     # CIA table's 1500 cm-1 the H2 and He Jacobian columns are Rayleigh alone.
     # At 1450 cm-1 (rayleigh_cia) the CIA terms dominate and Rayleigh adds
@@ -196,6 +210,25 @@ def rayleigh_vectors(n=64, seed=7):
     return out
 
 
+def banded_cia(cia, vwave, half=3):
+    """The table with every wavenumber row farther than ``half`` rows from
+    the case's wavenumbers set to zero: ciaread.f reads only the two rows
+    bracketing a wavenumber, so results are unchanged and the stored case
+    stays small."""
+    keep = np.zeros(cia.kcia.shape[0], dtype=bool)
+    for v in vwave:
+        i = int(v / cia.dnu)
+        keep[max(0, i - half):i + half + 2] = True
+    k = cia.kcia.copy()
+    k[~keep] = 0.0
+    return synth.CIATable(dnu=cia.dnu, temps=cia.temps, kcia=k)
+
+
+# cases that run with a CIA table of their own: every pair the Titan case
+# touches (H2-H2, H2-He, H2-N2, N2-CH4, N2-N2, CH4-CH4, H2-CH4)
+CASE_CIA = {"titan_n2": dict(seed=5, pairs=(0, 1, 4, 5, 6, 7, 8))}
+
+
 def main(names=()):
     """Regenerate every fixture, or only the named cases (``rayleigh`` is the
     function-level Rayleigh vectors)."""
@@ -207,9 +240,10 @@ def main(names=()):
     for name, t, r, note in cases():
         if names and name not in names:
             continue
-        outs = run_ref(t, cia, r)
+        own = banded_cia(synth.make_cia(**CASE_CIA[name]), r.vwave) if name in CASE_CIA else None
+        outs = run_ref(t, own or cia, r)
         path = os.path.join(HERE, f"case_{name}.npz")
-        save_case(path, t, r, outs, note)
+        save_case(path, t, r, outs, note, cia=own)
         print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
               f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
 

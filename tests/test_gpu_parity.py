@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from radtrancode_amd import synth
-from tests.golden.golden_io import case_paths, load_case, load_cia
+from tests.golden.golden_io import case_cia, case_paths, load_case, load_cia
 from tests.tolerance import (TOL_JAC_FP64, TOL_JAC_MIXED, TOL_RAD_FP64, TOL_RAD_MIXED,
                              jac_err, rad_err)
 
@@ -28,7 +28,14 @@ def _run(rad, tables, req, mode):
 def test_golden_case(gpu_lib, path, mode):
     tables, req, outs, note = load_case(path)
     print(f"case {path.split('/')[-1]} mode {mode}", flush=True)
-    o, g, d = _run(gpu_lib, tables, req, mode)
+    own = "cia_kcia" in np.load(path, allow_pickle=False).files
+    if own:  # a case with its own CIA table (the registry is global)
+        gpu_lib.register_cia(case_cia(path))
+    try:
+        o, g, d = _run(gpu_lib, tables, req, mode)
+    finally:
+        if own:
+            gpu_lib.register_cia(load_cia())
     tr, tj = (TOL_RAD_FP64, TOL_JAC_FP64) if mode == 0 else (TOL_RAD_MIXED, TOL_JAC_MIXED)
     er, eg, ej = (rad_err(o, outs["ref64_output"]), rad_err(g, outs["ref64_gtsurf"]),
                   jac_err(d, outs["ref64_doutputdq"]))
@@ -102,6 +109,24 @@ def test_rayleigh_fp64_tight(gpu_lib, path):
     o0, g0, d0 = gpu_lib.spectrum(off, mode=0)
     assert np.array_equal(o1, o0) and np.array_equal(d1, d0)
     assert rad_err(o0, outs["ref64_output"]) > 1e-7
+
+
+def test_n2_refused_only_in_its_near_infrared_window(gpu_lib):
+    """N2's tabulated near-infrared CIA (n2n2cia.f, n2h2cia.f) is not
+    provided: a wavenumber in 3995-4995 cm-1 with N2 present is refused with
+    RADG_EUNSUPPORTED; the same request elsewhere runs (test_golden_case
+    checks its values on the titan_n2 case)."""
+    import copy
+    from radtrancode_amd._native import RadgError
+    path = [p for p in CASES if p.endswith("case_titan_n2.npz")][0]
+    tables, req, _, _ = load_case(path)
+    gpu_lib.register_ktables(tables)
+    bad = copy.copy(req)
+    bad.vwave = req.vwave.copy()
+    bad.vwave[-1] = 4400.0
+    with pytest.raises(RadgError) as e:
+        gpu_lib.spectrum(bad, mode=0)
+    assert e.value.code == 2 and "N2" in str(e.value)  # RADG_EUNSUPPORTED
 
 
 def test_dust_grid_must_cover_the_wavenumbers(gpu_lib):

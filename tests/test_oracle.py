@@ -12,7 +12,7 @@ import pytest
 
 from oracle import oracle
 from radtrancode_amd import synth
-from tests.golden.golden_io import case_paths, load_case, load_cia
+from tests.golden.golden_io import case_cia, case_paths, load_case, load_cia
 from tests.tolerance import jac_err, rad_err
 
 CASES = case_paths()
@@ -21,7 +21,7 @@ CASES = case_paths()
 @pytest.mark.parametrize("path", CASES, ids=[p.split("case_")[-1][:-4] for p in CASES])
 def test_oracle_matches_reference_fp64(path):
     tables, req, outs, note = load_case(path)
-    cia = load_cia()
+    cia = case_cia(path)
     o, g, d = oracle.spectrum(tables, cia, req)
     assert rad_err(o, outs["ref64_output"]) < 1e-12, note
     assert rad_err(g, outs["ref64_gtsurf"]) < 1e-12, note

@@ -16,6 +16,7 @@
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
  *                            225,312) that the caller fills from its .xsc file
+ *   radg_set_nlte            /NLTE_FLAGS/ inlte_flag, nlte_k1, nlte_A (cirsradg_wave.f:328)
  *   radg_set_mode            (none: the reference's precision is fixed at compile time)
  *   radg_batch_*, radg_server_*  (none: device-resident batches and the serving loop
  *                            that replace nemesisPY/nemesis.py:4586-4664's process pool)
@@ -113,6 +114,12 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia);
  * ncont <= 10 (check_limits.f:24-33).  Batched requests carry their own. */
 int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xsec);
 
+/* /NLTE_FLAGS/ (cirsradg_wave.f:328) for the fixed-argument entry points:
+ * the emission temperature of every layer of an imod 3 path becomes
+ * calc_nlte_t.f's (flag 0 = LTE; 1 = collisional with k1 in [1e-20, 1e-3]
+ * and A; 2 = pressure form).  Batched requests carry their own. */
+int radg_set_nlte(int flag, double k1, double a);
+
 /* ------------------------------------------------ Fortran argument surface */
 
 /* Drop-in for CALL cirsradg_wave(...) from a REAL(4) host: same 37
@@ -192,6 +199,8 @@ typedef struct radg_request {
   const double *vwave, *vem, *emissivity, *path_weight;
   int ncont, nsec;
   const double *cont, *vsec, *xsec;
+  int inlte;                 /* /NLTE_FLAGS/ of calc_nlte_t.f: 0 LTE, 1, 2 */
+  double nlte_k1, nlte_a;
 } radg_request;
 
 typedef struct radg_batch radg_batch;

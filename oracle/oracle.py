@@ -67,7 +67,8 @@ class _REQ(C.Structure):
                 ("scale", C.c_void_p), ("emtemp", C.c_void_p), ("isw", C.c_void_p),
                 ("vem", C.c_void_p), ("emissivity", C.c_void_p),
                 ("ncont", C.c_int), ("nsec", C.c_int),
-                ("cont", C.c_void_p), ("vsec", C.c_void_p), ("xsec", C.c_void_p)]
+                ("cont", C.c_void_p), ("vsec", C.c_void_p), ("xsec", C.c_void_p),
+                ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double)]
 
 
 def _d(a):
@@ -108,7 +109,8 @@ class _Pack:
                         req.amform, req.dist, req.tsurf, req.radius1, req.radextra,
                         *[a.ctypes.data for a in f],
                         nc, len(req.vsec) if nc else 0,
-                        *([a.ctypes.data for a in dust] if nc else [None, None, None]))
+                        *([a.ctypes.data for a in dust] if nc else [None, None, None]),
+                        int(req.inlte), float(req.nlte_k1), float(req.nlte_a))
 
 
 def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):
@@ -153,6 +155,14 @@ def noverlapg(delg, amount, k_gn, dkgndt):
 
 def planck(ispace, v, t):
     return lib().orc_planck(int(ispace), float(v), float(t))
+
+
+def nlte_t(inlte, k1, a, t, p, v):
+    """calc_nlte_t.f: the non-LTE source temperature."""
+    f = lib().orc_nlte_t
+    f.restype = C.c_double
+    f.argtypes = [C.c_int] + [C.c_double] * 5
+    return f(int(inlte), float(k1), float(a), float(t), float(p), float(v))
 
 
 def rayleigh(iray, v, p=1.0, t=273.15, fheh2=0.0, fch4h2=0.0, fnh3=0.0):

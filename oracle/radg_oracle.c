@@ -18,6 +18,7 @@
  *   rayleigh*   radtran/scatter/rayleighj.f, rayleighv.f, rayleigha.f,
  *               rayleighls.f (the iray branch, cirsradg_wave.f:663-691)
  *   planck      radtran/radtran/planck_wave.f:1-42, planckg_wave.f:1-40
+ *   calc_nlte_t radtran/radtran/calc_nlte_t.f:1-89 (+ invplanck_wave.f)
  *   interpem    radtran/cirsrad/interpem.f:1-37
  *   interp      radtran/rtm_util/interp.f:1-26 (dust cross-sections)
  *   cirsradg    radtran/cirsradg/cirsradg_wave.f:333-1290 (imod 0, 1, 3)
@@ -470,6 +471,30 @@ static double orc_co2cia(double v0) {
   return a;
 }
 
+/* ----------------------------------------------------------- calc_nlte_t */
+/* calc_nlte_t.f:59-85: the temperature whose Planck radiance is f times
+ * the LTE one.  cirsradg_wave.f:933-935 passes its wavenumber index iwave
+ * (>= 1) as the Planck form, so both planck_wave and invplanck_wave take
+ * their iwave != 0 (wavelength) branch whatever ispace is; that is kept. */
+double orc_nlte_t(int inlte, double k1, double a, double t, double p, double v) {
+  const double kboltz = 1.380658e-23, c1 = 1.1911e-12, c2 = 1.439;
+  double f, b, y;
+  if (inlte == 1) {
+    const double ntot = (p * 101325. / (kboltz * t)) * 1.0e-6;
+    f = (k1 * ntot) / (a + k1 * ntot);
+  } else if (inlte == 2) {
+    f = p * 1.0e6 / (1.0 + p * 1.0e6);
+  } else {
+    return t;
+  }
+  /* planck_wave.f with iwave != 0 at T, then invplanck_wave.f of B*f */
+  y = 1e4 / v;
+  b = c1 * (y * y * y * y * y) / 1e4 / (exp(c2 * y / t) - 1.);
+  b = b * f;
+  if (b == 0.0 || v == 0.0) return 0.0;
+  return (c2 * 1.0e4 / v) / log((c1 * 1.0e16) / (b * (v * v * v * v * v)) + 1.0);
+}
+
 /* ---------------------------------------------------------------- interp */
 /* rtm_util/interp.f:1-26: linear interpolation (and extrapolation from the
  * end intervals) on a monotonic grid; x, y 0-based here. */
@@ -837,6 +862,8 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       (vwave < r->vsec[0] || vwave > r->vsec[r->nsec - 1]))
     return ORC_ERR_LIMITS;
   if (r->flagh2p == 1) return ORC_ERR_UNSUPPORTED;               /* nparacon_all */
+  /* calc_nlte_t.f:63-67 stops for k1 outside [1e-20, 1e-3] */
+  if (r->inlte == 1 && (r->nlte_k1 < 1.0e-20 || r->nlte_k1 > 1.0e-3)) return ORC_ERR_LIMITS;
   /* N2's near-infrared CIA (n2n2cia.f, n2h2cia.f: tabulated, non-zero only
    * for 3995 <= v <= 4995 cm-1) is not restated: N2 is refused there */
   for (k = 0; k < ngas; k++) {
@@ -1039,10 +1066,12 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
         int jm = (int)(0.5 * nlays);
         double p1, p2;
         if (ig == 0) {
-          /* calc_nlte_t is the identity for inlte_flag = 0 */
+          /* cirsradg_wave.f:931-944: Planck at calc_nlte_t's temperature */
           for (j = 0; j < nlays; j++) {
-            bb[ip * L + j] = orc_planck(r->ispace, vwave, r->emtemp[ip * L + j]);
-            dbdt[ip * L + j] = orc_planckg(r->ispace, vwave, r->emtemp[ip * L + j]);
+            const double tn = orc_nlte_t(r->inlte, r->nlte_k1, r->nlte_a, r->emtemp[ip * L + j],
+                                         r->press[r->layinc[ip * L + j] - 1], vwave);
+            bb[ip * L + j] = orc_planck(r->ispace, vwave, tn);
+            dbdt[ip * L + j] = orc_planckg(r->ispace, vwave, tn);
           }
         }
         if (jac_mode == 0) {

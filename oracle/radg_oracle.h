@@ -54,6 +54,9 @@ typedef struct orc_req {
    * [ncont][nsec] (cirsradg_wave.f:225,312). ncont = 0: no dust. */
   int ncont, nsec;
   const double *cont, *vsec, *xsec;
+  /* /NLTE_FLAGS/ (cirsradg_wave.f:328): 0 = LTE */
+  int inlte;
+  double nlte_k1, nlte_a;
 } orc_req;
 
 /* Status codes (the reference STOPs instead). */
@@ -96,6 +99,7 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t,
                  int inormal, int ngas, const int *idgas, const int *isogas,
                  const double *amount, const double *pp, double xlen,
                  double *absorb, int *iabs, double *dabsorb);       /* nciacon.f */
+double orc_nlte_t(int inlte, double k1, double a, double t, double p, double v); /* calc_nlte_t.f */
 double orc_interp(const double *x, const double *y, int n, double xin); /* interp.f */
 double orc_rayleigh(int iray, double v, double p, double t, double fheh2,
                     double fch4h2, double fnh3);                    /* rayleigh*.f */

@@ -103,6 +103,10 @@ class RefRunner:
         temps = np.ascontiguousarray(cia.temps, dtype=rdt)
         kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
         self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        if req.inlte:
+            self.lib.hsetnlte_(_p(np.array([req.inlte], dtype=np.int32)),
+                               _p(np.array([req.nlte_k1], dtype=rdt)),
+                               _p(np.array([req.nlte_a], dtype=rdt)))
         if req.ncont:
             nsec = np.array([len(req.vsec)], dtype=np.int32)
             ncont = np.array([req.ncont], dtype=np.int32)

@@ -94,6 +94,18 @@ C     (albedo times extinction) is not read by the emission branches.
       RETURN
       END
 
+C     /NLTE_FLAGS/ (cirsradg_wave.f:328): flag, k1, A of calc_nlte_t.f
+      SUBROUTINE HSETNLTE(IFLAG, XK1, XA)
+      IMPLICIT NONE
+      INTEGER IFLAG, INLTE
+      REAL XK1, XA, NLTEK1, NLTEA
+      COMMON /NLTE_FLAGS/ INLTE, NLTEK1, NLTEA
+      INLTE = IFLAG
+      NLTEK1 = XK1
+      NLTEA = XA
+      RETURN
+      END
+
       SUBROUTINE HREADK(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
       IMPLICIT NONE
       CHARACTER*(*) KL

@@ -22,7 +22,8 @@ MODE_BF16 = 2
 # Every symbol the C header (include/radg.h) declares.
 EXPORTS = [
     "radg_set_mode", "radg_get_mode", "radg_last_error", "radg_register_ktables",
-    "radg_read_klist", "radg_register_cia", "radg_register_dust", "cirsradg_wave_",
+    "radg_read_klist", "radg_register_cia", "radg_register_dust", "radg_set_nlte",
+    "cirsradg_wave_",
     "radg_cirsradg_status",
     "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
     "radg_batch_launch", "radg_batch_sync", "radg_batch_read", "radg_batch_timings",
@@ -53,7 +54,8 @@ class Request(C.Structure):
                 ("isw", C.c_void_p), ("vwave", C.c_void_p), ("vem", C.c_void_p),
                 ("emissivity", C.c_void_p), ("path_weight", C.c_void_p),
                 ("ncont", C.c_int), ("nsec", C.c_int), ("cont", C.c_void_p),
-                ("vsec", C.c_void_p), ("xsec", C.c_void_p)]
+                ("vsec", C.c_void_p), ("xsec", C.c_void_p),
+                ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double)]
 
 
 _lib = None
@@ -81,6 +83,7 @@ def lib():
     L.radg_ktable_bf16.argtypes = [ip, vp, vp]
     L.radg_register_cia.argtypes = [C.c_double, vp, vp]
     L.radg_register_dust.argtypes = [ip, ip, vp, vp]
+    L.radg_set_nlte.argtypes = [ip, C.c_double, C.c_double]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
     L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]
     L.radg_batch_bytes.argtypes = [ip, vp, ip, C.POINTER(C.c_size_t)]

@@ -117,6 +117,10 @@ struct Dust {
 };
 std::shared_ptr<Dust> g_dust;
 
+// /NLTE_FLAGS/ for the fixed-argument entry points
+int g_inlte = 0;
+double g_nlte_k1 = 0.0, g_nlte_a = 0.0;
+
 bool valid_mode(int mode) {
   return mode == RADG_MODE_FP64 || mode == RADG_MODE_MIXED || mode == RADG_MODE_BF16;
 }
@@ -204,6 +208,10 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
     for (long long i = 0; i < (long long)L * ncont; i++)
       if (!std::isfinite(q.cont[i])) return fail(RADG_EINPUT, "dust column cont not finite");
   }
+  // calc_nlte_t.f:63-67 stops for k1 outside [1e-20, 1e-3]
+  if (q.inlte == 1 && !(q.nlte_k1 >= 1.0e-20 && q.nlte_k1 <= 1.0e-3))
+    return fail(RADG_EINPUT, "inlte=1 needs 1e-20 <= k1 <= 1e-3 (k1=%g)", q.nlte_k1);
+  if (q.inlte == 1 && !std::isfinite(q.nlte_a)) return fail(RADG_EINPUT, "nlte A=%g", q.nlte_a);
   const int nparam = ngas + 1 + ncont;  // cirsradg_wave.f:375
   if (q.nsw < 0 || q.nsw > nparam) return fail(RADG_ELIMITS, "nsw=%d > nparam=%d", q.nsw, nparam);
   for (int k = 0; k < q.nsw; k++)
@@ -241,6 +249,9 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   d.npath = npath;
   d.nparam = nparam;
   d.ncont = ncont;
+  d.inlte = q.inlte;
+  d.nlte_k1 = q.nlte_k1;
+  d.nlte_a = q.nlte_a;
   d.nsw = q.nsw;
   d.nwave = nwave;
   d.ispace = q.ispace;
@@ -907,6 +918,16 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia) {
   HIPCHK(hipMemcpy(c->d_temps, temps, 25 * sizeof(double), hipMemcpyHostToDevice));
   std::lock_guard<std::mutex> lk(g_mu);
   g_cia = c;
+  return RADG_OK;
+}
+
+int radg_set_nlte(int flag, double k1, double a) {
+  if (flag == 1 && !(k1 >= 1.0e-20 && k1 <= 1.0e-3))
+    return fail(RADG_EINPUT, "inlte=1 needs 1e-20 <= k1 <= 1e-3 (k1=%g)", k1);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_inlte = flag;
+  g_nlte_k1 = k1;
+  g_nlte_a = a;
   return RADG_OK;
 }
 
@@ -1621,6 +1642,14 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
     dust = g_dust;
   }
   const int ncont = dust ? dust->ncont : 0;
+  int inlte;
+  double nk1, na;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    inlte = g_inlte;
+    nk1 = g_nlte_k1;
+    na = g_nlte_a;
+  }
   std::vector<double> ct((size_t)L * ncont);
   for (int j = 0; j < L; j++)
     for (int k = 0; k < ncont; k++) ct[(size_t)j * ncont + k] = cont[(size_t)j * RADG_MAXCON + k];
@@ -1654,6 +1683,9 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   q.vem = ve.data();
   q.emissivity = em.data();
   q.path_weight = nullptr;
+  q.inlte = inlte;
+  q.nlte_k1 = nk1;
+  q.nlte_a = na;
   q.ncont = ncont;
   if (ncont > 0) {
     q.nsec = dust->nsec;

@@ -16,10 +16,12 @@ struct DevReq {
   int L, ngas, npath, nparam, nsw, nwave;
   int ispace, tsurf_le0, inormal;
   int iray;                    // Rayleigh form 1..4, 0 = off (iray > 0 and itype != 0)
-  int ncont, pad1;             // dust types; nparam = ngas + 1 + ncont
+  int ncont;                   // dust types; nparam = ngas + 1 + ncont
+  int inlte;                   // calc_nlte_t.f flag (0 = LTE)
   long long dx0;               // per-(wavenumber, dust type) extinction
   long long ct0;               // per-(layer, dust type) particle columns
   double tsurf;
+  double nlte_k1, nlte_a;      // calc_nlte_t.f constants
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
   long long lay0;              // LayerConst / per-layer arrays

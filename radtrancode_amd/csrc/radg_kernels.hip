@@ -1143,6 +1143,28 @@ __device__ __forceinline__ double planckg_d(int ispace, double v, double t) {
   return ex * a / ((ex - 1.0) * (ex - 1.0));
 }
 
+// calc_nlte_t.f:59-85: the temperature whose Planck radiance is f times the
+// LTE one (f from collisions, inlte = 1, or from pressure, inlte = 2).  The
+// reference hands cirsradg_wave's wavenumber index (>= 1) to planck_wave and
+// invplanck_wave as their spectral-form switch, so both use the 1e4/v form
+// whatever ispace is; kept.
+__device__ __forceinline__ double nlte_t_d(int inlte, double k1, double a, double t, double p,
+                                           double v) {
+  double f;
+  if (inlte == 1) {
+    const double n = (p * 101325. / (1.380658e-23 * t)) * 1.0e-6;
+    f = (k1 * n) / (a + k1 * n);
+  } else if (inlte == 2) {
+    f = p * 1.0e6 / (1.0 + p * 1.0e6);
+  } else {
+    return t;
+  }
+  const double c1 = 1.1911e-12, c2 = 1.439, y = 1e4 / v;
+  const double b = c1 * (y * y * y * y * y) / 1e4 / (exp(c2 * y / t) - 1.0) * f;
+  if (b == 0.0) return 0.0;
+  return (c2 * 1.0e4 / v) / log(c1 * 1.0e16 / (b * (v * v * v * v * v)) + 1.0);
+}
+
 template <typename T>
 struct Kahan {
   T s, c;
@@ -1206,7 +1228,8 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     bb[q] = 0;
     db[q] = 0;
     if (act[q] && pi.imod == 3) {
-      const double em = a.emtemp[R.pl0 + (long long)p * L + m];
+      double em = a.emtemp[R.pl0 + (long long)p * L + m];
+      if (R.inlte) em = nlte_t_d(R.inlte, R.nlte_k1, R.nlte_a, em, a.press[R.lay0 + Lm[q]], v);
       bb[q] = planck_d(R.ispace, v, em);
       db[q] = (T)planckg_d(R.ispace, v, em);
     }

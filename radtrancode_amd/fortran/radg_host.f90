@@ -15,7 +15,8 @@ program radg_host
   integer, parameter :: maxsec = 16000, nparmax = maxgas + 2 + maxcon
   character(len=512) :: fin, fout, klist
   integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
-  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont
+  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont, inlte
+  real(c_double) :: nlte_k1, nlte_a
   real(c_double), allocatable :: vsec(:), xsec(:)
   real(c_double) :: dnu, ctemps(25)
   real(c_double), allocatable :: kcia(:, :, :)
@@ -48,12 +49,15 @@ program radg_host
   read (10) nsec, ncont
   allocate (vsec(max(nsec, 1)), xsec(max(nsec * ncont, 1)))
   if (ncont > 0) read (10) vsec(1:nsec), xsec(1:nsec * ncont)
+  ! /NLTE_FLAGS/
+  read (10) inlte, nlte_k1, nlte_a
   close (10)
 
   if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
   if (radg_register_cia(dnu, ctemps, kcia) /= RADG_OK) call die()
   if (radg_set_mode(mode) /= RADG_OK) call die()
   if (radg_register_dust(nsec, ncont, vsec, xsec) /= RADG_OK) call die()
+  if (radg_set_nlte(inlte, nlte_k1, nlte_a) /= RADG_OK) call die()
 
   nparam = ngas + 1 + ncont
   allocate (doutputdq(maxpat, maxlay, nparmax))

@@ -16,7 +16,7 @@ module radg
   integer(c_int), parameter, public :: RADG_OK = 0
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
-  public :: radg_register_dust
+  public :: radg_register_dust, radg_set_nlte
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -36,6 +36,8 @@ module radg
     type(c_ptr) :: vwave, vem, emissivity, path_weight
     integer(c_int) :: ncont, nsec            ! dust: 0 = none
     type(c_ptr) :: cont, vsec, xsec          ! cont(ncont,nlayer), vsec(nsec), xsec(nsec,ncont)
+    integer(c_int) :: inlte                  ! /NLTE_FLAGS/ of calc_nlte_t.f
+    real(c_double) :: nlte_k1, nlte_a
   end type
 
   ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
@@ -73,6 +75,13 @@ module radg
       import :: c_int, c_double
       integer(c_int), value :: nsec, ncont
       real(c_double) :: vsec(*), xsec(*)
+    end function
+
+    ! /NLTE_FLAGS/: inlte_flag, nlte_k1, nlte_A
+    integer(c_int) function radg_set_nlte(flag, k1, a) bind(C, name="radg_set_nlte")
+      import :: c_int, c_double
+      integer(c_int), value :: flag
+      real(c_double), value :: k1, a
     end function
 
     integer(c_int) function radg_cirsradg_status() bind(C, name="radg_cirsradg_status")

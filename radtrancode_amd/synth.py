@@ -128,6 +128,11 @@ class Request:
     cont: Optional[np.ndarray] = None
     vsec: Optional[np.ndarray] = None
     xsec: Optional[np.ndarray] = None
+    # non-LTE source function (calc_nlte_t.f, /NLTE_FLAGS/): 0 LTE, 1 the
+    # k1/A collisional form, 2 the pressure form
+    inlte: int = 0
+    nlte_k1: float = 0.0
+    nlte_a: float = 0.0
 
     @property
     def ncont(self) -> int:

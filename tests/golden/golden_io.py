@@ -21,6 +21,7 @@ _REQ_ARRAYS = ["press", "temp", "pp", "amount", "delh", "idgas", "isogas", "vwav
 _REQ_SCALARS = ["dist", "inormal", "iray", "ispace", "amform", "itype", "tsurf",
                 "radius1", "flagh2p", "radextra"]
 _REQ_OPTIONAL = ["cont", "vsec", "xsec"]  # dust, present only when ncont > 0
+_REQ_OPT_SCALARS = ["inlte", "nlte_k1", "nlte_a"]  # absent from older cases: LTE
 _KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
 _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 
@@ -38,6 +39,8 @@ def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = 
     for a in _REQ_OPTIONAL:
         if getattr(req, a) is not None:
             d["req_" + a] = np.asarray(getattr(req, a))
+    for s in _REQ_OPT_SCALARS:
+        d["req_" + s] = np.asarray(getattr(req, s))
     d["ntab"] = np.array(len(tables))
     for t, kt in enumerate(tables):
         for a in _KT_ARRAYS + _KT_SCALARS:
@@ -56,6 +59,9 @@ def load_case(path):
     for a in _REQ_OPTIONAL:
         if "req_" + a in z.files:
             kw[a] = z["req_" + a]
+    for s in _REQ_OPT_SCALARS:
+        if "req_" + s in z.files:
+            kw[s] = z["req_" + s].item()
     req = Request(**kw)
     tables = []
     for t in range(int(z["ntab"])):

@@ -117,6 +117,17 @@ def cases():
     yield "dust_const", t, r, ("one dust type with one cross-section (nsec = 1), optically "
                                "thick; isw skips gases 2 and 5")
 
+    # non-LTE source function (calc_nlte_t.f).  k1 = 1e-11 cm3/s and
+    # A = 1e5 /s put f = k1 N / (A + k1 N) near 0.3 at the top and 1 at depth
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.inlte, r.nlte_k1, r.nlte_a = 1, 1e-11, 1e5
+    yield "nlte_k1", t, r, "non-LTE, collisional form (inlte = 1, k1 = 1e-11, A = 1e5)"
+
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.inlte = 2
+    r.tsurf = 160.0
+    yield "nlte_p", t, r, "non-LTE, pressure form (inlte = 2), surface at 160 K"
+
     # Titan-like: N2 and CH4 with H2, the N2 and CH4 CIA pairs from a CIA
     # table of the case's own (CASE_CIA).  Amounts are scaled by 0.05 and the
     # k-tables centred on 1e-25 so the column stays semi-transparent: the

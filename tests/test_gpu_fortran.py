@@ -41,6 +41,7 @@ def _write_input(path, klist, req, cia, mode):
         if req.ncont:
             f.write(np.asarray(req.vsec, np.float64).tobytes()
                     + np.ascontiguousarray(req.xsec, np.float64).tobytes())
+        f.write(i4(req.inlte) + np.array([req.nlte_k1, req.nlte_a], np.float64).tobytes())
 
 
 def _read_output(path, req):
@@ -63,7 +64,7 @@ def _read_output(path, req):
     return (o, g, d), (so, sg, sd)
 
 
-@pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths", "dust"])
+@pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths", "dust", "nlte_k1"])
 def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     host = rbuild.FORTRAN_HOST
     assert os.path.exists(host), "Fortran host not built (flang missing at build time?)"

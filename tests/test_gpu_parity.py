@@ -142,3 +142,17 @@ def test_dust_grid_must_cover_the_wavenumbers(gpu_lib):
     with pytest.raises(RadgError) as e:
         gpu_lib.spectrum(bad, mode=0)
     assert e.value.code == 4 and "dust" in str(e.value)  # RADG_EINPUT
+
+
+def test_nlte_k1_range_is_checked(gpu_lib):
+    """calc_nlte_t.f:63-67 stops for k1 outside [1e-20, 1e-3]: RADG_EINPUT."""
+    import copy
+    from radtrancode_amd._native import RadgError
+    path = [p for p in CASES if p.endswith("case_nlte_k1.npz")][0]
+    tables, req, _, _ = load_case(path)
+    gpu_lib.register_ktables(tables)
+    bad = copy.copy(req)
+    bad.nlte_k1 = 1e-2
+    with pytest.raises(RadgError) as e:
+        gpu_lib.spectrum(bad, mode=0)
+    assert e.value.code == 4 and "k1" in str(e.value)

@@ -17,6 +17,7 @@
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
  *                            225,312) that the caller fills from its .xsc file
  *   radg_set_nlte            /NLTE_FLAGS/ inlte_flag, nlte_k1, nlte_A (cirsradg_wave.f:328)
+ *   radg_set_iform           /SOLARDAT/ iform, iread (cirsradg_wave.f:307-309)
  *   radg_set_mode            (none: the reference's precision is fixed at compile time)
  *   radg_batch_*, radg_server_*  (none: device-resident batches and the serving loop
  *                            that replace nemesisPY/nemesis.py:4586-4664's process pool)
@@ -120,6 +121,12 @@ int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xs
  * and A; 2 = pressure form).  Batched requests carry their own. */
 int radg_set_nlte(int flag, double k1, double a);
 
+/* /SOLARDAT/ iform and iread for the fixed-argument entry points (which take
+ * radius1 and radextra as arguments).  A solar spectrum file (iread = 999),
+ * which iform 1 and 4 divide or multiply by (cirsradg_wave.f:866-870,
+ * 919-925), is not provided: that combination is refused. */
+int radg_set_iform(int iform, int iread);
+
 /* ------------------------------------------------ Fortran argument surface */
 
 /* Drop-in for CALL cirsradg_wave(...) from a REAL(4) host: same 37
@@ -144,7 +151,8 @@ int radg_cirsradg_status(void);
 
 /* The whole wavenumber loop in one call.  Arguments as cirsradg_wave_ (fixed
  * leading dimensions; cont(maxcon,maxlay) is read for the registered dust
- * types), plus nwave and vwave[nwave].  Outputs, C order:
+ * types; radius1 and radextra scale imod 3 paths when radg_set_iform chose
+ * iform 1 or 3), plus nwave and vwave[nwave].  Outputs, C order:
  *   output[nwave][npath], gtsurf[nwave][npath],
  *   doutputdq[nwave][npath][nlayer][ngas+1+ncont]   (params: gas amounts
  *   1..ngas, then temperature, then the registered dust types; per molecule
@@ -159,6 +167,7 @@ int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray,
                      const int *isogas,
                      const float *emtemp, const int *itype, const int *nem,
                      const float *vem, const float *emissivity, const float *tsurf,
+                     const float *radius1, const float *radextra,
                      const int *flagh2p, const int *nsw, const int *isw,
                      int nwave, const float *vwave, float *output, float *gtsurf,
                      float *doutputdq);
@@ -171,6 +180,7 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
                      const int *isogas,
                      const double *emtemp, const int *itype, const int *nem,
                      const double *vem, const double *emissivity, const double *tsurf,
+                     const double *radius1, const double *radextra,
                      const int *flagh2p, const int *nsw, const int *isw,
                      int nwave, const double *vwave, double *output, double *gtsurf,
                      double *doutputdq);
@@ -201,6 +211,9 @@ typedef struct radg_request {
   const double *cont, *vsec, *xsec;
   int inlte;                 /* /NLTE_FLAGS/ of calc_nlte_t.f: 0 LTE, 1, 2 */
   double nlte_k1, nlte_a;
+  int iform;                 /* /SOLARDAT/ iform: 1 and 3 give imod 3 paths the */
+  double radius1, radextra;  /* planet's total flux, 4 pi^2 ((radius1 + radextra)
+                                km in cm)^2, times 1e-18 for 3 (cirsradg_wave.f:912-925) */
 } radg_request;
 
 typedef struct radg_batch radg_batch;

@@ -68,7 +68,8 @@ class _REQ(C.Structure):
                 ("vem", C.c_void_p), ("emissivity", C.c_void_p),
                 ("ncont", C.c_int), ("nsec", C.c_int),
                 ("cont", C.c_void_p), ("vsec", C.c_void_p), ("xsec", C.c_void_p),
-                ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double)]
+                ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double),
+                ("iform", C.c_int)]
 
 
 def _d(a):
@@ -110,7 +111,7 @@ class _Pack:
                         *[a.ctypes.data for a in f],
                         nc, len(req.vsec) if nc else 0,
                         *([a.ctypes.data for a in dust] if nc else [None, None, None]),
-                        int(req.inlte), float(req.nlte_k1), float(req.nlte_a))
+                        int(req.inlte), float(req.nlte_k1), float(req.nlte_a), int(req.iform))
 
 
 def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):

@@ -19,6 +19,7 @@
  *               rayleighls.f (the iray branch, cirsradg_wave.f:663-691)
  *   planck      radtran/radtran/planck_wave.f:1-42, planckg_wave.f:1-40
  *   calc_nlte_t radtran/radtran/calc_nlte_t.f:1-89 (+ invplanck_wave.f)
+ *   dpexp       radtran/radtran/dpexp.f:1-14
  *   interpem    radtran/cirsrad/interpem.f:1-37
  *   interp      radtran/rtm_util/interp.f:1-26 (dust cross-sections)
  *   cirsradg    radtran/cirsradg/cirsradg_wave.f:333-1290 (imod 0, 1, 3)
@@ -469,6 +470,15 @@ static double orc_co2cia(double v0) {
   if (xl >= 1.125 && xl <= 1.225) a = 0.5 * (0.31 + 0.79) * 1e-9;
   if (xl >= 1.06 && xl < 1.125) a = 0.5 * (0.29 + 0.67) * 1e-9;
   return a;
+}
+
+/* ----------------------------------------------------------------- dpexp */
+/* dpexp.f:1-14: exp with its argument clamped to [-500, 88.7] (imod 0 and 1
+ * transmissions, cirsradg_wave.f:861,888) */
+static double orc_dpexp(double x) {
+  double y = x > -500.0 ? x : -500.0;
+  if (y >= 88.7) y = 88.7;
+  return exp(y);
 }
 
 /* ----------------------------------------------------------- calc_nlte_t */
@@ -1045,14 +1055,14 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
         double tmp = 0.0;
         for (j = 0; j < nlays; j++) tmp += taus[j];
         if (r->imod[ip] == 0) {
-          co = exp(-tmp);
+          co = orc_dpexp(-tmp);
           for (j = 0; j < nlays; j++)
             for (k1 = 0; k1 < r->nsw; k1++) {
               k = r->isw[k1] - 1;
               dco[j * nparam + k] = -tmp * co * dtaudq[j * nparam + k];
             }
         } else {
-          co = 1.0 - exp(-tmp);
+          co = 1.0 - orc_dpexp(-tmp);
           for (j = 0; j < nlays; j++)
             for (k1 = 0; k1 < r->nsw; k1++) {
               k = r->isw[k1] - 1;
@@ -1060,8 +1070,14 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
             }
         }
       } else {
-        /* imod = 3, cirsradg_wave.f:896-1035; iform = 0 so xfac = 1 */
-        const double xfac = 1.0;
+        /* imod = 3, cirsradg_wave.f:896-1035.  xfac, :912-925 with no solar
+         * file (iread != 999): the planet's total flux for iform 1 and 3,
+         * with a further 1e-18 for iform 3 */
+        double xfac = 1.0;
+        if (r->iform == 1 || r->iform == 3)
+          xfac = xfac * PI_REF * 4. * PI_REF * ((r->radius1 + r->radextra) * 1e5) *
+                 ((r->radius1 + r->radextra) * 1e5);
+        if (r->iform == 3) xfac = xfac * 1e-18;
         double trold = 1.0, tr, tlayer, radground, gradground;
         int jm = (int)(0.5 * nlays);
         double p1, p2;

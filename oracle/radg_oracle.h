@@ -57,6 +57,9 @@ typedef struct orc_req {
   /* /NLTE_FLAGS/ (cirsradg_wave.f:328): 0 = LTE */
   int inlte;
   double nlte_k1, nlte_a;
+  /* /SOLARDAT/ iform (no solar file, iread != 999): 1 and 3 scale imod 3
+   * radiances to the planet's total flux (cirsradg_wave.f:912-925) */
+  int iform;
 } orc_req;
 
 /* Status codes (the reference STOPs instead). */

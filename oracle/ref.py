@@ -103,6 +103,8 @@ class RefRunner:
         temps = np.ascontiguousarray(cia.temps, dtype=rdt)
         kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
         self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        if req.iform:
+            self.lib.hsetiform_(_p(np.array([req.iform], dtype=np.int32)))
         if req.inlte:
             self.lib.hsetnlte_(_p(np.array([req.inlte], dtype=np.int32)),
                                _p(np.array([req.nlte_k1], dtype=rdt)),

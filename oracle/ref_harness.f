@@ -106,6 +106,18 @@ C     /NLTE_FLAGS/ (cirsradg_wave.f:328): flag, k1, A of calc_nlte_t.f
       RETURN
       END
 
+C     /SOLARDAT/ (cirsradg_wave.f:308-309): IFORM, with no solar file
+      SUBROUTINE HSETIFORM(JFORM)
+      IMPLICIT NONE
+      INCLUDE 'arrdef.f'
+      INTEGER JFORM, IREAD, IFORM, NSPT
+      REAL SOLRAD, SWAVE(MAXBIN), SRAD(MAXBIN)
+      COMMON /SOLARDAT/ IREAD, IFORM, SOLRAD, SWAVE, SRAD, NSPT
+      IFORM = JFORM
+      IREAD = 0
+      RETURN
+      END
+
       SUBROUTINE HREADK(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
       IMPLICIT NONE
       CHARACTER*(*) KL

@@ -99,6 +99,12 @@ def register_dust(vsec=None, xsec=None) -> None:
     check(lib().radg_register_dust(int(vs.shape[0]), int(xs.shape[0]), _vp(vs), _vp(xs)))
 
 
+def set_iform(iform=0, iread=0) -> None:
+    """``/SOLARDAT/`` iform (and iread, which must not be 999 with iform 1
+    or 4) for :func:`cirsradg_wave`."""
+    check(lib().radg_set_iform(int(iform), int(iread)))
+
+
 def set_nlte(flag=0, k1=0.0, a=0.0) -> None:
     """``/NLTE_FLAGS/`` of calc_nlte_t.f for :func:`cirsradg_wave`."""
     check(lib().radg_set_nlte(int(flag), float(k1), float(a)))
@@ -129,7 +135,8 @@ class _ReqArrays:
                                       self.layinc, self.scale, self.emtemp, self.isw,
                                       self.vwave, self.vem, self.emis, self.pw)],
             nc, len(dust[1]) if nc else 0, *([a.ctypes.data for a in dust] if nc else [None] * 3),
-            int(req.inlte), float(req.nlte_k1), float(req.nlte_a))
+            int(req.inlte), float(req.nlte_k1), float(req.nlte_a),
+            int(req.iform), float(req.radius1), float(req.radextra))
 
 
 class StateMap(C.Structure):

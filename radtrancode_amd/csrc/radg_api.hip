@@ -117,6 +117,9 @@ struct Dust {
 };
 std::shared_ptr<Dust> g_dust;
 
+// /SOLARDAT/ iform for the fixed-argument entry points
+int g_iform = 0;
+
 // /NLTE_FLAGS/ for the fixed-argument entry points
 int g_inlte = 0;
 double g_nlte_k1 = 0.0, g_nlte_a = 0.0;
@@ -252,6 +255,14 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   d.inlte = q.inlte;
   d.nlte_k1 = q.nlte_k1;
   d.nlte_a = q.nlte_a;
+  // cirsradg_wave.f:912-925 with no solar file; pi is constdef.f's 3.1415927
+  d.xfac = 1.0;
+  if (q.iform == 1 || q.iform == 3) {
+    const double pi = 3.1415927, rr = (q.radius1 + q.radextra) * 1e5;
+    d.xfac = d.xfac * pi * 4. * pi * (rr * rr);
+  }
+  if (q.iform == 3) d.xfac = d.xfac * 1e-18;
+  if (!std::isfinite(d.xfac)) return fail(RADG_EINPUT, "radius1=%g radextra=%g", q.radius1, q.radextra);
   d.nsw = q.nsw;
   d.nwave = nwave;
   d.ispace = q.ispace;
@@ -918,6 +929,14 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia) {
   HIPCHK(hipMemcpy(c->d_temps, temps, 25 * sizeof(double), hipMemcpyHostToDevice));
   std::lock_guard<std::mutex> lk(g_mu);
   g_cia = c;
+  return RADG_OK;
+}
+
+int radg_set_iform(int iform, int iread) {
+  if (iread == 999 && (iform == 1 || iform == 4))
+    return fail(RADG_EUNSUPPORTED, "iform=%d with a solar spectrum (iread=999) is not provided", iform);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_iform = iform;
   return RADG_OK;
 }
 
@@ -1613,8 +1632,8 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
               int flagh2p, const int *nlayin, const int *layinc, const R *scale, const int *imod,
               const int *idgas, const int *isogas, const R *emtemp, int nem, const R *vem,
               const R *emis, R tsurf, int nsw, const int *isw, int nwave, const R *vwave,
-              const R *cont, std::vector<double> &out, std::vector<double> &gts,
-              std::vector<double> &jac) {
+              const R *cont, R radius1, R radextra, std::vector<double> &out,
+              std::vector<double> &gts, std::vector<double> &jac) {
   if (L < 1 || L > RADG_MAXLAY || npath < 1 || npath > RADG_MAXPAT || ngas < 1 || ngas > RADG_MAXGAS)
     return fail(RADG_ELIMITS, "nlayer=%d npath=%d ngas=%d exceed the array limits", L, npath, ngas);
   if (nem < 1 || nem > RADG_MAXSEC) return fail(RADG_ELIMITS, "nem=%d", nem);
@@ -1642,10 +1661,11 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
     dust = g_dust;
   }
   const int ncont = dust ? dust->ncont : 0;
-  int inlte;
+  int inlte, iform;
   double nk1, na;
   {
     std::lock_guard<std::mutex> lk(g_mu);
+    iform = g_iform;
     inlte = g_inlte;
     nk1 = g_nlte_k1;
     na = g_nlte_a;
@@ -1686,6 +1706,9 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   q.inlte = inlte;
   q.nlte_k1 = nk1;
   q.nlte_a = na;
+  q.iform = iform;
+  q.radius1 = radius1;
+  q.radextra = radextra;
   q.ncont = ncont;
   if (ncont > 0) {
     q.nsec = dust->nsec;
@@ -1712,14 +1735,15 @@ int spectrum_fixed(const R *dist, const int *inormal, const int *iray, const R *
                    const int *nlayin, const int *layinc, const R *cont, const R *scale,
                    const int *imod, const int *idgas, const int *isogas, const R *emtemp,
                    const int *itype, const int *nem, const R *vem, const R *emissivity,
-                   const R *tsurf, const int *flagh2p, const int *nsw, const int *isw, int nwave,
-                   const R *vwave, R *output, R *gtsurf, R *doutputdq) {
+                   const R *tsurf, const R *radius1, const R *radextra, const int *flagh2p,
+                   const int *nsw, const int *isw, int nwave, const R *vwave, R *output,
+                   R *gtsurf, R *doutputdq) {
   (void)dist;
   std::vector<double> o, g, j;
   int rc = run_fixed<R>(delh, *nlayer, *npath, *ngas, press, temp, pp, amount, *ispace, *inormal,
                         *iray, *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas,
-                        emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, cont, o,
-                        g, j);
+                        emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, cont,
+                        *radius1, *radextra, o, g, j);
   if (rc) return rc;
   for (size_t i = 0; i < o.size(); i++) {
     output[i] = (R)o[i];
@@ -1743,12 +1767,13 @@ void cirsradg_wave_(const float *dist, const int *inormal, const int *iray, cons
                     float *gtsurf, const float *radius1, const int *flagh2p, const float *hfp,
                     const float *radextra, const int *nsw, const int *isw, float *output,
                     float *doutputdq) {
-  (void)dist; (void)iwave; (void)amform; (void)radius1; (void)hfp; (void)radextra;
+  (void)dist; (void)iwave; (void)amform; (void)hfp;
   std::vector<double> o, g, j;
   const int L = *nlayer, np_ = *npath, ng = *ngas;
   int rc = run_fixed<float>(delh, L, np_, ng, press, temp, pp, amount, *ispace, *inormal, *iray,
                             *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
-                            *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, cont, o, g, j);
+                            *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, cont, *radius1,
+                            *radextra, o, g, j);
   g_cirs_status = rc;
   if (rc) return;
   const int nparam = (int)(j.size() / ((size_t)np_ * L));  // ngas + 1 + ncont
@@ -1771,14 +1796,13 @@ int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray, con
                      const int *nlayin, const int *layinc, const float *cont, const float *scale,
                      const int *imod, const int *idgas, const int *isogas, const float *emtemp,
                      const int *itype, const int *nem, const float *vem, const float *emissivity,
-                     const float *tsurf, const int *flagh2p, const int *nsw, const int *isw,
-                     int nwave, const float *vwave, float *output, float *gtsurf,
-                     float *doutputdq) {
+                     const float *tsurf, const float *radius1, const float *radextra,
+                     const int *flagh2p, const int *nsw, const int *isw, int nwave,
+                     const float *vwave, float *output, float *gtsurf, float *doutputdq) {
   return spectrum_fixed<float>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
                                amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas,
-                               emtemp,
-                               itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave,
-                               output, gtsurf, doutputdq);
+                               emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra,
+                               flagh2p, nsw, isw, nwave, vwave, output, gtsurf, doutputdq);
 }
 
 int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, const double *delh,
@@ -1787,14 +1811,14 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, co
                      const int *nlayin, const int *layinc, const double *cont,
                      const double *scale, const int *imod, const int *idgas, const int *isogas,
                      const double *emtemp, const int *itype, const int *nem, const double *vem,
-                     const double *emissivity, const double *tsurf, const int *flagh2p,
-                     const int *nsw, const int *isw, int nwave, const double *vwave,
-                     double *output, double *gtsurf, double *doutputdq) {
+                     const double *emissivity, const double *tsurf, const double *radius1,
+                     const double *radextra, const int *flagh2p, const int *nsw, const int *isw,
+                     int nwave, const double *vwave, double *output, double *gtsurf,
+                     double *doutputdq) {
   return spectrum_fixed<double>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
                                 amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas,
-                                emtemp,
-                                itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave,
-                                output, gtsurf, doutputdq);
+                                emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra,
+                                flagh2p, nsw, isw, nwave, vwave, output, gtsurf, doutputdq);
 }
 
 // .kta reader (read_khead.f:18-120; regular grids, NT > 0)

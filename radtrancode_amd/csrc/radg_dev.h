@@ -22,6 +22,7 @@ struct DevReq {
   long long ct0;               // per-(layer, dust type) particle columns
   double tsurf;
   double nlte_k1, nlte_a;      // calc_nlte_t.f constants
+  double xfac;                 // imod 3 output scale (iform 1, 3: total flux)
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
   long long lay0;              // LayerConst / per-layer arrays

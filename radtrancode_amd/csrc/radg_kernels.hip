@@ -49,6 +49,13 @@ template <> __device__ __forceinline__ double dexp<double>(double x) { return ex
 template <typename T> __device__ __forceinline__ T dlog(T x);
 template <> __device__ __forceinline__ float dlog<float>(float x) { return logf(x); }
 template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
+// dpexp.f: exp with the argument clamped to [-500, 88.7] (the imod 0 and 1
+// transmissions, cirsradg_wave.f:861,888)
+template <typename T>
+__device__ __forceinline__ T dpexp(T x) {
+  const T y = x > (T)-500 ? x : (T)-500;
+  return dexp(y >= (T)88.7 ? (T)88.7 : y);
+}
 template <typename T> __device__ __forceinline__ T dexpm1(T x);
 template <> __device__ __forceinline__ float dexpm1<float>(float x) { return expm1f(x); }
 template <> __device__ __forceinline__ double dexpm1<double>(double x) { return expm1(x); }
@@ -1370,11 +1377,11 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       for (int q = 0; q < RL; q++) part += (double)tau[q];
       const T tmp = (T)wave_sum(part);
       if (pi.imod == 0) {
-        coR = dexp(-tmp);
+        coR = dpexp(-tmp);
 #pragma unroll
         for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? -tmp * coR : (T)0;
       } else {
-        coR = (T)1 - dexp(-tmp);
+        coR = (T)1 - dpexp(-tmp);
 #pragma unroll
         for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? tmp * ((T)1 - coR) : (T)0;
       }
@@ -1409,10 +1416,13 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     }
   }
   // ---- outputs ----
+  // xfac (cirsradg_wave.f:912-925) scales every imod 3 term: the planet's
+  // total flux for iform 1 and 3; 1 otherwise
+  const double xf = pi.imod == 3 ? R.xfac : 1.0;
   const long long ob = R.out0 + (long long)iw * npath + p;
   if (lane == 0) {
-    reinterpret_cast<T *>(a.out)[ob] = rad.s;
-    reinterpret_cast<T *>(a.gts)[ob] = gts.s;
+    reinterpret_cast<T *>(a.out)[ob] = (T)((double)rad.s * xf);
+    reinterpret_cast<T *>(a.gts)[ob] = (T)((double)gts.s * xf);
   }
   T *jac = reinterpret_cast<T *>(a.jac) + R.jac0 + ((long long)iw * npath + p) * L * nparam;
   // each lane writes whole rows: zeros, then its isw parameters
@@ -1428,7 +1438,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     if (!act[q]) continue;
     for (int k1 = 0; k1 < nsw; k1++) {
       const int k = R.isw[k1];
-      jac[(long long)m * nparam + k] = acc[m * nsw + k1];
+      jac[(long long)m * nparam + k] = (T)((double)acc[m * nsw + k1] * xf);
     }
   }
 }

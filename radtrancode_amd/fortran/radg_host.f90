@@ -15,7 +15,7 @@ program radg_host
   integer, parameter :: maxsec = 16000, nparmax = maxgas + 2 + maxcon
   character(len=512) :: fin, fout, klist
   integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
-  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont, inlte
+  integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont, inlte, iform
   real(c_double) :: nlte_k1, nlte_a
   real(c_double), allocatable :: vsec(:), xsec(:)
   real(c_double) :: dnu, ctemps(25)
@@ -51,6 +51,8 @@ program radg_host
   if (ncont > 0) read (10) vsec(1:nsec), xsec(1:nsec * ncont)
   ! /NLTE_FLAGS/
   read (10) inlte, nlte_k1, nlte_a
+  ! /SOLARDAT/ iform
+  read (10) iform
   close (10)
 
   if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
@@ -58,6 +60,7 @@ program radg_host
   if (radg_set_mode(mode) /= RADG_OK) call die()
   if (radg_register_dust(nsec, ncont, vsec, xsec) /= RADG_OK) call die()
   if (radg_set_nlte(inlte, nlte_k1, nlte_a) /= RADG_OK) call die()
+  if (radg_set_iform(iform, 0) /= RADG_OK) call die()
 
   nparam = ngas + 1 + ncont
   allocate (doutputdq(maxpat, maxlay, nparmax))
@@ -74,7 +77,8 @@ program radg_host
   allocate (sout(nwave * npath), sgts(nwave * npath), sdq(nwave * npath * nlayer * nparam))
   rc = radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp, amount, &
                         ispace, nlayin, layinc, cont, scale, imod, idgas, isogas, emtemp, itype, nem, &
-                        vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave, sout, sgts, sdq)
+                        vem, emissivity, tsurf, radius1, radextra, flagh2p, nsw, isw, nwave, vwave, &
+                        sout, sgts, sdq)
   if (rc /= RADG_OK) call die()
   write (11) sout, sgts, sdq
   close (11)

@@ -16,7 +16,7 @@ module radg
   integer(c_int), parameter, public :: RADG_OK = 0
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
-  public :: radg_register_dust, radg_set_nlte
+  public :: radg_register_dust, radg_set_nlte, radg_set_iform
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -38,6 +38,8 @@ module radg
     type(c_ptr) :: cont, vsec, xsec          ! cont(ncont,nlayer), vsec(nsec), xsec(nsec,ncont)
     integer(c_int) :: inlte                  ! /NLTE_FLAGS/ of calc_nlte_t.f
     real(c_double) :: nlte_k1, nlte_a
+    integer(c_int) :: iform                  ! /SOLARDAT/ iform (no solar file)
+    real(c_double) :: radius1, radextra      ! km
   end type
 
   ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
@@ -77,6 +79,12 @@ module radg
       real(c_double) :: vsec(*), xsec(*)
     end function
 
+    ! /SOLARDAT/: iform, iread (iread = 999 with iform 1 or 4 is refused)
+    integer(c_int) function radg_set_iform(iform, iread) bind(C, name="radg_set_iform")
+      import :: c_int
+      integer(c_int), value :: iform, iread
+    end function
+
     ! /NLTE_FLAGS/: inlte_flag, nlte_k1, nlte_A
     integer(c_int) function radg_set_nlte(flag, k1, a) bind(C, name="radg_set_nlte")
       import :: c_int, c_double
@@ -96,11 +104,11 @@ module radg
     ! output(npath,nwave) ... in Fortran order: see include/radg.h.
     integer(c_int) function radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, &
         press, temp, pp, amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas, &
-        emtemp, itype, nem, vem, emissivity, tsurf, flagh2p, nsw, isw, nwave, vwave, output, &
-        gtsurf, doutputdq) bind(C, name="radg_spectrum_r4")
+        emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra, flagh2p, nsw, isw, nwave, &
+        vwave, output, gtsurf, doutputdq) bind(C, name="radg_spectrum_r4")
       import :: c_int, c_float
       real(c_float) :: dist, delh(*), press(*), temp(*), pp(*), amount(*), scale(*), cont(*)
-      real(c_float) :: emtemp(*), vem(*), emissivity(*), tsurf, vwave(*)
+      real(c_float) :: emtemp(*), vem(*), emissivity(*), tsurf, vwave(*), radius1, radextra
       real(c_float) :: output(*), gtsurf(*), doutputdq(*)
       integer(c_int) :: inormal, iray, nlayer, npath, ngas, ispace, nlayin(*), layinc(*)
       integer(c_int) :: imod(*), idgas(*), isogas(*), itype, nem, flagh2p, nsw, isw(*)

@@ -133,6 +133,10 @@ class Request:
     inlte: int = 0
     nlte_k1: float = 0.0
     nlte_a: float = 0.0
+    # /SOLARDAT/ iform (cirsradg_wave.f:912-925): 1 and 3 turn imod 3
+    # radiances into the planet's total flux, 4 pi^2 (radius1 + radextra)^2
+    # (km -> cm), times 1e-18 for 3
+    iform: int = 0
 
     @property
     def ncont(self) -> int:

@@ -21,7 +21,7 @@ _REQ_ARRAYS = ["press", "temp", "pp", "amount", "delh", "idgas", "isogas", "vwav
 _REQ_SCALARS = ["dist", "inormal", "iray", "ispace", "amform", "itype", "tsurf",
                 "radius1", "flagh2p", "radextra"]
 _REQ_OPTIONAL = ["cont", "vsec", "xsec"]  # dust, present only when ncont > 0
-_REQ_OPT_SCALARS = ["inlte", "nlte_k1", "nlte_a"]  # absent from older cases: LTE
+_REQ_OPT_SCALARS = ["inlte", "nlte_k1", "nlte_a", "iform"]  # absent from older cases: 0
 _KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
 _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 

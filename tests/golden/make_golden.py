@@ -117,6 +117,28 @@ def cases():
     yield "dust_const", t, r, ("one dust type with one cross-section (nsec = 1), optically "
                                "thick; isw skips gases 2 and 5")
 
+    # wavelength space (ispace = 1): tables, vwave and emissivity grid in
+    # micron; CIA and Rayleigh at 1e4/vwave cm-1, Planck in its wavelength
+    # form, dust cross-sections on a micron grid
+    t, r = wavelength_inputs(41, 7.0, 0.01)
+    yield "wavelength", t, r, "ispace = 1 at 7.00-7.05 micron (CIA active)"
+
+    t, r = wavelength_inputs(42, 2.0, 0.01)
+    r.iray, r.itype = 1, 11
+    r = add_dust(r, 1, 12, seed=43, peak_tau=(0.8,))
+    yield "wavelength_ray_dust", t, r, "ispace = 1 at 2.00-2.05 micron with Rayleigh and one dust type"
+
+    # disc flux (iform = 3, as the direct-imaging example runs nemesisdisc):
+    # imod 3 outputs times 4 pi^2 ((radius1 + radextra) 1e5)^2 1e-18
+    t, r = wavelength_inputs(44, 3.0, 0.02)
+    r.iform, r.radius1, r.radextra = 3, 80000.0, 150.0
+    yield "iform3", t, r, "iform = 3 total flux, ispace = 1, radius1 + radextra = 80150 km"
+
+    t, c, r = synth.config1(nwave=5, nlayer=24, nkgas=2, ng=8)
+    r.iform, r.radius1 = 1, 60268.0
+    r.imod = np.array([3, 0, 3, 1, 3], dtype=np.int32)
+    yield "iform1", t, r, "iform = 1 total flux on imod 3 paths; imod 0 and 1 paths unscaled"
+
     # non-LTE source function (calc_nlte_t.f).  k1 = 1e-11 cm3/s and
     # A = 1e5 /s put f = k1 N / (A + k1 N) near 0.3 at the top and 1 at depth
     t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
@@ -182,6 +204,16 @@ def add_dust(r, ncont, nsec, seed, peak_tau=(1.5, 0.4)):
     r.cont, r.vsec, r.xsec = cont, vsec, xsec
     r.isw = np.arange(1, r.ngas + 2 + ncont, dtype=np.int32)
     return r
+
+
+def wavelength_inputs(seed, w0, dw, nwave=6, nkgas=3, nlayer=30, ng=10):
+    """A request in wavelength space: the table grid and vwave in micron."""
+    ids = list(synth.KGAS_IDS[:nkgas])
+    t = synth.make_ktables(ids, w0, dw, nwave, ng=ng, np_=10, nt=6, seed=seed)
+    r = synth.make_request(nlayer=nlayer, nkgas=nkgas, nwave=nwave, seed=seed,
+                           table_vmin=w0, table_delv=dw)
+    r.ispace = 1
+    return t, r
 
 
 def rayleigh_inputs(iray, seed, v0, nkgas=3, nlayer=30, ng=10, nwave=6, delv=2.0,

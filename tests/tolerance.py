@@ -19,6 +19,29 @@ TOL_RAD_MIXED = 1e-4
 TOL_JAC_MIXED = 1e-3
 
 
+# FP32's smallest normal number.  An FP32 output (the reference's REAL(4)
+# build, the mixed form) cannot hold an FP64 value below it, e.g. an opaque
+# path's transmission exp(-500) (dpexp.f): such reference values are
+# compared as 0.
+FP32_TINY = 1.1754943508222875e-38
+
+
+def fp32_range(b):
+    b = np.asarray(b, dtype=np.float64)
+    return np.where(np.abs(b) < FP32_TINY, 0.0, b)
+
+
+def fp32_range_jac(d, ngas, gas_scale=1.0):
+    """The Jacobian ``d[..., nparam]`` with entries an FP32 store cannot hold
+    set to 0: below FP32_TINY, or FP32_TINY * ``gas_scale`` in the first
+    ``ngas`` (gas) columns when the store holds them scaled (the mixed form
+    keeps them per 1e20 molecules: gas_scale = 1e-20)."""
+    d = np.asarray(d, dtype=np.float64)
+    floor = np.full(d.shape[-1], FP32_TINY)
+    floor[:ngas] *= gas_scale
+    return np.where(np.abs(d) < floor, 0.0, d)
+
+
 def rad_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -14,6 +14,7 @@
  *                            from .kta headers (radtran/cirsrad/read_khead.f:18-120)
  *   radg_register_ktables    the same registry from arrays already in memory
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
+ *   radg_register_parah2     /FPTABLE/ filled by radtran/radtran/fpread_all.f:72-143
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
  *                            225,312) that the caller fills from its .xsc file
  *   radg_set_nlte            /NLTE_FLAGS/ inlte_flag, nlte_k1, nlte_A (cirsradg_wave.f:328)
@@ -115,6 +116,14 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia);
  * ncont <= 10 (check_limits.f:24-33).  Batched requests carry their own. */
 int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xsec);
 
+/* /FPTABLE/, the para-H2 CIA that fpread_all.f reads (ciacom.f): npara
+ * para fractions fp[npara] (uniform, the first 0.25 as nparacon_all.f
+ * assumes), temps[25], kh2h2/kh2he[1501][25][npara] (kh2h2(npara,25,1501)),
+ * kcia5[1501][25][5] (H2-N2, N2-CH4, N2-N2, CH4-CH4, H2-CH4) on the grid
+ * (i-1)*dnu.  Used by requests with flagh2p = 1; npara = 0 clears it. */
+int radg_register_parah2(double dnu, const double *temps, int npara, const double *fp,
+                         const double *kh2h2, const double *kh2he, const double *kcia5);
+
 /* /NLTE_FLAGS/ (cirsradg_wave.f:328) for the fixed-argument entry points:
  * the emission temperature of every layer of an imod 3 path becomes
  * calc_nlte_t.f's (flag 0 = LTE; 1 = collisional with k1 in [1e-20, 1e-3]
@@ -168,7 +177,7 @@ int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray,
                      const float *emtemp, const int *itype, const int *nem,
                      const float *vem, const float *emissivity, const float *tsurf,
                      const float *radius1, const float *radextra,
-                     const int *flagh2p, const int *nsw, const int *isw,
+                     const int *flagh2p, const float *hfp, const int *nsw, const int *isw,
                      int nwave, const float *vwave, float *output, float *gtsurf,
                      float *doutputdq);
 int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
@@ -181,7 +190,7 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
                      const double *emtemp, const int *itype, const int *nem,
                      const double *vem, const double *emissivity, const double *tsurf,
                      const double *radius1, const double *radextra,
-                     const int *flagh2p, const int *nsw, const int *isw,
+                     const int *flagh2p, const double *hfp, const int *nsw, const int *isw,
                      int nwave, const double *vwave, double *output, double *gtsurf,
                      double *doutputdq);
 
@@ -214,6 +223,9 @@ typedef struct radg_request {
   int iform;                 /* /SOLARDAT/ iform: 1 and 3 give imod 3 paths the */
   double radius1, radextra;  /* planet's total flux, 4 pi^2 ((radius1 + radextra)
                                 km in cm)^2, times 1e-18 for 3 (cirsradg_wave.f:912-925) */
+  const double *hfp;         /* [L] para-H2 fractions when flagh2p = 1 (nparacon_all.f;
+                                < 0: the table's last fraction); fpara is then the last
+                                Jacobian parameter */
 } radg_request;
 
 typedef struct radg_batch radg_batch;
@@ -259,6 +271,8 @@ int radg_batch_destroy(radg_batch *b);
  *   dte[nlayer][npro]  d(layer T)/d(level T), /LGRAD/ DTE
  *   dco[nlayer][npro]  d(layer dust)/d(level dust), /LGRAD/ DCO (used for every
  *                      dust type, map2pro.f:72-76); may be NULL without dust
+ *   dfp[nlayer][npro]  d(layer para fraction)/d(level), /LGRAD/ DFP
+ *                      (map2pro.f:77-80); may be NULL unless flagh2p = 1
  *   xmap[nv][nparam][npro]  d(profile quantity)/d(state element), map2xvec.f's
  *                      xmap(maxv, maxgas+2+maxcon, maxpro) for the request's
  *                      nparam = ngas + 1 + ncont parameters
@@ -267,6 +281,7 @@ typedef struct radg_statemap {
   int nv, npro;
   const double *dam, *dte, *xmap;
   const double *dco;
+  const double *dfp;
 } radg_statemap;
 /* Attach maps[nreq] to a batch (copied to the device).  Every later launch
  * then also forms, on the device and in FP64,

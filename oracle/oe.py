@@ -118,18 +118,21 @@ def oe_step(kk, sa, se_var, xa, xn, y, yn):
                 chisq=chisq)
 
 
-def map2pro(dq, nlayin, layinc, dam, dte, ngas, dco=None):
+def map2pro(dq, nlayin, layinc, dam, dte, ngas, dco=None, dfp=None, ncont=None):
     """map2pro.f:49-84 for imod != 8.  ``dq`` [npath, L, nparam] (layer
     Jacobian, the cirsradg_wave doutputdq order: ngas gases, T, then dust
-    types), ``layinc`` [npath, L] 1-based, ``dam``/``dte``/``dco`` [L, npro]:
-    DAM for gases, DTE for T, DCO for every dust type (map2pro.f:62-76).
-    Returns doutmoddq [npath, nparam, npro]."""
+    types, then the para-H2 fraction), ``layinc`` [npath, L] 1-based,
+    ``dam``/``dte``/``dco``/``dfp`` [L, npro]: DAM for gases, DTE for T, DCO
+    for the ``ncont`` dust types, DFP after them (map2pro.f:62-80; ``ncont``
+    defaults to every remaining column).  Returns doutmoddq [npath, nparam,
+    npro]."""
     npath, _, npar = dq.shape
+    nc = npar - ngas - 1 if ncont is None else ncont
     out = np.zeros((npath, npar, dam.shape[1]))
     for p in range(npath):
         idx = np.asarray(layinc[p][:nlayin[p]]) - 1
         for k in range(npar):
-            d = dam if k < ngas else dte if k == ngas else dco
+            d = dam if k < ngas else dte if k == ngas else dco if k - ngas - 1 < nc else dfp
             out[p, k] = dq[p, :nlayin[p], k] @ d[idx]
     return out
 

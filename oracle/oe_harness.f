@@ -86,21 +86,23 @@ C imod, layinc, ngas, ncont; /MODCOM/ npro; /LGRAD/ DAM, DTE, DCO) and
 C runs map2pro then map2xvec as cirsrtfg_wave.f:403-425 does for one
 C wavenumber.  Compact arguments (Fortran order): NLAYC(NPTH),
 C LAYC(NLYR,NPTH) (1-based layer numbers), DAMC(NLYR,NPR), DTEC(NLYR,NPR),
-C DCOC(NLYR,NPR), XMAPC(NVC,NGS+1+NCT,NPR), DQC(NPTH,NLYR,NGS+1+NCT) for
-C NCT dust types; output DXC(NPTH,NVC).
+C DCOC(NLYR,NPR), DFPC(NLYR,NPR), XMAPC(NVC,NPAR,NPR), DQC(NPTH,NLYR,NPAR)
+C with NPAR = NGS+1+NCT+NFP for NCT dust types and NFP (0 or 1) para-H2
+C fractions; output DXC(NPTH,NVC).
 
-      SUBROUTINE HMAP(NPTH, NLYR, NGS, NCT, NPR, NVC, NLAYC, LAYC, DAMC,
-     1  DTEC, DCOC, XMAPC, DQC, DXC)
+      SUBROUTINE HMAP(NPTH, NLYR, NGS, NCT, NFP, NPR, NVC, NLAYC, LAYC,
+     1  DAMC, DTEC, DCOC, DFPC, XMAPC, DQC, DXC)
       IMPLICIT NONE
       INCLUDE 'arrdef.f'
       INCLUDE 'pathcom.f'
       INCLUDE 'laycom.f'
       INCLUDE 'laygrad.f'
-      INTEGER NPTH, NLYR, NGS, NCT, NPR, NVC, I, J, K, NPAR
+      INTEGER NPTH, NLYR, NGS, NCT, NFP, NPR, NVC, I, J, K, NPAR
       INTEGER NLAYC(NPTH), LAYC(NLYR,NPTH)
-      REAL DAMC(NLYR,NPR), DTEC(NLYR,NPR), DCOC(NLYR,NPR)
-      REAL XMAPC(NVC,NGS+1+NCT,NPR)
-      REAL DQC(NPTH,NLYR,NGS+1+NCT), DXC(NPTH,NVC)
+      REAL DAMC(NLYR,NPR), DTEC(NLYR,NPR)
+      REAL DCOC(NLYR,NPR), DFPC(NLYR,NPR)
+      REAL XMAPC(NVC,NGS+1+NCT+NFP,NPR)
+      REAL DQC(NPTH,NLYR,NGS+1+NCT+NFP), DXC(NPTH,NVC)
       REAL DQL(MAXPAT,MAXLAY,MAXGAS+2+MAXCON)
       REAL DPRF(MAXPAT,MAXGAS+2+MAXCON,MAXPRO)
       REAL XMV(MAXV,MAXGAS+2+MAXCON,MAXPRO), DXV(MAXPAT,MAXV)
@@ -110,7 +112,7 @@ C NCT dust types; output DXC(NPTH,NVC).
       NGAS = NGS
       NCONT = NCT
       NPRO = NPR
-      NPAR = NGS + 1 + NCT
+      NPAR = NGS + 1 + NCT + NFP
       DO J = 1, NPTH
         NLAYIN(J) = NLAYC(J)
         IMOD(J) = 3
@@ -123,6 +125,7 @@ C NCT dust types; output DXC(NPTH,NVC).
           DAM(I,J) = DAMC(I,J)
           DTE(I,J) = DTEC(I,J)
           DCO(I,J) = DCOC(I,J)
+          DFP(I,J) = DFPC(I,J)
         ENDDO
       ENDDO
       DO K = 1, NPAR

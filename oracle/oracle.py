@@ -50,7 +50,10 @@ class _KT(C.Structure):
 
 
 class _CIA(C.Structure):
-    _fields_ = [("dnu", C.c_double), ("temps", C.c_void_p), ("kcia", C.c_void_p)]
+    _fields_ = [("dnu", C.c_double), ("temps", C.c_void_p), ("kcia", C.c_void_p),
+                ("npara", C.c_int), ("dnu2", C.c_double), ("temps2", C.c_void_p),
+                ("fp", C.c_void_p), ("kh2h2", C.c_void_p), ("kh2he", C.c_void_p),
+                ("kcia5", C.c_void_p)]
 
 
 class _REQ(C.Structure):
@@ -94,7 +97,14 @@ class _Pack:
         self.kts, self.ntab = kts, len(tables)
         ct, kc = _d(cia.temps), _d(cia.kcia)
         self.keep += [ct, kc]
-        self.cia = _CIA(cia.dnu, ct.ctypes.data, kc.ctypes.data)
+        pa = getattr(cia, "para", None)
+        if pa is None:
+            self.cia = _CIA(cia.dnu, ct.ctypes.data, kc.ctypes.data, 0, 0.0)
+        else:
+            arrs = [_d(pa.temps), _d(pa.fp), _d(pa.kh2h2), _d(pa.kh2he), _d(pa.kcia5)]
+            self.keep += arrs
+            self.cia = _CIA(cia.dnu, ct.ctypes.data, kc.ctypes.data, len(pa.fp), pa.dnu,
+                            *[a.ctypes.data for a in arrs])
         L = req.nlayer
         hfp = req.hfp if req.hfp is not None else np.zeros(L)
         f = [_d(req.press), _d(req.temp), _d(req.delh), _d(hfp), _d(req.pp),

@@ -14,6 +14,7 @@
  *   pindex      radtran/ciatable/pindex.f:1-32
  *   ciaread     radtran/radtran/ciaread.f:104-196
  *   nciacon     radtran/radtran/nciacon.f:1-310
+ *   nparacon_all radtran/radtran/nparacon_all.f:1-338 (+ fpread_all.f:168-257)
  *   co2cia      radtran/radtran/co2cia.f
  *   rayleigh*   radtran/scatter/rayleighj.f, rayleighv.f, rayleigha.f,
  *               rayleighls.f (the iray branch, cirsradg_wave.f:663-691)
@@ -592,19 +593,16 @@ double orc_rayleigh(int iray, double v, double p, double t, double fheh2, double
   }
 }
 
-/* ---------------------------------------------------------------- nciacon */
-/* nciacon.f:1-310.  iabs[5] are 1-based gas numbers (0 = absent),
- * dabsorb[7].  The near-infrared N2-N2 and N2-H2 terms (nciacon.f:287-296)
- * vanish outside 3995-4995 cm-1, where the caller refuses N2. */
-void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
-                 int ngas, const int *idgas, const int *isogas,
-                 const double *amount, const double *pp, double xlen,
-                 double *absorb, int *iabs, double *dabsorb) {
+/* nciacon.f:65-180 (and nparacon_all.f's copy of it): the mole fractions
+ * q[5] = H2, He, N2, CH4, CO2 with their 1-based gas numbers iabs[5], the
+ * zeroed dabsorb[7] and the path factor tau = height * amagat^2. */
+static void orc_cia_setup(double p, double t, int ngas, const int *idgas, const int *isogas,
+                          const double *amount, const double *pp, double xlen, double *q,
+                          int *iabs, double *dabsorb, double *tau_out) {
   const double P0 = 1.0, T0 = 273.15;
   const double MODBOLTZA = 10.0 * KBOLTZMANN / 1.013;
   double qh2 = 0, qhe = 0, qn2 = 0, qch4 = 0, qco2 = 0;
-  double alpha[CIA_NPAIRS], dalpha[CIA_NPAIRS];
-  double xp1, height = 0.0, tau, totam = 0.0, amag1, sum, aco2;
+  double xp1, height = 0.0, tau, totam = 0.0, amag1;
   int i;
   for (i = 0; i < 5; i++) {
     iabs[i] = 0;
@@ -655,6 +653,27 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
     amag1 = (totam / height) / AMAGAT;
     tau = height * amag1 * amag1;
   }
+  q[0] = qh2;
+  q[1] = qhe;
+  q[2] = qn2;
+  q[3] = qch4;
+  q[4] = qco2;
+  *tau_out = tau;
+}
+
+/* ---------------------------------------------------------------- nciacon */
+/* nciacon.f:1-310.  iabs[5] are 1-based gas numbers (0 = absent),
+ * dabsorb[7].  The near-infrared N2-N2 and N2-H2 terms (nciacon.f:287-296)
+ * vanish outside 3995-4995 cm-1, where the caller refuses N2. */
+void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
+                 int ngas, const int *idgas, const int *isogas,
+                 const double *amount, const double *pp, double xlen,
+                 double *absorb, int *iabs, double *dabsorb) {
+  double q[5], tau, sum, aco2;
+  double alpha[CIA_NPAIRS], dalpha[CIA_NPAIRS];
+  int i;
+  orc_cia_setup(p, t, ngas, idgas, isogas, amount, pp, xlen, q, iabs, dabsorb, &tau);
+  const double qh2 = q[0], qhe = q[1], qn2 = q[2], qch4 = q[3], qco2 = q[4];
   sum = 0.0;
   orc_ciaread(cia, v0, t, alpha, dalpha);
   if (inormal == 0) {
@@ -689,6 +708,132 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
   aco2 = orc_co2cia(v0);
   sum = sum + aco2 * qco2 * qco2;
   dabsorb[4] += 2 * qco2 * aco2;
+  *absorb = sum * tau;
+  for (i = 0; i < 7; i++) dabsorb[i] *= tau;
+}
+
+/* ------------------------------------------------------------- fpread_all */
+/* fpread_all.f:168-257 on the /FPTABLE/ already in memory: bilinear (v, T)
+ * interpolation of every para fraction's H2-H2 and H2-He coefficients and of
+ * the five other pairs, and their T-derivatives.  The temperature is
+ * clamped by pindex in place as in ciaread.  Off the wavenumber grid the
+ * reference returns without writing its outputs (stale values); here they
+ * are 0. */
+static void orc_fpread_all(const orc_cia *cia, double vv, double temp, double *absh2,
+                           double *abshe, double *dah2dt, double *dahedt, double *deltaf,
+                           double *abs5, double *dabs5) {
+  const int np = cia->npara;
+  const double d2 = cia->dnu2;
+  double temp1 = temp, t, u, dudt;
+  int cv = 0, ct, i;
+  *deltaf = cia->fp[1] - cia->fp[0];
+  for (i = 0; i < np; i++) absh2[i] = abshe[i] = dah2dt[i] = dahedt[i] = 0.0;
+  for (i = 0; i < 5; i++) abs5[i] = dabs5[i] = 0.0;
+  if (vv > (CIA_NUMWN - 1) * d2 || vv < 0.0) return;
+  for (i = 1; i <= CIA_NUMWN - 1; i++) {  /* pindex on vvk2(i) = (i-1)*dnu2 */
+    if (vv >= (i - 1) * d2) cv = i;
+    else break;
+  }
+  ct = orc_pindex(cia->temps2, CIA_NUMT, &temp1);
+  t = (vv - (cv - 1) * d2) / (cv * d2 - (cv - 1) * d2);
+  u = (temp1 - cia->temps2[ct - 1]) / (cia->temps2[ct] - cia->temps2[ct - 1]);
+  dudt = 1.0 / (cia->temps2[ct] - cia->temps2[ct - 1]);
+  /* kh2h2(i,ct,cv) == kh2h2[cv-1][ct-1][i] */
+#define KP(a, c, tt, i) (a)[((size_t)((c) - 1) * CIA_NUMT + ((tt) - 1)) * np + (i)]
+  for (i = 0; i < np; i++) {
+    const double a11 = KP(cia->kh2h2, cv, ct, i), a12 = KP(cia->kh2h2, cv + 1, ct, i);
+    const double a22 = KP(cia->kh2h2, cv + 1, ct + 1, i), a21 = KP(cia->kh2h2, cv, ct + 1, i);
+    const double b11 = KP(cia->kh2he, cv, ct, i), b12 = KP(cia->kh2he, cv + 1, ct, i);
+    const double b22 = KP(cia->kh2he, cv + 1, ct + 1, i), b21 = KP(cia->kh2he, cv, ct + 1, i);
+    absh2[i] = (1.0 - t) * (1.0 - u) * a11 + t * (1.0 - u) * a12 + t * u * a22 + (1.0 - t) * u * a21;
+    abshe[i] = (1.0 - t) * (1.0 - u) * b11 + t * (1.0 - u) * b12 + t * u * b22 + (1.0 - t) * u * b21;
+    dah2dt[i] = (-(1.0 - t) * a11 - t * a12 + t * a22 + (1.0 - t) * a21) * dudt;
+    dahedt[i] = (-(1.0 - t) * b11 - t * b12 + t * b22 + (1.0 - t) * b21) * dudt;
+  }
+#undef KP
+  for (i = 0; i < 5; i++) {
+    const double *k5 = cia->kcia5;
+    const double a11 = k5[((size_t)(cv - 1) * CIA_NUMT + (ct - 1)) * 5 + i];
+    const double a12 = k5[((size_t)cv * CIA_NUMT + (ct - 1)) * 5 + i];
+    const double a22 = k5[((size_t)cv * CIA_NUMT + ct) * 5 + i];
+    const double a21 = k5[((size_t)(cv - 1) * CIA_NUMT + ct) * 5 + i];
+    abs5[i] = (1.0 - t) * (1.0 - u) * a11 + t * (1.0 - u) * a12 + t * u * a22 + (1.0 - t) * u * a21;
+    dabs5[i] = (-(1.0 - t) * a11 - t * a12 + t * a22 + (1.0 - t) * a21) * dudt;
+  }
+}
+
+/* ----------------------------------------------------------- nparacon_all */
+/* nparacon_all.f:1-338: CIA with a para-H2 fraction fpara (fpara < 0: the
+ * table's last fraction, no fpara derivative).  The fraction index follows
+ * the reference, x1 = 1 + (fpara - 0.25)/deltaf, j = INT(x1), clamped to
+ * [1, npara-2] with f = 0 below and f = 1 above, so the last interval is
+ * never interpolated into.  dabsorb[6] is d(tau)/d(fpara). */
+static void orc_nparacon_all(const orc_cia *cia, double vwave, double p, double t, int ngas,
+                             const int *idgas, const int *isogas, const double *amount,
+                             const double *pp, double fpara, double xlen, double *absorb,
+                             int *iabs, double *dabsorb) {
+  double q[5], tau, sum, aco2, deltaf, f;
+  double h2a[64], hea[64], dh2[64], dhe[64], a5[5], da5[5];
+  double ah2h2, ah2he, dh2dt, dhedt, dh2dfp, dhedfp, x1;
+  const int kpara = cia->npara;
+  int i, j;
+  orc_cia_setup(p, t, ngas, idgas, isogas, amount, pp, xlen, q, iabs, dabsorb, &tau);
+  const double qh2 = q[0], qhe = q[1], qn2 = q[2], qch4 = q[3], qco2 = q[4];
+  orc_fpread_all(cia, vwave, t, h2a, hea, dh2, dhe, &deltaf, a5, da5);
+  if (fpara < 0.0) {
+    ah2h2 = h2a[kpara - 1];
+    ah2he = hea[kpara - 1];
+    dh2dt = dh2[kpara - 1];
+    dhedt = dhe[kpara - 1];
+    dh2dfp = 0.0;
+    dhedfp = 0.0;
+  } else {
+    x1 = 1.0 + (fpara - 0.25) / deltaf;
+    j = (int)x1;
+    f = x1 - j;
+    if (j < 1) {
+      j = 1;
+      f = 0.0;
+    } else if (j > kpara - 2) {
+      j = kpara - 2;
+      f = 1.0;
+    }
+    /* 1-based j -> h2a[j-1], h2a[j] */
+    ah2h2 = (1.0 - f) * h2a[j - 1] + f * h2a[j];
+    ah2he = (1.0 - f) * hea[j - 1] + f * hea[j];
+    dh2dt = (1.0 - f) * dh2[j - 1] + f * dh2[j];
+    dhedt = (1.0 - f) * dhe[j - 1] + f * dhe[j];
+    dh2dfp = (h2a[j] - h2a[j - 1]) / deltaf;
+    dhedfp = (hea[j] - hea[j - 1]) / deltaf;
+  }
+  sum = ah2h2 * qh2 * qh2 + ah2he * qh2 * qhe;
+  dabsorb[0] = (2 * qh2 * ah2h2 + qhe * ah2he);
+  dabsorb[1] = qh2 * ah2he;
+  dabsorb[5] = (dh2dt * qh2 * qh2 + dhedt * qh2 * qhe);
+  dabsorb[6] = (dh2dfp * qh2 * qh2 + dhedfp * qh2 * qhe);
+  /* H2-N2, N2-CH4, N2-N2, CH4-CH4, H2-CH4 (nparacon_all.f's alpha(1..5)) */
+  sum = sum + (a5[0] * qh2 * qn2);
+  dabsorb[0] += qn2 * a5[0];
+  dabsorb[2] += qh2 * a5[0];
+  dabsorb[5] += qh2 * qn2 * da5[0];
+  sum = sum + (a5[1] * qn2 * qch4);
+  dabsorb[2] += qch4 * a5[1];
+  dabsorb[3] += qn2 * a5[1];
+  dabsorb[5] += qn2 * qch4 * da5[1];
+  sum = sum + (a5[2] * qn2 * qn2);
+  dabsorb[2] += 2 * qn2 * a5[2];
+  dabsorb[5] += qn2 * qn2 * da5[2];
+  sum = sum + (a5[3] * qch4 * qch4);
+  dabsorb[3] += 2 * qch4 * a5[3];
+  dabsorb[5] += qch4 * qch4 * da5[3];
+  sum = sum + (a5[4] * qh2 * qch4);
+  dabsorb[0] += qch4 * a5[4];
+  dabsorb[3] += qh2 * a5[4];
+  dabsorb[5] += qh2 * qch4 * da5[4];
+  aco2 = orc_co2cia(vwave);
+  sum = sum + aco2 * qco2 * qco2;
+  dabsorb[4] += 2 * qco2 * aco2;
+  /* the near-infrared N2 terms vanish where N2 is allowed (see nciacon) */
   *absorb = sum * tau;
   for (i = 0; i < 7; i++) dabsorb[i] *= tau;
 }
@@ -871,7 +1016,8 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
   if (r->ncont > 0 && r->nsec > 1 &&
       (vwave < r->vsec[0] || vwave > r->vsec[r->nsec - 1]))
     return ORC_ERR_LIMITS;
-  if (r->flagh2p == 1) return ORC_ERR_UNSUPPORTED;               /* nparacon_all */
+  if (r->flagh2p == 1 && (cia->npara < 3 || cia->npara > 64 || !r->hfp))
+    return ORC_ERR_UNSUPPORTED;                                   /* no /FPTABLE/ */
   /* calc_nlte_t.f:63-67 stops for k1 outside [1e-20, 1e-3] */
   if (r->inlte == 1 && (r->nlte_k1 < 1.0e-20 || r->nlte_k1 > 1.0e-3)) return ORC_ERR_LIMITS;
   /* N2's near-infrared CIA (n2n2cia.f, n2h2cia.f: tabulated, non-zero only
@@ -899,7 +1045,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
   }
   if (ng < 1 || ng > MAXG) return ORC_ERR_LIMITS;
 
-  nparam = ngas + 1 + r->ncont;  /* cirsradg_wave.f:375 */
+  nparam = ngas + 1 + r->ncont + (r->flagh2p == 1);  /* cirsradg_wave.f:375-376 */
   /* cirsradg_wave.f:393-414: the last H2, He, CH4, NH3 (isotope 0 or 1) */
   for (k = 0; k < ngas; k++) {
     if (r->isogas[k] != 0 && r->isogas[k] != 1) continue;
@@ -976,14 +1122,20 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       aam[k] = r->amount[j * ngas + k];
       ppp[k] = r->pp[j * ngas + k];
     }
-    orc_nciacon(cia, vv, r->press[j], r->temp[j], r->inormal, ngas, r->idgas, r->isogas,
-                aam, ppp, r->delh[j], &absorb, iabs, dabsorb);
+    if (r->flagh2p == 1)  /* cirsradg_wave.f:633-637 */
+      orc_nparacon_all(cia, vv, r->press[j], r->temp[j], ngas, r->idgas, r->isogas, aam, ppp,
+                       r->hfp[j], r->delh[j], &absorb, iabs, dabsorb);
+    else
+      orc_nciacon(cia, vv, r->press[j], r->temp[j], r->inormal, ngas, r->idgas, r->isogas,
+                  aam, ppp, r->delh[j], &absorb, iabs, dabsorb);
     taucon[j] += absorb;
     for (l = 0; l < 5; l++) {
       int kk = iabs[l];
       if (kk > 0) dtaucondq[j * nparam + kk - 1] += dabsorb[l] / (utotl[j] * 1.0e20);
     }
     dtaucondq[j * nparam + ngas] += dabsorb[5];
+    /* para-H2 fraction, cirsradg_wave.f:658-661 */
+    if (r->flagh2p == 1) dtaucondq[j * nparam + nparam - 1] += dabsorb[6];
 
     /* Rayleigh, cirsradg_wave.f:663-691: an extinction utotl * xray in the
      * continuum and xray * 1e-20 in every gas's d(tau)/d(amount); its P, T
@@ -1226,7 +1378,7 @@ int orc_spectrum(const orc_ktable *tabs, int ntab, const orc_cia *cia,
                  const orc_req *r, const double *vwave, int nwave, int jac_mode,
                  int nthreads, double *output, double *gtsurf, double *doutputdq) {
   int iw, rc = ORC_OK;
-  const size_t per = (size_t)r->npath * r->nlayer * (r->ngas + 1 + r->ncont);
+  const size_t per = (size_t)r->npath * r->nlayer * (r->ngas + 1 + r->ncont + (r->flagh2p == 1));
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(max : rc)

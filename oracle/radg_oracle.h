@@ -27,11 +27,16 @@ typedef struct orc_ktable {
   const double *k;     /* [npoint][np][nt][ng], k * 1e20 */
 } orc_ktable;
 
-/* The /CIATABLE/ block (ciacom.f:9-18): kcia(9,25,1501) == [1501][25][9]. */
+/* The /CIATABLE/ block (ciacom.f:9-18): kcia(9,25,1501) == [1501][25][9];
+ * with npara > 0 also /FPTABLE/ (the para-H2 tables fpread_all.f reads):
+ * kh2h2/kh2he(npara,25,1501) == [1501][25][npara], kcia5(5,25,1501). */
 typedef struct orc_cia {
   double dnu;
   const double *temps; /* [25] */
   const double *kcia;  /* [1501][25][9] */
+  int npara;
+  double dnu2;
+  const double *temps2, *fp, *kh2h2, *kh2he, *kcia5;
 } orc_cia;
 
 /* The argument surface of cirsradg_wave (cirsradg_wave.f:1-5), compact C

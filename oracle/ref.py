@@ -103,6 +103,15 @@ class RefRunner:
         temps = np.ascontiguousarray(cia.temps, dtype=rdt)
         kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
         self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        pa = getattr(cia, "para", None)
+        if pa is not None:
+            self._para = [np.array([pa.dnu], dtype=rdt), np.ascontiguousarray(pa.temps, dtype=rdt),
+                          np.array([len(pa.fp)], dtype=np.int32),
+                          np.ascontiguousarray(pa.fp, dtype=rdt),
+                          np.ascontiguousarray(pa.kh2h2, dtype=rdt),
+                          np.ascontiguousarray(pa.kh2he, dtype=rdt),
+                          np.ascontiguousarray(pa.kcia5, dtype=rdt)]
+            self.lib.hsetpara_(*[_p(a) for a in self._para])
         if req.iform:
             self.lib.hsetiform_(_p(np.array([req.iform], dtype=np.int32)))
         if req.inlte:

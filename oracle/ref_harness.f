@@ -118,6 +118,43 @@ C     /SOLARDAT/ (cirsradg_wave.f:308-309): IFORM, with no solar file
       RETURN
       END
 
+C     /FPTABLE/ (ciacom.f) in memory, so fpread_all.f reads no file:
+C     NP para fractions FPV, KH2H2/KH2HE(NP,NUMT,NUMWN), KC5(5,NUMT,NUMWN)
+      SUBROUTINE HSETPARA(DNU, TEMPS, NP, FPV, KHH, KHE, KC5)
+      IMPLICIT NONE
+      INCLUDE 'ciacom.f'
+      INTEGER NP, I, J, K
+      REAL DNU, TEMPS(NUMT), FPV(NP)
+      REAL KHH(NP,NUMT,NUMWN), KHE(NP,NUMT,NUMWN), KC5(5,NUMT,NUMWN)
+      IREAD2 = 0
+      DNU2 = DNU
+      IPARA2 = NP
+      DO I = 1, NUMT
+        TEMPK2(I) = TEMPS(I)
+      ENDDO
+      DO I = 1, NUMWN
+        VVK2(I) = (I-1)*DNU2
+      ENDDO
+      DO I = 1, NP
+        FP(I) = FPV(I)
+      ENDDO
+      DO K = 1, NUMWN
+        DO J = 1, NUMT
+          DO I = 1, NP
+            KH2H2(I,J,K) = KHH(I,J,K)
+            KH2HE(I,J,K) = KHE(I,J,K)
+          ENDDO
+          DO I = 1, 5
+            KCIA5(I,J,K) = KC5(I,J,K)
+          ENDDO
+        ENDDO
+      ENDDO
+      IT1K = 0
+      IT2K = 0
+      IVK = 0
+      RETURN
+      END
+
       SUBROUTINE HREADK(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
       IMPLICIT NONE
       CHARACTER*(*) KL

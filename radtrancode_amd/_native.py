@@ -23,7 +23,7 @@ MODE_BF16 = 2
 EXPORTS = [
     "radg_set_mode", "radg_get_mode", "radg_last_error", "radg_register_ktables",
     "radg_read_klist", "radg_register_cia", "radg_register_dust", "radg_set_nlte",
-    "radg_set_iform",
+    "radg_set_iform", "radg_register_parah2",
     "cirsradg_wave_",
     "radg_cirsradg_status",
     "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
@@ -57,7 +57,8 @@ class Request(C.Structure):
                 ("ncont", C.c_int), ("nsec", C.c_int), ("cont", C.c_void_p),
                 ("vsec", C.c_void_p), ("xsec", C.c_void_p),
                 ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double),
-                ("iform", C.c_int), ("radius1", C.c_double), ("radextra", C.c_double)]
+                ("iform", C.c_int), ("radius1", C.c_double), ("radextra", C.c_double),
+                ("hfp", C.c_void_p)]
 
 
 _lib = None
@@ -87,6 +88,7 @@ def lib():
     L.radg_register_dust.argtypes = [ip, ip, vp, vp]
     L.radg_set_nlte.argtypes = [ip, C.c_double, C.c_double]
     L.radg_set_iform.argtypes = [ip, ip]
+    L.radg_register_parah2.argtypes = [C.c_double, vp, ip, vp, vp, vp, vp]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
     L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]
     L.radg_batch_bytes.argtypes = [ip, vp, ip, C.POINTER(C.c_size_t)]

@@ -79,11 +79,28 @@ def read_klist(path: str, recl: int = 4) -> None:
 
 
 def register_cia(cia) -> None:
+    """``/CIATABLE/`` and, if ``cia.para`` is set, ``/FPTABLE/`` (else the
+    para-H2 table is cleared)."""
     temps = np.ascontiguousarray(cia.temps, dtype=np.float64)
     kcia = np.ascontiguousarray(cia.kcia, dtype=np.float64)
     if kcia.shape != (1501, 25, 9):
         raise ValueError("kcia must be [1501][25][9]")
     check(lib().radg_register_cia(C.c_double(cia.dnu), _vp(temps), _vp(kcia)))
+    register_parah2(getattr(cia, "para", None))
+
+
+def register_parah2(para) -> None:
+    """``/FPTABLE/`` from a ``synth.ParaCIATable`` (None clears it)."""
+    if para is None:
+        check(lib().radg_register_parah2(C.c_double(0.0), None, 0, None, None, None, None))
+        return
+    a = [np.ascontiguousarray(x, dtype=np.float64)
+         for x in (para.temps, para.fp, para.kh2h2, para.kh2he, para.kcia5)]
+    npara = a[1].shape[0]
+    if a[2].shape != (1501, 25, npara) or a[3].shape != a[2].shape or a[4].shape != (1501, 25, 5):
+        raise ValueError("para-H2 tables must be [1501][25][npara] and [1501][25][5]")
+    check(lib().radg_register_parah2(C.c_double(para.dnu), _vp(a[0]), int(npara), _vp(a[1]),
+                                      _vp(a[2]), _vp(a[3]), _vp(a[4])))
 
 
 def register_dust(vsec=None, xsec=None) -> None:
@@ -127,6 +144,7 @@ class _ReqArrays:
         nc = req.ncont
         dust = (d(req.cont), d(req.vsec), d(req.xsec)) if nc else ()
         self.dust = dust
+        self.hfp = d(req.hfp) if req.flagh2p == 1 else None
         self.s = Request(
             req.nlayer, req.npath, req.ngas, len(self.isw), len(self.vwave), len(self.vem),
             req.inormal, req.iray, req.ispace, req.itype, req.flagh2p, float(req.tsurf),
@@ -136,13 +154,14 @@ class _ReqArrays:
                                       self.vwave, self.vem, self.emis, self.pw)],
             nc, len(dust[1]) if nc else 0, *([a.ctypes.data for a in dust] if nc else [None] * 3),
             int(req.inlte), float(req.nlte_k1), float(req.nlte_a),
-            int(req.iform), float(req.radius1), float(req.radextra))
+            int(req.iform), float(req.radius1), float(req.radextra),
+            None if self.hfp is None else self.hfp.ctypes.data)
 
 
 class StateMap(C.Structure):
     """``radg_statemap`` (include/radg.h)."""
     _fields_ = [("nv", C.c_int), ("npro", C.c_int), ("dam", C.c_void_p), ("dte", C.c_void_p),
-                ("xmap", C.c_void_p), ("dco", C.c_void_p)]
+                ("xmap", C.c_void_p), ("dco", C.c_void_p), ("dfp", C.c_void_p)]
 
 
 def _requests(reqs):
@@ -200,7 +219,7 @@ class Batch:
         """Attach layer -> profile -> state maps (map2pro.f + map2xvec.f) to
         the requests: ``maps[r]`` is None or a dict with ``dam``, ``dte``
         [nlayer, npro], ``xmap`` [nv, nparam, npro] and, for a request with
-        dust, ``dco`` [nlayer, npro] (FP64)."""
+        dust, ``dco``, with FLAGH2P = 1, ``dfp`` [nlayer, npro] (FP64)."""
         n = len(self.reqs)
         arr = (StateMap * n)()
         keep = []
@@ -213,14 +232,17 @@ class Batch:
             if dam.shape != dte.shape or dam.shape[0] != q.nlayer or \
                     xm.shape[1:] != (q.nparam, dam.shape[1]):
                 raise ValueError(f"request {r}: statemap shapes do not match the request")
-            dco = None
-            if m.get("dco") is not None:
-                dco = np.ascontiguousarray(m["dco"], dtype=np.float64)
-                if dco.shape != dam.shape:
-                    raise ValueError(f"request {r}: dco must be [nlayer, npro]")
-            keep += [dam, dte, xm, dco]
+            extra = []
+            for key in ("dco", "dfp"):
+                e = None
+                if m.get(key) is not None:
+                    e = np.ascontiguousarray(m[key], dtype=np.float64)
+                    if e.shape != dam.shape:
+                        raise ValueError(f"request {r}: {key} must be [nlayer, npro]")
+                extra.append(e)
+            keep += [dam, dte, xm] + extra
             arr[r] = StateMap(xm.shape[0], dam.shape[1], dam.ctypes.data, dte.ctypes.data,
-                              xm.ctypes.data, None if dco is None else dco.ctypes.data)
+                              xm.ctypes.data, *[None if e is None else e.ctypes.data for e in extra])
         self._maps = [None if m is None else m["xmap"].shape[0] for m in maps]
         check(lib().radg_batch_set_statemaps(self.h, C.cast(arr, C.c_void_p), int(per_path)))
 
@@ -310,6 +332,8 @@ def fortran_arrays(req, real=np.float32):
     a["vem"][:len(req.vem)] = req.vem
     a["emissivity"][:len(req.emissivity)] = req.emissivity
     a["isw"][:len(req.isw)] = req.isw
+    if req.hfp is not None:
+        a["hfp"][:L] = req.hfp
     if req.ncont:
         a["cont"][:req.ncont, :L] = np.asarray(req.cont).T
     return a

@@ -107,8 +107,21 @@ struct CIA {
   }
 };
 
+// /FPTABLE/ (ciacom.f), the para-H2 CIA that fpread_all.f reads
+struct ParaCIA {
+  double dnu = 0;
+  int npara = 0;
+  double *d_kh2h2 = nullptr, *d_kh2he = nullptr, *d_kcia5 = nullptr, *d_temps = nullptr,
+         *d_fp = nullptr;
+  ~ParaCIA() {
+    for (double *p : {d_kh2h2, d_kh2he, d_kcia5, d_temps, d_fp})
+      if (p) (void)hipFree(p);
+  }
+};
+
 std::shared_ptr<TableSet> g_tables;
 std::shared_ptr<CIA> g_cia;
+std::shared_ptr<ParaCIA> g_para;
 
 // /DUST/ for the fixed-argument entry points (cirsradg_wave.f:225,312)
 struct Dust {
@@ -189,13 +202,13 @@ struct Packed {
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0; }
 
-int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P) {
+int pack_request(const radg_request &q, const TableSet &ts, int mode, const ParaCIA *para,
+                 Packed &P) {
   const int L = q.nlayer, ngas = q.ngas, npath = q.npath, nwave = q.nwave;
   if (L < 1 || L > RADG_MAXLAY) return fail(RADG_ELIMITS, "nlayer=%d outside 1..%d", L, RADG_MAXLAY);
   if (npath < 1 || npath > RADG_MAXPAT) return fail(RADG_ELIMITS, "npath=%d outside 1..%d", npath, RADG_MAXPAT);
   if (ngas < 1 || ngas > RADG_MAXGAS) return fail(RADG_ELIMITS, "ngas=%d outside 1..%d", ngas, RADG_MAXGAS);
   if (nwave < 1) return fail(RADG_ELIMITS, "nwave=%d", nwave);
-  if (q.flagh2p == 1) return fail(RADG_EUNSUPPORTED, "flagh2p=1 (para-H2 CIA, nparacon_all) is not provided");
   if (q.ispace != 0 && q.ispace != 1) return fail(RADG_EINPUT, "ispace=%d", q.ispace);
   // dust, check_limits.f:24-33 and cirsradg_wave.f:439-453
   const int ncont = q.ncont;
@@ -215,7 +228,15 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   if (q.inlte == 1 && !(q.nlte_k1 >= 1.0e-20 && q.nlte_k1 <= 1.0e-3))
     return fail(RADG_EINPUT, "inlte=1 needs 1e-20 <= k1 <= 1e-3 (k1=%g)", q.nlte_k1);
   if (q.inlte == 1 && !std::isfinite(q.nlte_a)) return fail(RADG_EINPUT, "nlte A=%g", q.nlte_a);
-  const int nparam = ngas + 1 + ncont;  // cirsradg_wave.f:375
+  // para-H2 (cirsradg_wave.f:376,633-637): the fraction per layer is hfp
+  const int h2p = q.flagh2p == 1 ? 1 : 0;
+  if (h2p) {
+    if (!para) return fail(RADG_ENOTABLES, "FLAGH2P=1 needs a para-H2 table (radg_register_parah2)");
+    if (!q.hfp) return fail(RADG_EINPUT, "FLAGH2P=1 without hfp");
+    for (int j = 0; j < L; j++)
+      if (!std::isfinite(q.hfp[j])) return fail(RADG_EINPUT, "hfp(%d)=%g", j + 1, q.hfp[j]);
+  }
+  const int nparam = ngas + 1 + ncont + h2p;  // cirsradg_wave.f:375-376
   if (q.nsw < 0 || q.nsw > nparam) return fail(RADG_ELIMITS, "nsw=%d > nparam=%d", q.nsw, nparam);
   for (int k = 0; k < q.nsw; k++)
     if (q.isw[k] < 1 || q.isw[k] > nparam) return fail(RADG_EINPUT, "isw(%d)=%d outside 1..%d", k + 1, q.isw[k], nparam);
@@ -252,6 +273,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
   d.npath = npath;
   d.nparam = nparam;
   d.ncont = ncont;
+  d.flagh2p = h2p;
   d.inlte = q.inlte;
   d.nlte_k1 = q.nlte_k1;
   d.nlte_a = q.nlte_a;
@@ -318,6 +340,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, Packed &P)
     c.utotl = ut * 1.e-20;
     c.temp = q.temp[j];
     c.press = q.press[j];
+    c.fpara = h2p ? q.hfp[j] : 0.0;
     {
       const double *ppj = q.pp + (size_t)j * ngas;
       const double fh2 = jh2 >= 0 ? ppj[jh2] / q.press[j] : 0.0;
@@ -504,6 +527,7 @@ struct radg_batch {
   int mode = 0, nreq = 0;
   std::shared_ptr<TableSet> ts;
   std::shared_ptr<CIA> cia;
+  std::shared_ptr<ParaCIA> para;
   std::vector<DevReq> reqs;
   char *dmem = nullptr;
   size_t bytes = 0;
@@ -536,7 +560,7 @@ struct radg_batch {
   char *hst = nullptr;     // pinned: [weighted span][per-path span]
   // host copy of each request's path layers (0-based layinc, zero past nlays)
   struct HostReq {
-    int L = 0, npath = 0, ngas = 0, nwave = 0, ncont = 0;
+    int L = 0, npath = 0, ngas = 0, nwave = 0, ncont = 0, h2p = 0;
     std::vector<int> layinc, nlays;
   };
   std::vector<HostReq> hreq;
@@ -683,12 +707,12 @@ struct Layout {
 struct Plan {
   std::vector<Packed> packs;
   size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
-      o_vw, o_es, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
-      o_err;
+      o_vw, o_es, o_wg, o_pw, o_dx, o_ct, o_dp, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts,
+      o_wjac, o_err;
   size_t total = 0;
   long long nA = 0, nB = 0, nC = 0;
   long long nlay = 0, nlg = 0, npath = 0, npl = 0, nwv = 0, nwg = 0, ninter = 0, nout = 0, njac = 0,
-            nwout = 0, nwjac = 0, ndx = 0, nct = 0;
+            nwout = 0, nwjac = 0, ndx = 0, nct = 0, ndp = 0;
   std::vector<long long> sA, sB, sC;
   int maxngas = 1, maxnsw = 1, maxlays = 1;
   std::vector<int> cls_rl;
@@ -696,14 +720,15 @@ struct Plan {
   size_t o_clsmap = 0, o_clsstart = 0;
 };
 
-int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts, Plan &pl) {
+int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
+               const ParaCIA *para, Plan &pl) {
   if (nreq < 1) return fail(RADG_EINPUT, "nreq=%d", nreq);
   pl.packs.resize(nreq);
   pl.sA.assign(nreq + 1, 0);
   pl.sB.assign(nreq + 1, 0);
   pl.sC.assign(nreq + 1, 0);
   for (int r = 0; r < nreq; r++) {
-    int rc = pack_request(reqs[r], ts, mode, pl.packs[r]);
+    int rc = pack_request(reqs[r], ts, mode, para, pl.packs[r]);
     if (rc) return rc;
     DevReq &d = pl.packs[r].d;
     const radg_request &q = reqs[r];
@@ -723,6 +748,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     d.wjac0 = pl.nwjac;
     d.dx0 = pl.ndx;
     d.ct0 = pl.nct;
+    d.dp0 = pl.ndp;
+    if (d.flagh2p) pl.ndp += (long long)q.nwave * q.nlayer;
     pl.ndx += (long long)q.nwave * q.ncont;
     pl.nct += (long long)q.nlayer * q.ncont;
     pl.nlay += q.nlayer;
@@ -778,6 +805,7 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_pw = lo.take<double>(pl.npath);
   pl.o_dx = lo.take<double>(pl.ndx);
   pl.o_ct = lo.take<double>(pl.nct);
+  pl.o_dp = lo.take<double>(pl.ndp);
   pl.o_clsmap = lo.take<int>(nreq);
   pl.o_clsstart = lo.take<long long>(nreq + pl.cls_rl.size());
   pl.o_inter = lo.take<char>((size_t)pl.ninter * ts_);
@@ -932,6 +960,38 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia) {
   return RADG_OK;
 }
 
+int radg_register_parah2(double dnu, const double *temps, int npara, const double *fp,
+                         const double *kh2h2, const double *kh2he, const double *kcia5) {
+  if (npara == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_para.reset();
+    return RADG_OK;
+  }
+  if (npara < 3 || npara > 64) return fail(RADG_ELIMITS, "npara=%d outside 3..64", npara);
+  if (!(dnu > 0) || !temps || !fp || !kh2h2 || !kh2he || !kcia5)
+    return fail(RADG_EINPUT, "para-H2 table: dnu=%g or a null array", dnu);
+  for (int i = 1; i < 25; i++)
+    if (!(temps[i] > temps[i - 1])) return fail(RADG_EINPUT, "para-H2 temperatures must increase");
+  if (!(fp[1] > fp[0])) return fail(RADG_EINPUT, "para-H2 fractions must increase");
+  auto c = std::make_shared<ParaCIA>();
+  c->dnu = dnu;
+  c->npara = npara;
+  const size_t nk = (size_t)1501 * 25 * npara, n5 = (size_t)1501 * 25 * 5;
+  HIPCHK(hipMalloc(&c->d_kh2h2, nk * sizeof(double)));
+  HIPCHK(hipMalloc(&c->d_kh2he, nk * sizeof(double)));
+  HIPCHK(hipMalloc(&c->d_kcia5, n5 * sizeof(double)));
+  HIPCHK(hipMalloc(&c->d_temps, 25 * sizeof(double)));
+  HIPCHK(hipMalloc(&c->d_fp, npara * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_kh2h2, kh2h2, nk * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_kh2he, kh2he, nk * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_kcia5, kcia5, n5 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_temps, temps, 25 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_fp, fp, npara * sizeof(double), hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_para = c;
+  return RADG_OK;
+}
+
 int radg_set_iform(int iform, int iread) {
   if (iread == 999 && (iform == 1 || iform == 4))
     return fail(RADG_EUNSUPPORTED, "iform=%d with a solar spectrum (iread=999) is not provided", iform);
@@ -972,13 +1032,15 @@ int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xs
 
 int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes) {
   std::shared_ptr<TableSet> ts;
+  std::shared_ptr<ParaCIA> para;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
+    para = g_para;
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
   Plan pl;
-  int rc = plan_batch(nreq, reqs, mode, *ts, pl);
+  int rc = plan_batch(nreq, reqs, mode, *ts, para.get(), pl);
   if (rc) return rc;
   *bytes = pl.total;
   return RADG_OK;
@@ -989,15 +1051,17 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   std::shared_ptr<TableSet> ts;
   std::shared_ptr<CIA> cia;
+  std::shared_ptr<ParaCIA> para;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
     cia = g_cia;
+    para = g_para;
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
   Plan pl;
-  int rc = plan_batch(nreq, reqs, mode, *ts, pl);
+  int rc = plan_batch(nreq, reqs, mode, *ts, para.get(), pl);
   if (rc) return rc;
   rc = ensure_form(*ts, mode);
   if (rc) return rc;
@@ -1008,6 +1072,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   b->nreq = nreq;
   b->ts = ts;
   b->cia = cia;
+  b->para = para;
   b->bytes = pl.total;
   b->dmem = static_cast<char *>(dev_cache().get(pl.total));
   if (!b->dmem) return fail(RADG_EHIP, "device allocation of %zu bytes failed", pl.total);
@@ -1080,6 +1145,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     hq.ngas = reqs[r].ngas;
     hq.nwave = reqs[r].nwave;
     hq.ncont = reqs[r].ncont;
+    hq.h2p = P.d.flagh2p;
     hq.layinc = P.layinc;
     hq.nlays.resize(hq.npath);
     for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
@@ -1121,6 +1187,20 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
   a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
   a.cont = reinterpret_cast<const double *>(m + pl.o_ct);
+  a.dpara = reinterpret_cast<double *>(m + pl.o_dp);
+  if (para) {
+    a.pkh2h2 = para->d_kh2h2;
+    a.pkh2he = para->d_kh2he;
+    a.pkcia5 = para->d_kcia5;
+    a.ptemps = para->d_temps;
+    a.pfp = para->d_fp;
+    a.pdnu = para->dnu;
+    a.npara = para->npara;
+  } else {
+    a.pkh2h2 = a.pkh2he = a.pkcia5 = a.ptemps = a.pfp = nullptr;
+    a.pdnu = 0.0;
+    a.npara = 0;
+  }
   a.inter = m + pl.o_inter;
   a.out = m + pl.o_out;
   a.gts = m + pl.o_gts;
@@ -1397,17 +1477,19 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
   std::vector<radg_batch::SMap> sm(nreq);
   // ---- sizes and path classes
   Layout lo;
-  std::vector<size_t> o_dam(nreq), o_dte(nreq), o_dco(nreq), o_xm(nreq), o_rm(nreq);
+  std::vector<size_t> o_dam(nreq), o_dte(nreq), o_dco(nreq), o_dfp(nreq), o_xm(nreq), o_rm(nreq);
   int n_pc = 0, n_pk64 = 0, n_pkT = 0, n_ps = 0;
   for (int r = 0; r < nreq; r++) {
     const radg_statemap &mp = maps[r];
     if (mp.nv == 0) continue;
     const radg_batch::HostReq &hq = b->hreq[r];
-    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont;
+    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont + hq.h2p;
     if (mp.nv < 0 || mp.npro < 1 || !mp.dam || !mp.dte || !mp.xmap)
       return fail(RADG_EINPUT, "request %d: statemap needs nv >= 0, npro >= 1 and dam, dte, xmap", r);
     if (hq.ncont > 0 && !mp.dco)
       return fail(RADG_EINPUT, "request %d: statemap of a request with dust needs dco", r);
+    if (hq.h2p && !mp.dfp)
+      return fail(RADG_EINPUT, "request %d: statemap of a FLAGH2P=1 request needs dfp", r);
     radg_batch::SMap &S = sm[r];
     S.nv = mp.nv;
     S.npro = mp.npro;
@@ -1433,6 +1515,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     o_dam[r] = lo.take<double>((size_t)L * mp.npro);
     o_dte[r] = lo.take<double>((size_t)L * mp.npro);
     o_dco[r] = lo.take<double>(hq.ncont > 0 ? (size_t)L * mp.npro : 0);
+    o_dfp[r] = lo.take<double>(hq.h2p ? (size_t)L * mp.npro : 0);
     o_xm[r] = lo.take<double>((size_t)mp.nv * npar * mp.npro);
     o_rm[r] = lo.take<int>((size_t)S.ncls * L);
     n_pc += S.ncls * npar;
@@ -1450,7 +1533,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     radg_batch::SMap &S = sm[r];
     if (S.nv == 0) continue;
     const radg_batch::HostReq &hq = b->hreq[r];
-    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont;
+    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont + hq.h2p;
     S.o_cm = lo.take<double>((size_t)S.ncls * L * npar * S.nv);
     S.o_kk = lo.take<double>((size_t)hq.nwave * S.nv);
     if (S.per_path) S.o_dx = lo.take<double>((size_t)hq.nwave * hq.npath * S.nv);
@@ -1508,10 +1591,11 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     const radg_statemap &mp = maps[r];
     const radg_batch::HostReq &hq = b->hreq[r];
     const DevReq &d = b->reqs[r];
-    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont, nv = S.nv, npro = S.npro;
+    const int L = hq.L, npar = hq.ngas + 1 + hq.ncont + hq.h2p, nv = S.nv, npro = S.npro;
     std::memcpy(hs + o_dam[r], mp.dam, (size_t)L * npro * 8);
     std::memcpy(hs + o_dte[r], mp.dte, (size_t)L * npro * 8);
     if (hq.ncont > 0) std::memcpy(hs + o_dco[r], mp.dco, (size_t)L * npro * 8);
+    if (hq.h2p) std::memcpy(hs + o_dfp[r], mp.dfp, (size_t)L * npro * 8);
     std::memcpy(hs + o_xm[r], mp.xmap, (size_t)nv * npar * npro * 8);
     int *rm = reinterpret_cast<int *>(hs + o_rm[r]);
     std::vector<int> rep(S.ncls, -1);
@@ -1523,6 +1607,7 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     const double *ddam = reinterpret_cast<const double *>(dm + o_dam[r]);
     const double *ddte = reinterpret_cast<const double *>(dm + o_dte[r]);
     const double *ddco = reinterpret_cast<const double *>(dm + o_dco[r]);
+    const double *ddfp = reinterpret_cast<const double *>(dm + o_dfp[r]);
     const double *dxm = reinterpret_cast<const double *>(dm + o_xm[r]);
     const int *drm = reinterpret_cast<const int *>(dm + o_rm[r]);
     double *dcm = reinterpret_cast<double *>(dm + S.o_cm);
@@ -1530,8 +1615,9 @@ int radg_batch_set_statemaps(radg_batch *b, const radg_statemap *maps, int per_p
     const long long csz = (long long)L * npar * nv;
     for (int c = 0; c < S.ncls; c++)
       for (int k = 0; k < npar; k++) {
-        // map2pro.f:62-76: DAM for the gases, DTE for T, DCO for every dust type
-        const double *D = k < hq.ngas ? ddam : k == hq.ngas ? ddte : ddco;
+        // map2pro.f:62-80: DAM for the gases, DTE for T, DCO for every dust
+        // type, DFP for the para-H2 fraction
+        const double *D = k < hq.ngas ? ddam : k == hq.ngas ? ddte : k - hq.ngas - 1 < hq.ncont ? ddco : ddfp;
         pc.push_back(gp(D, drm + (size_t)c * L, dxm + (size_t)k * npro,
                         dcm + c * csz + (long long)k * nv, npro, 1, 1, (long long)npar * npro,
                         (long long)npar * nv, 1, L, nv, npro));
@@ -1632,7 +1718,7 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
               int flagh2p, const int *nlayin, const int *layinc, const R *scale, const int *imod,
               const int *idgas, const int *isogas, const R *emtemp, int nem, const R *vem,
               const R *emis, R tsurf, int nsw, const int *isw, int nwave, const R *vwave,
-              const R *cont, R radius1, R radextra, std::vector<double> &out,
+              const R *cont, R radius1, R radextra, const R *hfp, std::vector<double> &out,
               std::vector<double> &gts, std::vector<double> &jac) {
   if (L < 1 || L > RADG_MAXLAY || npath < 1 || npath > RADG_MAXPAT || ngas < 1 || ngas > RADG_MAXGAS)
     return fail(RADG_ELIMITS, "nlayer=%d npath=%d ngas=%d exceed the array limits", L, npath, ngas);
@@ -1709,6 +1795,11 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   q.iform = iform;
   q.radius1 = radius1;
   q.radextra = radextra;
+  std::vector<double> hf;
+  if (flagh2p == 1) {
+    hf.assign(hfp, hfp + L);
+    q.hfp = hf.data();
+  }
   q.ncont = ncont;
   if (ncont > 0) {
     q.nsec = dust->nsec;
@@ -1724,7 +1815,8 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   if ((rc = radg_batch_sync(b))) return rc;
   out.assign((size_t)nwave * npath, 0.0);
   gts.assign((size_t)nwave * npath, 0.0);
-  jac.assign((size_t)nwave * npath * L * (ngas + 1 + ncont), 0.0);
+  // every column radg_batch_read writes: gases, T, dust, the para-H2 fraction
+  jac.assign((size_t)nwave * npath * L * b->reqs[0].nparam, 0.0);
   return radg_batch_read(b, 0, out.data(), gts.data(), jac.data(), nullptr, nullptr, nullptr);
 }
 
@@ -1736,14 +1828,14 @@ int spectrum_fixed(const R *dist, const int *inormal, const int *iray, const R *
                    const int *imod, const int *idgas, const int *isogas, const R *emtemp,
                    const int *itype, const int *nem, const R *vem, const R *emissivity,
                    const R *tsurf, const R *radius1, const R *radextra, const int *flagh2p,
-                   const int *nsw, const int *isw, int nwave, const R *vwave, R *output,
-                   R *gtsurf, R *doutputdq) {
+                   const R *hfp, const int *nsw, const int *isw, int nwave, const R *vwave,
+                   R *output, R *gtsurf, R *doutputdq) {
   (void)dist;
   std::vector<double> o, g, j;
   int rc = run_fixed<R>(delh, *nlayer, *npath, *ngas, press, temp, pp, amount, *ispace, *inormal,
                         *iray, *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas,
                         emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, cont,
-                        *radius1, *radextra, o, g, j);
+                        *radius1, *radextra, hfp, o, g, j);
   if (rc) return rc;
   for (size_t i = 0; i < o.size(); i++) {
     output[i] = (R)o[i];
@@ -1767,13 +1859,13 @@ void cirsradg_wave_(const float *dist, const int *inormal, const int *iray, cons
                     float *gtsurf, const float *radius1, const int *flagh2p, const float *hfp,
                     const float *radextra, const int *nsw, const int *isw, float *output,
                     float *doutputdq) {
-  (void)dist; (void)iwave; (void)amform; (void)hfp;
+  (void)dist; (void)iwave; (void)amform;
   std::vector<double> o, g, j;
   const int L = *nlayer, np_ = *npath, ng = *ngas;
   int rc = run_fixed<float>(delh, L, np_, ng, press, temp, pp, amount, *ispace, *inormal, *iray,
                             *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
                             *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, cont, *radius1,
-                            *radextra, o, g, j);
+                            *radextra, hfp, o, g, j);
   g_cirs_status = rc;
   if (rc) return;
   const int nparam = (int)(j.size() / ((size_t)np_ * L));  // ngas + 1 + ncont
@@ -1797,12 +1889,13 @@ int radg_spectrum_r4(const float *dist, const int *inormal, const int *iray, con
                      const int *imod, const int *idgas, const int *isogas, const float *emtemp,
                      const int *itype, const int *nem, const float *vem, const float *emissivity,
                      const float *tsurf, const float *radius1, const float *radextra,
-                     const int *flagh2p, const int *nsw, const int *isw, int nwave,
-                     const float *vwave, float *output, float *gtsurf, float *doutputdq) {
+                     const int *flagh2p, const float *hfp, const int *nsw, const int *isw,
+                     int nwave, const float *vwave, float *output, float *gtsurf,
+                     float *doutputdq) {
   return spectrum_fixed<float>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
                                amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas,
                                emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra,
-                               flagh2p, nsw, isw, nwave, vwave, output, gtsurf, doutputdq);
+                               flagh2p, hfp, nsw, isw, nwave, vwave, output, gtsurf, doutputdq);
 }
 
 int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, const double *delh,
@@ -1812,13 +1905,13 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, co
                      const double *scale, const int *imod, const int *idgas, const int *isogas,
                      const double *emtemp, const int *itype, const int *nem, const double *vem,
                      const double *emissivity, const double *tsurf, const double *radius1,
-                     const double *radextra, const int *flagh2p, const int *nsw, const int *isw,
-                     int nwave, const double *vwave, double *output, double *gtsurf,
-                     double *doutputdq) {
+                     const double *radextra, const int *flagh2p, const double *hfp,
+                     const int *nsw, const int *isw, int nwave, const double *vwave,
+                     double *output, double *gtsurf, double *doutputdq) {
   return spectrum_fixed<double>(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp,
                                 amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas,
                                 emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra,
-                                flagh2p, nsw, isw, nwave, vwave, output, gtsurf, doutputdq);
+                                flagh2p, hfp, nsw, isw, nwave, vwave, output, gtsurf, doutputdq);
 }
 
 // .kta reader (read_khead.f:18-120; regular grids, NT > 0)

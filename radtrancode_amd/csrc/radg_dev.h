@@ -18,6 +18,8 @@ struct DevReq {
   int iray;                    // Rayleigh form 1..4, 0 = off (iray > 0 and itype != 0)
   int ncont;                   // dust types; nparam = ngas + 1 + ncont
   int inlte;                   // calc_nlte_t.f flag (0 = LTE)
+  int flagh2p, pad2;           // 1: para-H2 CIA (nparacon_all.f), fpara the last parameter
+  long long dp0;               // per-(wavenumber, layer) d(tau)/d(fpara)
   long long dx0;               // per-(wavenumber, dust type) extinction
   long long ct0;               // per-(layer, dust type) particle columns
   double tsurf;
@@ -49,6 +51,7 @@ struct LayerConst {
   double temp;           // K
   double qh2, qhe, qn2, qch4, qco2, tau;
   double press;          // atm
+  double fpara;          // para-H2 fraction (hfp), FLAGH2P = 1
   double fheh2, fch4h2, fnh3;  // rayleighls mixture, cirsradg_wave.f:520-523,671-677
 };
 
@@ -110,6 +113,11 @@ struct KernelArgs {
   const double *pweight;       // [path]
   const double *dext;          // [wavenumber][ncont] dust extinction (cm2) at vwave
   const double *cont;          // [layer][ncont] dust particles/cm2
+  // /FPTABLE/ (FLAGH2P = 1): kh2h2/kh2he [1501][25][npara], kcia5 [1501][25][5]
+  const double *pkh2h2, *pkh2he, *pkcia5, *ptemps, *pfp;
+  double pdnu;
+  int npara;
+  double *dpara;               // [request, wavenumber, layer] d(tau)/d(fpara)
   // outputs
   void *inter;                 // intermediates, compute type
   void *out, *gts, *jac;       // compute type; jac gas columns per 1e20 molecules

@@ -519,6 +519,124 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
   for (int i = 0; i < 7; i++) dab[i] *= lc.tau;
 }
 
+// CIA with a para-H2 fraction per layer (FLAGH2P = 1): nparacon_all.f with
+// fpread_all.f:168-257 on the registered /FPTABLE/.  Only the two para
+// fractions the layer's fraction falls between are interpolated in (v, T)
+// (each fraction's value is independent of the others).  The fraction index
+// follows the reference: x1 = 1 + (fpara - 0.25)/deltaf, j = INT(x1) kept in
+// [1, npara-2] with f = 0 below and f = 1 above; fpara < 0 takes the last
+// fraction and has no derivative.  dab[6] = d(tau)/d(fpara).
+__device__ __forceinline__ void cia_para_layer(const KernelArgs &a, double vv, const LayerConst &lc,
+                                               double *absorb, double *dab) {
+  const int np = a.npara;
+  const double deltaf = a.pfp[1] - a.pfp[0];
+  int j0, j1;
+  double f = 0.0, has_dfp = 0.0;
+  if (lc.fpara < 0.0) {
+    j0 = j1 = np - 1;
+  } else {
+    const double x1 = 1.0 + (lc.fpara - 0.25) / deltaf;
+    int j = (int)x1;
+    f = x1 - j;
+    if (j < 1) {
+      j = 1;
+      f = 0.0;
+    } else if (j > np - 2) {
+      j = np - 2;
+      f = 1.0;
+    }
+    j0 = j - 1;
+    j1 = j;
+    has_dfp = 1.0;
+  }
+  double h2[2] = {0.0, 0.0}, he[2] = {0.0, 0.0}, dh2[2] = {0.0, 0.0}, dhe[2] = {0.0, 0.0};
+  double a5[5], da5[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) a5[i] = da5[i] = 0.0;
+  if (!(vv > 1500.0 * a.pdnu || vv < 0.0)) {
+    const double *tt = a.ptemps;
+    double temp1 = lc.temp;
+    if (temp1 > tt[24]) temp1 = tt[24];
+    if (temp1 < tt[0]) temp1 = tt[0];
+    int cv = (int)floor(vv / a.pdnu) + 1;
+    if (cv > 1500) cv = 1500;
+    if (cv < 1) cv = 1;
+    while (cv > 1 && vv < (cv - 1) * a.pdnu) cv--;
+    while (cv < 1500 && vv >= cv * a.pdnu) cv++;
+    int ct = 1;
+    for (int i = 1; i <= 24; i++) {
+      if (temp1 >= tt[i - 1]) ct = i;
+      else break;
+    }
+    const double t = (vv - (cv - 1) * a.pdnu) / (cv * a.pdnu - (cv - 1) * a.pdnu);
+    const double u = (temp1 - tt[ct - 1]) / (tt[ct] - tt[ct - 1]);
+    const double dudt = 1.0 / (tt[ct] - tt[ct - 1]);
+    const long long r11 = ((long long)(cv - 1) * 25 + (ct - 1)), r12 = ((long long)cv * 25 + (ct - 1));
+    const long long r22 = ((long long)cv * 25 + ct), r21 = ((long long)(cv - 1) * 25 + ct);
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int i = s ? j1 : j0;
+      const double a11 = a.pkh2h2[r11 * np + i], a12 = a.pkh2h2[r12 * np + i];
+      const double a22 = a.pkh2h2[r22 * np + i], a21 = a.pkh2h2[r21 * np + i];
+      const double b11 = a.pkh2he[r11 * np + i], b12 = a.pkh2he[r12 * np + i];
+      const double b22 = a.pkh2he[r22 * np + i], b21 = a.pkh2he[r21 * np + i];
+      h2[s] = (1.0 - t) * (1.0 - u) * a11 + t * (1.0 - u) * a12 + t * u * a22 + (1.0 - t) * u * a21;
+      he[s] = (1.0 - t) * (1.0 - u) * b11 + t * (1.0 - u) * b12 + t * u * b22 + (1.0 - t) * u * b21;
+      dh2[s] = (-(1.0 - t) * a11 - t * a12 + t * a22 + (1.0 - t) * a21) * dudt;
+      dhe[s] = (-(1.0 - t) * b11 - t * b12 + t * b22 + (1.0 - t) * b21) * dudt;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      const double a11 = a.pkcia5[r11 * 5 + i], a12 = a.pkcia5[r12 * 5 + i];
+      const double a22 = a.pkcia5[r22 * 5 + i], a21 = a.pkcia5[r21 * 5 + i];
+      a5[i] = (1.0 - t) * (1.0 - u) * a11 + t * (1.0 - u) * a12 + t * u * a22 + (1.0 - t) * u * a21;
+      da5[i] = (-(1.0 - t) * a11 - t * a12 + t * a22 + (1.0 - t) * a21) * dudt;
+    }
+  }
+  const double ah2h2 = (1.0 - f) * h2[0] + f * h2[1], ah2he = (1.0 - f) * he[0] + f * he[1];
+  const double dh2dt = (1.0 - f) * dh2[0] + f * dh2[1], dhedt = (1.0 - f) * dhe[0] + f * dhe[1];
+  const double dh2dfp = has_dfp * (h2[1] - h2[0]) / deltaf, dhedfp = has_dfp * (he[1] - he[0]) / deltaf;
+  const double qh2 = lc.qh2, qhe = lc.qhe, qn2 = lc.qn2, qch4 = lc.qch4, qco2 = lc.qco2;
+  double sum = ah2h2 * qh2 * qh2 + ah2he * qh2 * qhe;
+  dab[0] = 2 * qh2 * ah2h2 + qhe * ah2he;
+  dab[1] = qh2 * ah2he;
+  dab[2] = dab[3] = dab[4] = 0.0;
+  dab[5] = dh2dt * qh2 * qh2 + dhedt * qh2 * qhe;
+  dab[6] = dh2dfp * qh2 * qh2 + dhedfp * qh2 * qhe;
+  // H2-N2, N2-CH4, N2-N2, CH4-CH4, H2-CH4
+  sum += a5[0] * qh2 * qn2;
+  dab[0] += qn2 * a5[0];
+  dab[2] += qh2 * a5[0];
+  dab[5] += qh2 * qn2 * da5[0];
+  sum += a5[1] * qn2 * qch4;
+  dab[2] += qch4 * a5[1];
+  dab[3] += qn2 * a5[1];
+  dab[5] += qn2 * qch4 * da5[1];
+  sum += a5[2] * qn2 * qn2;
+  dab[2] += 2 * qn2 * a5[2];
+  dab[5] += qn2 * qn2 * da5[2];
+  sum += a5[3] * qch4 * qch4;
+  dab[3] += 2 * qch4 * a5[3];
+  dab[5] += qch4 * qch4 * da5[3];
+  sum += a5[4] * qh2 * qch4;
+  dab[0] += qch4 * a5[4];
+  dab[3] += qh2 * a5[4];
+  dab[5] += qh2 * qch4 * da5[4];
+  {  // co2cia.f
+    const double xl = 1e4 / vv;
+    double aco2 = 0.0;
+    if (xl >= 2.15 && xl <= 2.55) aco2 = 4e-8;
+    if (xl >= 1.7 && xl <= 1.76) aco2 = 6e-9;
+    if (xl >= 1.25 && xl < 1.35) aco2 = 1.5e-9;
+    if (xl >= 1.125 && xl <= 1.225) aco2 = 0.5 * (0.31 + 0.79) * 1e-9;
+    if (xl >= 1.06 && xl < 1.125) aco2 = 0.5 * (0.29 + 0.67) * 1e-9;
+    sum += aco2 * qco2 * qco2;
+    dab[4] += 2 * qco2 * aco2;
+  }
+  *absorb = sum * lc.tau;
+  for (int i = 0; i < 7; i++) dab[i] *= lc.tau;
+}
+
 // Rayleigh cross-section per molecule (cm2) at wavenumber vv (cm-1), in FP64
 // for every form; pi is the reference's literal 3.1415927.
 //   1: rayleighj.f (H2/He mixture, fH2 = 0.864, isotropic)
@@ -1098,8 +1216,12 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   double tdust = 0.0;
   for (int k = 0; k < R.ncont; k++)
     tdust += a.dext[R.dx0 + (long long)iw * R.ncont + k] * a.cont[R.ct0 + (long long)j * R.ncont + k];
-  cia_layer(a, vv, R.inormal, lc, &absorb, dab);
+  if (R.flagh2p)
+    cia_para_layer(a, vv, lc, &absorb, dab);
+  else
+    cia_layer(a, vv, R.inormal, lc, &absorb, dab);
   absorb = tdust + absorb;
+  if (R.flagh2p) a.dpara[R.dp0 + (long long)iw * L + j] = dab[6];
   // Rayleigh (cirsradg_wave.f:663-691): utotl * xray more extinction and
   // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.  The gas
   // rows of dtcon, like dk/dq, are per 1e20 molecules (the 1e-20 is applied
@@ -1399,9 +1521,13 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
         const int k = R.isw[k1];
         // gases and T: continuum + overlap derivatives; dust type k-ngas-1:
         // d(tau)/d(particles/cm2) is its extinction, the same in every layer
+        // the para-H2 fraction (FLAGH2P = 1, the last parameter): its per-layer
+        // d(tau)/d(fpara) from the continuum kernel
         const T dq = (k <= ngas ? dtcon[(long long)k * L + Lm[q]] +
                                       dkq[((long long)g * (ngas + 1) + k) * L + Lm[q]]
-                                : (T)a.dext[R.dx0 + (long long)iw * R.ncont + (k - ngas - 1)]) *
+                      : k - ngas - 1 < R.ncont
+                          ? (T)a.dext[R.dx0 + (long long)iw * R.ncont + (k - ngas - 1)]
+                          : (T)a.dpara[R.dp0 + (long long)iw * L + Lm[q]]) *
                      sc[q];
         T c = dRdtau[q] * dq;
         if (k == ngas) c = c + dT[q];

@@ -39,7 +39,7 @@ using clk = std::chrono::steady_clock;
 struct OwnedRequest {
   radg_request q{};
   std::vector<double> press, temp, delh, pp, amount, scale, emtemp, vwave, vem, emis, pw;
-  std::vector<double> cont, vsec, xsec;
+  std::vector<double> cont, vsec, xsec, hfp;
   std::vector<int> idgas, isogas, nlayin, imod, layinc, isw;
 
   explicit OwnedRequest(const radg_request &s) {
@@ -92,6 +92,12 @@ struct OwnedRequest {
       q.xsec = xsec.data();
     } else {
       q.cont = q.vsec = q.xsec = nullptr;
+    }
+    if (s.flagh2p == 1 && s.hfp && s.nlayer > 0 && s.nlayer <= RADG_MAXLAY) {
+      hfp.assign(s.hfp, s.hfp + L);
+      q.hfp = hfp.data();
+    } else {
+      q.hfp = nullptr;
     }
   }
 };

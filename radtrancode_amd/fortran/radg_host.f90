@@ -16,7 +16,9 @@ program radg_host
   character(len=512) :: fin, fout, klist
   integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
   integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont, inlte, iform
-  real(c_double) :: nlte_k1, nlte_a
+  real(c_double) :: nlte_k1, nlte_a, pdnu, ptemps(25)
+  integer(c_int) :: npara
+  real(c_double), allocatable :: pfp(:), pkhh(:), pkhe(:), pk5(:)
   real(c_double), allocatable :: vsec(:), xsec(:)
   real(c_double) :: dnu, ctemps(25)
   real(c_double), allocatable :: kcia(:, :, :)
@@ -53,6 +55,12 @@ program radg_host
   read (10) inlte, nlte_k1, nlte_a
   ! /SOLARDAT/ iform
   read (10) iform
+  ! /FPTABLE/: npara (0 = none), then dnu, temps(25), fp(npara), kh2h2, kh2he, kcia5
+  read (10) npara
+  if (npara > 0) then
+    allocate (pfp(npara), pkhh(npara * 25 * 1501), pkhe(npara * 25 * 1501), pk5(5 * 25 * 1501))
+    read (10) pdnu, ptemps, pfp, pkhh, pkhe, pk5
+  end if
   close (10)
 
   if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
@@ -61,8 +69,12 @@ program radg_host
   if (radg_register_dust(nsec, ncont, vsec, xsec) /= RADG_OK) call die()
   if (radg_set_nlte(inlte, nlte_k1, nlte_a) /= RADG_OK) call die()
   if (radg_set_iform(iform, 0) /= RADG_OK) call die()
+  if (npara > 0) then
+    if (radg_register_parah2(pdnu, ptemps, npara, pfp, pkhh, pkhe, pk5) /= RADG_OK) call die()
+  end if
 
   nparam = ngas + 1 + ncont
+  if (flagh2p == 1) nparam = nparam + 1
   allocate (doutputdq(maxpat, maxlay, nparmax))
   open (11, file=trim(fout), access='stream', form='unformatted', status='replace')
   do iwave = 1, nwave
@@ -77,7 +89,7 @@ program radg_host
   allocate (sout(nwave * npath), sgts(nwave * npath), sdq(nwave * npath * nlayer * nparam))
   rc = radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, press, temp, pp, amount, &
                         ispace, nlayin, layinc, cont, scale, imod, idgas, isogas, emtemp, itype, nem, &
-                        vem, emissivity, tsurf, radius1, radextra, flagh2p, nsw, isw, nwave, vwave, &
+                        vem, emissivity, tsurf, radius1, radextra, flagh2p, hfp, nsw, isw, nwave, vwave, &
                         sout, sgts, sdq)
   if (rc /= RADG_OK) call die()
   write (11) sout, sgts, sdq

@@ -16,7 +16,7 @@ module radg
   integer(c_int), parameter, public :: RADG_OK = 0
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
-  public :: radg_register_dust, radg_set_nlte, radg_set_iform
+  public :: radg_register_dust, radg_set_nlte, radg_set_iform, radg_register_parah2
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -40,6 +40,7 @@ module radg
     real(c_double) :: nlte_k1, nlte_a
     integer(c_int) :: iform                  ! /SOLARDAT/ iform (no solar file)
     real(c_double) :: radius1, radextra      ! km
+    type(c_ptr) :: hfp                       ! hfp(nlayer) when flagh2p = 1
   end type
 
   ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
@@ -50,6 +51,7 @@ module radg
     integer(c_int) :: nv, npro
     type(c_ptr) :: dam, dte, xmap
     type(c_ptr) :: dco
+    type(c_ptr) :: dfp                       ! /LGRAD/ DFP when flagh2p = 1
   end type
 
   interface
@@ -79,6 +81,15 @@ module radg
       real(c_double) :: vsec(*), xsec(*)
     end function
 
+    ! /FPTABLE/: kh2h2/kh2he(npara,25,1501), kcia5(5,25,1501); npara = 0 clears
+    integer(c_int) function radg_register_parah2(dnu, temps, npara, fp, kh2h2, kh2he, kcia5) &
+        bind(C, name="radg_register_parah2")
+      import :: c_int, c_double
+      real(c_double), value :: dnu
+      integer(c_int), value :: npara
+      real(c_double) :: temps(25), fp(*), kh2h2(*), kh2he(*), kcia5(*)
+    end function
+
     ! /SOLARDAT/: iform, iread (iread = 999 with iform 1 or 4 is refused)
     integer(c_int) function radg_set_iform(iform, iread) bind(C, name="radg_set_iform")
       import :: c_int
@@ -104,10 +115,11 @@ module radg
     ! output(npath,nwave) ... in Fortran order: see include/radg.h.
     integer(c_int) function radg_spectrum_r4(dist, inormal, iray, delh, nlayer, npath, ngas, &
         press, temp, pp, amount, ispace, nlayin, layinc, cont, scale, imod, idgas, isogas, &
-        emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra, flagh2p, nsw, isw, nwave, &
-        vwave, output, gtsurf, doutputdq) bind(C, name="radg_spectrum_r4")
+        emtemp, itype, nem, vem, emissivity, tsurf, radius1, radextra, flagh2p, hfp, nsw, isw, &
+        nwave, vwave, output, gtsurf, doutputdq) bind(C, name="radg_spectrum_r4")
       import :: c_int, c_float
       real(c_float) :: dist, delh(*), press(*), temp(*), pp(*), amount(*), scale(*), cont(*)
+      real(c_float) :: hfp(*)
       real(c_float) :: emtemp(*), vem(*), emissivity(*), tsurf, vwave(*), radius1, radextra
       real(c_float) :: output(*), gtsurf(*), doutputdq(*)
       integer(c_int) :: inormal, iray, nlayer, npath, ngas, ispace, nlayin(*), layinc(*)

@@ -81,10 +81,26 @@ class KTable:
 
 @dataclass
 class CIATable:
-    """The ``/CIATABLE/`` content: ``kcia(9,25,1501)`` as ``[wn, T, pair]``."""
+    """The ``/CIATABLE/`` content: ``kcia(9,25,1501)`` as ``[wn, T, pair]``;
+    ``para``, when set, is the ``/FPTABLE/`` used with FLAGH2P = 1."""
     dnu: float
     temps: np.ndarray      # [25]
     kcia: np.ndarray       # [1501, 25, 9] float64
+    para: Optional["ParaCIATable"] = None
+
+
+@dataclass
+class ParaCIATable:
+    """The ``/FPTABLE/`` content that fpread_all.f reads (ciacom.f): H2-H2
+    and H2-He coefficients on a grid of para-H2 fractions ``fp`` (the first
+    at 0.25, uniform spacing, as nparacon_all.f's indexing assumes) and the
+    five other pairs (H2-N2, N2-CH4, N2-N2, CH4-CH4, H2-CH4)."""
+    dnu: float
+    temps: np.ndarray      # [25]
+    fp: np.ndarray         # [npara]
+    kh2h2: np.ndarray      # [1501, 25, npara]  (kh2h2(npara,25,1501))
+    kh2he: np.ndarray      # [1501, 25, npara]
+    kcia5: np.ndarray      # [1501, 25, 5]      (kcia5(5,25,1501))
 
 
 @dataclass
@@ -220,6 +236,36 @@ def make_cia(seed=3, dnu=1.0, pairs=(0, 1)) -> CIATable:
                         for a in range(3) for b in range(3))
         kcia[:, :, p] = np.power(10.0, lg)
     return CIATable(dnu=float(dnu), temps=temps, kcia=kcia)
+
+
+def make_para_cia(seed=6, dnu=1.0, npara=24, other_pairs=(0, 3, 4)) -> ParaCIATable:
+    """Smooth positive para-H2-dependent CIA: each fraction's H2-H2 and H2-He
+    coefficients are a random log-polynomial in (wavenumber, T) plus a term
+    linear in the para fraction; ``other_pairs`` of the five extra pairs are
+    non-zero."""
+    rng = np.random.default_rng(seed)
+    temps = np.linspace(40.0, 400.0, CIA_NUMT)
+    fp = np.linspace(0.25, 1.0, npara)
+    v = np.arange(CIA_NUMWN) * dnu
+    vs = (v - 1000.0) / 1000.0
+    ts = (temps - 150.0) / 150.0
+
+    def surface(base):
+        c = rng.normal(0.0, 0.4, size=(3, 3))
+        return base + sum(c[a, b] * vs[:, None] ** (a + 1) * ts[None, :] ** b
+                          for a in range(3) for b in range(3))
+
+    kh2h2 = np.empty((CIA_NUMWN, CIA_NUMT, npara))
+    kh2he = np.empty((CIA_NUMWN, CIA_NUMT, npara))
+    s_h2, s_he = surface(-5.4), surface(-5.8)
+    g_h2, g_he = rng.normal(0.0, 0.3, 2)
+    for i, f in enumerate(fp):
+        kh2h2[:, :, i] = np.power(10.0, s_h2 + g_h2 * (f - 0.25))
+        kh2he[:, :, i] = np.power(10.0, s_he + g_he * (f - 0.25))
+    kcia5 = np.zeros((CIA_NUMWN, CIA_NUMT, 5))
+    for p in other_pairs:
+        kcia5[:, :, p] = np.power(10.0, surface(-6.0))
+    return ParaCIATable(dnu=float(dnu), temps=temps, fp=fp, kh2h2=kh2h2, kh2he=kh2he, kcia5=kcia5)
 
 
 def make_atmosphere(nlayer=120, nkgas=3, seed=1, p_bot_bar=10.0,

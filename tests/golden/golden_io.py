@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from radtrancode_amd.synth import CIATable, KTable, Request
+from radtrancode_amd.synth import CIATable, KTable, ParaCIATable, Request
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -32,6 +32,10 @@ def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = 
     d = {"note": np.array(note)}
     if cia is not None:
         d.update(cia_dnu=np.array(cia.dnu), cia_temps=cia.temps, cia_kcia=cia.kcia)
+        if cia.para is not None:
+            pa = cia.para
+            d.update(para_dnu=np.array(pa.dnu), para_temps=pa.temps, para_fp=pa.fp,
+                     para_kh2h2=pa.kh2h2, para_kh2he=pa.kh2he, para_kcia5=pa.kcia5)
     for a in _REQ_ARRAYS:
         d["req_" + a] = np.asarray(getattr(req, a))
     for s in _REQ_SCALARS:
@@ -82,7 +86,13 @@ def case_cia(path) -> CIATable:
     shared one."""
     z = np.load(path, allow_pickle=False)
     if "cia_kcia" in z.files:
-        return CIATable(dnu=float(z["cia_dnu"]), temps=z["cia_temps"], kcia=z["cia_kcia"])
+        para = None
+        if "para_kh2h2" in z.files:
+            para = ParaCIATable(dnu=float(z["para_dnu"]), temps=z["para_temps"], fp=z["para_fp"],
+                                kh2h2=z["para_kh2h2"], kh2he=z["para_kh2he"],
+                                kcia5=z["para_kcia5"])
+        return CIATable(dnu=float(z["cia_dnu"]), temps=z["cia_temps"], kcia=z["cia_kcia"],
+                        para=para)
     return load_cia()
 
 

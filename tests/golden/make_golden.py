@@ -139,6 +139,24 @@ def cases():
     r.imod = np.array([3, 0, 3, 1, 3], dtype=np.int32)
     yield "iform1", t, r, "iform = 1 total flux on imod 3 paths; imod 0 and 1 paths unscaled"
 
+    # para-H2 (FLAGH2P = 1, nparacon_all.f): a para fraction per layer from
+    # 0.25 (deep, equilibrium at high T) rising aloft, with layers at the
+    # fraction grid's ends, one inside its last interval and one with
+    # fpara < 0 (the table's last fraction)
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.flagh2p = 1
+    r.hfp = np.linspace(0.25, 0.95, r.nlayer)
+    r.hfp[0], r.hfp[7], r.hfp[12], r.hfp[20] = 0.2, 1.0, 0.99, -1.0
+    r.isw = np.arange(1, r.nparam + 1, dtype=np.int32)
+    yield "parah2", t, r, "para-H2 fraction per layer (FLAGH2P = 1), 24 fractions"
+
+    t, c, r = synth.config1(nwave=5, nlayer=24, nkgas=2, ng=8)
+    r.flagh2p = 1
+    r.hfp = np.linspace(0.3, 0.8, r.nlayer)
+    r = add_dust(r, 1, 10, seed=45)
+    r.isw = np.arange(1, r.nparam + 1, dtype=np.int32)
+    yield "parah2_dust", t, r, "para-H2 with a 12-fraction table and one dust type (para column last)"
+
     # non-LTE source function (calc_nlte_t.f).  k1 = 1e-11 cm3/s and
     # A = 1e5 /s put f = k1 N / (A + k1 N) near 0.3 at the top and 1 at depth
     t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
@@ -253,23 +271,40 @@ def rayleigh_vectors(n=64, seed=7):
     return out
 
 
+def _band(vwave, dnu, n, half):
+    keep = np.zeros(n, dtype=bool)
+    for v in vwave:
+        i = int(v / dnu)
+        keep[max(0, i - half):i + half + 2] = True
+    return keep
+
+
 def banded_cia(cia, vwave, half=3):
     """The table with every wavenumber row farther than ``half`` rows from
     the case's wavenumbers set to zero: ciaread.f reads only the two rows
     bracketing a wavenumber, so results are unchanged and the stored case
     stays small."""
-    keep = np.zeros(cia.kcia.shape[0], dtype=bool)
-    for v in vwave:
-        i = int(v / cia.dnu)
-        keep[max(0, i - half):i + half + 2] = True
+    keep = _band(vwave, cia.dnu, cia.kcia.shape[0], half)
     k = cia.kcia.copy()
     k[~keep] = 0.0
-    return synth.CIATable(dnu=cia.dnu, temps=cia.temps, kcia=k)
+    para = None
+    if cia.para is not None:
+        pa = cia.para
+        kp = _band(vwave, pa.dnu, pa.kh2h2.shape[0], half)
+        arrs = [a.copy() for a in (pa.kh2h2, pa.kh2he, pa.kcia5)]
+        for a in arrs:
+            a[~kp] = 0.0
+        para = synth.ParaCIATable(dnu=pa.dnu, temps=pa.temps, fp=pa.fp, kh2h2=arrs[0],
+                                  kh2he=arrs[1], kcia5=arrs[2])
+    return synth.CIATable(dnu=cia.dnu, temps=cia.temps, kcia=k, para=para)
 
 
 # cases that run with a CIA table of their own: every pair the Titan case
 # touches (H2-H2, H2-He, H2-N2, N2-CH4, N2-N2, CH4-CH4, H2-CH4)
-CASE_CIA = {"titan_n2": dict(seed=5, pairs=(0, 1, 4, 5, 6, 7, 8))}
+CASE_CIA = {"titan_n2": dict(seed=5, pairs=(0, 1, 4, 5, 6, 7, 8)),
+            "parah2": dict(seed=7, pairs=(0, 1)), "parah2_dust": dict(seed=8, pairs=(0, 1))}
+# cases whose CIA table also carries a para-H2 table (FLAGH2P = 1)
+CASE_PARA = {"parah2": dict(seed=9), "parah2_dust": dict(seed=10, npara=12)}
 
 
 def main(names=()):
@@ -283,7 +318,12 @@ def main(names=()):
     for name, t, r, note in cases():
         if names and name not in names:
             continue
-        own = banded_cia(synth.make_cia(**CASE_CIA[name]), r.vwave) if name in CASE_CIA else None
+        own = None
+        if name in CASE_CIA:
+            own = synth.make_cia(**CASE_CIA[name])
+            if name in CASE_PARA:
+                own.para = synth.make_para_cia(**CASE_PARA[name])
+            own = banded_cia(own, r.vwave, half=1 if name in CASE_PARA else 3)
         outs = run_ref(t, own or cia, r)
         path = os.path.join(HERE, f"case_{name}.npz")
         save_case(path, t, r, outs, note, cia=own)

@@ -13,7 +13,7 @@ import pytest
 from radtrancode_amd import build as rbuild
 from radtrancode_amd.cirsradg import MAXCON, MAXGAS, MAXLAY, MAXPAT, NPARMAX, fortran_arrays
 from radtrancode_amd.kta import write_kta
-from tests.golden.golden_io import case_paths, load_case, load_cia
+from tests.golden.golden_io import case_cia, case_paths, load_case, load_cia
 from tests.tolerance import fp32_range, fp32_range_jac, jac_err, rad_err
 
 pytestmark = pytest.mark.gpu
@@ -43,6 +43,11 @@ def _write_input(path, klist, req, cia, mode):
                     + np.ascontiguousarray(req.xsec, np.float64).tobytes())
         f.write(i4(req.inlte) + np.array([req.nlte_k1, req.nlte_a], np.float64).tobytes())
         f.write(i4(req.iform))
+        pa = getattr(cia, "para", None)
+        f.write(i4(0 if pa is None else len(pa.fp)))
+        if pa is not None:
+            f.write(b"".join(np.ascontiguousarray(x, np.float64).tobytes()
+                             for x in ([pa.dnu], pa.temps, pa.fp, pa.kh2h2, pa.kh2he, pa.kcia5)))
 
 
 def _read_output(path, req):
@@ -66,13 +71,13 @@ def _read_output(path, req):
 
 
 @pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths", "dust", "nlte_k1",
-                                  "iform1", "wavelength_ray_dust"])
+                                  "iform1", "wavelength_ray_dust", "parah2_dust"])
 def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     host = rbuild.FORTRAN_HOST
     assert os.path.exists(host), "Fortran host not built (flang missing at build time?)"
     path = [p for p in case_paths() if p.endswith(f"case_{name}.npz")][0]
     tables, req, outs, note = load_case(path)
-    cia = load_cia()
+    cia = case_cia(path)
     names = []
     for t, kt in enumerate(tables):
         kp = str(tmp_path / f"g{t}.kta")
@@ -107,7 +112,7 @@ def _rounded(req):
     import copy
     r = copy.copy(req)
     for k in ("press", "temp", "pp", "amount", "delh", "vwave", "scale", "emtemp", "vem",
-              "emissivity", "cont"):
+              "emissivity", "cont", "hfp"):
         if getattr(req, k) is not None:
             setattr(r, k, np.asarray(getattr(req, k), dtype=np.float32).astype(np.float64))
     r.tsurf = float(np.float32(req.tsurf))

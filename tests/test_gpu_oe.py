@@ -167,3 +167,36 @@ def test_statemaps_with_dust(gpu_lib, mode):
         assert _rel(mp["doutdx"][w], dx) < 1e-9, w
         assert _rel(mp["kk"][w], q.path_weight @ dx) < 1e-9, w
     b.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["fp64", "mixed"])
+def test_statemaps_with_para_h2(gpu_lib, mode):
+    """FLAGH2P = 1 with one dust type: the dust column maps through DCO and
+    the para-H2 fraction (the last parameter) through DFP (map2pro.f:72-80)."""
+    from tests.golden.golden_io import case_cia, case_paths, load_case, load_cia
+    path = [p for p in case_paths() if p.endswith("case_parah2_dust.npz")][0]
+    tables, q, _, _ = load_case(path)
+    gpu_lib.register_ktables(tables)
+    gpu_lib.register_cia(case_cia(path))
+    try:
+        rng = np.random.default_rng(9)
+        npro, nv = 19, 13
+        m = dict(dam=rng.normal(size=(q.nlayer, npro)) * 1e20, dte=rng.normal(size=(q.nlayer, npro)),
+                 dco=rng.normal(size=(q.nlayer, npro)) * 1e7, dfp=rng.normal(size=(q.nlayer, npro)),
+                 xmap=rng.normal(size=(nv, q.nparam, npro)))
+        print(f"statemaps with para-H2, mode {mode}", flush=True)
+        b = gpu_lib.Batch([q], mode=mode)
+        b.set_statemaps([m], per_path=True)
+        b.launch()
+        b.sync()
+        res = b.read(0, weighted=False)
+        mp = b.read_mapped(0, per_path=True)
+        for w in range(q.nwave):
+            dmod = oracle_oe.map2pro(res["doutputdq"][w], q.nlayin, q.layinc, m["dam"], m["dte"],
+                                     q.ngas, dco=m["dco"], dfp=m["dfp"], ncont=q.ncont)
+            dx = oracle_oe.map2xvec(dmod, m["xmap"])
+            assert _rel(mp["doutdx"][w], dx) < 1e-9, w
+            assert _rel(mp["kk"][w], q.path_weight @ dx) < 1e-9, w
+        b.close()
+    finally:
+        gpu_lib.register_cia(load_cia())

@@ -707,12 +707,12 @@ struct Layout {
 struct Plan {
   std::vector<Packed> packs;
   size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
-      o_vw, o_es, o_wg, o_pw, o_dx, o_ct, o_dp, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts,
-      o_wjac, o_err;
+      o_vw, o_es, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
+      o_err;
   size_t total = 0;
   long long nA = 0, nB = 0, nC = 0;
   long long nlay = 0, nlg = 0, npath = 0, npl = 0, nwv = 0, nwg = 0, ninter = 0, nout = 0, njac = 0,
-            nwout = 0, nwjac = 0, ndx = 0, nct = 0, ndp = 0;
+            nwout = 0, nwjac = 0, ndx = 0, nct = 0;
   std::vector<long long> sA, sB, sC;
   int maxngas = 1, maxnsw = 1, maxlays = 1;
   std::vector<int> cls_rl;
@@ -748,8 +748,7 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     d.wjac0 = pl.nwjac;
     d.dx0 = pl.ndx;
     d.ct0 = pl.nct;
-    d.dp0 = pl.ndp;
-    if (d.flagh2p) pl.ndp += (long long)q.nwave * q.nlayer;
+
     pl.ndx += (long long)q.nwave * q.ncont;
     pl.nct += (long long)q.nlayer * q.ncont;
     pl.nlay += q.nlayer;
@@ -761,7 +760,7 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     pl.nA += (long long)q.nwave * q.nlayer;
     pl.nB += (long long)q.nwave * q.npath;
     pl.nC += q.nwave;
-    pl.ninter += (long long)q.nwave * inter_block(ts.ng, q.nlayer, q.ngas);
+    pl.ninter += (long long)q.nwave * inter_block(ts.ng, q.nlayer, q.ngas, d.nparam);
     pl.nout += (long long)q.nwave * q.npath;
     pl.njac += (long long)q.nwave * q.npath * q.nlayer * d.nparam;
     pl.nwout += q.nwave;
@@ -805,7 +804,6 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_pw = lo.take<double>(pl.npath);
   pl.o_dx = lo.take<double>(pl.ndx);
   pl.o_ct = lo.take<double>(pl.nct);
-  pl.o_dp = lo.take<double>(pl.ndp);
   pl.o_clsmap = lo.take<int>(nreq);
   pl.o_clsstart = lo.take<long long>(nreq + pl.cls_rl.size());
   pl.o_inter = lo.take<char>((size_t)pl.ninter * ts_);
@@ -1187,7 +1185,6 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
   a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
   a.cont = reinterpret_cast<const double *>(m + pl.o_ct);
-  a.dpara = reinterpret_cast<double *>(m + pl.o_dp);
   if (para) {
     a.pkh2h2 = para->d_kh2h2;
     a.pkh2he = para->d_kh2he;

@@ -19,7 +19,6 @@ struct DevReq {
   int ncont;                   // dust types; nparam = ngas + 1 + ncont
   int inlte;                   // calc_nlte_t.f flag (0 = LTE)
   int flagh2p, pad2;           // 1: para-H2 CIA (nparacon_all.f), fpara the last parameter
-  long long dp0;               // per-(wavenumber, layer) d(tau)/d(fpara)
   long long dx0;               // per-(wavenumber, dust type) extinction
   long long ct0;               // per-(layer, dust type) particle columns
   double tsurf;
@@ -73,12 +72,14 @@ struct PathInfo {
 };
 
 // Intermediate block per (request, wavenumber), in elements of the compute
-// type:  ktot[NG][L] | dk[NG][ngas+1][L] | tcon[L] | dtcon[ngas+1][L]
+// type:  ktot[NG][L] | dk[NG][ngas+1][L] | tcon[L] | dtcon[nparam][L]
 // dk and dtcon hold derivatives with respect to amount*1e-20 for gas slots
 // (the reference's dkdq before its 1e-20 factor, cirsradg_wave.f:749) and
-// with respect to T for slot ngas.
-__host__ __device__ inline long long inter_block(int ng, int L, int ngas) {
-  return (long long)ng * L * (ngas + 2) + (long long)L * (ngas + 2);
+// with respect to T for slot ngas; dtcon's further rows are the continuum's
+// only parameters: each dust type (its extinction, the same in every layer)
+// and the para-H2 fraction.
+__host__ __device__ inline long long inter_block(int ng, int L, int ngas, int nparam) {
+  return (long long)ng * L * (ngas + 2) + (long long)L * (1 + nparam);
 }
 
 struct KernelArgs {
@@ -117,7 +118,6 @@ struct KernelArgs {
   const double *pkh2h2, *pkh2he, *pkcia5, *ptemps, *pfp;
   double pdnu;
   int npara;
-  double *dpara;               // [request, wavenumber, layer] d(tau)/d(fpara)
   // outputs
   void *inter;                 // intermediates, compute type
   void *out, *gts, *jac;       // compute type; jac gas columns per 1e20 molecules

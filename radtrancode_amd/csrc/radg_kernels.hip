@@ -1179,7 +1179,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
   }
 
   // ---- write the layer's k-distribution and derivatives ------------------
-  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas);
+  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas, R.nparam);
   T *ktot = inter;                                  // [NG][L]
   T *dkq = inter + (long long)NG * L;               // [NG][ngas+1][L]
   for (int i = lane; i < NG; i += W) {
@@ -1206,7 +1206,7 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   const int j = (int)(loc - (long long)iw * L);
   const LayerConst lc = a.lay[R.lay0 + j];
   const double vwave = a.vwave[R.wave0 + iw];
-  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas);
+  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas, R.nparam);
   T *tcon = inter + (long long)NG * L + (long long)NG * (ngas + 1) * L;
   T *dtcon = tcon + L;
   double absorb, dab[7];
@@ -1221,7 +1221,6 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   else
     cia_layer(a, vv, R.inormal, lc, &absorb, dab);
   absorb = tdust + absorb;
-  if (R.flagh2p) a.dpara[R.dp0 + (long long)iw * L + j] = dab[6];
   // Rayleigh (cirsradg_wave.f:663-691): utotl * xray more extinction and
   // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.  The gas
   // rows of dtcon, like dk/dq, are per 1e20 molecules (the 1e-20 is applied
@@ -1240,6 +1239,11 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
     else d += dray;
     dtcon[(long long)s * L + j] = (T)d;
   }
+  // the continuum-only parameters: each dust type's extinction
+  // (cirsradg_wave.f:573), then the para-H2 fraction (:659-661)
+  for (int c = 0; c < R.ncont; c++)
+    dtcon[(long long)(ngas + 1 + c) * L + j] = (T)a.dext[R.dx0 + (long long)iw * R.ncont + c];
+  if (R.flagh2p) dtcon[(long long)(R.nparam - 1) * L + j] = (T)dab[6];
   tcon[j] = (T)absorb;
 }
 
@@ -1330,7 +1334,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const int nlays = pi.nlays;
   const double v = a.vwave[R.wave0 + iw];
   const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
-                   (long long)iw * inter_block(NG, L, ngas);
+                   (long long)iw * inter_block(NG, L, ngas, R.nparam);
   const T *ktot = inter;
   const T *dkq = inter + (long long)NG * L;
   const T *tcon = dkq + (long long)NG * (ngas + 1) * L;
@@ -1519,16 +1523,13 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       const int m = lane * RL + q;
       for (int k1 = 0; k1 < nsw; k1++) {
         const int k = R.isw[k1];
-        // gases and T: continuum + overlap derivatives; dust type k-ngas-1:
-        // d(tau)/d(particles/cm2) is its extinction, the same in every layer
-        // the para-H2 fraction (FLAGH2P = 1, the last parameter): its per-layer
-        // d(tau)/d(fpara) from the continuum kernel
-        const T dq = (k <= ngas ? dtcon[(long long)k * L + Lm[q]] +
-                                      dkq[((long long)g * (ngas + 1) + k) * L + Lm[q]]
-                      : k - ngas - 1 < R.ncont
-                          ? (T)a.dext[R.dx0 + (long long)iw * R.ncont + (k - ngas - 1)]
-                          : (T)a.dpara[R.dp0 + (long long)iw * L + Lm[q]]) *
-                     sc[q];
+        // continuum rows for every parameter; the overlap's dk/dq only for the
+        // gases and T (dust and the para-H2 fraction act on the continuum
+        // only).  Branch-free (the row index clamped, the value selected), so
+        // the loop unrolls as it does without those parameters.
+        const int kd = k < ngas ? k : ngas;
+        const T dkv = dkq[((long long)g * (ngas + 1) + kd) * L + Lm[q]];
+        const T dq = (dtcon[(long long)k * L + Lm[q]] + (k <= ngas ? dkv : (T)0)) * sc[q];
         T c = dRdtau[q] * dq;
         if (k == ngas) c = c + dT[q];
         const T x = c * dg;

@@ -15,6 +15,7 @@
  *   radg_register_ktables    the same registry from arrays already in memory
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
  *   radg_register_parah2     /FPTABLE/ filled by radtran/radtran/fpread_all.f:72-143
+ *   radg_register_e3         /E3TABLE/ filled by radtran/radtran/e3interp.f:30-51
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
  *                            225,312) that the caller fills from its .xsc file
  *   radg_set_nlte            /NLTE_FLAGS/ inlte_flag, nlte_k1, nlte_A (cirsradg_wave.f:328)
@@ -49,6 +50,7 @@ extern "C" {
 #define RADG_MAXG 51
 #define RADG_MAXK 30
 #define RADG_NPARMAX (RADG_MAXGAS + 2 + RADG_MAXCON)
+#define RADG_E3N 1001 /* e3interp.f's mtab */
 /* The GPU overlap sorts NG*NG keys in one wavefront's registers. */
 #define RADG_GPU_MAXG 32
 
@@ -124,8 +126,15 @@ int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xs
 int radg_register_parah2(double dnu, const double *temps, int npara, const double *fp,
                          const double *kh2h2, const double *kh2he, const double *kcia5);
 
+/* /E3TABLE/ for disc-integrated emission (imod 18): E3 of the optical depth
+ * -ln(trans) at the n = 1001 transmissions trans[n] = (i-1)/1000, as
+ * e3tab.dat holds it (e3interp.f:33-44).  The gradient is formed from the
+ * table as e3interp.f:47-51 does.  Needed by requests with an imod 18 path;
+ * n = 0 clears it. */
+int radg_register_e3(int n, const double *trans, const double *e3);
+
 /* /NLTE_FLAGS/ (cirsradg_wave.f:328) for the fixed-argument entry points:
- * the emission temperature of every layer of an imod 3 path becomes
+ * the emission temperature of every layer of an imod 3 or 18 path becomes
  * calc_nlte_t.f's (flag 0 = LTE; 1 = collisional with k1 in [1e-20, 1e-3]
  * and A; 2 = pressure form).  Batched requests carry their own. */
 int radg_set_nlte(int flag, double k1, double a);
@@ -160,8 +169,8 @@ int radg_cirsradg_status(void);
 
 /* The whole wavenumber loop in one call.  Arguments as cirsradg_wave_ (fixed
  * leading dimensions; cont(maxcon,maxlay) is read for the registered dust
- * types; radius1 and radextra scale imod 3 paths when radg_set_iform chose
- * iform 1 or 3), plus nwave and vwave[nwave].  Outputs, C order:
+ * types; radius1 and radextra scale imod 3 and 18 paths when radg_set_iform
+ * chose iform 1 or 3), plus nwave and vwave[nwave].  Outputs, C order:
  *   output[nwave][npath], gtsurf[nwave][npath],
  *   doutputdq[nwave][npath][nlayer][ngas+1+ncont]   (params: gas amounts
  *   1..ngas, then temperature, then the registered dust types; per molecule
@@ -201,7 +210,9 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
  *   nlayin/imod[npath]; layinc (1-based layer numbers), scale, emtemp
  *   [npath][L]; isw[nsw] (1-based parameter ids); vwave[nwave];
  *   vem/emissivity[nem]; path_weight[npath] (angle quadrature weights used
- *   for the weighted spectrum; may be NULL).
+ *   for the weighted spectrum; may be NULL).  imod 0, 1, 3 and 18 (disc
+ *   integrated, cirsradg_wave.f:1116-1218: needs radg_register_e3 and
+ *   tsurf > 0; its surface term carries no Jacobian, as in the reference).
  * Dust (cirsradg_wave.f:550-577), ncont = 0 for none: cont[L][ncont]
  *   particles/cm2, cross-section grid vsec[nsec] (ascending cm-1) and
  *   extinction xsec[ncont][nsec] (cm2 per particle; nsec = 1: constant).
@@ -222,7 +233,8 @@ typedef struct radg_request {
   double nlte_k1, nlte_a;
   int iform;                 /* /SOLARDAT/ iform: 1 and 3 give imod 3 paths the */
   double radius1, radextra;  /* planet's total flux, 4 pi^2 ((radius1 + radextra)
-                                km in cm)^2, times 1e-18 for 3 (cirsradg_wave.f:912-925) */
+                                km in cm)^2, times 1e-18 for 3 (cirsradg_wave.f:912-925);
+                                imod 18 paths 4 pi (...)^2 (:1134-1149) */
   const double *hfp;         /* [L] para-H2 fractions when flagh2p = 1 (nparacon_all.f;
                                 < 0: the table's last fraction); fpara is then the last
                                 Jacobian parameter */

@@ -53,7 +53,7 @@ class _CIA(C.Structure):
     _fields_ = [("dnu", C.c_double), ("temps", C.c_void_p), ("kcia", C.c_void_p),
                 ("npara", C.c_int), ("dnu2", C.c_double), ("temps2", C.c_void_p),
                 ("fp", C.c_void_p), ("kh2h2", C.c_void_p), ("kh2he", C.c_void_p),
-                ("kcia5", C.c_void_p)]
+                ("kcia5", C.c_void_p), ("e3tran", C.c_void_p), ("e3val", C.c_void_p)]
 
 
 class _REQ(C.Structure):
@@ -105,6 +105,11 @@ class _Pack:
             self.keep += arrs
             self.cia = _CIA(cia.dnu, ct.ctypes.data, kc.ctypes.data, len(pa.fp), pa.dnu,
                             *[a.ctypes.data for a in arrs])
+        e3 = getattr(cia, "e3", None)
+        if e3 is not None:
+            t3, v3 = _d(e3.trans), _d(e3.e3)
+            self.keep += [t3, v3]
+            self.cia.e3tran, self.cia.e3val = t3.ctypes.data, v3.ctypes.data
         L = req.nlayer
         hfp = req.hfp if req.hfp is not None else np.zeros(L)
         f = [_d(req.press), _d(req.temp), _d(req.delh), _d(hfp), _d(req.pp),

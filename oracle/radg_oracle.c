@@ -838,6 +838,44 @@ static void orc_nparacon_all(const orc_cia *cia, double vwave, double p, double 
   for (i = 0; i < 7; i++) dabsorb[i] *= tau;
 }
 
+/* --------------------------------------------------------------- e3interp */
+/* e3interp.f:14-79 on the table in memory: E3 and dE3/dtr at transmission x,
+ * with egrad from the table's differences as e3interp.f:47-51 forms it. */
+static void orc_e3interp(const orc_cia *cia, double x, double *e3, double *de3) {
+  const int mtab = 1001;
+  const double dtrans = 1.0 / (double)(mtab - 1);
+  const double *tt = cia->e3tran, *te = cia->e3val;
+  int j;
+  double f, g0, g1;
+  if (x < 0.0 || isnan(x)) {
+    *e3 = 0.0;
+    *de3 = 0.0;
+    return;
+  }
+  j = 1 + (int)(x / dtrans);
+  if (j >= mtab) j = mtab - 1;
+  f = (x - tt[j - 1]) / dtrans;
+  if (f > 1.0) f = 1.0;
+#define EGRAD(i) ((i) == 1 ? (te[1] - te[0]) / dtrans                                      \
+                  : (i) == mtab ? (te[mtab - 1] - te[mtab - 2]) / dtrans                   \
+                                : 0.5 * (te[(i)] - te[(i) - 2]) / dtrans)
+  g0 = EGRAD(j);
+  g1 = EGRAD(j + 1);
+#undef EGRAD
+  *e3 = (1.0 - f) * te[j - 1] + f * te[j];
+  *de3 = (1.0 - f) * g0 + f * g1;
+}
+
+double orc_e3(const double *trans, const double *e3, double x, double *de3) {
+  orc_cia c;
+  double v;
+  memset(&c, 0, sizeof c);
+  c.e3tran = trans;
+  c.e3val = e3;
+  orc_e3interp(&c, x, &v, de3);
+  return v;
+}
+
 /* --------------------------------------------------------------- interpem */
 static int orc_interpem(int nem, const double *vem, const double *em, double x,
                         double *out) {
@@ -1008,7 +1046,12 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
     if (r->nlayin[ip] > 2 * MAXPAT || r->nlayin[ip] < 1) return ORC_ERR_LIMITS;
     for (j = 0; j < r->nlayin[ip]; j++)
       if (r->layinc[ip * L + j] > L || r->layinc[ip * L + j] < 1) return ORC_ERR_LIMITS;
-    if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3) return ORC_ERR_UNSUPPORTED;
+    if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3 && r->imod[ip] != 18)
+      return ORC_ERR_UNSUPPORTED;
+    /* imod 18 needs /E3TABLE/ and a ground temperature (its ground term is
+     * always added, and planckg at tsurf <= 0 is not a number) */
+    if (r->imod[ip] == 18 && (!cia->e3tran || !cia->e3val || !(r->tsurf > 0.0)))
+      return ORC_ERR_UNSUPPORTED;
   }
   if (r->ncont < 0 || r->ncont > MAXCON || (r->ncont > 0 && (r->nsec < 1 || r->nsec > MAXSEC)))
     return ORC_ERR_LIMITS;                                        /* check_limits.f:24-33 */
@@ -1221,6 +1264,102 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
               dco[j * nparam + k] = tmp * (1 - co) * dtaudq[j * nparam + k];
             }
         }
+      } else if (r->imod[ip] == 18) {
+        /* disc-integrated emission, cirsradg_wave.f:1116-1211: the layer
+         * weights are differences of E3(tr) (e3interp.f), xfac = 4 pi
+         * ((radius1 + radextra) 1e5)^2 for iform 1 and 3 (no extra pi, unlike
+         * imod 3), times 2 pi; the ground term is always added and carries
+         * no Jacobian (:1196-1205). */
+        double xfac = 1.0, trold = 1.0, e3old = 0.5, e3new, e3g, tlayer, tr, bsurf;
+        if (r->iform == 1 || r->iform == 3)
+          xfac = xfac * 4. * PI_REF * ((r->radius1 + r->radextra) * 1e5) *
+                 ((r->radius1 + r->radextra) * 1e5);
+        if (r->iform == 3) xfac = xfac * 1e-18;
+        if (ig == 0) {
+          for (j = 0; j < nlays; j++) {
+            const double tn = orc_nlte_t(r->inlte, r->nlte_k1, r->nlte_a, r->emtemp[ip * L + j],
+                                         r->press[r->layinc[ip * L + j] - 1], vwave);
+            bb[ip * L + j] = orc_planck(r->ispace, vwave, tn);
+            dbdt[ip * L + j] = orc_planckg(r->ispace, vwave, tn);
+          }
+        }
+        bsurf = orc_planck(r->ispace, vwave, r->tsurf);
+        if (jac_mode == 0) {
+          /* de3dq(j1) = e3grad(tr_j) dtr_j/dq, dtrdq as in imod 3 */
+          for (k1 = 0; k1 < r->nsw; k1++) {
+            k = r->isw[k1] - 1;
+            for (j = 0; j < nlays; j++) dtolddq[j * nparam + k] = dtrdq[j * nparam + k] = 0.0;
+          }
+          {
+            double *de3old = (double *)calloc((size_t)nlays * nparam, sizeof(double));
+            double *de3 = (double *)calloc((size_t)nlays * nparam, sizeof(double));
+            if (!de3old || !de3) {
+              free(de3old);
+              free(de3);
+              rc = ORC_ERR_ALLOC;
+              goto done;
+            }
+            for (j = 0; j < nlays; j++) {
+              const double bj = bb[ip * L + j];
+              tlayer = exp(-taus[j]);
+              tr = trold * tlayer;
+              orc_e3interp(cia, tr, &e3new, &e3g);
+              co = co + xfac * 2.0 * PI_REF * (e3old - e3new) * bj;
+              for (k1 = 0; k1 < r->nsw; k1++) {
+                int j1;
+                k = r->isw[k1] - 1;
+                for (j1 = 0; j1 <= j; j1++) {
+                  if (j1 < j)
+                    dtrdq[j1 * nparam + k] = dtolddq[j1 * nparam + k] * tlayer;
+                  else
+                    dtrdq[j1 * nparam + k] = -dtaudq[j1 * nparam + k] * tr;
+                  de3[j1 * nparam + k] = e3g * dtrdq[j1 * nparam + k];
+                  dco[j1 * nparam + k] +=
+                      bj * xfac * 2.0 * PI_REF * (de3old[j1 * nparam + k] - de3[j1 * nparam + k]);
+                }
+                if (k == ngas)
+                  dco[j * nparam + k] += xfac * 2.0 * PI_REF * (e3old - e3new) * dbdt[ip * L + j];
+              }
+              e3old = e3new;
+              trold = tr;
+              for (int j1 = 0; j1 <= j; j1++)
+                for (k1 = 0; k1 < r->nsw; k1++) {
+                  k = r->isw[k1] - 1;
+                  dtolddq[j1 * nparam + k] = dtrdq[j1 * nparam + k];
+                  de3old[j1 * nparam + k] = de3[j1 * nparam + k];
+                }
+            }
+            free(de3old);
+            free(de3);
+          }
+        } else {
+          /* O(L): dR/dtau_m = xfac 2 pi sum_{j>=m} (B_j - B_{j+1}) g_j tr_j,
+           * B_{N+1} = 0, g_j = e3grad(tr_j) */
+          double s = 0.0;
+          trv[0] = 1.0;
+          for (j = 0; j < nlays; j++) {
+            trv[j + 1] = trv[j] * exp(-taus[j]);
+            orc_e3interp(cia, trv[j + 1], &e3new, &e3g);
+            tlv[j] = e3new;  /* E3(tr_j) */
+            wv[j] = e3g;     /* e3grad(tr_j) */
+            co = co + xfac * 2.0 * PI_REF * (e3old - e3new) * bb[ip * L + j];
+            e3old = e3new;
+          }
+          for (j = nlays - 1; j >= 0; j--) {
+            const double bnext = j + 1 < nlays ? bb[ip * L + j + 1] : 0.0;
+            const double e3prev = j > 0 ? tlv[j - 1] : 0.5;
+            s += (bb[ip * L + j] - bnext) * wv[j] * trv[j + 1];
+            for (k1 = 0; k1 < r->nsw; k1++) {
+              k = r->isw[k1] - 1;
+              dco[j * nparam + k] = xfac * 2.0 * PI_REF * s * dtaudq[j * nparam + k];
+              if (k == ngas)
+                dco[j * nparam + k] += xfac * 2.0 * PI_REF * (e3prev - tlv[j]) * dbdt[ip * L + j];
+            }
+          }
+        }
+        co = co + xfac * 2.0 * PI_REF * e3old * (esurf * bsurf);
+        tempgtsurf[ip * ng + ig] =
+            xfac * 2.0 * PI_REF * e3old * esurf * orc_planckg(r->ispace, vwave, r->tsurf);
       } else {
         /* imod = 3, cirsradg_wave.f:896-1035.  xfac, :912-925 with no solar
          * file (iread != 999): the planet's total flux for iform 1 and 3,

@@ -37,6 +37,9 @@ typedef struct orc_cia {
   int npara;
   double dnu2;
   const double *temps2, *fp, *kh2h2, *kh2he, *kcia5;
+  /* /E3TABLE/ (e3interp.f): 1001 transmissions and E3 values (NULL: none;
+   * imod 18 needs it) */
+  const double *e3tran, *e3val;
 } orc_cia;
 
 /* The argument surface of cirsradg_wave (cirsradg_wave.f:1-5), compact C
@@ -109,6 +112,8 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t,
                  double *absorb, int *iabs, double *dabsorb);       /* nciacon.f */
 double orc_nlte_t(int inlte, double k1, double a, double t, double p, double v); /* calc_nlte_t.f */
 double orc_interp(const double *x, const double *y, int n, double xin); /* interp.f */
+/* e3interp.f on a 1001-point table: E3 at transmission x, dE3/dtr in *de3 */
+double orc_e3(const double *trans, const double *e3, double x, double *de3);
 double orc_rayleigh(int iray, double v, double p, double t, double fheh2,
                     double fch4h2, double fnh3);                    /* rayleigh*.f */
 

@@ -103,6 +103,11 @@ class RefRunner:
         temps = np.ascontiguousarray(cia.temps, dtype=rdt)
         kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
         self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        e3 = getattr(cia, "e3", None)
+        if e3 is not None:
+            self._e3 = [np.ascontiguousarray(e3.trans, dtype=np.float64),
+                        np.ascontiguousarray(e3.e3, dtype=np.float64)]
+            self.lib.hsete3_(_p(self._e3[0]), _p(self._e3[1]))
         pa = getattr(cia, "para", None)
         if pa is not None:
             self._para = [np.array([pa.dnu], dtype=rdt), np.ascontiguousarray(pa.temps, dtype=rdt),

@@ -155,6 +155,28 @@ C     NP para fractions FPV, KH2H2/KH2HE(NP,NUMT,NUMWN), KC5(5,NUMT,NUMWN)
       RETURN
       END
 
+C     /E3TABLE/ (e3interp.f:14-51) in memory, so no e3tab.dat is read:
+C     the transmission grid, E3 and its gradient as e3interp.f forms it
+      SUBROUTINE HSETE3(TT, TE)
+      IMPLICIT NONE
+      INTEGER MTAB, I
+      PARAMETER (MTAB=1001)
+      DOUBLE PRECISION TT(MTAB), TE(MTAB)
+      DOUBLE PRECISION TABTRAN(MTAB), TABE3(MTAB), EGRAD(MTAB), DTRANS
+      COMMON /E3TABLE/ TABTRAN, TABE3, EGRAD
+      DTRANS = 1.0/FLOAT(MTAB-1)
+      DO I = 1, MTAB
+        TABTRAN(I) = TT(I)
+        TABE3(I) = TE(I)
+      ENDDO
+      EGRAD(1) = (TABE3(2)-TABE3(1))/DTRANS
+      EGRAD(MTAB) = (TABE3(MTAB)-TABE3(MTAB-1))/DTRANS
+      DO I = 2, MTAB-1
+        EGRAD(I) = 0.5*(TABE3(I+1)-TABE3(I-1))/DTRANS
+      ENDDO
+      RETURN
+      END
+
       SUBROUTINE HREADK(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE, NKL)
       IMPLICIT NONE
       CHARACTER*(*) KL

@@ -11,7 +11,8 @@ Reference correspondence:
 * :func:`register_ktables` / :func:`read_klist` -- the ``/INTERPK/`` registry
   that ``read_klist.f`` fills; :func:`register_cia` -- ``/CIATABLE/``;
   :func:`register_dust` -- ``/DUST/`` for :func:`cirsradg_wave` (batched
-  requests carry their own dust arrays).
+  requests carry their own dust arrays); :func:`register_parah2` --
+  ``/FPTABLE/``; :func:`register_e3` -- ``/E3TABLE/`` for imod 18.
 * :class:`Batch` -- many requests resident on the device, launched together.
 
 Errors raise :class:`RadgError` where the reference would STOP.
@@ -80,13 +81,27 @@ def read_klist(path: str, recl: int = 4) -> None:
 
 def register_cia(cia) -> None:
     """``/CIATABLE/`` and, if ``cia.para`` is set, ``/FPTABLE/`` (else the
-    para-H2 table is cleared)."""
+    para-H2 table is cleared); likewise ``/E3TABLE/`` from ``cia.e3``."""
     temps = np.ascontiguousarray(cia.temps, dtype=np.float64)
     kcia = np.ascontiguousarray(cia.kcia, dtype=np.float64)
     if kcia.shape != (1501, 25, 9):
         raise ValueError("kcia must be [1501][25][9]")
     check(lib().radg_register_cia(C.c_double(cia.dnu), _vp(temps), _vp(kcia)))
     register_parah2(getattr(cia, "para", None))
+    register_e3(getattr(cia, "e3", None))
+
+
+def register_e3(e3) -> None:
+    """``/E3TABLE/`` (e3interp.f) from a ``synth.E3Table`` of 1001 points
+    (None clears it); needed by imod 18 paths."""
+    if e3 is None:
+        check(lib().radg_register_e3(0, None, None))
+        return
+    t = np.ascontiguousarray(e3.trans, dtype=np.float64)
+    v = np.ascontiguousarray(e3.e3, dtype=np.float64)
+    if t.shape != v.shape or t.ndim != 1:
+        raise ValueError("E3 table: trans and e3 must be 1-D of one length")
+    check(lib().radg_register_e3(int(t.shape[0]), _vp(t), _vp(v)))
 
 
 def register_parah2(para) -> None:

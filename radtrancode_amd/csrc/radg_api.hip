@@ -28,7 +28,8 @@ hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t ld
 hipError_t radg_convert_f32(const double *src, float *dst, long long n, hipStream_t st);
 hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n, double off,
                              double sc, hipStream_t st);
-hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, size_t lds, hipStream_t st);
+hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, bool disc, size_t lds,
+                                hipStream_t st);
 hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st);
 hipError_t radg_launch_gemm(const GemmProblem *probs_dev, int nprob, int maxM, int maxN,
                             int a_fp32, hipStream_t st);
@@ -119,9 +120,25 @@ struct ParaCIA {
   }
 };
 
+// /E3TABLE/ (e3interp.f:14-51): transmission, E3 and its gradient, 1001
+// points each, for imod 18
+struct E3Tab {
+  double *d_tab = nullptr;  // [3][RADG_E3N]
+  ~E3Tab() {
+    if (d_tab) (void)hipFree(d_tab);
+  }
+};
+
+// The tables beside the k-tables and CIA that some requests need
+struct AuxTables {
+  std::shared_ptr<ParaCIA> para;
+  std::shared_ptr<E3Tab> e3;
+};
+
 std::shared_ptr<TableSet> g_tables;
 std::shared_ptr<CIA> g_cia;
 std::shared_ptr<ParaCIA> g_para;
+std::shared_ptr<E3Tab> g_e3;
 
 // /DUST/ for the fixed-argument entry points (cirsradg_wave.f:225,312)
 struct Dust {
@@ -202,8 +219,9 @@ struct Packed {
 
 bool finite_pos(double x) { return std::isfinite(x) && x > 0; }
 
-int pack_request(const radg_request &q, const TableSet &ts, int mode, const ParaCIA *para,
+int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxTables &aux,
                  Packed &P) {
+  const ParaCIA *para = aux.para.get();
   const int L = q.nlayer, ngas = q.ngas, npath = q.npath, nwave = q.nwave;
   if (L < 1 || L > RADG_MAXLAY) return fail(RADG_ELIMITS, "nlayer=%d outside 1..%d", L, RADG_MAXLAY);
   if (npath < 1 || npath > RADG_MAXPAT) return fail(RADG_ELIMITS, "npath=%d outside 1..%d", npath, RADG_MAXPAT);
@@ -254,14 +272,20 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const Para
   for (int p = 0; p < npath; p++) {
     const int nl = q.nlayin[p];
     if (nl < 1 || nl > L) return fail(RADG_ELIMITS, "nlayin(%d)=%d outside 1..nlayer", p + 1, nl);
-    if (q.imod[p] != 0 && q.imod[p] != 1 && q.imod[p] != 3)
-      return fail(RADG_EUNSUPPORTED, "imod(%d)=%d: only 0, 1 and 3 are provided", p + 1, q.imod[p]);
+    if (q.imod[p] != 0 && q.imod[p] != 1 && q.imod[p] != 3 && q.imod[p] != 18)
+      return fail(RADG_EUNSUPPORTED, "imod(%d)=%d: only 0, 1, 3 and 18 are provided", p + 1, q.imod[p]);
+    // imod 18 interpolates E3 (e3interp.f) and always adds the surface at
+    // tsurf (cirsradg_wave.f:1159,1213)
+    if (q.imod[p] == 18 && !aux.e3)
+      return fail(RADG_ENOTABLES, "imod(%d)=18 needs the E3 table (radg_register_e3)", p + 1);
+    if (q.imod[p] == 18 && !finite_pos(q.tsurf))
+      return fail(RADG_EUNSUPPORTED, "imod(%d)=18 needs tsurf > 0 (tsurf=%g)", p + 1, q.tsurf);
     for (int m = 0; m < nl; m++) {
       const int li = q.layinc[(long long)p * L + m];
       if (li < 1 || li > L) return fail(RADG_ELIMITS, "layinc(%d,%d)=%d exceeds nlayer", m + 1, p + 1, li);
       if (!std::isfinite(q.scale[(long long)p * L + m]))
         return fail(RADG_EINPUT, "scale(%d,%d) not finite", m + 1, p + 1);
-      if (q.imod[p] == 3 && !finite_pos(q.emtemp[(long long)p * L + m]))
+      if ((q.imod[p] == 3 || q.imod[p] == 18) && !finite_pos(q.emtemp[(long long)p * L + m]))
         return fail(RADG_EINPUT, "emtemp(%d,%d)=%g", m + 1, p + 1, q.emtemp[(long long)p * L + m]);
     }
     maxlays = std::max(maxlays, nl);
@@ -285,6 +309,18 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const Para
   }
   if (q.iform == 3) d.xfac = d.xfac * 1e-18;
   if (!std::isfinite(d.xfac)) return fail(RADG_EINPUT, "radius1=%g radextra=%g", q.radius1, q.radextra);
+  // imod 18 (cirsradg_wave.f:1134-1149): 4 pi r^2 without imod 3's further
+  // pi, and the 2 pi that every imod 18 term carries
+  {
+    const double pi = 3.1415927;
+    double x = 1.0;
+    if (q.iform == 1 || q.iform == 3) {
+      const double rr = (q.radius1 + q.radextra) * 1e5;
+      x = x * 4. * pi * (rr * rr);
+    }
+    if (q.iform == 3) x = x * 1e-18;
+    d.xfac18 = x * 2.0 * pi;
+  }
   d.nsw = q.nsw;
   d.nwave = nwave;
   d.ispace = q.ispace;
@@ -402,7 +438,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const Para
     if (j1 < 1) j1 = 1;
     const double pa = q.press[q.layinc[(long long)p * L + j1 - 1] - 1];
     const double pb = q.press[q.layinc[(long long)p * L + nl - 1] - 1];
-    pi.surf = pb > pa;  // cirsradg_wave.f:992-999
+    pi.surf = pb > pa || pi.imod == 18;  // cirsradg_wave.f:992-999; imod 18 always
     for (int m = 0; m < nl; m++) {
       P.layinc[(size_t)p * L + m] = q.layinc[(long long)p * L + m] - 1;
       P.scale[(size_t)p * L + m] = q.scale[(long long)p * L + m];
@@ -527,7 +563,7 @@ struct radg_batch {
   int mode = 0, nreq = 0;
   std::shared_ptr<TableSet> ts;
   std::shared_ptr<CIA> cia;
-  std::shared_ptr<ParaCIA> para;
+  AuxTables aux;
   std::vector<DevReq> reqs;
   char *dmem = nullptr;
   size_t bytes = 0;
@@ -539,6 +575,7 @@ struct radg_batch {
   // other requests of its batch)
   struct EmClass {
     int RL, nreq;
+    bool disc;  // a member has an imod 18 path
     long long nB;
     const int *reqmap;
     const long long *start;
@@ -721,14 +758,14 @@ struct Plan {
 };
 
 int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
-               const ParaCIA *para, Plan &pl) {
+               const AuxTables &aux, Plan &pl) {
   if (nreq < 1) return fail(RADG_EINPUT, "nreq=%d", nreq);
   pl.packs.resize(nreq);
   pl.sA.assign(nreq + 1, 0);
   pl.sB.assign(nreq + 1, 0);
   pl.sC.assign(nreq + 1, 0);
   for (int r = 0; r < nreq; r++) {
-    int rc = pack_request(reqs[r], ts, mode, para, pl.packs[r]);
+    int rc = pack_request(reqs[r], ts, mode, aux, pl.packs[r]);
     if (rc) return rc;
     DevReq &d = pl.packs[r].d;
     const radg_request &q = reqs[r];
@@ -990,6 +1027,37 @@ int radg_register_parah2(double dnu, const double *temps, int npara, const doubl
   return RADG_OK;
 }
 
+int radg_register_e3(int n, const double *trans, const double *e3) {
+  if (n == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_e3.reset();
+    return RADG_OK;
+  }
+  if (n != RADG_E3N) return fail(RADG_ELIMITS, "E3 table: n=%d, e3interp.f holds %d points", n, RADG_E3N);
+  if (!trans || !e3) return fail(RADG_EINPUT, "E3 table: null array");
+  for (int i = 0; i < n; i++)
+    if (!std::isfinite(trans[i]) || !std::isfinite(e3[i]))
+      return fail(RADG_EINPUT, "E3 table: entry %d not finite", i + 1);
+  // e3interp.f:25-51: the table's spacing is 1/(mtab-1) and the gradient its
+  // one-sided differences at the ends, central differences inside
+  std::vector<double> h((size_t)3 * n);
+  const double dtrans = 1.0 / (double)(n - 1);
+  for (int i = 0; i < n; i++) {
+    h[i] = trans[i];
+    h[n + i] = e3[i];
+  }
+  double *g = h.data() + 2 * n;
+  g[0] = (e3[1] - e3[0]) / dtrans;
+  g[n - 1] = (e3[n - 1] - e3[n - 2]) / dtrans;
+  for (int i = 1; i < n - 1; i++) g[i] = 0.5 * (e3[i + 1] - e3[i - 1]) / dtrans;
+  auto c = std::make_shared<E3Tab>();
+  HIPCHK(hipMalloc(&c->d_tab, h.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_tab, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_e3 = c;
+  return RADG_OK;
+}
+
 int radg_set_iform(int iform, int iread) {
   if (iread == 999 && (iform == 1 || iform == 4))
     return fail(RADG_EUNSUPPORTED, "iform=%d with a solar spectrum (iread=999) is not provided", iform);
@@ -1030,15 +1098,15 @@ int radg_register_dust(int nsec, int ncont, const double *vsec, const double *xs
 
 int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes) {
   std::shared_ptr<TableSet> ts;
-  std::shared_ptr<ParaCIA> para;
+  AuxTables aux;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
-    para = g_para;
+    aux = {g_para, g_e3};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
   Plan pl;
-  int rc = plan_batch(nreq, reqs, mode, *ts, para.get(), pl);
+  int rc = plan_batch(nreq, reqs, mode, *ts, aux, pl);
   if (rc) return rc;
   *bytes = pl.total;
   return RADG_OK;
@@ -1049,17 +1117,17 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   std::shared_ptr<TableSet> ts;
   std::shared_ptr<CIA> cia;
-  std::shared_ptr<ParaCIA> para;
+  AuxTables aux;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
     cia = g_cia;
-    para = g_para;
+    aux = {g_para, g_e3};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
   Plan pl;
-  int rc = plan_batch(nreq, reqs, mode, *ts, para.get(), pl);
+  int rc = plan_batch(nreq, reqs, mode, *ts, aux, pl);
   if (rc) return rc;
   rc = ensure_form(*ts, mode);
   if (rc) return rc;
@@ -1070,7 +1138,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   b->nreq = nreq;
   b->ts = ts;
   b->cia = cia;
-  b->para = para;
+  b->aux = aux;
   b->bytes = pl.total;
   b->dmem = static_cast<char *>(dev_cache().get(pl.total));
   if (!b->dmem) return fail(RADG_EHIP, "device allocation of %zu bytes failed", pl.total);
@@ -1185,6 +1253,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
   a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
   a.cont = reinterpret_cast<const double *>(m + pl.o_ct);
+  a.e3tab = aux.e3 ? aux.e3->d_tab : nullptr;
+  const ParaCIA *para = aux.para.get();
   if (para) {
     a.pkh2h2 = para->d_kh2h2;
     a.pkh2he = para->d_kh2he;
@@ -1223,9 +1293,11 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
       ec.nreq = (int)pl.cls_members[c].size();
       ec.nB = 0;
       int maxnsw = 1;
+      ec.disc = false;
       for (int r : pl.cls_members[c]) {
         ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
         maxnsw = std::max(maxnsw, reqs[r].nsw);
+        for (int p = 0; p < reqs[r].npath; p++) ec.disc = ec.disc || reqs[r].imod[p] == 18;
       }
       ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
       ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
@@ -1269,7 +1341,7 @@ int radg_batch_launch(radg_batch *b, void *stream) {
     a.nB = ec.nB;
     a.reqmapB = ec.reqmap;
     a.nreqB = ec.nreq;
-    HIPCHK(radg_launch_emission(a, fp64, ec.RL, ec.lds, st));
+    HIPCHK(radg_launch_emission(a, fp64, ec.RL, ec.disc, ec.lds, st));
   }
   HIPCHK(hipEventRecord(b->ev[2], st));
   HIPCHK(radg_launch_quadrature(b->args, fp64, st));

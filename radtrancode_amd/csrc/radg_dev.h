@@ -9,6 +9,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/radg.h"
+
 #define RADG_WAVE 64
 #define RADG_MAXPARAM 41  // ngas + 1 + ncont with ngas <= 30, ncont <= 10 (no para-H2)
 
@@ -24,6 +26,7 @@ struct DevReq {
   double tsurf;
   double nlte_k1, nlte_a;      // calc_nlte_t.f constants
   double xfac;                 // imod 3 output scale (iform 1, 3: total flux)
+  double xfac18;               // imod 18 output scale, its 2 pi included
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
   long long lay0;              // LayerConst / per-layer arrays
@@ -118,6 +121,8 @@ struct KernelArgs {
   const double *pkh2h2, *pkh2he, *pkcia5, *ptemps, *pfp;
   double pdnu;
   int npara;
+  // /E3TABLE/ (imod 18): transmission, E3, gradient [3][RADG_E3N]
+  const double *e3tab;
   // outputs
   void *inter;                 // intermediates, compute type
   void *out, *gts, *jac;       // compute type; jac gas columns per 1e20 molecules

@@ -1298,6 +1298,25 @@ __device__ __forceinline__ double nlte_t_d(int inlte, double k1, double a, doubl
   return (c2 * 1.0e4 / v) / log(c1 * 1.0e16 / (b * (v * v * v * v * v)) + 1.0);
 }
 
+// e3interp.f:57-77 on the device table [trans | E3 | gradient], RADG_E3N
+// points each: E3 at transmission x and dE3/dtr, both linear in the
+// interval.  A negative or NaN x gives zero; x >= 1 uses the last interval.
+__device__ __forceinline__ void e3interp_d(const double *tab, double x, double &e3, double &de3) {
+  if (!(x >= 0.0)) {
+    e3 = 0.0;
+    de3 = 0.0;
+    return;
+  }
+  const double dtrans = 1.0 / (double)(RADG_E3N - 1);
+  int j = 1 + (int)(fmin(x, 2.0) / dtrans);
+  if (j >= RADG_E3N) j = RADG_E3N - 1;
+  double f = (x - tab[j - 1]) / dtrans;
+  if (f > 1.0) f = 1.0;
+  const double *te = tab + RADG_E3N, *tg = tab + 2 * RADG_E3N;
+  e3 = (1.0 - f) * te[j - 1] + f * te[j];
+  de3 = (1.0 - f) * tg[j - 1] + f * tg[j];
+}
+
 template <typename T>
 struct Kahan {
   T s, c;
@@ -1317,8 +1336,10 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 }  // namespace
 
-// LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.
-template <typename T, int RL>
+// LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.  DISC: the form with the
+// imod 18 branch, launched only for batches that hold an imod 18 path (its
+// registers would cost the other forms occupancy).
+template <typename T, int RL, bool DISC>
 __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
   if (item >= a.nB) return;
@@ -1350,7 +1371,11 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   bool act[RL];
   T sc[RL], tc[RL], db[RL];
   double bb[RL];  // Planck source of each layer, FP64
-  const double xfac = 1.0;
+  // xfac (cirsradg_wave.f:912-925 for imod 3, :1134-1149 with its 2 pi for
+  // imod 18) scales each g-ordinate's terms in FP64 before they are rounded
+  // to T, as the reference applies it per term: with iform 1 it reaches 1e21
+  // and an unscaled FP32 term would fall below FP32's range.
+  const double xfac = pi.imod == 3 ? R.xfac : (DISC && pi.imod == 18) ? R.xfac18 : 1.0;
 #pragma unroll
   for (int q = 0; q < RL; q++) {
     const int m = lane * RL + q;
@@ -1360,7 +1385,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     tc[q] = act[q] ? tcon[Lm[q]] : (T)0;
     bb[q] = 0;
     db[q] = 0;
-    if (act[q] && pi.imod == 3) {
+    if (act[q] && (pi.imod == 3 || (DISC && pi.imod == 18))) {
       double em = a.emtemp[R.pl0 + (long long)p * L + m];
       if (R.inlte) em = nlte_t_d(R.inlte, R.nlte_k1, R.nlte_a, em, a.press[R.lay0 + Lm[q]], v);
       bb[q] = planck_d(R.ispace, v, em);
@@ -1371,9 +1396,10 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       cmp[(lane * RL + q) * nsw + k1] = 0;
     }
   }
-  // ground term (cirsradg_wave.f:992-1024)
+  // ground term (cirsradg_wave.f:992-1024; imod 18 :1159,1212-1218, where
+  // tsurf > 0 always)
   double radground = 0.0, gradground = 0.0;
-  if (pi.imod == 3 && pi.surf) {
+  if ((pi.imod == 3 || (DISC && pi.imod == 18)) && pi.surf) {
     if (R.tsurf_le0) {
       const int ob = (nlays - 1) / RL, oq = (nlays - 1) - ob * RL;
       double b0 = 0.0, g0 = 0.0;
@@ -1497,19 +1523,96 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
         dRdtau[q] = act[q] ? (T)(xfac * trx[q] * Dm[q]) : (T)0;
         dT[q] = act[q] ? (T)(xfac * trp[q] * (double)om[q]) * db[q] : (T)0;
       }
+    } else if (DISC && pi.imod == 18) {
+      // disc-integrated emission (cirsradg_wave.f:1116-1218): layer j
+      // weighs B_j by E3(tr_{j-1}) - E3(tr_j), the surface by E3(tr_N);
+      // xfac carries the 2 pi.  With g_j = dE3/dtr at tr_j
+      // and B_N = 0,
+      //   dR/dtau_m = sum_{j>=m} (B_j - B_{j+1}) g_j tr_j,
+      // the reference's O(L^2) de3dq recurrence (:1171-1186) summed in
+      // closed form; a suffix sum over lanes, FP64.
+      // the lane's layer transmissions are formed twice (here for the lane
+      // product, below sequentially from the lane's entry value, as the
+      // reference's tr = trold*tlayer) so no per-layer array stays live
+      double run = 1.0;
+#pragma unroll
+      for (int q = 0; q < RL; q++) run *= act[q] ? (double)dexp(-tau[q]) : 1.0;
+      double s = run;
+#pragma unroll
+      for (int ds = 0; ds < 6; ds++) {
+        const int d = 1 << ds;
+        const double o = __shfl_up(s, d);
+        if (lane >= d) s *= o;
+      }
+      double ex = __shfl_up(s, 1);
+      if (lane == 0) ex = 1.0;
+      double e3x[RL], cq[RL];
+      double e3up = 0.5, part = 0.0, tot = 0.0;
+      {
+        const double bnext0 = __shfl_down(bb[0], 1);  // the layer below this lane's last
+        double tr = ex;
+#pragma unroll
+        for (int q = 0; q < RL; q++) {
+          const int m = lane * RL + q;
+          tr *= act[q] ? (double)dexp(-tau[q]) : 1.0;
+          double gq;
+          e3interp_d(a.e3tab, tr, e3x[q], gq);
+          const double bn = (q == RL - 1) ? bnext0 : bb[q + 1];
+          cq[q] = act[q] ? (bb[q] - (m + 1 < nlays ? bn : 0.0)) * gq * tr : 0.0;
+          tot += cq[q];
+        }
+        const double up = __shfl_up(e3x[RL - 1], 1);
+        if (lane > 0) e3up = up;
+#pragma unroll
+        for (int q = 0; q < RL; q++) {
+          const double e3p = q == 0 ? e3up : e3x[q - 1];
+          if (act[q]) part += (e3p - e3x[q]) * bb[q];
+          dT[q] = act[q] ? (T)(xfac * (e3p - e3x[q]) * (double)db[q]) : (T)0;
+        }
+      }
+      // E3 at the bottom of the path: the last active layer's
+      double e3N;
+      {
+        const int ob = (nlays - 1) / RL, oq = (nlays - 1) - ob * RL;
+        double v0 = 0.0;
+#pragma unroll
+        for (int q = 0; q < RL; q++)
+          if (q == oq) v0 = e3x[q];
+        e3N = __shfl(v0, ob);
+      }
+      coR = (T)(xfac * (wave_sum(part) + e3N * radground));
+      coG = (T)(xfac * e3N * gradground);
+      double ia = tot;
+#pragma unroll
+      for (int ds = 0; ds < 6; ds++) {
+        const int d = 1 << ds;
+        const double o = __shfl_down(ia, d);
+        if (lane + d < W) ia += o;
+      }
+      double sv = __shfl_down(ia, 1);
+      if (lane == W - 1) sv = 0.0;
+#pragma unroll
+      for (int q = RL - 1; q >= 0; q--) {
+        sv += cq[q];
+        dRdtau[q] = act[q] ? (T)(xfac * sv) : (T)0;
+      }
     } else {
+      // the reference's formulas (:863-895) with e^-tau in T and the rest in
+      // FP64: imod 1's 1 - (1 - e^-tau) would cancel in FP32 beyond tau ~ 10
       double part = 0.0;
 #pragma unroll
       for (int q = 0; q < RL; q++) part += (double)tau[q];
-      const T tmp = (T)wave_sum(part);
+      const double tmp = wave_sum(part);
+      const double e = (double)dpexp((T)-tmp);
       if (pi.imod == 0) {
-        coR = dpexp(-tmp);
+        coR = (T)e;
 #pragma unroll
-        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? -tmp * coR : (T)0;
+        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? (T)(-tmp * e) : (T)0;
       } else {
-        coR = (T)1 - dpexp(-tmp);
+        const double co = 1.0 - e;
+        coR = (T)co;
 #pragma unroll
-        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? tmp * ((T)1 - coR) : (T)0;
+        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? (T)(tmp * (1.0 - co)) : (T)0;
       }
 #pragma unroll
       for (int q = 0; q < RL; q++) dT[q] = 0;
@@ -1543,13 +1646,10 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     }
   }
   // ---- outputs ----
-  // xfac (cirsradg_wave.f:912-925) scales every imod 3 term: the planet's
-  // total flux for iform 1 and 3; 1 otherwise
-  const double xf = pi.imod == 3 ? R.xfac : 1.0;
   const long long ob = R.out0 + (long long)iw * npath + p;
   if (lane == 0) {
-    reinterpret_cast<T *>(a.out)[ob] = (T)((double)rad.s * xf);
-    reinterpret_cast<T *>(a.gts)[ob] = (T)((double)gts.s * xf);
+    reinterpret_cast<T *>(a.out)[ob] = rad.s;
+    reinterpret_cast<T *>(a.gts)[ob] = gts.s;
   }
   T *jac = reinterpret_cast<T *>(a.jac) + R.jac0 + ((long long)iw * npath + p) * L * nparam;
   // each lane writes whole rows: zeros, then its isw parameters
@@ -1565,7 +1665,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     if (!act[q]) continue;
     for (int k1 = 0; k1 < nsw; k1++) {
       const int k = R.isw[k1];
-      jac[(long long)m * nparam + k] = (T)((double)acc[m * nsw + k1] * xf);
+      jac[(long long)m * nparam + k] = acc[m * nsw + k1];
     }
   }
 }
@@ -1613,10 +1713,13 @@ hipError_t launch_layer_E(const KernelArgs &a, size_t lds, hipStream_t st) {
   return hipGetLastError();
 }
 template <typename T, int RL>
-hipError_t launch_em_RL(const KernelArgs &a, size_t lds, hipStream_t st) {
+hipError_t launch_em_RL(const KernelArgs &a, bool disc, size_t lds, hipStream_t st) {
   long long nb = ((a.nB + 7) / 8) * 8;
   if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL((emission_kernel<T, RL>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
+  if (disc)
+    hipLaunchKernelGGL((emission_kernel<T, RL, true>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
+  else
+    hipLaunchKernelGGL((emission_kernel<T, RL, false>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
   return hipGetLastError();
 }
 }  // namespace
@@ -1687,20 +1790,20 @@ hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t ld
   return hipErrorInvalidValue;
 }
 
-hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, size_t lds, hipStream_t st) {
+hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, bool disc, size_t lds, hipStream_t st) {
   if (fp64) {
     switch (RL) {
-      case 1: return launch_em_RL<double, 1>(a, lds, st);
-      case 2: return launch_em_RL<double, 2>(a, lds, st);
-      case 4: return launch_em_RL<double, 4>(a, lds, st);
-      case 8: return launch_em_RL<double, 8>(a, lds, st);
+      case 1: return launch_em_RL<double, 1>(a, disc, lds, st);
+      case 2: return launch_em_RL<double, 2>(a, disc, lds, st);
+      case 4: return launch_em_RL<double, 4>(a, disc, lds, st);
+      case 8: return launch_em_RL<double, 8>(a, disc, lds, st);
     }
   } else {
     switch (RL) {
-      case 1: return launch_em_RL<float, 1>(a, lds, st);
-      case 2: return launch_em_RL<float, 2>(a, lds, st);
-      case 4: return launch_em_RL<float, 4>(a, lds, st);
-      case 8: return launch_em_RL<float, 8>(a, lds, st);
+      case 1: return launch_em_RL<float, 1>(a, disc, lds, st);
+      case 2: return launch_em_RL<float, 2>(a, disc, lds, st);
+      case 4: return launch_em_RL<float, 4>(a, disc, lds, st);
+      case 8: return launch_em_RL<float, 8>(a, disc, lds, st);
     }
   }
   return hipErrorInvalidValue;

@@ -17,8 +17,9 @@ program radg_host
   integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
   integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont, inlte, iform
   real(c_double) :: nlte_k1, nlte_a, pdnu, ptemps(25)
-  integer(c_int) :: npara
+  integer(c_int) :: npara, ne3
   real(c_double), allocatable :: pfp(:), pkhh(:), pkhe(:), pk5(:)
+  real(c_double) :: e3t(1001), e3v(1001)
   real(c_double), allocatable :: vsec(:), xsec(:)
   real(c_double) :: dnu, ctemps(25)
   real(c_double), allocatable :: kcia(:, :, :)
@@ -61,6 +62,9 @@ program radg_host
     allocate (pfp(npara), pkhh(npara * 25 * 1501), pkhe(npara * 25 * 1501), pk5(5 * 25 * 1501))
     read (10) pdnu, ptemps, pfp, pkhh, pkhe, pk5
   end if
+  ! /E3TABLE/: ne3 (0 = none or 1001), then trans(1001), e3(1001)
+  read (10) ne3
+  if (ne3 > 0) read (10) e3t, e3v
   close (10)
 
   if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
@@ -71,6 +75,9 @@ program radg_host
   if (radg_set_iform(iform, 0) /= RADG_OK) call die()
   if (npara > 0) then
     if (radg_register_parah2(pdnu, ptemps, npara, pfp, pkhh, pkhe, pk5) /= RADG_OK) call die()
+  end if
+  if (ne3 > 0) then
+    if (radg_register_e3(ne3, e3t, e3v) /= RADG_OK) call die()
   end if
 
   nparam = ngas + 1 + ncont

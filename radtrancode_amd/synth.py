@@ -87,6 +87,26 @@ class CIATable:
     temps: np.ndarray      # [25]
     kcia: np.ndarray       # [1501, 25, 9] float64
     para: Optional["ParaCIATable"] = None
+    e3: Optional["E3Table"] = None
+
+
+@dataclass
+class E3Table:
+    """``/E3TABLE/`` of e3interp.f: the exponential integral E3 against
+    transmission on 1001 points (imod 18)."""
+    trans: np.ndarray      # [1001]
+    e3: np.ndarray         # [1001]
+
+
+def make_e3_table() -> E3Table:
+    """E3(tau) at tau = -ln(tr), tr = i/1000 (the grid e3interp.f assumes),
+    from scipy's exponential integral; E3 = 0 at tr = 0 and 1/2 at tr = 1."""
+    from scipy.special import expn
+    tr = np.arange(1001) / 1000.0
+    e3 = np.zeros(1001)
+    e3[1:] = expn(3, -np.log(tr[1:]))
+    e3[-1] = 0.5
+    return E3Table(trans=tr, e3=e3)
 
 
 @dataclass

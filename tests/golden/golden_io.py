@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from radtrancode_amd.synth import CIATable, KTable, ParaCIATable, Request
+from radtrancode_amd.synth import CIATable, E3Table, KTable, ParaCIATable, Request
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -26,10 +26,14 @@ _KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
 _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 
 
-def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = None):
+def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = None,
+              e3: E3Table = None):
     """``cia``: a CIA table of the case's own, stored with it (else the
-    shared ``cia.npz`` applies)."""
+    shared ``cia.npz`` applies); ``e3``: the E3 table an imod 18 case ran
+    with."""
     d = {"note": np.array(note)}
+    if e3 is not None:
+        d.update(e3_trans=e3.trans, e3_val=e3.e3)
     if cia is not None:
         d.update(cia_dnu=np.array(cia.dnu), cia_temps=cia.temps, cia_kcia=cia.kcia)
         if cia.para is not None:
@@ -83,7 +87,7 @@ def save_cia(path, cia: CIATable):
 
 def case_cia(path) -> CIATable:
     """The CIA table a case runs with: its own if it holds one, else the
-    shared one."""
+    shared one, carrying the case's E3 table if it has one."""
     z = np.load(path, allow_pickle=False)
     if "cia_kcia" in z.files:
         para = None
@@ -91,9 +95,13 @@ def case_cia(path) -> CIATable:
             para = ParaCIATable(dnu=float(z["para_dnu"]), temps=z["para_temps"], fp=z["para_fp"],
                                 kh2h2=z["para_kh2h2"], kh2he=z["para_kh2he"],
                                 kcia5=z["para_kcia5"])
-        return CIATable(dnu=float(z["cia_dnu"]), temps=z["cia_temps"], kcia=z["cia_kcia"],
-                        para=para)
-    return load_cia()
+        cia = CIATable(dnu=float(z["cia_dnu"]), temps=z["cia_temps"], kcia=z["cia_kcia"],
+                       para=para)
+    else:
+        cia = load_cia()
+    if "e3_val" in z.files:
+        cia.e3 = E3Table(trans=z["e3_trans"], e3=z["e3_val"])
+    return cia
 
 
 def load_cia(path=None) -> CIATable:

@@ -194,6 +194,37 @@ def cases():
                                         + (", CH4 and NH3 present" if iray == 4 else "")
                                         + (", CIA overlapping" if v0 < 1500 else ", no CIA"))
 
+    # disc-integrated emission (imod 18, cirsradg_wave.f:1116-1211): the
+    # layer weights are differences of 2 pi E3 of the transmission, the
+    # surface term 2 pi E3(tr) B(tsurf) is always on.  Amounts are scaled so
+    # the column is semi-transparent and the surface shows; iform 1 or 3
+    # multiply by the disc area 4 pi ((radius1 + radextra) 1e5)^2 (times
+    # 1e-18 for 3)
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.amount, r.pp = r.amount * 0.05, r.pp * 0.05
+    r.tsurf = 170.0
+    r.imod = np.array([18, 0, 18, 3, 18], dtype=np.int32)
+    r.nlayin[2] = 20
+    yield "disc", t, r, "imod 18 paths (one stopping at layer 20) beside imod 0 and 3, tsurf 170 K"
+
+    t, r = wavelength_inputs(46, 4.0, 0.02)
+    r.amount, r.pp = r.amount * 0.05, r.pp * 0.05
+    r.tsurf = 180.0
+    r.imod = np.full(r.npath, 18, dtype=np.int32)
+    r.iform, r.radius1, r.radextra = 3, 71492.0, 90.0
+    yield "disc_iform3", t, r, "imod 18 with iform = 3 in wavelength space (total flux)"
+
+    t, c, r = synth.config1(nwave=5, nlayer=24, nkgas=2, ng=8)
+    r.amount, r.pp = r.amount * 0.05, r.pp * 0.05
+    r.tsurf = 150.0
+    r.vem = np.array([500.0, 2000.0])
+    r.emissivity = np.array([0.85, 0.9])
+    r.imod = np.array([18, 18, 3, 18, 1], dtype=np.int32)
+    r.inlte, r.nlte_k1, r.nlte_a = 1, 1e-11, 1e5
+    r = add_dust(r, 1, 10, seed=47)
+    r.iform, r.radius1 = 1, 6052.0
+    yield "disc_nlte_dust", t, r, "imod 18 with non-LTE, one dust type, emissivity < 1, iform = 1"
+
 
 def add_dust(r, ncont, nsec, seed, peak_tau=(1.5, 0.4)):
     """Dust on a request (cirsradg_wave.f:550-577): particle type 0 is a
@@ -305,6 +336,8 @@ CASE_CIA = {"titan_n2": dict(seed=5, pairs=(0, 1, 4, 5, 6, 7, 8)),
             "parah2": dict(seed=7, pairs=(0, 1)), "parah2_dust": dict(seed=8, pairs=(0, 1))}
 # cases whose CIA table also carries a para-H2 table (FLAGH2P = 1)
 CASE_PARA = {"parah2": dict(seed=9), "parah2_dust": dict(seed=10, npara=12)}
+# cases with imod 18 paths, which run with the E3 table
+CASE_E3 = ("disc", "disc_iform3", "disc_nlte_dust")
 
 
 def main(names=()):
@@ -324,9 +357,15 @@ def main(names=()):
             if name in CASE_PARA:
                 own.para = synth.make_para_cia(**CASE_PARA[name])
             own = banded_cia(own, r.vwave, half=1 if name in CASE_PARA else 3)
-        outs = run_ref(t, own or cia, r)
+        use = own or cia
+        e3 = None
+        if name in CASE_E3:
+            e3 = synth.make_e3_table()
+            use = copy.copy(use)
+            use.e3 = e3
+        outs = run_ref(t, use, r)
         path = os.path.join(HERE, f"case_{name}.npz")
-        save_case(path, t, r, outs, note, cia=own)
+        save_case(path, t, r, outs, note, cia=own, e3=e3)
         print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
               f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
 

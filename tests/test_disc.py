@@ -1,0 +1,152 @@
+"""Disc-integrated emission (imod 18, cirsradg_wave.f:1116-1218) in the oracle.
+
+Pinned by the reference's own cirsradg_wave on the disc, disc_iform3 and
+disc_nlte_dust golden cases (test_oracle.py runs every case_*.npz), and here
+by:
+* the E3 table against the exponential integral's own derivative
+  dE3(-ln tr)/dtr = E2(-ln tr)/tr, and e3interp.f's edge rules;
+* an isothermal column over a black surface at the same temperature, where
+  the E3 differences telescope and R = pi B(T) whatever the opacity; a
+  transparent column, where R = pi esurf B(tsurf);
+* the O(L) suffix-sum Jacobian against the O(L^2) recurrence the reference
+  runs;
+* central differences of the whole model in amounts and temperature.
+"""
+import copy
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+from scipy.special import expn
+
+from oracle import oracle
+from radtrancode_amd import synth
+from tests.golden.golden_io import case_cia, load_case
+from tests.tolerance import jac_err, rad_err
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+E3 = synth.make_e3_table()
+
+
+def _e3(x):
+    f = oracle.lib().orc_e3
+    f.restype = C.c_double
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)]
+    t = np.ascontiguousarray(E3.trans)
+    v = np.ascontiguousarray(E3.e3)
+    g = C.c_double()
+    e = f(t.ctypes.data, v.ctypes.data, float(x), C.byref(g))
+    return e, g.value
+
+
+def test_e3_table_and_gradient():
+    assert E3.e3[0] == 0.0 and E3.e3[-1] == 0.5
+    for x in (0.05, 0.2371, 0.5, 0.8125, 0.97):
+        e, g = _e3(x)
+        tau = -np.log(x)
+        assert e == pytest.approx(expn(3, tau), rel=2e-6)
+        # the table's central differences against E2(tau)/tr
+        assert g == pytest.approx(expn(2, tau) / x, rel=2e-3)
+    # a negative or NaN transmission gives zero (e3interp.f:57-66); one
+    # past the table uses its last interval
+    assert _e3(-1e-3) == (0.0, 0.0)
+    assert _e3(float("nan")) == (0.0, 0.0)
+    assert _e3(1.0)[0] == pytest.approx(0.5, rel=1e-15)
+
+
+def _disc_request(nwave=4, nlayer=16, nkgas=2, ng=8, scale=0.05):
+    t, c, r = synth.config1(nwave=nwave, nlayer=nlayer, nkgas=nkgas, ng=ng)
+    c.e3 = E3
+    r.amount, r.pp = r.amount * scale, r.pp * scale
+    r.tsurf = 160.0
+    r.imod = np.full(r.npath, 18, dtype=np.int32)
+    return t, c, r
+
+
+def test_isothermal_black_surface_gives_pi_b():
+    t, c, r = _disc_request()
+    T = 140.0
+    r.temp = np.full_like(r.temp, T)
+    r.emtemp = np.full_like(r.emtemp, T)
+    r.tsurf = T
+    o, g, d = oracle.spectrum(t, c, r)
+    b = np.array([oracle.planck(0, v, T) for v in r.vwave])
+    assert np.allclose(o, np.pi * b[:, None], rtol=1e-12)
+    # and the column is not transparent: the surface carries part of it
+    assert np.all(g < 0.9 * np.pi * np.array([oracle.planckg(0, v, T) for v in r.vwave])[:, None])
+
+
+def test_transparent_column_shows_the_surface():
+    t, c, r = _disc_request(scale=0.0)
+    c = synth.CIATable(dnu=c.dnu, temps=c.temps, kcia=np.zeros_like(c.kcia), e3=E3)
+    r.vem = np.array([500.0, 2000.0])
+    r.emissivity = np.array([0.8, 0.8])
+    o, g, d = oracle.spectrum(t, c, r)
+    bs = np.array([oracle.planck(0, v, r.tsurf) for v in r.vwave])
+    gs = np.array([oracle.planckg(0, v, r.tsurf) for v in r.vwave])
+    assert np.allclose(o, np.pi * 0.8 * bs[:, None], rtol=1e-12)
+    assert np.allclose(g, np.pi * 0.8 * gs[:, None], rtol=1e-12)
+
+
+@pytest.mark.parametrize("tag", ["disc", "disc_iform3", "disc_nlte_dust"])
+def test_linear_and_quadratic_recurrences_agree(tag):
+    path = os.path.join(HERE, f"case_{tag}.npz")
+    tables, req, _, _ = load_case(path)
+    cia = case_cia(path)
+    o0, g0, d0 = oracle.spectrum(tables, cia, req, jac_mode=0)
+    o1, g1, d1 = oracle.spectrum(tables, cia, req, jac_mode=1)
+    assert rad_err(o1, o0) < 1e-14
+    assert rad_err(g1, g0) < 1e-14
+    assert jac_err(d1, d0) < 1e-11
+
+
+def test_disc_jacobian_matches_finite_differences():
+    """The reference differentiates E3 through its tabulated gradient
+    (e3interp.f:47-51, linearly interpolated), the model itself through the
+    table's linear pieces; the two agree to the table's resolution."""
+    t, c, r = _disc_request(nwave=3, nlayer=12, nkgas=3, ng=8)
+    o, g, d = oracle.spectrum(t, c, r)
+    ip = 1
+    for j in (0, 5, 10):
+        lay = r.layinc[ip, j] - 1
+        for k in range(3):
+            h = 1e-3 * r.amount[lay, k]
+            rp, rm = copy.deepcopy(r), copy.deepcopy(r)
+            rp.amount[lay, k] += h
+            rm.amount[lay, k] -= h
+            fd = (oracle.spectrum(t, c, rp)[0] - oracle.spectrum(t, c, rm)[0]) / (2 * h)
+            scale = np.max(np.abs(d[:, ip, :, k]))
+            assert scale > 0
+            assert np.max(np.abs(fd[:, ip] - d[:, ip, j, k])) < 5e-3 * scale
+        h = 1e-3
+        rp, rm = copy.deepcopy(r), copy.deepcopy(r)
+        for rr, s in ((rp, h), (rm, -h)):
+            rr.temp[lay] += s
+            rr.emtemp = rr.temp[rr.layinc - 1]
+        fd = (oracle.spectrum(t, c, rp)[0] - oracle.spectrum(t, c, rm)[0]) / (2 * h)
+        ngas = r.ngas
+        scale = np.max(np.abs(d[:, ip, :, ngas]))
+        assert np.max(np.abs(fd[:, ip] - d[:, ip, j, ngas])) < 5e-3 * scale
+
+
+def test_disc_surface_temperature_gradient():
+    t, c, r = _disc_request()
+    o, g, _ = oracle.spectrum(t, c, r)
+    h = 1e-3
+    rp, rm = copy.copy(r), copy.copy(r)
+    rp.tsurf, rm.tsurf = r.tsurf + h, r.tsurf - h
+    fd = (oracle.spectrum(t, c, rp)[0] - oracle.spectrum(t, c, rm)[0]) / (2 * h)
+    assert np.allclose(fd, g, rtol=1e-6)
+
+
+def test_disc_needs_the_table_and_a_surface():
+    t, c, r = _disc_request()
+    c2 = copy.copy(c)
+    c2.e3 = None
+    with pytest.raises(RuntimeError):
+        oracle.spectrum(t, c2, r)
+    r2 = copy.copy(r)
+    r2.tsurf = 0.0
+    with pytest.raises(RuntimeError):
+        oracle.spectrum(t, c, r2)

@@ -14,7 +14,7 @@ from radtrancode_amd import build as rbuild
 from radtrancode_amd.cirsradg import MAXCON, MAXGAS, MAXLAY, MAXPAT, NPARMAX, fortran_arrays
 from radtrancode_amd.kta import write_kta
 from tests.golden.golden_io import case_cia, case_paths, load_case, load_cia
-from tests.tolerance import fp32_range, fp32_range_jac, jac_err, rad_err
+from tests.tolerance import fp32_range, fp32_range_jac, jac_err, jac_err_cols, rad_err
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +48,10 @@ def _write_input(path, klist, req, cia, mode):
         if pa is not None:
             f.write(b"".join(np.ascontiguousarray(x, np.float64).tobytes()
                              for x in ([pa.dnu], pa.temps, pa.fp, pa.kh2h2, pa.kh2he, pa.kcia5)))
+        e3 = getattr(cia, "e3", None)
+        f.write(i4(0 if e3 is None else len(e3.trans)))
+        if e3 is not None:
+            f.write(np.asarray(e3.trans, np.float64).tobytes() + np.asarray(e3.e3, np.float64).tobytes())
 
 
 def _read_output(path, req):
@@ -71,7 +75,8 @@ def _read_output(path, req):
 
 
 @pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths", "dust", "nlte_k1",
-                                  "iform1", "wavelength_ray_dust", "parah2_dust"])
+                                  "iform1", "wavelength_ray_dust", "parah2_dust",
+                                  "disc_nlte_dust"])
 def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     host = rbuild.FORTRAN_HOST
     assert os.path.exists(host), "Fortran host not built (flang missing at build time?)"
@@ -103,9 +108,15 @@ def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     assert rad_err(o, fp32_range(o_ref)) < 1e-5, note  # REAL(4) outputs
     assert rad_err(g, fp32_range(g_ref)) < 1e-5, note
     assert jac_err(fp32_range_jac(d, req.ngas), fp32_range_jac(d_ref, req.ngas)) < 1e-4, note
-    # and it stays close to what the reference's own REAL(4) build returns
+    # and it stays close to what the reference's own REAL(4) build returns,
+    # in every Jacobian column where that build agrees with its REAL(8) one
+    # (on an opaque imod 0 or 1 path 1 - (1 - e^-500) is 0 in REAL(4), so the
+    # REAL(4) build's column is zero there)
     assert rad_err(o, outs["ref32_output"]) < 1e-5, note
-    assert jac_err(d, outs["ref32_doutputdq"]) < 1e-3, note
+    j32, j64 = outs["ref32_doutputdq"], outs["ref64_doutputdq"]
+    own = jac_err_cols(j32, j64) < 1e-3
+    assert own.mean() > 0.5, note
+    assert np.max(jac_err_cols(d, j32)[own]) < 1e-3, note
 
 
 def _rounded(req):

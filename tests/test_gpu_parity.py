@@ -28,8 +28,9 @@ def _run(rad, tables, req, mode):
 def test_golden_case(gpu_lib, path, mode):
     tables, req, outs, note = load_case(path)
     print(f"case {path.split('/')[-1]} mode {mode}", flush=True)
-    own = "cia_kcia" in np.load(path, allow_pickle=False).files
-    if own:  # a case with its own CIA table (the registry is global)
+    files = np.load(path, allow_pickle=False).files
+    own = "cia_kcia" in files or "e3_val" in files
+    if own:  # a case with its own CIA or E3 table (the registry is global)
         gpu_lib.register_cia(case_cia(path))
     try:
         o, g, d = _run(gpu_lib, tables, req, mode)

@@ -56,11 +56,15 @@ def rad_err(a, b):
     return err
 
 
-def jac_err(a, b, layer_axis=-2):
+def jac_err_cols(a, b, layer_axis=-2):
+    """jac_err per column (every index but the layer axis)."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    scale = np.max(np.abs(b), axis=layer_axis, keepdims=True)
-    diff = np.max(np.abs(a - b), axis=layer_axis, keepdims=True)
+    scale = np.max(np.abs(b), axis=layer_axis)
+    diff = np.max(np.abs(a - b), axis=layer_axis)
     ok0 = scale == 0
-    err = np.where(ok0, np.where(diff > 1e-300, np.inf, 0.0), diff / np.where(ok0, 1.0, scale))
-    return float(np.max(err))
+    return np.where(ok0, np.where(diff > 1e-300, np.inf, 0.0), diff / np.where(ok0, 1.0, scale))
+
+
+def jac_err(a, b, layer_axis=-2):
+    return float(np.max(jac_err_cols(a, b, layer_axis)))

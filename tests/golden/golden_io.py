@@ -104,6 +104,12 @@ def case_cia(path) -> CIATable:
     return cia
 
 
+def load_e3(path=None) -> E3Table:
+    """The reference's E3 table (raddata/e3tab.dat) kept as ``e3tab.npz``."""
+    z = np.load(path or os.path.join(HERE, "e3tab.npz"), allow_pickle=False)
+    return E3Table(trans=z["trans"], e3=z["e3"])
+
+
 def load_cia(path=None) -> CIATable:
     z = np.load(path or os.path.join(HERE, "cia.npz"), allow_pickle=False)
     return CIATable(dnu=float(z["dnu"]), temps=z["temps"], kcia=z["kcia"])

@@ -26,7 +26,20 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from radtrancode_amd import synth  # noqa: E402
 from oracle import ref  # noqa: E402
-from tests.golden.golden_io import save_case, save_cia  # noqa: E402
+from tests.golden.golden_io import load_e3, save_case, save_cia  # noqa: E402
+
+REF_E3TAB = "/root/reference/raddata/e3tab.dat"
+
+
+def save_e3tab(path=os.path.join(HERE, "e3tab.npz")):
+    """The reference's own E3 table (raddata/e3tab.dat, the file e3interp.f:33-44
+    reads: a count, a header line, then tau, transmission, E3 per row) as a
+    fixture, the values as the file holds them."""
+    with open(REF_E3TAB) as f:
+        n = int(f.readline().split()[0])
+        f.readline()
+        rows = np.array([[float(x) for x in f.readline().split()] for _ in range(n)])
+    np.savez_compressed(path, tau=rows[:, 0], trans=rows[:, 1], e3=rows[:, 2])
 
 
 def slice_tables(tables, i0, n):
@@ -194,9 +207,10 @@ def cases():
                                         + (", CH4 and NH3 present" if iray == 4 else "")
                                         + (", CIA overlapping" if v0 < 1500 else ", no CIA"))
 
-    # disc-integrated emission (imod 18, cirsradg_wave.f:1116-1211): the
-    # layer weights are differences of 2 pi E3 of the transmission, the
-    # surface term 2 pi E3(tr) B(tsurf) is always on.  Amounts are scaled so
+    # disc-integrated emission (imod 18, cirsradg_wave.f:1116-1211) with the
+    # reference's E3 table (e3tab.npz): the layer weights are differences of
+    # 2 pi E3 of the transmission, the surface term 2 pi E3(tr) B(tsurf) is
+    # always on.  Amounts are scaled so
     # the column is semi-transparent and the surface shows; iform 1 or 3
     # multiply by the disc area 4 pi ((radius1 + radextra) 1e5)^2 (times
     # 1e-18 for 3)
@@ -346,6 +360,8 @@ def main(names=()):
     cia = synth.make_cia()
     if not names:
         save_cia(os.path.join(HERE, "cia.npz"), cia)
+    if not names or "e3tab" in names:
+        save_e3tab()
     if not names or "rayleigh" in names:
         np.savez_compressed(os.path.join(HERE, "rayleigh.npz"), **rayleigh_vectors())
     for name, t, r, note in cases():
@@ -360,7 +376,7 @@ def main(names=()):
         use = own or cia
         e3 = None
         if name in CASE_E3:
-            e3 = synth.make_e3_table()
+            e3 = load_e3()
             use = copy.copy(use)
             use.e3 = e3
         outs = run_ref(t, use, r)

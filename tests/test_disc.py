@@ -3,8 +3,9 @@
 Pinned by the reference's own cirsradg_wave on the disc, disc_iform3 and
 disc_nlte_dust golden cases (test_oracle.py runs every case_*.npz), and here
 by:
-* the E3 table against the exponential integral's own derivative
-  dE3(-ln tr)/dtr = E2(-ln tr)/tr, and e3interp.f's edge rules;
+* the reference's E3 table (raddata/e3tab.dat, kept as e3tab.npz) against
+  the exponential integral and its derivative dE3(-ln tr)/dtr = E2(-ln tr)/tr,
+  and e3interp.f's edge rules;
 * an isothermal column over a black surface at the same temperature, where
   the E3 differences telescope and R = pi B(T) whatever the opacity; a
   transparent column, where R = pi esurf B(tsurf);
@@ -22,11 +23,11 @@ from scipy.special import expn
 
 from oracle import oracle
 from radtrancode_amd import synth
-from tests.golden.golden_io import case_cia, load_case
+from tests.golden.golden_io import case_cia, load_case, load_e3
 from tests.tolerance import jac_err, rad_err
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-E3 = synth.make_e3_table()
+E3 = load_e3()
 
 
 def _e3(x):
@@ -41,18 +42,30 @@ def _e3(x):
 
 
 def test_e3_table_and_gradient():
-    assert E3.e3[0] == 0.0 and E3.e3[-1] == 0.5
-    for x in (0.05, 0.2371, 0.5, 0.8125, 0.97):
+    assert E3.e3[0] == 0.0 and E3.e3[-1] == 0.5 and E3.trans[-1] == 1.0
+    # the file's transmissions are i/1000 as REAL(4) wrote them
+    assert np.max(np.abs(E3.trans - np.arange(1001) / 1000.0)) < 1e-7
+    # E3 to 2e-6 from tr = 0.215 up and 2e-4 from tr = 0.05; below, the file
+    # departs from E3 (by 28% at tr = 0.001), and that is the table the
+    # reference runs with
+    for i0, tol in ((215, 2e-6), (50, 2e-4)):
+        tr = E3.trans[i0:-1]
+        assert np.max(np.abs(E3.e3[i0:-1] / expn(3, -np.log(tr)) - 1)) < tol
+    assert abs(E3.e3[1] / expn(3, -np.log(E3.trans[1])) - 1) > 0.2
+    for x, tol in ((0.05, 5e-3), (0.2371, 1e-4), (0.5, 1e-4), (0.8125, 1e-4), (0.97, 1e-4)):
         e, g = _e3(x)
         tau = -np.log(x)
-        assert e == pytest.approx(expn(3, tau), rel=2e-6)
+        assert e == pytest.approx(expn(3, tau), rel=2e-5)
         # the table's central differences against E2(tau)/tr
-        assert g == pytest.approx(expn(2, tau) / x, rel=2e-3)
-    # a negative or NaN transmission gives zero (e3interp.f:57-66); one
-    # past the table uses its last interval
+        assert g == pytest.approx(expn(2, tau) / x, rel=tol)
+    # at a grid point the value is the table's
+    assert _e3(E3.trans[437])[0] == pytest.approx(E3.e3[437], rel=1e-12)
+    # a negative or NaN transmission gives zero (e3interp.f:57-66); tr = 1
+    # uses the last interval, whose lower end the file holds as 0.999000013
     assert _e3(-1e-3) == (0.0, 0.0)
     assert _e3(float("nan")) == (0.0, 0.0)
-    assert _e3(1.0)[0] == pytest.approx(0.5, rel=1e-15)
+    assert _e3(1.0)[0] == pytest.approx(0.5, rel=1e-7)
+    assert _e3(1.5)[0] == 0.5  # past the table: the interval's end, f capped at 1
 
 
 def _disc_request(nwave=4, nlayer=16, nkgas=2, ng=8, scale=0.05):

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from radtrancode_amd import synth
-from tests.golden.golden_io import load_cia
+from tests.golden.golden_io import load_cia, load_e3
 from tests.tolerance import (TOL_JAC_FP64, TOL_JAC_MIXED, TOL_RAD_FP64, TOL_RAD_MIXED,
                              jac_err, rad_err)
 
@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 def _disc(L, nk, ng, nw):
     tables, cia, req = synth.config1(nwave=nw, nlayer=L, nkgas=nk, ng=ng)
     cia = load_cia()
-    cia.e3 = synth.make_e3_table()
+    cia.e3 = load_e3()
     # semi-transparent: the surface carries 1e-4 to 0.3 of the radiance and the
     # deepest layers' Jacobian rows are a quarter of the largest
     req.amount, req.pp = req.amount * 0.01, req.pp * 0.01

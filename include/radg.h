@@ -16,6 +16,8 @@
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
  *   radg_register_parah2     /FPTABLE/ filled by radtran/radtran/fpread_all.f:72-143
  *   radg_register_e3         /E3TABLE/ filled by radtran/radtran/e3interp.f:30-51
+ *   radg_register_n2nir      the tables inside radtran/radtran/n2n2cia.f:7-47 and
+ *                            n2h2cia.f:3-80 (nciacon.f:287-296, nparacon_all.f:312-320)
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
  *                            225,312) that the caller fills from its .xsc file
  *   radg_set_nlte            /NLTE_FLAGS/ inlte_flag, nlte_k1, nlte_A (cirsradg_wave.f:328)
@@ -132,6 +134,15 @@ int radg_register_parah2(double dnu, const double *temps, int npara, const doubl
  * table as e3interp.f:47-51 does.  Needed by requests with an imod 18 path;
  * n = 0 clears it. */
 int radg_register_e3(int n, const double *trans, const double *e3);
+
+/* N2's near-infrared CIA: N2-N2 (n2n2cia.f) and N2-H2 (n2h2cia.f) as those
+ * functions return them (cm-1 amagat-2) at their knots wn1[n1], wn2[n2]
+ * (increasing; the reference's are 4500:5:4825 and 3995:5:4995 cm-1, so a
+ * caller fills k1[i] = n2n2cia(wn1[i])).  Linear between knots (verint.f),
+ * zero outside.  Needed by requests with N2 at 3995-4995 cm-1; n1 = n2 = 0
+ * clears them. */
+int radg_register_n2nir(int n1, const double *wn1, const double *k1, int n2, const double *wn2,
+                        const double *k2);
 
 /* /NLTE_FLAGS/ (cirsradg_wave.f:328) for the fixed-argument entry points:
  * the emission temperature of every layer of an imod 3 or 18 path becomes

@@ -53,7 +53,9 @@ class _CIA(C.Structure):
     _fields_ = [("dnu", C.c_double), ("temps", C.c_void_p), ("kcia", C.c_void_p),
                 ("npara", C.c_int), ("dnu2", C.c_double), ("temps2", C.c_void_p),
                 ("fp", C.c_void_p), ("kh2h2", C.c_void_p), ("kh2he", C.c_void_p),
-                ("kcia5", C.c_void_p), ("e3tran", C.c_void_p), ("e3val", C.c_void_p)]
+                ("kcia5", C.c_void_p), ("e3tran", C.c_void_p), ("e3val", C.c_void_p),
+                ("n1", C.c_int), ("n2", C.c_int), ("wn1", C.c_void_p), ("k1", C.c_void_p),
+                ("wn2", C.c_void_p), ("k2", C.c_void_p)]
 
 
 class _REQ(C.Structure):
@@ -110,6 +112,12 @@ class _Pack:
             t3, v3 = _d(e3.trans), _d(e3.e3)
             self.keep += [t3, v3]
             self.cia.e3tran, self.cia.e3val = t3.ctypes.data, v3.ctypes.data
+        nir = getattr(cia, "n2nir", None)
+        if nir is not None:
+            arrs = [_d(nir.wn_n2n2), _d(nir.k_n2n2), _d(nir.wn_n2h2), _d(nir.k_n2h2)]
+            self.keep += arrs
+            self.cia.n1, self.cia.n2 = len(arrs[0]), len(arrs[2])
+            self.cia.wn1, self.cia.k1, self.cia.wn2, self.cia.k2 = [a.ctypes.data for a in arrs]
         L = req.nlayer
         hfp = req.hfp if req.hfp is not None else np.zeros(L)
         f = [_d(req.press), _d(req.temp), _d(req.delh), _d(hfp), _d(req.pp),

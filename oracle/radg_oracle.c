@@ -462,6 +462,51 @@ void orc_ciaread(const orc_cia *cia, double vv, double temp, double *alpha,
   }
 }
 
+/* verint.f:24-48: linear interpolation, the interval above the first grid
+ * point past xin (ascending or descending grids), the end intervals
+ * extrapolated; a zero-width interval or a NaN result gives y(i). */
+double orc_verint(const double *x, const double *y, int n, double xin) {
+  int i;
+  double yout;
+  if (x[0] < x[n - 1]) {
+    for (i = 1; i <= n; i++)
+      if (x[i - 1] > xin) break;
+  } else {
+    for (i = 1; i <= n; i++)
+      if (x[i - 1] < xin) break;
+  }
+  if (i > n) i = n;
+  if (i == 1) i = 2;
+  yout = y[i - 2] + (y[i - 1] - y[i - 2]) * (xin - x[i - 2]) / (x[i - 1] - x[i - 2]);
+  if (x[i - 2] == x[i - 1]) yout = y[i - 1];
+  if (isnan(yout)) yout = y[i - 1];
+  return yout;
+}
+
+/* n2n2cia.f and n2h2cia.f on the tables in memory: zero off the knots'
+ * range, verint inside (the tables hold the functions' values, the
+ * reference's 1e-5 km-1 -> cm-1 factor included). */
+static void orc_n2nir(const orc_cia *cia, double v, double *an2n2, double *an2h2) {
+  *an2n2 = *an2h2 = 0.0;
+  if (cia->n1 > 1 && !(v < cia->wn1[0]) && !(v > cia->wn1[cia->n1 - 1]))
+    *an2n2 = orc_verint(cia->wn1, cia->k1, cia->n1, v);
+  if (cia->n2 > 1 && !(v < cia->wn2[0] || v > cia->wn2[cia->n2 - 1]))
+    *an2h2 = orc_verint(cia->wn2, cia->k2, cia->n2, v);
+}
+
+/* nciacon.f:287-296 and nparacon_all.f:312-320: the N2-N2 and N2-H2
+ * near-infrared terms, added to sum and DABSORB before the path factor */
+static void orc_add_n2nir(const orc_cia *cia, double v, double qh2, double qn2, double *sum,
+                          double *dabsorb) {
+  double an2n2, an2h2;
+  orc_n2nir(cia, v, &an2n2, &an2h2);
+  *sum = *sum + an2n2 * qn2 * qn2;
+  dabsorb[2] += 2 * qn2 * an2n2;
+  *sum = *sum + an2h2 * qn2 * qh2;
+  dabsorb[2] += qh2 * an2h2;
+  dabsorb[0] += qn2 * an2h2;
+}
+
 /* co2cia.f */
 static double orc_co2cia(double v0) {
   double xl = 1e4 / v0, a = 0.0;
@@ -663,8 +708,7 @@ static void orc_cia_setup(double p, double t, int ngas, const int *idgas, const 
 
 /* ---------------------------------------------------------------- nciacon */
 /* nciacon.f:1-310.  iabs[5] are 1-based gas numbers (0 = absent),
- * dabsorb[7].  The near-infrared N2-N2 and N2-H2 terms (nciacon.f:287-296)
- * vanish outside 3995-4995 cm-1, where the caller refuses N2. */
+ * dabsorb[7]. */
 void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
                  int ngas, const int *idgas, const int *isogas,
                  const double *amount, const double *pp, double xlen,
@@ -708,6 +752,7 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t, int inormal,
   aco2 = orc_co2cia(v0);
   sum = sum + aco2 * qco2 * qco2;
   dabsorb[4] += 2 * qco2 * aco2;
+  orc_add_n2nir(cia, v0, qh2, qn2, &sum, dabsorb);
   *absorb = sum * tau;
   for (i = 0; i < 7; i++) dabsorb[i] *= tau;
 }
@@ -833,7 +878,7 @@ static void orc_nparacon_all(const orc_cia *cia, double vwave, double p, double 
   aco2 = orc_co2cia(vwave);
   sum = sum + aco2 * qco2 * qco2;
   dabsorb[4] += 2 * qco2 * aco2;
-  /* the near-infrared N2 terms vanish where N2 is allowed (see nciacon) */
+  orc_add_n2nir(cia, vwave, qh2, qn2, &sum, dabsorb);
   *absorb = sum * tau;
   for (i = 0; i < 7; i++) dabsorb[i] *= tau;
 }
@@ -1064,10 +1109,11 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
   /* calc_nlte_t.f:63-67 stops for k1 outside [1e-20, 1e-3] */
   if (r->inlte == 1 && (r->nlte_k1 < 1.0e-20 || r->nlte_k1 > 1.0e-3)) return ORC_ERR_LIMITS;
   /* N2's near-infrared CIA (n2n2cia.f, n2h2cia.f: tabulated, non-zero only
-   * for 3995 <= v <= 4995 cm-1) is not restated: N2 is refused there */
+   * for 3995 <= v <= 4995 cm-1) needs its tables when N2 is present there */
   for (k = 0; k < ngas; k++) {
     const double vn = r->ispace == 1 ? 1e4 / vwave : vwave;
-    if (r->idgas[k] == 22 && vn >= 3995.0 && vn <= 4995.0) return ORC_ERR_UNSUPPORTED;
+    if (r->idgas[k] == 22 && vn >= 3995.0 && vn <= 4995.0 && (cia->n1 < 2 || cia->n2 < 2))
+      return ORC_ERR_UNSUPPORTED;
   }
 
   /* g-grid: all tables share it (read_klist.f:117-130). */

@@ -40,6 +40,11 @@ typedef struct orc_cia {
   /* /E3TABLE/ (e3interp.f): 1001 transmissions and E3 values (NULL: none;
    * imod 18 needs it) */
   const double *e3tran, *e3val;
+  /* N2's near-infrared CIA (n2n2cia.f, n2h2cia.f) as those functions return
+   * it at their knots, cm-1 amagat-2: N2-N2 on wn1[n1], N2-H2 on wn2[n2]
+   * (n1 = n2 = 0: none; then N2 is refused at 3995-4995 cm-1) */
+  int n1, n2;
+  const double *wn1, *k1, *wn2, *k2;
 } orc_cia;
 
 /* The argument surface of cirsradg_wave (cirsradg_wave.f:1-5), compact C
@@ -112,6 +117,7 @@ void orc_nciacon(const orc_cia *cia, double v0, double p, double t,
                  double *absorb, int *iabs, double *dabsorb);       /* nciacon.f */
 double orc_nlte_t(int inlte, double k1, double a, double t, double p, double v); /* calc_nlte_t.f */
 double orc_interp(const double *x, const double *y, int n, double xin); /* interp.f */
+double orc_verint(const double *x, const double *y, int n, double xin); /* verint.f */
 /* e3interp.f on a 1001-point table: E3 at transmission x, dE3/dtr in *de3 */
 double orc_e3(const double *trans, const double *e3, double x, double *de3);
 double orc_rayleigh(int iray, double v, double p, double t, double fheh2,

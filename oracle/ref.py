@@ -68,6 +68,17 @@ def rayleigh(prec, iray, v, p=1.0, t=273.15, fheh2=0.0, fch4h2=0.0, fnh3=0.0):
     return float(fn(*[C.byref(h) for h in held]))
 
 
+def n2nir(prec, which, v):
+    """The reference's own N2N2CIA or N2H2CIA (``radtran/radtran/``) at
+    wavenumber v, cm-1 amagat-2."""
+    lb = _load(prec)
+    ct = C.c_double if prec == 64 else C.c_float
+    fn = getattr(lb, {"n2n2": "n2n2cia_", "n2h2": "n2h2cia_"}[which])
+    fn.restype = ct
+    h = ct(v)
+    return float(fn(C.byref(h)))
+
+
 class RefRunner:
     """One loaded reference library with tables registered for a request.
 

@@ -23,7 +23,7 @@ MODE_BF16 = 2
 EXPORTS = [
     "radg_set_mode", "radg_get_mode", "radg_last_error", "radg_register_ktables",
     "radg_read_klist", "radg_register_cia", "radg_register_dust", "radg_set_nlte",
-    "radg_set_iform", "radg_register_parah2", "radg_register_e3",
+    "radg_set_iform", "radg_register_parah2", "radg_register_e3", "radg_register_n2nir",
     "cirsradg_wave_",
     "radg_cirsradg_status",
     "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
@@ -90,6 +90,7 @@ def lib():
     L.radg_set_iform.argtypes = [ip, ip]
     L.radg_register_parah2.argtypes = [C.c_double, vp, ip, vp, vp, vp, vp]
     L.radg_register_e3.argtypes = [ip, vp, vp]
+    L.radg_register_n2nir.argtypes = [ip, vp, vp, ip, vp, vp]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
     L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]
     L.radg_batch_bytes.argtypes = [ip, vp, ip, C.POINTER(C.c_size_t)]

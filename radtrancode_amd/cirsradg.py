@@ -12,7 +12,8 @@ Reference correspondence:
   that ``read_klist.f`` fills; :func:`register_cia` -- ``/CIATABLE/``;
   :func:`register_dust` -- ``/DUST/`` for :func:`cirsradg_wave` (batched
   requests carry their own dust arrays); :func:`register_parah2` --
-  ``/FPTABLE/``; :func:`register_e3` -- ``/E3TABLE/`` for imod 18.
+  ``/FPTABLE/``; :func:`register_e3` -- ``/E3TABLE/`` for imod 18;
+  :func:`register_n2nir` -- N2's near-infrared CIA tables.
 * :class:`Batch` -- many requests resident on the device, launched together.
 
 Errors raise :class:`RadgError` where the reference would STOP.
@@ -81,7 +82,8 @@ def read_klist(path: str, recl: int = 4) -> None:
 
 def register_cia(cia) -> None:
     """``/CIATABLE/`` and, if ``cia.para`` is set, ``/FPTABLE/`` (else the
-    para-H2 table is cleared); likewise ``/E3TABLE/`` from ``cia.e3``."""
+    para-H2 table is cleared); likewise ``/E3TABLE/`` from ``cia.e3`` and
+    N2's near-infrared CIA from ``cia.n2nir``."""
     temps = np.ascontiguousarray(cia.temps, dtype=np.float64)
     kcia = np.ascontiguousarray(cia.kcia, dtype=np.float64)
     if kcia.shape != (1501, 25, 9):
@@ -89,6 +91,20 @@ def register_cia(cia) -> None:
     check(lib().radg_register_cia(C.c_double(cia.dnu), _vp(temps), _vp(kcia)))
     register_parah2(getattr(cia, "para", None))
     register_e3(getattr(cia, "e3", None))
+    register_n2nir(getattr(cia, "n2nir", None))
+
+
+def register_n2nir(t) -> None:
+    """N2's near-infrared CIA from a ``synth.N2NirTable`` (None clears it):
+    ``n2n2cia.f`` and ``n2h2cia.f`` values at their knots."""
+    if t is None:
+        check(lib().radg_register_n2nir(0, None, None, 0, None, None))
+        return
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (t.wn_n2n2, t.k_n2n2, t.wn_n2h2, t.k_n2h2)]
+    if a[0].shape != a[1].shape or a[2].shape != a[3].shape:
+        raise ValueError("N2 near-infrared CIA: knots and values must match")
+    check(lib().radg_register_n2nir(int(a[0].shape[0]), _vp(a[0]), _vp(a[1]),
+                                     int(a[2].shape[0]), _vp(a[2]), _vp(a[3])))
 
 
 def register_e3(e3) -> None:

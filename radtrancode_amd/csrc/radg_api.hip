@@ -129,16 +129,49 @@ struct E3Tab {
   }
 };
 
+// N2's near-infrared CIA (n2n2cia.f, n2h2cia.f): each function's values
+// (cm-1 amagat-2) at its knots
+struct N2Nir {
+  std::vector<double> wn1, k1, wn2, k2;  // N2-N2, N2-H2
+};
+
 // The tables beside the k-tables and CIA that some requests need
 struct AuxTables {
   std::shared_ptr<ParaCIA> para;
   std::shared_ptr<E3Tab> e3;
+  std::shared_ptr<N2Nir> n2nir;
 };
 
 std::shared_ptr<TableSet> g_tables;
 std::shared_ptr<CIA> g_cia;
 std::shared_ptr<ParaCIA> g_para;
 std::shared_ptr<E3Tab> g_e3;
+std::shared_ptr<N2Nir> g_n2nir;
+
+// verint.f:24-48: linear interpolation in the interval above the first grid
+// point past xin (ascending or descending), the end intervals extrapolated;
+// a zero-width interval or a NaN result gives y(i)
+double verint(const double *x, const double *y, int n, double xin) {
+  int i = 1;
+  if (x[0] < x[n - 1]) {
+    while (i <= n && !(x[i - 1] > xin)) i++;
+  } else {
+    while (i <= n && !(x[i - 1] < xin)) i++;
+  }
+  if (i > n) i = n;
+  if (i == 1) i = 2;
+  double yout = y[i - 2] + (y[i - 1] - y[i - 2]) * (xin - x[i - 2]) / (x[i - 1] - x[i - 2]);
+  if (x[i - 2] == x[i - 1]) yout = y[i - 1];
+  if (std::isnan(yout)) yout = y[i - 1];
+  return yout;
+}
+
+// n2n2cia.f / n2h2cia.f: zero off the knots' range, verint inside
+double n2nir_coef(const std::vector<double> &x, const std::vector<double> &y, double v) {
+  const int n = (int)x.size();
+  if (n < 2 || v < x[0] || v > x[n - 1]) return 0.0;
+  return verint(x.data(), y.data(), n, v);
+}
 
 // /DUST/ for the fixed-argument entry points (cirsradg_wave.f:225,312)
 struct Dust {
@@ -211,6 +244,7 @@ struct Packed {
   std::vector<LayerConst> lay;
   std::vector<double> amo, press, scale, emtemp, vwave, esurf, pweight;
   std::vector<double> dext, cont;  // dust: [nwave][ncont] extinction, [L][ncont] columns
+  std::vector<double> n2nir;       // [nwave][2]: N2-N2, N2-H2 near-infrared CIA (cm-1 amagat-2)
   std::vector<PathInfo> paths;
   std::vector<int> layinc;
   std::vector<WaveGas> wg;
@@ -451,6 +485,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   P.vwave.assign(q.vwave, q.vwave + nwave);
   P.esurf.resize(nwave);
   P.dext.assign((size_t)nwave * ncont, 0.0);
+  P.n2nir.assign((size_t)nwave * 2, 0.0);
   P.cont.assign(q.cont, q.cont + (size_t)L * ncont);
   P.wg.resize((size_t)nwave * ngas);
   const long long ntab = (long long)np * nt * NG;
@@ -460,12 +495,18 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   for (int iw = 0; iw < nwave; iw++) {
     const double v = q.vwave[iw];
     if (!std::isfinite(v) || v <= 0) return fail(RADG_EINPUT, "vwave(%d)=%g", iw + 1, v);
-    // N2's near-infrared CIA (n2n2cia.f, n2h2cia.f; nciacon.f:287-296) is
-    // tabulated for 3995-4995 cm-1 and zero elsewhere; it is not provided
+    // N2's near-infrared CIA (n2n2cia.f, n2h2cia.f; nciacon.f:287-296,
+    // nparacon_all.f:312-320), at the wavenumber the CIA sees: tabulated for
+    // 3995-4995 cm-1 and zero elsewhere, so with N2 present there the tables
+    // must be registered
     {
       const double vn = q.ispace == 1 ? 1e4 / v : v;
-      if (has_n2 && vn >= 3995.0 && vn <= 4995.0)
-        return fail(RADG_EUNSUPPORTED, "N2 near-infrared CIA (3995-4995 cm-1) is not provided");
+      if (aux.n2nir) {
+        P.n2nir[(size_t)iw * 2] = n2nir_coef(aux.n2nir->wn1, aux.n2nir->k1, vn);
+        P.n2nir[(size_t)iw * 2 + 1] = n2nir_coef(aux.n2nir->wn2, aux.n2nir->k2, vn);
+      } else if (has_n2 && vn >= 3995.0 && vn <= 4995.0) {
+        return fail(RADG_ENOTABLES, "N2 at %g cm-1 needs its near-infrared CIA (radg_register_n2nir)", vn);
+      }
     }
     // interpem.f:17-30 (called unconditionally by cirsradg_wave.f:381)
     const int nem = q.nem;
@@ -744,7 +785,7 @@ struct Layout {
 struct Plan {
   std::vector<Packed> packs;
   size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
-      o_vw, o_es, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
+      o_vw, o_es, o_n2, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
       o_err;
   size_t total = 0;
   long long nA = 0, nB = 0, nC = 0;
@@ -837,6 +878,7 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_emt = lo.take<double>(pl.npl);
   pl.o_vw = lo.take<double>(pl.nwv);
   pl.o_es = lo.take<double>(pl.nwv);
+  pl.o_n2 = lo.take<double>(2 * pl.nwv);
   pl.o_wg = lo.take<WaveGas>(pl.nwg);
   pl.o_pw = lo.take<double>(pl.npath);
   pl.o_dx = lo.take<double>(pl.ndx);
@@ -1027,6 +1069,34 @@ int radg_register_parah2(double dnu, const double *temps, int npara, const doubl
   return RADG_OK;
 }
 
+int radg_register_n2nir(int n1, const double *wn1, const double *k1, int n2, const double *wn2,
+                        const double *k2) {
+  if (n1 == 0 && n2 == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_n2nir.reset();
+    return RADG_OK;
+  }
+  if (n1 < 2 || n2 < 2 || n1 > 100000 || n2 > 100000)
+    return fail(RADG_ELIMITS, "N2 near-infrared CIA: n1=%d n2=%d, each needs 2..100000 knots", n1, n2);
+  if (!wn1 || !k1 || !wn2 || !k2) return fail(RADG_EINPUT, "N2 near-infrared CIA: null array");
+  for (const auto &t : {std::make_pair(wn1, n1), std::make_pair(wn2, n2)})
+    for (int i = 1; i < t.second; i++)
+      if (!(t.first[i] > t.first[i - 1]))
+        return fail(RADG_EINPUT, "N2 near-infrared CIA: knots must increase");
+  for (int i = 0; i < n1; i++)
+    if (!std::isfinite(k1[i])) return fail(RADG_EINPUT, "N2-N2 value %d not finite", i + 1);
+  for (int i = 0; i < n2; i++)
+    if (!std::isfinite(k2[i])) return fail(RADG_EINPUT, "N2-H2 value %d not finite", i + 1);
+  auto t = std::make_shared<N2Nir>();
+  t->wn1.assign(wn1, wn1 + n1);
+  t->k1.assign(k1, k1 + n1);
+  t->wn2.assign(wn2, wn2 + n2);
+  t->k2.assign(k2, k2 + n2);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_n2nir = t;
+  return RADG_OK;
+}
+
 int radg_register_e3(int n, const double *trans, const double *e3) {
   if (n == 0) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1102,7 +1172,7 @@ int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
-    aux = {g_para, g_e3};
+    aux = {g_para, g_e3, g_n2nir};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
   Plan pl;
@@ -1122,7 +1192,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
     cia = g_cia;
-    aux = {g_para, g_e3};
+    aux = {g_para, g_e3, g_n2nir};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
@@ -1196,6 +1266,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     put(h, pl.o_emt + d.pl0 * sizeof(double), P.emtemp.data(), P.emtemp.size());
     put(h, pl.o_vw + d.wave0 * sizeof(double), P.vwave.data(), P.vwave.size());
     put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
+    put(h, pl.o_n2 + 2 * d.wave0 * sizeof(double), P.n2nir.data(), P.n2nir.size());
     put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
     put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
     put(h, pl.o_dx + d.dx0 * sizeof(double), P.dext.data(), P.dext.size());
@@ -1249,6 +1320,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.emtemp = reinterpret_cast<const double *>(m + pl.o_emt);
   a.vwave = reinterpret_cast<const double *>(m + pl.o_vw);
   a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
+  a.n2nir = reinterpret_cast<const double *>(m + pl.o_n2);
   a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
   a.dext = reinterpret_cast<const double *>(m + pl.o_dx);

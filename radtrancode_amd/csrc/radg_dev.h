@@ -113,6 +113,7 @@ struct KernelArgs {
   const int *layinc;           // [path][L] 0-based
   const double *scale, *emtemp;
   const double *vwave, *esurf;
+  const double *n2nir;  // [wavenumber][2]: N2-N2, N2-H2 near-infrared CIA
   const WaveGas *wg;
   const double *pweight;       // [path]
   const double *dext;          // [wavenumber][ncont] dust extinction (cm2) at vwave

@@ -431,10 +431,23 @@ __device__ __forceinline__ double wave_incl_sum(double s) {
 __device__ __forceinline__ double wave_shr1(double s) { return dpp_d<0x138, 0xF, true>(s); }
 
 // ----------------------------------------------------------- CIA, FP64
+// nciacon.f:287-296 and nparacon_all.f:312-320: N2's near-infrared N2-N2
+// and N2-H2 terms, the coefficients interpolated per wavenumber on the host
+__device__ __forceinline__ void cia_n2nir(const double *n2, double qh2, double qn2, double &sum,
+                                          double *dab) {
+  const double an2n2 = n2[0], an2h2 = n2[1];
+  sum += an2n2 * qn2 * qn2;
+  dab[2] += 2 * qn2 * an2n2;
+  sum += an2h2 * qn2 * qh2;
+  dab[2] += qh2 * an2h2;
+  dab[0] += qn2 * an2h2;
+}
+
 // ciaread.f:104-196 + nciacon.f:209-303 for one layer.  dab[0..6] as
-// DABSORB (already multiplied by the path factor tau).
+// DABSORB (already multiplied by the path factor tau).  n2: this
+// wavenumber's near-infrared N2 coefficients.
 __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int inormal, const LayerConst &lc,
-                          double *absorb, double *dab) {
+                                          const double *n2, double *absorb, double *dab) {
   double alpha[9], dal[9];
 #pragma unroll
   for (int i = 0; i < 9; i++) {
@@ -515,6 +528,7 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
     sum += aco2 * qco2 * qco2;
     dab[4] += 2 * qco2 * aco2;
   }
+  cia_n2nir(n2, qh2, qn2, sum, dab);
   *absorb = sum * lc.tau;
   for (int i = 0; i < 7; i++) dab[i] *= lc.tau;
 }
@@ -527,7 +541,7 @@ __device__ __forceinline__ void cia_layer(const KernelArgs &a, double vv, int in
 // [1, npara-2] with f = 0 below and f = 1 above; fpara < 0 takes the last
 // fraction and has no derivative.  dab[6] = d(tau)/d(fpara).
 __device__ __forceinline__ void cia_para_layer(const KernelArgs &a, double vv, const LayerConst &lc,
-                                               double *absorb, double *dab) {
+                                               const double *n2, double *absorb, double *dab) {
   const int np = a.npara;
   const double deltaf = a.pfp[1] - a.pfp[0];
   int j0, j1;
@@ -633,6 +647,7 @@ __device__ __forceinline__ void cia_para_layer(const KernelArgs &a, double vv, c
     sum += aco2 * qco2 * qco2;
     dab[4] += 2 * qco2 * aco2;
   }
+  cia_n2nir(n2, qh2, qn2, sum, dab);
   *absorb = sum * lc.tau;
   for (int i = 0; i < 7; i++) dab[i] *= lc.tau;
 }
@@ -1216,10 +1231,11 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   double tdust = 0.0;
   for (int k = 0; k < R.ncont; k++)
     tdust += a.dext[R.dx0 + (long long)iw * R.ncont + k] * a.cont[R.ct0 + (long long)j * R.ncont + k];
+  const double *n2 = a.n2nir + 2 * (R.wave0 + iw);
   if (R.flagh2p)
-    cia_para_layer(a, vv, lc, &absorb, dab);
+    cia_para_layer(a, vv, lc, n2, &absorb, dab);
   else
-    cia_layer(a, vv, R.inormal, lc, &absorb, dab);
+    cia_layer(a, vv, R.inormal, lc, n2, &absorb, dab);
   absorb = tdust + absorb;
   // Rayleigh (cirsradg_wave.f:663-691): utotl * xray more extinction and
   // xray * 1e-20 in every gas's d(tau)/d(amount); no T derivative.  The gas

@@ -17,9 +17,10 @@ program radg_host
   integer(c_int) :: klen, recl, nlayer, npath, ngas, nwave, nem, nsw, inormal, iray, ispace
   integer(c_int) :: itype, flagh2p, amform, iwave, nparam, rc, mode, nsec, ncont, inlte, iform
   real(c_double) :: nlte_k1, nlte_a, pdnu, ptemps(25)
-  integer(c_int) :: npara, ne3
+  integer(c_int) :: npara, ne3, nn1, nn2
   real(c_double), allocatable :: pfp(:), pkhh(:), pkhe(:), pk5(:)
   real(c_double) :: e3t(1001), e3v(1001)
+  real(c_double), allocatable :: wn1(:), k1(:), wn2(:), k2(:)
   real(c_double), allocatable :: vsec(:), xsec(:)
   real(c_double) :: dnu, ctemps(25)
   real(c_double), allocatable :: kcia(:, :, :)
@@ -65,6 +66,10 @@ program radg_host
   ! /E3TABLE/: ne3 (0 = none or 1001), then trans(1001), e3(1001)
   read (10) ne3
   if (ne3 > 0) read (10) e3t, e3v
+  ! N2's near-infrared CIA: nn1, nn2 (0, 0 = none), then the two knot tables
+  read (10) nn1, nn2
+  allocate (wn1(max(nn1, 1)), k1(max(nn1, 1)), wn2(max(nn2, 1)), k2(max(nn2, 1)))
+  if (nn1 > 0) read (10) wn1, k1, wn2, k2
   close (10)
 
   if (radg_read_klist_f(klist(1:klen), recl) /= RADG_OK) call die()
@@ -78,6 +83,9 @@ program radg_host
   end if
   if (ne3 > 0) then
     if (radg_register_e3(ne3, e3t, e3v) /= RADG_OK) call die()
+  end if
+  if (nn1 > 0) then
+    if (radg_register_n2nir(nn1, wn1, k1, nn2, wn2, k2) /= RADG_OK) call die()
   end if
 
   nparam = ngas + 1 + ncont

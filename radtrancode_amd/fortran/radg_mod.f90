@@ -17,7 +17,7 @@ module radg
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
   public :: radg_register_dust, radg_set_nlte, radg_set_iform, radg_register_parah2
-  public :: radg_register_e3
+  public :: radg_register_e3, radg_register_n2nir
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -89,6 +89,14 @@ module radg
       real(c_double), value :: dnu
       integer(c_int), value :: npara
       real(c_double) :: temps(25), fp(*), kh2h2(*), kh2he(*), kcia5(*)
+    end function
+
+    ! N2's near-infrared CIA: n2n2cia/n2h2cia values at their knots; 0, 0 clears
+    integer(c_int) function radg_register_n2nir(n1, wn1, k1, n2, wn2, k2) &
+        bind(C, name="radg_register_n2nir")
+      import :: c_int, c_double
+      integer(c_int), value :: n1, n2
+      real(c_double) :: wn1(*), k1(*), wn2(*), k2(*)
     end function
 
     ! /E3TABLE/ (imod 18): n = 1001 transmissions and E3 values; n = 0 clears

@@ -88,6 +88,18 @@ class CIATable:
     kcia: np.ndarray       # [1501, 25, 9] float64
     para: Optional["ParaCIATable"] = None
     e3: Optional["E3Table"] = None
+    n2nir: Optional["N2NirTable"] = None
+
+
+@dataclass
+class N2NirTable:
+    """N2's near-infrared CIA, ``n2n2cia.f`` and ``n2h2cia.f``: each
+    function's value (cm-1 amagat-2) at its knots, linearly interpolated
+    between them and zero outside."""
+    wn_n2n2: np.ndarray
+    k_n2n2: np.ndarray
+    wn_n2h2: np.ndarray
+    k_n2h2: np.ndarray
 
 
 @dataclass

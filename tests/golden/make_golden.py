@@ -26,9 +26,31 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from radtrancode_amd import synth  # noqa: E402
 from oracle import ref  # noqa: E402
-from tests.golden.golden_io import load_e3, save_case, save_cia  # noqa: E402
+from tests.golden.golden_io import load_e3, load_n2nir, save_case, save_cia  # noqa: E402
 
 REF_E3TAB = "/root/reference/raddata/e3tab.dat"
+
+
+# the knots of n2n2cia.f (66 points from 4500 cm-1) and n2h2cia.f (201 from
+# 3995 cm-1), 5 cm-1 apart
+N2N2_KNOTS = 4500.0 + 5.0 * np.arange(66)
+N2H2_KNOTS = 3995.0 + 5.0 * np.arange(201)
+
+
+def save_n2nir(path=os.path.join(HERE, "n2nir.npz")):
+    """N2's near-infrared CIA from the reference's own N2N2CIA and N2H2CIA
+    (REAL(8) build): the values at the knots (the table a caller registers)
+    and, to pin the knots and the interpolation, at every mid-point and at
+    points just outside each range."""
+    d = {}
+    for which, knots in (("n2n2", N2N2_KNOTS), ("n2h2", N2H2_KNOTS)):
+        probe = np.concatenate([0.5 * (knots[1:] + knots[:-1]), knots[:1] - 0.5, knots[-1:] + 0.5,
+                                knots[:1] + 1.25, knots[-1:] - 3.75])
+        d[f"{which}_wn"] = knots
+        d[f"{which}_k"] = np.array([ref.n2nir(64, which, v) for v in knots])
+        d[f"{which}_probe_wn"] = probe
+        d[f"{which}_probe_k"] = np.array([ref.n2nir(64, which, v) for v in probe])
+    np.savez_compressed(path, **d)
 
 
 def save_e3tab(path=os.path.join(HERE, "e3tab.npz")):
@@ -207,6 +229,31 @@ def cases():
                                         + (", CH4 and NH3 present" if iray == 4 else "")
                                         + (", CIA overlapping" if v0 < 1500 else ", no CIA"))
 
+    # N2's near-infrared CIA (n2n2cia.f on 4500-4825 cm-1, n2h2cia.f on
+    # 3995-4995 cm-1): the Titan-like N2 + CH4 + H2 column at wavenumbers
+    # inside, at the edges of and outside each table, in wavenumber and in
+    # wavelength space
+    for tag, ispace, grid, vw in (
+            ("titan_nir", 0, (4440.0, 20.0, 30), [4460.0, 4500.0, 4580.0, 4660.0, 4820.0, 4840.0,
+                                                  4980.0, 5000.0]),
+            ("titan_nir_um", 1, (1.99, 0.01, 56), [2.0, 2.02, 2.08, 2.13, 2.17, 2.21, 2.25, 2.5])):
+        t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=2, ng=10)
+        ids = [6, 27]
+        t = synth.make_ktables(ids, grid[0], grid[1], grid[2], ng=10, np_=10, nt=6, seed=33 + ispace,
+                               mu=-25.0)
+        r.idgas = np.array(ids + [synth.H2_ID, 22], dtype=np.int32)
+        r.isogas = np.zeros(4, dtype=np.int32)
+        q = np.array([0.015, 3e-6, 0.001, 0.984])
+        r.pp = r.press[:, None] * q[None, :] * 0.05
+        r.amount = r.amount.sum(axis=1)[:, None] * q[None, :] * 0.05
+        r.ispace = ispace
+        r.vwave = np.array(vw)
+        r.vem = np.array([1.5, 3.0]) if ispace else np.array([3900.0, 5100.0])
+        r.emissivity = np.array([1.0, 1.0])
+        r.isw = np.arange(1, r.nparam + 1, dtype=np.int32)
+        yield tag, t, r, (f"N2 + CH4 + H2 with N2's near-infrared CIA, ispace = {ispace}, "
+                          "wavenumbers inside and outside both tables")
+
     # disc-integrated emission (imod 18, cirsradg_wave.f:1116-1211) with the
     # reference's E3 table (e3tab.npz): the layer weights are differences of
     # 2 pi E3 of the transmission, the surface term 2 pi E3(tr) B(tsurf) is
@@ -352,6 +399,8 @@ CASE_CIA = {"titan_n2": dict(seed=5, pairs=(0, 1, 4, 5, 6, 7, 8)),
 CASE_PARA = {"parah2": dict(seed=9), "parah2_dust": dict(seed=10, npara=12)}
 # cases with imod 18 paths, which run with the E3 table
 CASE_E3 = ("disc", "disc_iform3", "disc_nlte_dust")
+# cases with N2 at 3995-4995 cm-1, which need N2's near-infrared CIA
+CASE_N2NIR = ("titan_nir", "titan_nir_um")
 
 
 def main(names=()):
@@ -362,6 +411,8 @@ def main(names=()):
         save_cia(os.path.join(HERE, "cia.npz"), cia)
     if not names or "e3tab" in names:
         save_e3tab()
+    if not names or "n2nir" in names:
+        save_n2nir()
     if not names or "rayleigh" in names:
         np.savez_compressed(os.path.join(HERE, "rayleigh.npz"), **rayleigh_vectors())
     for name, t, r, note in cases():
@@ -374,14 +425,16 @@ def main(names=()):
                 own.para = synth.make_para_cia(**CASE_PARA[name])
             own = banded_cia(own, r.vwave, half=1 if name in CASE_PARA else 3)
         use = own or cia
-        e3 = None
+        e3 = nir = None
         if name in CASE_E3:
             e3 = load_e3()
             use = copy.copy(use)
             use.e3 = e3
+        if name in CASE_N2NIR:  # for the oracle and the GPU; the reference has its own
+            nir = load_n2nir()
         outs = run_ref(t, use, r)
         path = os.path.join(HERE, f"case_{name}.npz")
-        save_case(path, t, r, outs, note, cia=own, e3=e3)
+        save_case(path, t, r, outs, note, cia=own, e3=e3, n2nir=nir)
         print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
               f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
 

@@ -52,6 +52,11 @@ def _write_input(path, klist, req, cia, mode):
         f.write(i4(0 if e3 is None else len(e3.trans)))
         if e3 is not None:
             f.write(np.asarray(e3.trans, np.float64).tobytes() + np.asarray(e3.e3, np.float64).tobytes())
+        nir = getattr(cia, "n2nir", None)
+        f.write(i4([0, 0] if nir is None else [len(nir.wn_n2n2), len(nir.wn_n2h2)]))
+        if nir is not None:
+            f.write(b"".join(np.asarray(x, np.float64).tobytes()
+                             for x in (nir.wn_n2n2, nir.k_n2n2, nir.wn_n2h2, nir.k_n2h2)))
 
 
 def _read_output(path, req):
@@ -76,7 +81,7 @@ def _read_output(path, req):
 
 @pytest.mark.parametrize("name", ["medium", "surface_imod", "partial_paths", "dust", "nlte_k1",
                                   "iform1", "wavelength_ray_dust", "parah2_dust",
-                                  "disc_nlte_dust"])
+                                  "disc_nlte_dust", "titan_nir_um"])
 def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     host = rbuild.FORTRAN_HOST
     assert os.path.exists(host), "Fortran host not built (flang missing at build time?)"
@@ -110,11 +115,11 @@ def test_fortran_host_calls_cirsradg_wave_unchanged(gpu_lib, tmp_path, name):
     assert jac_err(fp32_range_jac(d, req.ngas), fp32_range_jac(d_ref, req.ngas)) < 1e-4, note
     # and it stays close to what the reference's own REAL(4) build returns,
     # in every Jacobian column where that build agrees with its REAL(8) one
-    # (on an opaque imod 0 or 1 path 1 - (1 - e^-500) is 0 in REAL(4), so the
-    # REAL(4) build's column is zero there)
+    # to 5e-4 (on an opaque imod 0 or 1 path 1 - (1 - e^-500) is 0 in
+    # REAL(4), so the REAL(4) build's column is zero there)
     assert rad_err(o, outs["ref32_output"]) < 1e-5, note
     j32, j64 = outs["ref32_doutputdq"], outs["ref64_doutputdq"]
-    own = jac_err_cols(j32, j64) < 1e-3
+    own = jac_err_cols(j32, j64) < 5e-4
     assert own.mean() > 0.5, note
     assert np.max(jac_err_cols(d, j32)[own]) < 1e-3, note
 

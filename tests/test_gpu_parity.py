@@ -29,8 +29,8 @@ def test_golden_case(gpu_lib, path, mode):
     tables, req, outs, note = load_case(path)
     print(f"case {path.split('/')[-1]} mode {mode}", flush=True)
     files = np.load(path, allow_pickle=False).files
-    own = "cia_kcia" in files or "e3_val" in files
-    if own:  # a case with its own CIA or E3 table (the registry is global)
+    own = any(k in files for k in ("cia_kcia", "e3_val", "n2n2_k"))
+    if own:  # a case with tables of its own (the registry is global)
         gpu_lib.register_cia(case_cia(path))
     try:
         o, g, d = _run(gpu_lib, tables, req, mode)
@@ -116,22 +116,25 @@ def test_rayleigh_fp64_tight(gpu_lib, path):
     assert rad_err(o0, outs["ref64_output"]) > 1e-7
 
 
-def test_n2_refused_only_in_its_near_infrared_window(gpu_lib):
-    """N2's tabulated near-infrared CIA (n2n2cia.f, n2h2cia.f) is not
-    provided: a wavenumber in 3995-4995 cm-1 with N2 present is refused with
-    RADG_EUNSUPPORTED; the same request elsewhere runs (test_golden_case
-    checks its values on the titan_n2 case)."""
-    import copy
+def test_n2_near_infrared_needs_its_tables(gpu_lib):
+    """N2 at 3995-4995 cm-1 needs N2's near-infrared CIA (n2n2cia.f,
+    n2h2cia.f, radg_register_n2nir): without it the request is refused with
+    RADG_ENOTABLES; with it the same request runs and matches the reference
+    (test_golden_case checks the titan_nir cases)."""
     from radtrancode_amd._native import RadgError
-    path = [p for p in CASES if p.endswith("case_titan_n2.npz")][0]
-    tables, req, _, _ = load_case(path)
+    path = [p for p in CASES if p.endswith("case_titan_nir.npz")][0]
+    tables, req, outs, _ = load_case(path)
     gpu_lib.register_ktables(tables)
-    bad = copy.copy(req)
-    bad.vwave = req.vwave.copy()
-    bad.vwave[-1] = 4400.0
+    gpu_lib.register_cia(load_cia())
     with pytest.raises(RadgError) as e:
-        gpu_lib.spectrum(bad, mode=0)
-    assert e.value.code == 2 and "N2" in str(e.value)  # RADG_EUNSUPPORTED
+        gpu_lib.spectrum(req, mode=0)
+    assert e.value.code == 3 and "N2" in str(e.value)  # RADG_ENOTABLES
+    gpu_lib.register_cia(case_cia(path))
+    try:
+        o, _, _ = gpu_lib.spectrum(req, mode=0)
+    finally:
+        gpu_lib.register_cia(load_cia())
+    assert rad_err(o, outs["ref64_output"]) < TOL_RAD_FP64
 
 
 def test_dust_grid_must_cover_the_wavenumbers(gpu_lib):

@@ -221,9 +221,13 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray,
  *   nlayin/imod[npath]; layinc (1-based layer numbers), scale, emtemp
  *   [npath][L]; isw[nsw] (1-based parameter ids); vwave[nwave];
  *   vem/emissivity[nem]; path_weight[npath] (angle quadrature weights used
- *   for the weighted spectrum; may be NULL).  imod 0, 1, 3 and 18 (disc
+ *   for the weighted spectrum; may be NULL).  imod 0, 1, 3, 18 (disc
  *   integrated, cirsradg_wave.f:1116-1218: needs radg_register_e3 and
- *   tsurf > 0; its surface term carries no Jacobian, as in the reference).
+ *   tsurf > 0; its surface term carries no Jacobian, as in the reference)
+ *   and the SCR paths 13, 14 (cells: the first layer's 1 - e^-tau,
+ *   (1 + e^-tau)/2) and 8 (layinc(1) a cell times layinc(2) an atmosphere,
+ *   both earlier paths; its Jacobian rows are nlayin(1) on path 1's layers,
+ *   :1037-1102, 1251-1252, map2pro.f:52-57).
  * Dust (cirsradg_wave.f:550-577), ncont = 0 for none: cont[L][ncont]
  *   particles/cm2, cross-section grid vsec[nsec] (ascending cm-1) and
  *   extinction xsec[ncont][nsec] (cm2 per particle; nsec = 1: constant).

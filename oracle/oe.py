@@ -118,8 +118,9 @@ def oe_step(kk, sa, se_var, xa, xn, y, yn):
                 chisq=chisq)
 
 
-def map2pro(dq, nlayin, layinc, dam, dte, ngas, dco=None, dfp=None, ncont=None):
-    """map2pro.f:49-84 for imod != 8.  ``dq`` [npath, L, nparam] (layer
+def map2pro(dq, nlayin, layinc, dam, dte, ngas, dco=None, dfp=None, ncont=None, imod=None):
+    """map2pro.f:49-84.  An imod 8 path (``imod`` [npath], default none) is
+    mapped over nlayin(1) rows on path 1's layers (:52-57).  ``dq`` [npath, L, nparam] (layer
     Jacobian, the cirsradg_wave doutputdq order: ngas gases, T, then dust
     types, then the para-H2 fraction), ``layinc`` [npath, L] 1-based,
     ``dam``/``dte``/``dco``/``dfp`` [L, npro]: DAM for gases, DTE for T, DCO
@@ -130,10 +131,11 @@ def map2pro(dq, nlayin, layinc, dam, dte, ngas, dco=None, dfp=None, ncont=None):
     nc = npar - ngas - 1 if ncont is None else ncont
     out = np.zeros((npath, npar, dam.shape[1]))
     for p in range(npath):
-        idx = np.asarray(layinc[p][:nlayin[p]]) - 1
+        p1 = 0 if imod is not None and imod[p] == 8 else p
+        idx = np.asarray(layinc[p1][:nlayin[p1]]) - 1
         for k in range(npar):
             d = dam if k < ngas else dte if k == ngas else dco if k - ngas - 1 < nc else dfp
-            out[p, k] = dq[p, :nlayin[p], k] @ d[idx]
+            out[p, k] = dq[p, :nlayin[p1], k] @ d[idx]
     return out
 
 

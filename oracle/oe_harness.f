@@ -91,14 +91,14 @@ C with NPAR = NGS+1+NCT+NFP for NCT dust types and NFP (0 or 1) para-H2
 C fractions; output DXC(NPTH,NVC).
 
       SUBROUTINE HMAP(NPTH, NLYR, NGS, NCT, NFP, NPR, NVC, NLAYC, LAYC,
-     1  DAMC, DTEC, DCOC, DFPC, XMAPC, DQC, DXC)
+     1  IMODC, DAMC, DTEC, DCOC, DFPC, XMAPC, DQC, DXC)
       IMPLICIT NONE
       INCLUDE 'arrdef.f'
       INCLUDE 'pathcom.f'
       INCLUDE 'laycom.f'
       INCLUDE 'laygrad.f'
       INTEGER NPTH, NLYR, NGS, NCT, NFP, NPR, NVC, I, J, K, NPAR
-      INTEGER NLAYC(NPTH), LAYC(NLYR,NPTH)
+      INTEGER NLAYC(NPTH), LAYC(NLYR,NPTH), IMODC(NPTH)
       REAL DAMC(NLYR,NPR), DTEC(NLYR,NPR)
       REAL DCOC(NLYR,NPR), DFPC(NLYR,NPR)
       REAL XMAPC(NVC,NGS+1+NCT+NFP,NPR)
@@ -115,7 +115,7 @@ C fractions; output DXC(NPTH,NVC).
       NPAR = NGS + 1 + NCT + NFP
       DO J = 1, NPTH
         NLAYIN(J) = NLAYC(J)
-        IMOD(J) = 3
+        IMOD(J) = IMODC(J)
         DO I = 1, NLYR
           LAYINC(I,J) = LAYC(I,J)
         ENDDO

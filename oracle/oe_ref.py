@@ -65,16 +65,18 @@ def oe_step(kk, sa, se_var, xa, xn, y, yn):
                 phi=phi.value, chisq=chisq.value, icheck=ichk.value)
 
 
-def map_state(dq, nlayin, layinc, dam, dte, xmap, dco=None, ncont=0, dfp=None):
+def map_state(dq, nlayin, layinc, dam, dte, xmap, dco=None, ncont=0, dfp=None, imod=None):
     """HMAP: map2pro then map2xvec for one wavenumber.  ``dq`` [npath, L,
     nparam] with nparam = ngas + 1 + ncont (+ 1 with ``dfp``), ``layinc``
     [npath, L] 1-based, ``dam``/``dte``/``dco``/``dfp`` [L, npro], ``xmap``
-    [nv, nparam, npro].  Returns doutdx [npath, nv]."""
+    [nv, nparam, npro], ``imod`` [npath] (default 3; 8 maps over path 1's
+    layers).  Returns doutdx [npath, nv]."""
     npath, L, npar = dq.shape
     npro = dam.shape[1]
     nv = xmap.shape[0]
     nl = _f(nlayin, np.int32)
     lay = _f(np.asarray(layinc).T, np.int32)          # LAYC(NLAY, NPTH)
+    im = _f(np.full(npath, 3) if imod is None else imod, np.int32)
     nfp = 0 if dfp is None else 1
     dco = np.zeros_like(dam) if dco is None else dco
     dfp = np.zeros_like(dam) if dfp is None else dfp
@@ -82,6 +84,6 @@ def map_state(dq, nlayin, layinc, dam, dte, xmap, dco=None, ncont=0, dfp=None):
     xmf, dqf = _f(xmap), _f(dq)
     dx = np.zeros((npath, nv), order="F")
     _load().hmap_(_i(npath), _i(L), _i(npar - 1 - ncont - nfp), _i(ncont), _i(nfp), _i(npro),
-                  _i(nv), _p(nl), _p(lay), _p(damf), _p(dtef), _p(dcof), _p(dfpf), _p(xmf),
+                  _i(nv), _p(nl), _p(lay), _p(im), _p(damf), _p(dtef), _p(dcof), _p(dfpf), _p(xmf),
                   _p(dqf), _p(dx))
     return np.array(dx)

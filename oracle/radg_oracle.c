@@ -1091,8 +1091,23 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
     if (r->nlayin[ip] > 2 * MAXPAT || r->nlayin[ip] < 1) return ORC_ERR_LIMITS;
     for (j = 0; j < r->nlayin[ip]; j++)
       if (r->layinc[ip * L + j] > L || r->layinc[ip * L + j] < 1) return ORC_ERR_LIMITS;
-    if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3 && r->imod[ip] != 18)
+    if (r->imod[ip] != 0 && r->imod[ip] != 1 && r->imod[ip] != 3 && r->imod[ip] != 8 &&
+        r->imod[ip] != 13 && r->imod[ip] != 14 && r->imod[ip] != 18)
       return ORC_ERR_UNSUPPORTED;
+    /* imod 8 (cirsradg_wave.f:1037-1082): the product of path layinc(1), a
+     * cell (imod 13 or 14, else the reference stops), and path layinc(2),
+     * both computed before it in the same g-ordinate; its Jacobian rows are
+     * summed over nlayin(1) (:1251-1252), which the atmosphere path must
+     * cover.  Anything else would read values the reference never wrote. */
+    if (r->imod[ip] == 8) {
+      int i1, i2;
+      if (r->nlayin[ip] < 2) return ORC_ERR_LIMITS;
+      i1 = r->layinc[ip * L] - 1;
+      i2 = r->layinc[ip * L + 1] - 1;
+      if (i1 < 0 || i2 < 0 || i1 >= ip || i2 >= ip) return ORC_ERR_UNSUPPORTED;
+      if (r->imod[i1] != 13 && r->imod[i1] != 14) return ORC_ERR_LIMITS;
+      if (r->imod[i2] == 8 || r->nlayin[0] > r->nlayin[i2]) return ORC_ERR_UNSUPPORTED;
+    }
     /* imod 18 needs /E3TABLE/ and a ground temperature (its ground term is
      * always added, and planckg at tsurf <= 0 is not a number) */
     if (r->imod[ip] == 18 && (!cia->e3tran || !cia->e3val || !(r->tsurf > 0.0)))
@@ -1291,7 +1306,24 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
         taus[j] = tautmp[li] * sc;
         for (k = 0; k < nparam; k++) dtaudq[j * nparam + k] = dtautmpdq[li * nparam + k] * sc;
       }
-      if (r->imod[ip] == 0 || r->imod[ip] == 1) {
+      if (r->imod[ip] == 13 || r->imod[ip] == 14) {
+        /* SCR sideband and wideband cell transmissions, cirsradg_wave.f:1084-1102:
+         * the first layer only, no Jacobian */
+        co = r->imod[ip] == 13 ? 1.0 - exp(-taus[0]) : 0.5 * (1.0 + exp(-taus[0]));
+      } else if (r->imod[ip] == 8) {
+        /* cirsradg_wave.f:1037-1082: cell times atmosphere, the atmosphere's
+         * Jacobian rows and surface gradient times the cell */
+        const int i1 = r->layinc[ip * L] - 1, i2 = r->layinc[ip * L + 1] - 1;
+        const double c1 = corkout[i1 * ng + ig];
+        const double *d2 = dcoutdq + ((size_t)i2 * ng + ig) * L * nparam;
+        co = c1 * corkout[i2 * ng + ig];
+        for (j = 0; j < r->nlayin[i2]; j++)
+          for (k1 = 0; k1 < r->nsw; k1++) {
+            k = r->isw[k1] - 1;
+            dco[j * nparam + k] = c1 * d2[j * nparam + k];
+          }
+        tempgtsurf[ip * ng + ig] = tempgtsurf[i2 * ng + ig] * c1;
+      } else if (r->imod[ip] == 0 || r->imod[ip] == 1) {
         /* cirsradg_wave.f:854-895 (no solar file: xfac = 1) */
         double tmp = 0.0;
         for (j = 0; j < nlays; j++) tmp += taus[j];
@@ -1516,7 +1548,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
 
   /* g-quadrature, cirsradg_wave.f:1240-1286 */
   for (ip = 0; ip < npath; ip++) {
-    const int nlay1 = r->nlayin[ip];
+    const int nlay1 = r->imod[ip] == 8 ? r->nlayin[0] : r->nlayin[ip];
     output[ip] = 0.0;
     gtsurf[ip] = 0.0;
     for (j = 0; j < L; j++)

@@ -306,8 +306,25 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   for (int p = 0; p < npath; p++) {
     const int nl = q.nlayin[p];
     if (nl < 1 || nl > L) return fail(RADG_ELIMITS, "nlayin(%d)=%d outside 1..nlayer", p + 1, nl);
-    if (q.imod[p] != 0 && q.imod[p] != 1 && q.imod[p] != 3 && q.imod[p] != 18)
-      return fail(RADG_EUNSUPPORTED, "imod(%d)=%d: only 0, 1, 3 and 18 are provided", p + 1, q.imod[p]);
+    const int im = q.imod[p];
+    if (im != 0 && im != 1 && im != 3 && im != 8 && im != 13 && im != 14 && im != 18)
+      return fail(RADG_EUNSUPPORTED, "imod(%d)=%d: only 0, 1, 3, 8, 13, 14 and 18 are provided", p + 1, im);
+    // imod 8 (cirsradg_wave.f:1037-1082): path layinc(1) must be a cell (imod
+    // 13 or 14; the reference stops otherwise), both it and the atmosphere
+    // layinc(2) must come before this path (the reference reads their values
+    // for this g-ordinate), the atmosphere not itself an imod 8 product, and
+    // it must cover the nlayin(1) rows that are summed and mapped
+    if (im == 8) {
+      if (nl < 2) return fail(RADG_ELIMITS, "imod(%d)=8 needs nlayin >= 2 (its two paths)", p + 1);
+      const int i1 = q.layinc[(long long)p * L] - 1, i2 = q.layinc[(long long)p * L + 1] - 1;
+      if (i1 < 0 || i2 < 0 || i1 >= p || i2 >= p)
+        return fail(RADG_EUNSUPPORTED, "imod(%d)=8: its paths %d and %d must precede it", p + 1, i1 + 1, i2 + 1);
+      if (q.imod[i1] != 13 && q.imod[i1] != 14)
+        return fail(RADG_EINPUT, "imod(%d)=8: path %d must be a cell (imod 13 or 14)", p + 1, i1 + 1);
+      if (q.imod[i2] == 8 || q.nlayin[0] > q.nlayin[i2])
+        return fail(RADG_EUNSUPPORTED, "imod(%d)=8: atmosphere path %d must be no product and cover nlayin(1)",
+                    p + 1, i2 + 1);
+    }
     // imod 18 interpolates E3 (e3interp.f) and always adds the surface at
     // tsurf (cirsradg_wave.f:1159,1213)
     if (q.imod[p] == 18 && !aux.e3)
@@ -468,15 +485,25 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
     PathInfo &pi = P.paths[p];
     pi.nlays = nl;
     pi.imod = q.imod[p];
+    pi.cell = pi.atm = -1;
     int j1 = (int)(0.5 * nl);
     if (j1 < 1) j1 = 1;
     const double pa = q.press[q.layinc[(long long)p * L + j1 - 1] - 1];
     const double pb = q.press[q.layinc[(long long)p * L + nl - 1] - 1];
     pi.surf = pb > pa || pi.imod == 18;  // cirsradg_wave.f:992-999; imod 18 always
-    for (int m = 0; m < nl; m++) {
-      P.layinc[(size_t)p * L + m] = q.layinc[(long long)p * L + m] - 1;
-      P.scale[(size_t)p * L + m] = q.scale[(long long)p * L + m];
-      P.emtemp[(size_t)p * L + m] = q.emtemp[(long long)p * L + m];
+    // an imod 8 path's layer list is two path numbers; its rows are
+    // nlayin(1) rows on path 1's layers, as map2pro.f:52-57 maps them
+    const int src = pi.imod == 8 ? 0 : p;
+    if (pi.imod == 8) {
+      pi.cell = q.layinc[(long long)p * L] - 1;
+      pi.atm = q.layinc[(long long)p * L + 1] - 1;
+      pi.nlays = q.nlayin[0];
+      pi.surf = 0;
+    }
+    for (int m = 0; m < pi.nlays; m++) {
+      P.layinc[(size_t)p * L + m] = q.layinc[(long long)src * L + m] - 1;
+      P.scale[(size_t)p * L + m] = q.scale[(long long)src * L + m];
+      P.emtemp[(size_t)p * L + m] = q.emtemp[(long long)src * L + m];
     }
     P.pweight[p] = q.path_weight ? q.path_weight[p] : 0.0;
   }
@@ -616,7 +643,7 @@ struct radg_batch {
   // other requests of its batch)
   struct EmClass {
     int RL, nreq;
-    bool disc;  // a member has an imod 18 path
+    bool disc;  // a member has an imod 8, 13, 14 or 18 path
     long long nB;
     const int *reqmap;
     const long long *start;
@@ -1369,7 +1396,10 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
       for (int r : pl.cls_members[c]) {
         ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
         maxnsw = std::max(maxnsw, reqs[r].nsw);
-        for (int p = 0; p < reqs[r].npath; p++) ec.disc = ec.disc || reqs[r].imod[p] == 18;
+        for (int p = 0; p < reqs[r].npath; p++) {
+          const int im = reqs[r].imod[p];
+          ec.disc = ec.disc || im == 8 || im == 13 || im == 14 || im == 18;
+        }
       }
       ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
       ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;

@@ -70,8 +70,12 @@ struct WaveGas {
 // BF16 code of k <= 0 (bf16 -infinity)
 #define RADG_BF16_NONPOS 0xFF80u
 
+// nlays: the path's layers, and the Jacobian rows it writes; for an imod 8
+// path (cell x atmosphere, cirsradg_wave.f:1037-1082) nlayin(1), the rows
+// the reference sums and maps (:1251-1252, map2pro.f:52-57), and cell/atm
+// the 0-based paths it multiplies
 struct PathInfo {
-  int nlays, imod, surf, pad;
+  int nlays, imod, surf, cell, atm, pad[3];
 };
 
 // Intermediate block per (request, wavenumber), in elements of the compute

@@ -1353,8 +1353,11 @@ __device__ __forceinline__ double wave_sum(double x) {
 }  // namespace
 
 // LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.  DISC: the form with the
-// imod 18 branch, launched only for batches that hold an imod 18 path (its
-// registers would cost the other forms occupancy).
+// imod 8, 13, 14 and 18 branches, launched only for batches that hold such a
+// path (the imod 18 registers would cost the other forms occupancy).
+// An imod 8 item (cell x atmosphere, cirsradg_wave.f:1037-1082) computes its
+// atmosphere path again, every g-ordinate's terms times the cell's
+// transmission at that g, and writes the product into its own slots.
 template <typename T, int RL, bool DISC>
 __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
@@ -1367,8 +1370,12 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const long long loc = item - a.startB[rqc];
   const int iw = (int)(loc / npath);
   const int p = (int)(loc - (long long)iw * npath);
-  const PathInfo pi = a.paths[R.path0 + p];
+  const PathInfo pi0 = a.paths[R.path0 + p];
+  const bool scr = DISC && pi0.imod == 8;
+  const int pg = scr ? pi0.atm : p;  // the path whose layers are computed
+  const PathInfo pi = scr ? a.paths[R.path0 + pg] : pi0;
   const int nlays = pi.nlays;
+  const int nrows = pi0.nlays;  // Jacobian rows written (imod 8: nlayin(1) <= nlays)
   const double v = a.vwave[R.wave0 + iw];
   const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
                    (long long)iw * inter_block(NG, L, ngas, R.nparam);
@@ -1396,13 +1403,13 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   for (int q = 0; q < RL; q++) {
     const int m = lane * RL + q;
     act[q] = m < nlays;
-    Lm[q] = act[q] ? a.layinc[R.pl0 + (long long)p * L + m] : 0;
-    sc[q] = act[q] ? (T)a.scale[R.pl0 + (long long)p * L + m] : (T)0;
+    Lm[q] = act[q] ? a.layinc[R.pl0 + (long long)pg * L + m] : 0;
+    sc[q] = act[q] ? (T)a.scale[R.pl0 + (long long)pg * L + m] : (T)0;
     tc[q] = act[q] ? tcon[Lm[q]] : (T)0;
     bb[q] = 0;
     db[q] = 0;
     if (act[q] && (pi.imod == 3 || (DISC && pi.imod == 18))) {
-      double em = a.emtemp[R.pl0 + (long long)p * L + m];
+      double em = a.emtemp[R.pl0 + (long long)pg * L + m];
       if (R.inlte) em = nlte_t_d(R.inlte, R.nlte_k1, R.nlte_a, em, a.press[R.lay0 + Lm[q]], v);
       bb[q] = planck_d(R.ispace, v, em);
       db[q] = (T)planckg_d(R.ispace, v, em);
@@ -1432,6 +1439,15 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       radground = es * planck_d(R.ispace, v, R.tsurf);
       gradground = es * planckg_d(R.ispace, v, R.tsurf);
     }
+  }
+
+  // imod 8: the cell path's one layer (imod 13 or 14, cirsradg_wave.f:1084-1102)
+  int cL = 0, cim = 0;
+  T csc = 0;
+  if (scr) {
+    cim = a.paths[R.path0 + pi0.cell].imod;
+    cL = a.layinc[R.pl0 + (long long)pi0.cell * L];
+    csc = (T)a.scale[R.pl0 + (long long)pi0.cell * L];
   }
 
   Kahan<T> rad, gts;
@@ -1539,6 +1555,13 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
         dRdtau[q] = act[q] ? (T)(xfac * trx[q] * Dm[q]) : (T)0;
         dT[q] = act[q] ? (T)(xfac * trp[q] * (double)om[q]) * db[q] : (T)0;
       }
+    } else if (DISC && (pi.imod == 13 || pi.imod == 14)) {
+      // SCR sideband and wideband cells (cirsradg_wave.f:1084-1102): the
+      // first layer's transmission, no Jacobian, no surface
+      const double e = exp(-(double)__shfl(tau[0], 0));
+      coR = (T)(pi.imod == 13 ? 1.0 - e : 0.5 * (1.0 + e));
+#pragma unroll
+      for (int q = 0; q < RL; q++) dRdtau[q] = dT[q] = (T)0;
     } else if (DISC && pi.imod == 18) {
       // disc-integrated emission (cirsradg_wave.f:1116-1218): layer j
       // weighs B_j by E3(tr_{j-1}) - E3(tr_j), the surface by E3(tr_N);
@@ -1633,6 +1656,17 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
 #pragma unroll
       for (int q = 0; q < RL; q++) dT[q] = 0;
     }
+    if (scr) {  // times the cell at this g-ordinate (cirsradg_wave.f:1063-1081)
+      const double e = exp(-(double)((tcon[cL] + ktot[(long long)g * L + cL]) * csc));
+      const double fc = cim == 13 ? 1.0 - e : 0.5 * (1.0 + e);
+      coR = (T)((double)coR * fc);
+      coG = (T)((double)coG * fc);
+#pragma unroll
+      for (int q = 0; q < RL; q++) {
+        dRdtau[q] = (T)((double)dRdtau[q] * fc);
+        dT[q] = (T)((double)dT[q] * fc);
+      }
+    }
     rad.add(coR * dg);
     gts.add(coG * dg);
     // Jacobian: dR/dq = dR/dtau * scale * (dtaucon/dq + dk/dq) (+ Planck term)
@@ -1678,7 +1712,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     const int m = lane * RL + q;
     if (m >= L) continue;
     for (int k = 0; k < nparam; k++) jac[(long long)m * nparam + k] = (T)0;
-    if (!act[q]) continue;
+    if (!act[q] || m >= nrows) continue;
     for (int k1 = 0; k1 < nsw; k1++) {
       const int k = R.isw[k1];
       jac[(long long)m * nparam + k] = acc[m * nsw + k1];

@@ -229,6 +229,45 @@ def cases():
                                         + (", CH4 and NH3 present" if iray == 4 else "")
                                         + (", CIA overlapping" if v0 < 1500 else ", no CIA"))
 
+    # SCR radiometer paths (cirsradg_wave.f:1037-1102): cells are one-layer
+    # paths (imod 13 sideband 1 - e^-tau, imod 14 wideband (1 + e^-tau)/2);
+    # an imod 8 path multiplies the cell named by its layinc(1) with the
+    # atmosphere named by layinc(2), per g-ordinate
+    # (cell layers and scales chosen for transmissions of 0.4-0.9)
+    for tag, atm, cells, cl in (("scr", 3, (13, 14), ((2, 1.0), (4, 1.0))),
+                                ("scr_trans", 0, (14, 13), ((3, 1.0), (10, 2.0)))):
+        t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+        r.imod = np.array([atm, cells[0], 8, cells[1], 8], dtype=np.int32)
+        r.nlayin = np.array([30, 1, 2, 1, 2], dtype=np.int32)
+        lay = r.layinc.copy()
+        sc = r.scale.copy()
+        (lay[1, 0], sc[1, 0]), (lay[3, 0], sc[3, 0]) = cl  # the cells' layers
+        lay[2, :2] = [2, 1]                # cell 2 x atmosphere 1
+        lay[4, :2] = [4, 1]                # cell 4 x atmosphere 1
+        r.layinc, r.scale = lay, sc
+        if atm == 0:
+            r.amount, r.pp = r.amount * 0.02, r.pp * 0.02
+        else:  # semi-transparent, with a surface gradient for the products to carry
+            r.amount, r.pp = r.amount * 0.01, r.pp * 0.01
+            r.tsurf = 150.0
+            r.vem = np.array([500.0, 2000.0])
+            r.emissivity = np.array([0.9, 0.95])
+        yield tag, t, r, (f"SCR: imod {atm} atmosphere, imod {cells[0]} and {cells[1]} cells, "
+                          "two imod 8 products")
+
+    # the imod 8 rows are nlayin(1) (cirsradg_wave.f:1251-1252): path 1 a
+    # 15-layer imod 1 path, the product's atmosphere path 2 of 30 layers
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.amount, r.pp = r.amount * 0.01, r.pp * 0.01
+    r.imod = np.array([1, 3, 14, 8, 13], dtype=np.int32)
+    r.nlayin = np.array([15, 30, 1, 2, 1], dtype=np.int32)
+    lay, sc = r.layinc.copy(), r.scale.copy()
+    lay[2, 0], sc[2, 0] = 4, 1.0
+    lay[3, :2] = [3, 2]
+    lay[4, 0], sc[4, 0] = 2, 1.0
+    r.layinc, r.scale = lay, sc
+    yield "scr_rows", t, r, "imod 8 of a 30-layer atmosphere summed over path 1's 15 rows"
+
     # N2's near-infrared CIA (n2n2cia.f on 4500-4825 cm-1, n2h2cia.f on
     # 3995-4995 cm-1): the Titan-like N2 + CH4 + H2 column at wavenumbers
     # inside, at the edges of and outside each table, in wavenumber and in

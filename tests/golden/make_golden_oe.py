@@ -17,7 +17,8 @@ Cases:
 * ``map_small``: map2pro + map2xvec for 3 paths x 20 layers x 2 gases,
   21 levels, 30 state elements;
 * ``map_dust``: the same with 2 dust types mapped through DCO;
-* ``map_para``: one dust type (DCO) and the para-H2 fraction (DFP).
+* ``map_para``: one dust type (DCO) and the para-H2 fraction (DFP);
+* ``map_scr``: path 3 an imod 8 product, mapped over path 1's layers.
 
 ``python tests/golden/make_golden_oe.py maps`` writes only the two map cases.
 """
@@ -48,9 +49,10 @@ def oe_inputs(nx, ny, seed):
     return dict(kk=kk, sa=sa, se_var=se, xa=xa, xn=xn, y=y, yn=yn)
 
 
-def map_inputs(seed=11, npath=3, L=20, ngas=2, npro=21, nv=30, ncont=0, para=False):
+def map_inputs(seed=11, npath=3, L=20, ngas=2, npro=21, nv=30, ncont=0, para=False, scr=False):
     """Layer Jacobians and maps; with ``ncont`` dust types the Jacobian and
-    xmap gain their columns and a DCO matrix is added."""
+    xmap gain their columns and a DCO matrix is added; with ``scr`` path 3
+    is an imod 8 product (mapped over path 1's 20 layers)."""
     rng = np.random.default_rng(seed)
     npar = ngas + 1 + ncont + (1 if para else 0)
     m = dict(dq=rng.normal(size=(npath, L, npar)), nlayin=np.array([20, 18, 20]),
@@ -61,6 +63,8 @@ def map_inputs(seed=11, npath=3, L=20, ngas=2, npro=21, nv=30, ncont=0, para=Fal
         m.update(dco=rng.normal(size=(L, npro)), ncont=ncont)
     if para:
         m.update(dfp=rng.normal(size=(L, npro)))
+    if scr:
+        m.update(imod=np.array([3, 3, 8]), nlayin=np.array([20, 18, 2]))
     return m
 
 
@@ -80,7 +84,8 @@ def main(only_maps=False):
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print(name, nx, ny, "phi", r["phi"])
     for name, kw in (("map_small", {}), ("map_dust", dict(seed=12, ncont=2)),
-                     ("map_para", dict(seed=13, ncont=1, para=True))):
+                     ("map_para", dict(seed=13, ncont=1, para=True)),
+                     ("map_scr", dict(seed=14, scr=True))):
         m = map_inputs(**kw)
         dx = oe_ref.map_state(**m)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), doutdx=dx, **m)

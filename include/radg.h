@@ -16,6 +16,7 @@
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
  *   radg_register_parah2     /FPTABLE/ filled by radtran/radtran/fpread_all.f:72-143
  *   radg_register_e3         /E3TABLE/ filled by radtran/radtran/e3interp.f:30-51
+ *   radg_register_solar      /SOLARDAT/ filled for radtran/cirsrad/get_solar_wave.f:1-46
  *   radg_register_n2nir      the tables inside radtran/radtran/n2n2cia.f:7-47 and
  *                            n2h2cia.f:3-80 (nciacon.f:287-296, nparacon_all.f:312-320)
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
@@ -151,10 +152,19 @@ int radg_register_n2nir(int n1, const double *wn1, const double *k1, int n2, con
 int radg_set_nlte(int flag, double k1, double a);
 
 /* /SOLARDAT/ iform and iread for the fixed-argument entry points (which take
- * radius1 and radextra as arguments).  A solar spectrum file (iread = 999),
- * which iform 1 and 4 divide or multiply by (cirsradg_wave.f:866-870,
- * 919-925), is not provided: that combination is refused. */
+ * radius1 and radextra as arguments).  iread = 999 (a stellar spectrum) must
+ * match radg_register_solar having been called, and the reverse. */
 int radg_set_iform(int iform, int iread);
+
+/* /SOLARDAT/ swave, srad, nspt: the stellar spectrum get_solar_wave.f reads
+ * (the reference's iread = 999), wave[npt] monotonic in the calculation's
+ * units, rad[npt] its total power.  With it, iform 1 divides imod 3 and 18
+ * paths by the spectrum at each wavenumber (cirsradg_wave.f:921-925,
+ * 1144-1149) and iform 4 multiplies imod 0 transmissions by its flux at the
+ * request's dist AU (:866-869; get_solar_wave.f:19-46: interp.f inside the
+ * range, zero outside, over 4 pi (dist AU)^2 for dist > 0).  Batches use the
+ * spectrum registered when they are created; npt = 0 clears it. */
+int radg_register_solar(int npt, const double *wave, const double *rad);
 
 /* ------------------------------------------------ Fortran argument surface */
 
@@ -253,6 +263,8 @@ typedef struct radg_request {
   const double *hfp;         /* [L] para-H2 fractions when flagh2p = 1 (nparacon_all.f;
                                 < 0: the table's last fraction); fpara is then the last
                                 Jacobian parameter */
+  double dist;               /* AU from the star: iform 4 scales imod 0 by the stellar flux
+                                there, with a stellar spectrum registered */
 } radg_request;
 
 typedef struct radg_batch radg_batch;

@@ -55,7 +55,8 @@ class _CIA(C.Structure):
                 ("fp", C.c_void_p), ("kh2h2", C.c_void_p), ("kh2he", C.c_void_p),
                 ("kcia5", C.c_void_p), ("e3tran", C.c_void_p), ("e3val", C.c_void_p),
                 ("n1", C.c_int), ("n2", C.c_int), ("wn1", C.c_void_p), ("k1", C.c_void_p),
-                ("wn2", C.c_void_p), ("k2", C.c_void_p)]
+                ("wn2", C.c_void_p), ("k2", C.c_void_p), ("nsol", C.c_int),
+                ("swave", C.c_void_p), ("srad", C.c_void_p)]
 
 
 class _REQ(C.Structure):
@@ -118,6 +119,11 @@ class _Pack:
             self.keep += arrs
             self.cia.n1, self.cia.n2 = len(arrs[0]), len(arrs[2])
             self.cia.wn1, self.cia.k1, self.cia.wn2, self.cia.k2 = [a.ctypes.data for a in arrs]
+        sol = getattr(cia, "solar", None)
+        if sol is not None:
+            sw, sr = _d(sol.wave), _d(sol.rad)
+            self.keep += [sw, sr]
+            self.cia.nsol, self.cia.swave, self.cia.srad = len(sw), sw.ctypes.data, sr.ctypes.data
         L = req.nlayer
         hfp = req.hfp if req.hfp is not None else np.zeros(L)
         f = [_d(req.press), _d(req.temp), _d(req.delh), _d(hfp), _d(req.pp),

@@ -507,6 +507,23 @@ static void orc_add_n2nir(const orc_cia *cia, double v, double qh2, double qn2, 
   dabsorb[0] += qn2 * an2h2;
 }
 
+/* get_solar_wave.f:19-46: interp.f on the stellar spectrum, zero outside
+ * its range, over 4 pi (dist AU)^2 for dist > 0 (constdef.f's AU and pi) */
+static double orc_solar(const orc_cia *cia, double x, double dist) {
+  const int n = cia->nsol;
+  double y = orc_interp(cia->swave, cia->srad, n, x);
+  if (cia->swave[0] < cia->swave[n - 1]) {
+    if (x < cia->swave[0] || x > cia->swave[n - 1]) y = 0.0;
+  } else if (x > cia->swave[0] || x < cia->swave[n - 1]) {
+    y = 0.0;
+  }
+  if (dist > 0.0) {
+    const double au = 1.49598e13, area = 4 * PI_REF * (dist * au) * (dist * au);
+    return y / area;
+  }
+  return y;
+}
+
 /* co2cia.f */
 static double orc_co2cia(double v0) {
   double xl = 1e4 / v0, a = 0.0;
@@ -1324,16 +1341,19 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
           }
         tempgtsurf[ip * ng + ig] = tempgtsurf[i2 * ng + ig] * c1;
       } else if (r->imod[ip] == 0 || r->imod[ip] == 1) {
-        /* cirsradg_wave.f:854-895 (no solar file: xfac = 1) */
+        /* cirsradg_wave.f:854-895; imod 0 with a stellar spectrum and iform 4
+         * is times the stellar flux at dist (:862-880) */
         double tmp = 0.0;
         for (j = 0; j < nlays; j++) tmp += taus[j];
         if (r->imod[ip] == 0) {
+          const double xf0 = (cia->nsol > 0 && r->iform == 4) ? orc_solar(cia, vwave, r->dist) : 1.0;
           co = orc_dpexp(-tmp);
           for (j = 0; j < nlays; j++)
             for (k1 = 0; k1 < r->nsw; k1++) {
               k = r->isw[k1] - 1;
-              dco[j * nparam + k] = -tmp * co * dtaudq[j * nparam + k];
+              dco[j * nparam + k] = -tmp * co * dtaudq[j * nparam + k] * xf0;
             }
+          co = co * xf0;
         } else {
           co = 1.0 - orc_dpexp(-tmp);
           for (j = 0; j < nlays; j++)
@@ -1352,6 +1372,9 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
         if (r->iform == 1 || r->iform == 3)
           xfac = xfac * 4. * PI_REF * ((r->radius1 + r->radextra) * 1e5) *
                  ((r->radius1 + r->radextra) * 1e5);
+        /* with a stellar spectrum, iform 1 divides by its total power
+         * (cirsradg_wave.f:921-925, 1144-1149: xdist = -1) */
+        if (cia->nsol > 0 && r->iform == 1) xfac = xfac / orc_solar(cia, vwave, -1.0);
         if (r->iform == 3) xfac = xfac * 1e-18;
         if (ig == 0) {
           for (j = 0; j < nlays; j++) {
@@ -1446,6 +1469,9 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
         if (r->iform == 1 || r->iform == 3)
           xfac = xfac * PI_REF * 4. * PI_REF * ((r->radius1 + r->radextra) * 1e5) *
                  ((r->radius1 + r->radextra) * 1e5);
+        /* with a stellar spectrum, iform 1 divides by its total power
+         * (cirsradg_wave.f:921-925, 1144-1149: xdist = -1) */
+        if (cia->nsol > 0 && r->iform == 1) xfac = xfac / orc_solar(cia, vwave, -1.0);
         if (r->iform == 3) xfac = xfac * 1e-18;
         double trold = 1.0, tr, tlayer, radground, gradground;
         int jm = (int)(0.5 * nlays);

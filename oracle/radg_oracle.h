@@ -45,6 +45,10 @@ typedef struct orc_cia {
    * (n1 = n2 = 0: none; then N2 is refused at 3995-4995 cm-1) */
   int n1, n2;
   const double *wn1, *k1, *wn2, *k2;
+  /* /SOLARDAT/ stellar spectrum (iread = 999 when nsol > 0): swave[nsol],
+   * srad[nsol] (get_solar_wave.f) */
+  int nsol;
+  const double *swave, *srad;
 } orc_cia;
 
 /* The argument surface of cirsradg_wave (cirsradg_wave.f:1-5), compact C

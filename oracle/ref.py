@@ -130,6 +130,12 @@ class RefRunner:
             self.lib.hsetpara_(*[_p(a) for a in self._para])
         if req.iform:
             self.lib.hsetiform_(_p(np.array([req.iform], dtype=np.int32)))
+        sol = getattr(cia, "solar", None)
+        if sol is not None:  # after HSETIFORM, which clears iread
+            self._sol = [np.ascontiguousarray(sol.wave, dtype=np.float64),
+                         np.ascontiguousarray(sol.rad, dtype=np.float64)]
+            self.lib.hsetsolar_(_p(np.array([len(self._sol[0])], dtype=np.int32)),
+                                _p(self._sol[0]), _p(self._sol[1]))
         if req.inlte:
             self.lib.hsetnlte_(_p(np.array([req.inlte], dtype=np.int32)),
                                _p(np.array([req.nlte_k1], dtype=rdt)),

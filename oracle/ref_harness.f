@@ -118,6 +118,24 @@ C     /SOLARDAT/ (cirsradg_wave.f:308-309): IFORM, with no solar file
       RETURN
       END
 
+C     /SOLARDAT/ with a stellar spectrum (iread = 999), as the caller's
+C     solar-file read leaves it for get_solar_wave.f: N points W, R
+      SUBROUTINE HSETSOLAR(N, W, R)
+      IMPLICIT NONE
+      INCLUDE 'arrdef.f'
+      INTEGER N, I, IREAD, IFORM, NSPT
+      DOUBLE PRECISION W(N), R(N)
+      REAL SOLRAD, SWAVE(MAXBIN), SRAD(MAXBIN)
+      COMMON /SOLARDAT/ IREAD, IFORM, SOLRAD, SWAVE, SRAD, NSPT
+      IREAD = 999
+      NSPT = N
+      DO I = 1, N
+        SWAVE(I) = W(I)
+        SRAD(I) = R(I)
+      ENDDO
+      RETURN
+      END
+
 C     /FPTABLE/ (ciacom.f) in memory, so fpread_all.f reads no file:
 C     NP para fractions FPV, KH2H2/KH2HE(NP,NUMT,NUMWN), KC5(5,NUMT,NUMWN)
       SUBROUTINE HSETPARA(DNU, TEMPS, NP, FPV, KHH, KHE, KC5)

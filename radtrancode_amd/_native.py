@@ -24,6 +24,7 @@ EXPORTS = [
     "radg_set_mode", "radg_get_mode", "radg_last_error", "radg_register_ktables",
     "radg_read_klist", "radg_register_cia", "radg_register_dust", "radg_set_nlte",
     "radg_set_iform", "radg_register_parah2", "radg_register_e3", "radg_register_n2nir",
+    "radg_register_solar",
     "cirsradg_wave_",
     "radg_cirsradg_status",
     "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",
@@ -58,7 +59,7 @@ class Request(C.Structure):
                 ("vsec", C.c_void_p), ("xsec", C.c_void_p),
                 ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double),
                 ("iform", C.c_int), ("radius1", C.c_double), ("radextra", C.c_double),
-                ("hfp", C.c_void_p)]
+                ("hfp", C.c_void_p), ("dist", C.c_double)]
 
 
 _lib = None
@@ -91,6 +92,7 @@ def lib():
     L.radg_register_parah2.argtypes = [C.c_double, vp, ip, vp, vp, vp, vp]
     L.radg_register_e3.argtypes = [ip, vp, vp]
     L.radg_register_n2nir.argtypes = [ip, vp, vp, ip, vp, vp]
+    L.radg_register_solar.argtypes = [ip, vp, vp]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
     L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]
     L.radg_batch_bytes.argtypes = [ip, vp, ip, C.POINTER(C.c_size_t)]

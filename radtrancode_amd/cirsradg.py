@@ -92,6 +92,20 @@ def register_cia(cia) -> None:
     register_parah2(getattr(cia, "para", None))
     register_e3(getattr(cia, "e3", None))
     register_n2nir(getattr(cia, "n2nir", None))
+    register_solar(getattr(cia, "solar", None))
+
+
+def register_solar(s) -> None:
+    """``/SOLARDAT/``'s stellar spectrum (iread = 999) from a
+    ``synth.SolarSpectrum`` (None clears it)."""
+    if s is None:
+        check(lib().radg_register_solar(0, None, None))
+        return
+    w = np.ascontiguousarray(s.wave, dtype=np.float64)
+    r = np.ascontiguousarray(s.rad, dtype=np.float64)
+    if w.shape != r.shape or w.ndim != 1:
+        raise ValueError("stellar spectrum: wave and rad must be 1-D of one length")
+    check(lib().radg_register_solar(int(w.shape[0]), _vp(w), _vp(r)))
 
 
 def register_n2nir(t) -> None:
@@ -148,8 +162,8 @@ def register_dust(vsec=None, xsec=None) -> None:
 
 
 def set_iform(iform=0, iread=0) -> None:
-    """``/SOLARDAT/`` iform (and iread, which must not be 999 with iform 1
-    or 4) for :func:`cirsradg_wave`."""
+    """``/SOLARDAT/`` iform and iread for :func:`cirsradg_wave`; iread = 999
+    must match a registered stellar spectrum (:func:`register_solar`)."""
     check(lib().radg_set_iform(int(iform), int(iread)))
 
 
@@ -186,7 +200,7 @@ class _ReqArrays:
             nc, len(dust[1]) if nc else 0, *([a.ctypes.data for a in dust] if nc else [None] * 3),
             int(req.inlte), float(req.nlte_k1), float(req.nlte_a),
             int(req.iform), float(req.radius1), float(req.radextra),
-            None if self.hfp is None else self.hfp.ctypes.data)
+            None if self.hfp is None else self.hfp.ctypes.data, float(req.dist))
 
 
 class StateMap(C.Structure):

@@ -135,11 +135,18 @@ struct N2Nir {
   std::vector<double> wn1, k1, wn2, k2;  // N2-N2, N2-H2
 };
 
+// /SOLARDAT/ swave, srad, nspt (get_solar_wave.f): a registered spectrum is
+// the reference's iread = 999
+struct Solar {
+  std::vector<double> wave, rad;
+};
+
 // The tables beside the k-tables and CIA that some requests need
 struct AuxTables {
   std::shared_ptr<ParaCIA> para;
   std::shared_ptr<E3Tab> e3;
   std::shared_ptr<N2Nir> n2nir;
+  std::shared_ptr<Solar> solar;
 };
 
 std::shared_ptr<TableSet> g_tables;
@@ -147,6 +154,25 @@ std::shared_ptr<CIA> g_cia;
 std::shared_ptr<ParaCIA> g_para;
 std::shared_ptr<E3Tab> g_e3;
 std::shared_ptr<N2Nir> g_n2nir;
+
+std::shared_ptr<Solar> g_solar;
+int g_iread = 0;
+
+// get_solar_wave.f:19-40 without the distance: interp.f on the spectrum,
+// zero outside its range (ascending or descending)
+double solar_at(const Solar &s, double x) {
+  const int n = (int)s.wave.size();
+  int i = 1;  // interp.f: the first grid point above x (1-based), at least 2
+  while (i <= n && !(s.wave[i - 1] > x)) i++;
+  i = std::max(2, std::min(i, n));
+  double y = s.rad[i - 2] + (s.rad[i - 1] - s.rad[i - 2]) * (x - s.wave[i - 2]) / (s.wave[i - 1] - s.wave[i - 2]);
+  if (s.wave[0] < s.wave[n - 1]) {
+    if (x < s.wave[0] || x > s.wave[n - 1]) y = 0.0;
+  } else if (x > s.wave[0] || x < s.wave[n - 1]) {
+    y = 0.0;
+  }
+  return y;
+}
 
 // verint.f:24-48: linear interpolation in the interval above the first grid
 // point past xin (ascending or descending), the end intervals extrapolated;
@@ -245,6 +271,7 @@ struct Packed {
   std::vector<double> amo, press, scale, emtemp, vwave, esurf, pweight;
   std::vector<double> dext, cont;  // dust: [nwave][ncont] extinction, [L][ncont] columns
   std::vector<double> n2nir;       // [nwave][2]: N2-N2, N2-H2 near-infrared CIA (cm-1 amagat-2)
+  std::vector<double> solar;       // [nwave]: the registered stellar spectrum at vwave
   std::vector<PathInfo> paths;
   std::vector<int> layinc;
   std::vector<WaveGas> wg;
@@ -371,6 +398,15 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
     }
     if (q.iform == 3) x = x * 1e-18;
     d.xfac18 = x * 2.0 * pi;
+  }
+  // with a stellar spectrum (iread = 999): iform 1 divides imod 3 and 18
+  // paths by it (cirsradg_wave.f:921-925,1144-1149), iform 4 multiplies imod
+  // 0 transmissions by its flux at dist AU (:866-869, get_solar_wave.f:41-46)
+  d.solar = aux.solar && (q.iform == 1 || q.iform == 4) ? q.iform : 0;
+  d.solar_area = 1.0;
+  if (d.solar == 4 && q.dist > 0.0) {
+    const double pi = 3.1415927, au = 1.49598e13;
+    d.solar_area = 4 * pi * (q.dist * au) * (q.dist * au);
   }
   d.nsw = q.nsw;
   d.nwave = nwave;
@@ -513,6 +549,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   P.esurf.resize(nwave);
   P.dext.assign((size_t)nwave * ncont, 0.0);
   P.n2nir.assign((size_t)nwave * 2, 0.0);
+  P.solar.assign((size_t)nwave, 0.0);
   P.cont.assign(q.cont, q.cont + (size_t)L * ncont);
   P.wg.resize((size_t)nwave * ngas);
   const long long ntab = (long long)np * nt * NG;
@@ -534,6 +571,13 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
       } else if (has_n2 && vn >= 3995.0 && vn <= 4995.0) {
         return fail(RADG_ENOTABLES, "N2 at %g cm-1 needs its near-infrared CIA (radg_register_n2nir)", vn);
       }
+    }
+    // the stellar spectrum at this wavenumber; iform 1 divides by it, which
+    // the reference would do at 0 too (an infinite result): refused
+    if (aux.solar && (q.iform == 1 || q.iform == 4)) {
+      P.solar[iw] = solar_at(*aux.solar, v);
+      if (q.iform == 1 && !(P.solar[iw] != 0.0 && std::isfinite(P.solar[iw])))
+        return fail(RADG_EINPUT, "iform=1: wavenumber %g outside the stellar spectrum", v);
     }
     // interpem.f:17-30 (called unconditionally by cirsradg_wave.f:381)
     const int nem = q.nem;
@@ -812,7 +856,7 @@ struct Layout {
 struct Plan {
   std::vector<Packed> packs;
   size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
-      o_vw, o_es, o_n2, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
+      o_vw, o_es, o_n2, o_sol, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
       o_err;
   size_t total = 0;
   long long nA = 0, nB = 0, nC = 0;
@@ -906,6 +950,7 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_vw = lo.take<double>(pl.nwv);
   pl.o_es = lo.take<double>(pl.nwv);
   pl.o_n2 = lo.take<double>(2 * pl.nwv);
+  pl.o_sol = lo.take<double>(pl.nwv);
   pl.o_wg = lo.take<WaveGas>(pl.nwg);
   pl.o_pw = lo.take<double>(pl.npath);
   pl.o_dx = lo.take<double>(pl.ndx);
@@ -1155,11 +1200,35 @@ int radg_register_e3(int n, const double *trans, const double *e3) {
   return RADG_OK;
 }
 
-int radg_set_iform(int iform, int iread) {
-  if (iread == 999 && (iform == 1 || iform == 4))
-    return fail(RADG_EUNSUPPORTED, "iform=%d with a solar spectrum (iread=999) is not provided", iform);
+int radg_register_solar(int npt, const double *wave, const double *rad) {
+  if (npt == 0) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_solar.reset();
+    return RADG_OK;
+  }
+  if (npt < 2 || npt > 1000000) return fail(RADG_ELIMITS, "stellar spectrum: npt=%d outside 2..1000000", npt);
+  if (!wave || !rad) return fail(RADG_EINPUT, "stellar spectrum: null array");
+  for (int i = 0; i < npt; i++)
+    if (!std::isfinite(wave[i]) || !std::isfinite(rad[i]))
+      return fail(RADG_EINPUT, "stellar spectrum: entry %d not finite", i + 1);
+  for (int i = 1; i < npt; i++)
+    if ((wave[i] - wave[i - 1]) * (wave[npt - 1] - wave[0]) <= 0)
+      return fail(RADG_EINPUT, "stellar spectrum: wavelengths must be monotonic");
+  auto s = std::make_shared<Solar>();
+  s->wave.assign(wave, wave + npt);
+  s->rad.assign(rad, rad + npt);
   std::lock_guard<std::mutex> lk(g_mu);
+  g_solar = s;
+  return RADG_OK;
+}
+
+int radg_set_iform(int iform, int iread) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  // iread = 999 is the registered stellar spectrum (radg_register_solar)
+  if ((iread == 999) != (g_solar != nullptr))
+    return fail(RADG_EINPUT, "iread=%d but a stellar spectrum is %sregistered", iread, g_solar ? "" : "not ");
   g_iform = iform;
+  g_iread = iread;
   return RADG_OK;
 }
 
@@ -1199,7 +1268,7 @@ int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
-    aux = {g_para, g_e3, g_n2nir};
+    aux = {g_para, g_e3, g_n2nir, g_solar};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
   Plan pl;
@@ -1219,7 +1288,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
     cia = g_cia;
-    aux = {g_para, g_e3, g_n2nir};
+    aux = {g_para, g_e3, g_n2nir, g_solar};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
@@ -1294,6 +1363,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     put(h, pl.o_vw + d.wave0 * sizeof(double), P.vwave.data(), P.vwave.size());
     put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
     put(h, pl.o_n2 + 2 * d.wave0 * sizeof(double), P.n2nir.data(), P.n2nir.size());
+    put(h, pl.o_sol + d.wave0 * sizeof(double), P.solar.data(), P.solar.size());
     put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
     put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
     put(h, pl.o_dx + d.dx0 * sizeof(double), P.dext.data(), P.dext.size());
@@ -1348,6 +1418,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.vwave = reinterpret_cast<const double *>(m + pl.o_vw);
   a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
   a.n2nir = reinterpret_cast<const double *>(m + pl.o_n2);
+  a.solar = reinterpret_cast<const double *>(m + pl.o_sol);
   a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
   a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
@@ -1889,7 +1960,7 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
               int flagh2p, const int *nlayin, const int *layinc, const R *scale, const int *imod,
               const int *idgas, const int *isogas, const R *emtemp, int nem, const R *vem,
               const R *emis, R tsurf, int nsw, const int *isw, int nwave, const R *vwave,
-              const R *cont, R radius1, R radextra, const R *hfp, std::vector<double> &out,
+              const R *cont, R radius1, R radextra, const R *hfp, R dist, std::vector<double> &out,
               std::vector<double> &gts, std::vector<double> &jac) {
   if (L < 1 || L > RADG_MAXLAY || npath < 1 || npath > RADG_MAXPAT || ngas < 1 || ngas > RADG_MAXGAS)
     return fail(RADG_ELIMITS, "nlayer=%d npath=%d ngas=%d exceed the array limits", L, npath, ngas);
@@ -1966,6 +2037,7 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
   q.iform = iform;
   q.radius1 = radius1;
   q.radextra = radextra;
+  q.dist = dist;
   std::vector<double> hf;
   if (flagh2p == 1) {
     hf.assign(hfp, hfp + L);
@@ -2001,12 +2073,11 @@ int spectrum_fixed(const R *dist, const int *inormal, const int *iray, const R *
                    const R *tsurf, const R *radius1, const R *radextra, const int *flagh2p,
                    const R *hfp, const int *nsw, const int *isw, int nwave, const R *vwave,
                    R *output, R *gtsurf, R *doutputdq) {
-  (void)dist;
   std::vector<double> o, g, j;
   int rc = run_fixed<R>(delh, *nlayer, *npath, *ngas, press, temp, pp, amount, *ispace, *inormal,
                         *iray, *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas,
                         emtemp, *nem, vem, emissivity, *tsurf, *nsw, isw, nwave, vwave, cont,
-                        *radius1, *radextra, hfp, o, g, j);
+                        *radius1, *radextra, hfp, *dist, o, g, j);
   if (rc) return rc;
   for (size_t i = 0; i < o.size(); i++) {
     output[i] = (R)o[i];
@@ -2030,13 +2101,13 @@ void cirsradg_wave_(const float *dist, const int *inormal, const int *iray, cons
                     float *gtsurf, const float *radius1, const int *flagh2p, const float *hfp,
                     const float *radextra, const int *nsw, const int *isw, float *output,
                     float *doutputdq) {
-  (void)dist; (void)iwave; (void)amform;
+  (void)iwave; (void)amform;
   std::vector<double> o, g, j;
   const int L = *nlayer, np_ = *npath, ng = *ngas;
   int rc = run_fixed<float>(delh, L, np_, ng, press, temp, pp, amount, *ispace, *inormal, *iray,
                             *itype, *flagh2p, nlayin, layinc, scale, imod, idgas, isogas, emtemp,
                             *nem, vem, emissivity, *tsurf, *nsw, isw, 1, vwave, cont, *radius1,
-                            *radextra, hfp, o, g, j);
+                            *radextra, hfp, *dist, o, g, j);
   g_cirs_status = rc;
   if (rc) return;
   const int nparam = (int)(j.size() / ((size_t)np_ * L));  // ngas + 1 + ncont

@@ -20,13 +20,15 @@ struct DevReq {
   int iray;                    // Rayleigh form 1..4, 0 = off (iray > 0 and itype != 0)
   int ncont;                   // dust types; nparam = ngas + 1 + ncont
   int inlte;                   // calc_nlte_t.f flag (0 = LTE)
-  int flagh2p, pad2;           // 1: para-H2 CIA (nparacon_all.f), fpara the last parameter
+  int flagh2p;                 // 1: para-H2 CIA (nparacon_all.f), fpara the last parameter
+  int solar;                   // stellar spectrum in use: 1 divides imod 3/18, 4 scales imod 0
   long long dx0;               // per-(wavenumber, dust type) extinction
   long long ct0;               // per-(layer, dust type) particle columns
   double tsurf;
   double nlte_k1, nlte_a;      // calc_nlte_t.f constants
   double xfac;                 // imod 3 output scale (iform 1, 3: total flux)
   double xfac18;               // imod 18 output scale, its 2 pi included
+  double solar_area;           // iform 4: 4 pi (dist AU)^2 (1 for dist <= 0)
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
   long long lay0;              // LayerConst / per-layer arrays
@@ -118,6 +120,7 @@ struct KernelArgs {
   const double *scale, *emtemp;
   const double *vwave, *esurf;
   const double *n2nir;  // [wavenumber][2]: N2-N2, N2-H2 near-infrared CIA
+  const double *solar;  // [wavenumber]: the stellar spectrum (iread = 999)
   const WaveGas *wg;
   const double *pweight;       // [path]
   const double *dext;          // [wavenumber][ncont] dust extinction (cm2) at vwave

@@ -1398,7 +1398,9 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   // imod 18) scales each g-ordinate's terms in FP64 before they are rounded
   // to T, as the reference applies it per term: with iform 1 it reaches 1e21
   // and an unscaled FP32 term would fall below FP32's range.
-  const double xfac = pi.imod == 3 ? R.xfac : (DISC && pi.imod == 18) ? R.xfac18 : 1.0;
+  double xfac = pi.imod == 3 ? R.xfac : (DISC && pi.imod == 18) ? R.xfac18 : 1.0;
+  if (R.solar == 1 && (pi.imod == 3 || pi.imod == 18)) xfac = xfac / a.solar[R.wave0 + iw];
+  if (R.solar == 4 && pi.imod == 0) xfac = a.solar[R.wave0 + iw] / R.solar_area;
 #pragma unroll
   for (int q = 0; q < RL; q++) {
     const int m = lane * RL + q;
@@ -1644,9 +1646,9 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       const double tmp = wave_sum(part);
       const double e = (double)dpexp((T)-tmp);
       if (pi.imod == 0) {
-        coR = (T)e;
+        coR = (T)(e * xfac);  // xfac: the stellar flux with iform 4 (:862-880), else 1
 #pragma unroll
-        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? (T)(-tmp * e) : (T)0;
+        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? (T)(-tmp * e * xfac) : (T)0;
       } else {
         const double co = 1.0 - e;
         coR = (T)co;

@@ -17,7 +17,7 @@ module radg
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
   public :: radg_register_dust, radg_set_nlte, radg_set_iform, radg_register_parah2
-  public :: radg_register_e3, radg_register_n2nir
+  public :: radg_register_e3, radg_register_n2nir, radg_register_solar
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -39,9 +39,10 @@ module radg
     type(c_ptr) :: cont, vsec, xsec          ! cont(ncont,nlayer), vsec(nsec), xsec(nsec,ncont)
     integer(c_int) :: inlte                  ! /NLTE_FLAGS/ of calc_nlte_t.f
     real(c_double) :: nlte_k1, nlte_a
-    integer(c_int) :: iform                  ! /SOLARDAT/ iform (no solar file)
+    integer(c_int) :: iform                  ! /SOLARDAT/ iform
     real(c_double) :: radius1, radextra      ! km
     type(c_ptr) :: hfp                       ! hfp(nlayer) when flagh2p = 1
+    real(c_double) :: dist                   ! AU from the star (iform 4)
   end type
 
   ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
@@ -89,6 +90,13 @@ module radg
       real(c_double), value :: dnu
       integer(c_int), value :: npara
       real(c_double) :: temps(25), fp(*), kh2h2(*), kh2he(*), kcia5(*)
+    end function
+
+    ! /SOLARDAT/ stellar spectrum (iread = 999): npt wavelengths and powers; 0 clears
+    integer(c_int) function radg_register_solar(npt, wave, rad) bind(C, name="radg_register_solar")
+      import :: c_int, c_double
+      integer(c_int), value :: npt
+      real(c_double) :: wave(*), rad(*)
     end function
 
     ! N2's near-infrared CIA: n2n2cia/n2h2cia values at their knots; 0, 0 clears

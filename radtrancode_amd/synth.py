@@ -89,6 +89,25 @@ class CIATable:
     para: Optional["ParaCIATable"] = None
     e3: Optional["E3Table"] = None
     n2nir: Optional["N2NirTable"] = None
+    solar: Optional["SolarSpectrum"] = None
+
+
+@dataclass
+class SolarSpectrum:
+    """``/SOLARDAT/``'s stellar spectrum (get_solar_wave.f): total power
+    ``rad`` at ``wave`` in the calculation's units (iread = 999)."""
+    wave: np.ndarray
+    rad: np.ndarray
+
+
+def make_solar(v0, v1, n=120, seed=17) -> SolarSpectrum:
+    """A smooth positive stellar spectrum on [v0, v1]: a 5800 K Planck shape
+    in wavenumber scaled to the Sun's total power, with a 5% ripple."""
+    rng = np.random.default_rng(seed)
+    v = np.linspace(v0, v1, n)
+    b = 1.1911e-12 * v ** 3 / np.expm1(1.439 * v / 5800.0)
+    rad = b * 4 * np.pi ** 2 * (6.96e10) ** 2 * (1 + 0.05 * np.sin(v / 37.0 + rng.uniform(0, 6)))
+    return SolarSpectrum(wave=v, rad=rad)
 
 
 @dataclass

@@ -11,7 +11,8 @@ import os
 
 import numpy as np
 
-from radtrancode_amd.synth import CIATable, E3Table, KTable, N2NirTable, ParaCIATable, Request
+from radtrancode_amd.synth import (CIATable, E3Table, KTable, N2NirTable, ParaCIATable, Request,
+                                   SolarSpectrum)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -27,13 +28,15 @@ _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 
 
 def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = None,
-              e3: E3Table = None, n2nir: N2NirTable = None):
+              e3: E3Table = None, n2nir: N2NirTable = None, solar: SolarSpectrum = None):
     """``cia``: a CIA table of the case's own, stored with it (else the
     shared ``cia.npz`` applies); ``e3``: the E3 table an imod 18 case ran
     with; ``n2nir``: N2's near-infrared CIA a case needs."""
     d = {"note": np.array(note)}
     if e3 is not None:
         d.update(e3_trans=e3.trans, e3_val=e3.e3)
+    if solar is not None:
+        d.update(sol_wave=solar.wave, sol_rad=solar.rad)
     if n2nir is not None:
         d.update(n2n2_wn=n2nir.wn_n2n2, n2n2_k=n2nir.k_n2n2, n2h2_wn=n2nir.wn_n2h2,
                  n2h2_k=n2nir.k_n2h2)
@@ -104,6 +107,8 @@ def case_cia(path) -> CIATable:
         cia = load_cia()
     if "e3_val" in z.files:
         cia.e3 = E3Table(trans=z["e3_trans"], e3=z["e3_val"])
+    if "sol_rad" in z.files:
+        cia.solar = SolarSpectrum(wave=z["sol_wave"], rad=z["sol_rad"])
     if "n2n2_k" in z.files:
         cia.n2nir = N2NirTable(wn_n2n2=z["n2n2_wn"], k_n2n2=z["n2n2_k"], wn_n2h2=z["n2h2_wn"],
                                k_n2h2=z["n2h2_k"])

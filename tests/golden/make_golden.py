@@ -255,6 +255,20 @@ def cases():
         yield tag, t, r, (f"SCR: imod {atm} atmosphere, imod {cells[0]} and {cells[1]} cells, "
                           "two imod 8 products")
 
+    # with a stellar spectrum (iread = 999): iform 1 gives imod 3 paths as a
+    # fraction of the star's total power (cirsradg_wave.f:912-925), iform 4
+    # multiplies imod 0 transmissions by its flux at dist AU (:862-880)
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.iform, r.radius1 = 1, 71492.0
+    r.imod = np.array([3, 3, 0, 3, 1], dtype=np.int32)
+    yield "solar_ratio", t, r, "iform = 1 with a stellar spectrum: planet / star flux"
+
+    t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
+    r.amount, r.pp = r.amount * 0.02, r.pp * 0.02
+    r.iform, r.dist = 4, 9.54
+    r.imod = np.array([0, 3, 0, 1, 0], dtype=np.int32)
+    yield "solar_trans", t, r, "iform = 4 with a stellar spectrum: transmissions times the flux at 9.54 AU"
+
     # the imod 8 rows are nlayin(1) (cirsradg_wave.f:1251-1252): path 1 a
     # 15-layer imod 1 path, the product's atmosphere path 2 of 30 layers
     t, c, r = synth.config1(nwave=6, nlayer=30, nkgas=3, ng=10)
@@ -440,6 +454,8 @@ CASE_PARA = {"parah2": dict(seed=9), "parah2_dust": dict(seed=10, npara=12)}
 CASE_E3 = ("disc", "disc_iform3", "disc_nlte_dust")
 # cases with N2 at 3995-4995 cm-1, which need N2's near-infrared CIA
 CASE_N2NIR = ("titan_nir", "titan_nir_um")
+# cases run with a stellar spectrum (/SOLARDAT/, iread = 999)
+CASE_SOLAR = ("solar_ratio", "solar_trans")
 
 
 def main(names=()):
@@ -471,9 +487,14 @@ def main(names=()):
             use.e3 = e3
         if name in CASE_N2NIR:  # for the oracle and the GPU; the reference has its own
             nir = load_n2nir()
+        sol = None
+        if name in CASE_SOLAR:
+            sol = synth.make_solar(500.0, 1500.0)
+            use = copy.copy(use)
+            use.solar = sol
         outs = run_ref(t, use, r)
         path = os.path.join(HERE, f"case_{name}.npz")
-        save_case(path, t, r, outs, note, cia=own, e3=e3, n2nir=nir)
+        save_case(path, t, r, outs, note, cia=own, e3=e3, n2nir=nir, solar=sol)
         print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
               f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
 

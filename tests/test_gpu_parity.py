@@ -29,7 +29,7 @@ def test_golden_case(gpu_lib, path, mode):
     tables, req, outs, note = load_case(path)
     print(f"case {path.split('/')[-1]} mode {mode}", flush=True)
     files = np.load(path, allow_pickle=False).files
-    own = any(k in files for k in ("cia_kcia", "e3_val", "n2n2_k"))
+    own = any(k in files for k in ("cia_kcia", "e3_val", "n2n2_k", "sol_rad"))
     if own:  # a case with tables of its own (the registry is global)
         gpu_lib.register_cia(case_cia(path))
     try:
@@ -164,3 +164,34 @@ def test_nlte_k1_range_is_checked(gpu_lib):
     with pytest.raises(RadgError) as e:
         gpu_lib.spectrum(bad, mode=0)
     assert e.value.code == 4 and "k1" in str(e.value)
+
+
+def test_stellar_spectrum_rules(gpu_lib):
+    """iread = 999 is the registered stellar spectrum: radg_set_iform refuses
+    an iread that does not match it (RADG_EINPUT), and iform 1 at a
+    wavenumber off the spectrum, where the reference divides by zero, is
+    refused (RADG_EINPUT)."""
+    import copy
+    from radtrancode_amd._native import RadgError
+    path = [p for p in CASES if p.endswith("case_solar_ratio.npz")][0]
+    tables, req, _, _ = load_case(path)
+    cia = case_cia(path)
+    gpu_lib.register_ktables(tables)
+    gpu_lib.register_cia(load_cia())
+    try:
+        with pytest.raises(RadgError) as e:
+            gpu_lib.set_iform(1, 999)
+        assert e.value.code == 4
+        gpu_lib.register_cia(cia)
+        gpu_lib.set_iform(1, 999)
+        with pytest.raises(RadgError) as e:
+            gpu_lib.set_iform(1, 0)
+        assert e.value.code == 4
+        off = copy.copy(req)
+        off.vwave = req.vwave + 2000.0
+        with pytest.raises(RadgError) as e:
+            gpu_lib.spectrum(off, mode=0)
+        assert e.value.code == 4 and "stellar" in str(e.value)
+    finally:
+        gpu_lib.register_cia(load_cia())
+        gpu_lib.set_iform(0, 0)

@@ -687,7 +687,7 @@ struct radg_batch {
   // other requests of its batch)
   struct EmClass {
     int RL, nreq;
-    bool disc;  // a member has an imod 8, 13, 14 or 18 path
+    bool disc;  // a member has an imod 8, 13, 14 or 18 path or uses the stellar spectrum
     long long nB;
     const int *reqmap;
     const long long *start;
@@ -1471,6 +1471,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
           const int im = reqs[r].imod[p];
           ec.disc = ec.disc || im == 8 || im == 13 || im == 14 || im == 18;
         }
+        ec.disc = ec.disc || pl.packs[r].d.solar != 0;
       }
       ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
       ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;

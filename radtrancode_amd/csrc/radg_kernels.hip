@@ -1352,13 +1352,14 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 }  // namespace
 
-// LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.  DISC: the form with the
-// imod 8, 13, 14 and 18 branches, launched only for batches that hold such a
-// path (the imod 18 registers would cost the other forms occupancy).
+// LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.  EXT: the form with the
+// imod 8, 13, 14 and 18 branches and the stellar-spectrum factors, launched
+// only for classes that hold such a path or request (their registers would
+// cost the other forms occupancy).
 // An imod 8 item (cell x atmosphere, cirsradg_wave.f:1037-1082) computes its
 // atmosphere path again, every g-ordinate's terms times the cell's
 // transmission at that g, and writes the product into its own slots.
-template <typename T, int RL, bool DISC>
+template <typename T, int RL, bool EXT>
 __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
   if (item >= a.nB) return;
@@ -1371,7 +1372,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const int iw = (int)(loc / npath);
   const int p = (int)(loc - (long long)iw * npath);
   const PathInfo pi0 = a.paths[R.path0 + p];
-  const bool scr = DISC && pi0.imod == 8;
+  const bool scr = EXT && pi0.imod == 8;
   const int pg = scr ? pi0.atm : p;  // the path whose layers are computed
   const PathInfo pi = scr ? a.paths[R.path0 + pg] : pi0;
   const int nlays = pi.nlays;
@@ -1398,9 +1399,9 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   // imod 18) scales each g-ordinate's terms in FP64 before they are rounded
   // to T, as the reference applies it per term: with iform 1 it reaches 1e21
   // and an unscaled FP32 term would fall below FP32's range.
-  double xfac = pi.imod == 3 ? R.xfac : (DISC && pi.imod == 18) ? R.xfac18 : 1.0;
-  if (R.solar == 1 && (pi.imod == 3 || pi.imod == 18)) xfac = xfac / a.solar[R.wave0 + iw];
-  if (R.solar == 4 && pi.imod == 0) xfac = a.solar[R.wave0 + iw] / R.solar_area;
+  double xfac = pi.imod == 3 ? R.xfac : (EXT && pi.imod == 18) ? R.xfac18 : 1.0;
+  if (EXT && R.solar == 1 && (pi.imod == 3 || pi.imod == 18)) xfac = xfac / a.solar[R.wave0 + iw];
+  if (EXT && R.solar == 4 && pi.imod == 0) xfac = a.solar[R.wave0 + iw] / R.solar_area;
 #pragma unroll
   for (int q = 0; q < RL; q++) {
     const int m = lane * RL + q;
@@ -1410,7 +1411,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
     tc[q] = act[q] ? tcon[Lm[q]] : (T)0;
     bb[q] = 0;
     db[q] = 0;
-    if (act[q] && (pi.imod == 3 || (DISC && pi.imod == 18))) {
+    if (act[q] && (pi.imod == 3 || (EXT && pi.imod == 18))) {
       double em = a.emtemp[R.pl0 + (long long)pg * L + m];
       if (R.inlte) em = nlte_t_d(R.inlte, R.nlte_k1, R.nlte_a, em, a.press[R.lay0 + Lm[q]], v);
       bb[q] = planck_d(R.ispace, v, em);
@@ -1424,7 +1425,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   // ground term (cirsradg_wave.f:992-1024; imod 18 :1159,1212-1218, where
   // tsurf > 0 always)
   double radground = 0.0, gradground = 0.0;
-  if ((pi.imod == 3 || (DISC && pi.imod == 18)) && pi.surf) {
+  if ((pi.imod == 3 || (EXT && pi.imod == 18)) && pi.surf) {
     if (R.tsurf_le0) {
       const int ob = (nlays - 1) / RL, oq = (nlays - 1) - ob * RL;
       double b0 = 0.0, g0 = 0.0;
@@ -1557,14 +1558,14 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
         dRdtau[q] = act[q] ? (T)(xfac * trx[q] * Dm[q]) : (T)0;
         dT[q] = act[q] ? (T)(xfac * trp[q] * (double)om[q]) * db[q] : (T)0;
       }
-    } else if (DISC && (pi.imod == 13 || pi.imod == 14)) {
+    } else if (EXT && (pi.imod == 13 || pi.imod == 14)) {
       // SCR sideband and wideband cells (cirsradg_wave.f:1084-1102): the
       // first layer's transmission, no Jacobian, no surface
       const double e = exp(-(double)__shfl(tau[0], 0));
       coR = (T)(pi.imod == 13 ? 1.0 - e : 0.5 * (1.0 + e));
 #pragma unroll
       for (int q = 0; q < RL; q++) dRdtau[q] = dT[q] = (T)0;
-    } else if (DISC && pi.imod == 18) {
+    } else if (EXT && pi.imod == 18) {
       // disc-integrated emission (cirsradg_wave.f:1116-1218): layer j
       // weighs B_j by E3(tr_{j-1}) - E3(tr_j), the surface by E3(tr_N);
       // xfac carries the 2 pi.  With g_j = dE3/dtr at tr_j
@@ -1646,9 +1647,11 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       const double tmp = wave_sum(part);
       const double e = (double)dpexp((T)-tmp);
       if (pi.imod == 0) {
-        coR = (T)(e * xfac);  // xfac: the stellar flux with iform 4 (:862-880), else 1
+        // x0: the stellar flux with iform 4 (:862-880), else 1
+        const double x0 = EXT ? xfac : 1.0;
+        coR = (T)(e * x0);
 #pragma unroll
-        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? (T)(-tmp * e * xfac) : (T)0;
+        for (int q = 0; q < RL; q++) dRdtau[q] = act[q] ? (T)(-tmp * e * x0) : (T)0;
       } else {
         const double co = 1.0 - e;
         coR = (T)co;

@@ -19,6 +19,9 @@
  *   radg_register_solar      /SOLARDAT/ filled for radtran/cirsrad/get_solar_wave.f:1-46
  *   radg_register_n2nir      the tables inside radtran/radtran/n2n2cia.f:7-47 and
  *                            n2h2cia.f:3-80 (nciacon.f:287-296, nparacon_all.f:312-320)
+ *   radg_register_gascon     /GASCONDEF/ (radtran/includes/gascom.f) with the tables
+ *                            radtran/radtran/ngascon.f:86-92 (BLOCK DATA), ch4cont.f:35-43
+ *                            and nh3cont.f:35-44 read
  *   radg_register_dust       /DUST/ vsec, xsec, nsec, ncont (radtran/cirsradg/cirsradg_wave.f:
  *                            225,312) that the caller fills from its .xsc file
  *   radg_set_nlte            /NLTE_FLAGS/ inlte_flag, nlte_k1, nlte_A (cirsradg_wave.f:328)
@@ -152,8 +155,11 @@ int radg_register_n2nir(int n1, const double *wn1, const double *k1, int n2, con
 int radg_set_nlte(int flag, double k1, double a);
 
 /* /SOLARDAT/ iform and iread for the fixed-argument entry points (which take
- * radius1 and radextra as arguments).  iread = 999 (a stellar spectrum) must
- * match radg_register_solar having been called, and the reverse. */
+ * radius1 and radextra as arguments).  iread = 999 reads the spectrum
+ * radg_register_solar registered (refused without one; while it is set the
+ * spectrum cannot be cleared); any other iread leaves the spectrum unused by
+ * these entry points, as cirsradg_wave.f:866,921 test iread.  Batched
+ * requests treat a registered spectrum as iread = 999. */
 int radg_set_iform(int iform, int iread);
 
 /* /SOLARDAT/ swave, srad, nspt: the stellar spectrum get_solar_wave.f reads
@@ -165,6 +171,37 @@ int radg_set_iform(int iform, int iread);
  * range, zero outside, over 4 pi (dist AU)^2 for dist > 0).  Batches use the
  * spectrum registered when they are created; npt = 0 clears it. */
 int radg_register_solar(int npt, const double *wave, const double *rad);
+
+/* /GASCONDEF/ (radtran/includes/gascom.f) and the tables ngascon.f reads
+ * (radtran/radtran/ngascon.f:86-92,144-244, ch4cont.f, nh3cont.f), for the
+ * gas continua cirsradg_wave.f:590-611 adds per layer and gas.
+ *   ih2o = 1: the CKD H2O continuum of every gas with id 1, on nh2o >= 2
+ *     points h2o_v0 + i*h2o_dv cm-1: h2o_s296 (self, 296 K; /CH2OS0/),
+ *     h2o_s260 (self, 260 K; /CH2OS1/), h2o_f296 (foreign, 296 K;
+ *     /CH2OF0/), with /COMCON/'s c2, ts, ps;
+ *   ich4 != 0: CH4 (id 6, isotope 0/1) from Karkoschka's table, nch4
+ *     wavelengths ch4_wl (micron, non-decreasing) and coefficients ch4_k as
+ *     karkoschka.dat holds them (ch4cont.f:44-57);
+ *   inh3 != 0: NH3 (id 11, isotope 0/1) from Lutz & Owen's table
+ *     (lutzowen.dat, nh3cont.f:46-70), likewise;
+ *   ih2o = 2, io3 != 0 and iuvscat bits are accepted here but requests
+ *     whose gases they would act on are refused with RADG_EUNSUPPORTED
+ *     (ngascon.f:215-221 leaves the absorption unassigned; the ozone and UV
+ *     cross-section files are not in the reference tree).
+ * The H- continua need no flag and no table: bound-free for id 91 isotope 0
+ * (hmin_bf.f) and, with e- (id 80) present, free-free for id 48 isotope 0
+ * (hmin_ff.f), whose electron state (electron.f) requires every 80 and 91
+ * to precede every such 48 in the gas list.  Batches use the definition
+ * registered when they are created; NULL clears it (every flag 0). */
+typedef struct radg_gascon {
+  int ih2o, ich4, io3, inh3, iuvscat;
+  int nh2o;
+  double h2o_v0, h2o_dv, h2o_c2, h2o_ts, h2o_ps;
+  const double *h2o_s296, *h2o_s260, *h2o_f296;
+  int nch4, nnh3;
+  const double *ch4_wl, *ch4_k, *nh3_wl, *nh3_k;
+} radg_gascon;
+int radg_register_gascon(const radg_gascon *g);
 
 /* ------------------------------------------------ Fortran argument surface */
 

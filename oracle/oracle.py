@@ -56,7 +56,14 @@ class _CIA(C.Structure):
                 ("kcia5", C.c_void_p), ("e3tran", C.c_void_p), ("e3val", C.c_void_p),
                 ("n1", C.c_int), ("n2", C.c_int), ("wn1", C.c_void_p), ("k1", C.c_void_p),
                 ("wn2", C.c_void_p), ("k2", C.c_void_p), ("nsol", C.c_int),
-                ("swave", C.c_void_p), ("srad", C.c_void_p)]
+                ("swave", C.c_void_p), ("srad", C.c_void_p),
+                ("ih2o", C.c_int), ("ich4", C.c_int), ("io3", C.c_int), ("inh3", C.c_int),
+                ("iuvscat", C.c_int), ("nh2o", C.c_int), ("h2o_v0", C.c_double),
+                ("h2o_dv", C.c_double), ("h2o_c2", C.c_double), ("h2o_ts", C.c_double),
+                ("h2o_ps", C.c_double), ("h2o_s296", C.c_void_p), ("h2o_s260", C.c_void_p),
+                ("h2o_f296", C.c_void_p), ("nch4", C.c_int), ("nnh3", C.c_int),
+                ("ch4_wl", C.c_void_p), ("ch4_k", C.c_void_p), ("nh3_wl", C.c_void_p),
+                ("nh3_k", C.c_void_p), ("hmin_static", C.c_int)]
 
 
 class _REQ(C.Structure):
@@ -124,6 +131,27 @@ class _Pack:
             sw, sr = _d(sol.wave), _d(sol.rad)
             self.keep += [sw, sr]
             self.cia.nsol, self.cia.swave, self.cia.srad = len(sw), sw.ctypes.data, sr.ctypes.data
+        gc = getattr(cia, "gascon", None)
+        if gc is not None:
+            c = self.cia
+            c.ih2o, c.ich4, c.io3, c.inh3, c.iuvscat = (int(gc.ih2o), int(gc.ich4), int(gc.io3),
+                                                         int(gc.inh3), int(gc.iuvscat))
+            c.h2o_v0, c.h2o_dv, c.h2o_c2, c.h2o_ts, c.h2o_ps = (gc.h2o_v0, gc.h2o_dv, gc.h2o_c2,
+                                                                gc.h2o_ts, gc.h2o_ps)
+            if gc.h2o_s296 is not None:
+                arrs = [_d(gc.h2o_s296), _d(gc.h2o_s260), _d(gc.h2o_f296)]
+                self.keep += arrs
+                c.nh2o = len(arrs[0])
+                c.h2o_s296, c.h2o_s260, c.h2o_f296 = [a.ctypes.data for a in arrs]
+            if gc.ch4_wl is not None:
+                arrs = [_d(gc.ch4_wl), _d(gc.ch4_k)]
+                self.keep += arrs
+                c.nch4, (c.ch4_wl, c.ch4_k) = len(arrs[0]), [a.ctypes.data for a in arrs]
+            if gc.nh3_wl is not None:
+                arrs = [_d(gc.nh3_wl), _d(gc.nh3_k)]
+                self.keep += arrs
+                c.nnh3, (c.nh3_wl, c.nh3_k) = len(arrs[0]), [a.ctypes.data for a in arrs]
+        self.cia.hmin_static = int(getattr(cia, "hmin_static", 0))
         L = req.nlayer
         hfp = req.hfp if req.hfp is not None else np.zeros(L)
         f = [_d(req.press), _d(req.temp), _d(req.delh), _d(hfp), _d(req.pp),
@@ -214,6 +242,38 @@ def sort2g(ra):
     ib[1:] = np.arange(1, n + 1)
     lib().orc_sort2g(C.c_int(n), a.ctypes.data_as(C.c_void_p), ib.ctypes.data_as(C.c_void_p))
     return a[1:], ib[1:]
+
+
+def hmin_ff(v0, temp):
+    """hmin_ff.f with a zero starting sum: H- free-free, cm4 dyne-1."""
+    f = lib().orc_hmin_ff
+    f.restype = C.c_double
+    f.argtypes = [C.c_double, C.c_double]
+    return f(float(v0), float(temp))
+
+
+def hmin_bf(v0):
+    """hmin_bf.f with a zero starting sum: H- bound-free cross-section."""
+    f = lib().orc_hmin_bf
+    f.restype = C.c_double
+    f.argtypes = [C.c_double]
+    return f(float(v0))
+
+
+def tabcont(gascon, which, nu):
+    """ch4cont.f or nh3cont.f (``which`` 'ch4'/'nh3') on ``gascon``'s
+    table, 1e20 cm2/molecule."""
+    wl, k = (gascon.ch4_wl, gascon.ch4_k) if which == "ch4" else (gascon.nh3_wl, gascon.nh3_k)
+    wl, k = _d(wl), _d(k)
+    c = _CIA()
+    if which == "ch4":
+        c.nch4, c.ch4_wl, c.ch4_k = len(wl), wl.ctypes.data, k.ctypes.data
+    else:
+        c.nnh3, c.nh3_wl, c.nh3_k = len(wl), wl.ctypes.data, k.ctypes.data
+    f = getattr(lib(), f"orc_{which}cont")
+    f.restype = C.c_double
+    f.argtypes = [C.c_void_p, C.c_double]
+    return f(C.addressof(c), float(nu))
 
 
 def ciaread(cia, vv, temp):

@@ -16,6 +16,8 @@
  *   nciacon     radtran/radtran/nciacon.f:1-310
  *   nparacon_all radtran/radtran/nparacon_all.f:1-338 (+ fpread_all.f:168-257)
  *   co2cia      radtran/radtran/co2cia.f
+ *   ngascon     radtran/radtran/ngascon.f:136-267 (CKD H2O, ch4cont.f, nh3cont.f,
+ *               hmin_bf.f, hmin_ff.f; the /ELECTRON/ state of cirsradg_wave.f:590-611)
  *   rayleigh*   radtran/scatter/rayleighj.f, rayleighv.f, rayleigha.f,
  *               rayleighls.f (the iray branch, cirsradg_wave.f:663-691)
  *   planck      radtran/radtran/planck_wave.f:1-42, planckg_wave.f:1-40
@@ -655,6 +657,232 @@ double orc_rayleigh(int iray, double v, double p, double t, double fheh2, double
   }
 }
 
+/* --------------------------------------------------------------- ngascon */
+/* H- free-free, hmin_ff.f:44-63: the sum over Bell & Berrington's table 3a
+ * (wavelength > 0.3645 micron) or 3b (0.1823-0.3645 micron) terms; *inr = 0
+ * outside both ranges, where the reference sets summ to 0 (:61-62). */
+static double orc_hmin_ff_terms(double v0, double temp, int *inr) {
+  static const double a1[6] = {0.0, 2483.346, -3449.889, 2200.040, -696.271, 88.283};
+  static const double b1[6] = {0.0, 285.827, -1158.382, 2427.719, -1841.400, 444.517};
+  static const double c1[6] = {0.0, -2054.291, 8746.523, -13651.105, 8624.970, -1863.864};
+  static const double d1[6] = {0.0, 2827.776, -11485.632, 16755.524, -10051.530, 2095.288};
+  static const double e1[6] = {0.0, -1341.537, 5303.609, -7510.494, 4400.067, -901.788};
+  static const double f1[6] = {0.0, 208.952, -812.939, 1132.738, -655.020, 132.985};
+  static const double a2[4] = {518.1021, 473.2636, -482.2089, 115.5291};
+  static const double b2[4] = {-734.8666, 1443.4137, -737.1616, 169.6374};
+  static const double c2[4] = {1021.1775, -1977.3395, 1096.8827, -245.649};
+  static const double d2[4] = {-479.0721, 922.3575, -521.1341, 114.243};
+  static const double e2[4] = {93.1373, -178.9275, 101.7963, -21.9972};
+  static const double f2[4] = {-6.4285, 12.3600, -7.0571, 1.5097};
+  const double wv = 1e4 / v0, th = 5040.0 / temp;
+  const double *A, *B, *Cc, *D, *E, *F;
+  double s = 0.0;
+  int n, k;
+  if (wv > 0.3645) {
+    A = a1; B = b1; Cc = c1; D = d1; E = e1; F = f1; n = 6;
+  } else if (wv < 0.3645 && wv > 0.1823) {
+    A = a2; B = b2; Cc = c2; D = d2; E = e2; F = f2; n = 4;
+  } else {
+    *inr = 0;
+    return 0.0;
+  }
+  *inr = 1;
+  for (k = 0; k < n; k++)
+    s = s + pow(th, ((k + 1) + 1.0) / 2.0) *
+                (wv * wv * A[k] + B[k] + Cc[k] / wv + D[k] / (wv * wv) + E[k] / (wv * wv * wv) +
+                 F[k] / ((wv * wv) * (wv * wv)));
+  return s;
+}
+
+/* hmin_ff.f:65-67 */
+static double orc_hmin_ff_value(double summ) { return summ < 0.0 ? 0.0 : 1e-29 * summ; }
+
+/* H- bound-free, hmin_bf.f:23-33: the sum of the six terms below the
+ * photo-detachment threshold lambda0 = 1.6419 micron (*inr = 0 above it,
+ * where summ is set to 0), and sig = 1e-18 wv^3 (1/wv - 1/lambda0)^1.5
+ * (:34). */
+static double orc_hmin_bf_terms(double v0, int *inr, double *sig) {
+  static const double coeff[6] = {152.519, 49.534, -118.858, 92.536, -34.194, 4.982};
+  const double lambda0 = 1.6419, wv = 1e4 / v0;
+  double s = 0.0;
+  int i;
+  if (!(wv < lambda0)) {
+    *inr = 0;
+    *sig = 0.0;
+    return 0.0;
+  }
+  *inr = 1;
+  for (i = 0; i < 6; i++) s = s + coeff[i] * pow((1.0 / wv) - (1.0 / lambda0), ((i + 1) - 1.0) / 2.0);
+  *sig = 1e-18 * pow(wv, 3.0) * pow((1.0 / wv) - (1.0 / lambda0), 3.0 / 2.0);
+  return s;
+}
+
+double orc_hmin_ff(double v0, double temp) {
+  int inr;
+  return orc_hmin_ff_value(orc_hmin_ff_terms(v0, temp, &inr));
+}
+
+double orc_hmin_bf(double v0) {
+  int inr;
+  double sig, s = orc_hmin_bf_terms(v0, &inr, &sig);
+  return inr ? sig * s : 0.0;
+}
+
+/* ch4cont.f:44-57 and nh3cont.f:46-70: the table's coefficient at
+ * lambda = 1e4/nu micron (10 micron for nu <= 1000), 0 off the table,
+ * linear between the points locate.f brackets.  At the table's last point
+ * locate returns n, and the reference reads one element past the wavelength
+ * column; the last interval is used here. */
+static double orc_tabcont(const double *wl, const double *kt, int n, double nu) {
+  double lambda = 10.0, fr;
+  int k;
+  if (n < 2) return 0.0;
+  if (nu > 1000.0) lambda = 1e4 / nu;
+  if (lambda < wl[0] || lambda > wl[n - 1]) return 0.0;
+  k = orc_locate(wl, n, lambda);
+  if (k >= n) k = n - 1;
+  if (k < 1) k = 1;
+  fr = (lambda - wl[k - 1]) / (wl[k] - wl[k - 1]);
+  return kt[k - 1] + fr * (kt[k] - kt[k - 1]);
+}
+
+double orc_ch4cont(const orc_cia *cia, double nu) {
+  return orc_tabcont(cia->ch4_wl, cia->ch4_k, cia->nch4, nu) / 26850.0;
+}
+
+double orc_nh3cont(const orc_cia *cia, double nu) {
+  return 1000 * orc_tabcont(cia->nh3_wl, cia->nh3_k, cia->nnh3, nu) / 26850.0;
+}
+
+/* CKD 2.1 self-continuum modification factors at 700, 710, ... 1200 cm-1
+ * (ngascon.f:103-113) */
+static const double CKD_XFAC[51] = {
+    1.00000, 1.01792, 1.03767, 1.05749, 1.07730, 1.09708, 1.10489, 1.11268, 1.12047,
+    1.12822, 1.13597, 1.14367, 1.15135, 1.15904, 1.16669, 1.17431, 1.18786, 1.20134,
+    1.21479, 1.22821, 1.24158, 1.26580, 1.28991, 1.28295, 1.27600, 1.26896, 1.25550,
+    1.24213, 1.22879, 1.21560, 1.20230, 1.18162, 1.16112, 1.14063, 1.12016, 1.10195,
+    1.09207, 1.08622, 1.08105, 1.07765, 1.07398, 1.06620, 1.05791, 1.04905, 1.03976,
+    1.02981, 1.00985, 1.00000, 1.00000, 1.00000, 1.00000};
+
+/* The CKD H2O continuum, ngascon.f:144-213 */
+static double orc_h2ocont(const orc_cia *c, double ff, double amount, double ppress,
+                          double press, double temp) {
+  const double alpha2 = 200.0 * 200.0, alphs2 = 120.0 * 120.0, betas = 5.0e-6, v0s = 1310.0,
+               factrs = 0.15;
+  const double hwsqf = 330. * 330., betaf = 8.e-11, v0f = 1130., factrf = 0.97, v0f2 = 1900.,
+               hwsqf2 = 150. * 150., beta2 = 3.e-06;
+  const double vt1 = c->h2o_v0 + c->h2o_dv * (c->nh2o - 1);
+  double gr, csff60, csff96, cfff96, tfac, csfft, sfac, vs2, s1, s2, vf2, vf6, vf4, fscal;
+  int mk;
+  if (!(ff >= c->h2o_v0 && ff < vt1)) return 0.0;
+  mk = 1 + (int)((ff - c->h2o_v0) / c->h2o_dv);
+  if (mk == c->nh2o) mk = c->nh2o - 1;
+  gr = (ff - (c->h2o_v0 + c->h2o_dv * (mk - 1))) / c->h2o_dv;
+  csff60 = c->h2o_s260[mk - 1] + gr * (c->h2o_s260[mk] - c->h2o_s260[mk - 1]);
+  csff96 = c->h2o_s296[mk - 1] + gr * (c->h2o_s296[mk] - c->h2o_s296[mk - 1]);
+  cfff96 = c->h2o_f296[mk - 1] + gr * (c->h2o_f296[mk] - c->h2o_f296[mk - 1]);
+  tfac = (temp - 296.0) / (260.0 - 296.0);
+  csfft = csff96 * pow(csff60 / csff96, tfac);
+  if (ff >= 700.0 && ff < 1200.0) {
+    const int ic = (int)(0.1 * (ff - 700.0));
+    const double gc = (ff - (700.0 + ic * 10.0)) / 10.0;
+    sfac = CKD_XFAC[ic] + gc * (CKD_XFAC[ic + 1] - CKD_XFAC[ic]);
+  } else {
+    sfac = 1.0;
+  }
+  vs2 = (ff - v0s) * (ff - v0s);
+  s1 = (1.0 - 0.2333 * (alpha2 / ((ff - 1050.0) * (ff - 1050.0) + alpha2)));
+  s2 = (1.0 - factrs * (alphs2 / (vs2 + (betas * (vs2 * vs2)) + alphs2)));
+  csfft = sfac * s1 * s2 * csfft;
+  vf2 = (ff - v0f) * (ff - v0f);
+  vf6 = vf2 * vf2 * vf2;
+  fscal = (1. - factrf * (hwsqf / (vf2 + (betaf * vf6) + hwsqf)));
+  vf2 = (ff - v0f2) * (ff - v0f2);
+  vf4 = vf2 * vf2;
+  fscal = fscal * (1. - 0.6 * (hwsqf2 / (vf2 + beta2 * vf4 + hwsqf2)));
+  cfff96 = cfff96 * fscal;
+  {
+    const double c11 = 0.5 * c->h2o_c2 / temp;
+    const double a1 = ff * amount * tanh(c11 * ff);
+    const double a2 = c->h2o_ts / (temp * c->h2o_ps);
+    const double a3 = 1.0e-20 * (ppress * csfft + (press - ppress) * cfff96);
+    return a1 * a2 * a3;
+  }
+}
+
+/* /ELECTRON/ (electron.f) as cirsradg_wave.f:590-599 sets it, and the
+ * running `summ` of the H- functions (see orc_cia.hmin_static) */
+typedef struct orc_hstate {
+  double pelec, amounthmin, ppresshmin;
+  int eflag;
+  double sum_ff, sum_bf;
+} orc_hstate;
+
+/* ngascon.f:136-267 for one gas: absorption (a tau) of gas (id, iso) at
+ * wavenumber v0.  Returns ORC_ERR_UNSUPPORTED for the branches not
+ * restated: IH2O = 2 (the reference leaves ABSORB unassigned, :215-221),
+ * ozone (IO3, :238-244: its cross-section file is not in the reference
+ * tree) and the UV cross-sections (iuvscat, :274-318). */
+static int orc_ngascon(const orc_cia *c, double v0, int id, int iso, double amount,
+                       double ppress, double press, double temp, orc_hstate *hs,
+                       double *absorb) {
+  const int iso01 = iso == 0 || iso == 1;
+  *absorb = 0.0;
+  if (c->iuvscat > 0) {
+    static const int uvid[10] = {11, 28, 26, 27, 30, 6, 5, 33, 1, 34};
+    int b;
+    for (b = 0; b < 10; b++)
+      if (id == uvid[b] && (c->iuvscat & (1 << b))) return ORC_ERR_UNSUPPORTED;
+  }
+  if (id == 1 && c->ih2o == 1) {
+    if (c->nh2o < 2) return ORC_ERR_UNSUPPORTED;
+    *absorb = orc_h2ocont(c, v0, amount, ppress, press, temp);
+  } else if (id == 1 && iso01 && c->ih2o == 2) {
+    return ORC_ERR_UNSUPPORTED;
+  } else if (id == 6 && iso01 && c->ich4 != 0) {
+    if (c->nch4 < 2) return ORC_ERR_UNSUPPORTED;
+    *absorb = amount * 1e-20 * orc_ch4cont(c, v0);
+  } else if (id == 11 && iso01 && c->inh3 != 0) {
+    if (c->nnh3 < 2) return ORC_ERR_UNSUPPORTED;
+    *absorb = amount * 1e-20 * orc_nh3cont(c, v0);
+  } else if (id == 3 && iso01 && c->io3 != 0) {
+    return ORC_ERR_UNSUPPORTED;
+  } else if (id == 48 && iso == 0 && hs->eflag > 0) {
+    /* free-free: needs e- (80) and H- (91) set earlier in this layer */
+    int inr;
+    const double t = orc_hmin_ff_terms(v0, temp, &inr);
+    const double nh = (ppress * 1.013e6 * hs->pelec) / hs->ppresshmin;
+    if (c->hmin_static) hs->sum_ff = inr ? hs->sum_ff + t : 0.0;
+    else hs->sum_ff = t;
+    *absorb = hs->amounthmin * orc_hmin_ff_value(hs->sum_ff) * nh;
+  } else if (id == 91 && iso == 0) {
+    int inr;
+    double sig;
+    const double t = orc_hmin_bf_terms(v0, &inr, &sig);
+    if (c->hmin_static) hs->sum_bf = inr ? hs->sum_bf + t : 0.0;
+    else hs->sum_bf = t;
+    *absorb = amount * (inr ? sig * hs->sum_bf : 0.0);
+  }
+  return ORC_OK;
+}
+
+/* The gas order the electron state needs: with e- (80) present, H- free-free
+ * (48, isotope 0) reads the e- partial pressure and the H- (91) amount and
+ * partial pressure cirsradg_wave.f:590-599 stored for earlier gases, so
+ * every 80 and 91 must come before every such 48, and a 91 must be present;
+ * otherwise the values would be a previous layer's or call's. */
+static int orc_hmin_order_ok(int ngas, const int *idgas, const int *isogas) {
+  int k, has80 = 0, has91 = 0, first48 = -1, last8091 = -1;
+  for (k = 0; k < ngas; k++) {
+    if (idgas[k] == 80) has80 = 1;
+    if (idgas[k] == 91) has91 = 1;
+    if (idgas[k] == 80 || idgas[k] == 91) last8091 = k;
+    if (idgas[k] == 48 && isogas[k] == 0 && first48 < 0) first48 = k;
+  }
+  if (!has80 || first48 < 0) return 1;
+  return has91 && last8091 < first48;
+}
+
 /* nciacon.f:65-180 (and nparacon_all.f's copy of it): the mole fractions
  * q[5] = H2, He, N2, CH4, CO2 with their 1-based gas numbers iabs[5], the
  * zeroed dabsorb[7] and the path factor tau = height * amagat^2. */
@@ -1092,6 +1320,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
   const int L = r->nlayer, ngas = r->ngas, npath = r->npath;
   int ng = 0, nparam, ip, j, k, k1, l, ig, rc = ORC_OK;
   int jh2 = -1, jhe = -1, jch4 = -1, jnh3 = -1;
+  orc_hstate hs = {0.0, 0.0, 0.0, 0, 0.0, 0.0};
   const double *delg = NULL;
   const orc_ktable *gtab[MAXGAS];
   double esurf = 0.0, vv = vwave;
@@ -1130,6 +1359,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
     if (r->imod[ip] == 18 && (!cia->e3tran || !cia->e3val || !(r->tsurf > 0.0)))
       return ORC_ERR_UNSUPPORTED;
   }
+  if (!orc_hmin_order_ok(ngas, r->idgas, r->isogas)) return ORC_ERR_UNSUPPORTED;
   if (r->ncont < 0 || r->ncont > MAXCON || (r->ncont > 0 && (r->nsec < 1 || r->nsec > MAXSEC)))
     return ORC_ERR_LIMITS;                                        /* check_limits.f:24-33 */
   /* cirsradg_wave.f:439-453: the wavenumber must lie on the dust grid */
@@ -1237,8 +1467,25 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       taucon[j] += tau * r->cont[(size_t)j * r->ncont + k];
       dtaucondq[j * nparam + ngas + 1 + k] = tau;
     }
-    /* ngascon returns 0 for every gas: all gas-continuum flags are off
-     * (ngascon.f:72-190 with /GASCONDEF/ zero). */
+    /* gas continua, cirsradg_wave.f:590-611: the electron state first, then
+     * ngascon; d(tau)/d(amount) = absorb/amount, with no T derivative */
+    for (k = 0; k < ngas; k++) {
+      const double am = r->amount[j * ngas + k], pk = r->pp[j * ngas + k];
+      double ab;
+      if (r->idgas[k] == 80) {
+        hs.pelec = pk;
+        hs.eflag = 1;
+      }
+      if (r->idgas[k] == 91) {
+        hs.amounthmin = am;
+        hs.ppresshmin = pk;
+      }
+      rc = orc_ngascon(cia, vv, r->idgas[k], r->isogas[k], am, pk, r->press[j], r->temp[j], &hs,
+                       &ab);
+      if (rc != ORC_OK) goto done;
+      taucon[j] += ab;
+      if (ab > 0 && am > 0.0) dtaucondq[j * nparam + k] = ab / am;
+    }
     for (k = 0; k < ngas; k++) {
       aam[k] = r->amount[j * ngas + k];
       ppp[k] = r->pp[j * ngas + k];

@@ -49,6 +49,28 @@ typedef struct orc_cia {
    * srad[nsol] (get_solar_wave.f) */
   int nsol;
   const double *swave, *srad;
+  /* /GASCONDEF/ (gascom.f) and the tables ngascon.f reads.
+   *   ih2o = 1: the CKD H2O continuum on nh2o points h2o_v0 + i*h2o_dv
+   *     (/CH2OS0/ self at 296 K, /CH2OS1/ self at 260 K, /CH2OF0/ foreign
+   *     at 296 K) with /COMCON/'s c2, ts, ps;
+   *   ich4 != 0: Karkoschka's CH4 table (ch4cont.f: wavelength micron,
+   *     coefficient as karkoschka.dat holds it);
+   *   inh3 != 0: Lutz & Owen's NH3 table (nh3cont.f, lutzowen.dat);
+   *   io3, iuvscat: not restated (ORC_ERR_UNSUPPORTED where they act).
+   * hmin_static: test knob.  hmin_bf.f and hmin_ff.f add to a local
+   * `summ` they never initialise; compiled with static locals (flang
+   * -fno-automatic, ifort's default) it carries over from call to call,
+   * reset only by a call outside the function's wavelength range.  0 (the
+   * restatement): each call starts from 0.  1: the carry-over is emulated
+   * within one cirsradg_wave call, starting from 0 (a caller that resets
+   * both functions before each call sees this). */
+  int ih2o, ich4, io3, inh3, iuvscat;
+  int nh2o;
+  double h2o_v0, h2o_dv, h2o_c2, h2o_ts, h2o_ps;
+  const double *h2o_s296, *h2o_s260, *h2o_f296;
+  int nch4, nnh3;
+  const double *ch4_wl, *ch4_k, *nh3_wl, *nh3_k;
+  int hmin_static;
 } orc_cia;
 
 /* The argument surface of cirsradg_wave (cirsradg_wave.f:1-5), compact C
@@ -126,6 +148,13 @@ double orc_verint(const double *x, const double *y, int n, double xin); /* verin
 double orc_e3(const double *trans, const double *e3, double x, double *de3);
 double orc_rayleigh(int iray, double v, double p, double t, double fheh2,
                     double fch4h2, double fnh3);                    /* rayleigh*.f */
+/* H- opacity with a fresh `summ` per call: free-free at (v0 cm-1, temp K),
+ * cm4 dyne-1 (hmin_ff.f), and bound-free cross-section at v0 (hmin_bf.f) */
+double orc_hmin_ff(double v0, double temp);
+double orc_hmin_bf(double v0);
+/* ch4cont.f / nh3cont.f on the registered tables, in 1e20 cm2/molecule */
+double orc_ch4cont(const orc_cia *cia, double nu);
+double orc_nh3cont(const orc_cia *cia, double nu);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(_HERE, "_ref")
+# ch4cont.f and nh3cont.f open raddata/ files through datarchive.f, which
+# finds the tree from $RADREPO
+REF_TREE = "/root/reference"
 
 MAXLAY, MAXPAT, MAXGAS, MAXCON, MAXSEC = 400, 200, 30, 10, 16000
 NPARMAX = MAXGAS + 2 + MAXCON
@@ -79,6 +82,53 @@ def n2nir(prec, which, v):
     return float(fn(C.byref(h)))
 
 
+def _hmin_reset(lb, ct):
+    """Zero the static `summ` of HMIN_BF and HMIN_FF (REAL(8)/(4) builds
+    with static locals): a call outside each function's wavelength range
+    sets it to 0 (hmin_bf.f:31-33 above 1.6419 micron, hmin_ff.f:61-62 below
+    0.1823 micron)."""
+    bf, ff = lb.hmin_bf_, lb.hmin_ff_
+    bf.restype = ff.restype = ct
+    v, t = ct(1000.0), ct(1000.0)
+    bf(C.byref(v))
+    v = ct(1.0e5)
+    ff(C.byref(v), C.byref(t))
+
+
+def hmin(prec, which, v, t=1000.0):
+    """The reference's own HMIN_BF(v) or HMIN_FF(v, t) from a zero sum."""
+    lb = _load(prec)
+    ct = C.c_double if prec == 64 else C.c_float
+    _hmin_reset(lb, ct)
+    hv, ht = ct(v), ct(t)
+    if which == "bf":
+        return float(lb.hmin_bf_(C.byref(hv)))
+    return float(lb.hmin_ff_(C.byref(hv), C.byref(ht)))
+
+
+def tabcont(prec, which, v):
+    """The reference's own CH4CONT or NH3CONT at wavenumber v (reads
+    raddata/karkoschka.dat or lutzowen.dat on first use)."""
+    os.environ.setdefault("RADREPO", REF_TREE)
+    lb = _load(prec)
+    ct = C.c_double if prec == 64 else C.c_float
+    fn = getattr(lb, {"ch4": "ch4cont_", "nh3": "nh3cont_"}[which])
+    fn.restype = ct
+    h = ct(v)
+    return float(fn(C.byref(h)))
+
+
+def h2o_tables():
+    """The CKD H2O continuum data of ngascon.f's BLOCK DATA as the REAL(8)
+    build holds it (the harness's HGETH2O): grid (vb1, vt1, dv1), npt,
+    self 296 K, self 260 K, foreign 296 K, /COMCON/ (ts, ps, c2, pi)."""
+    lb = _load(64)
+    grid, npt = np.zeros(3), np.zeros(1, np.int32)
+    s96, s60, f96, con = np.zeros(2001), np.zeros(2001), np.zeros(2001), np.zeros(4)
+    lb.hgeth2o_(_p(grid), _p(npt), _p(s96), _p(s60), _p(f96), _p(con))
+    return grid, int(npt[0]), s96, s60, f96, con
+
+
 class RefRunner:
     """One loaded reference library with tables registered for a request.
 
@@ -128,6 +178,14 @@ class RefRunner:
                           np.ascontiguousarray(pa.kh2he, dtype=rdt),
                           np.ascontiguousarray(pa.kcia5, dtype=rdt)]
             self.lib.hsetpara_(*[_p(a) for a in self._para])
+        gc = getattr(cia, "gascon", None)
+        if gc is not None:
+            os.environ.setdefault("RADREPO", REF_TREE)
+            self.lib.hsetgascon_(*[_p(np.array([v], dtype=np.int32))
+                                   for v in (gc.ih2o, gc.ich4, gc.io3, gc.inh3, gc.iuvscat)])
+        # H- (91) and H (48) run HMIN_BF/HMIN_FF, whose sums are reset
+        # before every call
+        self.hmin = bool(np.isin(req.idgas, (48, 91)).any())
         if req.iform:
             self.lib.hsetiform_(_p(np.array([req.iform], dtype=np.int32)))
         sol = getattr(cia, "solar", None)
@@ -198,6 +256,8 @@ class RefRunner:
     def call(self, iwave: int):
         """``cirsradg_wave`` for the 0-based wavenumber ``iwave``."""
         s = self.s
+        if self.hmin:
+            _hmin_reset(self.lib, C.c_double if self.prec == 64 else C.c_float)
         iw = np.array([iwave + 1], dtype=np.int32)
         vv = np.array([self.vwave[iwave]], dtype=self.rdt)
         self.lib.cirsradg_wave_(

@@ -7,7 +7,10 @@ C from arrays instead, so a caller can drive cirsradg_wave directly:
 C
 C   HRADINIT  switches off every optional physics flag
 C             (/diagnostic/, /lbltable/, /nlte_flags/, /dust/, /solardat/,
-C             /GASCONDEF/), as a NEMESIS run with zero flags would.
+C             /GASCONDEF/) and clears /ELECTRON/, as a fresh NEMESIS run
+C             with zero flags would.
+C   HSETGASCON sets the /GASCONDEF/ flags; HGETH2O copies out the CKD H2O
+C             tables ngascon.f's BLOCK DATA holds.
 C   HSETCIA   loads /CIATABLE/ as ciaread.f:63-103 would from a .tab file.
 C   HREADK    registers .kta files through the reference's read_klist.
 C   HCLOSEK   closes the units read_klist opened.
@@ -19,6 +22,7 @@ C 8 bytes, which is the FP64 form of the reference.
       IMPLICIT NONE
       INCLUDE 'arrdef.f'
       INCLUDE 'gascom.f'
+      INCLUDE 'electron.f'
       INTEGER IDIAG, IQUIET, ILBL, INLTE, NSEC, NCONT
       INTEGER IREAD, IFORM, NSPT
       REAL NLTEK1, NLTEA, VSEC(MAXSEC), XSEC(2,MAXCON,MAXSEC)
@@ -43,6 +47,52 @@ C 8 bytes, which is the FP64 form of the reference.
       IO3 = 0
       INH3 = 0
       IUVSCAT = 0
+      PELEC = 0.0
+      AMOUNTHMIN = 0.0
+      PPRESSHMIN = 0.0
+      EFLAG = 0
+      RETURN
+      END
+
+C     /GASCONDEF/ (gascom.f): the gas-continuum flags ngascon.f reads
+      SUBROUTINE HSETGASCON(JH2O, JCH4, JO3, JNH3, JUV)
+      IMPLICIT NONE
+      INCLUDE 'gascom.f'
+      INTEGER JH2O, JCH4, JO3, JNH3, JUV
+      IH2O = JH2O
+      ICH4 = JCH4
+      IO3 = JO3
+      INH3 = JNH3
+      IUVSCAT = JUV
+      RETURN
+      END
+
+C     The CKD H2O continuum tables ngascon.f's BLOCK DATA fills
+C     (/CH2OS0/ with its grid, /CH2OS1/, /CH2OF0/) and /COMCON/, copied out
+C     as data: GRID = VB1, VT1, DV1; CON = TS, PS, C2, PI
+      SUBROUTINE HGETH2O(GRID, NPT, S96, S60, F96, CON)
+      IMPLICIT NONE
+      REAL GRID(3), S96(2001), S60(2001), F96(2001), CON(4)
+      INTEGER NPT, I
+      REAL VB1, VT1, DV1, H2OS96, H2OS60, H2OF96, TS, PS, C2, PI
+      INTEGER NPT1
+      COMMON /CH2OF0/ H2OF96(2001)
+      COMMON /CH2OS0/ VB1, VT1, DV1, NPT1, H2OS96(2001)
+      COMMON /CH2OS1/ H2OS60(2001)
+      COMMON /COMCON/ TS, PS, C2, PI
+      GRID(1) = VB1
+      GRID(2) = VT1
+      GRID(3) = DV1
+      NPT = NPT1
+      DO I = 1, 2001
+        S96(I) = H2OS96(I)
+        S60(I) = H2OS60(I)
+        F96(I) = H2OF96(I)
+      ENDDO
+      CON(1) = TS
+      CON(2) = PS
+      CON(3) = C2
+      CON(4) = PI
       RETURN
       END
 

@@ -24,7 +24,7 @@ EXPORTS = [
     "radg_set_mode", "radg_get_mode", "radg_last_error", "radg_register_ktables",
     "radg_read_klist", "radg_register_cia", "radg_register_dust", "radg_set_nlte",
     "radg_set_iform", "radg_register_parah2", "radg_register_e3", "radg_register_n2nir",
-    "radg_register_solar",
+    "radg_register_solar", "radg_register_gascon",
     "cirsradg_wave_",
     "radg_cirsradg_status",
     "radg_spectrum_r4", "radg_spectrum_r8", "radg_batch_create", "radg_batch_bytes",

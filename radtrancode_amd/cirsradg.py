@@ -13,7 +13,8 @@ Reference correspondence:
   :func:`register_dust` -- ``/DUST/`` for :func:`cirsradg_wave` (batched
   requests carry their own dust arrays); :func:`register_parah2` --
   ``/FPTABLE/``; :func:`register_e3` -- ``/E3TABLE/`` for imod 18;
-  :func:`register_n2nir` -- N2's near-infrared CIA tables.
+  :func:`register_n2nir` -- N2's near-infrared CIA tables;
+  :func:`register_gascon` -- ``/GASCONDEF/`` with ngascon.f's tables.
 * :class:`Batch` -- many requests resident on the device, launched together.
 
 Errors raise :class:`RadgError` where the reference would STOP.
@@ -93,6 +94,49 @@ def register_cia(cia) -> None:
     register_e3(getattr(cia, "e3", None))
     register_n2nir(getattr(cia, "n2nir", None))
     register_solar(getattr(cia, "solar", None))
+    register_gascon(getattr(cia, "gascon", None))
+
+
+class _GasCon(C.Structure):
+    _fields_ = [("ih2o", C.c_int), ("ich4", C.c_int), ("io3", C.c_int), ("inh3", C.c_int),
+                ("iuvscat", C.c_int), ("nh2o", C.c_int), ("h2o_v0", C.c_double),
+                ("h2o_dv", C.c_double), ("h2o_c2", C.c_double), ("h2o_ts", C.c_double),
+                ("h2o_ps", C.c_double), ("h2o_s296", C.c_void_p), ("h2o_s260", C.c_void_p),
+                ("h2o_f296", C.c_void_p), ("nch4", C.c_int), ("nnh3", C.c_int),
+                ("ch4_wl", C.c_void_p), ("ch4_k", C.c_void_p), ("nh3_wl", C.c_void_p),
+                ("nh3_k", C.c_void_p)]
+
+
+def register_gascon(g) -> None:
+    """``/GASCONDEF/`` and ngascon.f's tables from a ``synth.GasCon`` (None
+    clears it: every flag 0, the H- continua still active)."""
+    if g is None:
+        check(lib().radg_register_gascon(None))
+        return
+    keep = []
+
+    def arr(x):
+        if x is None:
+            return None, 0
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data, a.shape[0]
+
+    s96, n96 = arr(g.h2o_s296)
+    s60, n60 = arr(g.h2o_s260)
+    f96, nf = arr(g.h2o_f296)
+    if not (n96 == n60 == nf):
+        raise ValueError("CKD H2O tables must have one length")
+    cw, nc = arr(g.ch4_wl)
+    ck, nck = arr(g.ch4_k)
+    nw, nn = arr(g.nh3_wl)
+    nk, nnk = arr(g.nh3_k)
+    if nc != nck or nn != nnk:
+        raise ValueError("CH4/NH3 tables: wavelengths and coefficients must match")
+    st = _GasCon(int(g.ih2o), int(g.ich4), int(g.io3), int(g.inh3), int(g.iuvscat), n96,
+                 float(g.h2o_v0), float(g.h2o_dv), float(g.h2o_c2), float(g.h2o_ts),
+                 float(g.h2o_ps), s96, s60, f96, nc, nn, cw, ck, nw, nk)
+    check(lib().radg_register_gascon(C.byref(st)))
 
 
 def register_solar(s) -> None:

@@ -141,12 +141,21 @@ struct Solar {
   std::vector<double> wave, rad;
 };
 
+// /GASCONDEF/ and the tables ngascon.f reads (radg_register_gascon)
+struct GasCon {
+  int ih2o = 0, ich4 = 0, io3 = 0, inh3 = 0, iuvscat = 0;
+  double h2o_v0 = 0, h2o_dv = 0, h2o_c2 = 0, h2o_ts = 0, h2o_ps = 0;
+  std::vector<double> s296, s260, f296;  // CKD H2O
+  std::vector<double> ch4_wl, ch4_k, nh3_wl, nh3_k;
+};
+
 // The tables beside the k-tables and CIA that some requests need
 struct AuxTables {
   std::shared_ptr<ParaCIA> para;
   std::shared_ptr<E3Tab> e3;
   std::shared_ptr<N2Nir> n2nir;
   std::shared_ptr<Solar> solar;
+  std::shared_ptr<GasCon> gascon;
 };
 
 std::shared_ptr<TableSet> g_tables;
@@ -157,16 +166,92 @@ std::shared_ptr<N2Nir> g_n2nir;
 
 std::shared_ptr<Solar> g_solar;
 int g_iread = 0;
+std::shared_ptr<GasCon> g_gascon;
+
+// ch4cont.f:44-57, nh3cont.f:46-70: the coefficient at lambda = 1e4/nu
+// micron (10 micron for nu <= 1000), 0 off the table, linear between the
+// points frecipes/locate.f brackets (the last interval at the last point)
+double tabcont(const std::vector<double> &wl, const std::vector<double> &kt, double nu) {
+  const int n = (int)wl.size();
+  const double lambda = nu > 1000.0 ? 1e4 / nu : 10.0;
+  if (n < 2 || lambda < wl[0] || lambda > wl[n - 1]) return 0.0;
+  // locate's JL: the number of points below lambda
+  int k = (int)(std::lower_bound(wl.begin(), wl.end(), lambda) - wl.begin());
+  k = std::max(1, std::min(k, n - 1));
+  const double fr = (lambda - wl[k - 1]) / (wl[k] - wl[k - 1]);
+  return kt[k - 1] + fr * (kt[k] - kt[k - 1]);
+}
+
+// hmin_bf.f:23-41 with a zero starting sum: the H- bound-free cross-section
+double hmin_bf(double v0) {
+  static const double coeff[6] = {152.519, 49.534, -118.858, 92.536, -34.194, 4.982};
+  const double lambda0 = 1.6419, wv = 1e4 / v0;
+  if (!(wv < lambda0)) return 0.0;
+  const double x = 1.0 / wv - 1.0 / lambda0;
+  double s = 0.0;
+  for (int i = 0; i < 6; i++) s += coeff[i] * std::pow(x, 0.5 * i);
+  return 1e-18 * std::pow(wv, 3.0) * std::pow(x, 1.5) * s;
+}
+
+// CKD 2.1 self-continuum modification factors at 700, 710, ... 1200 cm-1
+// (ngascon.f:103-113)
+const double CKD_XFAC[51] = {
+    1.00000, 1.01792, 1.03767, 1.05749, 1.07730, 1.09708, 1.10489, 1.11268, 1.12047,
+    1.12822, 1.13597, 1.14367, 1.15135, 1.15904, 1.16669, 1.17431, 1.18786, 1.20134,
+    1.21479, 1.22821, 1.24158, 1.26580, 1.28991, 1.28295, 1.27600, 1.26896, 1.25550,
+    1.24213, 1.22879, 1.21560, 1.20230, 1.18162, 1.16112, 1.14063, 1.12016, 1.10195,
+    1.09207, 1.08622, 1.08105, 1.07765, 1.07398, 1.06620, 1.05791, 1.04905, 1.03976,
+    1.02981, 1.00985, 1.00000, 1.00000, 1.00000, 1.00000};
+
+// The wavenumber-only factors of the CKD H2O continuum, ngascon.f:147-201:
+// out[0] = 1 in the table's range (else the continuum is 0), then the self
+// continuum at 296 and 260 K, the foreign one times its CKD 2.1 scaling,
+// and the self continuum's scaling (SFAC S1 S2)
+void h2o_factors(const GasCon &g, double ff, double *out) {
+  const int n = (int)g.s296.size();
+  const double vt1 = g.h2o_v0 + g.h2o_dv * (n - 1);
+  for (int i = 0; i < 5; i++) out[i] = 0.0;
+  if (!(ff >= g.h2o_v0 && ff < vt1)) return;
+  int mk = 1 + (int)((ff - g.h2o_v0) / g.h2o_dv);
+  if (mk == n) mk = n - 1;
+  const double gr = (ff - (g.h2o_v0 + g.h2o_dv * (mk - 1))) / g.h2o_dv;
+  auto lin = [&](const std::vector<double> &y) { return y[mk - 1] + gr * (y[mk] - y[mk - 1]); };
+  double sfac = 1.0;
+  if (ff >= 700.0 && ff < 1200.0) {
+    const int ic = (int)(0.1 * (ff - 700.0));
+    const double gc = (ff - (700.0 + ic * 10.0)) / 10.0;
+    sfac = CKD_XFAC[ic] + gc * (CKD_XFAC[ic + 1] - CKD_XFAC[ic]);
+  }
+  const double alpha2 = 200.0 * 200.0, alphs2 = 120.0 * 120.0, betas = 5.0e-6, v0s = 1310.0,
+               factrs = 0.15;
+  const double hwsqf = 330. * 330., betaf = 8.e-11, v0f = 1130., factrf = 0.97, v0f2 = 1900.,
+               hwsqf2 = 150. * 150., beta2 = 3.e-06;
+  const double vs2 = (ff - v0s) * (ff - v0s);
+  const double s1 = 1.0 - 0.2333 * (alpha2 / ((ff - 1050.0) * (ff - 1050.0) + alpha2));
+  const double s2 = 1.0 - factrs * (alphs2 / (vs2 + betas * (vs2 * vs2) + alphs2));
+  double vf2 = (ff - v0f) * (ff - v0f);
+  double fscal = 1. - factrf * (hwsqf / (vf2 + betaf * (vf2 * vf2 * vf2) + hwsqf));
+  vf2 = (ff - v0f2) * (ff - v0f2);
+  fscal = fscal * (1. - 0.6 * (hwsqf2 / (vf2 + beta2 * (vf2 * vf2) + hwsqf2)));
+  out[0] = 1.0;
+  out[1] = lin(g.s296);
+  out[2] = lin(g.s260);
+  out[3] = lin(g.f296) * fscal;
+  out[4] = sfac * s1 * s2;
+}
 
 // get_solar_wave.f:19-40 without the distance: interp.f on the spectrum,
 // zero outside its range (ascending or descending)
 double solar_at(const Solar &s, double x) {
   const int n = (int)s.wave.size();
-  int i = 1;  // interp.f: the first grid point above x (1-based), at least 2
-  while (i <= n && !(s.wave[i - 1] > x)) i++;
+  const bool asc = s.wave[0] < s.wave[n - 1];
+  // interp.f: the first grid point past x (above it on an ascending grid,
+  // below it on a descending one; 1-based), N if none, at least 2
+  int i = 1;
+  while (i <= n && !(asc ? s.wave[i - 1] > x : s.wave[i - 1] < x)) i++;
   i = std::max(2, std::min(i, n));
   double y = s.rad[i - 2] + (s.rad[i - 1] - s.rad[i - 2]) * (x - s.wave[i - 2]) / (s.wave[i - 1] - s.wave[i - 2]);
-  if (s.wave[0] < s.wave[n - 1]) {
+  if (asc) {
     if (x < s.wave[0] || x > s.wave[n - 1]) y = 0.0;
   } else if (x > s.wave[0] || x < s.wave[n - 1]) {
     y = 0.0;
@@ -272,6 +357,8 @@ struct Packed {
   std::vector<double> dext, cont;  // dust: [nwave][ncont] extinction, [L][ncont] columns
   std::vector<double> n2nir;       // [nwave][2]: N2-N2, N2-H2 near-infrared CIA (cm-1 amagat-2)
   std::vector<double> solar;       // [nwave]: the registered stellar spectrum at vwave
+  std::vector<double> ppg;         // [L][ngas] partial pressures (gas continua)
+  std::vector<double> gcw;         // [nwave][RADG_GCW] gas-continuum factors (or empty)
   std::vector<PathInfo> paths;
   std::vector<int> layinc;
   std::vector<WaveGas> wg;
@@ -420,6 +507,72 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   for (int k = 0; k < RADG_MAXPARAM; k++) d.isw[k] = k < q.nsw ? q.isw[k] - 1 : -1;
   for (int l = 0; l < 5; l++) d.iabs[l] = -1;
 
+  // gas continua (cirsradg_wave.f:590-611, ngascon.f:144-318): each gas's
+  // kind under the registered /GASCONDEF/ flags.  H- free-free reads the
+  // electron state (electron.f) that earlier gases of the same layer set:
+  // e- (80) partial pressure, H- (91) amount and partial pressure.
+  const GasCon gnone;
+  const GasCon &gc = aux.gascon ? *aux.gascon : gnone;
+  bool any_gc = false;
+  {
+    int k80 = -1, k91 = -1, first48 = -1, last8091 = -1;
+    for (int k = 0; k < ngas; k++) {
+      if (q.idgas[k] == 80) k80 = last8091 = k;
+      if (q.idgas[k] == 91) k91 = last8091 = k;
+      if (q.idgas[k] == 48 && q.isogas[k] == 0 && first48 < 0) first48 = k;
+    }
+    static const int uvid[10] = {11, 28, 26, 27, 30, 6, 5, 33, 1, 34};  // ngascon.f:274-318
+    for (int k = 0; k < ngas; k++) {
+      const int id = q.idgas[k], iso = q.isogas[k];
+      const bool iso01 = iso == 0 || iso == 1;
+      signed char kind = RADG_GC_NONE;
+      for (int b = 0; b < 10 && gc.iuvscat > 0; b++)
+        if (id == uvid[b] && (gc.iuvscat & (1 << b)))
+          return fail(RADG_EUNSUPPORTED, "gas %d (id %d): UV cross-sections (iuvscat=%d) are not provided",
+                      k + 1, id, gc.iuvscat);
+      if (id == 1 && gc.ih2o == 1) {
+        if (gc.s296.size() < 2) return fail(RADG_ENOTABLES, "IH2O=1 without the CKD H2O tables");
+        kind = RADG_GC_H2O;
+      } else if (id == 1 && iso01 && gc.ih2o == 2) {
+        return fail(RADG_EUNSUPPORTED, "IH2O=2: ngascon.f leaves the H2O absorption unassigned");
+      } else if (id == 6 && iso01 && gc.ich4 != 0) {
+        if (gc.ch4_wl.size() < 2) return fail(RADG_ENOTABLES, "ICH4 set without Karkoschka's CH4 table");
+        kind = RADG_GC_CH4;
+      } else if (id == 11 && iso01 && gc.inh3 != 0) {
+        if (gc.nh3_wl.size() < 2) return fail(RADG_ENOTABLES, "INH3 set without Lutz & Owen's NH3 table");
+        kind = RADG_GC_NH3;
+      } else if (id == 3 && iso01 && gc.io3 != 0) {
+        return fail(RADG_EUNSUPPORTED, "IO3 set: the ozone UV cross-sections are not provided");
+      } else if (id == 48 && iso == 0 && k80 >= 0) {
+        if (k91 < 0 || last8091 > first48)
+          return fail(RADG_EUNSUPPORTED,
+                      "H- free-free (gas 48) needs e- (80) and H- (91) before it in the gas list");
+        for (int j = 0; j < L; j++)
+          if (!(q.pp[j * ngas + k91] > 0))
+            return fail(RADG_EINPUT, "layer %d: H- free-free divides by the H- partial pressure %g", j + 1,
+                        q.pp[j * ngas + k91]);
+        kind = RADG_GC_HFF;
+      } else if (id == 91 && iso == 0) {
+        kind = RADG_GC_HBF;
+      }
+      d.gcon[k] = kind;
+      any_gc = any_gc || kind != RADG_GC_NONE;
+    }
+    for (int k = ngas; k < RADG_MAXGAS; k++) d.gcon[k] = RADG_GC_NONE;
+    d.ke80 = (short)std::max(k80, 0);
+    d.ke91 = (short)std::max(k91, 0);
+  }
+  P.ppg.assign(q.pp, q.pp + (size_t)L * ngas);
+  P.gcw.assign(any_gc ? (size_t)nwave * RADG_GCW : 0, 0.0);
+  for (int iw = 0; any_gc && iw < nwave; iw++) {
+    const double vn = q.ispace == 1 ? 1e4 / q.vwave[iw] : q.vwave[iw];
+    double *gw = P.gcw.data() + (size_t)iw * RADG_GCW;
+    if (gc.ich4 != 0) gw[RADG_GCW_CH4] = tabcont(gc.ch4_wl, gc.ch4_k, vn) / 26850.0;
+    if (gc.inh3 != 0) gw[RADG_GCW_NH3] = 1000 * tabcont(gc.nh3_wl, gc.nh3_k, vn) / 26850.0;
+    gw[RADG_GCW_HBF] = hmin_bf(vn);
+    if (gc.ih2o == 1) h2o_factors(gc, vn, gw + RADG_GCW_H2OIN);
+  }
+
   // cirsradg_wave.f:393-414: the gases whose fractions feed rayleighls (the
   // last H2, He, CH4, NH3 of isotope 0 or 1); an absent one contributes 0.
   int jh2 = -1, jhe = -1, jch4 = -1, jnh3 = -1;
@@ -545,6 +698,10 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   }
 
   // ---- per wavenumber
+  // iform 1 divides imod 3 and 18 paths by the star (cirsradg_wave.f:921-925,
+  // 1144-1149); other paths never read the spectrum
+  bool divides_by_star = false;
+  for (int p = 0; p < npath; p++) divides_by_star = divides_by_star || q.imod[p] == 3 || q.imod[p] == 18;
   P.vwave.assign(q.vwave, q.vwave + nwave);
   P.esurf.resize(nwave);
   P.dext.assign((size_t)nwave * ncont, 0.0);
@@ -576,7 +733,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
     // the reference would do at 0 too (an infinite result): refused
     if (aux.solar && (q.iform == 1 || q.iform == 4)) {
       P.solar[iw] = solar_at(*aux.solar, v);
-      if (q.iform == 1 && !(P.solar[iw] != 0.0 && std::isfinite(P.solar[iw])))
+      if (q.iform == 1 && divides_by_star && !(P.solar[iw] != 0.0 && std::isfinite(P.solar[iw])))
         return fail(RADG_EINPUT, "iform=1: wavenumber %g outside the stellar spectrum", v);
     }
     // interpem.f:17-30 (called unconditionally by cirsradg_wave.f:381)
@@ -856,12 +1013,12 @@ struct Layout {
 struct Plan {
   std::vector<Packed> packs;
   size_t o_reqs, o_sA, o_sB, o_sC, o_lay, o_amo, o_press, o_paths, o_layinc, o_scale, o_emt,
-      o_vw, o_es, o_n2, o_sol, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout, o_wgts, o_wjac,
-      o_err;
+      o_vw, o_es, o_n2, o_sol, o_ppg, o_gcw, o_wg, o_pw, o_dx, o_ct, o_inter, o_out, o_gts, o_jac, o_wout,
+      o_wgts, o_wjac, o_err;
   size_t total = 0;
   long long nA = 0, nB = 0, nC = 0;
   long long nlay = 0, nlg = 0, npath = 0, npl = 0, nwv = 0, nwg = 0, ninter = 0, nout = 0, njac = 0,
-            nwout = 0, nwjac = 0, ndx = 0, nct = 0;
+            nwout = 0, nwjac = 0, ndx = 0, nct = 0, ngcw = 0;
   std::vector<long long> sA, sB, sC;
   int maxngas = 1, maxnsw = 1, maxlays = 1;
   std::vector<int> cls_rl;
@@ -898,6 +1055,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     d.dx0 = pl.ndx;
     d.ct0 = pl.nct;
 
+    d.gcw0 = pl.packs[r].gcw.empty() ? -1 : pl.ngcw;
+    pl.ngcw += (long long)pl.packs[r].gcw.size();
     pl.ndx += (long long)q.nwave * q.ncont;
     pl.nct += (long long)q.nlayer * q.ncont;
     pl.nlay += q.nlayer;
@@ -951,6 +1110,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_es = lo.take<double>(pl.nwv);
   pl.o_n2 = lo.take<double>(2 * pl.nwv);
   pl.o_sol = lo.take<double>(pl.nwv);
+  pl.o_ppg = lo.take<double>(pl.nlg);
+  pl.o_gcw = lo.take<double>(pl.ngcw);
   pl.o_wg = lo.take<WaveGas>(pl.nwg);
   pl.o_pw = lo.take<double>(pl.npath);
   pl.o_dx = lo.take<double>(pl.ndx);
@@ -1203,6 +1364,10 @@ int radg_register_e3(int n, const double *trans, const double *e3) {
 int radg_register_solar(int npt, const double *wave, const double *rad) {
   if (npt == 0) {
     std::lock_guard<std::mutex> lk(g_mu);
+    // /SOLARDAT/ must stay consistent: iread = 999 names this spectrum
+    // (get_solar_wave.f:14-17 stops without one)
+    if (g_iread == 999)
+      return fail(RADG_EINPUT, "iread=999 is set: clear it (radg_set_iform) before the spectrum");
     g_solar.reset();
     return RADG_OK;
   }
@@ -1222,11 +1387,65 @@ int radg_register_solar(int npt, const double *wave, const double *rad) {
   return RADG_OK;
 }
 
+int radg_register_gascon(const radg_gascon *g) {
+  if (!g) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_gascon.reset();
+    return RADG_OK;
+  }
+  auto c = std::make_shared<GasCon>();
+  c->ih2o = g->ih2o;
+  c->ich4 = g->ich4;
+  c->io3 = g->io3;
+  c->inh3 = g->inh3;
+  c->iuvscat = g->iuvscat;
+  auto table = [](int n, const double *x, const double *y, const char *what, std::vector<double> &ox,
+                  std::vector<double> &oy) {
+    if (n == 0) return RADG_OK;
+    if (n < 2 || n > 1000000 || !x || !y) return fail(RADG_EINPUT, "%s table: n=%d", what, n);
+    // lutzowen.dat repeats a wavelength: non-decreasing is accepted, and
+    // the interval locate.f brackets is never a repeated pair
+    for (int i = 0; i < n; i++)
+      if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || (i > 0 && x[i] < x[i - 1]))
+        return fail(RADG_EINPUT, "%s table: entry %d not finite or decreasing", what, i + 1);
+    if (!(x[n - 1] > x[0])) return fail(RADG_EINPUT, "%s table: no wavelength range", what);
+    ox.assign(x, x + n);
+    oy.assign(y, y + n);
+    return RADG_OK;
+  };
+  int rc = table(g->nch4, g->ch4_wl, g->ch4_k, "CH4", c->ch4_wl, c->ch4_k);
+  if (!rc) rc = table(g->nnh3, g->nh3_wl, g->nh3_k, "NH3", c->nh3_wl, c->nh3_k);
+  if (rc) return rc;
+  if (g->nh2o != 0) {
+    const int n = g->nh2o;
+    if (n < 2 || n > 1000000 || !g->h2o_s296 || !g->h2o_s260 || !g->h2o_f296)
+      return fail(RADG_EINPUT, "CKD H2O tables: n=%d", n);
+    if (!(g->h2o_dv > 0) || !std::isfinite(g->h2o_v0) || !std::isfinite(g->h2o_c2) ||
+        !(g->h2o_ts > 0) || !(g->h2o_ps > 0))
+      return fail(RADG_EINPUT, "CKD H2O tables: grid or constants invalid");
+    for (int i = 0; i < n; i++)
+      if (!std::isfinite(g->h2o_s296[i]) || !std::isfinite(g->h2o_s260[i]) || !std::isfinite(g->h2o_f296[i]))
+        return fail(RADG_EINPUT, "CKD H2O tables: entry %d not finite", i + 1);
+    c->h2o_v0 = g->h2o_v0;
+    c->h2o_dv = g->h2o_dv;
+    c->h2o_c2 = g->h2o_c2;
+    c->h2o_ts = g->h2o_ts;
+    c->h2o_ps = g->h2o_ps;
+    c->s296.assign(g->h2o_s296, g->h2o_s296 + n);
+    c->s260.assign(g->h2o_s260, g->h2o_s260 + n);
+    c->f296.assign(g->h2o_f296, g->h2o_f296 + n);
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_gascon = c;
+  return RADG_OK;
+}
+
 int radg_set_iform(int iform, int iread) {
   std::lock_guard<std::mutex> lk(g_mu);
-  // iread = 999 is the registered stellar spectrum (radg_register_solar)
-  if ((iread == 999) != (g_solar != nullptr))
-    return fail(RADG_EINPUT, "iread=%d but a stellar spectrum is %sregistered", iread, g_solar ? "" : "not ");
+  // iread = 999 reads the registered stellar spectrum (radg_register_solar);
+  // another iread leaves a registered spectrum unused by these entry points
+  if (iread == 999 && !g_solar)
+    return fail(RADG_EINPUT, "iread=999 but no stellar spectrum is registered");
   g_iform = iform;
   g_iread = iread;
   return RADG_OK;
@@ -1268,7 +1487,7 @@ int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
-    aux = {g_para, g_e3, g_n2nir, g_solar};
+    aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
   Plan pl;
@@ -1278,7 +1497,21 @@ int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes
   return RADG_OK;
 }
 
+namespace {
+// Batches see the registered stellar spectrum as iread = 999; the
+// fixed-argument entry points see it only when /SOLARDAT/'s iread is 999
+// (radg_set_iform), as cirsradg_wave.f:921,866 test iread.
+int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, radg_batch **out,
+                 bool fixed_args);
+}  // namespace
+
 int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, radg_batch **out) {
+  return batch_create(nreq, reqs, mode, budget, out, false);
+}
+
+namespace {
+int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, radg_batch **out,
+                 bool fixed_args) {
   *out = nullptr;
   if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   std::shared_ptr<TableSet> ts;
@@ -1288,7 +1521,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
     cia = g_cia;
-    aux = {g_para, g_e3, g_n2nir, g_solar};
+    aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
+    if (fixed_args && g_iread != 999) aux.solar.reset();
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
@@ -1364,6 +1598,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
     put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
     put(h, pl.o_n2 + 2 * d.wave0 * sizeof(double), P.n2nir.data(), P.n2nir.size());
     put(h, pl.o_sol + d.wave0 * sizeof(double), P.solar.data(), P.solar.size());
+    put(h, pl.o_ppg + d.lg0 * sizeof(double), P.ppg.data(), P.ppg.size());
+    if (d.gcw0 >= 0) put(h, pl.o_gcw + d.gcw0 * sizeof(double), P.gcw.data(), P.gcw.size());
     put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
     put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
     put(h, pl.o_dx + d.dx0 * sizeof(double), P.dext.data(), P.dext.size());
@@ -1419,6 +1655,11 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
   a.n2nir = reinterpret_cast<const double *>(m + pl.o_n2);
   a.solar = reinterpret_cast<const double *>(m + pl.o_sol);
+  a.ppg = reinterpret_cast<const double *>(m + pl.o_ppg);
+  a.gcw = reinterpret_cast<const double *>(m + pl.o_gcw);
+  a.h2o_c2 = aux.gascon ? aux.gascon->h2o_c2 : 0.0;
+  a.h2o_ts = aux.gascon ? aux.gascon->h2o_ts : 0.0;
+  a.h2o_ps = aux.gascon ? aux.gascon->h2o_ps : 0.0;
   a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
   a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
   a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
@@ -1487,6 +1728,8 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
   *out = b.release();
   return RADG_OK;
 }
+
+}  // namespace
 
 int radg_batch_launch(radg_batch *b, void *stream) {
   if (!b) return fail(RADG_EINPUT, "null batch");
@@ -2052,7 +2295,7 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
     q.xsec = dust->xsec.data();
   }
   radg_batch *b = nullptr;
-  int rc = radg_batch_create(1, &q, g_mode, 0, &b);
+  int rc = batch_create(1, &q, g_mode, 0, &b, true);
   if (rc) return rc;
   std::unique_ptr<radg_batch> guard(b);
   if ((rc = radg_batch_launch(b, nullptr))) return rc;

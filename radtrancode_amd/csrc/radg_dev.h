@@ -14,6 +14,16 @@
 #define RADG_WAVE 64
 #define RADG_MAXPARAM 41  // ngas + 1 + ncont with ngas <= 30, ncont <= 10 (no para-H2)
 
+// gas-continuum kinds (ngascon.f:144-262) and the per-wavenumber factors the
+// host forms for them: CH4 and NH3 coefficients (1e20 cm2/molecule), the H-
+// bound-free cross-section, and the CKD H2O terms (in range flag, self
+// continuum at 296 and 260 K, foreign continuum with its CKD 2.1 scaling,
+// the self continuum's scaling)
+enum { RADG_GC_NONE = 0, RADG_GC_H2O = 1, RADG_GC_CH4 = 2, RADG_GC_NH3 = 3, RADG_GC_HBF = 4,
+       RADG_GC_HFF = 5 };
+enum { RADG_GCW_CH4 = 0, RADG_GCW_NH3, RADG_GCW_HBF, RADG_GCW_H2OIN, RADG_GCW_S296,
+       RADG_GCW_S260, RADG_GCW_F296, RADG_GCW_SFAC, RADG_GCW };
+
 struct DevReq {
   int L, ngas, npath, nparam, nsw, nwave;
   int ispace, tsurf_le0, inormal;
@@ -30,6 +40,12 @@ struct DevReq {
   double xfac18;               // imod 18 output scale, its 2 pi included
   double solar_area;           // iform 4: 4 pi (dist AU)^2 (1 for dist <= 0)
   int iabs[5];                 // nciacon gas slots (0-based gas index, -1 = absent)
+  // gas continua (ngascon.f): kind per gas (RADG_GC_*), the e- (80) and H-
+  // (91) gas slots the H- free-free term reads, and the offset of the
+  // per-wavenumber factors [nwave][RADG_GCW] (-1: no continuum gas)
+  signed char gcon[RADG_MAXGAS];
+  short ke80, ke91;
+  long long gcw0;
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
   long long lay0;              // LayerConst / per-layer arrays
   long long lg0;               // per-(layer, gas) arrays: amo
@@ -121,6 +137,9 @@ struct KernelArgs {
   const double *vwave, *esurf;
   const double *n2nir;  // [wavenumber][2]: N2-N2, N2-H2 near-infrared CIA
   const double *solar;  // [wavenumber]: the stellar spectrum (iread = 999)
+  const double *ppg;    // [layer][gas] partial pressures (atm), gas continua
+  const double *gcw;    // [wavenumber][RADG_GCW] gas-continuum factors
+  double h2o_c2, h2o_ts, h2o_ps;  // /COMCON/ for the CKD H2O continuum
   const WaveGas *wg;
   const double *pweight;       // [path]
   const double *dext;          // [wavenumber][ncont] dust extinction (cm2) at vwave

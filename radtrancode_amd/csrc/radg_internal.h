@@ -29,6 +29,9 @@ inline void radg_jac_to_host(const void *src, int fp64, size_t n, int nparam, in
   }
 }
 
+// Records a failure: the message radg_last_error() returns on this thread.
+int radg_fail_msg(int code, const char *msg);
+
 // Fills *v for request r of a launched batch whose copies have completed.
 int radg_batch_pinned_view(radg_batch *b, int r, PinnedView *v);
 

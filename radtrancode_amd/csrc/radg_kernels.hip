@@ -703,6 +703,72 @@ __device__ __forceinline__ double rayleigh_xs(int iray, double vv, const LayerCo
   return 8.0 * pi3 / (3.0 * ((wl * wl) * (wl * wl)) * (lo * lo)) * 1e-8 * s / w;
 }
 
+// H- free-free absorption coefficient (cm4 dyne-1) at wavenumber v0 and
+// temperature t: John (1988) / Bell & Berrington (1987) tables 3a (above
+// 0.3645 micron) and 3b (0.1823-0.3645 micron) as hmin_ff.f:44-67 sums them,
+// each call from a zero sum; negative sums give 0.
+__device__ __forceinline__ double hmin_ff_d(double v0, double t) {
+  const double wv = 1e4 / v0, th = 5040.0 / t;
+  const double w2 = wv * wv, w3 = w2 * wv, w4 = w2 * w2;
+  double s = 0.0;
+  if (wv > 0.3645) {
+    const double A[6] = {0.0, 2483.346, -3449.889, 2200.040, -696.271, 88.283};
+    const double B[6] = {0.0, 285.827, -1158.382, 2427.719, -1841.400, 444.517};
+    const double C[6] = {0.0, -2054.291, 8746.523, -13651.105, 8624.970, -1863.864};
+    const double D[6] = {0.0, 2827.776, -11485.632, 16755.524, -10051.530, 2095.288};
+    const double E[6] = {0.0, -1341.537, 5303.609, -7510.494, 4400.067, -901.788};
+    const double F[6] = {0.0, 208.952, -812.939, 1132.738, -655.020, 132.985};
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+      s += pow(th, 0.5 * (k + 2)) * (w2 * A[k] + B[k] + C[k] / wv + D[k] / w2 + E[k] / w3 + F[k] / w4);
+  } else if (wv < 0.3645 && wv > 0.1823) {
+    const double A[4] = {518.1021, 473.2636, -482.2089, 115.5291};
+    const double B[4] = {-734.8666, 1443.4137, -737.1616, 169.6374};
+    const double C[4] = {1021.1775, -1977.3395, 1096.8827, -245.649};
+    const double D[4] = {-479.0721, 922.3575, -521.1341, 114.243};
+    const double E[4] = {93.1373, -178.9275, 101.7963, -21.9972};
+    const double F[4] = {-6.4285, 12.3600, -7.0571, 1.5097};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      s += pow(th, 0.5 * (k + 2)) * (w2 * A[k] + B[k] + C[k] / wv + D[k] / w2 + E[k] / w3 + F[k] / w4);
+  }
+  return s < 0.0 ? 0.0 : 1e-29 * s;
+}
+
+// ngascon.f:136-262 for gas k of layer j at wavenumber vv: its absorption
+// (an optical depth).  The wavenumber-only factors come from the host (gw);
+// the H2O continuum's temperature scaling and the H- free-free term are
+// formed here.
+__device__ __forceinline__ double gascon_d(const KernelArgs &a, const DevReq &R, int k, int j,
+                                           double vv, const LayerConst &lc, const double *gw) {
+  const int kind = R.gcon[k];
+  const long long lg = R.lg0 + (long long)j * R.ngas;
+  const double amo = a.amo[lg + k], amount = amo * 1e20;
+  if (kind == RADG_GC_CH4) return amo * gw[RADG_GCW_CH4];
+  if (kind == RADG_GC_NH3) return amo * gw[RADG_GCW_NH3];
+  if (kind == RADG_GC_HBF) return amount * gw[RADG_GCW_HBF];
+  if (kind == RADG_GC_HFF) {
+    // electron.f state as cirsradg_wave.f:590-599 leaves it for this layer
+    const double pelec = a.ppg[lg + R.ke80], pph = a.ppg[lg + R.ke91];
+    const double amh = a.amo[lg + R.ke91] * 1e20;
+    const double nh = (a.ppg[lg + k] * 1.013e6 * pelec) / pph;
+    return amh * hmin_ff_d(vv, lc.temp) * nh;
+  }
+  if (kind == RADG_GC_H2O && gw[RADG_GCW_H2OIN] != 0.0) {
+    // ngascon.f:162-207
+    const double t = lc.temp, ppress = a.ppg[lg + k];
+    const double s96 = gw[RADG_GCW_S296];
+    const double tfac = (t - 296.0) / (260.0 - 296.0);
+    const double csfft = gw[RADG_GCW_SFAC] * (s96 * pow(gw[RADG_GCW_S260] / s96, tfac));
+    const double c11 = 0.5 * a.h2o_c2 / t;
+    const double a1 = vv * amount * tanh(c11 * vv);
+    const double a2 = a.h2o_ts / (t * a.h2o_ps);
+    const double a3 = 1.0e-20 * (ppress * csfft + (lc.press - ppress) * gw[RADG_GCW_F296]);
+    return a1 * a2 * a3;
+  }
+  return 0.0;
+}
+
 }  // namespace
 
 // ============================================================ layer kernel
@@ -1205,8 +1271,8 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
 }
 
 // ============================================================= CIA kernel
-// CIA continuum (nciacon.f, ciaread.f; ngascon is zero: no gas-continuum
-// flags) for one (request, wavenumber, layer) per thread, FP64, written to
+// Continuum (dust, gas continua of ngascon.f, CIA of nciacon.f/ciaread.f,
+// Rayleigh) for one (request, wavenumber, layer) per thread, FP64, written to
 // the tcon/dtcon part of the intermediate block.  Scalar work per layer, so
 // it runs one item per lane rather than inside the per-layer wavefront.
 template <typename T>
@@ -1231,6 +1297,20 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   double tdust = 0.0;
   for (int k = 0; k < R.ncont; k++)
     tdust += a.dext[R.dx0 + (long long)iw * R.ncont + k] * a.cont[R.ct0 + (long long)j * R.ncont + k];
+  // gas continua (cirsradg_wave.f:590-611): each gas's absorption, and
+  // absorb / amount as its d(tau)/d(amount) (here per 1e20 molecules)
+  double gcd[RADG_MAXGAS];
+  if (R.gcw0 >= 0) {
+    const double *gw = a.gcw + R.gcw0 + (long long)iw * RADG_GCW;
+    for (int k = 0; k < ngas; k++) {
+      gcd[k] = 0.0;
+      if (R.gcon[k] == RADG_GC_NONE) continue;
+      const double ab = gascon_d(a, R, k, j, vv, lc, gw);
+      const double amo = a.amo[R.lg0 + (long long)j * ngas + k];
+      tdust += ab;
+      if (ab > 0.0 && amo > 0.0) gcd[k] = ab / amo;
+    }
+  }
   const double *n2 = a.n2nir + 2 * (R.wave0 + iw);
   if (R.flagh2p)
     cia_para_layer(a, vv, lc, n2, &absorb, dab);
@@ -1252,7 +1332,7 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
     for (int l = 0; l < 5; l++)
       if (R.iabs[l] == s) d += dab[l] / lc.utotl;
     if (s == ngas) d += dab[5];
-    else d += dray;
+    else d += dray + (R.gcw0 >= 0 ? gcd[s] : 0.0);
     dtcon[(long long)s * L + j] = (T)d;
   }
   // the continuum-only parameters: each dust type's extinction

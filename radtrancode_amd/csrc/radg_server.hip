@@ -18,7 +18,9 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <algorithm>
 #include <deque>
 #include <map>
@@ -35,7 +37,30 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
-// Deep copy of one radg_request.
+// The counts and pointers OwnedRequest's copy reads: each count within the
+// arrdef.f limits (check_limits.f) and every array it copies present.
+int request_shape(const radg_request &s, std::string &err) {
+  char buf[160];
+  auto bad = [&](int code, const char *fmt, long long v) {
+    std::snprintf(buf, sizeof buf, fmt, v);
+    err = buf;
+    return code;
+  };
+  if (s.nlayer < 1 || s.nlayer > RADG_MAXLAY) return bad(RADG_ELIMITS, "nlayer=%lld outside 1..400", s.nlayer);
+  if (s.npath < 1 || s.npath > RADG_MAXPAT) return bad(RADG_ELIMITS, "npath=%lld outside 1..200", s.npath);
+  if (s.ngas < 1 || s.ngas > RADG_MAXGAS) return bad(RADG_ELIMITS, "ngas=%lld outside 1..30", s.ngas);
+  if (s.nsw < 0 || s.nsw > RADG_MAXGAS + 2 + RADG_MAXCON) return bad(RADG_ELIMITS, "nsw=%lld", s.nsw);
+  if (s.nwave < 1 || s.nwave > (1 << 24)) return bad(RADG_ELIMITS, "nwave=%lld", s.nwave);
+  if (s.nem < 1 || s.nem > RADG_MAXSEC) return bad(RADG_ELIMITS, "nem=%lld outside 1..16000", s.nem);
+  const void *need[] = {s.press, s.temp, s.delh, s.pp, s.amount, s.idgas, s.isogas, s.nlayin, s.imod,
+                        s.layinc, s.scale, s.emtemp, s.vwave, s.vem, s.emissivity};
+  for (size_t i = 0; i < sizeof need / sizeof need[0]; i++)
+    if (!need[i]) return bad(RADG_EINPUT, "request array %lld is null", (long long)i + 1);
+  if (s.nsw > 0 && !s.isw) return bad(RADG_EINPUT, "isw is null with nsw=%lld", s.nsw);
+  return RADG_OK;
+}
+
+// Deep copy of one radg_request, made once request_shape accepts it.
 struct OwnedRequest {
   radg_request q{};
   std::vector<double> press, temp, delh, pp, amount, scale, emtemp, vwave, vem, emis, pw;
@@ -356,10 +381,18 @@ int radg_server_create(int mode, size_t budget_bytes, int max_batch, int max_inf
 int radg_server_submit(radg_server *s, const radg_request *req, long long *ticket) {
   if (!s || !req || !ticket) return RADG_EINPUT;
   auto t = std::make_shared<Ticket>();
-  t->req.reset(new OwnedRequest(*req));
   t->t_submit = clk::now();
   size_t bytes = 0;
-  const int rc = radg_batch_bytes(1, &t->req->q, s->mode, &bytes);
+  // the sizes and pointers the deep copy reads are checked first, so a
+  // malformed request fails alone instead of reading past the caller's arrays
+  std::string bad;
+  int rc = request_shape(*req, bad);
+  if (rc == RADG_OK) {
+    t->req.reset(new OwnedRequest(*req));
+    rc = radg_batch_bytes(1, &t->req->q, s->mode, &bytes);
+  } else {
+    radg_fail_msg(rc, bad.c_str());
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   t->id = s->next_id++;
   *ticket = t->id;

@@ -17,7 +17,7 @@ module radg
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status
   public :: radg_register_dust, radg_set_nlte, radg_set_iform, radg_register_parah2
-  public :: radg_register_e3, radg_register_n2nir, radg_register_solar
+  public :: radg_register_e3, radg_register_n2nir, radg_register_solar, radg_register_gascon
   public :: radg_spectrum_r4, radg_error_message, radg_read_klist_f
   public :: radg_batch_create, radg_batch_launch, radg_batch_sync, radg_batch_read
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
@@ -43,6 +43,19 @@ module radg
     real(c_double) :: radius1, radextra      ! km
     type(c_ptr) :: hfp                       ! hfp(nlayer) when flagh2p = 1
     real(c_double) :: dist                   ! AU from the star (iform 4)
+  end type
+
+  ! radg_gascon of include/radg.h: the /GASCONDEF/ flags and the tables
+  ! ngascon.f reads; a NEMESIS caller passes c_loc of its own /CH2OS0/
+  ! H2OS96, /CH2OS1/ H2OS60 and /CH2OF0/ H2OF96 arrays (BLOCK DATA of
+  ! ngascon.f) and of the karkoschka.dat / lutzowen.dat columns
+  type, bind(C), public :: radg_gascon
+    integer(c_int) :: ih2o, ich4, io3, inh3, iuvscat
+    integer(c_int) :: nh2o
+    real(c_double) :: h2o_v0, h2o_dv, h2o_c2, h2o_ts, h2o_ps
+    type(c_ptr) :: h2o_s296, h2o_s260, h2o_f296
+    integer(c_int) :: nch4, nnh3
+    type(c_ptr) :: ch4_wl, ch4_k, nh3_wl, nh3_k
   end type
 
   ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
@@ -114,7 +127,13 @@ module radg
       real(c_double) :: trans(*), e3(*)
     end function
 
-    ! /SOLARDAT/: iform, iread (iread = 999 with iform 1 or 4 is refused)
+    ! /GASCONDEF/ and ngascon.f's tables (every flag 0 and no table: none)
+    integer(c_int) function radg_register_gascon(g) bind(C, name="radg_register_gascon")
+      import :: c_int, radg_gascon
+      type(radg_gascon), intent(in) :: g
+    end function
+
+    ! /SOLARDAT/: iform, iread (iread = 999 reads the registered spectrum)
     integer(c_int) function radg_set_iform(iform, iread) bind(C, name="radg_set_iform")
       import :: c_int
       integer(c_int), value :: iform, iread

@@ -90,6 +90,34 @@ class CIATable:
     e3: Optional["E3Table"] = None
     n2nir: Optional["N2NirTable"] = None
     solar: Optional["SolarSpectrum"] = None
+    gascon: Optional["GasCon"] = None
+
+
+@dataclass
+class GasCon:
+    """``/GASCONDEF/`` (gascom.f) and the tables ngascon.f reads: the CKD
+    H2O continuum (``h2o_s296``/``h2o_s260`` self at 296/260 K,
+    ``h2o_f296`` foreign, on ``h2o_v0 + i*h2o_dv`` cm-1, with /COMCON/'s
+    ``c2``, ``ts``, ``ps``), Karkoschka's CH4 and Lutz & Owen's NH3 tables
+    (wavelength micron, coefficient as the data files hold it).  The H-
+    continua need no flag."""
+    ih2o: int = 0
+    ich4: int = 0
+    io3: int = 0
+    inh3: int = 0
+    iuvscat: int = 0
+    h2o_v0: float = 0.0
+    h2o_dv: float = 10.0
+    h2o_c2: float = 1.4387863
+    h2o_ts: float = 296.0
+    h2o_ps: float = 1.0
+    h2o_s296: Optional[np.ndarray] = None
+    h2o_s260: Optional[np.ndarray] = None
+    h2o_f296: Optional[np.ndarray] = None
+    ch4_wl: Optional[np.ndarray] = None
+    ch4_k: Optional[np.ndarray] = None
+    nh3_wl: Optional[np.ndarray] = None
+    nh3_k: Optional[np.ndarray] = None
 
 
 @dataclass

@@ -11,8 +11,8 @@ import os
 
 import numpy as np
 
-from radtrancode_amd.synth import (CIATable, E3Table, KTable, N2NirTable, ParaCIATable, Request,
-                                   SolarSpectrum)
+from radtrancode_amd.synth import (CIATable, E3Table, GasCon, KTable, N2NirTable, ParaCIATable,
+                                   Request, SolarSpectrum)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -28,11 +28,19 @@ _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
 
 
 def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = None,
-              e3: E3Table = None, n2nir: N2NirTable = None, solar: SolarSpectrum = None):
+              e3: E3Table = None, n2nir: N2NirTable = None, solar: SolarSpectrum = None,
+              gascon_flags=None, hmin_static: bool = False):
     """``cia``: a CIA table of the case's own, stored with it (else the
     shared ``cia.npz`` applies); ``e3``: the E3 table an imod 18 case ran
-    with; ``n2nir``: N2's near-infrared CIA a case needs."""
+    with; ``n2nir``: N2's near-infrared CIA a case needs; ``gascon_flags``:
+    the /GASCONDEF/ flags (ih2o, ich4, io3, inh3, iuvscat) it ran with, with
+    the tables of ``gascon_tables.npz``; ``hmin_static``: its reference
+    outputs carry HMIN_BF/HMIN_FF's static sum over the layers of a call."""
     d = {"note": np.array(note)}
+    if gascon_flags is not None:
+        d["gascon_flags"] = np.asarray(gascon_flags, dtype=np.int32)
+    if hmin_static:
+        d["hmin_static"] = np.array(1)
     if e3 is not None:
         d.update(e3_trans=e3.trans, e3_val=e3.e3)
     if solar is not None:
@@ -112,7 +120,29 @@ def case_cia(path) -> CIATable:
     if "n2n2_k" in z.files:
         cia.n2nir = N2NirTable(wn_n2n2=z["n2n2_wn"], k_n2n2=z["n2n2_k"], wn_n2h2=z["n2h2_wn"],
                                k_n2h2=z["n2h2_k"])
+    if "gascon_flags" in z.files:
+        cia.gascon = load_gascon(z["gascon_flags"])
     return cia
+
+
+def case_hmin_static(path) -> bool:
+    """Whether the case's reference outputs carry HMIN_BF/HMIN_FF's static
+    sum (the oracle reproduces them with ``cia.hmin_static = 1``)."""
+    return "hmin_static" in np.load(path, allow_pickle=False).files
+
+
+def load_gascon(flags=(0, 0, 0, 0, 0), path=None) -> GasCon:
+    """The gas-continuum tables as the reference holds them
+    (``gascon_tables.npz``: ngascon.f's CKD H2O BLOCK DATA, karkoschka.dat,
+    lutzowen.dat) with the given /GASCONDEF/ flags."""
+    z = np.load(path or os.path.join(HERE, "gascon_tables.npz"), allow_pickle=False)
+    f = [int(x) for x in flags]
+    return GasCon(ih2o=f[0], ich4=f[1], io3=f[2], inh3=f[3], iuvscat=f[4],
+                  h2o_v0=float(z["h2o_grid"][0]), h2o_dv=float(z["h2o_grid"][2]),
+                  h2o_c2=float(z["comcon"][2]), h2o_ts=float(z["comcon"][0]),
+                  h2o_ps=float(z["comcon"][1]), h2o_s296=z["h2o_s296"], h2o_s260=z["h2o_s260"],
+                  h2o_f296=z["h2o_f296"], ch4_wl=z["ch4_wl"], ch4_k=z["ch4_k"],
+                  nh3_wl=z["nh3_wl"], nh3_k=z["nh3_k"])
 
 
 def load_n2nir(path=None) -> N2NirTable:

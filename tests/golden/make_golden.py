@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from radtrancode_amd import synth  # noqa: E402
 from oracle import ref  # noqa: E402
-from tests.golden.golden_io import load_e3, load_n2nir, save_case, save_cia  # noqa: E402
+from tests.golden.golden_io import load_e3, load_gascon, load_n2nir, save_case, save_cia  # noqa: E402
 
 REF_E3TAB = "/root/reference/raddata/e3tab.dat"
 
@@ -339,6 +339,104 @@ def cases():
     r.iform, r.radius1 = 1, 6052.0
     yield "disc_nlte_dust", t, r, "imod 18 with non-LTE, one dust type, emissivity < 1, iform = 1"
 
+    # gas continua (ngascon.f): H- free-free and bound-free, the CKD H2O
+    # continuum (IH2O = 1) and the CH4/NH3 tables (ICH4, INH3)
+    t, r = hminus_inputs(31, 2000.0, 2.0)
+    yield "hminus_ir", t, r, "H- free-free (91, 80, 48) in a 1270-2730 K column at 2000 cm-1"
+    t, r = hminus_inputs(32, 8000.0, 3.0)
+    yield "hminus_nir", t, r, "H- bound-free and free-free at 8000 cm-1 (1.25 micron)"
+    t, r = gascon_inputs(33, (27, 26), 1010.0, 37.0, 150.0, 0.8, 1.0)
+    add_continuum_gas(r, 1, 2e-2, seed=33)
+    yield "h2o_ckd", t, r, ("CKD H2O continuum (IH2O = 1, H2O with no k-table) at 1010-1195 cm-1, "
+                            "190-290 K")
+    t, r = gascon_inputs(34, (6, 11), 15300.0, 20.0, 900.0, 1.5, 1.0)
+    yield "ch4_nh3_vis", t, r, "Karkoschka CH4 and Lutz-Owen NH3 continua (ICH4, INH3) at 0.65 micron"
+
+
+def hminus_inputs(seed, v0, delv, nwave=6, nlayer=30, ng=10, vmr=(1e-9, 3e-7, 1e-2)):
+    """A hot (1270-2730 K) 30-layer column with two k-gases, H2/He and then
+    H- (91), e- (80) and H (48), all isotope 0, whose VMRs scatter about
+    ``vmr``: H- bound-free (hmin_bf.f) acts above 6090 cm-1, free-free
+    (hmin_ff.f, with the e- pressure) everywhere in the infrared."""
+    ids = list(synth.KGAS_IDS[:2])
+    t = synth.make_ktables(ids, v0, delv, nwave, ng=ng, np_=10, nt=6, seed=seed, tmin=800.0,
+                           tmax=3000.0)
+    r = synth.make_request(nlayer=nlayer, nkgas=2, nwave=nwave, seed=seed, table_vmin=v0,
+                           table_delv=delv)
+    r.temp = 1300.0 + 12.0 * (r.temp - 50.0)
+    r.emtemp = r.temp[r.layinc - 1]
+    ncol = r.amount[:, -1] / 0.14  # the He column over its mole fraction
+    x = np.asarray(vmr)[None, :] * np.exp(np.random.default_rng(seed).normal(0, 0.3, (nlayer, 3)))
+    r.idgas = np.concatenate([r.idgas, [91, 80, 48]]).astype(np.int32)
+    r.isogas = np.zeros_like(r.idgas)
+    r.amount = np.concatenate([r.amount, x * ncol[:, None]], axis=1)
+    r.pp = np.concatenate([r.pp, x * r.press[:, None]], axis=1)
+    r.isw = np.arange(1, r.ngas + 2, dtype=np.int32)
+    return t, r
+
+
+def gascon_inputs(seed, ids, v0, delv, tshift, tscale, vmr_scale, nwave=6, nlayer=30, ng=10):
+    """A 30-layer column whose k-gases are ``ids`` (with H2/He), the
+    temperatures ``tshift + tscale * T``, the k-gas amounts times
+    ``vmr_scale`` (a gas-continuum case: H2O with IH2O = 1, CH4 with ICH4,
+    NH3 with INH3)."""
+    t = synth.make_ktables(list(ids), v0, delv, nwave, ng=ng, np_=10, nt=6, seed=seed,
+                           tmin=min(40.0, tshift), tmax=tshift + tscale * 230.0)
+    r = synth.make_request(nlayer=nlayer, nkgas=len(ids), nwave=nwave, seed=seed,
+                           table_vmin=v0, table_delv=delv)
+    r.idgas = np.array(list(ids) + [synth.H2_ID, synth.HE_ID], dtype=np.int32)
+    r.temp = tshift + tscale * r.temp
+    r.emtemp = r.temp[r.layinc - 1]
+    n = len(ids)
+    r.amount[:, :n] *= vmr_scale
+    r.pp[:, :n] *= vmr_scale
+    return t, r
+
+
+def add_continuum_gas(r, gid, vmr, seed):
+    """Append gas ``gid`` (isotope 0, no k-table) with a VMR scattered
+    about ``vmr`` to every layer."""
+    ncol = r.amount[:, -1] / 0.14  # the He column over its mole fraction
+    x = vmr * np.exp(np.random.default_rng(seed).normal(0, 0.3, r.nlayer))
+    r.idgas = np.concatenate([r.idgas, [gid]]).astype(np.int32)
+    r.isogas = np.zeros_like(r.idgas)
+    r.amount = np.concatenate([r.amount, (x * ncol)[:, None]], axis=1)
+    r.pp = np.concatenate([r.pp, (x * r.press)[:, None]], axis=1)
+    r.isw = np.arange(1, r.ngas + 2, dtype=np.int32)
+    return r
+
+
+def gascon_tables(path=os.path.join(HERE, "gascon_tables.npz")):
+    """The gas-continuum data as the reference holds it: the CKD H2O tables
+    of ngascon.f's BLOCK DATA with /COMCON/ (copied out of the REAL(8)
+    build's COMMON blocks by the harness), raddata/karkoschka.dat and
+    raddata/lutzowen.dat (read as ch4cont.f and nh3cont.f read them)."""
+    from radtrancode_amd.gascon import read_karkoschka, read_lutzowen
+    grid, npt, s96, s60, f96, con = ref.h2o_tables()
+    cw, ck = read_karkoschka(os.path.join(ref.REF_TREE, "raddata", "karkoschka.dat"))
+    nw, nk = read_lutzowen(os.path.join(ref.REF_TREE, "raddata", "lutzowen.dat"))
+    assert npt == 2001
+    np.savez_compressed(path, h2o_grid=grid, h2o_s296=s96, h2o_s260=s60, h2o_f296=f96,
+                        comcon=con, ch4_wl=cw, ch4_k=ck, nh3_wl=nw, nh3_k=nk)
+
+
+def gascon_vectors(n=64, seed=12):
+    """The reference's own HMIN_BF and HMIN_FF (each from a zero sum),
+    CH4CONT and NH3CONT at log-uniform wavenumbers (and temperatures for
+    HMIN_FF), with both ranges' ends, REAL(8) build."""
+    rng = np.random.default_rng(seed)
+    v = np.exp(rng.uniform(np.log(500.0), np.log(60000.0), n))
+    v = np.concatenate([v, [1e4 / 1.6419 * (1 + 1e-6), 1e4 / 0.3645 * (1 - 1e-6),
+                            1e4 / 0.3645 * (1 + 1e-6), 1e4 / 0.1823 * (1 - 1e-6)]])
+    t = rng.uniform(800.0, 6000.0, v.shape[0])
+    vt = np.concatenate([np.exp(rng.uniform(np.log(9000.0), np.log(34000.0), n)),
+                         1e4 / np.array([0.3004, 1.06, 0.547615, 0.660327, 0.55, 0.65])])
+    return dict(v=v, t=t, vt=vt,
+                ref64_bf=np.array([ref.hmin(64, "bf", x) for x in v]),
+                ref64_ff=np.array([ref.hmin(64, "ff", x, y) for x, y in zip(v, t)]),
+                ref64_ch4=np.array([ref.tabcont(64, "ch4", x) for x in vt]),
+                ref64_nh3=np.array([ref.tabcont(64, "nh3", x) for x in vt]))
+
 
 def add_dust(r, ncont, nsec, seed, peak_tau=(1.5, 0.4)):
     """Dust on a request (cirsradg_wave.f:550-577): particle type 0 is a
@@ -456,6 +554,9 @@ CASE_E3 = ("disc", "disc_iform3", "disc_nlte_dust")
 CASE_N2NIR = ("titan_nir", "titan_nir_um")
 # cases run with a stellar spectrum (/SOLARDAT/, iread = 999)
 CASE_SOLAR = ("solar_ratio", "solar_trans")
+# cases run with /GASCONDEF/ flags (ih2o, ich4, io3, inh3, iuvscat)
+CASE_GASCON = {"hminus_ir": (0, 0, 0, 0, 0), "hminus_nir": (0, 0, 0, 0, 0),
+               "h2o_ckd": (1, 0, 0, 0, 0), "ch4_nh3_vis": (0, 1, 0, 1, 0)}
 
 
 def main(names=()):
@@ -470,6 +571,9 @@ def main(names=()):
         save_n2nir()
     if not names or "rayleigh" in names:
         np.savez_compressed(os.path.join(HERE, "rayleigh.npz"), **rayleigh_vectors())
+    if not names or "gascon" in names:
+        gascon_tables()
+        np.savez_compressed(os.path.join(HERE, "gascon_funcs.npz"), **gascon_vectors())
     for name, t, r, note in cases():
         if names and name not in names:
             continue
@@ -492,9 +596,14 @@ def main(names=()):
             sol = synth.make_solar(500.0, 1500.0)
             use = copy.copy(use)
             use.solar = sol
+        gflags = CASE_GASCON.get(name)
+        if gflags is not None:
+            use = copy.copy(use)
+            use.gascon = load_gascon(gflags)
         outs = run_ref(t, use, r)
         path = os.path.join(HERE, f"case_{name}.npz")
-        save_case(path, t, r, outs, note, cia=own, e3=e3, n2nir=nir, solar=sol)
+        save_case(path, t, r, outs, note, cia=own, e3=e3, n2nir=nir, solar=sol,
+                  gascon_flags=gflags, hmin_static=bool(np.isin(r.idgas, (48, 91)).any()))
         print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
               f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
 

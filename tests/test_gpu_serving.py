@@ -117,6 +117,27 @@ def test_bad_requests_fail_alone(gpu_lib, tables):
     assert [r["status"] for r in res] == [0, 0, 0, 0]
 
 
+def test_malformed_requests_fail_before_any_copy(gpu_lib, tables):
+    """Counts outside the arrdef.f limits and null arrays are refused at
+    submit, before the server copies the request, and the server goes on."""
+    import ctypes as C
+    from radtrancode_amd._native import lib
+    good = _mix(1, seed=15)[0]
+    srv = serving.Server(mode=1)
+    for field, value, code in (("nlayer", -1, 1), ("nlayer", 401, 1), ("ngas", 0, 1),
+                               ("nwave", -5, 1), ("press", None, 4), ("layinc", None, 4)):
+        keep, arr = gpu_lib._requests([good])
+        setattr(arr[0], field, value)
+        t = C.c_longlong()
+        rc = lib().radg_server_submit(srv.h, C.cast(arr, C.c_void_p), C.byref(t))
+        print(f"malformed {field}={value}: status {rc}", flush=True)
+        assert rc == code, lib().radg_last_error()
+    t = srv.submit(good)
+    res = srv.wait(t, timeout=60.0)
+    srv.close()
+    assert res["status"] == 0
+
+
 def test_cancel_before_admission(gpu_lib, tables):
     big = synth.make_request(nlayer=200, nkgas=4, nwave=NPOINT, seed=5, table_vmin=600.0,
                              table_delv=0.16)

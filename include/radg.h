@@ -339,6 +339,26 @@ int radg_batch_timing_stats(radg_batch *b, int *nlaunch, float *sum_ms3);
  * nodes; plus the number of spectra and of layer-kernel work items. */
 int radg_batch_info(radg_batch *b, double *alg_bytes3, int *nspectra, int *nitems_a);
 int radg_batch_destroy(radg_batch *b);
+
+/* The forward model and analytic Jacobian of one request with NO device
+ * allocation.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The kernels are those of
+ * radg_batch_create + radg_batch_launch; the packed inputs, intermediates and
+ * per-path outputs lie in the library's file-scope device array of
+ * RADG_NOALLOC_BYTES per GPU, held until the process ends.  Outputs are the
+ * caller's device arrays (FP64): wout[nwave], wgts[nwave] and
+ * wjac[nwave][nlayer][nparam], the angle-weighted radiance, surface gradient
+ * and Jacobian radg_batch_read returns (gas columns per molecule).  The
+ * kernels are queued on hip_stream (NULL: the null stream); the call returns
+ * once they are queued.  A request that needs more scratch than
+ * RADG_NOALLOC_BYTES is refused with RADG_ELIMITS (split its wavenumbers);
+ * the table form of mode must exist (radg_set_mode(mode) after registering
+ * the tables makes it).  Calls on one GPU share the array: calls that overlap
+ * in time on one GPU (other streams, other threads) race. */
+#define RADG_NOALLOC_BYTES ((size_t)128 << 20)
+int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, double *wgts,
+                          double *wjac, void *hip_stream);
 /* ------------------------------------- layer -> profile -> state maps */
 /* Per request: the matrices cirsrtfg_wave.f:403-425 applies after each
  * cirsradg_wave call (0-based axes, C order):

@@ -34,6 +34,7 @@ EXPORTS = [
     "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch", "radg_ktable_bf16",
     "radg_batch_set_statemaps", "radg_batch_read_mapped", "radg_batch_device_outputs",
     "radg_oe_create", "radg_oe_cost", "radg_oe_propose", "radg_oe_destroy",
+    "radg_spectrum_noalloc",
 ]
 
 
@@ -124,6 +125,8 @@ def lib():
     L.radg_oe_cost.argtypes = [vp, vp, vp, vp, vp]
     L.radg_oe_propose.argtypes = [vp, vp, vp, vp, vp, vp]
     L.radg_oe_destroy.argtypes = [vp]
+    L.radg_spectrum_noalloc.restype = ip
+    L.radg_spectrum_noalloc.argtypes = [vp, ip, vp, vp, vp, vp]
     _lib = L
     return L
 

@@ -1135,6 +1135,160 @@ void put(char *h, size_t off, const X *src, size_t n) {
   if (n) std::memcpy(h + off, src, n * sizeof(X));
 }
 
+// The host image of a batch's device block up to its intermediates: the
+// request records, item starts, emission-class maps and every packed input.
+void stage_inputs(const Plan &pl, int nreq, const radg_request *reqs, char *h) {
+  std::memset(h, 0, pl.o_inter);
+  std::vector<DevReq> dr(nreq);
+  for (int r = 0; r < nreq; r++) dr[r] = pl.packs[r].d;
+  put(h, pl.o_reqs, dr.data(), nreq);
+  put(h, pl.o_sA, pl.sA.data(), nreq + 1);
+  put(h, pl.o_sB, pl.sB.data(), nreq + 1);
+  put(h, pl.o_sC, pl.sC.data(), nreq + 1);
+  {
+    std::vector<int> cmap;
+    std::vector<long long> cstart;
+    for (const auto &mem : pl.cls_members) {
+      long long acc = 0;
+      cstart.push_back(0);
+      for (int r : mem) {
+        cmap.push_back(r);
+        acc += (long long)reqs[r].nwave * reqs[r].npath;
+        cstart.push_back(acc);
+      }
+    }
+    put(h, pl.o_clsmap, cmap.data(), cmap.size());
+    put(h, pl.o_clsstart, cstart.data(), cstart.size());
+  }
+  for (int r = 0; r < nreq; r++) {
+    const Packed &P = pl.packs[r];
+    const DevReq &d = P.d;
+    put(h, pl.o_lay + d.lay0 * sizeof(LayerConst), P.lay.data(), P.lay.size());
+    put(h, pl.o_amo + d.lg0 * sizeof(double), P.amo.data(), P.amo.size());
+    put(h, pl.o_press + d.lay0 * sizeof(double), P.press.data(), P.press.size());
+    put(h, pl.o_paths + d.path0 * sizeof(PathInfo), P.paths.data(), P.paths.size());
+    put(h, pl.o_layinc + d.pl0 * sizeof(int), P.layinc.data(), P.layinc.size());
+    put(h, pl.o_scale + d.pl0 * sizeof(double), P.scale.data(), P.scale.size());
+    put(h, pl.o_emt + d.pl0 * sizeof(double), P.emtemp.data(), P.emtemp.size());
+    put(h, pl.o_vw + d.wave0 * sizeof(double), P.vwave.data(), P.vwave.size());
+    put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
+    put(h, pl.o_n2 + 2 * d.wave0 * sizeof(double), P.n2nir.data(), P.n2nir.size());
+    put(h, pl.o_sol + d.wave0 * sizeof(double), P.solar.data(), P.solar.size());
+    put(h, pl.o_ppg + d.lg0 * sizeof(double), P.ppg.data(), P.ppg.size());
+    if (d.gcw0 >= 0) put(h, pl.o_gcw + d.gcw0 * sizeof(double), P.gcw.data(), P.gcw.size());
+    put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
+    put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
+    put(h, pl.o_dx + d.dx0 * sizeof(double), P.dext.data(), P.dext.size());
+    put(h, pl.o_ct + d.ct0 * sizeof(double), P.cont.data(), P.cont.size());
+  }
+}
+
+// Kernel arguments for a batch whose device block starts at m (laid out by
+// plan_batch).
+void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const TableSet &ts_,
+                     const CIA &cia_, const AuxTables &aux, int mode) {
+  const TableSet *ts = &ts_;
+  const CIA *cia = &cia_;
+  a.reqs = reinterpret_cast<const DevReq *>(m + pl.o_reqs);
+  a.nreq = nreq;
+  a.startA = reinterpret_cast<const long long *>(m + pl.o_sA);
+  a.startB = reinterpret_cast<const long long *>(m + pl.o_sB);
+  a.startC = reinterpret_cast<const long long *>(m + pl.o_sC);
+  a.nA = pl.nA;
+  a.nB = pl.nB;
+  a.nC = pl.nC;
+  a.ng = ts->ng;
+  a.ktab = (mode == RADG_MODE_FP64)   ? (const void *)ts->d_k64
+           : (mode == RADG_MODE_BF16) ? (const void *)ts->d_k16
+                                      : (const void *)ts->d_k32;
+  a.np = ts->np;
+  a.nt = ts->nt;
+  a.G = ts->d_G;
+  a.delg64 = ts->d_delg64;
+  a.delg32 = ts->d_delg32;
+  a.kcia = cia->d_kcia;
+  a.cia_temps = cia->d_temps;
+  a.cia_dnu = cia->dnu;
+  a.lay = reinterpret_cast<const LayerConst *>(m + pl.o_lay);
+  a.amo = reinterpret_cast<const double *>(m + pl.o_amo);
+  a.press = reinterpret_cast<const double *>(m + pl.o_press);
+  a.paths = reinterpret_cast<const PathInfo *>(m + pl.o_paths);
+  a.layinc = reinterpret_cast<const int *>(m + pl.o_layinc);
+  a.scale = reinterpret_cast<const double *>(m + pl.o_scale);
+  a.emtemp = reinterpret_cast<const double *>(m + pl.o_emt);
+  a.vwave = reinterpret_cast<const double *>(m + pl.o_vw);
+  a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
+  a.n2nir = reinterpret_cast<const double *>(m + pl.o_n2);
+  a.solar = reinterpret_cast<const double *>(m + pl.o_sol);
+  a.ppg = reinterpret_cast<const double *>(m + pl.o_ppg);
+  a.gcw = reinterpret_cast<const double *>(m + pl.o_gcw);
+  a.h2o_c2 = aux.gascon ? aux.gascon->h2o_c2 : 0.0;
+  a.h2o_ts = aux.gascon ? aux.gascon->h2o_ts : 0.0;
+  a.h2o_ps = aux.gascon ? aux.gascon->h2o_ps : 0.0;
+  a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
+  a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
+  a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
+  a.cont = reinterpret_cast<const double *>(m + pl.o_ct);
+  a.e3tab = aux.e3 ? aux.e3->d_tab : nullptr;
+  const ParaCIA *para = aux.para.get();
+  if (para) {
+    a.pkh2h2 = para->d_kh2h2;
+    a.pkh2he = para->d_kh2he;
+    a.pkcia5 = para->d_kcia5;
+    a.ptemps = para->d_temps;
+    a.pfp = para->d_fp;
+    a.pdnu = para->dnu;
+    a.npara = para->npara;
+  } else {
+    a.pkh2h2 = a.pkh2he = a.pkcia5 = a.ptemps = a.pfp = nullptr;
+    a.pdnu = 0.0;
+    a.npara = 0;
+  }
+  a.inter = m + pl.o_inter;
+  a.out = m + pl.o_out;
+  a.gts = m + pl.o_gts;
+  a.jac = m + pl.o_jac;
+  a.wout = reinterpret_cast<double *>(m + pl.o_wout);
+  a.wgts = reinterpret_cast<double *>(m + pl.o_wgts);
+  a.wjac = reinterpret_cast<double *>(m + pl.o_wjac);
+  a.err = reinterpret_cast<int *>(m + pl.o_err);
+}
+
+// One emission launch per layers-per-lane class (plan_batch's classes), with
+// the class maps in the device block at m.
+std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_request *reqs, char *m,
+                                                  int fp64) {
+  std::vector<radg_batch::EmClass> em;
+  {
+    size_t off = 0, soff = 0;
+    for (size_t c = 0; c < pl.cls_rl.size(); c++) {
+      radg_batch::EmClass ec;
+      ec.RL = pl.cls_rl[c];
+      ec.nreq = (int)pl.cls_members[c].size();
+      ec.nB = 0;
+      int maxnsw = 1;
+      ec.disc = false;
+      for (int r : pl.cls_members[c]) {
+        ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
+        maxnsw = std::max(maxnsw, reqs[r].nsw);
+        for (int p = 0; p < reqs[r].npath; p++) {
+          const int im = reqs[r].imod[p];
+          ec.disc = ec.disc || im == 8 || im == 13 || im == 14 || im == 18;
+        }
+        ec.disc = ec.disc || pl.packs[r].d.solar != 0;
+      }
+      ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
+      ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
+      ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
+      off += ec.nreq;
+      soff += ec.nreq + 1;
+      em.push_back(ec);
+    }
+  }
+  return em;
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -1142,7 +1296,13 @@ extern "C" {
 int radg_set_mode(int mode) {
   if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   g_mode = mode;
-  return RADG_OK;
+  // the table form the mode reads, for the tables registered now
+  std::shared_ptr<TableSet> ts;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ts = g_tables;
+  }
+  return ts ? ensure_form(*ts, mode) : RADG_OK;
 }
 int radg_get_mode(void) { return g_mode; }
 const char *radg_last_error(void) { return g_err.c_str(); }
@@ -1561,51 +1721,9 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
   if (!b->hup) return fail(RADG_EHIP, "pinned allocation of %zu bytes failed", pl.o_inter);
   b->hup_bytes = pl.o_inter;
   b->up_pending = true;
-  char *h = b->hup;
-  std::memset(h, 0, pl.o_inter);
-  std::vector<DevReq> dr(nreq);
-  for (int r = 0; r < nreq; r++) dr[r] = pl.packs[r].d;
-  put(h, pl.o_reqs, dr.data(), nreq);
-  put(h, pl.o_sA, pl.sA.data(), nreq + 1);
-  put(h, pl.o_sB, pl.sB.data(), nreq + 1);
-  put(h, pl.o_sC, pl.sC.data(), nreq + 1);
-  {
-    std::vector<int> cmap;
-    std::vector<long long> cstart;
-    for (const auto &mem : pl.cls_members) {
-      long long acc = 0;
-      cstart.push_back(0);
-      for (int r : mem) {
-        cmap.push_back(r);
-        acc += (long long)reqs[r].nwave * reqs[r].npath;
-        cstart.push_back(acc);
-      }
-    }
-    put(h, pl.o_clsmap, cmap.data(), cmap.size());
-    put(h, pl.o_clsstart, cstart.data(), cstart.size());
-  }
-  for (int r = 0; r < nreq; r++) {
-    const Packed &P = pl.packs[r];
-    const DevReq &d = P.d;
-    put(h, pl.o_lay + d.lay0 * sizeof(LayerConst), P.lay.data(), P.lay.size());
-    put(h, pl.o_amo + d.lg0 * sizeof(double), P.amo.data(), P.amo.size());
-    put(h, pl.o_press + d.lay0 * sizeof(double), P.press.data(), P.press.size());
-    put(h, pl.o_paths + d.path0 * sizeof(PathInfo), P.paths.data(), P.paths.size());
-    put(h, pl.o_layinc + d.pl0 * sizeof(int), P.layinc.data(), P.layinc.size());
-    put(h, pl.o_scale + d.pl0 * sizeof(double), P.scale.data(), P.scale.size());
-    put(h, pl.o_emt + d.pl0 * sizeof(double), P.emtemp.data(), P.emtemp.size());
-    put(h, pl.o_vw + d.wave0 * sizeof(double), P.vwave.data(), P.vwave.size());
-    put(h, pl.o_es + d.wave0 * sizeof(double), P.esurf.data(), P.esurf.size());
-    put(h, pl.o_n2 + 2 * d.wave0 * sizeof(double), P.n2nir.data(), P.n2nir.size());
-    put(h, pl.o_sol + d.wave0 * sizeof(double), P.solar.data(), P.solar.size());
-    put(h, pl.o_ppg + d.lg0 * sizeof(double), P.ppg.data(), P.ppg.size());
-    if (d.gcw0 >= 0) put(h, pl.o_gcw + d.gcw0 * sizeof(double), P.gcw.data(), P.gcw.size());
-    put(h, pl.o_wg + d.wg0 * sizeof(WaveGas), P.wg.data(), P.wg.size());
-    put(h, pl.o_pw + d.path0 * sizeof(double), P.pweight.data(), P.pweight.size());
-    put(h, pl.o_dx + d.dx0 * sizeof(double), P.dext.data(), P.dext.size());
-    put(h, pl.o_ct + d.ct0 * sizeof(double), P.cont.data(), P.cont.size());
-    for (int i = 0; i < 3; i++) b->alg_bytes[i] += P.alg_bytes[i];
-  }
+  stage_inputs(pl, nreq, reqs, b->hup);
+  for (int r = 0; r < nreq; r++)
+    for (int i = 0; i < 3; i++) b->alg_bytes[i] += pl.packs[r].alg_bytes[i];
   b->hreq.resize(nreq);
   for (int r = 0; r < nreq; r++) {
     const Packed &P = pl.packs[r];
@@ -1621,72 +1739,9 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
     for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
   }
 
-  b->reqs = dr;
-  KernelArgs &a = b->args;
-  char *m = b->dmem;
-  a.reqs = reinterpret_cast<const DevReq *>(m + pl.o_reqs);
-  a.nreq = nreq;
-  a.startA = reinterpret_cast<const long long *>(m + pl.o_sA);
-  a.startB = reinterpret_cast<const long long *>(m + pl.o_sB);
-  a.startC = reinterpret_cast<const long long *>(m + pl.o_sC);
-  a.nA = pl.nA;
-  a.nB = pl.nB;
-  a.nC = pl.nC;
-  a.ng = ts->ng;
-  a.ktab = (mode == RADG_MODE_FP64)   ? (const void *)ts->d_k64
-           : (mode == RADG_MODE_BF16) ? (const void *)ts->d_k16
-                                      : (const void *)ts->d_k32;
-  a.np = ts->np;
-  a.nt = ts->nt;
-  a.G = ts->d_G;
-  a.delg64 = ts->d_delg64;
-  a.delg32 = ts->d_delg32;
-  a.kcia = cia->d_kcia;
-  a.cia_temps = cia->d_temps;
-  a.cia_dnu = cia->dnu;
-  a.lay = reinterpret_cast<const LayerConst *>(m + pl.o_lay);
-  a.amo = reinterpret_cast<const double *>(m + pl.o_amo);
-  a.press = reinterpret_cast<const double *>(m + pl.o_press);
-  a.paths = reinterpret_cast<const PathInfo *>(m + pl.o_paths);
-  a.layinc = reinterpret_cast<const int *>(m + pl.o_layinc);
-  a.scale = reinterpret_cast<const double *>(m + pl.o_scale);
-  a.emtemp = reinterpret_cast<const double *>(m + pl.o_emt);
-  a.vwave = reinterpret_cast<const double *>(m + pl.o_vw);
-  a.esurf = reinterpret_cast<const double *>(m + pl.o_es);
-  a.n2nir = reinterpret_cast<const double *>(m + pl.o_n2);
-  a.solar = reinterpret_cast<const double *>(m + pl.o_sol);
-  a.ppg = reinterpret_cast<const double *>(m + pl.o_ppg);
-  a.gcw = reinterpret_cast<const double *>(m + pl.o_gcw);
-  a.h2o_c2 = aux.gascon ? aux.gascon->h2o_c2 : 0.0;
-  a.h2o_ts = aux.gascon ? aux.gascon->h2o_ts : 0.0;
-  a.h2o_ps = aux.gascon ? aux.gascon->h2o_ps : 0.0;
-  a.wg = reinterpret_cast<const WaveGas *>(m + pl.o_wg);
-  a.pweight = reinterpret_cast<const double *>(m + pl.o_pw);
-  a.dext = reinterpret_cast<const double *>(m + pl.o_dx);
-  a.cont = reinterpret_cast<const double *>(m + pl.o_ct);
-  a.e3tab = aux.e3 ? aux.e3->d_tab : nullptr;
-  const ParaCIA *para = aux.para.get();
-  if (para) {
-    a.pkh2h2 = para->d_kh2h2;
-    a.pkh2he = para->d_kh2he;
-    a.pkcia5 = para->d_kcia5;
-    a.ptemps = para->d_temps;
-    a.pfp = para->d_fp;
-    a.pdnu = para->dnu;
-    a.npara = para->npara;
-  } else {
-    a.pkh2h2 = a.pkh2he = a.pkcia5 = a.ptemps = a.pfp = nullptr;
-    a.pdnu = 0.0;
-    a.npara = 0;
-  }
-  a.inter = m + pl.o_inter;
-  a.out = m + pl.o_out;
-  a.gts = m + pl.o_gts;
-  a.jac = m + pl.o_jac;
-  a.wout = reinterpret_cast<double *>(m + pl.o_wout);
-  a.wgts = reinterpret_cast<double *>(m + pl.o_wgts);
-  a.wjac = reinterpret_cast<double *>(m + pl.o_wjac);
-  a.err = reinterpret_cast<int *>(m + pl.o_err);
+  b->reqs.resize(nreq);
+  for (int r = 0; r < nreq; r++) b->reqs[r] = pl.packs[r].d;
+  set_kernel_args(b->args, b->dmem, pl, nreq, *ts, *cia, aux, mode);
   const int fp64 = mode == RADG_MODE_FP64;
   b->E = radg_layer_E(ts->ng);
   b->maxngas = pl.maxngas;
@@ -1696,32 +1751,7 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
   b->ldsB = radg_lds_emission_bytes(b->RL, pl.maxnsw, fp64);
   if (b->ldsA > 160 * 1024 || b->ldsB > 160 * 1024)
     return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", b->ldsA, b->ldsB);
-  {
-    size_t off = 0, soff = 0;
-    for (size_t c = 0; c < pl.cls_rl.size(); c++) {
-      radg_batch::EmClass ec;
-      ec.RL = pl.cls_rl[c];
-      ec.nreq = (int)pl.cls_members[c].size();
-      ec.nB = 0;
-      int maxnsw = 1;
-      ec.disc = false;
-      for (int r : pl.cls_members[c]) {
-        ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
-        maxnsw = std::max(maxnsw, reqs[r].nsw);
-        for (int p = 0; p < reqs[r].npath; p++) {
-          const int im = reqs[r].imod[p];
-          ec.disc = ec.disc || im == 8 || im == 13 || im == 14 || im == 18;
-        }
-        ec.disc = ec.disc || pl.packs[r].d.solar != 0;
-      }
-      ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
-      ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
-      ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
-      off += ec.nreq;
-      soff += ec.nreq + 1;
-      b->em.push_back(ec);
-    }
-  }
+  b->em = emission_classes(pl, reqs, b->dmem, fp64);
   // events are created on first use of each ring slot
   b->evs.assign(4 * radg_batch::RING, nullptr);
   b->ev = &b->evs[0];
@@ -1730,6 +1760,93 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
 }
 
 }  // namespace
+
+// ------------------------------------------ the entry point that allocates
+// nothing: every buffer beyond the caller's inputs and outputs lies in this
+// file-scope array, one copy per GPU, held until the process ends.  Host
+// code reaches it only through the address hipGetSymbolAddress returns on
+// each device (noalloc_scratch); the array's name is a host-side stand-in.
+static __device__ __align__(256) char scratch[RADG_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_dev[kMaxDevices] = {};
+
+// The current device's copy of scratch, looked up on the first call there.
+int noalloc_scratch(char **out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(RADG_ELIMITS, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(RADG_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<char *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return RADG_OK;
+}
+}  // namespace
+
+int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, double *wgts, double *wjac,
+                          void *hip_stream) {
+  if (!req || !wout || !wgts || !wjac) return fail(RADG_EINPUT, "null request or output");
+  if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
+  std::shared_ptr<TableSet> ts;
+  std::shared_ptr<CIA> cia;
+  AuxTables aux;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ts = g_tables;
+    cia = g_cia;
+    aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
+  }
+  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
+  if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
+  // making a table form allocates: it must exist already
+  if ((mode == RADG_MODE_MIXED && !ts->d_k32) || (mode == RADG_MODE_BF16 && !ts->d_k16))
+    return fail(RADG_EINPUT, "mode %d: its table form is made by radg_set_mode(%d) after registration", mode, mode);
+  Plan pl;
+  int rc = plan_batch(1, req, mode, *ts, aux, pl);
+  if (rc) return rc;
+  // inputs, intermediates and per-path outputs; the weighted outputs are the caller's
+  const size_t need = pl.o_wout;
+  if (need > RADG_NOALLOC_BYTES)
+    return fail(RADG_ELIMITS, "the request needs %zu bytes of scratch; the library holds %zu "
+                "(split its wavenumbers over calls)", need, (size_t)RADG_NOALLOC_BYTES);
+  const int fp64 = mode == RADG_MODE_FP64;
+  const int E = radg_layer_E(ts->ng);
+  const size_t ldsA = radg_lds_layer_bytes(ts->ng, pl.maxngas, E, fp64);
+  if (ldsA > 160 * 1024) return fail(RADG_ELIMITS, "LDS need %zu bytes exceeds 160 KiB", ldsA);
+  char *m = nullptr;
+  rc = noalloc_scratch(&m);
+  if (rc) return rc;
+  std::vector<char> h(pl.o_inter);
+  stage_inputs(pl, 1, req, h.data());
+  hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+  // from pageable memory: the copy is staged before the call returns
+  HIPCHK(hipMemcpyAsync(m, h.data(), pl.o_inter, hipMemcpyHostToDevice, st));
+  KernelArgs a{};
+  set_kernel_args(a, m, pl, 1, *ts, *cia, aux, mode);
+  a.wout = wout;
+  a.wgts = wgts;
+  a.wjac = wjac;
+  a.err = nullptr;
+  HIPCHK(radg_launch_layer_lds(a, mode, E, ldsA, st));
+  for (const auto &ec : emission_classes(pl, req, m, fp64)) {
+    KernelArgs e = a;
+    e.startB = ec.start;
+    e.nB = ec.nB;
+    e.reqmapB = ec.reqmap;
+    e.nreqB = ec.nreq;
+    HIPCHK(radg_launch_emission(e, fp64, ec.RL, ec.disc, ec.lds, st));
+  }
+  HIPCHK(radg_launch_quadrature(a, fp64, st));
+  return RADG_OK;
+}
 
 int radg_batch_launch(radg_batch *b, void *stream) {
   if (!b) return fail(RADG_EINPUT, "null batch");

@@ -29,6 +29,8 @@
  *   radg_set_mode            (none: the reference's precision is fixed at compile time)
  *   radg_batch_*, radg_server_*  (none: device-resident batches and the serving loop
  *                            that replace nemesisPY/nemesis.py:4586-4664's process pool)
+ *   radg_cirsconv            radtran/cirsrad/cirsconv.f:12-246 as the caller's tail applies
+ *                            it (radtran/cirsradg/cirsrtfg_wave.f:468-516), many spectra at once
  *   radg_batch_set_statemaps / _read_mapped  the caller's tail of the wavenumber loop,
  *                            map2pro + map2xvec (radtran/cirsradg/cirsrtfg_wave.f:403-425,
  *                            map2pro.f:49-84, map2xvec.f:41-54)
@@ -339,6 +341,33 @@ int radg_batch_timing_stats(radg_batch *b, int *nlaunch, float *sum_ms3);
  * nodes; plus the number of spectra and of layer-kernel work items. */
 int radg_batch_info(radg_batch *b, double *alg_bytes3, int *nspectra, int *nitems_a);
 int radg_batch_destroy(radg_batch *b);
+
+/* cirsconv.f (radtran/cirsrad/cirsconv.f:12-246), the convolution the caller's
+ * tail applies to every path's radiance, surface gradient and state-vector
+ * Jacobian column (radtran/cirsradg/cirsrtfg_wave.f:468-516), for ncol
+ * spectra at once: y[nwave][ncol] (device, FP64; wavenumber-major, as a
+ * batch's wout and kk) at the increasing wavenumbers vwave[nwave] ->
+ * yout[nconv][ncol] (device) at vconv[nconv].
+ *   fwhm > 0: the boxcar mean of the natural cubic spline through the
+ *     spectrum (cspline.f, csplint.f) over [vconv - w/2, vconv + w/2], a
+ *     20-point trapezoid divided by fwhm, with w = fwhm or, with nfwhm > 0
+ *     (FWHMEXIST), verint.f of the table vfwhm/xfwhm at vconv;
+ *   fwhm = 0: yout(i) = y(i) (nconv <= nwave);
+ *   fwhm < 0: the nfil filter functions of the runname.fil file -- filter k
+ *     centred on fil_vcen[k] (matching a vconv within 1e-5) with fil_n[k]
+ *     points, its wavenumbers and values concatenated in fil_v / fil_f.
+ * The map is formed once per call on the host in FP64 and applied in one
+ * launch; the call returns when yout is written. */
+typedef struct radg_conv {
+  double fwhm;
+  int nfwhm;
+  const double *vfwhm, *xfwhm;
+  int nfil;
+  const int *fil_n;
+  const double *fil_vcen, *fil_v, *fil_f;
+} radg_conv;
+int radg_cirsconv(int nwave, const double *vwave, int nconv, const double *vconv,
+                  const radg_conv *conv, int ncol, const double *y, double *yout, void *hip_stream);
 
 /* The forward model and analytic Jacobian of one request with NO device
  * allocation.  No counterpart in the reference: for callers that cannot

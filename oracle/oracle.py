@@ -276,6 +276,30 @@ def tabcont(gascon, which, nu):
     return f(C.addressof(c), float(nu))
 
 
+def cirsconv(fwhm, vwave, y, vconv, vfwhm=None, xfwhm=None, filters=None):
+    """cirsconv.f on one spectrum ``y`` at ``vwave`` -> the values at
+    ``vconv``; ``vfwhm``/``xfwhm``: the FWHM table (FWHMEXIST); ``filters``
+    (fwhm < 0): a list of (vcentral, v[], f[])."""
+    vw, yy, vc = _d(vwave), _d(y), _d(vconv)
+    out = np.zeros(vc.shape[0])
+    nf = 0 if vfwhm is None else len(vfwhm)
+    vf = _d(vfwhm if nf else [0.0])
+    xf = _d(xfwhm if nf else [0.0])
+    filters = filters or []
+    fn = np.array([len(f[1]) for f in filters] or [0], dtype=np.int32)
+    fc = _d([f[0] for f in filters] or [0.0])
+    fv = _d(np.concatenate([f[1] for f in filters]) if filters else [0.0])
+    ff = _d(np.concatenate([f[2] for f in filters]) if filters else [0.0])
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    f = lib().orc_cirsconv
+    f.restype = C.c_int
+    rc = f(C.c_double(fwhm), C.c_int(vw.shape[0]), p(vw), p(yy), C.c_int(vc.shape[0]), p(vc),
+           p(out), C.c_int(nf), p(vf), p(xf), C.c_int(len(filters)), p(fn), p(fc), p(fv), p(ff))
+    if rc != 0:
+        raise RuntimeError(f"oracle cirsconv returned status {rc}")
+    return out
+
+
 def ciaread(cia, vv, temp):
     lb = lib()
     ct, kc = _d(cia.temps), _d(cia.kcia)

@@ -25,6 +25,7 @@
  *   dpexp       radtran/radtran/dpexp.f:1-14
  *   interpem    radtran/cirsrad/interpem.f:1-37
  *   interp      radtran/rtm_util/interp.f:1-26 (dust cross-sections)
+ *   cirsconv    radtran/cirsrad/cirsconv.f:12-246 (+ rtm_util/cspline.f, csplint.f)
  *   cirsradg    radtran/cirsradg/cirsradg_wave.f:333-1290 (imod 0, 1, 3)
  *
  * Everything that the reference declares REAL is double here, matching the
@@ -581,6 +582,140 @@ double orc_interp(const double *x, const double *y, int n, double xin) {
   if (i > n) i = n;
   if (i == 1) i = 2;
   return y[i - 2] + (y[i - 1] - y[i - 2]) * (xin - x[i - 2]) / (x[i - 1] - x[i - 2]);
+}
+
+/* --------------------------------------------------------------- cirsconv */
+/* cspline.f:29-72 with natural ends (yp1, ypn > 0.99e30): second
+ * derivatives y2[n] of the cubic spline through (x, y); u is scratch [n]. */
+static void orc_cspline(const double *x, const double *y, int n, double *y2, double *u) {
+  int i, k;
+  y2[0] = 0.0;
+  u[0] = 0.0;
+  for (i = 1; i < n - 1; i++) {
+    const double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+    const double p = sig * y2[i - 1] + 2.;
+    y2[i] = (sig - 1.) / p;
+    if (x[i - 1] == x[i] || x[i] == x[i + 1]) {
+      u[i] = u[i - 1];
+      continue;
+    }
+    u[i] = (6. * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) /
+                (x[i + 1] - x[i - 1]) -
+            sig * u[i - 1]) /
+           p;
+  }
+  y2[n - 1] = (0.0 - 0.0 * u[n - 2]) / (0.0 * y2[n - 2] + 1.);
+  for (k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+/* csplint.f:10-40: the spline at x by bisection for its interval; results
+ * within 1e-37 of zero become 1e-37 (:34-38) */
+static double orc_csplint(const double *xa, const double *ya, const double *y2a, int n, double x) {
+  int klo = 1, khi = n;
+  double h, a, b, y1;
+  while (khi - klo > 1) {
+    const int k = (khi + klo) / 2;
+    if (xa[k - 1] > x) khi = k;
+    else klo = k;
+  }
+  h = xa[khi - 1] - xa[klo - 1];
+  a = (xa[khi - 1] - x) / h;
+  b = (x - xa[klo - 1]) / h;
+  y1 = a * ya[klo - 1] + b * ya[khi - 1] +
+       ((a * a * a - a) * y2a[klo - 1] + (b * b * b - b) * y2a[khi - 1]) * (h * h) / 6.;
+  return fabs(y1) > 1.00e-37 ? y1 : 1.00e-37;
+}
+
+int orc_cirsconv(double fwhm, int nwave, const double *vwave, const double *y, int nconv,
+                 const double *vconv, double *yout, int nfwhm, const double *vfwhm,
+                 const double *xfwhm, int nfil, const int *fil_n, const double *fil_vcen,
+                 const double *fil_v, const double *fil_f) {
+  int i, j, k;
+  if (fwhm == 0.0) {  /* :142-146 */
+    for (i = 0; i < nconv; i++) yout[i] = y[i];
+    return ORC_OK;
+  }
+  if (nwave < 2) return ORC_ERR_LIMITS;  /* :70-77 sets yout(1) = 1 */
+  if (fwhm > 0.0) {   /* :79-140 */
+    const int nstep = 20;
+    double *xc = (double *)malloc(sizeof(double) * (nwave + 2) * 4);
+    double *yc = xc + (nwave + 2), *y2 = yc + (nwave + 2), *u = y2 + (nwave + 2);
+    int nc = nwave, nc1 = 0;
+    if (!xc) return ORC_ERR_ALLOC;
+    for (i = 0; i < nwave; i++) {
+      xc[i] = vwave[i];
+      yc[i] = y[i];
+    }
+    if (vconv[nconv - 1] > vwave[nwave - 1] - fwhm / 2.) {
+      xc[nc] = vconv[nconv - 1] + fwhm;
+      yc[nc] = y[nwave - 1] + ((y[nwave - 1] - y[nwave - 2]) / (vwave[nwave - 1] - vwave[nwave - 2]) *
+                               (xc[nc] - vwave[nwave - 1]));
+      nc++;
+    }
+    if (vconv[0] < vwave[0] + fwhm / 2.) {
+      for (i = nc; i >= 1; i--) {
+        xc[i] = xc[i - 1];
+        yc[i] = yc[i - 1];
+      }
+      xc[0] = vconv[0] - fwhm;
+      yc[0] = y[0] + ((y[1] - y[0]) / (vwave[1] - vwave[0]) * (xc[0] - vwave[0]));
+      nc++;
+    }
+    for (i = 0; i < nc; i++)  /* NaN points are dropped (:101-108) */
+      if (!isnan(yc[i])) {
+        xc[nc1] = xc[i];
+        yc[nc1] = yc[i];
+        nc1++;
+      }
+    nc = nc1;
+    orc_cspline(xc, yc, nc, y2, u);
+    for (i = 0; i < nconv; i++) {
+      double yf = fwhm, x1, delx, sum = 0.0, yold = 0.0;
+      if (nfwhm > 0) yf = orc_verint(vfwhm, xfwhm, nfwhm, vconv[i]);
+      x1 = vconv[i] - yf / 2.;
+      delx = ((vconv[i] + yf / 2.) - x1) / (double)(nstep - 1);
+      for (j = 1; j <= nstep; j++) {
+        const double xi = x1 + (j - 1) * delx;
+        const double yi = orc_csplint(xc, yc, y2, nc, xi);
+        if (j == 1) sum = 0.;
+        else sum = sum + (yi + yold) * (delx / 2.);
+        yold = yi;
+      }
+      yout[i] = sum / fwhm;
+    }
+    free(xc);
+    return ORC_OK;
+  }
+  /* fwhm < 0, :148-243: filter functions, normalised over the wavenumber
+   * intervals (xnorm) */
+  {
+    int off = 0, done = 0;
+    char *seen = (char *)calloc((size_t)nconv, 1);
+    if (!seen) return ORC_ERR_ALLOC;
+    for (k = 0; k < nfil; k++) {
+      const double *fv = fil_v + off, *ff = fil_f + off;
+      const int ns = fil_n[k];
+      for (i = 0; i < nconv; i++) {
+        double sum = 0.0, sumf = 0.0;
+        if (!(fabs(fil_vcen[k] - vconv[i]) < 0.00001)) continue;
+        for (j = 0; j < nwave - 1; j++) {
+          double yy = 0.0;
+          if (vwave[j] >= fv[0] && vwave[j] <= fv[ns - 1]) yy = orc_interp(fv, ff, ns, vwave[j]);
+          if (yy != 0) {
+            const double delv = vwave[j + 1] - vwave[j];
+            sum = sum + yy * y[j] * delv;
+            sumf = sumf + yy * delv;
+          }
+        }
+        yout[i] = sum / sumf;
+        if (!seen[i]) done++;
+        seen[i] = 1;
+      }
+      off += ns;
+    }
+    free(seen);
+    return done == nconv ? ORC_OK : ORC_ERR_LIMITS;
+  }
 }
 
 /* ---------------------------------------------------------------- Rayleigh */

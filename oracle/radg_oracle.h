@@ -152,6 +152,18 @@ double orc_rayleigh(int iray, double v, double p, double t, double fheh2,
  * cm4 dyne-1 (hmin_ff.f), and bound-free cross-section at v0 (hmin_bf.f) */
 double orc_hmin_ff(double v0, double temp);
 double orc_hmin_bf(double v0);
+/* cirsconv.f:12-246 on one spectrum y[nwave] at vwave -> yout[nconv] at
+ * vconv: fwhm > 0 the boxcar of width fwhm (or, with nfwhm > 0, of the width
+ * verint.f gives from vfwhm/xfwhm at each vconv) integrated over cspline.f /
+ * csplint.f's natural spline by a 20-point trapezoid and divided by fwhm;
+ * fwhm = 0 the identity; fwhm < 0 the nfil filter functions of the .fil file
+ * (filter k: centre fil_vcen[k], fil_n[k] points fil_v/fil_f, concatenated).
+ * Returns ORC_ERR_LIMITS for nwave < 2 with fwhm != 0 and for a conv point
+ * no filter matches. */
+int orc_cirsconv(double fwhm, int nwave, const double *vwave, const double *y, int nconv,
+                 const double *vconv, double *yout, int nfwhm, const double *vfwhm,
+                 const double *xfwhm, int nfil, const int *fil_n, const double *fil_vcen,
+                 const double *fil_v, const double *fil_f);
 /* ch4cont.f / nh3cont.f on the registered tables, in 1e20 cm2/molecule */
 double orc_ch4cont(const orc_cia *cia, double nu);
 double orc_nh3cont(const orc_cia *cia, double nu);

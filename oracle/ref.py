@@ -118,6 +118,42 @@ def tabcont(prec, which, v):
     return float(fn(C.byref(h)))
 
 
+def cirsconv(prec, fwhm, vwave, y, vconv, vfwhm=None, xfwhm=None, filters=None, workdir=None):
+    """The reference's own CIRSCONV (``radtran/cirsrad/cirsconv.f``) on one
+    spectrum.  ``vfwhm``/``xfwhm``: the FWHMEXIST table; ``filters`` (for
+    fwhm < 0): a list of (vcentral, v[], f[]) written as ``runname.fil``."""
+    lb = _load(prec)
+    rdt = np.float64 if prec == 64 else np.float32
+    tmp = None
+    if workdir is None:
+        tmp = tempfile.TemporaryDirectory(prefix="radconv")
+        workdir = tmp.name
+    runname = os.path.join(workdir, "conv")
+    if filters is not None:
+        with open(runname + ".fil", "w") as f:
+            f.write(f"{len(filters)}\n")
+            for vc, fv, ff in filters:
+                f.write(f"{float(vc)!r}\n{len(fv)}\n")
+                for a, b in zip(fv, ff):
+                    f.write(f"{float(a)!r} {float(b)!r}\n")
+    name = runname.ljust(100).encode()
+    vw = np.ascontiguousarray(vwave, dtype=rdt)
+    yy = np.ascontiguousarray(y, dtype=rdt)
+    vc = np.ascontiguousarray(vconv, dtype=rdt)
+    yout = np.zeros(len(vc), dtype=rdt)
+    nf = 0 if vfwhm is None else len(vfwhm)
+    vf = np.ascontiguousarray(vfwhm if nf else [0.0], dtype=rdt)
+    xf = np.ascontiguousarray(xfwhm if nf else [0.0], dtype=rdt)
+    args = [C.c_char_p(name), _p(np.array([fwhm], dtype=rdt)), _p(np.array([len(vw)], np.int32)),
+            _p(vw), _p(yy), _p(np.array([len(vc)], np.int32)), _p(vc), _p(yout),
+            _p(np.array([1 if nf else 0], np.int32)), _p(np.array([max(nf, 1)], np.int32)),
+            _p(vf), _p(xf), C.c_size_t(100)]
+    lb.cirsconv_(*args)
+    if tmp is not None:
+        tmp.cleanup()
+    return yout.astype(np.float64)
+
+
 def h2o_tables():
     """The CKD H2O continuum data of ngascon.f's BLOCK DATA as the REAL(8)
     build holds it (the harness's HGETH2O): grid (vb1, vt1, dv1), npt,

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libradg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["radg_kernels.hip", "radg_api.hip", "radg_server.hip", "radg_oe.hip"]
+SOURCES = ["radg_kernels.hip", "radg_api.hip", "radg_server.hip", "radg_oe.hip", "radg_conv.hip"]
 
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
                 "-packed-fp32-ops"]

@@ -16,6 +16,8 @@ Reference correspondence:
   :func:`register_n2nir` -- N2's near-infrared CIA tables;
   :func:`register_gascon` -- ``/GASCONDEF/`` with ngascon.f's tables.
 * :class:`Batch` -- many requests resident on the device, launched together.
+* :func:`cirsconv` -- ``cirsconv.f``, the convolution of the caller's tail
+  (``cirsrtfg_wave.f:468-516``), for many spectra at once on the device.
 
 Errors raise :class:`RadgError` where the reference would STOP.
 """
@@ -137,6 +139,57 @@ def register_gascon(g) -> None:
                  float(g.h2o_v0), float(g.h2o_dv), float(g.h2o_c2), float(g.h2o_ts),
                  float(g.h2o_ps), s96, s60, f96, nc, nn, cw, ck, nw, nk)
     check(lib().radg_register_gascon(C.byref(st)))
+
+
+class _Conv(C.Structure):
+    _fields_ = [("fwhm", C.c_double), ("nfwhm", C.c_int), ("vfwhm", C.c_void_p),
+                ("xfwhm", C.c_void_p), ("nfil", C.c_int), ("fil_n", C.c_void_p),
+                ("fil_vcen", C.c_void_p), ("fil_v", C.c_void_p), ("fil_f", C.c_void_p)]
+
+
+def cirsconv(vwave, y, vconv, fwhm, vfwhm=None, xfwhm=None, filters=None, stream=None):
+    """``cirsconv.f`` (``radtran/cirsrad/cirsconv.f:12-246``) on the device for
+    many spectra at once: ``y`` [nwave] or [nwave, ncol] at ``vwave`` ->
+    [nconv] or [nconv, ncol] at ``vconv``.  ``y`` may be a numpy array (the
+    result is one too) or a CUDA ``torch`` tensor of float64 (the result
+    stays on the device).  ``vfwhm``/``xfwhm``: the FWHM table
+    (FWHMEXIST); ``filters`` (fwhm < 0): a list of (vcentral, v[], f[]), the
+    ``runname.fil`` content."""
+    import torch
+    vw = np.ascontiguousarray(vwave, dtype=np.float64)
+    vc = np.ascontiguousarray(vconv, dtype=np.float64)
+    on_dev = isinstance(y, torch.Tensor)
+    yd = y if on_dev else torch.from_numpy(np.ascontiguousarray(y, dtype=np.float64)).cuda()
+    if yd.dtype != torch.float64 or not yd.is_cuda or not yd.is_contiguous():
+        raise ValueError("y must be a contiguous float64 CUDA tensor or a numpy array")
+    one = yd.dim() == 1
+    ncol = 1 if one else int(yd.shape[1])
+    if yd.shape[0] != vw.shape[0]:
+        raise ValueError("y's first axis must match vwave")
+    out = torch.empty((vc.shape[0],) if one else (vc.shape[0], ncol), dtype=torch.float64,
+                      device=yd.device)
+    keep = []
+
+    def arr(x, dt=np.float64):
+        a = np.ascontiguousarray(x, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    cv = _Conv(float(fwhm), 0, None, None, 0, None, None, None, None)
+    if vfwhm is not None:
+        cv.nfwhm, cv.vfwhm, cv.xfwhm = len(vfwhm), arr(vfwhm), arr(xfwhm)
+    if filters:
+        cv.nfil = len(filters)
+        cv.fil_n = arr([len(f[1]) for f in filters], np.int32)
+        cv.fil_vcen = arr([f[0] for f in filters])
+        cv.fil_v = arr(np.concatenate([f[1] for f in filters]))
+        cv.fil_f = arr(np.concatenate([f[2] for f in filters]))
+    st = torch.cuda.current_stream().cuda_stream if stream is None else (
+        stream if isinstance(stream, int) else stream.cuda_stream)
+    check(lib().radg_cirsconv(int(vw.shape[0]), _vp(vw), int(vc.shape[0]), _vp(vc), C.byref(cv),
+                              int(ncol), C.c_void_p(yd.data_ptr()), C.c_void_p(out.data_ptr()),
+                              C.c_void_p(st)))
+    return out if on_dev else out.cpu().numpy()
 
 
 def register_solar(s) -> None:

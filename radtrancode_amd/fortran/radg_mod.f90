@@ -23,6 +23,7 @@ module radg
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
   public :: radg_batch_device_outputs
   public :: radg_oe_create, radg_oe_cost, radg_oe_propose, radg_oe_destroy
+  public :: radg_cirsconv
 
   ! radg_request of include/radg.h: sizes, then C addresses (c_loc) of the
   ! caller's arrays in the compact 0-based layout the header documents.
@@ -56,6 +57,16 @@ module radg
     type(c_ptr) :: h2o_s296, h2o_s260, h2o_f296
     integer(c_int) :: nch4, nnh3
     type(c_ptr) :: ch4_wl, ch4_k, nh3_wl, nh3_k
+  end type
+
+  ! radg_conv of include/radg.h: cirsconv.f's fwhm, FWHMEXIST table and
+  ! (fwhm < 0) the .fil filter functions
+  type, bind(C), public :: radg_conv
+    real(c_double) :: fwhm
+    integer(c_int) :: nfwhm
+    type(c_ptr) :: vfwhm, xfwhm
+    integer(c_int) :: nfil
+    type(c_ptr) :: fil_n, fil_vcen, fil_v, fil_f
   end type
 
   ! radg_statemap: the /LGRAD/ DAM, DTE and DCO and map2xvec's xmap of one
@@ -131,6 +142,18 @@ module radg
     integer(c_int) function radg_register_gascon(g) bind(C, name="radg_register_gascon")
       import :: c_int, radg_gascon
       type(radg_gascon), intent(in) :: g
+    end function
+
+    ! cirsconv.f for ncol spectra at once: y(ncol,nwave) -> yout(ncol,nconv),
+    ! both device arrays (c_loc of memory the caller holds on the GPU, e.g. a
+    ! batch's radg_batch_device_outputs)
+    integer(c_int) function radg_cirsconv(nwave, vwave, nconv, vconv, conv, ncol, y, yout, &
+        stream) bind(C, name="radg_cirsconv")
+      import :: c_int, c_double, c_ptr, radg_conv
+      integer(c_int), value :: nwave, nconv, ncol
+      real(c_double) :: vwave(*), vconv(*)
+      type(radg_conv), intent(in) :: conv
+      type(c_ptr), value :: y, yout, stream
     end function
 
     ! /SOLARDAT/: iform, iread (iread = 999 reads the registered spectrum)

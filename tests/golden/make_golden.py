@@ -438,6 +438,39 @@ def gascon_vectors(n=64, seed=12):
                 ref64_nh3=np.array([ref.tabcont(64, "nh3", x) for x in vt]))
 
 
+def conv_vectors(seed=21):
+    """The reference's own CIRSCONV on seeded spectra: boxcars that pass
+    either end of the grid (extrapolated end points), a wide one, an FWHM
+    table (FWHMEXIST), filter functions (fwhm < 0, a runname.fil file) and
+    fwhm = 0, on an irregular 120-point grid; REAL(8) build."""
+    rng = np.random.default_rng(seed)
+    v = np.unique(np.sort(600.0 + 100.0 * rng.random(120)))
+    y = 1e-7 * (1 + 0.3 * np.sin(v / 3.0)) * np.exp(rng.normal(0, 0.05, v.size))
+    out = dict(v=v, y=y)
+    cases = {"box_inside": (5.0, np.linspace(606.0, 694.0, 23), None),
+             "box_ends": (5.0, np.linspace(600.0, 700.0, 21), None),
+             "box_wide": (12.5, np.linspace(598.0, 702.0, 9), None),
+             "box_table": (5.0, np.linspace(605.0, 695.0, 19), (np.array([550., 650., 750.]),
+                                                               np.array([3.0, 6.0, 4.0]))),
+             "identity": (0.0, v[:40].copy(), None)}
+    for name, (fw, vc, tab) in cases.items():
+        vf, xf = (None, None) if tab is None else tab
+        out[f"{name}_fwhm"] = np.array(fw)
+        out[f"{name}_vconv"] = vc
+        if tab is not None:
+            out[f"{name}_vfwhm"], out[f"{name}_xfwhm"] = vf, xf
+        out[f"{name}_ref64"] = ref.cirsconv(64, fw, v, y, vc, vf, xf)
+    vc = np.linspace(610.0, 690.0, 9)
+    fil = [(c, np.linspace(c - 4.0, c + 4.0, 17), np.exp(-0.5 * (np.linspace(-4, 4, 17) / 1.7) ** 2))
+           for c in vc]
+    out["filter_vconv"] = vc
+    out["filter_n"] = np.array([len(f[1]) for f in fil], dtype=np.int32)
+    out["filter_v"] = np.concatenate([f[1] for f in fil])
+    out["filter_f"] = np.concatenate([f[2] for f in fil])
+    out["filter_ref64"] = ref.cirsconv(64, -1.0, v, y, vc, filters=fil)
+    return out
+
+
 def add_dust(r, ncont, nsec, seed, peak_tau=(1.5, 0.4)):
     """Dust on a request (cirsradg_wave.f:550-577): particle type 0 is a
     cloud deck near 0.7 bar, type 1 a haze spread over the upper layers;
@@ -571,6 +604,8 @@ def main(names=()):
         save_n2nir()
     if not names or "rayleigh" in names:
         np.savez_compressed(os.path.join(HERE, "rayleigh.npz"), **rayleigh_vectors())
+    if not names or "conv" in names:
+        np.savez_compressed(os.path.join(HERE, "conv.npz"), **conv_vectors())
     if not names or "gascon" in names:
         gascon_tables()
         np.savez_compressed(os.path.join(HERE, "gascon_funcs.npz"), **gascon_vectors())

@@ -14,6 +14,7 @@
  *                            from .kta headers (radtran/cirsrad/read_khead.f:18-120)
  *   radg_register_ktables    the same registry from arrays already in memory
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
+ *   radg_read_cia            the same from the table file ciaread.f:77-89 reads
  *   radg_register_parah2     /FPTABLE/ filled by radtran/radtran/fpread_all.f:72-143
  *   radg_register_e3         /E3TABLE/ filled by radtran/radtran/e3interp.f:30-51
  *   radg_register_solar      /SOLARDAT/ filled for radtran/cirsrad/get_solar_wave.f:1-46
@@ -91,15 +92,22 @@ const char *radg_last_error(void);
 
 /* Register the k-tables of one list (replaces any earlier registration).
  * All tables share one P/T/g grid, as read_klist.f:117-160 requires.
- *   press_atm[np], temp[nt], g_ord[ng], delg[ng]
- *   per table t: idgas[t], isogas[t], npoint[t], vmin[t], delv[t] (> 0) and
- *   k[t] -> npoint[t]*np*nt*ng values, k*1e20, g fastest then T then P
- *   (the .kta record order, calc_fnktablec.f:593-609). */
+ *   press_atm[np], g_ord[ng], delg[ng]; nt > 0: temp[nt], one temperature
+ *   grid; nt < 0 (the .kta's NT < 0): temp[np][|nt|], a grid per pressure
+ *   (read_khead.f:86-92, get_kg.f:145-173)
+ *   per table t: idgas[t], isogas[t], npoint[t], vmin[t], delv[t] and
+ *   k[t] -> npoint[t]*np*|nt|*ng values, k*1e20, g fastest then T then P
+ *   (the .kta record order, calc_fnktablec.f:593-609);
+ *   delv[t] <= 0: an irregular wavenumber grid vcen[t] -> npoint[t]
+ *   increasing values (read_khead.f:93-100), looked up as read_klist.f:
+ *   292-330 does (DELV < 0 the nearest point; DELV = 0 the point at or below,
+ *   merged with the next).  vcen may be NULL when every delv > 0.  Regular
+ *   and irregular tables do not mix in one list. */
 int radg_register_ktables(int ntab, const int *idgas, const int *isogas,
                           const int *npoint, const double *vmin, const double *delv,
                           int np, int nt, int ng, const double *press_atm,
                           const double *temp, const double *g_ord, const double *delg,
-                          const double *const *k);
+                          const double *const *k, const double *const *vcen);
 
 /* Read the .kta files named in a .kls list (one path per line) and register
  * them (read_klist.f:30-110, read_khead.f:18-120).  recl is the record
@@ -117,6 +125,12 @@ int radg_ktable_bf16(int itab, unsigned short *codes, double *off_sc);
 /* /CIATABLE/: temps[25] (K), kcia[1501][25][9] in C order, which is the
  * Fortran kcia(9,25,1501) (ciacom.f:9); wavenumber grid (i-1)*dnu. */
 int radg_register_cia(double dnu, const double *temps, const double *kcia);
+
+/* /CIATABLE/ from the table file ciaread.f:77-89 reads (the name and dnu of
+ * the run's .cia file): an unformatted sequential file of the 25 REAL*8
+ * temperatures and kcia(9,25,1501) in REAL*4 (as the raddata tables hold it) or
+ * REAL*8.  Registers it as radg_register_cia does. */
+int radg_read_cia(const char *path, double dnu);
 
 /* /DUST/ for the fixed-argument entry points (cirsradg_wave_,
  * radg_spectrum_r4/_r8), whose cont(maxcon,maxlay) argument then holds

@@ -46,7 +46,8 @@ class _KT(C.Structure):
                 ("np", C.c_int), ("nt", C.c_int), ("ng", C.c_int),
                 ("vmin", C.c_double), ("delv", C.c_double),
                 ("lnp", C.c_void_p), ("tk", C.c_void_p), ("g_ord", C.c_void_p),
-                ("delg", C.c_void_p), ("k", C.c_void_p)]
+                ("delg", C.c_void_p), ("k", C.c_void_p), ("t2", C.c_void_p),
+                ("vcen", C.c_void_p)]
 
 
 class _CIA(C.Structure):
@@ -101,9 +102,13 @@ class _Pack:
         kts = (_KT * max(1, len(tables)))()
         for t, kt in enumerate(tables):
             arrs = [_d(np.log(kt.press)), _d(kt.temp), _d(kt.g_ord), _d(kt.delg), _d(kt.k)]
-            self.keep += arrs
-            kts[t] = _KT(kt.idgas, kt.isogas, kt.npoint, kt.np_, kt.nt, kt.ng,
-                         kt.vmin, kt.delv, *[a.ctypes.data for a in arrs])
+            extra = [None if x is None else _d(x) for x in (getattr(kt, "temp2", None),
+                                                             getattr(kt, "vcen", None))]
+            self.keep += arrs + [x for x in extra if x is not None]
+            kts[t] = _KT(kt.idgas, kt.isogas, kt.npoint, kt.np_,
+                         kt.nt_signed if hasattr(kt, "nt_signed") else kt.nt, kt.ng,
+                         kt.vmin, kt.delv, *[a.ctypes.data for a in arrs],
+                         *[None if x is None else x.ctypes.data for x in extra])
         self.kts, self.ntab = kts, len(tables)
         ct, kc = _d(cia.temps), _d(cia.kcia)
         self.keep += [ct, kc]

@@ -1321,39 +1321,78 @@ static int orc_interpem(int nem, const double *vem, const double *em, double x,
 /* ----------------------------------------------------------------- get_kg */
 /* Table for gas igas at vwave: the first registered table with the gas's
  * (id, iso) whose grid covers vwave (read_klist.f:260-290, findloc.f). */
+/* read_klist.f:260-330: the first regular table of the gas covering the
+ * wavenumber, or the gas's irregular table (delv <= 0), which covers every
+ * wavenumber (the reference lets the last such table win) */
 static const orc_ktable *find_table(const orc_ktable *tabs, int ntab, int id,
                                     int iso, double vwave) {
   int t;
+  const orc_ktable *irr = NULL, *reg = NULL;
   for (t = 0; t < ntab; t++) {
     const orc_ktable *kt = &tabs[t];
     double xmax = kt->vmin + (kt->npoint - 1) * kt->delv;
     double tol = (kt->delv >= 0.0001) ? 0.02 * kt->delv : 0.25 * kt->delv;
-    if (kt->idgas != id || kt->isogas != iso || kt->delv <= 0.0) continue;
+    if (kt->idgas != id || kt->isogas != iso) continue;
+    if (kt->delv <= 0.0) {
+      irr = kt;
+      continue;
+    }
     /* grid ends widened by get_kg's coincidence tolerance (get_kg.f:199-206) */
-    if (vwave >= kt->vmin - tol && vwave <= xmax + tol) return kt;
+    if (!reg && vwave >= kt->vmin - tol && vwave <= xmax + tol) reg = kt;
   }
-  return NULL;
+  return irr ? irr : reg;
+}
+
+/* read_klist.f:292-330: the slice of an irregular table for wavenumber v.
+ * delv < 0: the point nearest v in per cent (the first of equals); delv = 0:
+ * the nearest point at or below v, with the lower slice's weight frac.
+ * Returns -1 when no point is at or below v (the reference stops). */
+static int orc_irregular_slice(const orc_ktable *kt, double v, double *frac) {
+  double best = 1e10;
+  int ipo, jpo = -1;
+  *frac = 1.0;
+  for (ipo = 0; ipo < kt->npoint; ipo++) {
+    const double dx = 100 * fabs((v - kt->vcen[ipo]) / kt->vcen[ipo]);
+    if (kt->delv == 0.0) {
+      if (dx < best && kt->vcen[ipo] <= v) {
+        jpo = ipo;
+        *frac = ipo < kt->npoint - 1
+                    ? 1.0 - (v - kt->vcen[ipo]) / (kt->vcen[ipo + 1] - kt->vcen[ipo])
+                    : 1.;
+        best = dx;
+      }
+    } else if (dx < best) {
+      jpo = ipo;
+      best = dx;
+    }
+  }
+  return jpo;
 }
 
 /* get_kg.f:1-484 for one gas at all layers.  kout/dkoutdt are [L][ng]. */
-static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press,
-                           const double *temp, double vwave, double *kout,
-                           double *dkoutdt) {
-  const int np = kt->np, nt = kt->nt, ng = kt->ng;
+static int orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press,
+                          const double *temp, double vwave, double *kout,
+                          double *dkoutdt) {
+  const int np = kt->np, nt = kt->nt < 0 ? -kt->nt : kt->nt, ng = kt->ng, t2 = kt->nt < 0;
   const size_t ntab = (size_t)np * nt * ng;
   const double pmin = kt->lnp[0], pmax = kt->lnp[np - 1];
-  const double tmin = kt->tk[0], tmax = kt->tk[nt - 1];
-  double eps, tmp, frac;
+  const double tmin = t2 ? 0.0 : kt->tk[0], tmax = t2 ? 0.0 : kt->tk[nt - 1];
+  double eps = 0.0, tmp = 0.0, frac, frack = 1.0;
   long n1;
   int coinc, il, i;
   const double *table, *table2;
   double ktest = 0.0;
 
-  if (kt->delv >= 0.0001) eps = 0.02 * kt->delv;
-  else eps = 0.25 * kt->delv;
-  n1 = (long)((vwave + eps - kt->vmin) / kt->delv); /* IDINT truncates */
-  if (n1 < 0) n1 = 0;
-  if (n1 > kt->npoint - 1) n1 = kt->npoint - 1;
+  if (kt->delv > 0.0) {
+    if (kt->delv >= 0.0001) eps = 0.02 * kt->delv;
+    else eps = 0.25 * kt->delv;
+    n1 = (long)((vwave + eps - kt->vmin) / kt->delv); /* IDINT truncates */
+    if (n1 < 0) n1 = 0;
+    if (n1 > kt->npoint - 1) n1 = kt->npoint - 1;
+  } else {
+    n1 = orc_irregular_slice(kt, vwave, &frack);
+    if (n1 < 0) return ORC_ERR_LIMITS;
+  }
   table = kt->k + (size_t)n1 * ntab;
 
   for (i = 0; i < (int)ntab; i++) {
@@ -1365,16 +1404,20 @@ static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press
   if (ktest == 0.0) {
     memset(kout, 0, sizeof(double) * nlayer * ng);
     memset(dkoutdt, 0, sizeof(double) * nlayer * ng);
-    return;
+    return ORC_OK;
   }
-  tmp = kt->vmin + kt->delv * n1;
-  frac = fabs(vwave - tmp);
-  coinc = frac < eps;
+  if (kt->delv > 0.0) {
+    tmp = kt->vmin + kt->delv * n1;
+    frac = fabs(vwave - tmp);
+    coinc = frac < eps;
+  } else {
+    coinc = kt->delv < 0.0;  /* get_kg.f:267-281 */
+  }
   table2 = (n1 + 1 < kt->npoint) ? table + ntab : table;
 
   for (il = 0; il < nlayer; il++) {
-    double p1 = log(press[il]), t1, v, u, dudt;
-    int cp, ct, i1, i2, i3, i4;
+    double p1 = log(press[il]), t1, v, u, dudt, u2, dudt2;
+    int cp, ct, ct2, i1, i2, i3, i4;
     double kg[MAXG + 2], dkdt[MAXG + 2];
     if (p1 < pmin) p1 = pmin;
     if (p1 > pmax) p1 = pmax;
@@ -1382,18 +1425,43 @@ static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press
     if (cp < 1) cp = 1;
     if (cp >= np) cp = np - 1;
     v = (p1 - kt->lnp[cp - 1]) / (kt->lnp[cp] - kt->lnp[cp - 1]);
-    t1 = temp[il];
-    if (t1 < tmin) t1 = tmin;
-    if (t1 > tmax) t1 = tmax;
-    ct = orc_locate(kt->tk, nt, t1);
-    if (ct < 1) ct = 1;
-    if (ct >= nt) ct = nt - 1;
-    u = (t1 - kt->tk[ct - 1]) / (kt->tk[ct] - kt->tk[ct - 1]);
-    dudt = 1.0 / (kt->tk[ct] - kt->tk[ct - 1]);
-    /* IOFF, get_kg.f:140-143 (0-based offsets) */
+    if (!t2) {
+      t1 = temp[il];
+      if (t1 < tmin) t1 = tmin;
+      if (t1 > tmax) t1 = tmax;
+      ct = orc_locate(kt->tk, nt, t1);
+      if (ct < 1) ct = 1;
+      if (ct >= nt) ct = nt - 1;
+      u = (t1 - kt->tk[ct - 1]) / (kt->tk[ct] - kt->tk[ct - 1]);
+      dudt = 1.0 / (kt->tk[ct] - kt->tk[ct - 1]);
+      ct2 = ct;
+      u2 = u;
+      dudt2 = dudt;
+    } else {
+      /* get_kg.f:145-173: each bracketing pressure's own temperature grid */
+      const double *tn = kt->t2 + (size_t)(cp - 1) * nt;
+      t1 = temp[il];
+      if (t1 < tn[0]) t1 = tn[0];
+      if (t1 > tn[nt - 1]) t1 = tn[nt - 1];
+      ct = orc_locate(tn, nt, t1);
+      if (ct < 1) ct = 1;
+      if (ct >= nt) ct = nt - 1;
+      u = (t1 - tn[ct - 1]) / (tn[ct] - tn[ct - 1]);
+      dudt = 1.0 / (tn[ct] - tn[ct - 1]);
+      tn = kt->t2 + (size_t)cp * nt;
+      t1 = temp[il];
+      if (t1 < tn[0]) t1 = tn[0];
+      if (t1 > tn[nt - 1]) t1 = tn[nt - 1];
+      ct2 = orc_locate(tn, nt, t1);
+      if (ct2 < 1) ct2 = 1;
+      if (ct2 >= nt) ct2 = nt - 1;
+      u2 = (t1 - tn[ct2 - 1]) / (tn[ct2] - tn[ct2 - 1]);
+      dudt2 = 1.0 / (tn[ct2] - tn[ct2 - 1]);
+    }
+    /* IOFF, get_kg.f:140-143,168-171 (0-based offsets) */
     i1 = (nt * ng * (cp - 1)) + (ng * (ct - 1));
-    i2 = (nt * ng * cp) + (ng * (ct - 1));
-    i3 = (nt * ng * cp) + (ng * ct);
+    i2 = (nt * ng * cp) + (ng * (ct2 - 1));
+    i3 = (nt * ng * cp) + (ng * ct2);
     i4 = (nt * ng * (cp - 1)) + (ng * ct);
 
     if (coinc) {
@@ -1408,8 +1476,13 @@ static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press
           y3 = log(y3);
           y4 = log(y4);
         }
-        x = (1.0 - v) * (1.0 - u) * y1 + v * (1.0 - u) * y2 + v * u * y3 + (1.0 - v) * u * y4;
-        dxdt = (-(1.0 - v) * y1 - v * y2 + v * y3 + (1 - v) * y4) * dudt;
+        if (!t2) {
+          x = (1.0 - v) * (1.0 - u) * y1 + v * (1.0 - u) * y2 + v * u * y3 + (1.0 - v) * u * y4;
+          dxdt = (-(1.0 - v) * y1 - v * y2 + v * y3 + (1 - v) * y4) * dudt;
+        } else {  /* get_kg.f:352-357 */
+          x = (1.0 - v) * ((1.0 - u) * y1 + u * y4) + v * ((1.0 - u2) * y2 + u2 * y3);
+          dxdt = (1.0 - v) * (-y1 + y4) * dudt + v * (-y2 + y3) * dudt2;
+        }
         if (klog) {
           kg[i + 1] = exp(x);
           dkdt[i + 1] = exp(x) * dxdt;
@@ -1419,20 +1492,31 @@ static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press
         }
       }
     } else {
-      double weight2 = (vwave - tmp) / kt->delv, w[2];
+      double w[2];
       double cont[2 * MAXG + 2], dcont[2 * MAXG + 2], fac[2 * MAXG + 2];
       int loop, count = 0;
-      if (weight2 > 1.0) weight2 = 1.0;
-      if (weight2 < 0.0) weight2 = 0.0;
-      w[0] = 1.0 - weight2;
-      w[1] = weight2;
+      if (kt->delv > 0.0) {  /* get_kg.f:380-385 */
+        double weight2 = (vwave - tmp) / kt->delv;
+        if (weight2 > 1.0) weight2 = 1.0;
+        if (weight2 < 0.0) weight2 = 0.0;
+        w[0] = 1.0 - weight2;
+        w[1] = weight2;
+      } else {  /* :386-389 */
+        w[0] = frack;
+        w[1] = 1.0 - frack;
+      }
       for (loop = 0; loop < 2; loop++) {
         const double *tb = loop == 0 ? table : table2;
         for (i = 0; i < ng; i++) {
           double y1 = tb[i1 + i], y2 = tb[i2 + i], y3 = tb[i3 + i], y4 = tb[i4 + i];
-          double x = (1.0 - v) * (1.0 - u) * y1 + v * (1.0 - u) * y2 + v * u * y3 +
-                     (1.0 - v) * u * y4;
-          double dxdt = (-(1.0 - v) * y1 - v * y2 + v * y3 + (1 - v) * y4) * dudt;
+          double x, dxdt;
+          if (!t2) {
+            x = (1.0 - v) * (1.0 - u) * y1 + v * (1.0 - u) * y2 + v * u * y3 + (1.0 - v) * u * y4;
+            dxdt = (-(1.0 - v) * y1 - v * y2 + v * y3 + (1 - v) * y4) * dudt;
+          } else {
+            x = (1.0 - v) * ((1.0 - u) * y1 + u * y4) + v * ((1.0 - u2) * y2 + u2 * y3);
+            dxdt = (1.0 - v) * (-y1 + y4) * dudt + v * (-y2 + y3) * dudt2;
+          }
           count++;
           cont[count] = x;
           dcont[count] = dxdt;
@@ -1446,6 +1530,7 @@ static void orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press
       dkoutdt[il * ng + i] = isnan(dkdt[i + 1]) ? 1e-37 : dkdt[i + 1];
     }
   }
+  return ORC_OK;
 }
 
 /* ------------------------------------------------------------ cirsradg */
@@ -1581,9 +1666,11 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
 
   /* get_kg, cirsradg_wave.f:502-503 */
   for (k = 0; k < ngas; k++) {
-    if (gtab[k])
-      orc_get_kg_gas(gtab[k], L, r->press, r->temp, vwave, kout + (size_t)k * L * ng,
-                     dkoutdt + (size_t)k * L * ng);
+    if (gtab[k]) {
+      rc = orc_get_kg_gas(gtab[k], L, r->press, r->temp, vwave, kout + (size_t)k * L * ng,
+                          dkoutdt + (size_t)k * L * ng);
+      if (rc != ORC_OK) goto done;
+    }
   }
 
   /* per-layer continuum and overlap, cirsradg_wave.f:510-760 */

@@ -15,7 +15,9 @@
 extern "C" {
 #endif
 
-/* One gas's k-table (the content of a .kta file, read_khead.f:38-109). */
+/* One gas's k-table (the content of a .kta file, read_khead.f:38-109).
+ * nt < 0: one temperature grid per pressure, t2[np][|nt|] (tk unused);
+ * delv <= 0: an irregular wavenumber grid vcen[npoint] (vmin unused). */
 typedef struct orc_ktable {
   int idgas, isogas;
   int npoint, np, nt, ng;
@@ -24,7 +26,9 @@ typedef struct orc_ktable {
   const double *tk;    /* [nt]  K */
   const double *g_ord; /* [ng] */
   const double *delg;  /* [ng] */
-  const double *k;     /* [npoint][np][nt][ng], k * 1e20 */
+  const double *k;     /* [npoint][np][|nt|][ng], k * 1e20 */
+  const double *t2;    /* [np][|nt|] K when nt < 0 */
+  const double *vcen;  /* [npoint] when delv <= 0 */
 } orc_ktable;
 
 /* The /CIATABLE/ block (ciacom.f:9-18): kcia(9,25,1501) == [1501][25][9];

@@ -197,9 +197,17 @@ class RefRunner:
         rdt = self.rdt
         self.lib.hradinit_()
         dnu = np.array([cia.dnu], dtype=rdt)
-        temps = np.ascontiguousarray(cia.temps, dtype=rdt)
-        kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
-        self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
+        cfile = getattr(cia, "file", None)
+        if cfile is not None and prec == 32:
+            # ciaread.f reads the table file itself (its REAL*4 kcia record
+            # is what the REAL(4) build declares)
+            os.environ.setdefault("RADREPO", REF_TREE)
+            nb = cfile.encode()
+            self.lib.hsetciafile_(C.c_char_p(nb), _p(dnu), C.c_size_t(len(nb)))
+        else:
+            temps = np.ascontiguousarray(cia.temps, dtype=rdt)
+            kc = np.ascontiguousarray(cia.kcia, dtype=rdt)
+            self.lib.hsetcia_(_p(dnu), _p(temps), _p(kc))
         e3 = getattr(cia, "e3", None)
         if e3 is not None:
             self._e3 = [np.ascontiguousarray(e3.trans, dtype=np.float64),

@@ -122,6 +122,22 @@ C     as data: GRID = VB1, VT1, DV1; CON = TS, PS, C2, PI
       RETURN
       END
 
+C     /CIATABLE/ to be read from the raddata table file NAME by ciaread.f
+C     itself on first use (ciaread.f:63-103), as a run's .cia file sets it
+      SUBROUTINE HSETCIAFILE(NAME, DNU)
+      IMPLICIT NONE
+      INCLUDE 'ciacom.f'
+      CHARACTER*(*) NAME
+      REAL DNU
+      IREAD1 = 1
+      ANAME1 = NAME
+      DNU1 = DNU
+      IT1C = 0
+      IT2C = 0
+      IVC = 0
+      RETURN
+      END
+
 C     /DUST/ (cirsradg_wave.f:225,312): NSEC wavenumbers, NCONT particle
 C     types, extinction cross-sections XEXT1(L,K); the second row of XSEC
 C     (albedo times extinction) is not read by the emission branches.

@@ -34,7 +34,7 @@ EXPORTS = [
     "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch", "radg_ktable_bf16",
     "radg_batch_set_statemaps", "radg_batch_read_mapped", "radg_batch_device_outputs",
     "radg_oe_create", "radg_oe_cost", "radg_oe_propose", "radg_oe_destroy",
-    "radg_spectrum_noalloc", "radg_cirsconv",
+    "radg_spectrum_noalloc", "radg_cirsconv", "radg_read_cia",
 ]
 
 
@@ -127,6 +127,8 @@ def lib():
     L.radg_oe_destroy.argtypes = [vp]
     L.radg_spectrum_noalloc.restype = ip
     L.radg_spectrum_noalloc.argtypes = [vp, ip, vp, vp, vp, vp]
+    L.radg_read_cia.restype = ip
+    L.radg_read_cia.argtypes = [C.c_char_p, C.c_double]
     L.radg_cirsconv.restype = ip
     L.radg_cirsconv.argtypes = [ip, vp, ip, vp, vp, ip, vp, vp, vp]
     _lib = L

@@ -46,7 +46,9 @@ def get_mode() -> int:
 
 
 def register_ktables(tables) -> None:
-    """Register ``synth.KTable`` objects sharing one P/T/g grid."""
+    """Register ``synth.KTable`` objects sharing one P/T/g grid (with
+    ``temp2`` set, one temperature grid per pressure; with ``delv <= 0``, the
+    irregular wavenumber grid ``vcen``)."""
     t0 = tables[0]
     n = len(tables)
     ids = np.array([t.idgas for t in tables], dtype=np.int32)
@@ -59,14 +61,18 @@ def register_ktables(tables) -> None:
         if (t.np_, t.nt, t.ng) != (t0.np_, t0.nt, t0.ng):
             raise ValueError("all tables must share the P/T/g grid")
     ptrs = (C.c_void_p * n)(*[k.ctypes.data for k in ks])
+    vcs = [None if getattr(t, "vcen", None) is None else
+           np.ascontiguousarray(t.vcen, dtype=np.float64) for t in tables]
+    vptrs = (C.c_void_p * n)(*[None if v is None else v.ctypes.data for v in vcs])
     press = np.ascontiguousarray(t0.press, dtype=np.float64)
-    temp = np.ascontiguousarray(t0.temp, dtype=np.float64)
+    t2 = getattr(t0, "temp2", None)
+    temp = np.ascontiguousarray(t0.temp if t2 is None else t2, dtype=np.float64)
     gord = np.ascontiguousarray(t0.g_ord, dtype=np.float64)
     delg = np.ascontiguousarray(t0.delg, dtype=np.float64)
     check(lib().radg_register_ktables(
         C.c_int(n), _vp(ids), _vp(isos), _vp(npts), _vp(vmin), _vp(delv),
-        C.c_int(t0.np_), C.c_int(t0.nt), C.c_int(t0.ng), _vp(press), _vp(temp),
-        _vp(gord), _vp(delg), ptrs))
+        C.c_int(t0.np_), C.c_int(t0.nt_signed), C.c_int(t0.ng), _vp(press), _vp(temp),
+        _vp(gord), _vp(delg), ptrs, vptrs))
 
 
 def ktable_bf16(itab: int, shape):
@@ -190,6 +196,13 @@ def cirsconv(vwave, y, vconv, fwhm, vfwhm=None, xfwhm=None, filters=None, stream
                               int(ncol), C.c_void_p(yd.data_ptr()), C.c_void_p(out.data_ptr()),
                               C.c_void_p(st)))
     return out if on_dev else out.cpu().numpy()
+
+
+def read_cia(path: str, dnu: float) -> None:
+    """``/CIATABLE/`` from a CIA table file (ciaread.f's layout), as the
+    run's ``.cia`` file names it with its ``dnu``; leaves the other tables
+    registered with it as they are."""
+    check(lib().radg_read_cia(path.encode(), C.c_double(dnu)))
 
 
 def register_solar(s) -> None:

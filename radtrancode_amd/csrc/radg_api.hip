@@ -73,14 +73,16 @@ constexpr double KBOLTZMANN = 1.37947e-23;
 struct Table {
   int idgas, isogas, npoint;
   double vmin, delv;
+  std::vector<double> vcen;   // irregular grid (delv <= 0, read_khead.f:93-100)
   long long off;              // element offset in the pool
   std::vector<uint8_t> zero;  // per slice: every value <= 0 (get_kg.f:224-250)
   double bf_off = 0, bf_sc = 1;  // BF16 form: log10(k) = code * bf_sc + bf_off
 };
 
 struct TableSet {
-  int np = 0, nt = 0, ng = 0;
-  std::vector<double> lnp, tk, g_ord, delg, G;
+  int np = 0, nt = 0, ng = 0;  // nt: temperatures per pressure
+  bool t2 = false;             // a temperature grid per pressure (NT < 0)
+  std::vector<double> lnp, tk, g_ord, delg, G;  // tk: [nt], or [np][nt] with t2
   std::vector<Table> tabs;
   long long total = 0;
   double *d_k64 = nullptr;
@@ -600,14 +602,29 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
     cp = std::min(std::max(cp, 1), np - 1);
     c.cp = cp;
     c.v = (p1 - ts.lnp[cp - 1]) / (ts.lnp[cp] - ts.lnp[cp - 1]);
-    double t1 = std::min(std::max(q.temp[j], ts.tk[0]), ts.tk[nt - 1]);
-    int ct = locate(ts.tk.data(), nt, t1);
-    ct = std::min(std::max(ct, 1), nt - 1);
-    c.ct = ct;
-    c.u = (t1 - ts.tk[ct - 1]) / (ts.tk[ct] - ts.tk[ct - 1]);
-    c.dudt = 1.0 / (ts.tk[ct] - ts.tk[ct - 1]);
-    for (int a = 0; a < 2; a++)
-      for (int b = 0; b < 2; b++) nodes.insert((long long)(cp - 1 + a) * nt + (ct - 1 + b));
+    // the T bracket on a temperature grid (get_kg.f:126-139); with one grid
+    // per pressure (NT < 0) on the grids of both bracketing pressures
+    // (:145-173)
+    auto tbracket = [&](const double *tn, int &ct, double &u, double &dudt) {
+      const double t1 = std::min(std::max(q.temp[j], tn[0]), tn[nt - 1]);
+      ct = std::min(std::max(locate(tn, nt, t1), 1), nt - 1);
+      u = (t1 - tn[ct - 1]) / (tn[ct] - tn[ct - 1]);
+      dudt = 1.0 / (tn[ct] - tn[ct - 1]);
+    };
+    if (!ts.t2) {
+      tbracket(ts.tk.data(), c.ct, c.u, c.dudt);
+      c.ct2 = c.ct;
+      c.u2 = c.u;
+      c.dudt2 = c.dudt;
+    } else {
+      tbracket(ts.tk.data() + (size_t)(cp - 1) * nt, c.ct, c.u, c.dudt);
+      tbracket(ts.tk.data() + (size_t)cp * nt, c.ct2, c.u2, c.dudt2);
+    }
+    c.pad2 = 0;
+    for (int b = 0; b < 2; b++) {
+      nodes.insert((long long)(cp - 1) * nt + (c.ct - 1 + b));
+      nodes.insert((long long)cp * nt + (c.ct2 - 1 + b));
+    }
     double ut = 0;
     for (int k = 0; k < ngas; k++) {
       ut += q.amount[j * ngas + k];
@@ -779,16 +796,54 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
       w.pad = 0;
       const Table *tb = nullptr;
       for (const Table &t : ts.tabs) {  // read_klist.f:260-290: first covering match
+        if (t.idgas != q.idgas[k] || t.isogas != q.isogas[k]) continue;
+        if (t.delv <= 0) {  // an irregular grid covers every wavenumber (:292-330)
+          tb = &t;
+          continue;
+        }
+        if (tb) continue;
         // the grid ends are widened by get_kg's coincidence tolerance so that
         // a REAL(4) wavenumber rounded past the last point still matches
         const double xmax = t.vmin + (t.npoint - 1) * t.delv;
         const double tol = (t.delv >= 0.0001) ? 0.02 * t.delv : 0.25 * t.delv;
-        if (t.idgas == q.idgas[k] && t.isogas == q.isogas[k] && v >= t.vmin - tol && v <= xmax + tol) {
-          tb = &t;
-          break;
-        }
+        if (v >= t.vmin - tol && v <= xmax + tol) tb = &t;
       }
       if (!tb) continue;
+      if (tb->delv <= 0) {
+        // read_klist.f:292-330: DELV < 0 takes the point nearest in per cent
+        // (get_kg.f:267-273: coincident); DELV = 0 the nearest at or below,
+        // merged with the next one (:274-279, :386-389)
+        long long ipo = -1;
+        double best = 1e10, frack = 1.0;
+        for (int i = 0; i < tb->npoint; i++) {
+          const double dx = 100 * std::fabs((v - tb->vcen[i]) / tb->vcen[i]);
+          if (tb->delv == 0) {
+            if (dx < best && tb->vcen[i] <= v) {
+              ipo = i;
+              frack = i < tb->npoint - 1 ? 1.0 - (v - tb->vcen[i]) / (tb->vcen[i + 1] - tb->vcen[i]) : 1.;
+              best = dx;
+            }
+          } else if (dx < best) {
+            ipo = i;
+            best = dx;
+          }
+        }
+        if (ipo < 0)
+          return fail(RADG_EINPUT, "wavenumber %g lies below table %d's irregular grid (DELV=0)", v,
+                      (int)(tb - ts.tabs.data()) + 1);
+        if (tb->zero[ipo]) continue;
+        w.base = tb->off + ipo * ntab;
+        w.sc = (float)tb->bf_sc;
+        w.off = (float)tb->bf_off;
+        if (tb->delv == 0) {
+          w.coinc = 0;
+          w.base2 = (ipo + 1 < tb->npoint) ? w.base + ntab : w.base;
+          w.w2 = 1.0 - frack;
+          tbytes += (double)nodes.size() * NG * kbytes;
+        }
+        tbytes += (double)nodes.size() * NG * kbytes;
+        continue;
+      }
       const double eps = (tb->delv >= 0.0001) ? 0.02 * tb->delv : 0.25 * tb->delv;
       long long n1 = (long long)((v + eps - tb->vmin) / tb->delv);
       n1 = std::min<long long>(std::max<long long>(n1, 0), tb->npoint - 1);
@@ -1203,6 +1258,7 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
                                       : (const void *)ts->d_k32;
   a.np = ts->np;
   a.nt = ts->nt;
+  a.t2 = ts->t2 ? 1 : 0;
   a.G = ts->d_G;
   a.delg64 = ts->d_delg64;
   a.delg32 = ts->d_delg32;
@@ -1308,24 +1364,32 @@ int radg_get_mode(void) { return g_mode; }
 const char *radg_last_error(void) { return g_err.c_str(); }
 
 int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const int *npoint,
-                          const double *vmin, const double *delv, int np, int nt, int ng,
+                          const double *vmin, const double *delv, int np, int nt_signed, int ng,
                           const double *press_atm, const double *temp, const double *g_ord,
-                          const double *delg, const double *const *k) {
+                          const double *delg, const double *const *k,
+                          const double *const *vcen) {
   if (ntab < 1) return fail(RADG_EINPUT, "ntab=%d", ntab);
+  const bool t2 = nt_signed < 0;
+  const int nt = t2 ? -nt_signed : nt_signed;
   if (np < 2 || np > RADG_MAXK || nt < 2 || nt > RADG_MAXK)
-    return fail(RADG_ELIMITS, "np=%d nt=%d outside 2..%d", np, nt, RADG_MAXK);
+    return fail(RADG_ELIMITS, "np=%d nt=%d outside 2..%d", np, nt_signed, RADG_MAXK);
   if (ng < 1 || ng > RADG_GPU_MAXG)
     return fail(RADG_ELIMITS, "ng=%d outside 1..%d (the overlap sort keeps NG*NG keys in one wavefront)",
                 ng, RADG_GPU_MAXG);
   auto ts = std::make_shared<TableSet>();
   ts->np = np;
   ts->nt = nt;
+  ts->t2 = t2;
   ts->ng = ng;
   for (int i = 0; i < np; i++) {
     if (!finite_pos(press_atm[i])) return fail(RADG_EINPUT, "table pressure %d = %g", i + 1, press_atm[i]);
     ts->lnp.push_back(std::log(press_atm[i]));  // read_khead.f:80-85
   }
-  ts->tk.assign(temp, temp + nt);
+  ts->tk.assign(temp, temp + (t2 ? (size_t)np * nt : (size_t)nt));
+  for (int r = 0; r < (t2 ? np : 1); r++)
+    for (int j = 1; j < nt; j++)
+      if (!(ts->tk[(size_t)r * nt + j] > ts->tk[(size_t)r * nt + j - 1]))
+        return fail(RADG_EINPUT, "table temperatures must increase (row %d)", r + 1);
   ts->g_ord.assign(g_ord, g_ord + ng);
   ts->delg.assign(delg, delg + ng);
   double wmin = 1e300, wmax = 0;
@@ -1344,14 +1408,26 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
   if (ts->G[ng] < 1.0) ts->G[ng] = 1.0;  // rankg.f:65
   const long long per = (long long)np * nt * ng;
   long long total = 0;
+  // read_klist.f:117-160 stops when the tables differ in DELV, so regular and
+  // irregular grids never share a list
+  for (int t = 1; t < ntab; t++)
+    if ((delv[t] > 0) != (delv[0] > 0))
+      return fail(RADG_EINPUT, "tables 1 and %d mix regular (DELV > 0) and irregular grids", t + 1);
   for (int t = 0; t < ntab; t++) {
-    if (npoint[t] < 1 || !(delv[t] > 0)) return fail(RADG_EUNSUPPORTED, "table %d: npoint=%d delv=%g (regular grids with DELV>0 only)", t + 1, npoint[t], delv[t]);
+    if (npoint[t] < 1 || !std::isfinite(delv[t])) return fail(RADG_EINPUT, "table %d: npoint=%d delv=%g", t + 1, npoint[t], delv[t]);
     Table tb;
     tb.idgas = idgas[t];
     tb.isogas = isogas[t];
     tb.npoint = npoint[t];
     tb.vmin = vmin[t];
     tb.delv = delv[t];
+    if (!(delv[t] > 0)) {  // read_khead.f:93-100: the wavenumbers follow the header
+      if (!vcen || !vcen[t]) return fail(RADG_EINPUT, "table %d: DELV=%g needs its wavenumbers", t + 1, delv[t]);
+      tb.vcen.assign(vcen[t], vcen[t] + npoint[t]);
+      for (int j = 0; j < npoint[t]; j++)
+        if (!finite_pos(tb.vcen[j]) || (j > 0 && !(tb.vcen[j] > tb.vcen[j - 1])))
+          return fail(RADG_EINPUT, "table %d: wavenumbers must be positive and increasing", t + 1);
+    }
     tb.off = total;
     tb.zero.assign(npoint[t], 1);
     double lmin = 1e300, lmax = 0;
@@ -1428,6 +1504,43 @@ int radg_register_cia(double dnu, const double *temps, const double *kcia) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_cia = c;
   return RADG_OK;
+}
+
+int radg_read_cia(const char *path, double dnu) {
+  // ciaread.f:77-89: two unformatted sequential records, the 25 REAL*8
+  // temperatures, then kcia(9,25,1501) in REAL (4 bytes) or REAL*8
+  if (!path) return fail(RADG_EINPUT, "null path");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(RADG_EIO, "cannot open %s", path);
+  std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  std::vector<std::pair<size_t, size_t>> rec;  // (offset, length)
+  for (size_t o = 0; o < raw.size();) {
+    int32_t n = 0, m = -1;
+    if (o + 4 > raw.size()) return fail(RADG_EIO, "%s: truncated record", path);
+    std::memcpy(&n, raw.data() + o, 4);
+    if (n < 0 || o + 8 + (size_t)n > raw.size()) return fail(RADG_EIO, "%s: bad record length", path);
+    std::memcpy(&m, raw.data() + o + 4 + n, 4);
+    if (m != n) return fail(RADG_EIO, "%s: record markers differ", path);
+    rec.emplace_back(o + 4, (size_t)n);
+    o += 8 + (size_t)n;
+  }
+  const size_t nk = (size_t)1501 * 25 * 9;
+  if (rec.size() < 2 || rec[0].second != 25 * 8 || (rec[1].second != 4 * nk && rec[1].second != 8 * nk))
+    return fail(RADG_EIO, "%s: not a CIA table (25 REAL*8 temperatures, then kcia(9,25,1501))", path);
+  double temps[25];
+  std::memcpy(temps, raw.data() + rec[0].first, sizeof temps);
+  std::vector<double> k(nk);
+  const char *src = raw.data() + rec[1].first;
+  for (size_t i = 0; i < nk; i++) {
+    if (rec[1].second == 4 * nk) {
+      float v;
+      std::memcpy(&v, src + 4 * i, 4);
+      k[i] = v;
+    } else {
+      std::memcpy(&k[i], src + 8 * i, 8);
+    }
+  }
+  return radg_register_cia(dnu, temps, k.data());
 }
 
 int radg_register_parah2(double dnu, const double *temps, int npara, const double *fp,
@@ -2535,7 +2648,7 @@ int radg_read_klist(const char *klist_path, int recl) {
   if (names.empty()) return fail(RADG_EIO, "%s lists no tables", klist_path);
   std::vector<int> ids, isos, npts;
   std::vector<double> vmins, delvs, press, temps, gord, delg;
-  std::vector<std::vector<double>> data;
+  std::vector<std::vector<double>> data, vcens;
   int np0 = 0, nt0 = 0, ng0 = 0;
   for (const auto &nm : names) {
     std::ifstream f(nm, std::ios::binary);
@@ -2558,23 +2671,30 @@ int radg_read_klist(const char *klist_path, int recl) {
       return v;
     };
     if (nrec < 10) return fail(RADG_EIO, "%s: truncated header", nm.c_str());
-    const int irec0 = gi(1), npoint = gi(2), np = gi(6), nt = gi(7), ng = gi(8);
+    const int irec0 = gi(1), npoint = gi(2), np = gi(6), nts = gi(7), ng = gi(8);
+    const int nt = std::abs(nts);
     const double vmin = gr(3), delv = gr(4);
-    if (nt <= 0 || delv <= 0) return fail(RADG_EUNSUPPORTED, "%s: NT<=0 or DELV<=0 tables are not supported", nm.c_str());
-    if (np < 2 || np > RADG_MAXK || nt < 2 || nt > RADG_MAXK || ng < 1 || ng > RADG_MAXG - 1)
-      return fail(RADG_ELIMITS, "%s: np=%d nt=%d ng=%d", nm.c_str(), np, nt, ng);
+    if (np < 2 || np > RADG_MAXK || nt < 2 || nt > RADG_MAXK || ng < 1 || ng > RADG_MAXG - 1 || npoint < 1)
+      return fail(RADG_ELIMITS, "%s: np=%d nt=%d ng=%d npoint=%d", nm.c_str(), np, nts, ng, npoint);
     const size_t per = (size_t)np * nt * ng;
-    if ((size_t)irec0 - 1 + (size_t)npoint * per > nrec) return fail(RADG_EIO, "%s: truncated data", nm.c_str());
+    const size_t ntemp = nts < 0 ? (size_t)np * nt : (size_t)nt;
+    const size_t r0 = 11 + 2 * ng + 2;  // read_khead.f:71-100
+    if (r0 + np + ntemp + (delv <= 0 ? npoint : 0) > nrec ||
+        (size_t)irec0 - 1 + (size_t)npoint * per > nrec)
+      return fail(RADG_EIO, "%s: truncated", nm.c_str());
     if (ids.empty()) {
-      np0 = np; nt0 = nt; ng0 = ng;
+      np0 = np; nt0 = nts; ng0 = ng;
       for (int j = 0; j < ng; j++) gord.push_back(gr(11 + j));
       for (int j = 0; j < ng; j++) delg.push_back(gr(11 + ng + j));
-      const size_t r0 = 11 + 2 * ng + 2;
       for (int j = 0; j < np; j++) press.push_back(gr(r0 + j));
-      for (int j = 0; j < nt; j++) temps.push_back(gr(r0 + np + j));
-    } else if (np != np0 || nt != nt0 || ng != ng0) {
+      for (size_t j = 0; j < ntemp; j++) temps.push_back(gr(r0 + np + j));
+    } else if (np != np0 || nts != nt0 || ng != ng0) {
       return fail(RADG_ELIMITS, "%s: grid differs from the first table (read_klist.f:117-130)", nm.c_str());
     }
+    std::vector<double> vc;
+    if (delv <= 0)
+      for (int j = 0; j < npoint; j++) vc.push_back(gr(r0 + np + ntemp + j));
+    vcens.push_back(std::move(vc));
     ids.push_back(gi(9));
     isos.push_back(gi(10));
     npts.push_back(npoint);
@@ -2584,11 +2704,12 @@ int radg_read_klist(const char *klist_path, int recl) {
     for (size_t i = 0; i < d.size(); i++) d[i] = gr(irec0 + i);
     data.push_back(std::move(d));
   }
-  std::vector<const double *> ptrs;
+  std::vector<const double *> ptrs, vptrs;
   for (auto &d : data) ptrs.push_back(d.data());
+  for (auto &v : vcens) vptrs.push_back(v.empty() ? nullptr : v.data());
   return radg_register_ktables((int)ids.size(), ids.data(), isos.data(), npts.data(), vmins.data(),
                                delvs.data(), np0, nt0, ng0, press.data(), temps.data(), gord.data(),
-                               delg.data(), ptrs.data());
+                               delg.data(), ptrs.data(), vptrs.data());
 }
 
 }  // extern "C"

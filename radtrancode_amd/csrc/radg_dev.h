@@ -67,6 +67,11 @@ struct DevReq {
 struct LayerConst {
   int cp, ct;            // 1-based lower bracket indices in P and T
   double v, u, dudt;     // P weight, T weight, dU/dT
+  // the T bracket on the upper pressure's grid: the same as ct, u, dudt for
+  // one temperature grid, its own for per-pressure grids (NT < 0,
+  // get_kg.f:145-173)
+  int ct2, pad2;
+  double u2, dudt2;
   double utotl;          // sum of amounts * 1e-20
   double temp;           // K
   double qh2, qhe, qn2, qch4, qco2, tau;
@@ -119,7 +124,8 @@ struct KernelArgs {
   int ng;
   // tables
   const void *ktab;            // double* (FP64 form) or float* (mixed form)
-  int np, nt;
+  int np, nt;                  // nt: temperatures per pressure (|NT|)
+  int t2;                      // 1: a temperature grid per pressure (NT < 0)
   const double *G;             // g boundaries [ng+1], G[ng] forced >= 1
   const double *delg64;
   const float *delg32;

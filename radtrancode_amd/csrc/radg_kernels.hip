@@ -829,10 +829,22 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
     const T *tab = reinterpret_cast<const T *>(a.ktab);
     const int nt = a.nt;
     const int i1 = (nt * NG * (lc.cp - 1)) + (NG * (lc.ct - 1));
-    const int i2 = (nt * NG * lc.cp) + (NG * (lc.ct - 1));
-    const int i3 = (nt * NG * lc.cp) + (NG * lc.ct);
+    const int i2 = (nt * NG * lc.cp) + (NG * (lc.ct2 - 1));
+    const int i3 = (nt * NG * lc.cp) + (NG * lc.ct2);
     const int i4 = (nt * NG * (lc.cp - 1)) + (NG * lc.ct);
     const T v = (T)lc.v, u = (T)lc.u, dudt = (T)lc.dudt;
+    const T u2 = (T)lc.u2, dudt2 = (T)lc.dudt2;
+    // bilinear in (ln P, T) on one temperature grid (get_kg.f:349-350), or
+    // per pressure row on each row's own grid (NT < 0, :352-357)
+    auto bilin = [&](T y1, T y2, T y3, T y4, T &x, T &dxdt) {
+      if (!a.t2) {
+        x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 + ((T)1 - v) * u * y4;
+        dxdt = (-((T)1 - v) * y1 - v * y2 + v * y3 + ((T)1 - v) * y4) * dudt;
+      } else {
+        x = ((T)1 - v) * (((T)1 - u) * y1 + u * y4) + v * (((T)1 - u2) * y2 + u2 * y3);
+        dxdt = ((T)1 - v) * (-y1 + y4) * dudt + v * (-y2 + y3) * dudt2;
+      }
+    };
     const float ln10 = 2.302585092994046f;
 
     // Wavenumber between two table points (get_kg.f:379-440): interpolate
@@ -867,9 +879,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
           y3 = sl[i3 + gi];
           y4 = sl[i4 + gi];
         }
-        x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 +
-            ((T)1 - v) * u * y4;
-        dxdt = (-((T)1 - v) * y1 - v * y2 + v * y3 + ((T)1 - v) * y4) * dudt;
+        bilin(y1, y2, y3, y4, x, dxdt);
         const T wl = (lane < NG) ? (T)(1.0 - wgk.w2) : (T)wgk.w2;
         fw = dlg[gi] * wl;
       }
@@ -982,9 +992,8 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
               y4 = dlog(y4);
             }
           }
-          const T x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 +
-                      ((T)1 - v) * u * y4;
-          const T dxdt = (-((T)1 - v) * y1 - v * y2 + v * y3 + ((T)1 - v) * y4) * dudt;
+          T x, dxdt;
+          bilin(y1, y2, y3, y4, x, dxdt);
           if (klog) {
             const T ex = dexp(x);
             kk = ex;

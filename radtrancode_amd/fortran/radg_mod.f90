@@ -23,7 +23,7 @@ module radg
   public :: radg_batch_destroy, radg_batch_set_statemaps, radg_batch_read_mapped
   public :: radg_batch_device_outputs
   public :: radg_oe_create, radg_oe_cost, radg_oe_propose, radg_oe_destroy
-  public :: radg_cirsconv
+  public :: radg_cirsconv, radg_read_cia, radg_read_cia_f
 
   ! radg_request of include/radg.h: sizes, then C addresses (c_loc) of the
   ! caller's arrays in the compact 0-based layout the header documents.
@@ -90,6 +90,12 @@ module radg
       import :: c_int, c_char
       character(kind=c_char), dimension(*) :: path
       integer(c_int), value :: recl
+    end function
+
+    integer(c_int) function radg_read_cia(path, dnu) bind(C, name="radg_read_cia")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), dimension(*) :: path
+      real(c_double), value :: dnu
     end function
 
     integer(c_int) function radg_register_cia(dnu, temps, kcia) bind(C, name="radg_register_cia")
@@ -289,6 +295,12 @@ contains
     character(len=*), intent(in) :: path
     integer, intent(in) :: recl
     radg_read_klist_f = radg_read_klist(trim(path) // c_null_char, int(recl, c_int))
+  end function
+
+  integer function radg_read_cia_f(path, dnu)
+    character(len=*), intent(in) :: path
+    real, intent(in) :: dnu
+    radg_read_cia_f = radg_read_cia(trim(path) // c_null_char, real(dnu, c_double))
   end function
 
   function radg_error_message() result(msg)

@@ -17,7 +17,7 @@ record                content
 11+NG .. 10+2NG       DEL_G
 11+2NG, 12+2NG        unused
 13+2NG ..             PRESS(NP) in atm, then TEMP(NT) (or TEMP2(NP,|NT|))
-then (DELV <= 0)      VCEN(NPOINT)
+then (DELV <= 0)      VCEN(NPOINT) (read_khead.f:93-100)
 IREC0 ..              k * 1e20 for each wavenumber: NP*|NT|*NG values,
                       g fastest, then T, then P
 ====================  =====================================================
@@ -34,9 +34,12 @@ from .synth import KTable
 
 def _header_records(kt: KTable, vcen=None):
     ng, np_, nt = kt.ng, kt.np_, kt.nt
-    nrec_head = 10 + 2 * ng + 2 + np_ + nt + (0 if kt.delv > 0 else kt.npoint)
+    ntemp = np_ * nt if kt.temp2 is not None else nt
+    if vcen is None:
+        vcen = kt.vcen
+    nrec_head = 10 + 2 * ng + 2 + np_ + ntemp + (0 if kt.delv > 0 else kt.npoint)
     irec0 = nrec_head + 1
-    ints = {1: irec0, 2: kt.npoint, 6: np_, 7: nt, 8: ng, 9: kt.idgas, 10: kt.isogas}
+    ints = {1: irec0, 2: kt.npoint, 6: np_, 7: kt.nt_signed, 8: ng, 9: kt.idgas, 10: kt.isogas}
     reals = {3: kt.vmin, 4: kt.delv, 5: kt.fwhm}
     for j in range(ng):
         reals[11 + j] = kt.g_ord[j]
@@ -45,9 +48,10 @@ def _header_records(kt: KTable, vcen=None):
     for j in range(np_):
         reals[r + j] = kt.press[j]
     r += np_
-    for j in range(nt):
-        reals[r + j] = kt.temp[j]
-    r += nt
+    temps = np.asarray(kt.temp2).reshape(-1) if kt.temp2 is not None else kt.temp
+    for j in range(ntemp):  # TEMP2(I,J) with J fastest (read_khead.f:86-92)
+        reals[r + j] = temps[j]
+    r += ntemp
     if kt.delv <= 0:
         for j in range(kt.npoint):
             reals[r + j] = vcen[j]
@@ -71,7 +75,8 @@ def write_kta(path: str, kt: KTable, recl: int = 4, vcen=None) -> None:
 
 
 def read_kta(path: str, recl: int = 4) -> KTable:
-    """Read a ``.kta`` file (regular grid, ``DELV > 0``) into a :class:`KTable`."""
+    """Read a ``.kta`` file into a :class:`KTable` (``temp2`` set for NT < 0,
+    ``vcen`` for DELV <= 0)."""
     raw = np.fromfile(path, dtype=np.uint8)
     if raw.size % recl:
         raise ValueError(f"{path}: size is not a multiple of {recl}")
@@ -84,19 +89,26 @@ def read_kta(path: str, recl: int = 4) -> KTable:
     def gr(r):
         return float(rec[r - 1].view(rdt)[0])
 
-    irec0, npoint, np_, nt, ng = gi(1), gi(2), gi(6), gi(7), gi(8)
-    if nt < 0:
-        raise ValueError("per-pressure temperature grids (NT < 0) are not supported")
+    irec0, npoint, np_, nts, ng = gi(1), gi(2), gi(6), gi(7), gi(8)
+    nt = abs(nts)
     vmin, delv, fwhm = gr(3), gr(4), gr(5)
-    if delv <= 0:
-        raise ValueError("irregular wavenumber grids (DELV <= 0) are not supported")
     g_ord = np.array([gr(11 + j) for j in range(ng)])
     delg = np.array([gr(11 + ng + j) for j in range(ng)])
     r = 11 + 2 * ng + 2
     press = np.array([gr(r + j) for j in range(np_)])
-    temp = np.array([gr(r + np_ + j) for j in range(nt)])
+    r += np_
+    temp2 = None
+    if nts < 0:
+        temp2 = np.array([gr(r + j) for j in range(np_ * nt)]).reshape(np_, nt)
+        temp = temp2[0].copy()
+        r += np_ * nt
+    else:
+        temp = np.array([gr(r + j) for j in range(nt)])
+        r += nt
+    vcen = np.array([gr(r + j) for j in range(npoint)]) if delv <= 0 else None
     ntab = np_ * nt * ng
     data = rec[irec0 - 1: irec0 - 1 + npoint * ntab].copy().view(rdt).reshape(
         npoint, np_, nt, ng).astype(np.float64)
     return KTable(idgas=gi(9), isogas=gi(10), vmin=vmin, delv=delv, fwhm=fwhm,
-                  press=press, temp=temp, g_ord=g_ord, delg=delg, k=data)
+                  press=press, temp=temp, g_ord=g_ord, delg=delg, k=data, temp2=temp2,
+                  vcen=vcen)

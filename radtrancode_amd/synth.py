@@ -61,6 +61,15 @@ class KTable:
     g_ord: np.ndarray      # [ng]
     delg: np.ndarray       # [ng]
     k: np.ndarray          # [npoint, np, nt, ng] float64, k_phys * 1e20
+    # per-pressure temperature grids [np, nt] (the file's NT < 0; temp then
+    # unused) and the wavenumbers of an irregular grid [npoint] (DELV <= 0)
+    temp2: Optional[np.ndarray] = None
+    vcen: Optional[np.ndarray] = None
+
+    @property
+    def nt_signed(self) -> int:
+        """NT as the .kta header holds it: negative with per-pressure grids."""
+        return -self.nt if self.temp2 is not None else self.nt
 
     @property
     def npoint(self) -> int:
@@ -91,6 +100,7 @@ class CIATable:
     n2nir: Optional["N2NirTable"] = None
     solar: Optional["SolarSpectrum"] = None
     gascon: Optional["GasCon"] = None
+    file: Optional[str] = None  # the raddata table file it came from, if any
 
 
 @dataclass

@@ -25,11 +25,12 @@ _REQ_OPTIONAL = ["cont", "vsec", "xsec"]  # dust, present only when ncont > 0
 _REQ_OPT_SCALARS = ["inlte", "nlte_k1", "nlte_a", "iform"]  # absent from older cases: 0
 _KT_ARRAYS = ["press", "temp", "g_ord", "delg", "k"]
 _KT_SCALARS = ["idgas", "isogas", "vmin", "delv", "fwhm"]
+_KT_OPTIONAL = ["temp2", "vcen"]  # NT < 0, DELV <= 0
 
 
 def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = None,
               e3: E3Table = None, n2nir: N2NirTable = None, solar: SolarSpectrum = None,
-              gascon_flags=None, hmin_static: bool = False):
+              gascon_flags=None, hmin_static: bool = False, cia_fixture: str = None):
     """``cia``: a CIA table of the case's own, stored with it (else the
     shared ``cia.npz`` applies); ``e3``: the E3 table an imod 18 case ran
     with; ``n2nir``: N2's near-infrared CIA a case needs; ``gascon_flags``:
@@ -37,6 +38,8 @@ def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = 
     the tables of ``gascon_tables.npz``; ``hmin_static``: its reference
     outputs carry HMIN_BF/HMIN_FF's static sum over the layers of a call."""
     d = {"note": np.array(note)}
+    if cia_fixture is not None:
+        d["cia_fixture"] = np.array(cia_fixture)
     if gascon_flags is not None:
         d["gascon_flags"] = np.asarray(gascon_flags, dtype=np.int32)
     if hmin_static:
@@ -67,6 +70,9 @@ def save_case(path, tables, req, outputs: dict, note: str = "", cia: CIATable = 
     for t, kt in enumerate(tables):
         for a in _KT_ARRAYS + _KT_SCALARS:
             d[f"kt{t}_{a}"] = np.asarray(getattr(kt, a))
+        for a in _KT_OPTIONAL:
+            if getattr(kt, a, None) is not None:
+                d[f"kt{t}_{a}"] = np.asarray(getattr(kt, a))
     for k, v in outputs.items():
         d[k] = np.asarray(v)
     np.savez_compressed(path, **d)
@@ -90,6 +96,9 @@ def load_case(path):
         tk = {a: z[f"kt{t}_{a}"] for a in _KT_ARRAYS}
         for s in _KT_SCALARS:
             tk[s] = z[f"kt{t}_{s}"].item()
+        for a in _KT_OPTIONAL:
+            if f"kt{t}_{a}" in z.files:
+                tk[a] = z[f"kt{t}_{a}"]
         tables.append(KTable(**tk))
     outs = {k: z[k] for k in z.files if k.startswith("ref")}
     return tables, req, outs, str(z["note"])
@@ -103,7 +112,9 @@ def case_cia(path) -> CIATable:
     """The CIA table a case runs with: its own if it holds one, else the
     shared one, carrying the case's E3 table if it has one."""
     z = np.load(path, allow_pickle=False)
-    if "cia_kcia" in z.files:
+    if "cia_fixture" in z.files:  # a reference table file kept as a fixture
+        cia = load_cia_fixture(str(z["cia_fixture"]))
+    elif "cia_kcia" in z.files:
         para = None
         if "para_kh2h2" in z.files:
             para = ParaCIATable(dnu=float(z["para_dnu"]), temps=z["para_temps"], fp=z["para_fp"],
@@ -157,6 +168,14 @@ def load_e3(path=None) -> E3Table:
     """The reference's E3 table (raddata/e3tab.dat) kept as ``e3tab.npz``."""
     z = np.load(path or os.path.join(HERE, "e3tab.npz"), allow_pickle=False)
     return E3Table(trans=z["trans"], e3=z["e3"])
+
+
+def load_cia_fixture(name: str) -> CIATable:
+    """A CIA table file of the reference's raddata kept as a fixture
+    (``cia_<name>.npz``: dnu, temps and kcia as the file holds them)."""
+    z = np.load(os.path.join(HERE, f"cia_{name}.npz"), allow_pickle=False)
+    return CIATable(dnu=float(z["dnu"]), temps=z["temps"], kcia=z["kcia"].astype(np.float64),
+                    file=str(z["table"]))
 
 
 def load_cia(path=None) -> CIATable:

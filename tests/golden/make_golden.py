@@ -339,6 +339,24 @@ def cases():
     r.iform, r.radius1 = 1, 6052.0
     yield "disc_nlte_dust", t, r, "imod 18 with non-LTE, one dust type, emissivity < 1, iform = 1"
 
+    # .kta forms: per-pressure temperature grids (NT < 0) and irregular
+    # wavenumber grids (DELV < 0: nearest point; DELV = 0: merge of the
+    # points either side)
+    t, r = kta_forms_inputs(35, t2=True)
+    yield "kta_t2", t, r, "k-tables with a temperature grid per pressure (NT < 0)"
+    t, r = kta_forms_inputs(36, delv=-1.0)
+    yield "kta_irregular_nearest", t, r, "irregular wavenumber grid, DELV < 0 (nearest point)"
+    t, r = kta_forms_inputs(37, delv=0.0)
+    yield "kta_irregular_between", t, r, "irregular wavenumber grid, DELV = 0 (two-point merge)"
+    t, r = kta_forms_inputs(38, t2=True, delv=0.0)
+    yield "kta_t2_irregular", t, r, "NT < 0 and DELV = 0 together"
+
+    # the reference's own CIA table file (raddata/isotest.tab, dnu = 1 as
+    # the Jupiter CIRS example's .cia has it), read by ciaread.f itself in
+    # the REAL(4) build
+    t, c, r = synth.config1(nwave=8, nlayer=60, nkgas=2, ng=10)
+    yield "cia_isotest", t, r, "CIA from raddata/isotest.tab (dnu 1), 60 layers, 600-1400 cm-1"
+
     # gas continua (ngascon.f): H- free-free and bound-free, the CKD H2O
     # continuum (IH2O = 1) and the CH4/NH3 tables (ICH4, INH3)
     t, r = hminus_inputs(31, 2000.0, 2.0)
@@ -404,6 +422,44 @@ def add_continuum_gas(r, gid, vmr, seed):
     r.pp = np.concatenate([r.pp, (x * r.press)[:, None]], axis=1)
     r.isw = np.arange(1, r.ngas + 2, dtype=np.int32)
     return r
+
+
+def kta_forms_inputs(seed, t2=False, delv=None, nwave=7, nlayer=30, ng=10, nkgas=3):
+    """A 30-layer request on k-tables with per-pressure temperature grids
+    (``t2``: the .kta's NT < 0, each pressure's grid shifted and stretched)
+    and/or an irregular wavenumber grid (``delv`` 0 or -1: DELV <= 0, the
+    calculation wavenumbers between the grid points, or near them)."""
+    rng = np.random.default_rng(seed)
+    npt = 12
+    ids = list(synth.KGAS_IDS[:nkgas])
+    t = synth.make_ktables(ids, 800.0, 3.0, npt, ng=ng, np_=10, nt=6, seed=seed)
+    vc = 800.0 + np.cumsum(rng.uniform(1.0, 5.0, npt))
+    if t2:
+        base = np.linspace(40.0, 220.0, 6)
+        grid = np.array([base * (0.9 + 0.02 * i) + 4.0 * i for i in range(10)])
+    for kt in t:
+        if t2:
+            kt.temp2 = grid.copy()
+            kt.temp = grid[0].copy()
+        if delv is not None:
+            kt.delv, kt.vmin, kt.vcen = float(delv), float(vc[0]), vc.copy()
+    r = synth.make_request(nlayer=nlayer, nkgas=nkgas, nwave=nwave, seed=seed, table_vmin=800.0,
+                           table_delv=3.0)
+    if delv == 0:  # between grid points (read_klist.f:299-312 takes the one below)
+        r.vwave = vc[2:2 + nwave] + rng.uniform(0.1, 0.9, nwave) * np.diff(vc)[2:2 + nwave]
+    elif delv is not None:  # near grid points: the nearest is taken (:313-323)
+        r.vwave = vc[1:1 + nwave] + rng.uniform(-0.3, 0.3, nwave)
+    return t, r
+
+
+def cia_fixture(name="isotest", dnu=1.0):
+    """raddata/<name>.tab as a fixture: its temperatures and kcia as the
+    file holds them (REAL*4), read as ciaread.f:77-89 reads it."""
+    from radtrancode_amd.cia import read_cia_table
+    fn = f"{name}.tab"
+    temps, k = read_cia_table(os.path.join(ref.REF_TREE, "raddata", fn))
+    np.savez_compressed(os.path.join(HERE, f"cia_{name}.npz"), dnu=np.array(dnu), temps=temps,
+                        kcia=k.astype(np.float32), table=np.array(fn))
 
 
 def gascon_tables(path=os.path.join(HERE, "gascon_tables.npz")):
@@ -587,6 +643,8 @@ CASE_E3 = ("disc", "disc_iform3", "disc_nlte_dust")
 CASE_N2NIR = ("titan_nir", "titan_nir_um")
 # cases run with a stellar spectrum (/SOLARDAT/, iread = 999)
 CASE_SOLAR = ("solar_ratio", "solar_trans")
+# cases run with a reference CIA table file (its fixture name)
+CASE_CIA_FILE = {"cia_isotest": "isotest"}
 # cases run with /GASCONDEF/ flags (ih2o, ich4, io3, inh3, iuvscat)
 CASE_GASCON = {"hminus_ir": (0, 0, 0, 0, 0), "hminus_nir": (0, 0, 0, 0, 0),
                "h2o_ckd": (1, 0, 0, 0, 0), "ch4_nh3_vis": (0, 1, 0, 1, 0)}
@@ -604,6 +662,8 @@ def main(names=()):
         save_n2nir()
     if not names or "rayleigh" in names:
         np.savez_compressed(os.path.join(HERE, "rayleigh.npz"), **rayleigh_vectors())
+    if not names or "cia_fixture" in names:
+        cia_fixture()
     if not names or "conv" in names:
         np.savez_compressed(os.path.join(HERE, "conv.npz"), **conv_vectors())
     if not names or "gascon" in names:
@@ -631,6 +691,10 @@ def main(names=()):
             sol = synth.make_solar(500.0, 1500.0)
             use = copy.copy(use)
             use.solar = sol
+        cfx = CASE_CIA_FILE.get(name)
+        if cfx is not None:
+            from tests.golden.golden_io import load_cia_fixture
+            use = load_cia_fixture(cfx)
         gflags = CASE_GASCON.get(name)
         if gflags is not None:
             use = copy.copy(use)
@@ -638,7 +702,8 @@ def main(names=()):
         outs = run_ref(t, use, r)
         path = os.path.join(HERE, f"case_{name}.npz")
         save_case(path, t, r, outs, note, cia=own, e3=e3, n2nir=nir, solar=sol,
-                  gascon_flags=gflags, hmin_static=bool(np.isin(r.idgas, (48, 91)).any()))
+                  gascon_flags=gflags, hmin_static=bool(np.isin(r.idgas, (48, 91)).any()),
+                  cia_fixture=cfx)
         print(f"{name:16s} {os.path.getsize(path) / 1024:8.1f} KiB  "
               f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
 

@@ -29,7 +29,8 @@ def test_golden_case(gpu_lib, path, mode):
     tables, req, outs, note = load_case(path)
     print(f"case {path.split('/')[-1]} mode {mode}", flush=True)
     files = np.load(path, allow_pickle=False).files
-    own = any(k in files for k in ("cia_kcia", "e3_val", "n2n2_k", "sol_rad", "gascon_flags"))
+    own = any(k in files for k in ("cia_kcia", "e3_val", "n2n2_k", "sol_rad", "gascon_flags",
+                                   "cia_fixture"))
     if own:  # a case with tables of its own (the registry is global)
         gpu_lib.register_cia(case_cia(path))
     if case_hmin_static(path):
@@ -215,3 +216,56 @@ def test_stellar_spectrum_rules(gpu_lib):
     finally:
         gpu_lib.set_iform(0, 0)
         gpu_lib.register_cia(load_cia())
+
+
+def test_cia_table_file_reader(gpu_lib, tmp_path):
+    """radg_read_cia on the reference's CIA table layout (REAL*4 as
+    raddata/isotest.tab, and REAL*8) gives what registering the arrays gives."""
+    from radtrancode_amd.cia import write_cia_table
+    from tests.golden.golden_io import load_cia_fixture
+    path = [p for p in CASES if p.endswith("case_cia_isotest.npz")][0]
+    tables, req, outs, _ = load_case(path)
+    c = load_cia_fixture("isotest")
+    gpu_lib.register_ktables(tables)
+    try:
+        gpu_lib.register_cia(c)
+        ref = gpu_lib.spectrum(req, mode=0)
+        for nb in (4, 8):
+            f = str(tmp_path / f"iso{nb}.tab")
+            write_cia_table(f, c.temps, c.kcia, real_bytes=nb)
+            print(f"radg_read_cia REAL*{nb}", flush=True)
+            gpu_lib.read_cia(f, c.dnu)
+            got = gpu_lib.spectrum(req, mode=0)
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b)
+    finally:
+        gpu_lib.register_cia(load_cia())
+
+
+@pytest.mark.parametrize("name", ["cfg1_slice", "kta_t2", "kta_irregular_nearest",
+                                  "kta_irregular_between", "kta_t2_irregular"])
+def test_read_klist_files(gpu_lib, tmp_path, name):
+    """radg_read_klist on .kta files written from a case's tables (4- and
+    8-byte records; per-pressure temperature grids and irregular wavenumber
+    grids included) gives what registering the arrays gives."""
+    from radtrancode_amd.kta import write_kta
+    path = [p for p in CASES if p.endswith(f"case_{name}.npz")][0]
+    tables, req, outs, _ = load_case(path)
+    gpu_lib.register_ktables(tables)
+    ref = gpu_lib.spectrum(req, mode=0)
+    for recl in (8, 4):
+        names = []
+        for t, kt in enumerate(tables):
+            f = str(tmp_path / f"g{t}_{recl}.kta")
+            write_kta(f, kt, recl=recl)
+            names.append(f)
+        kl = tmp_path / f"list{recl}.kls"
+        kl.write_text("\n".join(names) + "\n")
+        print(f"read_klist {name} recl {recl}", flush=True)
+        gpu_lib.read_klist(str(kl), recl=recl)
+        got = gpu_lib.spectrum(req, mode=0)
+        if recl == 8:
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b)
+        else:  # REAL*4 records round the tables
+            assert rad_err(got[0], ref[0]) < 1e-5

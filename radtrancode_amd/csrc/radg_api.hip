@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <set>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,9 @@ size_t radg_lds_emission_bytes(int RL, int nsw, int fp64);
 int radg_layer_E(int ng);
 int radg_emission_RL(int maxlays);
 hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t lds, hipStream_t st);
+size_t radg_lds_fused_bytes(int mode, int ng, int E, int RL, int maxgas, int maxnsw, long long maxblock,
+                            int *ws);
+hipError_t radg_launch_fused(const KernelArgs &a, int mode, int E, int RL, size_t lds, hipStream_t st);
 hipError_t radg_convert_f32(const double *src, float *dst, long long n, hipStream_t st);
 hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n, double off,
                              double sc, hipStream_t st);
@@ -865,6 +869,17 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
       tbytes += (double)nodes.size() * NG * kbytes;
     }
   }
+  // gases with a k-table anywhere on the spectrum get a dk row
+  P.d.nks = 0;
+  for (int k = 0; k < ngas; k++) {
+    bool any = false;
+    for (int iw = 0; iw < nwave && !any; iw++) any = P.wg[(size_t)iw * ngas + k].base >= 0;
+    P.d.kslot[k] = any ? (signed char)P.d.nks++ : (signed char)-1;
+  }
+  for (int k1 = 0; k1 < RADG_MAXPARAM; k1++) {
+    const int k = k1 < P.d.nsw ? P.d.isw[k1] : -1;
+    P.d.kdrow[k1] = (k >= 0 && k < ngas) ? P.d.kslot[k] : (k == ngas ? (signed char)P.d.nks : (signed char)-1);
+  }
   // algorithmic bytes: table nodes + layer inputs + CIA nodes + outputs
   const double obytes = (double)nwave * ((double)npath * (2.0 + (double)L * nparam) * obytes_el +
                                          (2.0 + (double)L * nparam) * 8.0);
@@ -894,16 +909,22 @@ struct radg_batch {
   KernelArgs args{};
   int E = 1, RL = 1, maxngas = 1, maxnsw = 1;
   size_t ldsA = 0, ldsB = 0;
-  // emission launches, one per layers-per-lane class (chosen per request from
-  // its own layer count, so a request's arithmetic never depends on the
-  // other requests of its batch)
+  // launches per class of requests with one emission layers-per-lane count
+  // (chosen per request from its own layer count, so a request's arithmetic
+  // never depends on the other requests of its batch).  A class whose
+  // layer results fit the LDS of one workgroup runs as the fused kernel;
+  // the others as the continuum, layer and emission kernels.  Both forms do
+  // the same arithmetic.
   struct EmClass {
     int RL, nreq;
     bool disc;  // a member has an imod 8, 13, 14 or 18 path or uses the stellar spectrum
-    long long nB;
+    long long nA, nB, nW;                      // layer, path, wavenumber items
     const int *reqmap;
-    const long long *start;
-    size_t lds;
+    const long long *startA, *start, *startW;  // class-local item prefix sums
+    size_t lds, ldsA;                          // emission, layer kernels
+    bool fused;
+    size_t ldsF;                               // fused kernel
+    int wsbytes;
   };
   std::vector<EmClass> em;
   double alg_bytes[3] = {0, 0, 0};
@@ -1078,7 +1099,7 @@ struct Plan {
   int maxngas = 1, maxnsw = 1, maxlays = 1;
   std::vector<int> cls_rl;
   std::vector<std::vector<int>> cls_members;
-  size_t o_clsmap = 0, o_clsstart = 0;
+  size_t o_clsmap = 0, o_clsstart = 0, o_clsstartA = 0, o_clsstartW = 0;
 };
 
 int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
@@ -1123,7 +1144,7 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     pl.nA += (long long)q.nwave * q.nlayer;
     pl.nB += (long long)q.nwave * q.npath;
     pl.nC += q.nwave;
-    pl.ninter += (long long)q.nwave * inter_block(ts.ng, q.nlayer, q.ngas, d.nparam);
+    pl.ninter += (long long)q.nwave * inter_block(ts.ng, q.nlayer, d.nks, d.nparam);
     pl.nout += (long long)q.nwave * q.npath;
     pl.njac += (long long)q.nwave * q.npath * q.nlayer * d.nparam;
     pl.nwout += q.nwave;
@@ -1173,6 +1194,8 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
   pl.o_ct = lo.take<double>(pl.nct);
   pl.o_clsmap = lo.take<int>(nreq);
   pl.o_clsstart = lo.take<long long>(nreq + pl.cls_rl.size());
+  pl.o_clsstartA = lo.take<long long>(nreq + pl.cls_rl.size());
+  pl.o_clsstartW = lo.take<long long>(nreq + pl.cls_rl.size());
   pl.o_inter = lo.take<char>((size_t)pl.ninter * ts_);
   pl.o_out = lo.take<char>((size_t)pl.nout * ts_);
   pl.o_gts = lo.take<char>((size_t)pl.nout * ts_);
@@ -1202,18 +1225,26 @@ void stage_inputs(const Plan &pl, int nreq, const radg_request *reqs, char *h) {
   put(h, pl.o_sC, pl.sC.data(), nreq + 1);
   {
     std::vector<int> cmap;
-    std::vector<long long> cstart;
+    std::vector<long long> cstart, cstartA, cstartW;
     for (const auto &mem : pl.cls_members) {
-      long long acc = 0;
+      long long acc = 0, accA = 0, accW = 0;
       cstart.push_back(0);
+      cstartA.push_back(0);
+      cstartW.push_back(0);
       for (int r : mem) {
         cmap.push_back(r);
         acc += (long long)reqs[r].nwave * reqs[r].npath;
+        accA += (long long)reqs[r].nwave * reqs[r].nlayer;
+        accW += reqs[r].nwave;
         cstart.push_back(acc);
+        cstartA.push_back(accA);
+        cstartW.push_back(accW);
       }
     }
     put(h, pl.o_clsmap, cmap.data(), cmap.size());
     put(h, pl.o_clsstart, cstart.data(), cstart.size());
+    put(h, pl.o_clsstartA, cstartA.data(), cstartA.size());
+    put(h, pl.o_clsstartW, cstartW.data(), cstartW.size());
   }
   for (int r = 0; r < nreq; r++) {
     const Packed &P = pl.packs[r];
@@ -1310,40 +1341,96 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
   a.err = reinterpret_cast<int *>(m + pl.o_err);
 }
 
-// One emission launch per layers-per-lane class (plan_batch's classes), with
-// the class maps in the device block at m.
+// The launches of each layers-per-lane class (plan_batch's classes), with
+// the class maps in the device block at m.  Classes run as the continuum,
+// layer and emission kernels.  RADG_FUSED=1 in the environment runs each
+// class whose workgroup (16 wavefronts) fits the CU's 160 KiB of LDS as the
+// fused kernel instead: the same arithmetic with the layer results kept in
+// LDS, slower at the 120-layer workload because a CU then holds one
+// workgroup and waits at its phase barriers.
 std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_request *reqs, char *m,
-                                                  int fp64) {
+                                                  int mode, int ng) {
+  const int fp64 = mode == RADG_MODE_FP64;
+  const int E = radg_layer_E(ng);
+  const char *fz = std::getenv("RADG_FUSED");
+  const bool allow_fused = fz && fz[0] == '1';
+  const size_t fused_cap = 160 * 1024;
   std::vector<radg_batch::EmClass> em;
-  {
-    size_t off = 0, soff = 0;
-    for (size_t c = 0; c < pl.cls_rl.size(); c++) {
-      radg_batch::EmClass ec;
-      ec.RL = pl.cls_rl[c];
-      ec.nreq = (int)pl.cls_members[c].size();
-      ec.nB = 0;
-      int maxnsw = 1;
-      ec.disc = false;
-      for (int r : pl.cls_members[c]) {
-        ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
-        maxnsw = std::max(maxnsw, reqs[r].nsw);
-        for (int p = 0; p < reqs[r].npath; p++) {
-          const int im = reqs[r].imod[p];
-          ec.disc = ec.disc || im == 8 || im == 13 || im == 14 || im == 18;
-        }
-        ec.disc = ec.disc || pl.packs[r].d.solar != 0;
+  size_t off = 0, soff = 0;
+  for (size_t c = 0; c < pl.cls_rl.size(); c++) {
+    radg_batch::EmClass ec;
+    ec.RL = pl.cls_rl[c];
+    ec.nreq = (int)pl.cls_members[c].size();
+    ec.nA = ec.nB = ec.nW = 0;
+    int maxnsw = 1, maxgas = 1;
+    long long maxblock = 0;
+    ec.disc = false;
+    for (int r : pl.cls_members[c]) {
+      ec.nA += (long long)reqs[r].nwave * reqs[r].nlayer;
+      ec.nB += (long long)reqs[r].nwave * reqs[r].npath;
+      ec.nW += reqs[r].nwave;
+      maxnsw = std::max(maxnsw, reqs[r].nsw);
+      maxgas = std::max(maxgas, reqs[r].ngas);
+      maxblock = std::max(maxblock, inter_block(ng, reqs[r].nlayer, pl.packs[r].d.nks, pl.packs[r].d.nparam));
+      for (int p = 0; p < reqs[r].npath; p++) {
+        const int im = reqs[r].imod[p];
+        ec.disc = ec.disc || im == 8 || im == 13 || im == 14 || im == 18;
       }
-      ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
-      ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
-      ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
-      off += ec.nreq;
-      soff += ec.nreq + 1;
-      em.push_back(ec);
+      ec.disc = ec.disc || pl.packs[r].d.solar != 0;
     }
+    ec.reqmap = reinterpret_cast<const int *>(m + pl.o_clsmap) + off;
+    ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
+    ec.startA = reinterpret_cast<const long long *>(m + pl.o_clsstartA) + soff;
+    ec.startW = reinterpret_cast<const long long *>(m + pl.o_clsstartW) + soff;
+    ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
+    ec.ldsA = radg_lds_layer_bytes(ng, maxgas, E, fp64);
+    ec.wsbytes = 0;
+    ec.ldsF = radg_lds_fused_bytes(mode, ng, E, ec.RL, maxgas, maxnsw, maxblock, &ec.wsbytes);
+    ec.fused = allow_fused && ec.ldsF > 0 && ec.ldsF <= fused_cap;
+    off += ec.nreq;
+    soff += ec.nreq + 1;
+    em.push_back(ec);
   }
   return em;
 }
 
+// The kernels of a batch on stream st: the layer pass of every class (the
+// fused kernel, or the continuum and layer kernels), event ev1, the
+// emission kernel of the classes not fused, event ev2, the path quadrature.
+int launch_classes(const KernelArgs &base, const std::vector<radg_batch::EmClass> &em, int mode,
+                   int E, hipStream_t st, hipEvent_t ev1, hipEvent_t ev2) {
+  const int fp64 = mode == RADG_MODE_FP64;
+  for (const auto &ec : em) {
+    KernelArgs a = base;
+    a.reqmapA = a.reqmapB = ec.reqmap;
+    a.nreqA = a.nreqB = ec.nreq;
+    if (ec.fused) {
+      a.startB = ec.startW;
+      a.nB = ec.nW;
+      a.wsbytes = ec.wsbytes;
+      HIPCHK(radg_launch_fused(a, mode, E, ec.RL, ec.ldsF, st));
+    } else {
+      if (ec.ldsA > 160 * 1024 || ec.lds > 160 * 1024)
+        return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", ec.ldsA, ec.lds);
+      a.startA = ec.startA;
+      a.nA = ec.nA;
+      HIPCHK(radg_launch_layer_lds(a, mode, E, ec.ldsA, st));
+    }
+  }
+  if (ev1) HIPCHK(hipEventRecord(ev1, st));
+  for (const auto &ec : em) {
+    if (ec.fused) continue;
+    KernelArgs a = base;
+    a.startB = ec.start;
+    a.nB = ec.nB;
+    a.reqmapB = ec.reqmap;
+    a.nreqB = ec.nreq;
+    HIPCHK(radg_launch_emission(a, fp64, ec.RL, ec.disc, ec.lds, st));
+  }
+  if (ev2) HIPCHK(hipEventRecord(ev2, st));
+  HIPCHK(radg_launch_quadrature(base, fp64, st));
+  return RADG_OK;
+}
 
 }  // namespace
 
@@ -1860,11 +1947,10 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
   b->maxngas = pl.maxngas;
   b->maxnsw = pl.maxnsw;
   b->RL = radg_emission_RL(pl.maxlays);
-  b->ldsA = radg_lds_layer_bytes(ts->ng, pl.maxngas, b->E, fp64);
-  b->ldsB = radg_lds_emission_bytes(b->RL, pl.maxnsw, fp64);
-  if (b->ldsA > 160 * 1024 || b->ldsB > 160 * 1024)
-    return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", b->ldsA, b->ldsB);
-  b->em = emission_classes(pl, reqs, b->dmem, fp64);
+  b->em = emission_classes(pl, reqs, b->dmem, mode, ts->ng);
+  for (const auto &ec : b->em)
+    if (!ec.fused && (ec.ldsA > 160 * 1024 || ec.lds > 160 * 1024))
+      return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", ec.ldsA, ec.lds);
   // events are created on first use of each ring slot
   b->evs.assign(4 * radg_batch::RING, nullptr);
   b->ev = &b->evs[0];
@@ -1930,10 +2016,7 @@ int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, doubl
   if (need > RADG_NOALLOC_BYTES)
     return fail(RADG_ELIMITS, "the request needs %zu bytes of scratch; the library holds %zu "
                 "(split its wavenumbers over calls)", need, (size_t)RADG_NOALLOC_BYTES);
-  const int fp64 = mode == RADG_MODE_FP64;
   const int E = radg_layer_E(ts->ng);
-  const size_t ldsA = radg_lds_layer_bytes(ts->ng, pl.maxngas, E, fp64);
-  if (ldsA > 160 * 1024) return fail(RADG_ELIMITS, "LDS need %zu bytes exceeds 160 KiB", ldsA);
   char *m = nullptr;
   rc = noalloc_scratch(&m);
   if (rc) return rc;
@@ -1948,17 +2031,7 @@ int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, doubl
   a.wgts = wgts;
   a.wjac = wjac;
   a.err = nullptr;
-  HIPCHK(radg_launch_layer_lds(a, mode, E, ldsA, st));
-  for (const auto &ec : emission_classes(pl, req, m, fp64)) {
-    KernelArgs e = a;
-    e.startB = ec.start;
-    e.nB = ec.nB;
-    e.reqmapB = ec.reqmap;
-    e.nreqB = ec.nreq;
-    HIPCHK(radg_launch_emission(e, fp64, ec.RL, ec.disc, ec.lds, st));
-  }
-  HIPCHK(radg_launch_quadrature(a, fp64, st));
-  return RADG_OK;
+  return launch_classes(a, emission_classes(pl, req, m, mode, ts->ng), mode, E, st, nullptr, nullptr);
 }
 
 int radg_batch_launch(radg_batch *b, void *stream) {
@@ -1980,18 +2053,10 @@ int radg_batch_launch(radg_batch *b, void *stream) {
     b->sup_pending = false;
   }
   HIPCHK(hipEventRecord(b->ev[0], st));
-  HIPCHK(radg_launch_layer_lds(b->args, b->mode, b->E, b->ldsA, st));
-  HIPCHK(hipEventRecord(b->ev[1], st));
-  for (const auto &ec : b->em) {
-    KernelArgs a = b->args;
-    a.startB = ec.start;
-    a.nB = ec.nB;
-    a.reqmapB = ec.reqmap;
-    a.nreqB = ec.nreq;
-    HIPCHK(radg_launch_emission(a, fp64, ec.RL, ec.disc, ec.lds, st));
+  {
+    const int rc = launch_classes(b->args, b->em, b->mode, b->E, st, b->ev[1], b->ev[2]);
+    if (rc) return rc;
   }
-  HIPCHK(hipEventRecord(b->ev[2], st));
-  HIPCHK(radg_launch_quadrature(b->args, fp64, st));
   if (b->smem) {
     HIPCHK(radg_launch_gemm(b->d_pc, b->n_pc, b->mx_pc[0], b->mx_pc[1], 0, st));
     HIPCHK(radg_launch_gemm(b->d_pk64, b->n_pk64, b->mx_pk64[0], b->mx_pk64[1], 0, st));

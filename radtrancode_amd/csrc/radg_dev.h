@@ -12,6 +12,7 @@
 #include "../../include/radg.h"
 
 #define RADG_WAVE 64
+#define RADG_FUSED_WAVES 16  // wavefronts per workgroup of the fused kernel
 #define RADG_MAXPARAM 41  // ngas + 1 + ncont with ngas <= 30, ncont <= 10 (no para-H2)
 
 // gas-continuum kinds (ngascon.f:144-262) and the per-wavenumber factors the
@@ -46,6 +47,13 @@ struct DevReq {
   signed char gcon[RADG_MAXGAS];
   short ke80, ke91;
   long long gcw0;
+  // gases with a k-table: row of each gas in the layer results' dk (-1: the
+  // gas has no table and its dk/dq is zero), nks of them
+  signed char kslot[RADG_MAXGAS];
+  int nks;
+  // dk row of each Jacobian parameter isw[k1] (kslot for a gas, nks for T,
+  // -1 for a gas without a table and the continuum-only parameters)
+  signed char kdrow[RADG_MAXPARAM];
   int isw[RADG_MAXPARAM];      // 0-based parameter ids
   long long lay0;              // LayerConst / per-layer arrays
   long long lg0;               // per-(layer, gas) arrays: amo
@@ -102,14 +110,53 @@ struct PathInfo {
 };
 
 // Intermediate block per (request, wavenumber), in elements of the compute
-// type:  ktot[NG][L] | dk[NG][ngas+1][L] | tcon[L] | dtcon[nparam][L]
-// dk and dtcon hold derivatives with respect to amount*1e-20 for gas slots
-// (the reference's dkdq before its 1e-20 factor, cirsradg_wave.f:749) and
-// with respect to T for slot ngas; dtcon's further rows are the continuum's
-// only parameters: each dust type (its extinction, the same in every layer)
-// and the para-H2 fraction.
-__host__ __device__ inline long long inter_block(int ng, int L, int ngas, int nparam) {
-  return (long long)ng * L * (ngas + 2) + (long long)L * (1 + nparam);
+// type:  ktot[NG][L] | dk[NG][nks+1][L] | tcon[L] | dtcon[nparam][L]
+// dk holds derivatives with respect to amount*1e-20 for the nks gases with a
+// k-table (row kslot[gas]; the reference's dkdq before its 1e-20 factor,
+// cirsradg_wave.f:749) and with respect to T in row nks.  dtcon holds them
+// for every gas (slot = gas) and T (slot ngas); its further rows are the
+// continuum's only parameters: each dust type (its extinction, the same in
+// every layer) and the para-H2 fraction.
+__host__ __device__ inline long long inter_block(int ng, int L, int nks, int nparam) {
+  return (long long)ng * L * (nks + 2) + (long long)L * (1 + nparam);
+}
+
+// LDS of the layer pass.  A workgroup holds a header shared by its
+// wavefronts, G [ng+1] and delg [ng] in FP64, delg [ng] in the compute type
+// and the FP64 weight of each of the 64*E sort slots, then one scratch area
+// per wavefront for the layer item it is on.
+__host__ __device__ inline int lds_round16(int b) { return (b + 15) & ~15; }
+__host__ __device__ inline int layer_header_bytes(int ng, int E, int ts) {
+  return lds_round16(8 * (ng + 1)) + lds_round16(8 * ng) + lds_round16(ts * ng) + 8 * RADG_WAVE * E;
+}
+// Sort slots of an overlap round: slot s = l*E + q of lane l holds element
+// (i, j) = (l / lpr, (l % lpr) * E + q), lpr = ceil(ng / E) lanes per row of
+// the running distribution; slots past a row's ng elements, and lanes past
+// ng * lpr, are padding.  s grows with (i, j) in lexicographic order.
+__host__ __device__ inline int layer_lpr(int ng, int E) { return (ng + E - 1) / E; }
+// Byte offsets within one wavefront's scratch (ts: bytes of the compute
+// type).  `uni` is shared, in time, by the rankk merge buffers, the exact
+// FP32 keys of an overlap sort and its per-bin records.
+struct LayerLayout {
+  int sSt, sGd, kgn, dkgn, bufA, bufB, sE, uni, total;
+};
+__host__ __device__ inline LayerLayout layer_layout(int ng, int ngas, int E, int ts) {
+  LayerLayout L;
+  int o = 0;
+  L.sSt = o;  o += lds_round16(8 * ng);
+  L.sGd = o;  o += lds_round16(8 * ng);
+  L.kgn = o;  o += lds_round16(ts * ngas * ng);
+  L.dkgn = o; o += lds_round16(ts * ngas * ng);
+  L.bufA = o; o += lds_round16(ts * (ngas + 2) * ng);
+  L.bufB = o; o += lds_round16(ts * (ngas + 2) * ng);
+  L.sE = o;   o += lds_round16(4 * ng);
+  int u = 2 * 8 * ng * (ngas + 2);                         // bin records
+  const int merge = 3 * RADG_WAVE * ts + (ts == 4 ? 4 * RADG_WAVE : 0);
+  if (u < merge) u = merge;
+  if (ts == 4 && u < 4 * RADG_WAVE * E) u = 4 * RADG_WAVE * E;  // exact keys
+  L.uni = o;  o += lds_round16(u);
+  L.total = o;
+  return L;
 }
 
 struct KernelArgs {
@@ -117,10 +164,13 @@ struct KernelArgs {
   int nreq;
   const long long *startA, *startB, *startC;  // [nreq+1] item prefix sums
   long long nA, nB, nC;
-  // emission launches run one layers-per-lane class at a time: startB and
-  // nB then cover the class's requests, listed in reqmapB[nreqB]
-  const int *reqmapB;
-  int nreqB;
+  // launches run one class of requests at a time: startA/nA (layer and
+  // continuum items) and startB/nB (emission items, or wavenumber items of
+  // the fused kernel) then cover the class's requests, listed in
+  // reqmapA[nreqA] and reqmapB[nreqB]
+  const int *reqmapA, *reqmapB;
+  int nreqA, nreqB;
+  int wsbytes;                 // fused kernel: LDS scratch bytes per wavefront
   int ng;
   // tables
   const void *ktab;            // double* (FP64 form) or float* (mixed form)

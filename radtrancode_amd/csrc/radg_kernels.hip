@@ -344,6 +344,12 @@ template <int E>
 __device__ __forceinline__ void bitonic_sort_u32(unsigned (&v)[E], int lane) {
   u32_merge_levels<E, 1>(v, lane);
 }
+// The same network when each lane's E values are already ascending: the
+// merge levels within a lane leave sorted runs unchanged and are skipped.
+template <int E>
+__device__ __forceinline__ void bitonic_sort_u32_lanesorted(unsigned (&v)[E], int lane) {
+  u32_merge_levels<E, ilog2c(E) + 1>(v, lane);
+}
 
 __device__ __forceinline__ bool kf_lt(float ka, int ia, float kb, int ib) {
   return (ka < kb) | ((ka == kb) & (ia < ib));
@@ -429,6 +435,11 @@ __device__ __forceinline__ double wave_incl_sum(double s) {
 }
 // value of lane - 1 (0 in lane 0)
 __device__ __forceinline__ double wave_shr1(double s) { return dpp_d<0x138, 0xF, true>(s); }
+template <typename T> __device__ __forceinline__ T dpp_t(T x);
+template <> __device__ __forceinline__ float dpp_t<float>(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xF, 0xF, true));
+}
+template <> __device__ __forceinline__ double dpp_t<double>(double x) { return wave_shr1(x); }
 
 // ----------------------------------------------------------- CIA, FP64
 // nciacon.f:287-296 and nparacon_all.f:312-320: N2's near-infrared N2-N2
@@ -769,60 +780,91 @@ __device__ __forceinline__ double gascon_d(const KernelArgs &a, const DevReq &R,
   return 0.0;
 }
 
-}  // namespace
+// ============================================================ layer pass
+// LDS of a layer item (layer_layout in radg_dev.h):
+//   header (per workgroup)  G[NG+1], delg[NG] (FP64), delg[NG] (T)
+//   per wavefront           sSt, sGd [NG] (FP64)  the straddling element's
+//                           start and end cumulative weight, per bin
+//                           kgn, dkgn [ngas][NG]  interpolated k, dk/dT
+//                           bufA, bufB [NS][NG]   overlap row state
+//                           sE [NG]               straddling element, per bin
+//                           uni                   rankk merge buffers | exact
+//                                                 FP32 keys | rebin matrix,
+//                                                 bin codes, straddle shares
+// with NS = ngas + 2 row slots (K, gas 0..ngas-1, T).
 
-// ============================================================ layer kernel
-// LDS layout (dynamic), sized by radg_lds_layer_bytes():
-//   G[NG+1] | recS[NG][NQ] | recC[NG][NQ] | recG[NG] | recW[NG] | dl64[NG] (FP64)
-//   kgn[ngas][NG] | dkgn[ngas][NG] | bufA[NS][NG] | bufB[NS][NG] | dlg[NG]  (T)
-//   rtype[NG]                                                      (int)
-//   rkK | rkD | rkF [64 each]  rankk merge of two slices, 8-byte aligned (T)
-//   keyE[64*E]  exact FP32 key of each element (FP32 form)       (float)
-// with NS = ngas + 2 row slots (K, gas 0..ngas-1, T) and NQ = ngas + 2 rebinned
-// quantities (key, gas slots, T).
-template <typename T, int E, int BF>
-__global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs a, long long nblocks) {
-  const long long item = xcd_item(nblocks);
-  if (item >= a.nA) return;
-  const int lane = threadIdx.x;
-  const int rq = find_req(a.startA, a.nreq, item);
-  const DevReq &R = a.reqs[rq];
-  const int L = R.L, ngas = R.ngas, NG = a.ng;
-  const long long loc = item - R.itemA0;
-  const int iw = (int)(loc / L);
-  const int j = (int)(loc - (long long)iw * L);
-  const LayerConst lc = a.lay[R.lay0 + j];
-  const int n = NG * NG;
-  const int NS = ngas + 2;
-  const int NQ = ngas + 2;
+// Orders the LDS accesses of one wavefront: LDS instructions of a wavefront
+// complete in issue order, so a compiler barrier is all that lanes reading
+// each other's writes need.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-  extern __shared__ double smem_d[];
-  double *G = smem_d;
-  double *recS = G + (NG + 1);
-  double *recC = recS + NG * NQ;
-  double *recG = recC + NG * NQ;
-  double *recW = recG + NG;
-  double *dl64 = recW + NG;
-  T *kgn = reinterpret_cast<T *>(dl64 + NG);
-  T *dkgn = kgn + ngas * NG;
-  T *bufA = dkgn + ngas * NG;
-  T *bufB = bufA + NS * NG;
-  T *dlg = bufB + NS * NG;
-  int *rtype = reinterpret_cast<int *>(dlg + NG);
-  T *rkK = reinterpret_cast<T *>(smem_d + ((reinterpret_cast<char *>(rtype + NG) -
-                                            reinterpret_cast<char *>(smem_d) + 7) / 8));
-  T *rkD = rkK + W, *rkF = rkD + W;                      // [W] each: rankk merge
-  float *keyE = reinterpret_cast<float *>(rkF + W);      // [W*E], FP32 form only
-  const float rng = 1.0f / (float)NG;  // element e -> i = floor((e + 0.5) / NG)
-
+// Fills the workgroup header: G (g boundaries, G[NG] >= 1, rankg.f:59-65),
+// the g weights in FP64 and in T, and the FP64 weight delg_i delg_j of the
+// element in each sort slot (0 for padding).
+template <typename T, int E>
+__device__ __forceinline__ void layer_header(const KernelArgs &a, char *hdr, int tid, int nthr) {
+  const int NG = a.ng;
+  double *G = reinterpret_cast<double *>(hdr);
+  double *dl64 = reinterpret_cast<double *>(hdr + lds_round16(8 * (NG + 1)));
+  T *dlg = reinterpret_cast<T *>(hdr + lds_round16(8 * (NG + 1)) + lds_round16(8 * NG));
+  double *w64 = reinterpret_cast<double *>(reinterpret_cast<char *>(dlg) + lds_round16((int)sizeof(T) * NG));
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
                                    : reinterpret_cast<const T *>(a.delg32);
-  for (int i = lane; i <= NG; i += W) G[i] = a.G[i];
-  for (int i = lane; i < NG; i += W) {
-    dlg[i] = delg[i];
+  for (int i = tid; i <= NG; i += nthr) G[i] = a.G[i];
+  for (int i = tid; i < NG; i += nthr) {
     dl64[i] = a.delg64[i];
-    rtype[i] = 0;
+    dlg[i] = delg[i];
   }
+  const int lpr = layer_lpr(NG, E);
+  for (int s = tid; s < W * E; s += nthr) {
+    const int l = s / E, i = l / lpr, jx = (l - i * lpr) * E + (s & (E - 1));
+    w64[s] = (i < NG && jx < NG) ? a.delg64[i] * a.delg64[jx] : 0.0;
+  }
+}
+
+// One (request, wavenumber iw, layer j) item on one wavefront: the k-table
+// interpolation of every gas (get_kg.f:308-476), then the random overlap of
+// noverlapg.f:64-200 round by round, each round a sort of the NG*NG mixed
+// keys (rankg.f:77-96) and a rebin into NG bins (rankg.f:104-152).  Writes
+// the layer's k-distribution to ok[i * ostride + j] and its derivatives to
+// od[(i * (nks + 1) + s) * ostride + j] (s = kslot[gas] < nks: the gases with
+// a k-table, per 1e20 molecules; s = nks: temperature).
+//
+// Sort slots: lane l holds E consecutive elements of one row of the NG x NG
+// key matrix (layer_lpr), so with k2 non-decreasing its keys arrive in
+// order and the in-lane merge levels of the network are skipped.  The
+// rebin takes, for the key and each payload, an FP64 prefix along the
+// sorted order recorded at each bin's closing slot; a bin is its record
+// minus the previous bin's.  Rounds whose gas has no k at this wavenumber
+// leave the distribution as it is (noverlapg.f:100,166).
+template <typename T, int E, int BF>
+__device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R, int iw, int j,
+                                           const char *hdr, char *ws, T *ok, T *od, long long ostride,
+                                           int lane) {
+  const int ngas = R.ngas, NG = a.ng;
+  const LayerConst lc = a.lay[R.lay0 + j];
+  const int NS = ngas + 2;
+  const double *G = reinterpret_cast<const double *>(hdr);
+  const double *dl64 = reinterpret_cast<const double *>(hdr + lds_round16(8 * (NG + 1)));
+  const T *dlg = reinterpret_cast<const T *>(hdr + lds_round16(8 * (NG + 1)) + lds_round16(8 * NG));
+  const LayerLayout lo = layer_layout(NG, ngas, E, (int)sizeof(T));
+  double *sSt = reinterpret_cast<double *>(ws + lo.sSt);
+  double *sGd = reinterpret_cast<double *>(ws + lo.sGd);
+  T *kgn = reinterpret_cast<T *>(ws + lo.kgn);
+  T *dkgn = reinterpret_cast<T *>(ws + lo.dkgn);
+  T *bufA = reinterpret_cast<T *>(ws + lo.bufA);
+  T *bufB = reinterpret_cast<T *>(ws + lo.bufB);
+  int *sE = reinterpret_cast<int *>(ws + lo.sE);
+  char *uni = ws + lo.uni;
+  const float rng = 1.0f / (float)NG;  // e -> i = floor((e + 0.5) / NG)
+  // lane -> (group gq, ordinate or bin gg): W / NG groups of NG lanes
+  const int gpp = W / NG;
+  const int gq = (int)(((float)lane + 0.5f) * rng);
+  const int gg = lane - gq * NG;
 
   // ---- k-table interpolation, get_kg.f:308-476 -----------------------------
   {
@@ -847,6 +889,65 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
     };
     const float ln10 = 2.302585092994046f;
 
+    // Coincident or zero slices (get_kg.f:319-376), W / NG gases at a time:
+    // ln k bilinear when the first ordinate of all four corners is > 0
+    // (KLOG), else k itself.
+    for (int k0 = 0; k0 < ngas; k0 += gpp) {
+      const int k = k0 + gq;
+      if (gq < gpp && k < ngas) {
+        const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
+        if (wgk.base < 0 || wgk.coinc) {
+          T kk = 0, dk = 0;
+          if (wgk.base >= 0) {
+            bool klog;
+            T y1, y2, y3, y4;
+            if (BF) {
+              // BF16 log10(k) codes: ln k comes straight from the code
+              const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + wgk.base;
+              klog = !(sl[i1] == RADG_BF16_NONPOS || sl[i2] == RADG_BF16_NONPOS ||
+                       sl[i3] == RADG_BF16_NONPOS || sl[i4] == RADG_BF16_NONPOS);
+              auto dec = [&](unsigned short c) -> float {
+                if (c == RADG_BF16_NONPOS) return klog ? -__builtin_huge_valf() : 0.0f;
+                const float l = (__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10;
+                return klog ? l : expf(l);
+              };
+              y1 = (T)dec(sl[i1 + gg]);
+              y2 = (T)dec(sl[i2 + gg]);
+              y3 = (T)dec(sl[i3 + gg]);
+              y4 = (T)dec(sl[i4 + gg]);
+            } else {
+              const T *sl = tab + wgk.base;
+              klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
+              y1 = sl[i1 + gg];
+              y2 = sl[i2 + gg];
+              y3 = sl[i3 + gg];
+              y4 = sl[i4 + gg];
+              if (klog) {
+                y1 = dlog(y1);
+                y2 = dlog(y2);
+                y3 = dlog(y3);
+                y4 = dlog(y4);
+              }
+            }
+            T x, dxdt;
+            bilin(y1, y2, y3, y4, x, dxdt);
+            if (klog) {
+              const T ex = dexp(x);
+              kk = ex;
+              dk = ex * dxdt;
+            } else {
+              kk = x;
+              dk = dxdt;
+            }
+            if (kk != kk) kk = (T)1e-37;
+            if (dk != dk) dk = (T)1e-37;
+          }
+          kgn[k * NG + gg] = kk;
+          dkgn[k * NG + gg] = dk;
+        }
+      }
+    }
+
     // Wavenumber between two table points (get_kg.f:379-440): interpolate
     // linear k on the slice at or below it (lanes 0..NG-1) and on the slice
     // above (lanes NG..2NG-1), weight the ordinates delg_i (1 - w2) and
@@ -854,7 +955,12 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
     // (whole points while their cumulative weight is <= the next edge, the
     // straddling point split by the fraction below the edge, each closed bin
     // divided by its accumulated weight).  FP64 accumulation in both forms.
-    auto merge_slices = [&](int k, const WaveGas &wgk) {
+    T *rkK = reinterpret_cast<T *>(uni);
+    T *rkD = rkK + W, *rkF = rkD + W;
+    float *keyM = reinterpret_cast<float *>(rkF + W);  // FP32 form: exact keys
+    for (int k = 0; k < ngas; k++) {
+      const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
+      if (wgk.base < 0 || wgk.coinc) continue;
       const int cnt = 2 * NG;
       const bool act = lane < cnt;
       const int gi = act ? (lane < NG ? lane : lane - NG) : 0;
@@ -883,27 +989,28 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
         const T wl = (lane < NG) ? (T)(1.0 - wgk.w2) : (T)wgk.w2;
         fw = dlg[gi] * wl;
       }
+      wave_sync();  // the previous merge's reads of the buffers are done
       rkD[lane] = dxdt;
       rkF[lane] = fw;
       T key[1];
       int idx[1];
       if constexpr (sizeof(T) == 4) {
         const float kq = act ? ((x > 0.0f) ? x : 0.0f) : __builtin_huge_valf();
-        keyE[lane] = kq;
+        keyM[lane] = kq;
         unsigned pk[1] = {(__float_as_uint(kq) & ~(unsigned)(W - 1)) | (unsigned)lane};
         bitonic_sort_u32<1>(pk, lane);
-        __syncthreads();
+        wave_sync();
         idx[0] = (int)(pk[0] & (unsigned)(W - 1));
-        key[0] = keyE[idx[0]];
+        key[0] = keyM[idx[0]];
         exact_fixup<1>(key, idx, lane);
       } else {
         key[0] = act ? x : dinf<T>();
         idx[0] = lane;
-        __syncthreads();
+        wave_sync();
         Sorter<T, 1>::sort(key, idx, lane);
       }
       const T sd = rkD[idx[0]], sf = rkF[idx[0]];
-      __syncthreads();
+      wave_sync();
       rkK[lane] = key[0];
       rkD[lane] = sd;
       rkF[lane] = sf;
@@ -911,7 +1018,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
         kgn[k * NG + i] = (T)0;
         dkgn[k * NG + i] = (T)0;
       }
-      __syncthreads();
+      wave_sync();
       double gprev = 0.0, sum = 0.0, ak = 0.0, ad = 0.0;
       int ig = 0;
       for (int I = 0; I < cnt; I++) {
@@ -945,90 +1052,50 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
         kgn[k * NG + ig] = (T)(ak / sum);
         dkgn[k * NG + ig] = (T)(ad / sum);
       }
-      __syncthreads();
+      wave_sync();
       for (int i = lane; i < NG; i += W) {  // get_kg.f:443-471
         const T kk = kgn[k * NG + i], dk = dkgn[k * NG + i];
         if (kk != kk) kgn[k * NG + i] = (T)1e-37;
         if (dk != dk) dkgn[k * NG + i] = (T)1e-37;
       }
-    };
-
-    for (int k = 0; k < ngas; k++) {
-      const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
-      if (wgk.base >= 0 && !wgk.coinc) {
-        merge_slices(k, wgk);
-        continue;
-      }
-      if (lane < NG) {
-        T kk = 0, dk = 0;
-        if (wgk.base >= 0) {
-          bool klog;
-          T y1, y2, y3, y4;
-          if (BF) {
-            // BF16 log10(k) codes: ln k comes straight from the code
-            const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + wgk.base;
-            klog = !(sl[i1] == RADG_BF16_NONPOS || sl[i2] == RADG_BF16_NONPOS ||
-                     sl[i3] == RADG_BF16_NONPOS || sl[i4] == RADG_BF16_NONPOS);
-            auto dec = [&](unsigned short c) -> float {
-              if (c == RADG_BF16_NONPOS) return klog ? -__builtin_huge_valf() : 0.0f;
-              const float l = (__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10;
-              return klog ? l : expf(l);
-            };
-            y1 = (T)dec(sl[i1 + lane]);
-            y2 = (T)dec(sl[i2 + lane]);
-            y3 = (T)dec(sl[i3 + lane]);
-            y4 = (T)dec(sl[i4 + lane]);
-          } else {
-            const T *sl = tab + wgk.base;
-            klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
-            y1 = sl[i1 + lane];
-            y2 = sl[i2 + lane];
-            y3 = sl[i3 + lane];
-            y4 = sl[i4 + lane];
-            if (klog) {
-              y1 = dlog(y1);
-              y2 = dlog(y2);
-              y3 = dlog(y3);
-              y4 = dlog(y4);
-            }
-          }
-          T x, dxdt;
-          bilin(y1, y2, y3, y4, x, dxdt);
-          if (klog) {
-            const T ex = dexp(x);
-            kk = ex;
-            dk = ex * dxdt;
-          } else {
-            kk = x;
-            dk = dxdt;
-          }
-          if (kk != kk) kk = (T)1e-37;
-          if (dk != dk) dk = (T)1e-37;
-        }
-        kgn[k * NG + lane] = kk;
-        dkgn[k * NG + lane] = dk;
-      }
     }
   }
-  __syncthreads();
+  wave_sync();
 
   // ---- random overlap, noverlapg.f:64-200 --------------------------------
-  // Row state (running sum): bufA[0] = K (key part), bufA[1+s] = gas slot s,
-  // bufA[1+ngas] = dK/dT.  Round 1 starts from gas 0 scaled by its amount.
+  for (int x = lane; x < NG; x += W) sE[x] = 0;  // bin record types, cleared per round
+  // Row state: slot 0 = K, slot 1+s = gas s, slot 1+ngas = dK/dT.  Round 1
+  // starts from gas 0 scaled by its amount.
   const double *amo = a.amo + R.lg0 + (long long)j * ngas;
   {
     const T a1 = (T)amo[0];
-    for (int i = lane; i < NG; i += W) {
-      const T k1 = kgn[i], d1 = dkgn[i];
-      bufA[i] = k1 * a1;
-      for (int s = 0; s < ngas; s++) bufA[(1 + s) * NG + i] = (s == 0) ? k1 : (T)0;
-      bufA[(1 + ngas) * NG + i] = d1 * a1;
+    for (int x = lane; x < NS * NG; x += W) {
+      const int sl = (int)(((float)x + 0.5f) * rng), i = x - sl * NG;
+      const T k1 = kgn[i];
+      T val = (T)0;
+      if (sl == 0) val = k1 * a1;
+      else if (sl == 1) val = k1;
+      else if (sl == NS - 1) val = dkgn[i] * a1;
+      bufA[x] = val;
     }
   }
-  __syncthreads();
+  wave_sync();
 
-  // first (i, j) of this lane's keys, e = lane*E = i*NG + j
-  const int i00 = (lane * E) / NG, j00 = lane * E - i00 * NG;
+  // this lane's sort slots: row li of the running distribution, columns
+  // lj0 .. lj0 + E - 1 of the new gas (layer_lpr)
+  const int lpr = layer_lpr(NG, E);
+  const float rlpr = 1.0f / (float)lpr;
+  const int li = (int)(((float)lane + 0.5f) * rlpr);
+  const int lj0 = (lane - li * lpr) * E;
+  const bool lrow = li < NG;
+  // (i, j) of slot s
+  auto slot_ij = [&](int sl, int &i, int &jx) {
+    const int l = sl / E;
+    i = (int)(((float)l + 0.5f) * rlpr);
+    jx = (l - i * lpr) * E + (sl & (E - 1));
+  };
+  const double *w64 = reinterpret_cast<const double *>(reinterpret_cast<const char *>(dlg) +
+                                                       lds_round16((int)sizeof(T) * NG));
 
   T *rows = bufA, *nxt = bufB;
   for (int r = 1; r < ngas; r++) {
@@ -1040,128 +1107,162 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
     const bool z1 = (r == 1) ? (kgn[NG - 1] == (T)0) : (rows[NG - 1] == (T)0);
     const bool z2 = (k2[NG - 1] == (T)0);
     if (z1) {
-      for (int i = lane; i < NG; i += W) {
-        nxt[i] = k2[i] * a2;
-        for (int s = 0; s < r; s++) nxt[(1 + s) * NG + i] = (T)0;
-        nxt[(1 + r) * NG + i] = k2[i];
-        for (int s = r + 1; s < ngas; s++) nxt[(1 + s) * NG + i] = (T)0;
-        nxt[(1 + ngas) * NG + i] = d2[i] * a2;
+      for (int x = lane; x < NS * NG; x += W) {
+        const int sl = (int)(((float)x + 0.5f) * rng), i = x - sl * NG;
+        T val = (T)0;
+        if (sl == 0) val = k2[i] * a2;
+        else if (sl == 1 + r) val = k2[i];
+        else if (sl == NS - 1) val = d2[i] * a2;
+        nxt[x] = val;
       }
     } else if (z2) {
-      for (int i = lane; i < NG; i += W) {
-        for (int s = 0; s < NS; s++) nxt[s * NG + i] = rows[s * NG + i];
-        nxt[(1 + r) * NG + i] = (T)0;
-      }
+      // the distribution is unchanged, and its slot for gas r is still zero
+      continue;
     } else {
-      // keys K_i + k2_j*a2 (noverlapg.f:117-128,175-187) of element
-      // e = i*NG + j; padding slots e >= n hold +inf.  Ties are ordered by e.
-      T key[E];
+      // keys K_i + k2_j*a2 (noverlapg.f:117-128,175-187); padding slots hold
+      // +inf.  Ties are ordered by slot, i.e. by (i, j).  With k2
+      // non-decreasing every lane's E keys arrive in order.
+      const bool k2up = __all(lane + 1 >= NG || !(k2[lane + 1] < k2[lane]));
       int idx[E];
-      int ij[E];  // (i << 8) | j of each sorted slot
-      if constexpr (sizeof(T) == 4) {
-        // FP32 keys are >= 0, so their bit patterns order like their
-        // values.  Sort one 32-bit word per slot, (key bits with the low
-        // log2(64E) mantissa bits cleared) | e, with min/max exchanges; then
-        // restore the exact (key, e) order inside runs of equal truncated
-        // keys from the exact keys kept in LDS.
-        constexpr unsigned IM = (unsigned)(W * E - 1);
-        unsigned pk[E];
-        {
-          int ii = i00, jj = j00;
+      {
+        T key[E];
+        const T K = lrow ? rows[li] : (T)0;
+        if constexpr (sizeof(T) == 4) {
+          // FP32 keys are >= 0, so their bit patterns order like their
+          // values.  Sort one 32-bit word per slot, (key bits with the low
+          // log2(64E) bits cleared) | slot, with min/max exchanges; then
+          // restore the exact (key, slot) order inside runs of equal
+          // truncated keys from the exact keys kept in LDS.
+          float *keyE = reinterpret_cast<float *>(uni);
+          constexpr unsigned IM = (unsigned)(W * E - 1);
+          unsigned pk[E];
 #pragma unroll
           for (int q = 0; q < E; q++) {
-            const int e = lane * E + q;
+            const int jx = lj0 + q;
             float kq = __builtin_huge_valf();
-            if (e < n) {
-              kq = rows[ii] + k2[jj] * a2;
+            if (lrow && jx < NG) {
+              kq = K + k2[jx] * a2;
               kq = (kq > 0.0f) ? kq : 0.0f;  // -0 and 0 order alike
             }
-            keyE[e] = kq;
-            pk[q] = (__float_as_uint(kq) & ~IM) | (unsigned)e;
-            if (++jj == NG) {
-              jj = 0;
-              ++ii;
-            }
+            keyE[lane * E + q] = kq;
+            pk[q] = (__float_as_uint(kq) & ~IM) | (unsigned)(lane * E + q);
           }
-        }
-        bitonic_sort_u32<E>(pk, lane);
-        __syncthreads();
+          if (k2up) bitonic_sort_u32_lanesorted<E>(pk, lane);
+          else bitonic_sort_u32<E>(pk, lane);
+          wave_sync();
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-          idx[q] = (int)(pk[q] & IM);
-          key[q] = keyE[idx[q]];
-        }
-        exact_fixup<E>(key, idx, lane);
-      } else {
-        int ii = i00, jj = j00;
+          for (int q = 0; q < E; q++) {
+            idx[q] = (int)(pk[q] & IM);
+            key[q] = keyE[idx[q]];
+          }
+          exact_fixup<E>(key, idx, lane);
+        } else {
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-          const int e = lane * E + q;
-          if (e < n) {
-            key[q] = rows[ii] + k2[jj] * a2;
-          } else {
-            key[q] = dinf<T>();
+          for (int q = 0; q < E; q++) {
+            const int jx = lj0 + q;
+            key[q] = (lrow && jx < NG) ? K + k2[jx] * a2 : dinf<T>();
+            idx[q] = lane * E + q;
           }
-          idx[q] = e;
-          if (++jj == NG) {
-            jj = 0;
-            ++ii;
-          }
+          Sorter<T, E>::sort(key, idx, lane);
         }
-        Sorter<T, E>::sort(key, idx, lane);
       }
-#pragma unroll
-      for (int q = 0; q < E; q++) {
-        const int i = (int)(((float)idx[q] + 0.5f) * rng);
-        ij[q] = (i << 8) | (idx[q] - i * NG);
-      }
+      wave_sync();  // the exact keys are read: the bin records take their place
+      // per-bin records (rankg.f:129-152 closes a bin at its straddling
+      // element, or at the end): recS[b][vq] the FP64 prefix of quantity vq
+      // through the closing slot, recC[b][vq] the closing slot's share of
+      // it that belongs to the next bin; recG[b] / recW[b] the same for the
+      // weight; rtype[b] 1 closed by a straddle, 2 open at the end, 0 never
+      // reached
+      const int NQ = ngas + 2;
+      double *recS = reinterpret_cast<double *>(uni);
+      double *recC = recS + NG * NQ;
+      double *recG = sGd, *recW = sSt;
+      int *rtype = sE;
 
-      // weights and their FP64 cumulative sum gdist (rankg.f:104-108).  The
-      // sum is taken over FP64 products: where a run of elements exactly
-      // fills a bin, FP32 products would spill a ~1e-8 share of the next
-      // (possibly far larger) key across the edge.
+      // (i, j) of each slot's element, packed (i << 8) | j; padding gets
+      // (NG << 8)
+      int ij[E];
       T wq[E];
       double gd[E];
       double run = 0.0;
 #pragma unroll
       for (int q = 0; q < E; q++) {
-        const bool real = ij[q] < (NG << 8);
-        const int i = ij[q] >> 8, jj = ij[q] & 255;
-        wq[q] = real ? dlg[i] * dlg[jj] : (T)0;
-        run += real ? dl64[i] * dl64[jj] : 0.0;
+        int i, jx;
+        slot_ij(idx[q], i, jx);
+        const bool real = i < NG && jx < NG;
+        ij[q] = real ? (i << 8) | jx : (NG << 8);
+        // weights and their FP64 cumulative sum gdist (rankg.f:104-108).  The
+        // sum is taken over FP64 products: where a run of elements exactly
+        // fills a bin, FP32 products would spill a ~1e-8 share of the next
+        // (possibly far larger) key across the edge.
+        const double w = w64[idx[q]];
+        wq[q] = (T)w;
+        run += w;
         gd[q] = run;
       }
-      const double s = wave_incl_sum(run);
-      const double ex = wave_shr1(s);
+      {
+        const double s = wave_incl_sum(run);
+        const double ex = wave_shr1(s);
 #pragma unroll
-      for (int q = 0; q < E; q++) gd[q] += ex;
-      // cumulative weight before slot q: gd[q-1], or the previous lane's
-      // last gd for q = 0
+        for (int q = 0; q < E; q++) gd[q] += ex;
+      }
+      // cumulative weight before slot 0: the previous lane's last gd
       const double gprev = wave_shr1(gd[E - 1]);
 
-      // bin of each element = number of edges G[1..NG] <= its start; an
-      // element straddles when its end reaches the next edge (rankg.f:120)
-      // ij[q] |= bin << 16 | straddle flag << 24; omf: 1 - frac of a
-      // straddling element
-      T omf[E];
+      // bin of each element = number of edges G[1..NG] <= its start; it
+      // straddles when its end reaches the next edge (rankg.f:120).  The
+      // lane's first bin b0 comes from a search; the next four edges are
+      // held in registers and the bin of each slot is b0 plus the number of
+      // them at or below its start.  A lane whose slots pass all four takes
+      // the edge-by-edge walk instead.
+      int b0;
       {
-        int lo = 0, hi = NG;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (G[mid] <= gprev) lo = mid;
-          else hi = mid - 1;
+        int lo2 = 0, hi2 = NG;
+        while (lo2 < hi2) {
+          const int mid = (lo2 + hi2 + 1) >> 1;
+          if (G[mid] <= gprev) lo2 = mid;
+          else hi2 = mid - 1;
         }
-        int b = lo;
+        b0 = lo2;
+      }
+      constexpr int LA = 4;
+      double ge[LA];
+#pragma unroll
+      for (int t = 0; t < LA; t++) ge[t] = (b0 + 1 + t <= NG) ? G[b0 + 1 + t] : dinf<double>();
+      auto edges_le = [&](double x) {
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < LA; t++) c += (ge[t] <= x) ? 1 : 0;
+        return c;
+      };
+      if (!__any(edges_le(gd[E - 1]) >= LA)) {
+        int cs = 0;
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int ce = edges_le(gd[q]);
+          const int b = b0 + cs;
+          ij[q] |= (b << 16) | ((ce > cs && b < NG) ? (1 << 24) : 0);
+          cs = ce;
+        }
+      } else {
+        int b = b0;
 #pragma unroll
         for (int q = 0; q < E; q++) {
           const double st = (q == 0) ? gprev : gd[q - 1];
           if (q > 0)
             while (b < NG && G[b + 1] <= st) b++;
-          const bool stq = (b < NG) && !(gd[q] < G[b + 1]);
-          ij[q] |= (b << 16) | (stq ? (1 << 24) : 0);
-          // 1 - frac of rankg.f:121, formed as a ratio of two FP64 differences
-          omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - st)) : (T)0;
+          ij[q] |= (b << 16) | (((b < NG) && !(gd[q] < G[b + 1])) ? (1 << 24) : 0);
         }
+      }
+      // omf: 1 - frac of rankg.f:121 for a straddling element, formed as a
+      // ratio of two FP64 differences
+      T omf[E];
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int b = (ij[q] >> 16) & 255;
+        const bool stq = (ij[q] >> 24) != 0;
+        const double st = (q == 0) ? gprev : gd[q - 1];
+        omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - st)) : (T)0;
       }
       // bin records: a straddling element closes its bin; the last slot
       // closes the bin it lies in (or, if it straddles, opens the next one)
@@ -1217,10 +1318,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
         }
       };
       const int nq = r + 3;
-      if constexpr (sizeof(T) == 4)
-        scan_quantity(0, [&](int q) { return keyE[((ij[q] >> 8) & 255) * NG + (ij[q] & 255)]; });
-      else
-        scan_quantity(0, [&](int q) { return key[q]; });
+      scan_quantity(0, [&](int q) { return rows[(ij[q] >> 8) & 255] + k2[ij[q] & 255] * a2; });
       for (int vq = 1; vq < nq; vq++) {
         const int s2 = vq - 1;
         if (s2 < r) {
@@ -1233,7 +1331,7 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
           scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255] + d2[ij[q] & 255] * a2; });
         }
       }
-      __syncthreads();
+      wave_sync();
       // bin b = its records minus the previous bin's, plus the share carried
       // over from the element straddling its lower edge; normalised by the
       // accumulated weight when a straddle closed it, or when it is the last
@@ -1262,43 +1360,35 @@ __global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs
         rtype[b] = 0;
       }
     }
-    __syncthreads();
+    wave_sync();
     T *t = rows;
     rows = nxt;
     nxt = t;
   }
 
-  // ---- write the layer's k-distribution and derivatives ------------------
-  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas, R.nparam);
-  T *ktot = inter;                                  // [NG][L]
-  T *dkq = inter + (long long)NG * L;               // [NG][ngas+1][L]
-  for (int i = lane; i < NG; i += W) {
-    ktot[(long long)i * L + j] = rows[i];
-    for (int s = 0; s <= ngas; s++)
-      dkq[((long long)i * (ngas + 1) + s) * L + j] = rows[(1 + s) * NG + i];
+  // ---- the layer's k-distribution and derivatives ------------------------
+  const int nks = R.nks;
+  for (int x = lane; x < NS * NG; x += W) {
+    const int sl = (int)(((float)x + 0.5f) * rng), i = x - sl * NG;
+    const T val = rows[x];
+    const int ks = (sl == 0) ? -2 : (sl == NS - 1) ? nks : (int)R.kslot[sl - 1];
+    if (ks == -2) ok[(long long)i * ostride + j] = val;
+    else if (ks >= 0) od[((long long)i * (nks + 1) + ks) * ostride + j] = val;
   }
+  wave_sync();  // the scratch is free for the next item
 }
 
-// ============================================================= CIA kernel
-// Continuum (dust, gas continua of ngascon.f, CIA of nciacon.f/ciaread.f,
-// Rayleigh) for one (request, wavenumber, layer) per thread, FP64, written to
-// the tcon/dtcon part of the intermediate block.  Scalar work per layer, so
-// it runs one item per lane rather than inside the per-layer wavefront.
+// Continuum of one (request, wavenumber iw, layer j) item, FP64: dust, the
+// gas continua of ngascon.f, the CIA of nciacon.f/ciaread.f and Rayleigh.
+// tcon[j] gets the optical depth, dtcon[s * cstride + j] its derivative for
+// parameter s (gas slots per 1e20 molecules, then T, then the
+// continuum-only parameters).
 template <typename T>
-__global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
-  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (item >= a.nA) return;
-  const int rq = find_req(a.startA, a.nreq, item);
-  const DevReq &R = a.reqs[rq];
-  const int L = R.L, ngas = R.ngas, NG = a.ng;
-  const long long loc = item - R.itemA0;
-  const int iw = (int)(loc / L);
-  const int j = (int)(loc - (long long)iw * L);
+__device__ __forceinline__ void continuum_layer(const KernelArgs &a, const DevReq &R, int iw, int j,
+                                                T *tcon, T *dtcon, long long cstride) {
+  const int ngas = R.ngas;
   const LayerConst lc = a.lay[R.lay0 + j];
   const double vwave = a.vwave[R.wave0 + iw];
-  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, ngas, R.nparam);
-  T *tcon = inter + (long long)NG * L + (long long)NG * (ngas + 1) * L;
-  T *dtcon = tcon + L;
   double absorb, dab[7];
   const double vv = (R.ispace == 1) ? 1e4 / vwave : vwave;
   // dust (cirsradg_wave.f:550-577), first as in the reference: extinction at
@@ -1342,14 +1432,54 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
       if (R.iabs[l] == s) d += dab[l] / lc.utotl;
     if (s == ngas) d += dab[5];
     else d += dray + (R.gcw0 >= 0 ? gcd[s] : 0.0);
-    dtcon[(long long)s * L + j] = (T)d;
+    dtcon[(long long)s * cstride + j] = (T)d;
   }
   // the continuum-only parameters: each dust type's extinction
   // (cirsradg_wave.f:573), then the para-H2 fraction (:659-661)
   for (int c = 0; c < R.ncont; c++)
-    dtcon[(long long)(ngas + 1 + c) * L + j] = (T)a.dext[R.dx0 + (long long)iw * R.ncont + c];
-  if (R.flagh2p) dtcon[(long long)(R.nparam - 1) * L + j] = (T)dab[6];
+    dtcon[(long long)(ngas + 1 + c) * cstride + j] = (T)a.dext[R.dx0 + (long long)iw * R.ncont + c];
+  if (R.flagh2p) dtcon[(long long)(R.nparam - 1) * cstride + j] = (T)dab[6];
   tcon[j] = (T)absorb;
+}
+}  // namespace
+
+// The layer pass as its own launch: one wavefront per (request,
+// wavenumber, layer) item, results to the intermediate block in HBM.
+template <typename T, int E, int BF>
+__global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs a, long long nblocks) {
+  const long long item = xcd_item(nblocks);
+  if (item >= a.nA) return;
+  const int lane = threadIdx.x;
+  const int rqc = find_req(a.startA, a.nreqA, item);
+  const DevReq &R = a.reqs[a.reqmapA[rqc]];
+  const int L = R.L, NG = a.ng;
+  const long long loc = item - a.startA[rqc];
+  const int iw = (int)(loc / L);
+  const int j = (int)(loc - (long long)iw * L);
+  extern __shared__ double smem_d[];
+  char *hdr = reinterpret_cast<char *>(smem_d);
+  layer_header<T, E>(a, hdr, lane, W);
+  wave_sync();
+  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
+  layer_item<T, E, BF>(a, R, iw, j, hdr, hdr + layer_header_bytes(NG, E, (int)sizeof(T)), inter,
+                       inter + (long long)NG * L, L, lane);
+}
+
+// The continuum as its own launch: one (request, wavenumber, layer) item
+// per thread, into the tcon/dtcon part of the intermediate block.
+template <typename T>
+__global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= a.nA) return;
+  const int rqc = find_req(a.startA, a.nreqA, item);
+  const DevReq &R = a.reqs[a.reqmapA[rqc]];
+  const int L = R.L, ngas = R.ngas, NG = a.ng;
+  const long long loc = item - a.startA[rqc];
+  const int iw = (int)(loc / L);
+  const int j = (int)(loc - (long long)iw * L);
+  T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
+  T *tcon = inter + (long long)NG * L + (long long)NG * (R.nks + 1) * L;
+  continuum_layer<T>(a, R, iw, j, tcon, tcon + L, L);
 }
 
 // ========================================================= emission kernel
@@ -1439,9 +1569,10 @@ __device__ __forceinline__ double wave_sum(double x) {
   for (int ds = 5; ds >= 0; ds--) x += __shfl_xor(x, 1 << ds);
   return x;
 }
-}  // namespace
 
-// LDS: acc[nlays][nsw] and comp[nlays][nsw] in T.  EXT: the form with the
+// One (request, wavenumber iw, path p) item on one wavefront, reading the
+// layer results ktot [NG][L], dkq [NG][ngas+1][L], tcon [L] and
+// dtcon [nparam][L], with acc and cmp [RL*64][nsw] of scratch.  EXT: the form with the
 // imod 8, 13, 14 and 18 branches and the stellar-spectrum factors, launched
 // only for classes that hold such a path or request (their registers would
 // cost the other forms occupancy).
@@ -1449,17 +1580,11 @@ __device__ __forceinline__ double wave_sum(double x) {
 // atmosphere path again, every g-ordinate's terms times the cell's
 // transmission at that g, and writes the product into its own slots.
 template <typename T, int RL, bool EXT>
-__global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
-  const long long item = xcd_item(nblocks);
-  if (item >= a.nB) return;
-  const int lane = threadIdx.x;
-  const int rqc = find_req(a.startB, a.nreqB, item);
-  const DevReq &R = a.reqs[a.reqmapB[rqc]];
+__device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq &R, int iw, int p,
+                                              const T *ktot, const T *dkq, const T *tcon,
+                                              const T *dtcon, T *acc, T *cmp, int lane) {
   const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath, nsw = R.nsw,
-            nparam = R.nparam;
-  const long long loc = item - a.startB[rqc];
-  const int iw = (int)(loc / npath);
-  const int p = (int)(loc - (long long)iw * npath);
+            nparam = R.nparam, nks = R.nks;
   const PathInfo pi0 = a.paths[R.path0 + p];
   const bool scr = EXT && pi0.imod == 8;
   const int pg = scr ? pi0.atm : p;  // the path whose layers are computed
@@ -1467,18 +1592,8 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const int nlays = pi.nlays;
   const int nrows = pi0.nlays;  // Jacobian rows written (imod 8: nlayin(1) <= nlays)
   const double v = a.vwave[R.wave0 + iw];
-  const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
-                   (long long)iw * inter_block(NG, L, ngas, R.nparam);
-  const T *ktot = inter;
-  const T *dkq = inter + (long long)NG * L;
-  const T *tcon = dkq + (long long)NG * (ngas + 1) * L;
-  const T *dtcon = tcon + L;
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
                                    : reinterpret_cast<const T *>(a.delg32);
-
-  extern __shared__ double smem_d[];
-  T *acc = reinterpret_cast<T *>(smem_d);  // [RL*64][nsw]
-  T *cmp = acc + (long long)RL * W * nsw;
 
   int Lm[RL];
   bool act[RL];
@@ -1771,12 +1886,12 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       for (int k1 = 0; k1 < nsw; k1++) {
         const int k = R.isw[k1];
         // continuum rows for every parameter; the overlap's dk/dq only for the
-        // gases and T (dust and the para-H2 fraction act on the continuum
-        // only).  Branch-free (the row index clamped, the value selected), so
-        // the loop unrolls as it does without those parameters.
-        const int kd = k < ngas ? k : ngas;
-        const T dkv = dkq[((long long)g * (ngas + 1) + kd) * L + Lm[q]];
-        const T dq = (dtcon[(long long)k * L + Lm[q]] + (k <= ngas ? dkv : (T)0)) * sc[q];
+        // gases with a k-table and T (dust and the para-H2 fraction act on
+        // the continuum only).  Branch-free (the row index clamped, the value
+        // selected), so the loop unrolls as it does without those parameters.
+        const int kd = R.kdrow[k1];
+        const T dkv = dkq[((long long)g * (nks + 1) + (kd >= 0 ? kd : 0)) * L + Lm[q]];
+        const T dq = (dtcon[(long long)k * L + Lm[q]] + (kd >= 0 ? dkv : (T)0)) * sc[q];
         T c = dRdtau[q] * dq;
         if (k == ngas) c = c + dT[q];
         const T x = c * dg;
@@ -1812,6 +1927,69 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
       jac[(long long)m * nparam + k] = acc[m * nsw + k1];
     }
   }
+}
+}  // namespace
+
+// Emission as its own launch: one wavefront per (request, wavenumber, path)
+// item of a layers-per-lane class, reading the intermediate block in HBM.
+// LDS: acc[RL*64][nsw] and comp[RL*64][nsw] in T.
+template <typename T, int RL, bool EXT>
+__global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
+  const long long item = xcd_item(nblocks);
+  if (item >= a.nB) return;
+  const int lane = threadIdx.x;
+  const int rqc = find_req(a.startB, a.nreqB, item);
+  const DevReq &R = a.reqs[a.reqmapB[rqc]];
+  const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath;
+  const long long loc = item - a.startB[rqc];
+  const int iw = (int)(loc / npath);
+  const int p = (int)(loc - (long long)iw * npath);
+  const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
+                   (long long)iw * inter_block(NG, L, R.nks, R.nparam);
+  const T *dkq = inter + (long long)NG * L;
+  const T *tcon = dkq + (long long)NG * (R.nks + 1) * L;
+  extern __shared__ double smem_d[];
+  T *acc = reinterpret_cast<T *>(smem_d);
+  emission_path<T, RL, EXT>(a, R, iw, p, inter, dkq, tcon, tcon + L, acc,
+                            acc + (long long)RL * W * R.nsw, lane);
+}
+
+// The whole wavenumber in one workgroup of RADG_FUSED_WAVES wavefronts:
+// the continuum of every layer (one layer per thread), the layer items
+// (round robin over the wavefronts), then the paths (round robin), with the
+// layer results kept in LDS: ktot[NG][L] | dk[NG][ngas+1][L] | tcon[L] |
+// dtcon[nparam][L] in T after the header, then one scratch area per
+// wavefront, shared in time by its layer items and its paths.  Items are
+// (request of the class, wavenumber): startB/reqmapB/nreqB/nB describe them.
+template <typename T, int E, int BF, int RL, bool EXT>
+__global__ __launch_bounds__(64 * RADG_FUSED_WAVES) void fused_kernel(KernelArgs a, long long nblocks) {
+  const long long item = xcd_item(nblocks);
+  if (item >= a.nB) return;
+  const int tid = threadIdx.x;
+  const int wv = tid / W, lane = tid - wv * W;
+  const int rqc = find_req(a.startB, a.nreqB, item);
+  const DevReq &R = a.reqs[a.reqmapB[rqc]];
+  const int L = R.L, ngas = R.ngas, NG = a.ng;
+  const int iw = (int)(item - a.startB[rqc]);
+  extern __shared__ double smem_d[];
+  char *hdr = reinterpret_cast<char *>(smem_d);
+  const int ts = (int)sizeof(T);
+  T *ktot = reinterpret_cast<T *>(hdr + layer_header_bytes(NG, E, ts));
+  T *dkq = ktot + (long long)NG * L;
+  T *tcon = dkq + (long long)NG * (R.nks + 1) * L;
+  T *dtcon = tcon + L;
+  char *ws = reinterpret_cast<char *>(ktot) +
+             lds_round16(ts * (int)inter_block(NG, L, R.nks, R.nparam)) + wv * a.wsbytes;
+  layer_header<T, E>(a, hdr, tid, W * RADG_FUSED_WAVES);
+  for (int j = tid; j < L; j += W * RADG_FUSED_WAVES) continuum_layer<T>(a, R, iw, j, tcon, dtcon, L);
+  __syncthreads();
+  for (int j = wv; j < L; j += RADG_FUSED_WAVES)
+    layer_item<T, E, BF>(a, R, iw, j, hdr, ws, ktot, dkq, L, lane);
+  __syncthreads();
+  T *acc = reinterpret_cast<T *>(ws);
+  for (int p = wv; p < R.npath; p += RADG_FUSED_WAVES)
+    emission_path<T, RL, EXT>(a, R, iw, p, ktot, dkq, tcon, dtcon, acc,
+                              acc + (long long)RL * W * R.nsw, lane);
 }
 
 // ======================================================= path quadrature
@@ -1866,17 +2044,21 @@ hipError_t launch_em_RL(const KernelArgs &a, bool disc, size_t lds, hipStream_t 
     hipLaunchKernelGGL((emission_kernel<T, RL, false>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
   return hipGetLastError();
 }
+template <typename T, int E, int BF>
+hipError_t launch_fused_E(const KernelArgs &a, int RL, size_t lds, hipStream_t st) {
+  long long nb = ((a.nB + 7) / 8) * 8;
+  if (nb == 0) return hipSuccess;
+  const dim3 blk(64 * RADG_FUSED_WAVES);
+  if (RL <= 2) hipLaunchKernelGGL((fused_kernel<T, E, BF, 2, true>), dim3((unsigned)nb), blk, lds, st, a, nb);
+  else if (RL == 4) hipLaunchKernelGGL((fused_kernel<T, E, BF, 4, true>), dim3((unsigned)nb), blk, lds, st, a, nb);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
 }  // namespace
 
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
-  const size_t ts = fp64 ? 8 : 4;
-  const size_t nq = ngas + 2, ns = ngas + 2;
-  size_t b = 8 * ((size_t)(ng + 1) + 2 * (size_t)ng * nq + 3 * (size_t)ng);
-  b += ts * ((size_t)2 * ngas * ng + (size_t)2 * ns * ng + ng);
-  b += 4 * (size_t)ng;
-  b += 8 + 3 * (size_t)W * ts;  // rankk merge buffers after 8-byte alignment
-  if (!fp64) b += 4 * (size_t)W * E;
-  return b;
+  const int ts = fp64 ? 8 : 4;
+  return (size_t)layer_header_bytes(ng, E, ts) + (size_t)layer_layout(ng, ngas, E, ts).total;
 }
 
 size_t radg_lds_emission_bytes(int RL, int nsw, int fp64) {
@@ -1884,10 +2066,30 @@ size_t radg_lds_emission_bytes(int RL, int nsw, int fp64) {
   return (size_t)2 * RL * W * nsw * ts + 16;
 }
 
+// The fused kernel's LDS for requests of up to maxL layers, maxgas gases,
+// maxnsw Jacobian parameters and maxblock intermediate elements per
+// wavenumber (inter_block), with emission's RL layers per lane; *ws gets the
+// scratch bytes per wavefront.  0: no fused form for this (mode, E, RL).
+size_t radg_lds_fused_bytes(int mode, int ng, int E, int RL, int maxgas, int maxnsw,
+                            long long maxblock, int *ws) {
+  const bool have = (mode == 2) ? (E == 8) : (E == 1 || E == 2 || E == 4 || E == 8);
+  if (!have || RL > 4) return 0;
+  const int ts = mode == 0 ? 8 : 4;
+  const int rl = RL <= 2 ? 2 : 4;
+  int w = layer_layout(ng, maxgas, E, ts).total;
+  const int em = lds_round16(2 * rl * W * maxnsw * ts);
+  if (w < em) w = em;
+  *ws = w;
+  return (size_t)layer_header_bytes(ng, E, ts) + (size_t)lds_round16(ts * (int)maxblock) +
+         (size_t)RADG_FUSED_WAVES * w;
+}
+
+// Slots per lane of an overlap sort: 64 E >= ng^2, with the rows of the
+// slot layout (layer_lpr) fitting the 64 lanes.
 int radg_layer_E(int ng) {
   const int n = ng * ng;
   int E = 1;
-  while (W * E < n) E <<= 1;
+  while (W * E < n || ng * layer_lpr(ng, E) > W) E <<= 1;
   return E;
 }
 
@@ -1929,6 +2131,28 @@ hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t ld
       case 4: return launch_layer_E<float, 4>(a, lds, st);
       case 8: return launch_layer_E<float, 8>(a, lds, st);
       case 16: return launch_layer_E<float, 16>(a, lds, st);
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+// The fused kernel over the wavenumber items a.startB/a.nB of one class.
+hipError_t radg_launch_fused(const KernelArgs &a, int mode, int E, int RL, size_t lds, hipStream_t st) {
+  if (mode == 0) {
+    switch (E) {
+      case 1: return launch_fused_E<double, 1, 0>(a, RL, lds, st);
+      case 2: return launch_fused_E<double, 2, 0>(a, RL, lds, st);
+      case 4: return launch_fused_E<double, 4, 0>(a, RL, lds, st);
+      case 8: return launch_fused_E<double, 8, 0>(a, RL, lds, st);
+    }
+  } else if (mode == 2) {
+    if (E == 8) return launch_fused_E<float, 8, 1>(a, RL, lds, st);
+  } else {
+    switch (E) {
+      case 1: return launch_fused_E<float, 1, 0>(a, RL, lds, st);
+      case 2: return launch_fused_E<float, 2, 0>(a, RL, lds, st);
+      case 4: return launch_fused_E<float, 4, 0>(a, RL, lds, st);
+      case 8: return launch_fused_E<float, 8, 0>(a, RL, lds, st);
     }
   }
   return hipErrorInvalidValue;

@@ -159,6 +159,14 @@ def config5_run(rad, n, cia, nthreads):
     rad_e = float(np.max(np.abs(wo - wo_ref) / np.abs(wo_ref)))
     sc = np.max(np.abs(wj_ref), axis=1, keepdims=True)
     jac_e = float(np.max(np.abs(wj - wj_ref) / np.where(sc > 0, sc, 1.0)))
+    # the same request with the IEEE-half codes
+    bh = rad.Batch([reqs[0]], mode=rad.MODE_F16)
+    bh.launch()
+    rh = bh.read(0)
+    bh.close()
+    wo_h, wj_h = rh["wout"][sample], rh["wjac"][sample]
+    rad_h = float(np.max(np.abs(wo_h - wo_ref) / np.abs(wo_ref)))
+    jac_h = float(np.max(np.abs(wj_h - wj_ref) / np.where(sc > 0, sc, 1.0)))
     lat_ms = np.array(lat) * 1e3
     del tabs
     return {
@@ -173,6 +181,9 @@ def config5_run(rad, n, cia, nthreads):
         "bf16_storage_error_vs_fp64_oracle": {
             "radiance_rel": rad_e, "jacobian_col_scaled": jac_e,
             "sample": f"{len(sample)} wavenumbers of request 0, angle-weighted"},
+        "f16_storage_error_vs_fp64_oracle": {
+            "radiance_rel": rad_h, "jacobian_col_scaled": jac_h,
+            "sample": f"{len(sample)} wavenumbers of request 0 with IEEE-half codes, angle-weighted"},
         "shape": "200 layers x 10000 wavenumbers (0.1 cm-1), 8 k-gases + H2/He, 30 P x 15 T "
                  "x 20 g, BF16 log10(k) tables (1.44 GB), 5 angles, weighted outputs",
     }
@@ -267,7 +278,7 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=100.0,
                     help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--bf16-steps", type=int, default=3)
-    ap.add_argument("--config5-requests", type=int, default=64,
+    ap.add_argument("--config5-requests", type=int, default=1024,
                     help="requests in the large-spectrum BF16 stress (0: skip it)")
     ap.add_argument("--retrievals", type=int, default=512,
                     help="retrievals in the batched optimal-estimation run (0: skip it)")
@@ -324,16 +335,27 @@ def main():
         b64.close()
         value64 = args.batch * args.fp64_steps / wall64
 
-    # ---- BF16 log10(k) table form, same batch
-    b16, wall16, lat16, kern16 = time_batch(rad, reqs, rad.MODE_BF16, args.bf16_steps, 1, torch)
-    info16 = b16.info()
-    res16 = b16.read(0, weighted=False)
-    b16.close()
-    value16 = args.batch * args.bf16_steps / wall16
-    o16, d16 = res16["output"][sample], res16["doutputdq"][sample]
-    rad_e16 = float(np.max(np.abs(o16 - o_ref) / np.abs(o_ref)))
-    jac_e16 = float(np.max(np.abs(d16 - d_ref) / np.where(sc > 0, sc, 1.0)))
-    alg16 = info16["table_bytes"] + info16["input_bytes"]
+    # ---- the 16-bit log10(k) table forms (BF16 and IEEE-half codes), same batch
+    forms16 = {}
+    for name, mode16 in (("bf16", rad.MODE_BF16), ("f16", rad.MODE_F16)):
+        steps16 = max(1, args.bf16_steps)
+        b16, wall16, lat16, kern16 = time_batch(rad, reqs, mode16, steps16, 1, torch)
+        info16 = b16.info()
+        res16 = b16.read(0, weighted=False)
+        b16.close()
+        o16, d16 = res16["output"][sample], res16["doutputdq"][sample]
+        alg16 = info16["table_bytes"] + info16["input_bytes"]
+        forms16[name] = {
+            "value": args.batch * steps16 / wall16, "ms_per_step": wall16 / steps16 * 1e3,
+            "kernel_ms": {"layer": float(kern16[0]), "emission": float(kern16[1]),
+                          "quadrature": float(kern16[2])},
+            "layer_kernel_alg_GBps": alg16 / (kern16[0] / 1e3) / 1e9,
+            "error_vs_fp64_oracle": {
+                "radiance_rel": float(np.max(np.abs(o16 - o_ref) / np.abs(o_ref))),
+                "jacobian_col_scaled": float(np.max(np.abs(d16 - d_ref) / np.where(sc > 0, sc, 1.0)))},
+            "codes": "log10(k) minus an FP32 offset per (wavenumber, gas, g-ordinate), rounded to "
+                     + ("BF16" if name == "bf16" else "IEEE half"),
+        }
 
     # ---- roofline of the dominant kernel (layer/overlap kernel): it reads
     # the table nodes and the layer inputs of every spectrum in the batch
@@ -387,12 +409,8 @@ def main():
         "fp64_form": {"value": value64, "ms_per_step": wall64 / args.fp64_steps * 1e3,
                       "kernel_ms": {"layer": float(kern64[0]), "emission": float(kern64[1]),
                                     "quadrature": float(kern64[2])}},
-        "bf16_form": {"value": value16, "ms_per_step": wall16 / args.bf16_steps * 1e3,
-                      "kernel_ms": {"layer": float(kern16[0]), "emission": float(kern16[1]),
-                                    "quadrature": float(kern16[2])},
-                      "layer_kernel_alg_GBps": alg16 / (kern16[0] / 1e3) / 1e9,
-                      "error_vs_fp64_oracle": {"radiance_rel": rad_e16,
-                                               "jacobian_col_scaled": jac_e16}},
+        "bf16_form": forms16["bf16"],
+        "f16_form": forms16["f16"],
         "speedup_vs_fp64": value / value64,
         "error_vs_fp64_oracle": {"radiance_rel": rad_err, "jacobian_col_scaled": jac_err,
                                  "sample": f"{len(sample)} wavenumbers of spectrum 0"},

@@ -79,10 +79,13 @@ extern "C" {
 #define RADG_MODE_FP64 0   /* every quantity in FP64 */
 #define RADG_MODE_MIXED 1  /* FP32 interpolation/exp/overlap; FP64 Planck, gdist and
                               transmission product; compensated g-sums */
-#define RADG_MODE_BF16 2   /* the mixed form reading k-tables stored as BF16 log10(k), one
-                              offset and scale per table (codes span [-1, 1] over the
-                              table's positive range); results carry the BF16 rounding of
-                              log10(k), up to |log10(k) - offset| * 2^-8 */
+#define RADG_MODE_BF16 2   /* the mixed form reading k-tables stored as 16-bit log10(k)
+                              codes: log10(k) = code + offset, one FP32 offset per
+                              (wavenumber point, g-ordinate) of a table, the mean of log10(k)
+                              over its (P, T) values; the code is BF16, so results carry a
+                              rounding of log10(k) up to |log10(k) - offset| * 2^-8 */
+#define RADG_MODE_F16 3    /* the same with IEEE half-precision codes (11 significant
+                              bits): rounding up to |log10(k) - offset| * 2^-11 */
 
 int radg_set_mode(int mode);
 int radg_get_mode(void);
@@ -114,13 +117,14 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas,
  * length in bytes (4 for the REAL(4) build, 8 for REAL(8)). */
 int radg_read_klist(const char *klist_path, int recl);
 
-/* Copy the BF16 form of registered table itab (0-based) to the host: codes
- * -> npoint*np*nt*ng values in the order of radg_register_ktables, and
- * off_sc[2] = {offset, scale} with log10(k) = code * scale + offset (k <= 0
- * is code 0xFF80).  The form is made on the device on first use.  No
- * counterpart in the reference (its tables are REAL(4)/REAL(8) only): an
- * inspection hook for callers that check or archive the encoded tables. */
-int radg_ktable_bf16(int itab, unsigned short *codes, double *off_sc);
+/* Copy the 16-bit form (mode RADG_MODE_BF16 or RADG_MODE_F16) of registered
+ * table itab (0-based) to the host: codes -> npoint*np*nt*ng values in the
+ * order of radg_register_ktables, offsets -> npoint*ng FP32 values, with
+ * log10(k) = code + offsets[point][g] (k <= 0: code 0xFF80 in BF16, 0xFC00 in
+ * half).  The form is made on the device on first use.  No counterpart in
+ * the reference (its tables are REAL(4)/REAL(8) only): an inspection hook
+ * for callers that check or archive the encoded tables. */
+int radg_ktable_codes(int mode, int itab, unsigned short *codes, float *offsets);
 
 /* /CIATABLE/: temps[25] (K), kcia[1501][25][9] in C order, which is the
  * Fortran kcia(9,25,1501) (ciacom.f:9); wavenumber grid (i-1)*dnu. */

@@ -18,6 +18,7 @@ RADG_ETIMEOUT = 9
 MODE_FP64 = 0
 MODE_MIXED = 1
 MODE_BF16 = 2
+MODE_F16 = 3
 
 # Every symbol the C header (include/radg.h) declares.
 EXPORTS = [
@@ -31,7 +32,7 @@ EXPORTS = [
     "radg_batch_launch", "radg_batch_sync", "radg_batch_read", "radg_batch_timings",
     "radg_batch_info", "radg_batch_destroy", "radg_batch_timing_stats",
     "radg_server_create", "radg_server_submit", "radg_server_cancel", "radg_server_wait",
-    "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch", "radg_ktable_bf16",
+    "radg_server_stats", "radg_server_destroy", "radg_batch_set_fetch", "radg_ktable_codes",
     "radg_batch_set_statemaps", "radg_batch_read_mapped", "radg_batch_device_outputs",
     "radg_oe_create", "radg_oe_cost", "radg_oe_propose", "radg_oe_destroy",
     "radg_spectrum_noalloc", "radg_cirsconv", "radg_read_cia",
@@ -83,9 +84,9 @@ def lib():
                  "radg_batch_timings", "radg_batch_info", "radg_batch_destroy",
                  "radg_batch_timing_stats", "radg_server_create", "radg_server_submit",
                  "radg_server_cancel", "radg_server_wait", "radg_server_stats",
-                 "radg_server_destroy", "radg_ktable_bf16"]:
+                 "radg_server_destroy", "radg_ktable_codes"]:
         getattr(L, name).restype = ip
-    L.radg_ktable_bf16.argtypes = [ip, vp, vp]
+    L.radg_ktable_codes.argtypes = [ip, ip, vp, vp]
     L.radg_register_cia.argtypes = [C.c_double, vp, vp]
     L.radg_register_dust.argtypes = [ip, ip, vp, vp]
     L.radg_set_nlte.argtypes = [ip, C.c_double, C.c_double]

@@ -27,7 +27,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._native import MODE_BF16, MODE_FP64, MODE_MIXED, RadgError, Request, check, lib  # noqa: F401
+from ._native import MODE_BF16, MODE_F16, MODE_FP64, MODE_MIXED, RadgError, Request, check, lib  # noqa: F401
 
 MAXLAY, MAXPAT, MAXGAS, MAXCON, MAXSEC = 400, 200, 30, 10, 16000
 NPARMAX = MAXGAS + 2 + MAXCON
@@ -75,13 +75,14 @@ def register_ktables(tables) -> None:
         _vp(gord), _vp(delg), ptrs, vptrs))
 
 
-def ktable_bf16(itab: int, shape):
-    """BF16 codes of registered table ``itab`` as made on the device, with
-    the table's (offset, scale): log10(k) = code * scale + offset."""
+def ktable_codes(mode: int, itab: int, shape):
+    """16-bit codes (``MODE_BF16`` or ``MODE_F16``) of registered table
+    ``itab`` as made on the device, and its offsets [npoint, ng]:
+    log10(k) = code + offset[point, g]."""
     codes = np.empty(shape, dtype=np.uint16)
-    off_sc = np.zeros(2, dtype=np.float64)
-    check(lib().radg_ktable_bf16(C.c_int(itab), _vp(codes), _vp(off_sc)))
-    return codes, float(off_sc[0]), float(off_sc[1])
+    offs = np.zeros((shape[0], shape[-1]), dtype=np.float32)
+    check(lib().radg_ktable_codes(C.c_int(mode), C.c_int(itab), _vp(codes), _vp(offs)))
+    return codes, offs
 
 
 def read_klist(path: str, recl: int = 4) -> None:

@@ -30,8 +30,10 @@ size_t radg_lds_fused_bytes(int mode, int ng, int E, int RL, int maxgas, int max
                             int *ws);
 hipError_t radg_launch_fused(const KernelArgs &a, int mode, int E, int RL, size_t lds, hipStream_t st);
 hipError_t radg_convert_f32(const double *src, float *dst, long long n, hipStream_t st);
-hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n, double off,
-                             double sc, hipStream_t st);
+hipError_t radg_slice_offsets(const double *src, float *off, long long nslice, int ntab, int ng,
+                              hipStream_t st);
+hipError_t radg_convert_code16(int bf, const double *src, const float *off, unsigned short *dst,
+                               long long n, int ntab, int ng, hipStream_t st);
 hipError_t radg_launch_emission(const KernelArgs &a, int fp64, int RL, bool disc, size_t lds,
                                 hipStream_t st);
 hipError_t radg_launch_quadrature(const KernelArgs &a, int fp64, hipStream_t st);
@@ -80,7 +82,6 @@ struct Table {
   std::vector<double> vcen;   // irregular grid (delv <= 0, read_khead.f:93-100)
   long long off;              // element offset in the pool
   std::vector<uint8_t> zero;  // per slice: every value <= 0 (get_kg.f:224-250)
-  double bf_off = 0, bf_sc = 1;  // BF16 form: log10(k) = code * bf_sc + bf_off
 };
 
 struct TableSet {
@@ -92,6 +93,8 @@ struct TableSet {
   double *d_k64 = nullptr;
   float *d_k32 = nullptr;            // made on first use by the mixed form
   unsigned short *d_k16 = nullptr;   // made on first use by the BF16 form
+  unsigned short *d_kh = nullptr;    // made on first use by the half form
+  float *d_koff = nullptr;           // per (slice, ordinate) offsets of both 16-bit forms
   std::mutex form_mu;
   double *d_G = nullptr, *d_delg64 = nullptr;
   float *d_delg32 = nullptr;
@@ -99,6 +102,8 @@ struct TableSet {
     if (d_k64) (void)hipFree(d_k64);
     if (d_k32) (void)hipFree(d_k32);
     if (d_k16) (void)hipFree(d_k16);
+    if (d_kh) (void)hipFree(d_kh);
+    if (d_koff) (void)hipFree(d_koff);
     if (d_G) (void)hipFree(d_G);
     if (d_delg64) (void)hipFree(d_delg64);
     if (d_delg32) (void)hipFree(d_delg32);
@@ -305,7 +310,8 @@ int g_inlte = 0;
 double g_nlte_k1 = 0.0, g_nlte_a = 0.0;
 
 bool valid_mode(int mode) {
-  return mode == RADG_MODE_FP64 || mode == RADG_MODE_MIXED || mode == RADG_MODE_BF16;
+  return mode == RADG_MODE_FP64 || mode == RADG_MODE_MIXED || mode == RADG_MODE_BF16 ||
+         mode == RADG_MODE_F16;
 }
 
 // Make the table form a mode reads (FP32 copy or BF16 log10 codes) from the
@@ -324,21 +330,30 @@ int ensure_form(TableSet &ts, int mode) {
     }
     ts.d_k32 = d;
   }
-  if (mode == RADG_MODE_BF16 && !ts.d_k16) {
-    unsigned short *d = nullptr;
-    HIPCHK(hipMalloc(&d, ts.total * sizeof(unsigned short)));
-    hipError_t e = hipSuccess;
-    for (size_t t = 0; t < ts.tabs.size() && e == hipSuccess; t++) {
-      const Table &tb = ts.tabs[t];
-      const long long n = (long long)tb.npoint * ts.np * ts.nt * ts.ng;
-      e = radg_convert_bf16(ts.d_k64 + tb.off, d + tb.off, n, tb.bf_off, tb.bf_sc, nullptr);
-    }
+  const int ntab = ts.np * ts.nt * ts.ng;
+  if ((mode == RADG_MODE_BF16 || mode == RADG_MODE_F16) && !ts.d_koff) {
+    float *d = nullptr;
+    HIPCHK(hipMalloc(&d, (ts.total / ntab) * ts.ng * sizeof(float)));
+    hipError_t e = radg_slice_offsets(ts.d_k64, d, ts.total / ntab, ntab, ts.ng, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
       (void)hipFree(d);
-      return fail(RADG_EHIP, "BF16 table conversion: %s", hipGetErrorString(e));
+      return fail(RADG_EHIP, "table offsets: %s", hipGetErrorString(e));
     }
-    ts.d_k16 = d;
+    ts.d_koff = d;
+  }
+  unsigned short **form = mode == RADG_MODE_BF16 ? &ts.d_k16 : mode == RADG_MODE_F16 ? &ts.d_kh : nullptr;
+  if (form && !*form) {
+    unsigned short *d = nullptr;
+    HIPCHK(hipMalloc(&d, ts.total * sizeof(unsigned short)));
+    hipError_t e = radg_convert_code16(mode == RADG_MODE_BF16 ? 1 : 2, ts.d_k64, ts.d_koff, d, ts.total,
+                                       ntab, ts.ng, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(RADG_EHIP, "16-bit table conversion: %s", hipGetErrorString(e));
+    }
+    *form = d;
   }
   return RADG_OK;
 }
@@ -731,7 +746,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
   P.cont.assign(q.cont, q.cont + (size_t)L * ncont);
   P.wg.resize((size_t)nwave * ngas);
   const long long ntab = (long long)np * nt * NG;
-  const size_t kbytes = (mode == RADG_MODE_FP64) ? 8 : (mode == RADG_MODE_BF16) ? 2 : 4;
+  const size_t kbytes = (mode == RADG_MODE_FP64) ? 8 : (mode == RADG_MODE_MIXED) ? 4 : 2;
   const size_t obytes_el = (mode == RADG_MODE_FP64) ? 8 : 4;
   double tbytes = 0;
   for (int iw = 0; iw < nwave; iw++) {
@@ -795,8 +810,7 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
       w.base2 = -1;
       w.w2 = 0.0;
       w.coinc = 1;
-      w.sc = 1.0f;
-      w.off = 0.0f;
+      w.ob = w.ob2 = 0;
       w.pad = 0;
       const Table *tb = nullptr;
       for (const Table &t : ts.tabs) {  // read_klist.f:260-290: first covering match
@@ -837,11 +851,11 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
                       (int)(tb - ts.tabs.data()) + 1);
         if (tb->zero[ipo]) continue;
         w.base = tb->off + ipo * ntab;
-        w.sc = (float)tb->bf_sc;
-        w.off = (float)tb->bf_off;
+        w.ob = w.base / ntab * NG;
         if (tb->delv == 0) {
           w.coinc = 0;
           w.base2 = (ipo + 1 < tb->npoint) ? w.base + ntab : w.base;
+          w.ob2 = w.base2 / ntab * NG;
           w.w2 = 1.0 - frack;
           tbytes += (double)nodes.size() * NG * kbytes;
         }
@@ -855,14 +869,14 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
       // get_kg.f:224-250 tests the slice at or below the wavenumber only
       if (tb->zero[n1]) continue;  // all-zero slice: k = 0
       w.base = tb->off + n1 * ntab;
-      w.sc = (float)tb->bf_sc;
-      w.off = (float)tb->bf_off;
+      w.ob = w.base / ntab * NG;
       if (!(std::fabs(v - tmp) < eps)) {
         // between table points: interpolate the two slices and merge them
         // with rankk (get_kg.f:379-440); the last point has no slice above
         // and reuses its own
         w.coinc = 0;
         w.base2 = (n1 + 1 < tb->npoint) ? w.base + ntab : w.base;
+        w.ob2 = w.base2 / ntab * NG;
         w.w2 = std::min(1.0, std::max(0.0, (v - tmp) / tb->delv));
         tbytes += (double)nodes.size() * NG * kbytes;
       }
@@ -1284,8 +1298,10 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
   a.nB = pl.nB;
   a.nC = pl.nC;
   a.ng = ts->ng;
+  a.koff = ts->d_koff;
   a.ktab = (mode == RADG_MODE_FP64)   ? (const void *)ts->d_k64
            : (mode == RADG_MODE_BF16) ? (const void *)ts->d_k16
+           : (mode == RADG_MODE_F16)  ? (const void *)ts->d_kh
                                       : (const void *)ts->d_k32;
   a.np = ts->np;
   a.nt = ts->nt;
@@ -1517,26 +1533,14 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
     }
     tb.off = total;
     tb.zero.assign(npoint[t], 1);
-    double lmin = 1e300, lmax = 0;
     for (long long s = 0; s < npoint[t]; s++) {
       const double *sl = k[t] + s * per;
       for (long long i = 0; i < per; i++) {
         if (!std::isfinite(sl[i]) || std::fabs(sl[i]) > 1e30)
           return fail(RADG_EINPUT, "table %d: value %g at wavenumber %lld is not finite or > 1e30",
                       t + 1, sl[i], s + 1);
-        if (sl[i] > 0.0) {
-          tb.zero[s] = 0;
-          lmin = std::min(lmin, sl[i]);
-          lmax = std::max(lmax, sl[i]);
-        }
+        if (sl[i] > 0.0) tb.zero[s] = 0;
       }
-    }
-    // BF16 form: codes span [-1, 1] over the table's positive range, so the
-    // rounding error of log10(k) is at most |log10(k) - bf_off| * 2^-8
-    if (lmax > 0) {
-      const double a = std::log10(lmin), z = std::log10(lmax);
-      tb.bf_off = 0.5 * (a + z);
-      tb.bf_sc = std::max(0.5 * (z - a), 1e-3);
     }
     total += (long long)npoint[t] * per;
     ts->tabs.push_back(std::move(tb));
@@ -1559,7 +1563,8 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
   return RADG_OK;
 }
 
-int radg_ktable_bf16(int itab, unsigned short *codes, double *off_sc) {
+int radg_ktable_codes(int mode, int itab, unsigned short *codes, float *offsets) {
+  if (mode != RADG_MODE_BF16 && mode != RADG_MODE_F16) return fail(RADG_EINPUT, "mode=%d is not a 16-bit form", mode);
   std::shared_ptr<TableSet> ts;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1567,13 +1572,15 @@ int radg_ktable_bf16(int itab, unsigned short *codes, double *off_sc) {
   }
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
   if (itab < 0 || itab >= (int)ts->tabs.size()) return fail(RADG_EINPUT, "itab=%d", itab);
-  int rc = ensure_form(*ts, RADG_MODE_BF16);
+  int rc = ensure_form(*ts, mode);
   if (rc) return rc;
   const Table &tb = ts->tabs[itab];
-  const long long n = (long long)tb.npoint * ts->np * ts->nt * ts->ng;
-  HIPCHK(hipMemcpy(codes, ts->d_k16 + tb.off, n * sizeof(unsigned short), hipMemcpyDeviceToHost));
-  off_sc[0] = tb.bf_off;
-  off_sc[1] = tb.bf_sc;
+  const long long ntab = (long long)ts->np * ts->nt * ts->ng;
+  const long long n = (long long)tb.npoint * ntab;
+  const unsigned short *src = mode == RADG_MODE_BF16 ? ts->d_k16 : ts->d_kh;
+  HIPCHK(hipMemcpy(codes, src + tb.off, n * sizeof(unsigned short), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(offsets, ts->d_koff + tb.off / ntab * ts->ng, (size_t)tb.npoint * ts->ng * sizeof(float),
+                   hipMemcpyDeviceToHost));
   return RADG_OK;
 }
 
@@ -2006,7 +2013,8 @@ int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, doubl
   if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
   // making a table form allocates: it must exist already
-  if ((mode == RADG_MODE_MIXED && !ts->d_k32) || (mode == RADG_MODE_BF16 && !ts->d_k16))
+  if ((mode == RADG_MODE_MIXED && !ts->d_k32) || (mode == RADG_MODE_BF16 && !ts->d_k16) ||
+      (mode == RADG_MODE_F16 && !ts->d_kh))
     return fail(RADG_EINPUT, "mode %d: its table form is made by radg_set_mode(%d) after registration", mode, mode);
   Plan pl;
   int rc = plan_batch(1, req, mode, *ts, aux, pl);

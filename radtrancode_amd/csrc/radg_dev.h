@@ -93,13 +93,16 @@ struct WaveGas {
   long long base;        // element offset of the slice in the table pool; -1: k = 0
   long long base2;       // next slice up in wavenumber (non-coincident only)
   double w2;             // weight of the upper slice, get_kg.f:381-384
+  // 16-bit forms: index of each slice's per-ordinate offsets in the offset
+  // pool (log10 k = code + offset[g])
+  long long ob, ob2;
   int coinc;             // 1: the wavenumber is on the table grid (get_kg.f:253-264)
-  float sc, off;         // BF16 storage: log10(k) = code * sc + off
   int pad;
 };
 
-// BF16 code of k <= 0 (bf16 -infinity)
+// 16-bit codes of k <= 0 (-infinity): BF16 and IEEE half
 #define RADG_BF16_NONPOS 0xFF80u
+#define RADG_F16_NONPOS 0xFC00u
 
 // nlays: the path's layers, and the Jacobian rows it writes; for an imod 8
 // path (cell x atmosphere, cirsradg_wave.f:1037-1082) nlayin(1), the rows
@@ -173,7 +176,8 @@ struct KernelArgs {
   int wsbytes;                 // fused kernel: LDS scratch bytes per wavefront
   int ng;
   // tables
-  const void *ktab;            // double* (FP64 form) or float* (mixed form)
+  const void *ktab;            // double* (FP64), float* (mixed) or 16-bit codes
+  const float *koff;           // 16-bit forms: offset per (slice, ordinate)
   int np, nt;                  // nt: temperatures per pressure (|NT|)
   int t2;                      // 1: a temperature grid per pressure (NT < 0)
   const double *G;             // g boundaries [ng+1], G[ng] forced >= 1

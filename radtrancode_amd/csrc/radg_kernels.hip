@@ -14,6 +14,7 @@
 //                    derivative with an O(L^2) recurrence), then the
 //                    g-quadrature (cirsradg_wave.f:1240-1286).
 //   quadrature_kernel  angle-weighted sum over paths (forwardavfovX.f:330,356).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -780,6 +781,17 @@ __device__ __forceinline__ double gascon_d(const KernelArgs &a, const DevReq &R,
   return 0.0;
 }
 
+// 16-bit log10(k) codes (BF: 1 BF16, 2 IEEE half): log10 k = code + off
+template <int BF>
+__device__ __forceinline__ bool code_nonpos(unsigned short c) {
+  return c == (BF == 1 ? RADG_BF16_NONPOS : RADG_F16_NONPOS);
+}
+template <int BF>
+__device__ __forceinline__ float code_value(unsigned short c) {
+  if constexpr (BF == 1) return __uint_as_float((unsigned int)c << 16);
+  else return __half2float(__ushort_as_half(c));
+}
+
 // ============================================================ layer pass
 // LDS of a layer item (layer_layout in radg_dev.h):
 //   header (per workgroup)  G[NG+1], delg[NG] (FP64), delg[NG] (T)
@@ -902,13 +914,14 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
             bool klog;
             T y1, y2, y3, y4;
             if (BF) {
-              // BF16 log10(k) codes: ln k comes straight from the code
+              // 16-bit log10(k) codes: ln k comes straight from the code
               const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + wgk.base;
-              klog = !(sl[i1] == RADG_BF16_NONPOS || sl[i2] == RADG_BF16_NONPOS ||
-                       sl[i3] == RADG_BF16_NONPOS || sl[i4] == RADG_BF16_NONPOS);
+              klog = !(code_nonpos<BF>(sl[i1]) || code_nonpos<BF>(sl[i2]) ||
+                       code_nonpos<BF>(sl[i3]) || code_nonpos<BF>(sl[i4]));
+              const float offg = a.koff[wgk.ob + gg];
               auto dec = [&](unsigned short c) -> float {
-                if (c == RADG_BF16_NONPOS) return klog ? -__builtin_huge_valf() : 0.0f;
-                const float l = (__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10;
+                if (code_nonpos<BF>(c)) return klog ? -__builtin_huge_valf() : 0.0f;
+                const float l = (code_value<BF>(c) + offg) * ln10;
                 return klog ? l : expf(l);
               };
               y1 = (T)dec(sl[i1 + gg]);
@@ -970,9 +983,10 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
         T y1, y2, y3, y4;
         if (BF) {
           const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + sb;
+          const float offg = a.koff[((lane < NG) ? wgk.ob : wgk.ob2) + gi];
           auto dec = [&](unsigned short c) -> T {
-            if (c == RADG_BF16_NONPOS) return (T)0;
-            return (T)expf((__uint_as_float((unsigned int)c << 16) * wgk.sc + wgk.off) * ln10);
+            if (code_nonpos<BF>(c)) return (T)0;
+            return (T)expf((code_value<BF>(c) + offg) * ln10);
           };
           y1 = dec(sl[i1 + gi]);
           y2 = dec(sl[i2 + gi]);
@@ -2072,7 +2086,7 @@ size_t radg_lds_emission_bytes(int RL, int nsw, int fp64) {
 // scratch bytes per wavefront.  0: no fused form for this (mode, E, RL).
 size_t radg_lds_fused_bytes(int mode, int ng, int E, int RL, int maxgas, int maxnsw,
                             long long maxblock, int *ws) {
-  const bool have = (mode == 2) ? (E == 8) : (E == 1 || E == 2 || E == 4 || E == 8);
+  const bool have = (mode >= 2) ? (E == 8) : (E == 1 || E == 2 || E == 4 || E == 8);
   if (!have || RL > 4) return 0;
   const int ts = mode == 0 ? 8 : 4;
   const int rl = RL <= 2 ? 2 : 4;
@@ -2099,7 +2113,8 @@ int radg_emission_RL(int maxlays) {
   return RL;
 }
 
-// mode: 0 FP64, 1 mixed (FP32 table), 2 mixed with BF16 log10(k) table
+// mode: 0 FP64, 1 mixed (FP32 table), 2 / 3 mixed with BF16 / IEEE-half
+// log10(k) tables
 hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t lds, hipStream_t st) {
   if (a.nA > 0) {
     const unsigned nb = (unsigned)((a.nA + 255) / 256);
@@ -2124,6 +2139,14 @@ hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t ld
       case 8: return launch_layer_E<float, 8, 1>(a, lds, st);
       case 16: return launch_layer_E<float, 16, 1>(a, lds, st);
     }
+  } else if (mode == 3) {
+    switch (E) {
+      case 1: return launch_layer_E<float, 1, 2>(a, lds, st);
+      case 2: return launch_layer_E<float, 2, 2>(a, lds, st);
+      case 4: return launch_layer_E<float, 4, 2>(a, lds, st);
+      case 8: return launch_layer_E<float, 8, 2>(a, lds, st);
+      case 16: return launch_layer_E<float, 16, 2>(a, lds, st);
+    }
   } else {
     switch (E) {
       case 1: return launch_layer_E<float, 1>(a, lds, st);
@@ -2147,6 +2170,8 @@ hipError_t radg_launch_fused(const KernelArgs &a, int mode, int E, int RL, size_
     }
   } else if (mode == 2) {
     if (E == 8) return launch_fused_E<float, 8, 1>(a, RL, lds, st);
+  } else if (mode == 3) {
+    if (E == 8) return launch_fused_E<float, 8, 2>(a, RL, lds, st);
   } else {
     switch (E) {
       case 1: return launch_fused_E<float, 1, 0>(a, RL, lds, st);
@@ -2195,20 +2220,69 @@ __global__ void to_f32_kernel(const double *src, float *dst, long long n) {
   if (i < n) dst[i] = (float)src[i];
 }
 
-// code = bf16((log10 k - off) / sc), round to nearest even; k <= 0 -> -inf
-__global__ void to_bf16_kernel(const double *src, unsigned short *dst, long long n, double off,
-                               double sc) {
+// Offset of each (slice, ordinate): the FP32 rounding of the mean of log10 k
+// over the slice's positive values at that ordinate (0 when there are none),
+// summed in pool order.  per: values per ordinate of a slice (np * nt).
+__global__ void slice_offsets_kernel(const double *src, float *off, long long nslice, int per, int ng) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslice * ng) return;
+  const long long sl = t / ng;
+  const int g = (int)(t - sl * ng);
+  const double *p = src + sl * per * ng + g;
+  double sum = 0.0;
+  int cnt = 0;
+  for (int i = 0; i < per; i++) {
+    const double k = p[(long long)i * ng];
+    if (k > 0.0) {
+      sum += log10(k);
+      cnt++;
+    }
+  }
+  off[t] = cnt > 0 ? (float)(sum / cnt) : 0.0f;
+}
+
+// IEEE half of an FP32 value, round to nearest even, by integer operations
+// on its bits (so that the FP32 rounding before it is never folded into one
+// rounding from FP64); the values here lie far below the half range's top.
+__device__ __forceinline__ unsigned short f32_to_f16_rne(float x) {
+  const unsigned int u = __float_as_uint(x);
+  const unsigned int sign = (u >> 16) & 0x8000u;
+  const int e = (int)((u >> 23) & 0xffu) - 127 + 15;
+  unsigned int m = (u & 0x7fffffu) | 0x800000u;  // significand with its leading 1
+  if ((u & 0x7fffffffu) == 0u) return (unsigned short)sign;
+  if (e >= 31) return (unsigned short)(sign | 0x7c00u);
+  // bits to drop: 13 for a normal half, more below the normal range
+  const int shift = e >= 1 ? 13 : 14 - e;
+  if (shift > 25) return (unsigned short)sign;
+  const unsigned int kept = m >> shift;
+  const unsigned int rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+  unsigned int h = (e >= 1) ? (((unsigned int)e << 10) | (kept & 0x3ffu)) : kept;
+  if (rem > half || (rem == half && (h & 1u))) h++;  // a carry moves into the exponent
+  return (unsigned short)(sign | h);
+}
+
+// code = round16(float(log10 k - off[slice, g])), round to nearest even;
+// k <= 0 -> -inf.  BF: 1 BF16, 2 IEEE half.
+template <int BF>
+__global__ void to_code16_kernel(const double *src, const float *off, unsigned short *dst, long long n,
+                                 int ntab, int ng) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double k = src[i];
   if (!(k > 0.0)) {
-    dst[i] = (unsigned short)RADG_BF16_NONPOS;
+    dst[i] = (unsigned short)(BF == 1 ? RADG_BF16_NONPOS : RADG_F16_NONPOS);
     return;
   }
-  const float x = (float)((log10(k) - off) / sc);
-  unsigned int b = __float_as_uint(x);
-  b += 0x7FFFu + ((b >> 16) & 1u);
-  dst[i] = (unsigned short)(b >> 16);
+  const long long sl = i / ntab;
+  const int g = (int)(i % ng);
+  const float x = (float)(log10(k) - (double)off[sl * ng + g]);
+  if constexpr (BF == 1) {
+    unsigned int b = __float_as_uint(x);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    dst[i] = (unsigned short)(b >> 16);
+  } else {
+    dst[i] = f32_to_f16_rne(x);
+  }
 }
 }  // namespace
 
@@ -2218,10 +2292,22 @@ hipError_t radg_convert_f32(const double *src, float *dst, long long n, hipStrea
   return hipGetLastError();
 }
 
-hipError_t radg_convert_bf16(const double *src, unsigned short *dst, long long n, double off,
-                             double sc, hipStream_t st) {
+// The per-(slice, ordinate) offsets of a pool of nslice slices of ntab
+// values (ng ordinates fastest), then its 16-bit codes (bf: 1 BF16, 2 half).
+hipError_t radg_slice_offsets(const double *src, float *off, long long nslice, int ntab, int ng,
+                              hipStream_t st) {
+  const long long n = nslice * ng;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src,
-                     dst, n, off, sc);
+  hipLaunchKernelGGL(slice_offsets_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src,
+                     off, nslice, ntab / ng, ng);
+  return hipGetLastError();
+}
+
+hipError_t radg_convert_code16(int bf, const double *src, const float *off, unsigned short *dst,
+                               long long n, int ntab, int ng, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (bf == 1) hipLaunchKernelGGL(to_code16_kernel<1>, grid, dim3(256), 0, st, src, off, dst, n, ntab, ng);
+  else hipLaunchKernelGGL(to_code16_kernel<2>, grid, dim3(256), 0, st, src, off, dst, n, ntab, ng);
   return hipGetLastError();
 }

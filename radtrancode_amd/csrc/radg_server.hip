@@ -356,7 +356,7 @@ extern "C" {
 int radg_server_create(int mode, size_t budget_bytes, int max_batch, int max_inflight,
                        int per_path_outputs, radg_server **out) {
   *out = nullptr;
-  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED && mode != RADG_MODE_BF16)
+  if (mode != RADG_MODE_FP64 && mode != RADG_MODE_MIXED && mode != RADG_MODE_BF16 && mode != RADG_MODE_F16)
     return RADG_EINPUT;
   if (max_batch < 1 || max_inflight < 1 || max_inflight > 16) return RADG_EINPUT;
   auto *s = new radg_server;

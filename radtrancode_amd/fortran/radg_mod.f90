@@ -12,7 +12,7 @@ module radg
   use iso_c_binding
   implicit none
   private
-  integer(c_int), parameter, public :: RADG_MODE_FP64 = 0, RADG_MODE_MIXED = 1
+  integer(c_int), parameter, public :: RADG_MODE_FP64 = 0, RADG_MODE_MIXED = 1, RADG_MODE_BF16 = 2, RADG_MODE_F16 = 3
   integer(c_int), parameter, public :: RADG_OK = 0
 
   public :: radg_set_mode, radg_read_klist, radg_register_cia, radg_cirsradg_status

@@ -189,6 +189,47 @@ def config5_run(rad, n, cia, nthreads):
     }
 
 
+def percall_run(rad, req, ncalls, cache):
+    """The unchanged Fortran call path: ``ncalls`` calls of cirsradg_wave_,
+    one wavenumber each, as cirsrtfg_wave.f:361-448 makes them, with the
+    REAL(4) arrays of the caller (arrdef.f leading dimensions) built once.
+    ``cache``: whether a call may re-stage the previous call's batch
+    (RADG_FIXED_CACHE)."""
+    import ctypes as C
+
+    os.environ["RADG_FIXED_CACHE"] = "1" if cache else "0"
+    a = rad.fortran_arrays(req)
+    out = np.zeros(rad.MAXPAT, np.float32)
+    gts = np.zeros(rad.MAXPAT, np.float32)
+    dq = np.zeros((rad.MAXPAT, rad.MAXLAY, rad.NPARMAX), np.float32, order="F")
+    f = lambda x: C.byref(C.c_float(float(x)))
+    i = lambda x: C.byref(C.c_int(int(x)))
+    p = lambda x: x.ctypes.data_as(C.c_void_p)
+    vv = C.c_float(0.0)
+    args = [f(req.dist), i(req.inormal), i(req.iray), p(a["delh"]), i(req.nlayer),
+            i(req.npath), i(req.ngas), p(a["press"]), p(a["temp"]), p(a["pp"]), p(a["amount"]),
+            i(1), i(req.ispace), i(req.amform), C.byref(vv), p(a["nlayin"]), p(a["layinc"]),
+            p(a["cont"]), p(a["scale"]), p(a["imod"]), p(a["idgas"]), p(a["isogas"]),
+            p(a["emtemp"]), i(req.itype), i(len(req.vem)), p(a["vem"]), p(a["emissivity"]),
+            f(req.tsurf), p(gts), f(req.radius1), i(req.flagh2p), p(a["hfp"]), f(req.radextra),
+            i(len(req.isw)), p(a["isw"]), p(out), p(dq)]
+    fn = rad.lib().cirsradg_wave_
+    lat = []
+    for k in range(ncalls + 10):
+        vv.value = float(req.vwave[k % req.nwave])
+        t0 = time.perf_counter()
+        fn(*args)
+        t1 = time.perf_counter()
+        rad.check(rad.lib().radg_cirsradg_status())
+        if k >= 10:
+            lat.append(t1 - t0)
+    os.environ.pop("RADG_FIXED_CACHE", None)
+    lat = np.array(lat) * 1e3
+    return {"calls": ncalls, "ms_mean": float(lat.mean()), "ms_p50": float(np.percentile(lat, 50)),
+            "ms_p99": float(np.percentile(lat, 99)),
+            "spectra_per_s": 1e3 / (float(lat.mean()) * req.nwave)}
+
+
 def serving_run(rad, n, rate):
     """The serving mix: ``n`` requests (layers 40-200, wavenumber windows of
     200-5000 points on a shared 5000-point grid, 1-6 k-gases + H2/He)
@@ -280,6 +321,8 @@ def main():
     ap.add_argument("--bf16-steps", type=int, default=3)
     ap.add_argument("--config5-requests", type=int, default=1024,
                     help="requests in the large-spectrum BF16 stress (0: skip it)")
+    ap.add_argument("--percall", type=int, default=1000,
+                    help="calls of the unchanged cirsradg_wave_ entry, one wavenumber each (0: skip)")
     ap.add_argument("--retrievals", type=int, default=512,
                     help="retrievals in the batched optimal-estimation run (0: skip it)")
     ap.add_argument("--retrieval-iters", type=int, default=10)
@@ -372,6 +415,16 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(tables, cia, reqs[0], args.cpu_seconds)
 
+    percall = None
+    if args.percall > 0:
+        rad.set_mode(rad.MODE_MIXED)
+        percall = {
+            "reused_batch": percall_run(rad, reqs[0], args.percall, True),
+            "batch_per_call": percall_run(rad, reqs[0], min(args.percall, 200), False),
+            "call": "cirsradg_wave_ (REAL(4) arrays, one wavenumber per call), mixed form, "
+                    "120 layers x 5 paths x 5 gases, wall clock per call",
+        }
+
     c5 = None
     if args.config5_requests > 0:
         c5 = config5_run(rad, args.config5_requests, cia, os.cpu_count() or 1)
@@ -415,6 +468,7 @@ def main():
         "error_vs_fp64_oracle": {"radiance_rel": rad_err, "jacobian_col_scaled": jac_err,
                                  "sample": f"{len(sample)} wavenumbers of spectrum 0"},
         "roofline": roof,
+        "per_call_cirsradg_wave": percall,
         "cpu_baseline": cpu,
         "serving": serve,
         "config5_bf16_stress": c5,

@@ -941,6 +941,7 @@ struct radg_batch {
     int wsbytes;
   };
   std::vector<EmClass> em;
+  std::vector<long long> sig;  // layout of the plan the device block was laid out by
   double alg_bytes[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
   bool stream_used = false;
@@ -1877,6 +1878,81 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
 }
 
 namespace {
+// The layout of a plan: its offsets and item counts.  Two plans with the
+// same layout fit the same device block.
+std::vector<long long> plan_sig(const Plan &pl) {
+  std::vector<long long> v = {(long long)pl.total, pl.nA, pl.nB, pl.nC, (long long)pl.o_reqs,
+                              (long long)pl.o_inter, (long long)pl.o_out, (long long)pl.o_jac,
+                              (long long)pl.o_wout, (long long)pl.o_wjac, (long long)pl.o_err,
+                              (long long)pl.o_clsstart, (long long)pl.cls_rl.size()};
+  for (int rl : pl.cls_rl) v.push_back(rl);
+  for (long long x : pl.sA) v.push_back(x);
+  for (long long x : pl.sB) v.push_back(x);
+  return v;
+}
+
+// Everything of a batch but its device block and pinned staging: the staged
+// inputs (uploaded by the next launch), the host copies, the kernel
+// arguments and the launch classes.  Events already made are kept.
+int batch_fill(radg_batch *b, const Plan &pl, int nreq, const radg_request *reqs, int mode,
+               const std::shared_ptr<TableSet> &ts, const std::shared_ptr<CIA> &cia, const AuxTables &aux) {
+  b->mode = mode;
+  b->nreq = nreq;
+  b->ts = ts;
+  b->cia = cia;
+  b->aux = aux;
+  b->bytes = pl.total;
+  b->o_out = pl.o_out;
+  b->o_gts = pl.o_gts;
+  b->o_jac = pl.o_jac;
+  b->o_wout = pl.o_wout;
+  b->o_wgts = pl.o_wgts;
+  b->o_wjac = pl.o_wjac;
+  const size_t tsz = (mode == RADG_MODE_FP64) ? 8 : 4;
+  b->span_p0 = pl.o_out;
+  b->span_p1 = pl.o_jac + (size_t)pl.njac * tsz;
+  b->span_w0 = pl.o_wout;
+  b->span_w1 = pl.o_wjac + (size_t)pl.nwjac * 8;
+  b->hup_bytes = pl.o_inter;
+  b->up_pending = true;
+  stage_inputs(pl, nreq, reqs, b->hup);
+  for (int i = 0; i < 3; i++) b->alg_bytes[i] = 0.0;
+  for (int r = 0; r < nreq; r++)
+    for (int i = 0; i < 3; i++) b->alg_bytes[i] += pl.packs[r].alg_bytes[i];
+  b->hreq.resize(nreq);
+  for (int r = 0; r < nreq; r++) {
+    const Packed &P = pl.packs[r];
+    radg_batch::HostReq &hq = b->hreq[r];
+    hq.L = reqs[r].nlayer;
+    hq.npath = reqs[r].npath;
+    hq.ngas = reqs[r].ngas;
+    hq.nwave = reqs[r].nwave;
+    hq.ncont = reqs[r].ncont;
+    hq.h2p = P.d.flagh2p;
+    hq.layinc = P.layinc;
+    hq.nlays.resize(hq.npath);
+    for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
+  }
+  b->reqs.resize(nreq);
+  for (int r = 0; r < nreq; r++) b->reqs[r] = pl.packs[r].d;
+  set_kernel_args(b->args, b->dmem, pl, nreq, *ts, *cia, aux, mode);
+  b->E = radg_layer_E(ts->ng);
+  b->maxngas = pl.maxngas;
+  b->maxnsw = pl.maxnsw;
+  b->RL = radg_emission_RL(pl.maxlays);
+  b->em = emission_classes(pl, reqs, b->dmem, mode, ts->ng);
+  for (const auto &ec : b->em)
+    if (!ec.fused && (ec.ldsA > 160 * 1024 || ec.lds > 160 * 1024))
+      return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", ec.ldsA, ec.lds);
+  // events are created on first use of each ring slot
+  if (b->evs.empty()) {
+    b->evs.assign(4 * radg_batch::RING, nullptr);
+    b->ev = &b->evs[0];
+  }
+  b->sig = plan_sig(pl);
+  return RADG_OK;
+}
+
 int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, radg_batch **out,
                  bool fixed_args) {
   *out = nullptr;
@@ -1901,68 +1977,45 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
   if (budget && pl.total > budget)
     return fail(RADG_EBUDGET, "batch needs %zu bytes, budget %zu", pl.total, budget);
   std::unique_ptr<radg_batch> b(new radg_batch);
-  b->mode = mode;
-  b->nreq = nreq;
-  b->ts = ts;
-  b->cia = cia;
-  b->aux = aux;
-  b->bytes = pl.total;
   b->dmem = static_cast<char *>(dev_cache().get(pl.total));
   if (!b->dmem) return fail(RADG_EHIP, "device allocation of %zu bytes failed", pl.total);
-  b->o_out = pl.o_out;
-  b->o_gts = pl.o_gts;
-  b->o_jac = pl.o_jac;
-  b->o_wout = pl.o_wout;
-  b->o_wgts = pl.o_wgts;
-  b->o_wjac = pl.o_wjac;
-  const size_t tsz = (mode == RADG_MODE_FP64) ? 8 : 4;
-  b->span_p0 = pl.o_out;
-  b->span_p1 = pl.o_jac + (size_t)pl.njac * tsz;
-  b->span_w0 = pl.o_wout;
-  b->span_w1 = pl.o_wjac + (size_t)pl.nwjac * 8;
   // Host staging of everything but the intermediates and outputs, in pinned
   // memory.  Nothing here touches the device: the staged inputs go up with
   // the first launch, on the launch's stream, so that packing a batch never
   // waits for the kernels of batches already running.
   b->hup = static_cast<char *>(host_cache().get(pl.o_inter));
   if (!b->hup) return fail(RADG_EHIP, "pinned allocation of %zu bytes failed", pl.o_inter);
-  b->hup_bytes = pl.o_inter;
-  b->up_pending = true;
-  stage_inputs(pl, nreq, reqs, b->hup);
-  for (int r = 0; r < nreq; r++)
-    for (int i = 0; i < 3; i++) b->alg_bytes[i] += pl.packs[r].alg_bytes[i];
-  b->hreq.resize(nreq);
-  for (int r = 0; r < nreq; r++) {
-    const Packed &P = pl.packs[r];
-    radg_batch::HostReq &hq = b->hreq[r];
-    hq.L = reqs[r].nlayer;
-    hq.npath = reqs[r].npath;
-    hq.ngas = reqs[r].ngas;
-    hq.nwave = reqs[r].nwave;
-    hq.ncont = reqs[r].ncont;
-    hq.h2p = P.d.flagh2p;
-    hq.layinc = P.layinc;
-    hq.nlays.resize(hq.npath);
-    for (int p = 0; p < hq.npath; p++) hq.nlays[p] = P.paths[p].nlays;
-  }
-
-  b->reqs.resize(nreq);
-  for (int r = 0; r < nreq; r++) b->reqs[r] = pl.packs[r].d;
-  set_kernel_args(b->args, b->dmem, pl, nreq, *ts, *cia, aux, mode);
-  const int fp64 = mode == RADG_MODE_FP64;
-  b->E = radg_layer_E(ts->ng);
-  b->maxngas = pl.maxngas;
-  b->maxnsw = pl.maxnsw;
-  b->RL = radg_emission_RL(pl.maxlays);
-  b->em = emission_classes(pl, reqs, b->dmem, mode, ts->ng);
-  for (const auto &ec : b->em)
-    if (!ec.fused && (ec.ldsA > 160 * 1024 || ec.lds > 160 * 1024))
-      return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", ec.ldsA, ec.lds);
-  // events are created on first use of each ring slot
-  b->evs.assign(4 * radg_batch::RING, nullptr);
-  b->ev = &b->evs[0];
+  rc = batch_fill(b.get(), pl, nreq, reqs, mode, ts, cia, aux);
+  if (rc) return rc;
   *out = b.release();
   return RADG_OK;
+}
+
+// Re-stage a batch for new requests of the same layout, in the tables,
+// CIA and mode it was made for, after its last launch has finished: 0 done,
+// 1 the requests need another layout (the batch is left as it was), else an
+// error status.
+int batch_restage(radg_batch *b, int nreq, const radg_request *reqs, int mode, bool fixed_args) {
+  std::shared_ptr<TableSet> ts;
+  std::shared_ptr<CIA> cia;
+  AuxTables aux;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ts = g_tables;
+    cia = g_cia;
+    aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
+    if (fixed_args && g_iread != 999) aux.solar.reset();
+  }
+  if (!ts || !cia || ts != b->ts || cia != b->cia || mode != b->mode || b->smem) return 1;
+  if (aux.para != b->aux.para || aux.e3 != b->aux.e3 || aux.n2nir != b->aux.n2nir ||
+      aux.solar != b->aux.solar || aux.gascon != b->aux.gascon)
+    return 1;
+  Plan pl;
+  int rc = plan_batch(nreq, reqs, mode, *ts, aux, pl);
+  if (rc) return rc;
+  if (plan_sig(pl) != b->sig) return 1;
+  if (b->launches > 0) HIPCHK(hipEventSynchronize(b->ev[3]));
+  return batch_fill(b, pl, nreq, reqs, mode, ts, cia, aux);
 }
 
 }  // namespace
@@ -2597,10 +2650,32 @@ int run_fixed(const R *delh, int L, int npath, int ngas, const R *press, const R
     q.vsec = dust->vsec.data();
     q.xsec = dust->xsec.data();
   }
+  // The calling thread keeps its last one-wavenumber batch: a call whose
+  // request has the same layout (a caller's wavenumber loop) re-stages its
+  // inputs into it instead of making a new one.  RADG_FIXED_CACHE=0 in the
+  // environment makes a batch per call.  The batch is never freed: it is
+  // held until the process ends.
+  static thread_local radg_batch *cached = nullptr;
+  const char *env_cache = std::getenv("RADG_FIXED_CACHE");
+  const bool use_cache = !(env_cache && env_cache[0] == '0');
   radg_batch *b = nullptr;
-  int rc = batch_create(1, &q, g_mode, 0, &b, true);
-  if (rc) return rc;
-  std::unique_ptr<radg_batch> guard(b);
+  int rc;
+  if (use_cache && cached) {
+    rc = batch_restage(cached, 1, &q, g_mode, true);
+    if (rc == RADG_OK) b = cached;
+    else if (rc != 1) return rc;
+  }
+  std::unique_ptr<radg_batch> guard;
+  if (!b) {
+    rc = batch_create(1, &q, g_mode, 0, &b, true);
+    if (rc) return rc;
+    if (use_cache) {
+      delete cached;
+      cached = b;
+    } else {
+      guard.reset(b);
+    }
+  }
   if ((rc = radg_batch_launch(b, nullptr))) return rc;
   if ((rc = radg_batch_sync(b))) return rc;
   out.assign((size_t)nwave * npath, 0.0);

@@ -13,6 +13,12 @@
  *   radg_read_klist          /INTERPK/ filled by radtran/cirsrad/read_klist.f:30-368
  *                            from .kta headers (radtran/cirsrad/read_khead.f:18-120)
  *   radg_register_ktables    the same registry from arrays already in memory
+ *   radg_read_klbllist       /INTERPKLBL/ filled by radtran/cirsrad/read_klbllist.f:30-198
+ *                            from .lta headers (radtran/cirsrad/read_klblhead.f:19-75)
+ *   radg_register_lbltables  the same registry from arrays already in memory
+ *   radg_set_ilbl            /LBLTABLE/ ILBL (radtran/cirsradg/cirsradg_wave.f:317,336-339,
+ *                            502-506,705-741: get_klblg.f and noverlapg1.f instead of
+ *                            get_kg.f and noverlapg.f)
  *   radg_register_cia        /CIATABLE/ filled by radtran/radtran/ciaread.f:63-103
  *   radg_read_cia            the same from the table file ciaread.f:77-89 reads
  *   radg_register_parah2     /FPTABLE/ filled by radtran/radtran/fpread_all.f:72-143
@@ -116,6 +122,39 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas,
  * them (read_klist.f:30-110, read_khead.f:18-120).  recl is the record
  * length in bytes (4 for the REAL(4) build, 8 for REAL(8)). */
 int radg_read_klist(const char *klist_path, int recl);
+
+/* /INTERPKLBL/: line-by-line tables (one g-ordinate of weight 1,
+ * cirsradg_wave.f:336-339), all on the wavenumbers vmin + i*delv, i < npoint
+ * (delv > 0) and one P/T grid (np pressures in atm; nt > 0 temperatures, or
+ * nt < 0: |nt| temperatures per pressure, temp[np][|nt|]).  k[t] holds table
+ * t's npoint*np*|nt| values of k * 1e20, temperature fastest, then pressure
+ * (the .lta record order, calc_lbltable.f:321-368).  A gas takes the last
+ * table of its (idgas, isogas) (read_klbllist.f:190-198); gases without one
+ * have k = 0.  Replaces any line-by-line tables registered before; the
+ * k-tables are kept. */
+int radg_register_lbltables(int ntab, const int *idgas, const int *isogas, int npoint,
+                            double vmin, double delv, int np, int nt, const double *press_atm,
+                            const double *temp, const double *const *k);
+
+/* Read the .lta files named in an .lls list (one path per line, the
+ * extension made .lta as file.f does) and register them
+ * (read_klbllist.f:30-198, read_klblhead.f:19-75): the tables must share
+ * NP, NT, DELV, NPOINT, VMIN and IREC0, with P/T grid ends within 1e-4
+ * (else RADG_EINPUT, where the reference stops); the last file's P/T grid
+ * is used.  recl: record length in bytes (4 or 8). */
+int radg_read_klbllist(const char *list_path, int recl);
+
+/* /LBLTABLE/ ILBL for the batches made afterwards (and every entry point
+ * that makes one): 0 runs get_kg + noverlapg on the k-tables, 2 runs
+ * get_klblg + noverlapg1 on the line-by-line tables (a wavenumber must lie
+ * within (vmin - delv - 0.02 delv, vmin + npoint delv - 0.02 delv), the
+ * slices get_klblg.f:182-185 can reach; else RADG_EINPUT).  ILBL = 1 is
+ * RADG_EUNSUPPORTED: noverlapg1.f:64-67 stops unless the k-tables have one
+ * g-ordinate.  The T column of the line-by-line Jacobian carries
+ * sum(amount_i dk_i/dT) through noverlapg1's branches; the reference's own
+ * is undefined there (noverlapg1.f:108 stores it into grad). */
+int radg_set_ilbl(int ilbl);
+int radg_get_ilbl(void);
 
 /* Copy the 16-bit form (mode RADG_MODE_BF16 or RADG_MODE_F16) of registered
  * table itab (0-based) to the host: codes -> npoint*np*nt*ng values in the

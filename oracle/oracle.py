@@ -83,7 +83,7 @@ class _REQ(C.Structure):
                 ("ncont", C.c_int), ("nsec", C.c_int),
                 ("cont", C.c_void_p), ("vsec", C.c_void_p), ("xsec", C.c_void_p),
                 ("inlte", C.c_int), ("nlte_k1", C.c_double), ("nlte_a", C.c_double),
-                ("iform", C.c_int)]
+                ("iform", C.c_int), ("ilbl", C.c_int)]
 
 
 def _d(a):
@@ -97,7 +97,7 @@ def _i(a):
 class _Pack:
     """Keeps the contiguous copies alive while the C structs point at them."""
 
-    def __init__(self, tables, cia, req):
+    def __init__(self, tables, cia, req, ilbl=0):
         self.keep = []
         kts = (_KT * max(1, len(tables)))()
         for t, kt in enumerate(tables):
@@ -173,11 +173,16 @@ class _Pack:
                         *[a.ctypes.data for a in f],
                         nc, len(req.vsec) if nc else 0,
                         *([a.ctypes.data for a in dust] if nc else [None, None, None]),
-                        int(req.inlte), float(req.nlte_k1), float(req.nlte_a), int(req.iform))
+                        int(req.inlte), float(req.nlte_k1), float(req.nlte_a), int(req.iform),
+                        int(ilbl))
 
 
-def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):
+def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1, ilbl=0):
     """Run the restatement over ``vwave`` (default: the request's grid).
+
+    ``ilbl=2``: ``tables`` are line-by-line tables (one g-ordinate each, all
+    on one wavenumber and P/T grid), looked up by ``get_klblg`` and summed by
+    ``noverlapg1`` (``cirsradg_wave.f:336-339,502-506,735-741``).
 
     Returns ``(output[nwave, npath], gtsurf[nwave, npath],
     doutputdq[nwave, npath, nlayer, nparam])`` where ``doutputdq`` is
@@ -186,7 +191,7 @@ def spectrum(tables, cia, req, vwave=None, jac_mode=0, nthreads=1):
     (``cirsradg_wave.f:1240-1286``).
     """
     lb = lib()
-    pk = _Pack(tables, cia, req)
+    pk = _Pack(tables, cia, req, ilbl)
     vw = _d(req.vwave if vwave is None else vwave)
     nw, npath, L, npar = vw.shape[0], req.npath, req.nlayer, req.nparam
     out = np.zeros((nw, npath))
@@ -214,6 +219,19 @@ def noverlapg(delg, amount, k_gn, dkgndt):
                      dkgndt.ctypes.data_as(C.c_void_p), kg.ctypes.data_as(C.c_void_p),
                      dk.ctypes.data_as(C.c_void_p))
     return kg, dk
+
+
+def noverlapg1(amount, k_gn, dkgndt):
+    """``noverlapg1.f`` on one layer (NG = 1): ``k``, ``dkdq[ngas + 1]``."""
+    lb = lib()
+    k_gn, dkgndt, amount = _d(k_gn), _d(dkgndt), _d(amount)
+    ngas = k_gn.shape[0]
+    kg = np.zeros(1)
+    dk = np.zeros(ngas + 1)
+    lb.orc_noverlapg1(C.c_int(ngas), amount.ctypes.data_as(C.c_void_p),
+                      k_gn.ctypes.data_as(C.c_void_p), dkgndt.ctypes.data_as(C.c_void_p),
+                      kg.ctypes.data_as(C.c_void_p), dk.ctypes.data_as(C.c_void_p))
+    return float(kg[0]), dk
 
 
 def planck(ispace, v, t):

@@ -372,6 +372,70 @@ void orc_noverlapg(const double *delg, int ng, int ngas, const double *amount,
   free(grad);
 }
 
+/* ------------------------------------------------------------ noverlapg1 */
+/* noverlapg1.f:1-160, the sum that replaces random overlap for line-by-line
+ * tables (NG = 1, else :64-67 stop): k = the amount-weighted sum of the
+ * gases' k, taken in gas order with the routine's branches for a zero term
+ * (:76-108 first pair, :116-146 later gases).  k_gn/dkgndt are [gas]; dkdq
+ * is [ngas + 1], the gas columns then T.
+ *
+ * k and the gas columns are the reference's: each gas's own k, except that
+ * a zero running sum before gas m >= 3 zeroes the columns of the gases
+ * before it (:123-128).  The T column follows the same branches with the
+ * running sum's derivative carried from gas to gas (amount_i dk_i/dT added
+ * for every term the sum takes).  The reference's own T column is undefined
+ * or wrong: its first pair stores the sum into grad, not dkdq (:108); later
+ * gases read dk1dT(1)/dk2dT(1) of the first pair (:129,137,146) where
+ * :119-120 set dk1dT(2)/dk2dT(2); one gas never enters the loop, so k_g and
+ * dkdq keep what an earlier call left. */
+void orc_noverlapg1(int ngas, const double *amount, const double *k_gn, const double *dkgndt,
+                    double *k_g, double *dkdq) {
+  double s, d;
+  int m, j;
+  if (ngas == 1) {
+    k_g[0] = k_gn[0] * amount[0];
+    dkdq[0] = k_gn[0];
+    dkdq[1] = dkgndt[0] * amount[0];
+    return;
+  }
+  {
+    const double a1 = amount[0], a2 = amount[1], k1 = k_gn[0], k2 = k_gn[1];
+    if (k1 == 0.0) {
+      s = k2 * a2;
+      dkdq[0] = 0.0;
+      dkdq[1] = k2;
+      d = dkgndt[1] * a2;
+    } else if (k2 == 0.0) {
+      s = k1 * a1;
+      dkdq[0] = k1;
+      dkdq[1] = 0.0;
+      d = dkgndt[0] * a1;
+    } else {
+      s = k1 * a1 + k2 * a2;
+      dkdq[0] = k1;
+      dkdq[1] = k2;
+      d = dkgndt[0] * a1 + dkgndt[1] * a2;
+    }
+  }
+  for (m = 2; m < ngas; m++) {
+    const double km = k_gn[m], am = amount[m];
+    if (s == 0.0) {
+      s = km * am;
+      for (j = 0; j < m; j++) dkdq[j] = 0.0;
+      dkdq[m] = km;
+      d = dkgndt[m] * am;
+    } else if (km == 0.0) {
+      dkdq[m] = 0.0;
+    } else {
+      s = s + km * am;
+      dkdq[m] = km;
+      d = d + dkgndt[m] * am;
+    }
+  }
+  k_g[0] = s;
+  dkdq[ngas] = d;
+}
+
 /* ---------------------------------------------------------------- planck */
 /* planck_wave.f:1-42 (FP64 constants, as in the REAL(8) build). */
 double orc_planck(int ispace, double v, double t) {
@@ -1369,10 +1433,16 @@ static int orc_irregular_slice(const orc_ktable *kt, double v, double *frac) {
   return jpo;
 }
 
-/* get_kg.f:1-484 for one gas at all layers.  kout/dkoutdt are [L][ng]. */
+/* get_kg.f:1-484 for one gas at all layers.  kout/dkoutdt are [L][ng].
+ * lbl: get_klblg.f:1-312 instead (a line-by-line table, ng = 1): the slice
+ * 1 + INT((v + 0.02 delv - vmin) / delv) (:182-185), which must exist
+ * (:186-189 print a diagnostic and read on past the table), always
+ * coincident (no merge), and the same zero-slice test (:200-222), clamps
+ * and KLOG rule (:253-261) as get_kg.  A NaN, where get_klblg stops
+ * (:284-301), takes get_kg's 1e-37; finite tables give none. */
 static int orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press,
                           const double *temp, double vwave, double *kout,
-                          double *dkoutdt) {
+                          double *dkoutdt, int lbl) {
   const int np = kt->np, nt = kt->nt < 0 ? -kt->nt : kt->nt, ng = kt->ng, t2 = kt->nt < 0;
   const size_t ntab = (size_t)np * nt * ng;
   const double pmin = kt->lnp[0], pmax = kt->lnp[np - 1];
@@ -1383,7 +1453,11 @@ static int orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press,
   const double *table, *table2;
   double ktest = 0.0;
 
-  if (kt->delv > 0.0) {
+  if (lbl) {
+    const double x = (vwave + 0.02 * kt->delv - kt->vmin) / kt->delv;
+    if (!(kt->delv > 0.0) || !(x > -1.0 && x < (double)kt->npoint)) return ORC_ERR_LIMITS;
+    n1 = (long)x; /* INT truncates toward zero */
+  } else if (kt->delv > 0.0) {
     if (kt->delv >= 0.0001) eps = 0.02 * kt->delv;
     else eps = 0.25 * kt->delv;
     n1 = (long)((vwave + eps - kt->vmin) / kt->delv); /* IDINT truncates */
@@ -1406,7 +1480,9 @@ static int orc_get_kg_gas(const orc_ktable *kt, int nlayer, const double *press,
     memset(dkoutdt, 0, sizeof(double) * nlayer * ng);
     return ORC_OK;
   }
-  if (kt->delv > 0.0) {
+  if (lbl) {
+    coinc = 1;
+  } else if (kt->delv > 0.0) {
     tmp = kt->vmin + kt->delv * n1;
     frac = fabs(vwave - tmp);
     coinc = frac < eps;
@@ -1598,8 +1674,26 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       return ORC_ERR_UNSUPPORTED;
   }
 
+  /* line-by-line tables (/LBLTABLE/ ILBL = 2, cirsradg_wave.f:336-339):
+   * one g-ordinate of weight 1; each gas takes the last table of its (id,
+   * iso) (read_klbllist.f:190-198), on the P/T grid of the last table read
+   * (read_klblhead.f fills the one /INTERPKLBL/ grid per file).  ILBL = 1
+   * reaches noverlapg1 with the k-tables' NG, which stops unless NG = 1. */
+  if (r->ilbl == 1 || r->ilbl < 0 || r->ilbl > 2) return ORC_ERR_UNSUPPORTED;
+  if (r->ilbl == 2) {
+    static const double one[1] = {1.0};
+    if (ntab < 1) return ORC_ERR_LIMITS;
+    for (k = 0; k < ngas; k++) {
+      int t;
+      gtab[k] = NULL;
+      for (t = 0; t < ntab; t++)
+        if (tabs[t].idgas == r->idgas[k] && tabs[t].isogas == r->isogas[k]) gtab[k] = &tabs[t];
+    }
+    ng = 1;
+    delg = one;
+  }
   /* g-grid: all tables share it (read_klist.f:117-130). */
-  for (k = 0; k < ngas; k++) {
+  for (k = 0; r->ilbl != 2 && k < ngas; k++) {
     gtab[k] = find_table(tabs, ntab, r->idgas[k], r->isogas[k], vwave);
     if (gtab[k]) {
       if (ng == 0) {
@@ -1610,7 +1704,7 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
       }
     }
   }
-  if (ng == 0 && ntab > 0) {
+  if (ng == 0 && ntab > 0 && r->ilbl != 2) {
     ng = tabs[0].ng;
     delg = tabs[0].delg;
   }
@@ -1666,9 +1760,20 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
 
   /* get_kg, cirsradg_wave.f:502-503 */
   for (k = 0; k < ngas; k++) {
-    if (gtab[k]) {
+    if (gtab[k] && r->ilbl == 2) {
+      /* get_klblg, cirsradg_wave.f:504-505, on the shared P/T grid */
+      orc_ktable kx = *gtab[k];
+      kx.np = tabs[ntab - 1].np;
+      kx.nt = tabs[ntab - 1].nt;
+      kx.lnp = tabs[ntab - 1].lnp;
+      kx.tk = tabs[ntab - 1].tk;
+      kx.t2 = tabs[ntab - 1].t2;
+      rc = orc_get_kg_gas(&kx, L, r->press, r->temp, vwave, kout + (size_t)k * L * ng,
+                          dkoutdt + (size_t)k * L * ng, 1);
+      if (rc != ORC_OK) goto done;
+    } else if (gtab[k]) {
       rc = orc_get_kg_gas(gtab[k], L, r->press, r->temp, vwave, kout + (size_t)k * L * ng,
-                          dkoutdt + (size_t)k * L * ng);
+                          dkoutdt + (size_t)k * L * ng, 0);
       if (rc != ORC_OK) goto done;
     }
   }
@@ -1762,7 +1867,8 @@ int orc_cirsradg_wave(const orc_ktable *tabs, int ntab, const orc_cia *cia,
         }
       }
     }
-    orc_noverlapg(delg, ng, ngas, amo, k_gn, dkgndt, k_g, dkdq);
+    if (r->ilbl == 2) orc_noverlapg1(ngas, amo, k_gn, dkgndt, k_g, dkdq);  /* :739-740 */
+    else orc_noverlapg(delg, ng, ngas, amo, k_gn, dkgndt, k_g, dkdq);
     for (l = 0; l < ng; l++) {
       kl_g[j * ng + l] = k_g[l];
       for (k = 0; k < ngas; k++)

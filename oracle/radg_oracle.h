@@ -103,6 +103,11 @@ typedef struct orc_req {
   /* /SOLARDAT/ iform (no solar file, iread != 999): 1 and 3 scale imod 3
    * radiances to the planet's total flux (cirsradg_wave.f:912-925) */
   int iform;
+  /* /LBLTABLE/ ILBL: 0 = k-tables; 2 = the tables are line-by-line tables
+   * (one g-ordinate, the .lta content of read_klblhead.f, all on one
+   * wavenumber and P/T grid) looked up by get_klblg and summed by
+   * noverlapg1 (cirsradg_wave.f:336-339,502-506,735-741).  1 is refused. */
+  int ilbl;
 } orc_req;
 
 /* Status codes (the reference STOPs instead). */
@@ -137,6 +142,8 @@ void orc_rankg(int nparam, const double *gw, int ng, int nloop,
 void orc_noverlapg(const double *delg, int ng, int ngas,
                    const double *amount, const double *k_gn,
                    const double *dkgndt, double *k_g, double *dkdq); /* noverlapg.f */
+void orc_noverlapg1(int ngas, const double *amount, const double *k_gn,
+                    const double *dkgndt, double *k_g, double *dkdq); /* noverlapg1.f */
 double orc_planck(int ispace, double v, double t);                  /* planck_wave.f */
 double orc_planckg(int ispace, double v, double t);                 /* planckg_wave.f */
 void orc_ciaread(const orc_cia *cia, double vv, double temp,

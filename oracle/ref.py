@@ -6,7 +6,9 @@ and ``libradref32.so`` (as written).  This module feeds them the same
 synthetic inputs as the restatement:
 
 * k-tables are written as ``.kta`` files and registered through the
-  reference's ``read_klist`` (``radtran/cirsrad/read_klist.f:30``);
+  reference's ``read_klist`` (``radtran/cirsrad/read_klist.f:30``); with
+  ``ilbl=2``, one-ordinate tables as ``.lta`` files through
+  ``read_klbllist`` (``radtran/cirsrad/read_klbllist.f:1``);
 * the CIA table goes straight into ``/CIATABLE/`` (the harness's HSETCIA);
 * ``cirsradg_wave`` is called once per wavenumber with the reference's fixed
   leading dimensions, as ``cirsrtfg_wave.f:397-401`` calls it.
@@ -172,8 +174,9 @@ class RefRunner:
     precision is active at a time.
     """
 
-    def __init__(self, tables, cia, req, prec: int = 64, workdir: str | None = None):
-        from radtrancode_amd.kta import write_kta
+    def __init__(self, tables, cia, req, prec: int = 64, workdir: str | None = None,
+                 ilbl: int = 0):
+        from radtrancode_amd.kta import write_kta, write_lta
 
         self.prec = prec
         self.rdt = np.float64 if prec == 64 else np.float32
@@ -185,10 +188,11 @@ class RefRunner:
             workdir = self._tmp.name
         names = []
         for t, kt in enumerate(tables):
-            name = os.path.join(workdir, f"g{t}.kta")
-            write_kta(name, kt, recl=8 if prec == 64 else 4)
+            # ILBL = 2: one-ordinate tables as .lta files for read_klbllist
+            name = os.path.join(workdir, f"g{t}." + ("lta" if ilbl else "kta"))
+            (write_lta if ilbl else write_kta)(name, kt, recl=8 if prec == 64 else 4)
             names.append(name)
-        klist = os.path.join(workdir, "tables.kls")
+        klist = os.path.join(workdir, "tables." + ("lls" if ilbl else "kls"))
         with open(klist, "w") as f:
             for n in names:
                 f.write(n + "\n")
@@ -255,8 +259,14 @@ class RefRunner:
         nw = np.array([self.vwave.shape[0]], dtype=np.int32)
         nkl = np.zeros(1, dtype=np.int32)
         kb = klist.encode()
-        self.lib.hreadk_(C.c_char_p(kb), _p(ngas), _p(idg), _p(iso), _p(nw),
-                         _p(self.vwave), _p(nkl), C.c_size_t(len(kb)))
+        if ilbl:
+            self.lib.hsetilbl_(_p(np.array([ilbl], dtype=np.int32)))
+            self.lib.hreadlbl_(C.c_char_p(kb), _p(ngas), _p(idg), _p(iso), _p(nw),
+                               _p(self.vwave), C.c_size_t(len(kb)))
+            nkl[0] = len(names)
+        else:
+            self.lib.hreadk_(C.c_char_p(kb), _p(ngas), _p(idg), _p(iso), _p(nw),
+                             _p(self.vwave), _p(nkl), C.c_size_t(len(kb)))
         self.nkl = int(nkl[0])
         self._pack_args()
 

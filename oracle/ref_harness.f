@@ -13,7 +13,9 @@ C   HSETGASCON sets the /GASCONDEF/ flags; HGETH2O copies out the CKD H2O
 C             tables ngascon.f's BLOCK DATA holds.
 C   HSETCIA   loads /CIATABLE/ as ciaread.f:63-103 would from a .tab file.
 C   HREADK    registers .kta files through the reference's read_klist.
-C   HCLOSEK   closes the units read_klist opened.
+C   HREADLBL  registers .lta files through read_klbllist; HSETILBL sets
+C             /LBLTABLE/ ILBL.
+C   HCLOSEK   closes the units read_klist or read_klbllist opened.
 C
 C Compiled twice: as written (REAL = 4 bytes) and with REAL promoted to
 C 8 bytes, which is the FP64 form of the reference.
@@ -278,5 +280,28 @@ C     the transmission grid, E3 and its gradient as e3interp.f forms it
       DO I = 1, NKL
         CLOSE(100 + I)
       ENDDO
+      RETURN
+      END
+
+C     /LBLTABLE/ ILBL (cirsradg_wave.f:317): 2 runs get_klblg and
+C     noverlapg1 on the tables read_klbllist registered
+      SUBROUTINE HSETILBL(JLBL)
+      IMPLICIT NONE
+      INTEGER JLBL, ILBL
+      COMMON /LBLTABLE/ ILBL
+      ILBL = JLBL
+      RETURN
+      END
+
+C     Registers .lta files through the reference's read_klbllist
+C     (units 101.. as read_klist's; HCLOSEK closes them)
+      SUBROUTINE HREADLBL(KL, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE)
+      IMPLICIT NONE
+      CHARACTER*(*) KL
+      INTEGER NGAS, IDGAS(NGAS), ISOGAS(NGAS), NWAVE
+      REAL VWAVE(NWAVE)
+      CHARACTER*100 KLIST
+      KLIST = KL
+      CALL READ_KLBLLIST(KLIST, NGAS, IDGAS, ISOGAS, NWAVE, VWAVE)
       RETURN
       END

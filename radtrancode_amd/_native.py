@@ -36,6 +36,7 @@ EXPORTS = [
     "radg_batch_set_statemaps", "radg_batch_read_mapped", "radg_batch_device_outputs",
     "radg_oe_create", "radg_oe_cost", "radg_oe_propose", "radg_oe_destroy",
     "radg_spectrum_noalloc", "radg_cirsconv", "radg_read_cia",
+    "radg_register_lbltables", "radg_read_klbllist", "radg_set_ilbl", "radg_get_ilbl",
 ]
 
 
@@ -96,6 +97,12 @@ def lib():
     L.radg_register_n2nir.argtypes = [ip, vp, vp, ip, vp, vp]
     L.radg_register_solar.argtypes = [ip, vp, vp]
     L.radg_read_klist.argtypes = [C.c_char_p, ip]
+    for name in ["radg_register_lbltables", "radg_read_klbllist", "radg_set_ilbl", "radg_get_ilbl"]:
+        getattr(L, name).restype = ip
+    L.radg_register_lbltables.argtypes = [ip, vp, vp, ip, C.c_double, C.c_double, ip, ip, vp, vp, vp]
+    L.radg_read_klbllist.argtypes = [C.c_char_p, ip]
+    L.radg_set_ilbl.argtypes = [ip]
+    L.radg_get_ilbl.argtypes = []
     L.radg_batch_create.argtypes = [ip, vp, ip, C.c_size_t, C.POINTER(vp)]
     L.radg_batch_bytes.argtypes = [ip, vp, ip, C.POINTER(C.c_size_t)]
     L.radg_batch_launch.argtypes = [vp, vp]

@@ -9,7 +9,9 @@ Reference correspondence:
 * :func:`spectrum` -- the wavenumber loop of ``CIRSRTFG_WAVE``
   (``radtran/cirsradg/cirsrtfg_wave.f:361-448``) for one request.
 * :func:`register_ktables` / :func:`read_klist` -- the ``/INTERPK/`` registry
-  that ``read_klist.f`` fills; :func:`register_cia` -- ``/CIATABLE/``;
+  that ``read_klist.f`` fills; :func:`register_lbltables` /
+  :func:`read_klbllist` -- ``/INTERPKLBL/`` (``read_klbllist.f``) and
+  :func:`set_ilbl` -- ``/LBLTABLE/`` ILBL; :func:`register_cia` -- ``/CIATABLE/``;
   :func:`register_dust` -- ``/DUST/`` for :func:`cirsradg_wave` (batched
   requests carry their own dust arrays); :func:`register_parah2` --
   ``/FPTABLE/``; :func:`register_e3` -- ``/E3TABLE/`` for imod 18;
@@ -88,6 +90,44 @@ def ktable_codes(mode: int, itab: int, shape):
 def read_klist(path: str, recl: int = 4) -> None:
     """Register the ``.kta`` files named in a ``.kls`` list."""
     check(lib().radg_read_klist(path.encode(), int(recl)))
+
+
+def register_lbltables(tables) -> None:
+    """Register line-by-line tables (``/INTERPKLBL/``, ILBL = 2): one-ordinate
+    ``KTable`` objects (``kta.lbl_table``) on one wavenumber grid; the P/T
+    grid is the last table's, as ``read_klbllist.f`` leaves it."""
+    t0, tl = tables[0], tables[-1]
+    n = len(tables)
+    for t in tables:
+        if t.ng != 1:
+            raise ValueError("line-by-line tables have one g-ordinate")
+        if (t.np_, t.nt_signed, t.npoint, t.vmin, t.delv) != (t0.np_, t0.nt_signed, t0.npoint,
+                                                               t0.vmin, t0.delv):
+            raise ValueError("line-by-line tables must share their wavenumber and P/T grid sizes")
+    ids = np.array([t.idgas for t in tables], dtype=np.int32)
+    isos = np.array([t.isogas for t in tables], dtype=np.int32)
+    ks = [np.ascontiguousarray(t.k, dtype=np.float64) for t in tables]
+    ptrs = (C.c_void_p * n)(*[k.ctypes.data for k in ks])
+    press = np.ascontiguousarray(tl.press, dtype=np.float64)
+    t2 = getattr(tl, "temp2", None)
+    temp = np.ascontiguousarray(tl.temp if t2 is None else t2, dtype=np.float64)
+    check(lib().radg_register_lbltables(
+        C.c_int(n), _vp(ids), _vp(isos), C.c_int(t0.npoint), C.c_double(t0.vmin),
+        C.c_double(t0.delv), C.c_int(t0.np_), C.c_int(t0.nt_signed), _vp(press), _vp(temp), ptrs))
+
+
+def read_klbllist(path: str, recl: int = 4) -> None:
+    """Register the ``.lta`` files named in an ``.lls`` list."""
+    check(lib().radg_read_klbllist(path.encode(), int(recl)))
+
+
+def set_ilbl(ilbl: int) -> None:
+    """``/LBLTABLE/`` ILBL: 0 k-tables, 2 line-by-line tables."""
+    check(lib().radg_set_ilbl(C.c_int(ilbl)))
+
+
+def get_ilbl() -> int:
+    return lib().radg_get_ilbl()
 
 
 def register_cia(cia) -> None:

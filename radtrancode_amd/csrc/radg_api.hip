@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -87,6 +88,7 @@ struct Table {
 struct TableSet {
   int np = 0, nt = 0, ng = 0;  // nt: temperatures per pressure
   bool t2 = false;             // a temperature grid per pressure (NT < 0)
+  bool lbl = false;            // line-by-line tables (ILBL = 2): ng = 1, one wavenumber grid
   std::vector<double> lnp, tk, g_ord, delg, G;  // tk: [nt], or [np][nt] with t2
   std::vector<Table> tabs;
   long long total = 0;
@@ -170,6 +172,10 @@ struct AuxTables {
 };
 
 std::shared_ptr<TableSet> g_tables;
+// /LBLTABLE/ ILBL (cirsradg_wave.f:317): 2 runs every batch made afterwards
+// on the line-by-line tables g_lbl (get_klblg.f, noverlapg1.f)
+std::shared_ptr<TableSet> g_lbl;
+std::atomic<int> g_ilbl{0};
 std::shared_ptr<CIA> g_cia;
 std::shared_ptr<ParaCIA> g_para;
 std::shared_ptr<E3Tab> g_e3;
@@ -308,6 +314,14 @@ int g_iform = 0;
 // /NLTE_FLAGS/ for the fixed-argument entry points
 int g_inlte = 0;
 double g_nlte_k1 = 0.0, g_nlte_a = 0.0;
+
+// The tables a batch made now runs on (call with g_mu held): the
+// line-by-line tables under ILBL = 2, else the k-tables.
+std::shared_ptr<TableSet> active_tables() { return g_ilbl == 2 ? g_lbl : g_tables; }
+int no_tables() {
+  return g_ilbl == 2 ? fail(RADG_ENOTABLES, "ILBL=2 and no line-by-line tables registered (radg_register_lbltables)")
+                     : fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
+}
 
 bool valid_mode(int mode) {
   return mode == RADG_MODE_FP64 || mode == RADG_MODE_MIXED || mode == RADG_MODE_BF16 ||
@@ -812,6 +826,26 @@ int pack_request(const radg_request &q, const TableSet &ts, int mode, const AuxT
       w.coinc = 1;
       w.ob = w.ob2 = 0;
       w.pad = 0;
+      if (ts.lbl) {
+        // get_klblg.f:167-222: the last table of the gas (read_klbllist.f:
+        // 190-198), slice 1 + INT((v + 0.02 delv - vmin) / delv), which must
+        // lie on the grid (:186-189 read past the table otherwise); never
+        // merged; an all-zero slice gives k = 0
+        const Table *tl = nullptr;
+        for (const Table &t : ts.tabs)
+          if (t.idgas == q.idgas[k] && t.isogas == q.isogas[k]) tl = &t;
+        if (!tl) continue;
+        const double x = (v + 0.02 * tl->delv - tl->vmin) / tl->delv;
+        if (!(x > -1.0 && x < (double)tl->npoint))
+          return fail(RADG_EINPUT, "wavenumber %g outside the line-by-line tables' grid [%g, %g]", v, tl->vmin,
+                      tl->vmin + (tl->npoint - 1) * tl->delv);
+        const long long n1 = (long long)x;  // INT truncates toward zero
+        if (tl->zero[n1]) continue;
+        w.base = tl->off + n1 * ntab;
+        w.ob = w.base / ntab * NG;
+        tbytes += (double)nodes.size() * kbytes;
+        continue;
+      }
       const Table *tb = nullptr;
       for (const Table &t : ts.tabs) {  // read_klist.f:260-290: first covering match
         if (t.idgas != q.idgas[k] || t.isogas != q.isogas[k]) continue;
@@ -1299,6 +1333,7 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
   a.nB = pl.nB;
   a.nC = pl.nC;
   a.ng = ts->ng;
+  a.lbl = ts->lbl ? 1 : 0;
   a.koff = ts->d_koff;
   a.ktab = (mode == RADG_MODE_FP64)   ? (const void *)ts->d_k64
            : (mode == RADG_MODE_BF16) ? (const void *)ts->d_k16
@@ -1366,7 +1401,7 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
 // LDS, slower at the 120-layer workload because a CU then holds one
 // workgroup and waits at its phase barriers.
 std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_request *reqs, char *m,
-                                                  int mode, int ng) {
+                                                  int mode, int ng, bool lbl) {
   const int fp64 = mode == RADG_MODE_FP64;
   const int E = radg_layer_E(ng);
   const char *fz = std::getenv("RADG_FUSED");
@@ -1403,7 +1438,7 @@ std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_req
     ec.ldsA = radg_lds_layer_bytes(ng, maxgas, E, fp64);
     ec.wsbytes = 0;
     ec.ldsF = radg_lds_fused_bytes(mode, ng, E, ec.RL, maxgas, maxnsw, maxblock, &ec.wsbytes);
-    ec.fused = allow_fused && ec.ldsF > 0 && ec.ldsF <= fused_cap;
+    ec.fused = allow_fused && !lbl && ec.ldsF > 0 && ec.ldsF <= fused_cap;
     off += ec.nreq;
     soff += ec.nreq + 1;
     em.push_back(ec);
@@ -1457,21 +1492,26 @@ int radg_set_mode(int mode) {
   if (!valid_mode(mode)) return fail(RADG_EINPUT, "mode=%d", mode);
   g_mode = mode;
   // the table form the mode reads, for the tables registered now
-  std::shared_ptr<TableSet> ts;
+  std::shared_ptr<TableSet> ts, lb;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
+    lb = g_lbl;
   }
-  return ts ? ensure_form(*ts, mode) : RADG_OK;
+  int rc = ts ? ensure_form(*ts, mode) : RADG_OK;
+  if (rc == RADG_OK && lb) rc = ensure_form(*lb, mode);
+  return rc;
 }
 int radg_get_mode(void) { return g_mode; }
 const char *radg_last_error(void) { return g_err.c_str(); }
 
-int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const int *npoint,
-                          const double *vmin, const double *delv, int np, int nt_signed, int ng,
-                          const double *press_atm, const double *temp, const double *g_ord,
-                          const double *delg, const double *const *k,
-                          const double *const *vcen) {
+// The k-table registry (lbl false) or the line-by-line one (lbl true: one
+// g-ordinate of weight 1, every table on one regular wavenumber grid).
+static int register_tables(bool lbl, int ntab, const int *idgas, const int *isogas, const int *npoint,
+                           const double *vmin, const double *delv, int np, int nt_signed, int ng,
+                           const double *press_atm, const double *temp, const double *g_ord,
+                           const double *delg, const double *const *k,
+                           const double *const *vcen) {
   if (ntab < 1) return fail(RADG_EINPUT, "ntab=%d", ntab);
   const bool t2 = nt_signed < 0;
   const int nt = t2 ? -nt_signed : nt_signed;
@@ -1485,6 +1525,7 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
   ts->nt = nt;
   ts->t2 = t2;
   ts->ng = ng;
+  ts->lbl = lbl;
   for (int i = 0; i < np; i++) {
     if (!finite_pos(press_atm[i])) return fail(RADG_EINPUT, "table pressure %d = %g", i + 1, press_atm[i]);
     ts->lnp.push_back(std::log(press_atm[i]));  // read_khead.f:80-85
@@ -1560,9 +1601,52 @@ int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const i
   HIPCHK(hipMemcpy(ts->d_delg64, delg, ng * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ts->d_delg32, dg32.data(), ng * sizeof(float), hipMemcpyHostToDevice));
   std::lock_guard<std::mutex> lk(g_mu);
-  g_tables = ts;
+  if (lbl) g_lbl = ts;
+  else g_tables = ts;
   return RADG_OK;
 }
+
+int radg_register_ktables(int ntab, const int *idgas, const int *isogas, const int *npoint,
+                          const double *vmin, const double *delv, int np, int nt_signed, int ng,
+                          const double *press_atm, const double *temp, const double *g_ord,
+                          const double *delg, const double *const *k,
+                          const double *const *vcen) {
+  return register_tables(false, ntab, idgas, isogas, npoint, vmin, delv, np, nt_signed, ng, press_atm,
+                         temp, g_ord, delg, k, vcen);
+}
+
+// /INTERPKLBL/ (read_klbllist.f:30-198): every table on the wavenumbers
+// vmin + i delv, i < npoint, and one P/T grid; one g-ordinate of weight 1
+// (cirsradg_wave.f:336-339).
+int radg_register_lbltables(int ntab, const int *idgas, const int *isogas, int npoint, double vmin,
+                            double delv, int np, int nt, const double *press_atm, const double *temp,
+                            const double *const *k) {
+  if (ntab < 1) return fail(RADG_EINPUT, "ntab=%d", ntab);
+  if (npoint < 1 || !std::isfinite(vmin) || !std::isfinite(delv) || !(delv > 0))
+    return fail(RADG_EINPUT, "LBL grid npoint=%d vmin=%g delv=%g (get_klblg.f:182-185 needs delv > 0)", npoint,
+                vmin, delv);
+  const std::vector<int> npts(ntab, npoint);
+  const std::vector<double> vmins(ntab, vmin), delvs(ntab, delv);
+  const double one = 1.0;
+  return register_tables(true, ntab, idgas, isogas, npts.data(), vmins.data(), delvs.data(), np, nt, 1,
+                         press_atm, temp, &one, &one, k, nullptr);
+}
+
+// /LBLTABLE/ ILBL: 0 the k-tables, 2 the line-by-line tables; ILBL = 1
+// reaches noverlapg1 with the k-tables' NG, where noverlapg1.f:64-67 stops.
+int radg_set_ilbl(int ilbl) {
+  if (ilbl == 1)
+    return fail(RADG_EUNSUPPORTED, "ILBL=1: noverlapg1.f:64-67 stops unless NG = 1; ILBL=2 is the LBL form");
+  if (ilbl != 0 && ilbl != 2) return fail(RADG_EINPUT, "ilbl=%d", ilbl);
+  std::shared_ptr<TableSet> ts;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ilbl = ilbl;
+    ts = g_lbl;
+  }
+  return (ilbl == 2 && ts) ? ensure_form(*ts, g_mode) : RADG_OK;
+}
+int radg_get_ilbl(void) { return g_ilbl; }
 
 int radg_ktable_codes(int mode, int itab, unsigned short *codes, float *offsets) {
   if (mode != RADG_MODE_BF16 && mode != RADG_MODE_F16) return fail(RADG_EINPUT, "mode=%d is not a 16-bit form", mode);
@@ -1571,7 +1655,7 @@ int radg_ktable_codes(int mode, int itab, unsigned short *codes, float *offsets)
     std::lock_guard<std::mutex> lk(g_mu);
     ts = g_tables;
   }
-  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
+  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
   if (itab < 0 || itab >= (int)ts->tabs.size()) return fail(RADG_EINPUT, "itab=%d", itab);
   int rc = ensure_form(*ts, mode);
   if (rc) return rc;
@@ -1854,10 +1938,10 @@ int radg_batch_bytes(int nreq, const radg_request *reqs, int mode, size_t *bytes
   AuxTables aux;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    ts = g_tables;
+    ts = active_tables();
     aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
   }
-  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered");
+  if (!ts) return no_tables();
   Plan pl;
   int rc = plan_batch(nreq, reqs, mode, *ts, aux, pl);
   if (rc) return rc;
@@ -1880,7 +1964,7 @@ int radg_batch_create(int nreq, const radg_request *reqs, int mode, size_t budge
 namespace {
 // The layout of a plan: its offsets and item counts.  Two plans with the
 // same layout fit the same device block.
-std::vector<long long> plan_sig(const Plan &pl) {
+extern "C++" std::vector<long long> plan_sig(const Plan &pl) {
   std::vector<long long> v = {(long long)pl.total, pl.nA, pl.nB, pl.nC, (long long)pl.o_reqs,
                               (long long)pl.o_inter, (long long)pl.o_out, (long long)pl.o_jac,
                               (long long)pl.o_wout, (long long)pl.o_wjac, (long long)pl.o_err,
@@ -1940,7 +2024,7 @@ int batch_fill(radg_batch *b, const Plan &pl, int nreq, const radg_request *reqs
   b->maxngas = pl.maxngas;
   b->maxnsw = pl.maxnsw;
   b->RL = radg_emission_RL(pl.maxlays);
-  b->em = emission_classes(pl, reqs, b->dmem, mode, ts->ng);
+  b->em = emission_classes(pl, reqs, b->dmem, mode, ts->ng, ts->lbl);
   for (const auto &ec : b->em)
     if (!ec.fused && (ec.ldsA > 160 * 1024 || ec.lds > 160 * 1024))
       return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", ec.ldsA, ec.lds);
@@ -1962,12 +2046,12 @@ int batch_create(int nreq, const radg_request *reqs, int mode, size_t budget, ra
   AuxTables aux;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    ts = g_tables;
+    ts = active_tables();
     cia = g_cia;
     aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
     if (fixed_args && g_iread != 999) aux.solar.reset();
   }
-  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
+  if (!ts) return no_tables();
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
   Plan pl;
   int rc = plan_batch(nreq, reqs, mode, *ts, aux, pl);
@@ -2001,7 +2085,7 @@ int batch_restage(radg_batch *b, int nreq, const radg_request *reqs, int mode, b
   AuxTables aux;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    ts = g_tables;
+    ts = active_tables();
     cia = g_cia;
     aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
     if (fixed_args && g_iread != 999) aux.solar.reset();
@@ -2059,11 +2143,11 @@ int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, doubl
   AuxTables aux;
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    ts = g_tables;
+    ts = active_tables();
     cia = g_cia;
     aux = {g_para, g_e3, g_n2nir, g_solar, g_gascon};
   }
-  if (!ts) return fail(RADG_ENOTABLES, "no k-tables registered (radg_register_ktables)");
+  if (!ts) return no_tables();
   if (!cia) return fail(RADG_ENOTABLES, "no CIA table registered (radg_register_cia)");
   // making a table form allocates: it must exist already
   if ((mode == RADG_MODE_MIXED && !ts->d_k32) || (mode == RADG_MODE_BF16 && !ts->d_k16) ||
@@ -2092,7 +2176,7 @@ int radg_spectrum_noalloc(const radg_request *req, int mode, double *wout, doubl
   a.wgts = wgts;
   a.wjac = wjac;
   a.err = nullptr;
-  return launch_classes(a, emission_classes(pl, req, m, mode, ts->ng), mode, E, st, nullptr, nullptr);
+  return launch_classes(a, emission_classes(pl, req, m, mode, ts->ng, ts->lbl), mode, E, st, nullptr, nullptr);
 }
 
 int radg_batch_launch(radg_batch *b, void *stream) {
@@ -2779,6 +2863,102 @@ int radg_spectrum_r8(const double *dist, const int *inormal, const int *iray, co
 }
 
 // .kta reader (read_khead.f:18-120; regular grids, NT > 0)
+// /INTERPKLBL/ from the .lta files an .lls list names (read_klbllist.f:
+// 30-198, read_klblhead.f:19-75).  Every table must share NP, NT, DELV,
+// NPOINT, VMIN and IREC0, and its pressure and temperature grid ends must
+// lie within 1e-4 of the first table's (:131-168); the grid used is the last
+// file's, as each header read overwrites the one grid.
+int radg_read_klbllist(const char *list_path, int recl) {
+  if (recl != 4 && recl != 8) return fail(RADG_EINPUT, "recl=%d (4 or 8)", recl);
+  std::ifstream kl(list_path);
+  if (!kl) return fail(RADG_EIO, "cannot open %s", list_path);
+  std::vector<std::string> names;
+  std::string line;
+  while (std::getline(kl, line)) {
+    std::string nm;
+    for (char c : line)
+      if (c != ' ' && c != '\t' && c != '\r') nm += c;  // remsp.f
+    if (nm.empty()) break;                               // read_klbllist.f:83: a blank line ends the list
+    // file.f: a '.' among the last four characters starts the extension,
+    // which becomes lta; else .lta is appended
+    size_t dot = std::string::npos;
+    for (size_t i = nm.size(); i-- > (nm.size() > 4 ? nm.size() - 4 : 0);) {
+      if (nm[i] == '/' || nm[i] == ']') break;
+      if (nm[i] == '.') {
+        dot = i;
+        break;
+      }
+    }
+    names.push_back(dot == std::string::npos ? nm + ".lta" : nm.substr(0, dot + 1) + "lta");
+  }
+  if (names.empty()) return fail(RADG_EIO, "%s lists no tables", list_path);
+  std::vector<int> ids, isos;
+  std::vector<double> press, temps;
+  std::vector<std::vector<double>> data;
+  int irec00 = 0, npoint0 = 0, np0 = 0, nt0 = 0;
+  double vmin0 = 0, delv0 = 0, lp1 = 0, lpn = 0, ta1 = 0, ta2 = 0;
+  for (const auto &nm : names) {
+    std::ifstream f(nm, std::ios::binary);
+    if (!f) return fail(RADG_EIO, "cannot open %s", nm.c_str());
+    std::vector<char> raw((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    const size_t nrec = raw.size() / recl;
+    auto gi = [&](size_t r) {
+      int32_t v;
+      std::memcpy(&v, raw.data() + (r - 1) * recl, 4);
+      return (int)v;
+    };
+    auto gr = [&](size_t r) {
+      if (recl == 4) {
+        float v;
+        std::memcpy(&v, raw.data() + (r - 1) * recl, 4);
+        return (double)v;
+      }
+      double v;
+      std::memcpy(&v, raw.data() + (r - 1) * recl, 8);
+      return v;
+    };
+    if (nrec < 8) return fail(RADG_EIO, "%s: truncated header", nm.c_str());
+    const int irec0 = gi(1), npoint = gi(2), np = gi(5), nts = gi(6);
+    const int nt = std::abs(nts);
+    const double vmin = gr(3), delv = gr(4);
+    if (np < 2 || np > RADG_MAXK || nt < 2 || nt > RADG_MAXK || npoint < 1 || irec0 < 1)
+      return fail(RADG_ELIMITS, "%s: np=%d nt=%d npoint=%d irec0=%d", nm.c_str(), np, nts, npoint, irec0);
+    const size_t ntemp = nts < 0 ? (size_t)np * nt : (size_t)nt;
+    const size_t per = (size_t)np * nt;
+    if (8 + np + ntemp > nrec || (size_t)irec0 - 1 + (size_t)npoint * per > nrec)
+      return fail(RADG_EIO, "%s: truncated", nm.c_str());
+    press.clear();
+    temps.clear();
+    for (int j = 0; j < np; j++) press.push_back(gr(9 + j));
+    for (size_t j = 0; j < ntemp; j++) temps.push_back(gr(9 + np + j));
+    for (double p : press)
+      if (!finite_pos(p)) return fail(RADG_EINPUT, "%s: pressure %g", nm.c_str(), p);
+    const double l1 = std::log(press[0]), ln = std::log(press[np - 1]);
+    const double t1 = temps[0], tn = temps[ntemp - 1];
+    if (ids.empty()) {
+      irec00 = irec0, npoint0 = npoint, np0 = np, nt0 = nts, vmin0 = vmin, delv0 = delv;
+      lp1 = l1, lpn = ln, ta1 = t1, ta2 = tn;
+    } else {
+      if (np != np0 || nts != nt0 || delv != delv0 || npoint != npoint0 || vmin != vmin0 || irec0 != irec00)
+        return fail(RADG_EINPUT, "%s: array sizes differ from the first table (read_klbllist.f:131-145)",
+                    nm.c_str());
+      if (std::fabs(l1 - lp1) > 1e-4 || std::fabs(ln - lpn) > 1e-4 || std::fabs(t1 - ta1) > 1e-4 ||
+          std::fabs(tn - ta2) > 1e-4)
+        return fail(RADG_EINPUT, "%s: P/T grid ends differ from the first table (read_klbllist.f:155-168)",
+                    nm.c_str());
+    }
+    ids.push_back(gi(7));
+    isos.push_back(gi(8));
+    std::vector<double> d(per * npoint);
+    for (size_t i = 0; i < d.size(); i++) d[i] = gr(irec0 + i);
+    data.push_back(std::move(d));
+  }
+  std::vector<const double *> ptrs;
+  for (auto &d : data) ptrs.push_back(d.data());
+  return radg_register_lbltables((int)ids.size(), ids.data(), isos.data(), npoint0, vmin0, delv0, np0, nt0,
+                                 press.data(), temps.data(), ptrs.data());
+}
+
 int radg_read_klist(const char *klist_path, int recl) {
   if (recl != 4 && recl != 8) return fail(RADG_EINPUT, "recl=%d (4 or 8)", recl);
   std::ifstream kl(klist_path);

@@ -175,6 +175,7 @@ struct KernelArgs {
   int nreqA, nreqB;
   int wsbytes;                 // fused kernel: LDS scratch bytes per wavefront
   int ng;
+  int lbl;                     // 1: line-by-line tables (ng = 1), the LBL layer kernel
   // tables
   const void *ktab;            // double* (FP64), float* (mixed) or 16-bit codes
   const float *koff;           // 16-bit forms: offset per (slice, ordinate)

@@ -792,6 +792,68 @@ __device__ __forceinline__ float code_value(unsigned short c) {
   else return __half2float(__ushort_as_half(c));
 }
 
+// k and dk/dT of one g-ordinate gg of the table slice at element offset
+// base (16-bit forms: offset row ob), bilinear in (ln P, T) between the
+// corners i1..i4 (get_kg.f:319-376, get_klblg.f:231-301): in ln k when the
+// first ordinate of all four corners is > 0 (KLOG), else in k; on one
+// temperature grid, or per pressure row on each row's own grid (NT < 0).
+// A NaN result becomes 1e-37 (get_kg.f:455-470).
+template <typename T, int BF>
+__device__ __forceinline__ void interp_point(const KernelArgs &a, long long base, long long ob, int i1,
+                                             int i2, int i3, int i4, int gg, T v, T u, T dudt, T u2,
+                                             T dudt2, T &kk, T &dk) {
+  bool klog;
+  T y1, y2, y3, y4;
+  if (BF) {
+    // 16-bit log10(k) codes: ln k comes straight from the code
+    const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + base;
+    klog = !(code_nonpos<BF>(sl[i1]) || code_nonpos<BF>(sl[i2]) || code_nonpos<BF>(sl[i3]) ||
+             code_nonpos<BF>(sl[i4]));
+    const float ln10 = 2.302585092994046f;
+    const float offg = a.koff[ob + gg];
+    auto dec = [&](unsigned short c) -> float {
+      if (code_nonpos<BF>(c)) return klog ? -__builtin_huge_valf() : 0.0f;
+      const float l = (code_value<BF>(c) + offg) * ln10;
+      return klog ? l : expf(l);
+    };
+    y1 = (T)dec(sl[i1 + gg]);
+    y2 = (T)dec(sl[i2 + gg]);
+    y3 = (T)dec(sl[i3 + gg]);
+    y4 = (T)dec(sl[i4 + gg]);
+  } else {
+    const T *sl = reinterpret_cast<const T *>(a.ktab) + base;
+    klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
+    y1 = sl[i1 + gg];
+    y2 = sl[i2 + gg];
+    y3 = sl[i3 + gg];
+    y4 = sl[i4 + gg];
+    if (klog) {
+      y1 = dlog(y1);
+      y2 = dlog(y2);
+      y3 = dlog(y3);
+      y4 = dlog(y4);
+    }
+  }
+  T x, dxdt;
+  if (!a.t2) {
+    x = ((T)1 - v) * ((T)1 - u) * y1 + v * ((T)1 - u) * y2 + v * u * y3 + ((T)1 - v) * u * y4;
+    dxdt = (-((T)1 - v) * y1 - v * y2 + v * y3 + ((T)1 - v) * y4) * dudt;
+  } else {
+    x = ((T)1 - v) * (((T)1 - u) * y1 + u * y4) + v * (((T)1 - u2) * y2 + u2 * y3);
+    dxdt = ((T)1 - v) * (-y1 + y4) * dudt + v * (-y2 + y3) * dudt2;
+  }
+  if (klog) {
+    const T ex = dexp(x);
+    kk = ex;
+    dk = ex * dxdt;
+  } else {
+    kk = x;
+    dk = dxdt;
+  }
+  if (kk != kk) kk = (T)1e-37;
+  if (dk != dk) dk = (T)1e-37;
+}
+
 // ============================================================ layer pass
 // LDS of a layer item (layer_layout in radg_dev.h):
 //   header (per workgroup)  G[NG+1], delg[NG] (FP64), delg[NG] (T)
@@ -910,51 +972,8 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
         const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
         if (wgk.base < 0 || wgk.coinc) {
           T kk = 0, dk = 0;
-          if (wgk.base >= 0) {
-            bool klog;
-            T y1, y2, y3, y4;
-            if (BF) {
-              // 16-bit log10(k) codes: ln k comes straight from the code
-              const unsigned short *sl = reinterpret_cast<const unsigned short *>(a.ktab) + wgk.base;
-              klog = !(code_nonpos<BF>(sl[i1]) || code_nonpos<BF>(sl[i2]) ||
-                       code_nonpos<BF>(sl[i3]) || code_nonpos<BF>(sl[i4]));
-              const float offg = a.koff[wgk.ob + gg];
-              auto dec = [&](unsigned short c) -> float {
-                if (code_nonpos<BF>(c)) return klog ? -__builtin_huge_valf() : 0.0f;
-                const float l = (code_value<BF>(c) + offg) * ln10;
-                return klog ? l : expf(l);
-              };
-              y1 = (T)dec(sl[i1 + gg]);
-              y2 = (T)dec(sl[i2 + gg]);
-              y3 = (T)dec(sl[i3 + gg]);
-              y4 = (T)dec(sl[i4 + gg]);
-            } else {
-              const T *sl = tab + wgk.base;
-              klog = !(sl[i1] <= (T)0 || sl[i2] <= (T)0 || sl[i3] <= (T)0 || sl[i4] <= (T)0);
-              y1 = sl[i1 + gg];
-              y2 = sl[i2 + gg];
-              y3 = sl[i3 + gg];
-              y4 = sl[i4 + gg];
-              if (klog) {
-                y1 = dlog(y1);
-                y2 = dlog(y2);
-                y3 = dlog(y3);
-                y4 = dlog(y4);
-              }
-            }
-            T x, dxdt;
-            bilin(y1, y2, y3, y4, x, dxdt);
-            if (klog) {
-              const T ex = dexp(x);
-              kk = ex;
-              dk = ex * dxdt;
-            } else {
-              kk = x;
-              dk = dxdt;
-            }
-            if (kk != kk) kk = (T)1e-37;
-            if (dk != dk) dk = (T)1e-37;
-          }
+          if (wgk.base >= 0)
+            interp_point<T, BF>(a, wgk.base, wgk.ob, i1, i2, i3, i4, gg, v, u, dudt, u2, dudt2, kk, dk);
           kgn[k * NG + gg] = kk;
           dkgn[k * NG + gg] = dk;
         }
@@ -1494,6 +1513,77 @@ __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
   T *tcon = inter + (long long)NG * L + (long long)NG * (R.nks + 1) * L;
   continuum_layer<T>(a, R, iw, j, tcon, tcon + L, L);
+}
+
+// The layer pass on line-by-line tables (ILBL = 2, one g-ordinate): one
+// (request, wavenumber, layer) item per thread.  Each gas's k and dk/dT
+// from its table slice (get_klblg.f:231-301), then the gases summed in
+// order with noverlapg1.f's branches for zero terms (:69-156): k = sum of
+// amount_i k_i; gas columns k_i, except that a zero running sum before gas
+// m >= 3 zeroes the columns of the gases before it (:123-128); the T column
+// carries the running sum's derivative through the same branches.  FP64
+// sums in every form; results to the rows the layer kernel writes.
+template <typename T, int BF>
+__global__ __launch_bounds__(256) void lbl_kernel(KernelArgs a) {
+  const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= a.nA) return;
+  const int rqc = find_req(a.startA, a.nreqA, item);
+  const DevReq &R = a.reqs[a.reqmapA[rqc]];
+  const int L = R.L, ngas = R.ngas, nks = R.nks;
+  const long long loc = item - a.startA[rqc];
+  const int iw = (int)(loc / L);
+  const int j = (int)(loc - (long long)iw * L);
+  T *ok = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(1, L, nks, R.nparam);
+  T *od = ok + L;
+  const LayerConst lc = a.lay[R.lay0 + j];
+  const int nt = a.nt;
+  const int i1 = nt * (lc.cp - 1) + (lc.ct - 1);
+  const int i2 = nt * lc.cp + (lc.ct2 - 1);
+  const int i3 = nt * lc.cp + lc.ct2;
+  const int i4 = nt * (lc.cp - 1) + lc.ct;
+  const T v = (T)lc.v, u = (T)lc.u, dudt = (T)lc.dudt, u2 = (T)lc.u2, dudt2 = (T)lc.dudt2;
+  double s = 0.0, d = 0.0, k1 = 0.0, dk1 = 0.0, a1 = 0.0;
+  int zmark = 0;
+  for (int k = 0; k < ngas; k++) {
+    const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
+    T kk = 0, dk = 0;
+    if (wgk.base >= 0) interp_point<T, BF>(a, wgk.base, wgk.ob, i1, i2, i3, i4, 0, v, u, dudt, u2, dudt2, kk, dk);
+    const double kd = (double)kk, dkd = (double)dk;
+    const double am = a.amo[R.lg0 + (long long)j * ngas + k];
+    if (k == 0) {  // the first pair waits for gas 2 (:76-108)
+      k1 = kd;
+      dk1 = dkd;
+      a1 = am;
+      s = kd * am;  // one gas: the loop never runs; k = amount k
+      d = dkd * am;
+    } else if (k == 1) {
+      if (k1 == 0.0) {
+        s = kd * am;
+        d = dkd * am;
+      } else if (kd == 0.0) {
+        s = k1 * a1;
+        d = dk1 * a1;
+      } else {
+        s = k1 * a1 + kd * am;
+        d = dk1 * a1 + dkd * am;
+      }
+    } else if (s == 0.0) {  // :123-130
+      s = kd * am;
+      d = dkd * am;
+      zmark = k;
+    } else if (kd != 0.0) {  // :141-146 (kd = 0: :134-138, the sum unchanged)
+      s = s + kd * am;
+      d = d + dkd * am;
+    }
+    const int ks = R.kslot[k];
+    if (ks >= 0) od[(long long)ks * L + j] = (T)(kd == 0.0 ? 0.0 : kd);
+  }
+  for (int k = 0; k < zmark; k++) {
+    const int ks = R.kslot[k];
+    if (ks >= 0) od[(long long)ks * L + j] = (T)0;
+  }
+  od[(long long)nks * L + j] = (T)d;
+  ok[j] = (T)s;
 }
 
 // ========================================================= emission kernel
@@ -2122,6 +2212,19 @@ hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t ld
     else hipLaunchKernelGGL(cia_kernel<float>, dim3(nb), dim3(256), 0, st, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+  }
+  if (a.lbl) {  // line-by-line tables: one thread per item, no LDS
+    if (a.ng != 1) return hipErrorInvalidValue;
+    if (a.nA == 0) return hipSuccess;
+    const unsigned nb = (unsigned)((a.nA + 255) / 256);
+    switch (mode) {
+      case 0: hipLaunchKernelGGL((lbl_kernel<double, 0>), dim3(nb), dim3(256), 0, st, a); break;
+      case 1: hipLaunchKernelGGL((lbl_kernel<float, 0>), dim3(nb), dim3(256), 0, st, a); break;
+      case 2: hipLaunchKernelGGL((lbl_kernel<float, 1>), dim3(nb), dim3(256), 0, st, a); break;
+      case 3: hipLaunchKernelGGL((lbl_kernel<float, 2>), dim3(nb), dim3(256), 0, st, a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
   }
   if (mode == 0) {
     switch (E) {

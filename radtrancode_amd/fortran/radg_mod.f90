@@ -24,6 +24,7 @@ module radg
   public :: radg_batch_device_outputs
   public :: radg_oe_create, radg_oe_cost, radg_oe_propose, radg_oe_destroy
   public :: radg_cirsconv, radg_read_cia, radg_read_cia_f
+  public :: radg_read_klbllist, radg_read_klbllist_f, radg_set_ilbl
 
   ! radg_request of include/radg.h: sizes, then C addresses (c_loc) of the
   ! caller's arrays in the compact 0-based layout the header documents.
@@ -90,6 +91,19 @@ module radg
       import :: c_int, c_char
       character(kind=c_char), dimension(*) :: path
       integer(c_int), value :: recl
+    end function
+
+    ! /INTERPKLBL/ from the .lta files an .lls list names (read_klbllist.f)
+    integer(c_int) function radg_read_klbllist(path, recl) bind(C, name="radg_read_klbllist")
+      import :: c_int, c_char
+      character(kind=c_char), dimension(*) :: path
+      integer(c_int), value :: recl
+    end function
+
+    ! /LBLTABLE/ ILBL: 0 k-tables, 2 line-by-line tables
+    integer(c_int) function radg_set_ilbl(ilbl) bind(C, name="radg_set_ilbl")
+      import :: c_int
+      integer(c_int), value :: ilbl
     end function
 
     integer(c_int) function radg_read_cia(path, dnu) bind(C, name="radg_read_cia")
@@ -295,6 +309,13 @@ contains
     character(len=*), intent(in) :: path
     integer, intent(in) :: recl
     radg_read_klist_f = radg_read_klist(trim(path) // c_null_char, int(recl, c_int))
+  end function
+
+  ! Register the .lta files listed in an .lls file (read_klbllist.f).
+  integer function radg_read_klbllist_f(path, recl)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: recl
+    radg_read_klbllist_f = radg_read_klbllist(trim(path) // c_null_char, int(recl, c_int))
   end function
 
   integer function radg_read_cia_f(path, dnu)

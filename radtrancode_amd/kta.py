@@ -112,3 +112,84 @@ def read_kta(path: str, recl: int = 4) -> KTable:
     return KTable(idgas=gi(9), isogas=gi(10), vmin=vmin, delv=delv, fwhm=fwhm,
                   press=press, temp=temp, g_ord=g_ord, delg=delg, k=data, temp2=temp2,
                   vcen=vcen)
+
+
+# ------------------------------------------------------------------ .lta
+# Line-by-line tables (/LBLTABLE/ ILBL = 2), read by
+# radtran/cirsrad/read_klblhead.f:1-75 and written by
+# radtran/radtran/calc_lbltable.f:228-368:
+#
+#   record 1 IREC0, 2 NPOINT (integer); 3 VMIN, 4 DELV (real);
+#   5 NP, 6 NT, 7 IDGAS, 8 ISOGAS (integer); 9 .. PRESS(NP) in atm, then
+#   TEMP(NT) or TEMP2(NP,|NT|) (J fastest); IREC0 = 12 + NP + NP*|NT| (NT < 0)
+#   or 12 + NP + NT; from IREC0, k * 1e20 per wavenumber: NP*|NT| values,
+#   T fastest, then P.
+#
+# An LBL table is held as a KTable with one g-ordinate of weight 1 (the
+# ng = 1, delg = 1 that cirsradg_wave.f:336-339 sets).
+
+
+def lbl_table(idgas, isogas, vmin, delv, press, temp, k, temp2=None) -> KTable:
+    """An LBL table as a one-ordinate :class:`KTable`; ``k`` is
+    ``[npoint, np, nt]`` (k * 1e20)."""
+    k = np.asarray(k, dtype=np.float64)
+    return KTable(idgas=int(idgas), isogas=int(isogas), vmin=float(vmin), delv=float(delv),
+                  fwhm=0.0, press=np.asarray(press, dtype=np.float64),
+                  temp=np.asarray(temp, dtype=np.float64), g_ord=np.array([1.0]),
+                  delg=np.array([1.0]), k=k.reshape(k.shape + (1,)),
+                  temp2=None if temp2 is None else np.asarray(temp2, dtype=np.float64))
+
+
+def write_lta(path: str, kt: KTable, recl: int = 4) -> None:
+    """Write a one-ordinate table as a ``.lta`` file with ``recl``-byte records."""
+    if recl not in (4, 8):
+        raise ValueError("recl must be 4 or 8")
+    if kt.ng != 1:
+        raise ValueError("an .lta table has one g-ordinate")
+    rdt = np.float32 if recl == 4 else np.float64
+    np_, nt = kt.np_, kt.nt
+    ntemp = np_ * nt if kt.temp2 is not None else nt
+    irec0 = 10 + np_ + ntemp + 2
+    head = np.zeros((irec0 - 1, recl), dtype=np.uint8)
+    for r, v in {1: irec0, 2: kt.npoint, 5: np_, 6: kt.nt_signed, 7: kt.idgas, 8: kt.isogas}.items():
+        head[r - 1, :4] = np.frombuffer(np.int32(v).tobytes(), dtype=np.uint8)
+    temps = np.asarray(kt.temp2).reshape(-1) if kt.temp2 is not None else kt.temp
+    reals = {3: kt.vmin, 4: kt.delv}
+    for j in range(np_):
+        reals[9 + j] = kt.press[j]
+    for j in range(ntemp):
+        reals[9 + np_ + j] = temps[j]
+    for r, v in reals.items():
+        head[r - 1, :] = np.frombuffer(rdt(v).tobytes(), dtype=np.uint8)
+    with open(path, "wb") as f:
+        f.write(head.tobytes())
+        f.write(np.ascontiguousarray(kt.k[..., 0], dtype=rdt).tobytes())
+
+
+def read_lta(path: str, recl: int = 4) -> KTable:
+    """Read a ``.lta`` file into a one-ordinate :class:`KTable`."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    if raw.size % recl:
+        raise ValueError(f"{path}: size is not a multiple of {recl}")
+    rec = raw.reshape(-1, recl)
+    rdt = np.float32 if recl == 4 else np.float64
+
+    def gi(r):
+        return int(rec[r - 1, :4].view(np.int32)[0])
+
+    def gr(r):
+        return float(rec[r - 1].view(rdt)[0])
+
+    irec0, npoint, np_, nts = gi(1), gi(2), gi(5), gi(6)
+    nt = abs(nts)
+    press = np.array([gr(9 + j) for j in range(np_)])
+    r = 9 + np_
+    temp2 = None
+    if nts < 0:
+        temp2 = np.array([gr(r + j) for j in range(np_ * nt)]).reshape(np_, nt)
+        temp = temp2[0].copy()
+    else:
+        temp = np.array([gr(r + j) for j in range(nt)])
+    data = rec[irec0 - 1: irec0 - 1 + npoint * np_ * nt].copy().view(rdt).reshape(
+        npoint, np_, nt).astype(np.float64)
+    return lbl_table(gi(7), gi(8), gr(3), gr(4), press, temp, data, temp2)

@@ -360,6 +360,24 @@ def make_para_cia(seed=6, dnu=1.0, npara=24, other_pairs=(0, 3, 4)) -> ParaCIATa
     return ParaCIATable(dnu=float(dnu), temps=temps, fp=fp, kh2h2=kh2h2, kh2he=kh2he, kcia5=kcia5)
 
 
+def make_lbltables(ids, vmin, delv, npoint, np_=20, nt=10, pmin_bar=5e-5, pmax_bar=20.0,
+                   tmin=40.0, tmax=220.0, seed=12, mu=-23.0, sigma=2.5) -> List[KTable]:
+    """Line-by-line tables (the ``.lta`` content, one g-ordinate of weight
+    1), all on one wavenumber and P/T grid as read_klbllist.f requires:
+    log10(k_phys) ~ N(mu, sigma) per (gas, wavenumber, P, T), stored as
+    ``k_phys * 1e20``."""
+    from .kta import lbl_table
+
+    rng = np.random.default_rng(seed)
+    press = np.geomspace(pmin_bar, pmax_bar, np_) / BAR_PER_ATM
+    temp = np.linspace(tmin, tmax, nt)
+    out = []
+    for gid in ids:
+        lk = rng.normal(mu, sigma, size=(npoint, np_, nt))
+        out.append(lbl_table(int(gid), 0, vmin, delv, press, temp, np.power(10.0, lk + 20.0)))
+    return out
+
+
 def make_atmosphere(nlayer=120, nkgas=3, seed=1, p_bot_bar=10.0,
                     p_top_bar=1e-4, vmr_lo=1e-9, vmr_hi=1e-3, q_h2=0.86,
                     q_he=0.14, mu_bar=2.3, grav=24.79, tnoise=2.0):

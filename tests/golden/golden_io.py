@@ -186,3 +186,9 @@ def load_cia(path=None) -> CIATable:
 def case_paths():
     return sorted(os.path.join(HERE, f) for f in os.listdir(HERE)
                   if f.startswith("case_") and f.endswith(".npz"))
+
+
+def lbl_case_paths():
+    """The line-by-line (ILBL = 2) cases: tables of one g-ordinate."""
+    return sorted(os.path.join(HERE, f) for f in os.listdir(HERE)
+                  if f.startswith("lbl_") and f.endswith(".npz"))

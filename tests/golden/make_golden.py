@@ -74,10 +74,10 @@ def slice_tables(tables, i0, n):
     return out
 
 
-def run_ref(tables, cia, req):
+def run_ref(tables, cia, req, ilbl=0):
     outs = {}
     for prec in (64, 32):
-        rr = ref.RefRunner(tables, cia, req, prec=prec)
+        rr = ref.RefRunner(tables, cia, req, prec=prec, ilbl=ilbl)
         o, g, d = rr.spectrum()
         rr.close()
         outs[f"ref{prec}_output"] = o
@@ -452,6 +452,69 @@ def kta_forms_inputs(seed, t2=False, delv=None, nwave=7, nlayer=30, ng=10, nkgas
     return t, r
 
 
+def lbl_inputs(seed, nwave=6, nlayer=30, nkgas=3, t2=False, npoint=40, delv=0.5, v0=800.0):
+    """A request on line-by-line tables (ILBL = 2): nkgas LBL gases then H2
+    and He, which have no table; wavenumbers on grid points."""
+    ids = list(synth.KGAS_IDS[:nkgas])
+    t = synth.make_lbltables(ids, v0, delv, npoint, np_=10, nt=6, seed=seed)
+    if t2:  # NT < 0: each pressure's temperature grid shifted and stretched
+        base = np.linspace(40.0, 220.0, 6)
+        grid = np.array([base * (0.9 + 0.02 * i) + 4.0 * i for i in range(10)])
+        for kt in t:
+            kt.temp2 = grid.copy()
+            kt.temp = grid[0].copy()
+    r = synth.make_request(nlayer=nlayer, nkgas=nkgas, nwave=nwave, seed=seed, table_vmin=v0,
+                           table_delv=delv * 5, wave_offset=1)
+    return t, r
+
+
+def lbl_cases():
+    """The ILBL = 2 cases (get_klblg.f + noverlapg1.f), saved as lbl_*.npz.
+    Their ref64 T column is not the routine's intended one (noverlapg1.f:108
+    stores the first pair's derivative into grad; later gases read the
+    first pair's dk/dT); the tests leave it out."""
+    t, r = lbl_inputs(31)
+    yield "basic", t, r, "L=30, 3 LBL gases + H2/He without tables, on grid points"
+
+    t, r = lbl_inputs(32, t2=True)
+    yield "t2", t, r, "LBL tables with per-pressure temperature grids (NT < 0)"
+
+    # between grid points (the slice at or below v + 0.02 delv), just below
+    # the grid (INT truncates toward zero: slice 1) and past its last point
+    t, r = lbl_inputs(33, npoint=40, delv=0.5)
+    rng = np.random.default_rng(33)
+    vb = 800.0 + 0.5 * np.array([3, 7, 11, 20]) + rng.uniform(0.05, 0.45, 4)
+    r.vwave = np.concatenate([vb, [800.0 - 0.3, 800.0 + 0.5 * 39 + 0.4]])
+    yield "between", t, r, "LBL between grid points, below the first and past the last"
+
+    # zero terms: an all-zero slice of gas 1 (k1 = 0 branch), a zero slice of
+    # gas 2 with gas 1 of zero amount in some layers (a zero running sum
+    # before gas 3 zeroes the columns before it, noverlapg1.f:123-128),
+    # corners <= 0 (KLOG false, linear interpolation)
+    t, r = lbl_inputs(34, nwave=6)
+    iw = ((r.vwave - t[0].vmin) / t[0].delv + 0.02).astype(int)
+    t[0].k[iw[1]] = 0.0
+    t[1].k[iw[2]] = 0.0
+    t[1].k[iw[3]] = 0.0
+    t[0].k[iw[3], :, :3] = 0.0
+    t[2].k[iw[4], 2:5, :] = 0.0
+    r.amount = r.amount.copy()
+    r.amount[5:15, 0] = 0.0
+    yield "zeros", t, r, "LBL zero slices, zero corners and zero amounts"
+
+    # two tables for gas 2 (read_klbllist.f:190-198: the last one listed
+    # wins) whose interior pressures differ from the others', so the shared
+    # P/T grid is the last file's (read_klblhead.f fills one grid per file)
+    t, r = lbl_inputs(35)
+    extra = copy.deepcopy(t[1])
+    extra.k = extra.k * 3.0
+    last = copy.deepcopy(t[2])
+    last.press = last.press.copy()
+    last.press[3:7] *= 1.3
+    t = [t[0], t[1], extra, last]
+    yield "last_table", t, r, "LBL: two tables for one gas, the last file's P/T grid"
+
+
 def cia_fixture(name="isotest", dnu=1.0):
     """raddata/<name>.tab as a fixture: its temperatures and kcia as the
     file holds them (REAL*4), read as ciaread.f:77-89 reads it."""
@@ -669,6 +732,14 @@ def main(names=()):
     if not names or "gascon" in names:
         gascon_tables()
         np.savez_compressed(os.path.join(HERE, "gascon_funcs.npz"), **gascon_vectors())
+    for name, t, r, note in lbl_cases():
+        if names and "lbl_" + name not in names:
+            continue
+        outs = run_ref(t, cia, r, ilbl=2)
+        path = os.path.join(HERE, f"lbl_{name}.npz")
+        save_case(path, t, r, outs, note)
+        print(f"lbl_{name:12s} {os.path.getsize(path) / 1024:8.1f} KiB  "
+              f"|R|max {np.abs(outs['ref64_output']).max():.3e}")
     for name, t, r, note in cases():
         if names and name not in names:
             continue

@@ -23,6 +23,8 @@
 
 // launchers (radg_kernels.hip)
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64);
+int radg_layer_waves(int ng, int ngas, int E, int fp64);
+int radg_layer_stride(int ng, int ngas, int E, int fp64);
 size_t radg_lds_emission_bytes(int RL, int nsw, int fp64);
 int radg_layer_E(int ng);
 int radg_emission_RL(int maxlays);
@@ -970,6 +972,7 @@ struct radg_batch {
     const int *reqmap;
     const long long *startA, *start, *startW;  // class-local item prefix sums
     size_t lds, ldsA;                          // emission, layer kernels
+    int lwaves, lwstride;                      // layer kernel: items per workgroup, LDS per item
     bool fused;
     size_t ldsF;                               // fused kernel
     int wsbytes;
@@ -1333,6 +1336,12 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
   a.nB = pl.nB;
   a.nC = pl.nC;
   a.ng = ts->ng;
+  {
+    // RADG_REBIN=records keeps every overlap round on the record rebin (the
+    // general form the row-count rebin falls back to)
+    const char *rb = std::getenv("RADG_REBIN");
+    a.rebin = (rb && std::strcmp(rb, "records") == 0) ? 0 : 1;
+  }
   a.lbl = ts->lbl ? 1 : 0;
   a.koff = ts->d_koff;
   a.ktab = (mode == RADG_MODE_FP64)   ? (const void *)ts->d_k64
@@ -1436,6 +1445,8 @@ std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_req
     ec.startW = reinterpret_cast<const long long *>(m + pl.o_clsstartW) + soff;
     ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
     ec.ldsA = radg_lds_layer_bytes(ng, maxgas, E, fp64);
+    ec.lwaves = radg_layer_waves(ng, maxgas, E, fp64);
+    ec.lwstride = radg_layer_stride(ng, maxgas, E, fp64);
     ec.wsbytes = 0;
     ec.ldsF = radg_lds_fused_bytes(mode, ng, E, ec.RL, maxgas, maxnsw, maxblock, &ec.wsbytes);
     ec.fused = allow_fused && !lbl && ec.ldsF > 0 && ec.ldsF <= fused_cap;
@@ -1466,6 +1477,8 @@ int launch_classes(const KernelArgs &base, const std::vector<radg_batch::EmClass
         return fail(RADG_ELIMITS, "LDS need %zu/%zu bytes exceeds 160 KiB", ec.ldsA, ec.lds);
       a.startA = ec.startA;
       a.nA = ec.nA;
+      a.lwaves = ec.lwaves;
+      a.lwstride = ec.lwstride;
       HIPCHK(radg_launch_layer_lds(a, mode, E, ec.ldsA, st));
     }
   }

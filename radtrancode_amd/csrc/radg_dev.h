@@ -125,21 +125,68 @@ __host__ __device__ inline long long inter_block(int ng, int L, int nks, int npa
 }
 
 // LDS of the layer pass.  A workgroup holds a header shared by its
-// wavefronts, G [ng+1] and delg [ng] in FP64, delg [ng] in the compute type
-// and the FP64 weight of each of the 64*E sort slots, then one scratch area
-// per wavefront for the layer item it is on.
+// wavefronts, then one scratch area per wavefront for the layer item it is
+// on.  Header: G [ng+3] (the g boundaries, G[ng] forced >= 1, then two +inf
+// so that a search may look two edges past the last), delg [ng] in FP64 and
+// in the compute type, the FP64 weight of each of the 64*E sort slots, and
+// Gw [ng+1], the unforced cumulative weights sum_{j<c} delg_j.
 __host__ __device__ inline int lds_round16(int b) { return (b + 15) & ~15; }
+struct HeaderLayout {
+  int G, dl64, dlg, w64, Gw, total;
+};
+__host__ __device__ inline HeaderLayout header_layout(int ng, int E, int ts) {
+  HeaderLayout H;
+  int o = 0;
+  H.G = o;    o += lds_round16(8 * (ng + 3));
+  H.dl64 = o; o += lds_round16(8 * ng);
+  H.dlg = o;  o += lds_round16(ts * ng);
+  H.w64 = o;  o += 8 * RADG_WAVE * E;
+  H.Gw = o;   o += lds_round16(8 * (ng + 1));
+  H.total = o;
+  return H;
+}
 __host__ __device__ inline int layer_header_bytes(int ng, int E, int ts) {
-  return lds_round16(8 * (ng + 1)) + lds_round16(8 * ng) + lds_round16(ts * ng) + 8 * RADG_WAVE * E;
+  return header_layout(ng, E, ts).total;
 }
 // Sort slots of an overlap round: slot s = l*E + q of lane l holds element
 // (i, j) = (l / lpr, (l % lpr) * E + q), lpr = ceil(ng / E) lanes per row of
 // the running distribution; slots past a row's ng elements, and lanes past
 // ng * lpr, are padding.  s grows with (i, j) in lexicographic order.
 __host__ __device__ inline int layer_lpr(int ng, int E) { return (ng + E - 1) / E; }
+// Row-count rebin of an overlap round (E >= 4): the bin edges are shared
+// by ngrp = 64 / ng groups of ng lanes, each group summing over about
+// ng / ngrp rows of the running distribution.
+__host__ __device__ inline int count_groups(int ng) { return RADG_WAVE / ng; }
+// Its scratch, inside `uni` (byte offsets from uni):
+//   rrec [ng+1][ngas+3] per row i: delg_i, delg_i K_i, delg_i dK_i/dT, then
+//                       delg_i times each gas payload (FP64); row ng zero
+//   recI [ng]           sort slot of the element straddling each edge
+//   gam  [64 E]         bytes: the number of bin edges at or below the
+//                       cumulative weight through each sort slot's element,
+//                       indexed by the element's slot (0x7f for padding)
+//   ctab [ng+1][2]      sum_{j<c} delg_j k2_j and delg_j dk2_j/dT (FP64)
+//   part [ngrp-1][ng][5] the other groups' partial sums (FP64), over gam and
+//                       ctab once the counts are summed
+struct CountLayout {
+  int rrec, recI, gam, ctab, part, total;
+};
+__host__ __device__ inline CountLayout count_layout(int ng, int ngas, int E) {
+  CountLayout C;
+  int o = 0;
+  C.rrec = o; o += lds_round16(8 * (ng + 1) * (ngas + 3));
+  C.recI = o; o += lds_round16(4 * ng);
+  C.gam = o;
+  C.part = o;
+  const int tabs = lds_round16(RADG_WAVE * E) + lds_round16(16 * (ng + 1));
+  const int part = lds_round16(40 * (count_groups(ng) - 1) * ng);
+  C.ctab = o + lds_round16(RADG_WAVE * E);
+  o += (tabs > part) ? tabs : part;
+  C.total = o;
+  return C;
+}
 // Byte offsets within one wavefront's scratch (ts: bytes of the compute
 // type).  `uni` is shared, in time, by the rankk merge buffers, the exact
-// FP32 keys of an overlap sort and its per-bin records.
+// FP32 keys of an overlap sort and its rebin's records or row-count tables.
 struct LayerLayout {
   int sSt, sGd, kgn, dkgn, bufA, bufB, sE, uni, total;
 };
@@ -157,6 +204,7 @@ __host__ __device__ inline LayerLayout layer_layout(int ng, int ngas, int E, int
   const int merge = 3 * RADG_WAVE * ts + (ts == 4 ? 4 * RADG_WAVE : 0);
   if (u < merge) u = merge;
   if (ts == 4 && u < 4 * RADG_WAVE * E) u = 4 * RADG_WAVE * E;  // exact keys
+  if (E >= 4 && u < count_layout(ng, ngas, E).total) u = count_layout(ng, ngas, E).total;
   L.uni = o;  o += lds_round16(u);
   L.total = o;
   return L;
@@ -174,7 +222,9 @@ struct KernelArgs {
   const int *reqmapA, *reqmapB;
   int nreqA, nreqB;
   int wsbytes;                 // fused kernel: LDS scratch bytes per wavefront
+  int lwaves, lwstride;        // layer kernel: wavefronts (items) per workgroup, scratch bytes each
   int ng;
+  int rebin;                   // overlap rebin: 1 row counts where they apply, 0 records only
   int lbl;                     // 1: line-by-line tables (ng = 1), the LBL layer kernel
   // tables
   const void *ktab;            // double* (FP64), float* (mixed) or 16-bit codes

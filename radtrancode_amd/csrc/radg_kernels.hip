@@ -882,22 +882,265 @@ __device__ __forceinline__ void wave_sync() {
 template <typename T, int E>
 __device__ __forceinline__ void layer_header(const KernelArgs &a, char *hdr, int tid, int nthr) {
   const int NG = a.ng;
-  double *G = reinterpret_cast<double *>(hdr);
-  double *dl64 = reinterpret_cast<double *>(hdr + lds_round16(8 * (NG + 1)));
-  T *dlg = reinterpret_cast<T *>(hdr + lds_round16(8 * (NG + 1)) + lds_round16(8 * NG));
-  double *w64 = reinterpret_cast<double *>(reinterpret_cast<char *>(dlg) + lds_round16((int)sizeof(T) * NG));
+  const HeaderLayout H = header_layout(NG, E, (int)sizeof(T));
+  double *G = reinterpret_cast<double *>(hdr + H.G);
+  double *dl64 = reinterpret_cast<double *>(hdr + H.dl64);
+  T *dlg = reinterpret_cast<T *>(hdr + H.dlg);
+  double *w64 = reinterpret_cast<double *>(hdr + H.w64);
+  double *Gw = reinterpret_cast<double *>(hdr + H.Gw);
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
                                    : reinterpret_cast<const T *>(a.delg32);
-  for (int i = tid; i <= NG; i += nthr) G[i] = a.G[i];
+  for (int i = tid; i < NG + 3; i += nthr) G[i] = (i <= NG) ? a.G[i] : dinf<double>();
   for (int i = tid; i < NG; i += nthr) {
     dl64[i] = a.delg64[i];
     dlg[i] = delg[i];
+  }
+  // unforced cumulative weights, summed in order as rankg.f:59-62 does
+  if (tid == 0) {
+    double s = 0.0;
+    Gw[0] = 0.0;
+    for (int i = 0; i < NG; i++) {
+      s += a.delg64[i];
+      Gw[i + 1] = s;
+    }
   }
   const int lpr = layer_lpr(NG, E);
   for (int s = tid; s < W * E; s += nthr) {
     const int l = s / E, i = l / lpr, jx = (l - i * lpr) * E + (s & (E - 1));
     w64[s] = (i < NG && jx < NG) ? a.delg64[i] * a.delg64[jx] : 0.0;
   }
+}
+
+// ---- row-count rebin of one overlap round --------------------------------
+// rankg.f:110-152 walks the sorted elements, adds whole elements to bin ig
+// while their cumulative weight stays below the edge G[ig+1], splits the
+// element that reaches it (frac below, 1 - frac above) and divides each
+// closed bin by its accumulated weight.  So bin b-1 is Q(b) - Q(b-1) over
+// its weight, with Q(b) the weighted sum over the elements wholly below
+// edge b plus frac_b times the straddler's weighted value.
+//
+// With the new gas's ordinates non-decreasing (k2_j <= k2_{j+1}) the keys
+// of row i, K_i + a2 k2_j, leave the sort in column order, so the elements
+// of row i wholly below edge b are its first c_bi columns: c_bi counts the
+// row's elements whose gamma (the number of edges at or below the
+// cumulative weight through the element) is below b.  Every value is a row
+// term plus a column term (noverlapg.f:117-128, 175-187), hence
+//   Q(b) = sum_i delg_i (R_i Gw[c_bi] + C[c_bi]) + frac_b w_s v_s,
+// with Gw and C prefix sums over the columns.  Lanes own the edges (ngrp
+// groups of NG lanes, each over a share of the rows); c_bi is four bytes
+// of gamma at a time: x + (0x80 - b) per byte sets the byte's top bit
+// exactly when gamma >= b.  All sums are FP64.  Returns false, leaving nxt
+// untouched, when an element's weight reaches past two edges: rankg.f
+// closes only one bin per element there, which the record rebin follows.
+template <int E> struct CountShape;   // ND: bytes / 4 of a row, RPG: rows per group
+template <> struct CountShape<4> { static constexpr int ND = 4, RPG = 4; };
+template <> struct CountShape<8> { static constexpr int ND = 6, RPG = 7; };
+template <> struct CountShape<16> { static constexpr int ND = 8, RPG = 16; };
+
+// The form applies when a row's slots are exactly ND words (E = 4: 13-16
+// g-ordinates, E = 8: 17-21, E = 16: 22-32).
+template <int E>
+__device__ __forceinline__ bool counts_fit(int NG) {
+  if constexpr (E < 4 || E > 16) {
+    return false;
+  } else {
+    const int lpr = layer_lpr(NG, E), ngrp = count_groups(NG);
+    return NG <= 32 && ngrp >= 1 && lpr * E == 4 * CountShape<E>::ND &&
+           (NG + ngrp - 1) / ngrp <= CountShape<E>::RPG;
+  }
+}
+
+template <typename T, int E>
+__device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const int r, const T a2,
+                                             const int (&idx)[E], const T (&key)[E], const T *rows,
+                                             T *nxt, const T *k2, const T *d2, const double *G,
+                                             const double *dl64, const double *w64,
+                                             const double *Gw, char *uni, double *recGs,
+                                             double *recGe, int lane_in) {
+  constexpr int ND = CountShape<E>::ND, RPG = CountShape<E>::RPG;
+  // an opaque copy of the lane number keeps the lane-dependent indices
+  // below from being hoisted out of the round loop (and spilled there)
+  int lane = lane_in;
+  asm volatile("" : "+v"(lane));
+  const int NS = ngas + 2, RR = ngas + 3;
+  const int lpr = layer_lpr(NG, E);
+  const CountLayout cl = count_layout(NG, ngas, E);
+  double *rrec = reinterpret_cast<double *>(uni + cl.rrec);
+  unsigned char *gam = reinterpret_cast<unsigned char *>(uni + cl.gam);
+  double *ctab = reinterpret_cast<double *>(uni + cl.ctab);
+  double *part = reinterpret_cast<double *>(uni + cl.part);
+  int *recI = reinterpret_cast<int *>(uni + cl.recI);
+
+  // cumulative weight through each slot, gdist of rankg.f:104-108
+  double gd[E];
+  double run = 0.0;
+#pragma unroll
+  for (int q = 0; q < E; q++) {
+    run += w64[idx[q]];
+    gd[q] = run;
+  }
+  {
+    const double ex = wave_shr1(wave_incl_sum(run));
+#pragma unroll
+    for (int q = 0; q < E; q++) gd[q] += ex;
+  }
+  const double gprev = wave_shr1(gd[E - 1]);
+  // gamma: edges G[1..NG] at or below the lane's start, then one slot at a
+  // time; a slot whose end reaches the next edge straddles it (rankg.f:120)
+  int g;
+  {
+    int lo = 0, hi = NG;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (G[mid] <= gprev) lo = mid;
+      else hi = mid - 1;
+    }
+    g = lo;
+  }
+  bool multi = false;
+#pragma unroll
+  for (int q = 0; q < E; q++) {
+    const double x = gd[q];
+    if (G[g + 1] <= x) {
+      recI[g] = idx[q];
+      recGs[g] = (q == 0) ? gprev : gd[q - 1];
+      recGe[g] = x;
+      multi |= (G[g + 2] <= x);
+      g++;
+    }
+    gam[idx[q]] = (key[q] == dinf<T>()) ? (unsigned char)0x7f : (unsigned char)g;
+  }
+  if (__any(multi)) return false;
+  const int gtot = __shfl(g, W - 1);  // edges at or below the total weight
+
+  // column prefix tables of the new gas
+  {
+    const bool jl = lane < NG;
+    const double xk = jl ? dl64[lane] * (double)k2[lane] : 0.0;
+    const double xd = jl ? dl64[lane] * (double)d2[lane] : 0.0;
+    const double ik = wave_incl_sum(xk), id = wave_incl_sum(xd);
+    if (jl) {
+      ctab[2 * lane + 2] = ik;
+      ctab[2 * lane + 3] = id;
+    }
+    if (lane == 0) {
+      ctab[0] = 0.0;
+      ctab[1] = 0.0;
+    }
+  }
+  // row terms times the row weight: delg_i, delg_i K_i, delg_i dK_i/dT,
+  // delg_i P_si for the gases already mixed; row NG is all zero and stands
+  // in for the rows a group does not have
+  for (int i = lane; i <= NG; i += W) {
+    const bool ri = i < NG;
+    const double w = ri ? dl64[i] : 0.0;
+    double *rr = rrec + i * RR;
+    rr[0] = w;
+    rr[1] = ri ? w * (double)rows[i] : 0.0;
+    rr[2] = ri ? w * (double)rows[(NS - 1) * NG + i] : 0.0;
+    for (int s = 0; s < r; s++) rr[3 + s] = ri ? w * (double)rows[(1 + s) * NG + i] : 0.0;
+  }
+  wave_sync();
+
+  // lane = edge bl + 1 of row group grp
+  const int ngrp = count_groups(NG);
+  const int grp = (int)(((float)lane + 0.5f) * (1.0f / (float)NG));
+  const int bl = lane - grp * NG;
+  const bool act = grp < ngrp;
+  const int rpg = (NG + ngrp - 1) / ngrp;
+  const int i0 = grp * rpg;
+  const int nrow = act ? max(0, min(rpg, NG - i0)) : 0;
+  const unsigned cb = 0x80808080u - (unsigned)(bl + 1) * 0x01010101u;
+  double Wt = 0.0, V = 0.0, Z = 0.0, QK = 0.0, QT = 0.0;
+  double gi[RPG];
+  // row i0 + t of the group, or the zero row (with the gamma of row 0)
+  auto row_of = [&](int t) { return (t < nrow) ? i0 + t : NG; };
+#pragma unroll
+  for (int t = 0; t < RPG; t++) {
+    const int i = row_of(t);
+    const unsigned *rowg =
+        reinterpret_cast<const unsigned *>(gam + ((i < NG) ? i : 0) * (4 * ND));
+    unsigned ge = 0;  // elements of row i with gamma >= edge
+#pragma unroll
+    for (int d = 0; d < ND; d++) ge += (unsigned)__popc((rowg[d] + cb) & 0x80808080u);
+    const int c = 4 * ND - (int)ge;
+    const double gw = Gw[c];
+    const double *rr = rrec + i * RR;
+    Wt += rr[0] * gw;
+    V += rr[0] * ctab[2 * c];
+    Z += rr[0] * ctab[2 * c + 1];
+    QK += rr[1] * gw;
+    QT += rr[2] * gw;
+    gi[t] = gw;
+  }
+  wave_sync();  // gam and ctab are read: the partial sums take their place
+  if (act && grp > 0) {
+    double *pp = part + ((grp - 1) * NG + bl) * 5;
+    pp[0] = Wt;
+    pp[1] = V;
+    pp[2] = Z;
+    pp[3] = QK;
+    pp[4] = QT;
+  }
+  wave_sync();
+  if (grp == 0) {
+    for (int g2 = 1; g2 < ngrp; g2++) {
+      const double *pp = part + ((g2 - 1) * NG + bl) * 5;
+      Wt += pp[0];
+      V += pp[1];
+      Z += pp[2];
+      QK += pp[3];
+      QT += pp[4];
+    }
+  }
+  // the straddler of edge bl + 1 and its share below the edge
+  const bool blane = lane < NG;
+  const bool has = blane && (bl + 1 <= gtot);
+  double f = 0.0;
+  int is = 0;
+  double QKf = QK + (double)a2 * V, QNf = V, QTf = QT + (double)a2 * Z, Wtf = Wt;
+  if (has) {
+    const int s = recI[bl];
+    const int l = s / E;
+    is = (int)(((float)l + 0.5f) * (1.0f / (float)lpr));
+    const int js = (l - is * lpr) * E + (s & (E - 1));
+    const double gs = recGs[bl];
+    f = (G[bl + 1] - gs) / (recGe[bl] - gs) * w64[s];
+    const T kv = rows[is] + k2[js] * a2;
+    const T tv = rows[(NS - 1) * NG + is] + d2[js] * a2;
+    Wtf += f;
+    QKf += f * (double)kv;
+    QNf += f * (double)k2[js];
+    QTf += f * (double)tv;
+  }
+  // bin bl: reached when it is the first or its lower edge was straddled,
+  // divided by its weight when closed by a straddle or the last bin
+  const double pW = wave_shr1(Wtf), pK = wave_shr1(QKf), pN = wave_shr1(QNf), pT = wave_shr1(QTf);
+  const bool reached = blane && (bl == 0 || bl <= gtot);
+  const bool norm = (bl + 1 <= gtot) || bl == NG - 1;
+  double inv = 1.0;
+  if (reached && norm) inv = 1.0 / (Wtf - pW);
+  if (blane) {
+    nxt[bl] = reached ? (T)((QKf - pK) * inv) : (T)0;
+    nxt[(1 + r) * NG + bl] = reached ? (T)((QNf - pN) * inv) : (T)0;
+    nxt[(NS - 1) * NG + bl] = reached ? (T)((QTf - pT) * inv) : (T)0;
+    for (int s4 = r + 1; s4 < ngas; s4++) nxt[(1 + s4) * NG + bl] = (T)0;
+  }
+  // the payloads of the gases already mixed: row terms only
+  for (int s = 0; s < r; s++) {
+    double qv = 0.0;
+#pragma unroll
+    for (int t = 0; t < RPG; t++) qv += rrec[row_of(t) * RR + 3 + s] * gi[t];
+    wave_sync();  // the previous partials are read
+    if (act && grp > 0) part[(grp - 1) * NG + bl] = qv;
+    wave_sync();
+    if (grp == 0)
+      for (int g2 = 1; g2 < ngrp; g2++) qv += part[(g2 - 1) * NG + bl];
+    if (has) qv += f * (double)rows[(1 + s) * NG + is];
+    const double pq = wave_shr1(qv);
+    if (blane) nxt[(1 + s) * NG + bl] = reached ? (T)((qv - pq) * inv) : (T)0;
+  }
+  return true;
 }
 
 // One (request, wavenumber iw, layer j) item on one wavefront: the k-table
@@ -922,9 +1165,10 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   const int ngas = R.ngas, NG = a.ng;
   const LayerConst lc = a.lay[R.lay0 + j];
   const int NS = ngas + 2;
-  const double *G = reinterpret_cast<const double *>(hdr);
-  const double *dl64 = reinterpret_cast<const double *>(hdr + lds_round16(8 * (NG + 1)));
-  const T *dlg = reinterpret_cast<const T *>(hdr + lds_round16(8 * (NG + 1)) + lds_round16(8 * NG));
+  const HeaderLayout H = header_layout(NG, E, (int)sizeof(T));
+  const double *G = reinterpret_cast<const double *>(hdr + H.G);
+  const double *dl64 = reinterpret_cast<const double *>(hdr + H.dl64);
+  const T *dlg = reinterpret_cast<const T *>(hdr + H.dlg);
   const LayerLayout lo = layer_layout(NG, ngas, E, (int)sizeof(T));
   double *sSt = reinterpret_cast<double *>(ws + lo.sSt);
   double *sGd = reinterpret_cast<double *>(ws + lo.sGd);
@@ -1127,8 +1371,8 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
     i = (int)(((float)l + 0.5f) * rlpr);
     jx = (l - i * lpr) * E + (sl & (E - 1));
   };
-  const double *w64 = reinterpret_cast<const double *>(reinterpret_cast<const char *>(dlg) +
-                                                       lds_round16((int)sizeof(T) * NG));
+  const double *w64 = reinterpret_cast<const double *>(hdr + H.w64);
+  const double *Gw = reinterpret_cast<const double *>(hdr + H.Gw);
 
   T *rows = bufA, *nxt = bufB;
   for (int r = 1; r < ngas; r++) {
@@ -1157,8 +1401,8 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
       // non-decreasing every lane's E keys arrive in order.
       const bool k2up = __all(lane + 1 >= NG || !(k2[lane + 1] < k2[lane]));
       int idx[E];
+      T key[E];
       {
-        T key[E];
         const T K = lrow ? rows[li] : (T)0;
         if constexpr (sizeof(T) == 4) {
           // FP32 keys are >= 0, so their bit patterns order like their
@@ -1199,198 +1443,206 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
           Sorter<T, E>::sort(key, idx, lane);
         }
       }
-      wave_sync();  // the exact keys are read: the bin records take their place
-      // per-bin records (rankg.f:129-152 closes a bin at its straddling
-      // element, or at the end): recS[b][vq] the FP64 prefix of quantity vq
-      // through the closing slot, recC[b][vq] the closing slot's share of
-      // it that belongs to the next bin; recG[b] / recW[b] the same for the
-      // weight; rtype[b] 1 closed by a straddle, 2 open at the end, 0 never
-      // reached
-      const int NQ = ngas + 2;
-      double *recS = reinterpret_cast<double *>(uni);
-      double *recC = recS + NG * NQ;
-      double *recG = sGd, *recW = sSt;
-      int *rtype = sE;
+      wave_sync();  // the exact keys are read: the rebin's tables take their place
+      bool done = false;
+      if constexpr (E >= 4) {
+        if (a.rebin && k2up && counts_fit<E>(NG))
+          done = rebin_counts<T, E>(NG, ngas, r, a2, idx, key, rows, nxt, k2, d2, G, dl64, w64, Gw,
+                                    uni, sSt, sGd, lane);
+      }
+      if (!done) {
+        // per-bin records (rankg.f:129-152 closes a bin at its straddling
+        // element, or at the end): recS[b][vq] the FP64 prefix of quantity vq
+        // through the closing slot, recC[b][vq] the closing slot's share of
+        // it that belongs to the next bin; recG[b] / recW[b] the same for the
+        // weight; rtype[b] 1 closed by a straddle, 2 open at the end, 0 never
+        // reached
+        const int NQ = ngas + 2;
+        double *recS = reinterpret_cast<double *>(uni);
+        double *recC = recS + NG * NQ;
+        double *recG = sGd, *recW = sSt;
+        int *rtype = sE;
 
-      // (i, j) of each slot's element, packed (i << 8) | j; padding gets
-      // (NG << 8)
-      int ij[E];
-      T wq[E];
-      double gd[E];
-      double run = 0.0;
-#pragma unroll
-      for (int q = 0; q < E; q++) {
-        int i, jx;
-        slot_ij(idx[q], i, jx);
-        const bool real = i < NG && jx < NG;
-        ij[q] = real ? (i << 8) | jx : (NG << 8);
-        // weights and their FP64 cumulative sum gdist (rankg.f:104-108).  The
-        // sum is taken over FP64 products: where a run of elements exactly
-        // fills a bin, FP32 products would spill a ~1e-8 share of the next
-        // (possibly far larger) key across the edge.
-        const double w = w64[idx[q]];
-        wq[q] = (T)w;
-        run += w;
-        gd[q] = run;
-      }
-      {
-        const double s = wave_incl_sum(run);
-        const double ex = wave_shr1(s);
-#pragma unroll
-        for (int q = 0; q < E; q++) gd[q] += ex;
-      }
-      // cumulative weight before slot 0: the previous lane's last gd
-      const double gprev = wave_shr1(gd[E - 1]);
+        // (i, j) of each slot's element, packed (i << 8) | j; padding gets
+        // (NG << 8)
+        int ij[E];
+        T wq[E];
+        double gd[E];
+        double run = 0.0;
+  #pragma unroll
+        for (int q = 0; q < E; q++) {
+          int i, jx;
+          slot_ij(idx[q], i, jx);
+          const bool real = i < NG && jx < NG;
+          ij[q] = real ? (i << 8) | jx : (NG << 8);
+          // weights and their FP64 cumulative sum gdist (rankg.f:104-108).  The
+          // sum is taken over FP64 products: where a run of elements exactly
+          // fills a bin, FP32 products would spill a ~1e-8 share of the next
+          // (possibly far larger) key across the edge.
+          const double w = w64[idx[q]];
+          wq[q] = (T)w;
+          run += w;
+          gd[q] = run;
+        }
+        {
+          const double s = wave_incl_sum(run);
+          const double ex = wave_shr1(s);
+  #pragma unroll
+          for (int q = 0; q < E; q++) gd[q] += ex;
+        }
+        // cumulative weight before slot 0: the previous lane's last gd
+        const double gprev = wave_shr1(gd[E - 1]);
 
-      // bin of each element = number of edges G[1..NG] <= its start; it
-      // straddles when its end reaches the next edge (rankg.f:120).  The
-      // lane's first bin b0 comes from a search; the next four edges are
-      // held in registers and the bin of each slot is b0 plus the number of
-      // them at or below its start.  A lane whose slots pass all four takes
-      // the edge-by-edge walk instead.
-      int b0;
-      {
-        int lo2 = 0, hi2 = NG;
-        while (lo2 < hi2) {
-          const int mid = (lo2 + hi2 + 1) >> 1;
-          if (G[mid] <= gprev) lo2 = mid;
-          else hi2 = mid - 1;
+        // bin of each element = number of edges G[1..NG] <= its start; it
+        // straddles when its end reaches the next edge (rankg.f:120).  The
+        // lane's first bin b0 comes from a search; the next four edges are
+        // held in registers and the bin of each slot is b0 plus the number of
+        // them at or below its start.  A lane whose slots pass all four takes
+        // the edge-by-edge walk instead.
+        int b0;
+        {
+          int lo2 = 0, hi2 = NG;
+          while (lo2 < hi2) {
+            const int mid = (lo2 + hi2 + 1) >> 1;
+            if (G[mid] <= gprev) lo2 = mid;
+            else hi2 = mid - 1;
+          }
+          b0 = lo2;
         }
-        b0 = lo2;
-      }
-      constexpr int LA = 4;
-      double ge[LA];
-#pragma unroll
-      for (int t = 0; t < LA; t++) ge[t] = (b0 + 1 + t <= NG) ? G[b0 + 1 + t] : dinf<double>();
-      auto edges_le = [&](double x) {
-        int c = 0;
-#pragma unroll
-        for (int t = 0; t < LA; t++) c += (ge[t] <= x) ? 1 : 0;
-        return c;
-      };
-      if (!__any(edges_le(gd[E - 1]) >= LA)) {
-        int cs = 0;
-#pragma unroll
-        for (int q = 0; q < E; q++) {
-          const int ce = edges_le(gd[q]);
-          const int b = b0 + cs;
-          ij[q] |= (b << 16) | ((ce > cs && b < NG) ? (1 << 24) : 0);
-          cs = ce;
-        }
-      } else {
-        int b = b0;
-#pragma unroll
-        for (int q = 0; q < E; q++) {
-          const double st = (q == 0) ? gprev : gd[q - 1];
-          if (q > 0)
-            while (b < NG && G[b + 1] <= st) b++;
-          ij[q] |= (b << 16) | (((b < NG) && !(gd[q] < G[b + 1])) ? (1 << 24) : 0);
-        }
-      }
-      // omf: 1 - frac of rankg.f:121 for a straddling element, formed as a
-      // ratio of two FP64 differences
-      T omf[E];
-#pragma unroll
-      for (int q = 0; q < E; q++) {
-        const int b = (ij[q] >> 16) & 255;
-        const bool stq = (ij[q] >> 24) != 0;
-        const double st = (q == 0) ? gprev : gd[q - 1];
-        omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - st)) : (T)0;
-      }
-      // bin records: a straddling element closes its bin; the last slot
-      // closes the bin it lies in (or, if it straddles, opens the next one)
-#pragma unroll
-      for (int q = 0; q < E; q++) {
-        const int e = lane * E + q;
-        const int bi = (ij[q] >> 16) & 255;
-        const bool stq = (ij[q] >> 24) != 0;
-        if (stq) {
-          recG[bi] = gd[q];
-          recW[bi] = (double)omf[q] * (double)wq[q];
-          rtype[bi] = 1;
-        }
-        if (e == W * E - 1) {
-          const int bb = stq ? bi + 1 : bi;
-          if (bb < NG) {
-            recG[bb] = gd[q];
-            recW[bb] = 0.0;
-            rtype[bb] = 2;
+        constexpr int LA = 4;
+        double ge[LA];
+  #pragma unroll
+        for (int t = 0; t < LA; t++) ge[t] = (b0 + 1 + t <= NG) ? G[b0 + 1 + t] : dinf<double>();
+        auto edges_le = [&](double x) {
+          int c = 0;
+  #pragma unroll
+          for (int t = 0; t < LA; t++) c += (ge[t] <= x) ? 1 : 0;
+          return c;
+        };
+        if (!__any(edges_le(gd[E - 1]) >= LA)) {
+          int cs = 0;
+  #pragma unroll
+          for (int q = 0; q < E; q++) {
+            const int ce = edges_le(gd[q]);
+            const int b = b0 + cs;
+            ij[q] |= (b << 16) | ((ce > cs && b < NG) ? (1 << 24) : 0);
+            cs = ce;
+          }
+        } else {
+          int b = b0;
+  #pragma unroll
+          for (int q = 0; q < E; q++) {
+            const double st = (q == 0) ? gprev : gd[q - 1];
+            if (q > 0)
+              while (b < NG && G[b + 1] <= st) b++;
+            ij[q] |= (b << 16) | (((b < NG) && !(gd[q] < G[b + 1])) ? (1 << 24) : 0);
           }
         }
-      }
-      // weighted key and payloads: FP64 prefix sums along the sorted order,
-      // recorded at the straddling slots and the last slot
-      const int NGH = NG << 8;  // (ij & 0xffff) < NGH: a real element
-      auto scan_quantity = [&](int vq, auto value) {
-        T cq[E];
-        double run2 = 0.0;
-#pragma unroll
+        // omf: 1 - frac of rankg.f:121 for a straddling element, formed as a
+        // ratio of two FP64 differences
+        T omf[E];
+  #pragma unroll
         for (int q = 0; q < E; q++) {
-          const T c = ((ij[q] & 0xffff) < NGH) ? value(q) * wq[q] : (T)0;
-          cq[q] = c;
-          run2 += (double)c;
+          const int b = (ij[q] >> 16) & 255;
+          const bool stq = (ij[q] >> 24) != 0;
+          const double st = (q == 0) ? gprev : gd[q - 1];
+          omf[q] = stq ? ratio<T>((T)(gd[q] - G[b + 1]), (T)(gd[q] - st)) : (T)0;
         }
-        double pq = wave_shr1(wave_incl_sum(run2));
-#pragma unroll
+        // bin records: a straddling element closes its bin; the last slot
+        // closes the bin it lies in (or, if it straddles, opens the next one)
+  #pragma unroll
         for (int q = 0; q < E; q++) {
           const int e = lane * E + q;
-          pq += (double)cq[q];
           const int bi = (ij[q] >> 16) & 255;
           const bool stq = (ij[q] >> 24) != 0;
           if (stq) {
-            recS[bi * NQ + vq] = pq;
-            recC[bi * NQ + vq] = (double)omf[q] * (double)cq[q];
+            recG[bi] = gd[q];
+            recW[bi] = (double)omf[q] * (double)wq[q];
+            rtype[bi] = 1;
           }
           if (e == W * E - 1) {
             const int bb = stq ? bi + 1 : bi;
             if (bb < NG) {
-              recS[bb * NQ + vq] = pq;
-              recC[bb * NQ + vq] = 0.0;
+              recG[bb] = gd[q];
+              recW[bb] = 0.0;
+              rtype[bb] = 2;
             }
           }
         }
-      };
-      const int nq = r + 3;
-      scan_quantity(0, [&](int q) { return rows[(ij[q] >> 8) & 255] + k2[ij[q] & 255] * a2; });
-      for (int vq = 1; vq < nq; vq++) {
-        const int s2 = vq - 1;
-        if (s2 < r) {
-          const T *src = rows + (1 + s2) * NG;
-          scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255]; });
-        } else if (s2 == r) {
-          scan_quantity(vq, [&](int q) { return k2[ij[q] & 255]; });
-        } else {
-          const T *src = rows + (1 + ngas) * NG;
-          scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255] + d2[ij[q] & 255] * a2; });
-        }
-      }
-      wave_sync();
-      // bin b = its records minus the previous bin's, plus the share carried
-      // over from the element straddling its lower edge; normalised by the
-      // accumulated weight when a straddle closed it, or when it is the last
-      // bin (rankg.f:129-152)
-      if (lane < NG) {
-        const int b = lane;
-        const int t = rtype[b];
-        if (t == 0) {
-          for (int sl = 0; sl < NS; sl++) nxt[sl * NG + b] = (T)0;
-        } else {
-          const bool hp = b > 0;
-          const double sw =
-              recG[b] - (hp ? recG[b - 1] : 0.0) + (hp ? recW[b - 1] : 0.0) - recW[b];
-          const bool norm = (t == 1) || (b == NG - 1);
-          const T smT = (T)sw;
-          for (int vq = 0; vq < nq; vq++) {
-            const double Sv = recS[b * NQ + vq] - (hp ? recS[(b - 1) * NQ + vq] : 0.0) +
-                              (hp ? recC[(b - 1) * NQ + vq] : 0.0) - recC[b * NQ + vq];
-            T val = (T)Sv;
-            if (norm) val = val / smT;
-            const int sl = (vq == 0) ? 0 : ((vq - 1 == r + 1) ? 1 + ngas : vq);
-            nxt[sl * NG + b] = val;
+        // weighted key and payloads: FP64 prefix sums along the sorted order,
+        // recorded at the straddling slots and the last slot
+        const int NGH = NG << 8;  // (ij & 0xffff) < NGH: a real element
+        auto scan_quantity = [&](int vq, auto value) {
+          T cq[E];
+          double run2 = 0.0;
+  #pragma unroll
+          for (int q = 0; q < E; q++) {
+            const T c = ((ij[q] & 0xffff) < NGH) ? value(q) * wq[q] : (T)0;
+            cq[q] = c;
+            run2 += (double)c;
           }
-          for (int s4 = r + 1; s4 < ngas; s4++) nxt[(1 + s4) * NG + b] = (T)0;
+          double pq = wave_shr1(wave_incl_sum(run2));
+  #pragma unroll
+          for (int q = 0; q < E; q++) {
+            const int e = lane * E + q;
+            pq += (double)cq[q];
+            const int bi = (ij[q] >> 16) & 255;
+            const bool stq = (ij[q] >> 24) != 0;
+            if (stq) {
+              recS[bi * NQ + vq] = pq;
+              recC[bi * NQ + vq] = (double)omf[q] * (double)cq[q];
+            }
+            if (e == W * E - 1) {
+              const int bb = stq ? bi + 1 : bi;
+              if (bb < NG) {
+                recS[bb * NQ + vq] = pq;
+                recC[bb * NQ + vq] = 0.0;
+              }
+            }
+          }
+        };
+        const int nq = r + 3;
+        scan_quantity(0, [&](int q) { return rows[(ij[q] >> 8) & 255] + k2[ij[q] & 255] * a2; });
+        for (int vq = 1; vq < nq; vq++) {
+          const int s2 = vq - 1;
+          if (s2 < r) {
+            const T *src = rows + (1 + s2) * NG;
+            scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255]; });
+          } else if (s2 == r) {
+            scan_quantity(vq, [&](int q) { return k2[ij[q] & 255]; });
+          } else {
+            const T *src = rows + (1 + ngas) * NG;
+            scan_quantity(vq, [&](int q) { return src[(ij[q] >> 8) & 255] + d2[ij[q] & 255] * a2; });
+          }
         }
-        rtype[b] = 0;
+        wave_sync();
+        // bin b = its records minus the previous bin's, plus the share carried
+        // over from the element straddling its lower edge; normalised by the
+        // accumulated weight when a straddle closed it, or when it is the last
+        // bin (rankg.f:129-152)
+        if (lane < NG) {
+          const int b = lane;
+          const int t = rtype[b];
+          if (t == 0) {
+            for (int sl = 0; sl < NS; sl++) nxt[sl * NG + b] = (T)0;
+          } else {
+            const bool hp = b > 0;
+            const double sw =
+                recG[b] - (hp ? recG[b - 1] : 0.0) + (hp ? recW[b - 1] : 0.0) - recW[b];
+            const bool norm = (t == 1) || (b == NG - 1);
+            const T smT = (T)sw;
+            for (int vq = 0; vq < nq; vq++) {
+              const double Sv = recS[b * NQ + vq] - (hp ? recS[(b - 1) * NQ + vq] : 0.0) +
+                                (hp ? recC[(b - 1) * NQ + vq] : 0.0) - recC[b * NQ + vq];
+              T val = (T)Sv;
+              if (norm) val = val / smT;
+              const int sl = (vq == 0) ? 0 : ((vq - 1 == r + 1) ? 1 + ngas : vq);
+              nxt[sl * NG + b] = val;
+            }
+            for (int s4 = r + 1; s4 < ngas; s4++) nxt[(1 + s4) * NG + b] = (T)0;
+          }
+          rtype[b] = 0;
+        }
       }
     }
     wave_sync();
@@ -1477,24 +1729,29 @@ __device__ __forceinline__ void continuum_layer(const KernelArgs &a, const DevRe
 }  // namespace
 
 // The layer pass as its own launch: one wavefront per (request,
-// wavenumber, layer) item, results to the intermediate block in HBM.
+// wavenumber, layer) item, a.lwaves consecutive items per workgroup sharing
+// the header, results to the intermediate block in HBM.
 template <typename T, int E, int BF>
-__global__ __launch_bounds__(64, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs a, long long nblocks) {
-  const long long item = xcd_item(nblocks);
+__global__ __launch_bounds__(256, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs a, long long nblocks) {
+  const int tid = threadIdx.x;
+  // the wavefront number is uniform: in a scalar register, every scratch
+  // address of the item stays scalar
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & (W - 1);
+  extern __shared__ double smem_d[];
+  char *hdr = reinterpret_cast<char *>(smem_d);
+  layer_header<T, E>(a, hdr, tid, (int)blockDim.x);
+  __syncthreads();
+  const long long item = xcd_item(nblocks) * a.lwaves + wv;
   if (item >= a.nA) return;
-  const int lane = threadIdx.x;
   const int rqc = find_req(a.startA, a.nreqA, item);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, NG = a.ng;
   const long long loc = item - a.startA[rqc];
   const int iw = (int)(loc / L);
   const int j = (int)(loc - (long long)iw * L);
-  extern __shared__ double smem_d[];
-  char *hdr = reinterpret_cast<char *>(smem_d);
-  layer_header<T, E>(a, hdr, lane, W);
-  wave_sync();
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
-  layer_item<T, E, BF>(a, R, iw, j, hdr, hdr + layer_header_bytes(NG, E, (int)sizeof(T)), inter,
+  layer_item<T, E, BF>(a, R, iw, j, hdr,
+                       hdr + layer_header_bytes(NG, E, (int)sizeof(T)) + wv * a.lwstride, inter,
                        inter + (long long)NG * L, L, lane);
 }
 
@@ -2133,9 +2390,11 @@ __global__ __launch_bounds__(64) void quadrature_kernel(KernelArgs a) {
 namespace {
 template <typename T, int E, int BF = 0>
 hipError_t launch_layer_E(const KernelArgs &a, size_t lds, hipStream_t st) {
-  long long nb = ((a.nA + 7) / 8) * 8;
+  if (a.lwaves < 1 || a.lwaves > 4) return hipErrorInvalidValue;
+  long long nb = ((a.nA + a.lwaves - 1) / a.lwaves + 7) / 8 * 8;
   if (nb == 0) return hipSuccess;
-  hipLaunchKernelGGL((layer_kernel<T, E, BF>), dim3((unsigned)nb), dim3(64), lds, st, a, nb);
+  hipLaunchKernelGGL((layer_kernel<T, E, BF>), dim3((unsigned)nb), dim3(64 * a.lwaves), lds, st, a,
+                     nb);
   return hipGetLastError();
 }
 template <typename T, int RL>
@@ -2160,9 +2419,24 @@ hipError_t launch_fused_E(const KernelArgs &a, int RL, size_t lds, hipStream_t s
 }
 }  // namespace
 
+// Layer items per workgroup: up to four share the header while the
+// workgroup's LDS stays within 48 KiB, so that several workgroups fit a CU.
+int radg_layer_waves(int ng, int ngas, int E, int fp64) {
+  const int ts = fp64 ? 8 : 4;
+  const int h = layer_header_bytes(ng, E, ts), w = layer_layout(ng, ngas, E, ts).total;
+  int lw = 4;
+  while (lw > 1 && h + lw * w > 48 * 1024) lw >>= 1;
+  return lw;
+}
+
+int radg_layer_stride(int ng, int ngas, int E, int fp64) {
+  return layer_layout(ng, ngas, E, fp64 ? 8 : 4).total;
+}
+
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
   const int ts = fp64 ? 8 : 4;
-  return (size_t)layer_header_bytes(ng, E, ts) + (size_t)layer_layout(ng, ngas, E, ts).total;
+  return (size_t)layer_header_bytes(ng, E, ts) +
+         (size_t)radg_layer_waves(ng, ngas, E, fp64) * (size_t)layer_layout(ng, ngas, E, ts).total;
 }
 
 size_t radg_lds_emission_bytes(int RL, int nsw, int fp64) {

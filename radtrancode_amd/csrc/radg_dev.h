@@ -126,8 +126,8 @@ __host__ __device__ inline long long inter_block(int ng, int L, int nks, int npa
 
 // LDS of the layer pass.  A workgroup holds a header shared by its
 // wavefronts, then one scratch area per wavefront for the layer item it is
-// on.  Header: G [ng+3] (the g boundaries, G[ng] forced >= 1, then two +inf
-// so that a search may look two edges past the last), delg [ng] in FP64 and
+// on.  Header: G [ng+6] (the g boundaries, G[ng] forced >= 1, then +inf so
+// that a search may look five edges past the last), delg [ng] in FP64 and
 // in the compute type, the FP64 weight of each of the 64*E sort slots, and
 // Gw [ng+1], the unforced cumulative weights sum_{j<c} delg_j.
 __host__ __device__ inline int lds_round16(int b) { return (b + 15) & ~15; }
@@ -137,7 +137,7 @@ struct HeaderLayout {
 __host__ __device__ inline HeaderLayout header_layout(int ng, int E, int ts) {
   HeaderLayout H;
   int o = 0;
-  H.G = o;    o += lds_round16(8 * (ng + 3));
+  H.G = o;    o += lds_round16(8 * (ng + 6));
   H.dl64 = o; o += lds_round16(8 * ng);
   H.dlg = o;  o += lds_round16(ts * ng);
   H.w64 = o;  o += 8 * RADG_WAVE * E;
@@ -157,32 +157,49 @@ __host__ __device__ inline int layer_lpr(int ng, int E) { return (ng + E - 1) / 
 // by ngrp = 64 / ng groups of ng lanes, each group summing over about
 // ng / ngrp rows of the running distribution.
 __host__ __device__ inline int count_groups(int ng) { return RADG_WAVE / ng; }
+// rows a lane sums over at most (E = 4: 13-16 g-ordinates, 8: 17-21, 16: 22-32)
+__host__ __device__ inline int count_rpg(int E) { return E == 4 ? 4 : E == 8 ? 7 : 16; }
+// rows of rrec: count_rpg(E) per group, group g's first rpg = ceil(ng /
+// ngrp) of them rows g rpg + t of the distribution (while < ng), the rest zero
+__host__ __device__ inline int count_rows(int ng, int E) { return count_groups(ng) * count_rpg(E); }
 // Its scratch, inside `uni` (byte offsets from uni):
-//   rrec [ng+1][ngas+3] per row i: delg_i, delg_i K_i, delg_i dK_i/dT, then
-//                       delg_i times each gas payload (FP64); row ng zero
-//   recI [ng]           sort slot of the element straddling each edge
 //   gam  [64 E]         bytes: the number of bin edges at or below the
 //                       cumulative weight through each sort slot's element,
-//                       indexed by the element's slot (0x7f for padding)
-//   ctab [ng+1][2]      sum_{j<c} delg_j k2_j and delg_j dk2_j/dT (FP64)
+//                       indexed by the element's slot
+//   ctab [ng+1][3]      sum_{j<c} delg_j, delg_j k2_j and delg_j dk2_j/dT (FP64)
 //   part [ngrp-1][ng][5] the other groups' partial sums (FP64), over gam and
 //                       ctab once the counts are summed
+//   recI [ng]           sort slot of the element straddling each edge
+//   rrec [nrow][ngas+3] per row i: delg_i, delg_i K_i, delg_i dK_i/dT, then
+//                       delg_i times each gas payload (FP64), in group
+//                       order with zero rows (nrow = count_rows); past every
+//                       other use of uni (`keep`), so the weights and zero
+//                       rows written once per item survive the rounds
 struct CountLayout {
-  int rrec, recI, gam, ctab, part, total;
+  int gam, ctab, part, recI, rrec, total;
 };
-__host__ __device__ inline CountLayout count_layout(int ng, int ngas, int E) {
+__host__ __device__ inline CountLayout count_layout(int ng, int ngas, int E, int keep) {
   CountLayout C;
-  int o = 0;
-  C.rrec = o; o += lds_round16(8 * (ng + 1) * (ngas + 3));
-  C.recI = o; o += lds_round16(4 * ng);
-  C.gam = o;
-  C.part = o;
-  const int tabs = lds_round16(RADG_WAVE * E) + lds_round16(16 * (ng + 1));
+  C.gam = 0;
+  C.part = 0;
+  C.ctab = lds_round16(RADG_WAVE * E);
+  const int tabs = C.ctab + lds_round16(24 * (ng + 1));
   const int part = lds_round16(40 * (count_groups(ng) - 1) * ng);
-  C.ctab = o + lds_round16(RADG_WAVE * E);
-  o += (tabs > part) ? tabs : part;
+  int o = tabs > part ? tabs : part;
+  C.recI = o; o += lds_round16(4 * ng);
+  if (o < keep) o = keep;
+  C.rrec = o; o += lds_round16(8 * count_rows(ng, E) * (ngas + 3));
   C.total = o;
   return C;
+}
+// the bytes of uni the other users need: the rankk merge buffers, the exact
+// FP32 keys of a sort, the record rebin's records
+__host__ __device__ inline int uni_keep(int ng, int ngas, int E, int ts) {
+  int u = 2 * 8 * ng * (ngas + 2);
+  const int merge = 3 * RADG_WAVE * ts + (ts == 4 ? 4 * RADG_WAVE : 0);
+  if (u < merge) u = merge;
+  if (ts == 4 && u < 4 * RADG_WAVE * E) u = 4 * RADG_WAVE * E;
+  return lds_round16(u);
 }
 // Byte offsets within one wavefront's scratch (ts: bytes of the compute
 // type).  `uni` is shared, in time, by the rankk merge buffers, the exact
@@ -200,11 +217,8 @@ __host__ __device__ inline LayerLayout layer_layout(int ng, int ngas, int E, int
   L.bufA = o; o += lds_round16(ts * (ngas + 2) * ng);
   L.bufB = o; o += lds_round16(ts * (ngas + 2) * ng);
   L.sE = o;   o += lds_round16(4 * ng);
-  int u = 2 * 8 * ng * (ngas + 2);                         // bin records
-  const int merge = 3 * RADG_WAVE * ts + (ts == 4 ? 4 * RADG_WAVE : 0);
-  if (u < merge) u = merge;
-  if (ts == 4 && u < 4 * RADG_WAVE * E) u = 4 * RADG_WAVE * E;  // exact keys
-  if (E >= 4 && u < count_layout(ng, ngas, E).total) u = count_layout(ng, ngas, E).total;
+  int u = uni_keep(ng, ngas, E, ts);
+  if (E >= 4) u = count_layout(ng, ngas, E, u).total;
   L.uni = o;  o += lds_round16(u);
   L.total = o;
   return L;

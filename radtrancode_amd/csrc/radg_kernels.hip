@@ -253,9 +253,9 @@ template <int E> struct Sorter<double, E> {
 
 // ------------------------------------------- 32-bit sort for FP32 keys
 // Value of lane (lane ^ m) for a lane mask m known after unrolling: DPP
-// quad permutes and row mirrors where the pattern exists on gfx9, two DPP
-// moves for 4 and 8, a swizzle within 32 lanes for 16 and 31, and
-// ds_bpermute only across the two wave halves.
+// quad permutes, row mirrors and the rotation of a 16-lane row by 8 where
+// the pattern exists on gfx9, two DPP moves for 4, a swizzle within 32
+// lanes for 16 and 31, and ds_bpermute only across the two wave halves.
 template <int CTRL>
 __device__ __forceinline__ int dpp_mov(int x) {
   // every lane has a source lane in these patterns, so no lane keeps an old value
@@ -269,7 +269,7 @@ __device__ __forceinline__ unsigned xlane(unsigned v) {
   else if constexpr (M == 3) return (unsigned)dpp_mov<0x1B>(x);             // quad_perm 3,2,1,0
   else if constexpr (M == 4) return (unsigned)dpp_mov<0x141>(dpp_mov<0x1B>(x));   // ^3, ^7
   else if constexpr (M == 7) return (unsigned)dpp_mov<0x141>(x);            // row_half_mirror
-  else if constexpr (M == 8) return (unsigned)dpp_mov<0x140>(dpp_mov<0x141>(x));  // ^7, ^15
+  else if constexpr (M == 8) return (unsigned)dpp_mov<0x128>(x);            // row_ror:8
   else if constexpr (M == 15) return (unsigned)dpp_mov<0x140>(x);           // row_mirror
   else if constexpr (M == 16 || M == 31)
     return (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x1F | (M << 10));
@@ -890,7 +890,7 @@ __device__ __forceinline__ void layer_header(const KernelArgs &a, char *hdr, int
   double *Gw = reinterpret_cast<double *>(hdr + H.Gw);
   const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
                                    : reinterpret_cast<const T *>(a.delg32);
-  for (int i = tid; i < NG + 3; i += nthr) G[i] = (i <= NG) ? a.G[i] : dinf<double>();
+  for (int i = tid; i < NG + 6; i += nthr) G[i] = (i <= NG) ? a.G[i] : dinf<double>();
   for (int i = tid; i < NG; i += nthr) {
     dl64[i] = a.delg64[i];
     dlg[i] = delg[i];
@@ -932,39 +932,59 @@ __device__ __forceinline__ void layer_header(const KernelArgs &a, char *hdr, int
 // exactly when gamma >= b.  All sums are FP64.  Returns false, leaving nxt
 // untouched, when an element's weight reaches past two edges: rankg.f
 // closes only one bin per element there, which the record rebin follows.
-template <int E> struct CountShape;   // ND: bytes / 4 of a row, RPG: rows per group
-template <> struct CountShape<4> { static constexpr int ND = 4, RPG = 4; };
-template <> struct CountShape<8> { static constexpr int ND = 6, RPG = 7; };
-template <> struct CountShape<16> { static constexpr int ND = 8, RPG = 16; };
+template <int E> struct CountShape;   // RPG: rows per group at most
+template <> struct CountShape<4> { static constexpr int RPG = 4; };
+template <> struct CountShape<8> { static constexpr int RPG = 7; };
+template <> struct CountShape<16> { static constexpr int RPG = 16; };
 
-// The form applies when a row's slots are exactly ND words (E = 4: 13-16
-// g-ordinates, E = 8: 17-21, E = 16: 22-32).
+// The form applies with 4, 8 or 16 sort slots per lane when a row's slots
+// start 8-byte aligned and its NG gamma bytes take 4, 5, 6 or 8 words (E =
+// 4: 13-16 g-ordinates, E = 8: 17-21, E = 16: 22-24 and 29-32).
 template <int E>
 __device__ __forceinline__ bool counts_fit(int NG) {
   if constexpr (E < 4 || E > 16) {
     return false;
   } else {
-    const int lpr = layer_lpr(NG, E), ngrp = count_groups(NG);
-    return NG <= 32 && ngrp >= 1 && lpr * E == 4 * CountShape<E>::ND &&
+    const int lpr = layer_lpr(NG, E), ngrp = count_groups(NG), nd = (NG + 3) >> 2;
+    return NG <= 32 && ngrp >= 1 && (lpr * E) % 8 == 0 && 4 * nd <= lpr * E &&
+           (nd == 4 || nd == 5 || nd == 6 || nd == 8) &&
            (NG + ngrp - 1) / ngrp <= CountShape<E>::RPG;
   }
 }
 
+// Once per item: the row weights delg_i and the zero rows of rrec.
 template <typename T, int E>
+__device__ __forceinline__ void counts_init(const int NG, const int ngas, const double *dl64,
+                                            char *uni, int lane) {
+  const CountLayout cl = count_layout(NG, ngas, E, uni_keep(NG, ngas, E, (int)sizeof(T)));
+  double *rrec = reinterpret_cast<double *>(uni + cl.rrec);
+  constexpr int RPG = CountShape<E>::RPG;
+  const int RR = ngas + 3, ngrp = count_groups(NG), rpg = (NG + ngrp - 1) / ngrp;
+  for (int u = lane; u < count_rows(NG, E); u += W) {
+    const int g2 = u / RPG, t = u - g2 * RPG, i = g2 * rpg + t;
+    const bool ri = t < rpg && i < NG;
+    double *rr = rrec + u * RR;
+    rr[0] = ri ? dl64[i] : 0.0;
+    if (!ri)
+      for (int x = 1; x < RR; x++) rr[x] = 0.0;
+  }
+}
+
+template <typename T, int E, int ND>
 __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const int r, const T a2,
-                                             const int (&idx)[E], const T (&key)[E], const T *rows,
-                                             T *nxt, const T *k2, const T *d2, const double *G,
+                                             const int (&idx)[E], const T *rows, T *nxt,
+                                             const T *k2, const T *d2, const double *G,
                                              const double *dl64, const double *w64,
                                              const double *Gw, char *uni, double *recGs,
                                              double *recGe, int lane_in) {
-  constexpr int ND = CountShape<E>::ND, RPG = CountShape<E>::RPG;
+  constexpr int RPG = CountShape<E>::RPG;
   // an opaque copy of the lane number keeps the lane-dependent indices
   // below from being hoisted out of the round loop (and spilled there)
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
   const int NS = ngas + 2, RR = ngas + 3;
   const int lpr = layer_lpr(NG, E);
-  const CountLayout cl = count_layout(NG, ngas, E);
+  const CountLayout cl = count_layout(NG, ngas, E, uni_keep(NG, ngas, E, (int)sizeof(T)));
   double *rrec = reinterpret_cast<double *>(uni + cl.rrec);
   unsigned char *gam = reinterpret_cast<unsigned char *>(uni + cl.gam);
   double *ctab = reinterpret_cast<double *>(uni + cl.ctab);
@@ -985,90 +1005,114 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
     for (int q = 0; q < E; q++) gd[q] += ex;
   }
   const double gprev = wave_shr1(gd[E - 1]);
-  // gamma: edges G[1..NG] at or below the lane's start, then one slot at a
-  // time; a slot whose end reaches the next edge straddles it (rankg.f:120)
-  int g;
-  {
-    int lo = 0, hi = NG;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (G[mid] <= gprev) lo = mid;
-      else hi = mid - 1;
-    }
-    g = lo;
-  }
-  bool multi = false;
+  // gamma: b0 edges of G[1..NG] at or below the lane's start (a search),
+  // then for each slot b0 plus the number of the next LA edges at or below
+  // its end.  A slot whose count rises straddles the edge it reaches
+  // (rankg.f:120); a rise of two, or a lane reaching its LA-th edge, hands
+  // the round to the record rebin (a lane's 8 elements span a bin edge at
+  // most once with Gauss-Legendre weights).  Padding slots write gamma
+  // bytes past a row's NG real ones, which the counts below mask.
+  constexpr int LA = 2;
+  int b0 = 0;
+#pragma unroll
+  for (int st = 32; st > 0; st >>= 1)
+    if (b0 + st <= NG && G[b0 + st] <= gprev) b0 += st;
+  double ge[LA];
+#pragma unroll
+  for (int t = 0; t < LA; t++) ge[t] = G[b0 + 1 + t];
+  int cprev = 0;
+  bool bad = false;
 #pragma unroll
   for (int q = 0; q < E; q++) {
     const double x = gd[q];
-    if (G[g + 1] <= x) {
-      recI[g] = idx[q];
-      recGs[g] = (q == 0) ? gprev : gd[q - 1];
-      recGe[g] = x;
-      multi |= (G[g + 2] <= x);
-      g++;
+    int c = 0;
+#pragma unroll
+    for (int t = 0; t < LA; t++) c += (ge[t] <= x) ? 1 : 0;
+    if (c > cprev) {
+      const int e = b0 + cprev;  // edge e + 1, recorded at e
+      recI[e] = idx[q];
+      recGs[e] = (q == 0) ? gprev : gd[q - 1];
+      recGe[e] = x;
     }
-    gam[idx[q]] = (key[q] == dinf<T>()) ? (unsigned char)0x7f : (unsigned char)g;
+    bad |= (c > cprev + 1);
+    cprev = c;
+    gam[idx[q]] = (unsigned char)(b0 + c);
   }
-  if (__any(multi)) return false;
-  const int gtot = __shfl(g, W - 1);  // edges at or below the total weight
+  bad |= (cprev >= LA);
+  if (__any(bad)) return false;
+  const int gtot = __shfl(b0 + cprev, W - 1);  // edges at or below the total weight
 
-  // column prefix tables of the new gas
+  // column prefix tables: the header's Gw and the new gas's two sums
   {
     const bool jl = lane < NG;
     const double xk = jl ? dl64[lane] * (double)k2[lane] : 0.0;
     const double xd = jl ? dl64[lane] * (double)d2[lane] : 0.0;
     const double ik = wave_incl_sum(xk), id = wave_incl_sum(xd);
     if (jl) {
-      ctab[2 * lane + 2] = ik;
-      ctab[2 * lane + 3] = id;
+      ctab[3 * lane + 3] = Gw[lane + 1];
+      ctab[3 * lane + 4] = ik;
+      ctab[3 * lane + 5] = id;
     }
     if (lane == 0) {
       ctab[0] = 0.0;
       ctab[1] = 0.0;
+      ctab[2] = 0.0;
     }
   }
-  // row terms times the row weight: delg_i, delg_i K_i, delg_i dK_i/dT,
-  // delg_i P_si for the gases already mixed; row NG is all zero and stands
-  // in for the rows a group does not have
-  for (int i = lane; i <= NG; i += W) {
-    const bool ri = i < NG;
-    const double w = ri ? dl64[i] : 0.0;
-    double *rr = rrec + i * RR;
-    rr[0] = w;
-    rr[1] = ri ? w * (double)rows[i] : 0.0;
-    rr[2] = ri ? w * (double)rows[(NS - 1) * NG + i] : 0.0;
-    for (int s = 0; s < r; s++) rr[3 + s] = ri ? w * (double)rows[(1 + s) * NG + i] : 0.0;
-  }
-  wave_sync();
-
-  // lane = edge bl + 1 of row group grp
+  // lane = edge bl + 1 of row group grp, rows i0 + t, t < RPG: rows with
+  // t >= rpg or past NG are the zero rows of rrec, whose gamma bytes are
+  // whatever follows (their counts are clamped to NG)
   const int ngrp = count_groups(NG);
+  const int rpg = (NG + ngrp - 1) / ngrp;
   const int grp = (int)(((float)lane + 0.5f) * (1.0f / (float)NG));
   const int bl = lane - grp * NG;
   const bool act = grp < ngrp;
-  const int rpg = (NG + ngrp - 1) / ngrp;
-  const int i0 = grp * rpg;
-  const int nrow = act ? max(0, min(rpg, NG - i0)) : 0;
+  const int i0 = act ? grp * rpg : 0;
+  // this round's row terms: delg_i K_i, delg_i dK_i/dT and delg_i P_si for
+  // the gases already mixed (row i at slot (i / rpg) RPG + i % rpg)
+  if (lane < NG) {
+    const int g2 = (int)(((float)lane + 0.5f) * (1.0f / (float)rpg));
+    double *rr = rrec + (g2 * RPG + lane - g2 * rpg) * RR;
+    const double w = dl64[lane];
+    rr[1] = w * (double)rows[lane];
+    rr[2] = w * (double)rows[(NS - 1) * NG + lane];
+    for (int s = 0; s < r; s++) rr[3 + s] = w * (double)rows[(1 + s) * NG + lane];
+  }
+  wave_sync();
+
+  // counts: 4 bytes of gamma per word (ND words, ND = ceil(NG / 4)),
+  // (x + 0x80 - b) per byte sets the byte's top bit exactly when gamma >= b;
+  // the last word's bytes past NG are forced to 0x7f (>= every edge)
+  const unsigned pm = (NG & 3) ? (0x7f7f7f7fu << (8 * (NG & 3))) : 0u;
   const unsigned cb = 0x80808080u - (unsigned)(bl + 1) * 0x01010101u;
   double Wt = 0.0, V = 0.0, Z = 0.0, QK = 0.0, QT = 0.0;
   double gi[RPG];
-  // row i0 + t of the group, or the zero row (with the gamma of row 0)
-  auto row_of = [&](int t) { return (t < nrow) ? i0 + t : NG; };
+  const unsigned char *rowb0 = gam + i0 * (lpr * E);
+  const double *rr0 = rrec + (act ? grp : 0) * RPG * RR;
 #pragma unroll
   for (int t = 0; t < RPG; t++) {
-    const int i = row_of(t);
-    const unsigned *rowg =
-        reinterpret_cast<const unsigned *>(gam + ((i < NG) ? i : 0) * (4 * ND));
-    unsigned ge = 0;  // elements of row i with gamma >= edge
+    const unsigned long long *rowg =
+        reinterpret_cast<const unsigned long long *>(rowb0 + t * (lpr * E));
+    unsigned ge = 0;  // bytes >= edge
 #pragma unroll
-    for (int d = 0; d < ND; d++) ge += (unsigned)__popc((rowg[d] + cb) & 0x80808080u);
-    const int c = 4 * ND - (int)ge;
-    const double gw = Gw[c];
-    const double *rr = rrec + i * RR;
+    for (int d = 0; d < ND; d += 2) {
+      const unsigned long long v = rowg[d >> 1];
+      unsigned lo = (unsigned)v;
+      if (d == ND - 1) lo |= pm;
+      ge += (unsigned)__popc((lo + cb) & 0x80808080u);
+      if (d + 1 < ND) {
+        unsigned hi = (unsigned)(v >> 32);
+        if (d + 1 == ND - 1) hi |= pm;
+        ge += (unsigned)__popc((hi + cb) & 0x80808080u);
+      }
+    }
+    const int c = min(4 * ND - (int)ge, NG);
+    const double *ct = ctab + 3 * c;
+    const double gw = ct[0];
+    const double *rr = rr0 + t * RR;
     Wt += rr[0] * gw;
-    V += rr[0] * ctab[2 * c];
-    Z += rr[0] * ctab[2 * c + 1];
+    V += rr[0] * ct[1];
+    Z += rr[0] * ct[2];
     QK += rr[1] * gw;
     QT += rr[2] * gw;
     gi[t] = gw;
@@ -1130,7 +1174,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   for (int s = 0; s < r; s++) {
     double qv = 0.0;
 #pragma unroll
-    for (int t = 0; t < RPG; t++) qv += rrec[row_of(t) * RR + 3 + s] * gi[t];
+    for (int t = 0; t < RPG; t++) qv += rr0[t * RR + 3 + s] * gi[t];
     wave_sync();  // the previous partials are read
     if (act && grp > 0) part[(grp - 1) * NG + bl] = qv;
     wave_sync();
@@ -1374,6 +1418,9 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   const double *w64 = reinterpret_cast<const double *>(hdr + H.w64);
   const double *Gw = reinterpret_cast<const double *>(hdr + H.Gw);
 
+  if constexpr (E >= 4) {
+    if (a.rebin && ngas > 1 && counts_fit<E>(NG)) counts_init<T, E>(NG, ngas, dl64, uni, lane);
+  }
   T *rows = bufA, *nxt = bufB;
   for (int r = 1; r < ngas; r++) {
     const T a2 = (T)amo[r];
@@ -1446,9 +1493,18 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
       wave_sync();  // the exact keys are read: the rebin's tables take their place
       bool done = false;
       if constexpr (E >= 4) {
-        if (a.rebin && k2up && counts_fit<E>(NG))
-          done = rebin_counts<T, E>(NG, ngas, r, a2, idx, key, rows, nxt, k2, d2, G, dl64, w64, Gw,
-                                    uni, sSt, sGd, lane);
+        if (a.rebin && k2up && counts_fit<E>(NG)) {
+          // words of gamma per row: 4 (E = 4), 5 or 6 (E = 8), 6 or 8 (E = 16)
+          const int nd = (NG + 3) >> 2;
+          constexpr int NA = (E == 4) ? 4 : (E == 8) ? 5 : 6;
+          constexpr int NB = (E == 4) ? 4 : (E == 8) ? 6 : 8;
+          if (nd == NA)
+            done = rebin_counts<T, E, NA>(NG, ngas, r, a2, idx, rows, nxt, k2, d2, G, dl64, w64, Gw,
+                                          uni, sSt, sGd, lane);
+          else if (nd == NB)
+            done = rebin_counts<T, E, NB>(NG, ngas, r, a2, idx, rows, nxt, k2, d2, G, dl64, w64, Gw,
+                                          uni, sSt, sGd, lane);
+        }
       }
       if (!done) {
         // per-bin records (rankg.f:129-152 closes a bin at its straddling

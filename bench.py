@@ -97,11 +97,83 @@ def cpu_baseline(tables, cia, req, seconds):
                        f"({what}) in {total_t:.1f} s")
 
 
+_CPU_JOB = {}
+
+
+def _cpu_worker(job):
+    """One host process of the parallel CPU baseline: the reference's
+    cirsradg_wave over consecutive wavenumbers of the spectrum from its own
+    starting point (wrapping), chunks of up to 64, until ``seconds`` of
+    compute; returns (wavenumbers, seconds)."""
+    start, seconds = job
+    tables, cia, req = _CPU_JOB["inputs"]
+    from oracle import ref
+
+    nw = req.nwave
+    pos, n, done, used = start, 8, 0, 0.0
+    while used < seconds:
+        n = min(n, nw - pos)
+        ts = [copy.copy(t) for t in tables]
+        for t, t0 in zip(ts, tables):
+            t.k = t0.k[pos:pos + n].copy()
+            t.vmin = t0.vmin + pos * t0.delv
+        r2 = copy.copy(req)
+        r2.vwave = req.vwave[pos:pos + n].copy()
+        rr = ref.RefRunner(ts, cia, r2, prec=64)
+        t0 = time.perf_counter()
+        rr.spectrum()
+        used += time.perf_counter() - t0
+        rr.close()
+        done += n
+        pos = (pos + n) % nw
+        n = min(2 * n, 64)
+    return done, used
+
+
+def cpu_baseline_parallel(tables, cia, req, seconds, nproc):
+    """The reference's cirsradg_wave (REAL(8) build) in ``nproc``
+    independent host processes at once, as NEMESIS runs independent
+    retrievals side by side; forked before the GPU is touched.  The rate is
+    the sum of the processes' own rates."""
+    import multiprocessing as mp
+
+    from oracle import ref
+
+    if not ref.available(64):
+        return None
+    _CPU_JOB["inputs"] = (tables, cia, req)
+    jobs = [(int(w * req.nwave // nproc), seconds) for w in range(nproc)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(nproc) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    _CPU_JOB.clear()
+    value = sum((d / req.nwave) / u for d, u in res)
+    return dict(value=value, unit="Jacobian-spectra/s", cores=nproc, kind="reference",
+                sample=f"{nproc} processes, {sum(d for d, _ in res)} wavenumbers of one "
+                       f"120-layer x 20 g spectrum in total (reference cirsradg_wave, REAL(8) "
+                       f"build, once per wavenumber), about {seconds:.0f} s of compute each, "
+                       f"{wall:.1f} s wall")
+
+
+def kernel_source_hash():
+    """SHA-256 (first 16 hex digits) of the layer kernel's sources: PMC
+    counters are merged into the bench line only when they were taken on
+    the same kernel code."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in ("radg_kernels.hip", "radg_dev.h"):
+        with open(os.path.join(ROOT, "radtrancode_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def pmc_figures(batch, t_layer_s):
     """HBM bytes per launch and VALU issue of the layer kernel from the
     committed rocprofv3 PMC passes (profiles/layer_kernel_traffic.json), when
-    they were taken on this batch size; the VALU rate uses this run's
-    kernel time."""
+    they were taken on this batch size and on these kernel sources; the
+    VALU rate uses this run's kernel time."""
     path = os.path.join(ROOT, "profiles", "layer_kernel_traffic.json")
     if not os.path.exists(path):
         return {}
@@ -109,13 +181,46 @@ def pmc_figures(batch, t_layer_s):
         p = json.load(f)
     if p.get("spectra_per_launch") != batch:
         return {}
+    if p.get("kernel_source_sha256") != kernel_source_hash():
+        return {"traffic": None,
+                "traffic_source": "not merged: the committed PMC passes were taken on other "
+                                  "kernel sources (" + str(p.get("kernel_source_sha256")) + ")"}
     traffic = (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]) * 1024
     rate = p["valu_wave_instructions"] / t_layer_s
     return {"traffic": traffic,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (" + p["source"] + ")",
+            "kernel_source_sha256": p["kernel_source_sha256"],
             "valu_issue": {"wave_instructions_per_launch": p["valu_wave_instructions"],
+                           "instructions_per_item": p.get("valu_per_item"),
+                           "per_item_breakdown": p.get("valu_per_item_breakdown"),
                            "achieved_per_s": rate, "peak_per_s": p["valu_issue_peak_per_s"],
                            "frac": rate / p["valu_issue_peak_per_s"]}}
+
+
+def layer_sweep():
+    """One batch per k-gas count (1, 2, 3: no overlap round, one, two) of 4
+    spectra x 200 wavenumbers x 120 layers x 20 g, launched once in the
+    mixed and the FP64 form.  Under rocprofv3 --pmc the layer-kernel
+    dispatches give the instructions of the interpolation and of each
+    overlap round per (wavenumber, layer) item."""
+    import torch
+
+    from radtrancode_amd import cirsradg as rad
+    from radtrancode_amd import synth
+
+    torch.cuda.set_device(0)
+    for nk in (1, 2, 3):
+        tables, cia, req0 = synth.config1(nwave=200, nlayer=120, nkgas=nk, ng=20)
+        rad.register_ktables(tables)
+        rad.register_cia(cia)
+        reqs = [req0] + [synth.make_request(nlayer=120, nkgas=nk, nwave=200, seed=100 + i)
+                         for i in range(1, 4)]
+        for mode in (rad.MODE_MIXED, rad.MODE_FP64):
+            b = rad.Batch(reqs, mode=mode)
+            b.launch()
+            b.sync()
+            b.close()
+        print(f"layer sweep: {nk} k-gases done", flush=True)
 
 
 def weighted(out, dq, pw):
@@ -313,9 +418,13 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="host processes of the parallel CPU baseline (at most the host's cores)")
     ap.add_argument("--skip-fp64", action="store_true", help="time only the mixed form")
     ap.add_argument("--serving-requests", type=int, default=4096,
                     help="requests in the Poisson serving run (0: skip it)")
+    ap.add_argument("--no-serving-capacity", action="store_true",
+                    help="skip the saturated serving run (all requests queued at once)")
     ap.add_argument("--serving-rate", type=float, default=100.0,
                     help="Poisson arrival rate of the serving run, requests/s")
     ap.add_argument("--bf16-steps", type=int, default=3)
@@ -326,6 +435,10 @@ def main():
     ap.add_argument("--retrievals", type=int, default=512,
                     help="retrievals in the batched optimal-estimation run (0: skip it)")
     ap.add_argument("--retrieval-iters", type=int, default=10)
+    ap.add_argument("--layer-sweep", action="store_true",
+                    help="only launch the layer pass once per k-gas count 1, 2, 3 (0, 1, 2 overlap "
+                         "rounds), mixed and FP64 forms, 4 spectra of 200 wavenumbers each: the "
+                         "dispatches a rocprofv3 --pmc pass splits the work per round by")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known "
                          "traffic, to scale rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -334,6 +447,21 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by wavenumber or by request with no exchange)", file=sys.stderr)
         sys.exit(2)
+    if args.layer_sweep:
+        layer_sweep()
+        return
+    tables, cia, reqs = make_batch_inputs(args.batch)
+    # the host baselines first, before anything touches the GPU (the
+    # parallel one forks its processes)
+    cpu = None
+    if not args.no_cpu:
+        single = cpu_baseline(tables, cia, reqs[0], args.cpu_seconds)
+        nproc = max(1, min(args.cpu_procs, os.cpu_count() or 1))
+        par = cpu_baseline_parallel(tables, cia, reqs[0], args.cpu_seconds, nproc)
+        cpu = single
+        if par is not None:
+            cpu = dict(par, single_core=single)
+
     import torch
 
     from oracle import oracle
@@ -348,7 +476,6 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    tables, cia, reqs = make_batch_inputs(args.batch)
     rad.register_ktables(tables)
     rad.register_cia(cia)
 
@@ -411,10 +538,6 @@ def main():
                 alg_bytes_per_launch=alg_read)
     roof.update(pmc_figures(args.batch, t_layer_s))
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(tables, cia, reqs[0], args.cpu_seconds)
-
     percall = None
     if args.percall > 0:
         rad.set_mode(rad.MODE_MIXED)
@@ -432,6 +555,14 @@ def main():
     serve = None
     if args.serving_requests > 0:
         serve = serving_run(rad, args.serving_requests, args.serving_rate)
+        serve["efficiency_vs_batch"] = serve["work_units_per_s"] / value
+        if not args.no_serving_capacity:
+            # the same requests all queued at once: the loop's capacity
+            cap = serving_run(rad, args.serving_requests, 1e6)
+            serve["saturated"] = {k: cap[k] for k in (
+                "wall_s", "requests_per_s", "work_units_per_s", "latency_ms_p50",
+                "latency_ms_p99", "batches", "largest_batch", "peak_inflight_bytes")}
+            serve["saturated"]["efficiency_vs_batch"] = cap["work_units_per_s"] / value
 
     oe_run = None
     if args.retrievals > 0:

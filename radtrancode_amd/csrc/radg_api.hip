@@ -1149,7 +1149,7 @@ struct Plan {
             nwout = 0, nwjac = 0, ndx = 0, nct = 0, ngcw = 0;
   std::vector<long long> sA, sB, sC;
   int maxngas = 1, maxnsw = 1, maxlays = 1;
-  std::vector<int> cls_rl;
+  std::vector<int> cls_rl, cls_ngas;
   std::vector<std::vector<int>> cls_members;
   size_t o_clsmap = 0, o_clsstart = 0, o_clsstartA = 0, o_clsstartW = 0;
 };
@@ -1208,15 +1208,19 @@ int plan_batch(int nreq, const radg_request *reqs, int mode, const TableSet &ts,
     pl.maxnsw = std::max(pl.maxnsw, std::max(q.nsw, 1));
     pl.maxlays = std::max(pl.maxlays, q.nlayer);
   }
-  // emission classes: requests grouped by their own layers-per-lane
+  // launch classes: requests grouped by their own layers-per-lane and gas
+  // count, so that each launch sizes its LDS (and so its occupancy) by its
+  // members alone
   pl.cls_rl.clear();
+  pl.cls_ngas.clear();
   pl.cls_members.clear();
   for (int r = 0; r < nreq; r++) {
-    const int rl = radg_emission_RL(reqs[r].nlayer);
+    const int rl = radg_emission_RL(reqs[r].nlayer), ng = reqs[r].ngas;
     size_t c = 0;
-    while (c < pl.cls_rl.size() && pl.cls_rl[c] != rl) c++;
+    while (c < pl.cls_rl.size() && (pl.cls_rl[c] != rl || pl.cls_ngas[c] != ng)) c++;
     if (c == pl.cls_rl.size()) {
       pl.cls_rl.push_back(rl);
+      pl.cls_ngas.push_back(ng);
       pl.cls_members.emplace_back();
     }
     pl.cls_members[c].push_back(r);
@@ -1983,6 +1987,7 @@ extern "C++" std::vector<long long> plan_sig(const Plan &pl) {
                               (long long)pl.o_wout, (long long)pl.o_wjac, (long long)pl.o_err,
                               (long long)pl.o_clsstart, (long long)pl.cls_rl.size()};
   for (int rl : pl.cls_rl) v.push_back(rl);
+  for (int ng : pl.cls_ngas) v.push_back(ng);
   for (long long x : pl.sA) v.push_back(x);
   for (long long x : pl.sB) v.push_back(x);
   return v;

@@ -123,10 +123,13 @@ def poisson_workload(n=4096, rate=200.0, seed=5, npoint=5000, l_range=(40, 200),
 def run_poisson(server: Server, arrivals, reqs, collect=None):
     """Submit each request at its arrival time; returns per-request latency
     (s), wall time and the requests' total work in units of the 120-layer x
-    1000-wavenumber, 3 k-gas spectrum: nwave/1000 x nlayer/120 x (overlap
-    rounds + 0.3) / 2.3, where a request with k k-gases has max(k - 1, 0)
-    overlap rounds and 0.3 stands for its interpolation and emission work (a
-    cost model of the layer kernel, which dominates)."""
+    1000-wavenumber, 3 k-gas spectrum: nwave/1000 x nlayer/120 x (0.33 +
+    0.335 r), where a request with k k-gases has r = max(k - 1, 0) overlap
+    rounds.  The weights are the measured costs per (wavenumber, layer) at
+    the 120-layer workload: the layer kernel's interpolation and stores
+    (654 VALU instructions, 2.46 ms of a 16-spectrum step) and each overlap
+    round (about 1390 instructions, 5.24 ms), with the emission kernel
+    (2.63 ms) counted with the fixed part, over the 15.57 ms step."""
     n = len(reqs)
     tickets = [None] * n
     t0 = time.perf_counter()
@@ -152,9 +155,14 @@ def run_poisson(server: Server, arrivals, reqs, collect=None):
             collect[i] = r
     th.join()
     wall = time.perf_counter() - t0
+    return dict(latency=lat, status=status, wall=wall, work=work_units(reqs))
+
+
+def work_units(reqs):
+    """Total work of requests in units of the 120-layer x 1000-wavenumber,
+    3 k-gas spectrum (the cost model run_poisson describes)."""
     def units(q):
         nk = int(sum(1 for g in q.idgas if g not in (synth.H2_ID, synth.HE_ID)))
-        return q.nwave / 1000.0 * q.nlayer / 120.0 * (max(nk - 1, 0) + 0.3) / 2.3
+        return q.nwave / 1000.0 * q.nlayer / 120.0 * (0.33 + 0.335 * max(nk - 1, 0))
 
-    work = float(sum(units(q) for q in reqs))
-    return dict(latency=lat, status=status, wall=wall, work=work)
+    return float(sum(units(q) for q in reqs))

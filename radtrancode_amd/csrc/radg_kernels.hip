@@ -362,14 +362,21 @@ __device__ __forceinline__ bool kf_lt(float ka, int ia, float kb, int ib) {
 // Odd-even transposition on the exact keys, until no adjacent pair is out of
 // order (a run of m slots needs at most m phases; the bound W*E is never
 // reached on a sorted input).
+// lane + 1's and lane - 1's value of a 32-bit quantity (DPP wave shifts;
+// 0 where there is no such lane)
+__device__ __forceinline__ int lane_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int lane_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ float lane_next(float x) { return __int_as_float(lane_next(__float_as_int(x))); }
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(lane_prev(__float_as_int(x))); }
+
 template <int E>
 __device__ __forceinline__ void exact_fixup(float (&kx)[E], int (&ex)[E], int lane) {
   auto unsorted = [&]() {
     bool bad = false;
 #pragma unroll
     for (int q = 0; q + 1 < E; q++) bad |= kf_lt(kx[q + 1], ex[q + 1], kx[q], ex[q]);
-    const float nk = __shfl_down(kx[0], 1);
-    const int ne = __shfl_down(ex[0], 1);
+    const float nk = lane_next(kx[0]);
+    const int ne = lane_next(ex[0]);
     bad |= (lane + 1 < W) && kf_lt(nk, ne, kx[E - 1], ex[E - 1]);
     return __any(bad);
   };
@@ -385,10 +392,10 @@ __device__ __forceinline__ void exact_fixup(float (&kx)[E], int (&ex)[E], int la
   };
   // pair (slot E-1 of lane l, slot 0 of lane l+1) for every l with en(l)
   auto cross = [&](auto en) {
-    const float nk = __shfl_down(kx[0], 1);
-    const int ne = __shfl_down(ex[0], 1);
-    const float pk = __shfl_up(kx[E - 1], 1);
-    const int pe = __shfl_up(ex[E - 1], 1);
+    const float nk = lane_next(kx[0]);
+    const int ne = lane_next(ex[0]);
+    const float pk = lane_prev(kx[E - 1]);
+    const int pe = lane_prev(ex[E - 1]);
     const bool lo = (lane + 1 < W) && en(lane) && kf_lt(nk, ne, kx[E - 1], ex[E - 1]);
     const bool hi = (lane > 0) && en(lane - 1) && kf_lt(kx[0], ex[0], pk, pe);
     if (lo) {
@@ -436,6 +443,67 @@ __device__ __forceinline__ double wave_incl_sum(double s) {
 }
 // value of lane - 1 (0 in lane 0)
 __device__ __forceinline__ double wave_shr1(double s) { return dpp_d<0x138, 0xF, true>(s); }
+// A DPP move of a double in which lanes without a source in the pattern, or
+// in rows the row mask leaves out, keep `old`.
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_dk(double old, double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)u, CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(u >> 32), CTRL,
+                                             RM, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+// lane l's double, l uniform
+__device__ __forceinline__ double read_lane_d(double x, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+// inclusive prefix product over the 64 lanes, the pattern of wave_incl_sum
+// with 1 for lanes that have no source
+__device__ __forceinline__ double wave_incl_prod(double s) {
+  s *= dpp_dk<0x111, 0xF>(1.0, s);
+  s *= dpp_dk<0x112, 0xF>(1.0, s);
+  s *= dpp_dk<0x114, 0xF>(1.0, s);
+  s *= dpp_dk<0x118, 0xF>(1.0, s);
+  s *= dpp_dk<0x142, 0xA>(1.0, s);
+  s *= dpp_dk<0x143, 0xC>(1.0, s);
+  return s;
+}
+// Inclusive suffix composition over the 64 lanes of the affine maps
+// x -> a + b x (lane l's map applied after lane l+1's): row shifts left 1, 2,
+// 4, 8 within each 16-lane row (identity where there is no source), then
+// each row's lanes compose with the totals of the rows after it, read from
+// the rows' first lanes.
+__device__ __forceinline__ void wave_suffix_affine(double &a, double &b, int lane) {
+#define RADG_SUFFIX_STEP(CT)                                 \
+  {                                                          \
+    const double oa = dpp_dk<CT, 0xF>(0.0, a);               \
+    const double ob = dpp_dk<CT, 0xF>(1.0, b);               \
+    a = a + b * oa;                                          \
+    b = b * ob;                                              \
+  }
+  RADG_SUFFIX_STEP(0x101)
+  RADG_SUFFIX_STEP(0x102)
+  RADG_SUFFIX_STEP(0x104)
+  RADG_SUFFIX_STEP(0x108)
+#undef RADG_SUFFIX_STEP
+  const double a1 = read_lane_d(a, 16), b1 = read_lane_d(b, 16);
+  const double a2 = read_lane_d(a, 32), b2 = read_lane_d(b, 32);
+  const double a3 = read_lane_d(a, 48), b3 = read_lane_d(b, 48);
+  // carries: row 2 takes row 3's total, row 1 rows 2 and 3, row 0 rows 1-3
+  const double c1a = a2 + b2 * a3, c1b = b2 * b3;
+  const double c0a = a1 + b1 * c1a, c0b = b1 * c1b;
+  const int row = lane >> 4;
+  const double ca = row == 0 ? c0a : row == 1 ? c1a : row == 2 ? a3 : 0.0;
+  const double cb = row == 0 ? c0b : row == 1 ? c1b : row == 2 ? b3 : 1.0;
+  a = a + b * ca;
+  b = b * cb;
+}
 template <typename T> __device__ __forceinline__ T dpp_t(T x);
 template <> __device__ __forceinline__ float dpp_t<float>(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xF, 0xF, true));
@@ -2097,16 +2165,9 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
         incl[q] = run;
       }
       // exclusive prefix product over lanes (FP64)
-      double s = run;
-#pragma unroll
-      for (int ds = 0; ds < 6; ds++) {
-        const int d = 1 << ds;
-        const double o = __shfl_up(s, d);
-        if (lane >= d) s *= o;
-      }
-      double ex = __shfl_up(s, 1);
-      if (lane == 0) ex = 1.0;
-      const double trN = __shfl(s, W - 1);
+      const double s = wave_incl_prod(run);
+      const double ex = dpp_dk<0x138, 0xF>(1.0, s);  // lane - 1's, 1 in lane 0
+      const double trN = read_lane_d(s, W - 1);
       // radiance, sum_j (tr_{j-1} - tr_j) B_j (cirsradg_wave.f:946-951)
       double part = 0.0;
       double trp[RL], trx[RL];
@@ -2116,7 +2177,7 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
         trx[q] = ex * incl[q];
         if (act[q]) part += trp[q] * (double)om[q] * bb[q];
       }
-      double rg = xfac * wave_sum(part);
+      double rg = xfac * read_lane_d(wave_incl_sum(part), W - 1);
       if (pi.surf) {
         rg += xfac * trN * radground;
         coG = (T)(xfac * trN * gradground);
@@ -2127,7 +2188,7 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
       //   D_{m-1} = (B_{m-1} - B_m) + t_m D_m,   D_{N-1} = B_{N-1} - R_ground
       // (W_{m-1} = B_m (1 - t_m) + t_m W_m), so it keeps its relative accuracy
       // where B and W nearly cancel.  Affine suffix scan over lanes, FP64.
-      const double bup0 = __shfl_up(bb[RL - 1], 1);  // layer above this lane's first
+      const double bup0 = wave_shr1(bb[RL - 1]);  // layer above this lane's first
       double dN;
       {
         const int ob = (nlays - 1) / RL, oq = (nlays - 1) - ob * RL;
@@ -2135,7 +2196,7 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
 #pragma unroll
         for (int q = 0; q < RL; q++)
           if (q == oq) b0 = bb[q];
-        dN = __shfl(b0, ob) - radground;
+        dN = read_lane_d(b0, ob) - radground;
       }
       double cm[RL];
 #pragma unroll
@@ -2153,20 +2214,9 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
         }
       }
       double ia = ca, ib = cb;  // inclusive suffix composite
-#pragma unroll
-      for (int ds = 0; ds < 6; ds++) {
-        const int d = 1 << ds;
-        const double oa = __shfl_down(ia, d), ob = __shfl_down(ib, d);
-        if (lane + d < W) {
-          ia = ia + ib * oa;
-          ib = ib * ob;
-        }
-      }
-      double xa = __shfl_down(ia, 1), xb = __shfl_down(ib, 1);
-      if (lane == W - 1) {
-        xa = 0.0;
-        xb = 1.0;
-      }
+      wave_suffix_affine(ia, ib, lane);
+      // lane + 1's (the identity in the last lane)
+      const double xa = dpp_dk<0x130, 0xF>(0.0, ia), xb = dpp_dk<0x130, 0xF>(1.0, ib);
       double dv = xa + xb * dN;  // D of this lane's deepest active layer
       double Dm[RL];
 #pragma unroll
@@ -2300,25 +2350,26 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
     for (int q = 0; q < RL; q++) {
       if (!act[q]) continue;
       const int m = lane * RL + q;
-      for (int k1 = 0; k1 < nsw; k1++) {
+      // continuum rows for every parameter; the overlap's dk/dq only for the
+      // gases with a k-table and T (dust and the para-H2 fraction act on
+      // the continuum only).  Branch-free (the row index clamped, the value
+      // selected), so the loop unrolls as it does without those parameters.
+      auto term = [&](int k1) {
         const int k = R.isw[k1];
-        // continuum rows for every parameter; the overlap's dk/dq only for the
-        // gases with a k-table and T (dust and the para-H2 fraction act on
-        // the continuum only).  Branch-free (the row index clamped, the value
-        // selected), so the loop unrolls as it does without those parameters.
         const int kd = R.kdrow[k1];
         const T dkv = dkq[((long long)g * (nks + 1) + (kd >= 0 ? kd : 0)) * L + Lm[q]];
         const T dq = (dtcon[(long long)k * L + Lm[q]] + (kd >= 0 ? dkv : (T)0)) * sc[q];
         T c = dRdtau[q] * dq;
         if (k == ngas) c = c + dT[q];
-        const T x = c * dg;
-        T &s0 = acc[m * nsw + k1];
-        T &c0 = cmp[m * nsw + k1];
+        return c * dg;
+      };
+      auto kahan = [](T &s0, T &c0, T x) {
         const T y = x - c0;
         const T t = s0 + y;
         c0 = (t - s0) - y;
         s0 = t;
-      }
+      };
+      for (int k1 = 0; k1 < nsw; k1++) kahan(acc[m * nsw + k1], cmp[m * nsw + k1], term(k1));
     }
   }
   // ---- outputs ----
@@ -2339,10 +2390,7 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
     if (m >= L) continue;
     for (int k = 0; k < nparam; k++) jac[(long long)m * nparam + k] = (T)0;
     if (!act[q] || m >= nrows) continue;
-    for (int k1 = 0; k1 < nsw; k1++) {
-      const int k = R.isw[k1];
-      jac[(long long)m * nparam + k] = acc[m * nsw + k1];
-    }
+    for (int k1 = 0; k1 < nsw; k1++) jac[(long long)m * nparam + R.isw[k1]] = acc[m * nsw + k1];
   }
 }
 }  // namespace

@@ -34,6 +34,11 @@ __device__ __forceinline__ long long xcd_item(long long nblocks) {
   return (b % 8) * chunk + b / 8;
 }
 
+// 1/n for index arithmetic of the form floor((x + 0.5) / n) with x < 1024
+// and n <= 64: the hardware reciprocal (1 ulp) keeps the product far inside
+// the 0.5/n margin, where an IEEE division would cost about ten instructions
+__device__ __forceinline__ float rcp_idx(int n) { return __builtin_amdgcn_rcpf((float)n); }
+
 __device__ __forceinline__ int find_req(const long long *start, int nreq, long long item) {
   int lo = 0, hi = nreq;  // start[lo] <= item < start[hi]
   while (hi - lo > 1) {
@@ -1132,14 +1137,14 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   // whatever follows (their counts are clamped to NG)
   const int ngrp = count_groups(NG);
   const int rpg = (NG + ngrp - 1) / ngrp;
-  const int grp = (int)(((float)lane + 0.5f) * (1.0f / (float)NG));
+  const int grp = (int)(((float)lane + 0.5f) * rcp_idx(NG));
   const int bl = lane - grp * NG;
   const bool act = grp < ngrp;
   const int i0 = act ? grp * rpg : 0;
   // this round's row terms: delg_i K_i, delg_i dK_i/dT and delg_i P_si for
   // the gases already mixed (row i at slot (i / rpg) RPG + i % rpg)
   if (lane < NG) {
-    const int g2 = (int)(((float)lane + 0.5f) * (1.0f / (float)rpg));
+    const int g2 = (int)(((float)lane + 0.5f) * rcp_idx(rpg));
     double *rr = rrec + (g2 * RPG + lane - g2 * rpg) * RR;
     const double w = dl64[lane];
     rr[1] = w * (double)rows[lane];
@@ -1214,7 +1219,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   if (has) {
     const int s = recI[bl];
     const int l = s / E;
-    is = (int)(((float)l + 0.5f) * (1.0f / (float)lpr));
+    is = (int)(((float)l + 0.5f) * rcp_idx(lpr));
     const int js = (l - is * lpr) * E + (s & (E - 1));
     const double gs = recGs[bl];
     f = (G[bl + 1] - gs) / (recGe[bl] - gs) * w64[s];
@@ -1290,7 +1295,7 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   T *bufB = reinterpret_cast<T *>(ws + lo.bufB);
   int *sE = reinterpret_cast<int *>(ws + lo.sE);
   char *uni = ws + lo.uni;
-  const float rng = 1.0f / (float)NG;  // e -> i = floor((e + 0.5) / NG)
+  const float rng = rcp_idx(NG);  // e -> i = floor((e + 0.5) / NG)
   // lane -> (group gq, ordinate or bin gg): W / NG groups of NG lanes
   const int gpp = W / NG;
   const int gq = (int)(((float)lane + 0.5f) * rng);
@@ -1322,8 +1327,11 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
     // Coincident or zero slices (get_kg.f:319-376), W / NG gases at a time:
     // ln k bilinear when the first ordinate of all four corners is > 0
     // (KLOG), else k itself.
+    // gases between two table points, one bit each (a wave-uniform mask)
+    unsigned long long between = 0;
     for (int k0 = 0; k0 < ngas; k0 += gpp) {
       const int k = k0 + gq;
+      bool merge = false;
       if (gq < gpp && k < ngas) {
         const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
         if (wgk.base < 0 || wgk.coinc) {
@@ -1332,8 +1340,13 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
             interp_point<T, BF>(a, wgk.base, wgk.ob, i1, i2, i3, i4, gg, v, u, dudt, u2, dudt2, kk, dk);
           kgn[k * NG + gg] = kk;
           dkgn[k * NG + gg] = dk;
+        } else {
+          merge = gg == 0;
         }
       }
+      const unsigned long long lanes = __ballot(merge);
+      for (int q = 0; q < gpp && k0 + q < ngas; q++)
+        if ((lanes >> (q * NG)) & 1ull) between |= 1ull << (k0 + q);
     }
 
     // Wavenumber between two table points (get_kg.f:379-440): interpolate
@@ -1346,9 +1359,9 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
     T *rkK = reinterpret_cast<T *>(uni);
     T *rkD = rkK + W, *rkF = rkD + W;
     float *keyM = reinterpret_cast<float *>(rkF + W);  // FP32 form: exact keys
-    for (int k = 0; k < ngas; k++) {
+    for (unsigned long long mk = between; mk != 0; mk &= mk - 1) {
+      const int k = __builtin_ctzll(mk);
       const WaveGas wgk = a.wg[R.wg0 + (long long)iw * ngas + k];
-      if (wgk.base < 0 || wgk.coinc) continue;
       const int cnt = 2 * NG;
       const bool act = lane < cnt;
       const int gi = act ? (lane < NG ? lane : lane - NG) : 0;
@@ -1473,7 +1486,7 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   // this lane's sort slots: row li of the running distribution, columns
   // lj0 .. lj0 + E - 1 of the new gas (layer_lpr)
   const int lpr = layer_lpr(NG, E);
-  const float rlpr = 1.0f / (float)lpr;
+  const float rlpr = rcp_idx(lpr);
   const int li = (int)(((float)lane + 0.5f) * rlpr);
   const int lj0 = (lane - li * lpr) * E;
   const bool lrow = li < NG;

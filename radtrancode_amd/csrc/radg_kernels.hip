@@ -1138,14 +1138,14 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   const int ngrp = count_groups(NG);
   const int rpg = (NG + ngrp - 1) / ngrp;
   const int grp = (int)(((float)lane + 0.5f) * rcp_idx(NG));
-  const int bl = lane - grp * NG;
+  const int bl = lane - (int)__umul24(grp, NG);
   const bool act = grp < ngrp;
-  const int i0 = act ? grp * rpg : 0;
+  const int i0 = act ? (int)__umul24(grp, rpg) : 0;
   // this round's row terms: delg_i K_i, delg_i dK_i/dT and delg_i P_si for
   // the gases already mixed (row i at slot (i / rpg) RPG + i % rpg)
   if (lane < NG) {
     const int g2 = (int)(((float)lane + 0.5f) * rcp_idx(rpg));
-    double *rr = rrec + (g2 * RPG + lane - g2 * rpg) * RR;
+    double *rr = rrec + __umul24(__umul24(g2, RPG) + lane - __umul24(g2, rpg), RR);
     const double w = dl64[lane];
     rr[1] = w * (double)rows[lane];
     rr[2] = w * (double)rows[(NS - 1) * NG + lane];
@@ -1160,12 +1160,16 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   const unsigned cb = 0x80808080u - (unsigned)(bl + 1) * 0x01010101u;
   double Wt = 0.0, V = 0.0, Z = 0.0, QK = 0.0, QT = 0.0;
   double gi[RPG];
-  const unsigned char *rowb0 = gam + i0 * (lpr * E);
-  const double *rr0 = rrec + (act ? grp : 0) * RPG * RR;
+  // LDS pointers with 32-bit offsets, strides as 24-bit products (full rate)
+  typedef __attribute__((address_space(3))) const unsigned long long lds_u64;
+  typedef __attribute__((address_space(3))) const double lds_f64;
+  const int rstride = lpr * E;  // bytes per gamma row
+  lds_u64 *rowg0 = (lds_u64 *)(gam + __umul24(i0, rstride));
+  lds_f64 *rr0 = (lds_f64 *)(rrec + __umul24(act ? grp : 0, RPG * RR));
+  lds_f64 *ctl = (lds_f64 *)ctab;
 #pragma unroll
   for (int t = 0; t < RPG; t++) {
-    const unsigned long long *rowg =
-        reinterpret_cast<const unsigned long long *>(rowb0 + t * (lpr * E));
+    lds_u64 *rowg = rowg0 + __umul24(t, rstride >> 3);
     unsigned ge = 0;  // bytes >= edge
 #pragma unroll
     for (int d = 0; d < ND; d += 2) {
@@ -1180,9 +1184,9 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
       }
     }
     const int c = min(4 * ND - (int)ge, NG);
-    const double *ct = ctab + 3 * c;
+    lds_f64 *ct = ctl + __umul24(c, 3);
     const double gw = ct[0];
-    const double *rr = rr0 + t * RR;
+    lds_f64 *rr = rr0 + __umul24(t, RR);
     Wt += rr[0] * gw;
     V += rr[0] * ct[1];
     Z += rr[0] * ct[2];
@@ -1192,7 +1196,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   }
   wave_sync();  // gam and ctab are read: the partial sums take their place
   if (act && grp > 0) {
-    double *pp = part + ((grp - 1) * NG + bl) * 5;
+    double *pp = part + __umul24(__umul24(grp - 1, NG) + bl, 5);
     pp[0] = Wt;
     pp[1] = V;
     pp[2] = Z;
@@ -1202,7 +1206,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   wave_sync();
   if (grp == 0) {
     for (int g2 = 1; g2 < ngrp; g2++) {
-      const double *pp = part + ((g2 - 1) * NG + bl) * 5;
+      const double *pp = part + __umul24(__umul24(g2 - 1, NG) + bl, 5);
       Wt += pp[0];
       V += pp[1];
       Z += pp[2];
@@ -1220,7 +1224,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
     const int s = recI[bl];
     const int l = s / E;
     is = (int)(((float)l + 0.5f) * rcp_idx(lpr));
-    const int js = (l - is * lpr) * E + (s & (E - 1));
+    const int js = (l - (int)__umul24(is, lpr)) * E + (s & (E - 1));
     const double gs = recGs[bl];
     f = (G[bl + 1] - gs) / (recGe[bl] - gs) * w64[s];
     const T kv = rows[is] + k2[js] * a2;
@@ -1247,12 +1251,12 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   for (int s = 0; s < r; s++) {
     double qv = 0.0;
 #pragma unroll
-    for (int t = 0; t < RPG; t++) qv += rr0[t * RR + 3 + s] * gi[t];
+    for (int t = 0; t < RPG; t++) qv += rr0[__umul24(t, RR) + 3 + s] * gi[t];
     wave_sync();  // the previous partials are read
-    if (act && grp > 0) part[(grp - 1) * NG + bl] = qv;
+    if (act && grp > 0) part[__umul24(grp - 1, NG) + bl] = qv;
     wave_sync();
     if (grp == 0)
-      for (int g2 = 1; g2 < ngrp; g2++) qv += part[(g2 - 1) * NG + bl];
+      for (int g2 = 1; g2 < ngrp; g2++) qv += part[__umul24(g2 - 1, NG) + bl];
     if (has) qv += f * (double)rows[(1 + s) * NG + is];
     const double pq = wave_shr1(qv);
     if (blane) nxt[(1 + s) * NG + bl] = reached ? (T)((qv - pq) * inv) : (T)0;

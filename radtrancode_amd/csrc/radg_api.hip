@@ -25,7 +25,7 @@
 size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64);
 int radg_layer_waves(int ng, int ngas, int E, int fp64);
 int radg_layer_stride(int ng, int ngas, int E, int fp64);
-size_t radg_lds_emission_bytes(int RL, int nsw, int fp64);
+size_t radg_lds_emission_bytes(int RL, int nkrow, int fp64);
 int radg_layer_E(int ng);
 int radg_emission_RL(int maxlays);
 hipError_t radg_launch_layer_lds(const KernelArgs &a, int mode, int E, size_t lds, hipStream_t st);
@@ -1427,7 +1427,7 @@ std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_req
     ec.RL = pl.cls_rl[c];
     ec.nreq = (int)pl.cls_members[c].size();
     ec.nA = ec.nB = ec.nW = 0;
-    int maxnsw = 1, maxgas = 1;
+    int maxnsw = 1, maxgas = 1, maxkrow = 1;
     long long maxblock = 0;
     ec.disc = false;
     for (int r : pl.cls_members[c]) {
@@ -1436,6 +1436,7 @@ std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_req
       ec.nW += reqs[r].nwave;
       maxnsw = std::max(maxnsw, reqs[r].nsw);
       maxgas = std::max(maxgas, reqs[r].ngas);
+      maxkrow = std::max(maxkrow, pl.packs[r].d.nks + 1);
       maxblock = std::max(maxblock, inter_block(ng, reqs[r].nlayer, pl.packs[r].d.nks, pl.packs[r].d.nparam));
       for (int p = 0; p < reqs[r].npath; p++) {
         const int im = reqs[r].imod[p];
@@ -1447,7 +1448,7 @@ std::vector<radg_batch::EmClass> emission_classes(const Plan &pl, const radg_req
     ec.start = reinterpret_cast<const long long *>(m + pl.o_clsstart) + soff;
     ec.startA = reinterpret_cast<const long long *>(m + pl.o_clsstartA) + soff;
     ec.startW = reinterpret_cast<const long long *>(m + pl.o_clsstartW) + soff;
-    ec.lds = radg_lds_emission_bytes(ec.RL, maxnsw, fp64);
+    ec.lds = radg_lds_emission_bytes(ec.RL, maxkrow, fp64);
     ec.ldsA = radg_lds_layer_bytes(ng, maxgas, E, fp64);
     ec.lwaves = radg_layer_waves(ng, maxgas, E, fp64);
     ec.lwstride = radg_layer_stride(ng, maxgas, E, fp64);

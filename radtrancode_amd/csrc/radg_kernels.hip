@@ -2074,7 +2074,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 
 // One (request, wavenumber iw, path p) item on one wavefront, reading the
 // layer results ktot [NG][L], dkq [NG][ngas+1][L], tcon [L] and
-// dtcon [nparam][L], with acc and cmp [RL*64][nsw] of scratch.  EXT: the form with the
+// dtcon [nparam][L], with s2 [nks+1][RL][64] (FP64) of scratch.  EXT: the form with the
 // imod 8, 13, 14 and 18 branches and the stellar-spectrum factors, launched
 // only for classes that hold such a path or request (their registers would
 // cost the other forms occupancy).
@@ -2084,7 +2084,7 @@ __device__ __forceinline__ double wave_sum(double x) {
 template <typename T, int RL, bool EXT>
 __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq &R, int iw, int p,
                                               const T *ktot, const T *dkq, const T *tcon,
-                                              const T *dtcon, T *acc, T *cmp, int lane) {
+                                              const T *dtcon, double *s2, int lane) {
   const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath, nsw = R.nsw,
             nparam = R.nparam, nks = R.nks;
   const PathInfo pi0 = a.paths[R.path0 + p];
@@ -2101,6 +2101,7 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
   bool act[RL];
   T sc[RL], tc[RL], db[RL];
   double bb[RL];  // Planck source of each layer, FP64
+  double s1[RL], s3[RL];  // sum_g delg_g dR/dtau and sum_g delg_g dR/dT (Planck term)
   // xfac (cirsradg_wave.f:912-925 for imod 3, :1134-1149 with its 2 pi for
   // imod 18) scales each g-ordinate's terms in FP64 before they are rounded
   // to T, as the reference applies it per term: with iform 1 it reaches 1e21
@@ -2123,10 +2124,9 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
       bb[q] = planck_d(R.ispace, v, em);
       db[q] = (T)planckg_d(R.ispace, v, em);
     }
-    for (int k1 = 0; k1 < nsw; k1++) {
-      acc[(lane * RL + q) * nsw + k1] = 0;
-      cmp[(lane * RL + q) * nsw + k1] = 0;
-    }
+    s1[q] = 0.0;
+    s3[q] = 0.0;
+    for (int kd = 0; kd <= nks; kd++) s2[(kd * RL + q) * W + lane] = 0.0;
   }
   // ground term (cirsradg_wave.f:992-1024; imod 18 :1159,1212-1218, where
   // tsurf > 0 always)
@@ -2362,31 +2362,23 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
     }
     rad.add(coR * dg);
     gts.add(coG * dg);
-    // Jacobian: dR/dq = dR/dtau * scale * (dtaucon/dq + dk/dq) (+ Planck term)
+    // Jacobian: dR/dq = sum_g delg_g dR/dtau scale (dtaucon/dq + dk/dq)
+    // (+ the Planck term for T).  scale and dtaucon/dq do not depend on g,
+    // so each layer keeps sum_g delg_g dR/dtau (s1), the Planck term's
+    // g-sum (s3) and, per k row (the gases with a k-table, then T),
+    // sum_g delg_g dR/dtau dk/dq (s2, lane-contiguous in LDS), all in FP64;
+    // the columns are formed once after the g loop.
 #pragma unroll
     for (int q = 0; q < RL; q++) {
       if (!act[q]) continue;
-      const int m = lane * RL + q;
-      // continuum rows for every parameter; the overlap's dk/dq only for the
-      // gases with a k-table and T (dust and the para-H2 fraction act on
-      // the continuum only).  Branch-free (the row index clamped, the value
-      // selected), so the loop unrolls as it does without those parameters.
-      auto term = [&](int k1) {
-        const int k = R.isw[k1];
-        const int kd = R.kdrow[k1];
-        const T dkv = dkq[((long long)g * (nks + 1) + (kd >= 0 ? kd : 0)) * L + Lm[q]];
-        const T dq = (dtcon[(long long)k * L + Lm[q]] + (kd >= 0 ? dkv : (T)0)) * sc[q];
-        T c = dRdtau[q] * dq;
-        if (k == ngas) c = c + dT[q];
-        return c * dg;
-      };
-      auto kahan = [](T &s0, T &c0, T x) {
-        const T y = x - c0;
-        const T t = s0 + y;
-        c0 = (t - s0) - y;
-        s0 = t;
-      };
-      for (int k1 = 0; k1 < nsw; k1++) kahan(acc[m * nsw + k1], cmp[m * nsw + k1], term(k1));
+      const double rd = (double)dRdtau[q] * (double)dg;
+      s1[q] += rd;
+      s3[q] += (double)dT[q] * (double)dg;
+      const T *dk = dkq + (long long)g * (nks + 1) * L + Lm[q];
+      for (int kd = 0; kd <= nks; kd++) {
+        double &acc = s2[(kd * RL + q) * W + lane];
+        acc = fma((double)dk[(long long)kd * L], rd, acc);
+      }
     }
   }
   // ---- outputs ----
@@ -2407,14 +2399,21 @@ __device__ __forceinline__ void emission_path(const KernelArgs &a, const DevReq 
     if (m >= L) continue;
     for (int k = 0; k < nparam; k++) jac[(long long)m * nparam + k] = (T)0;
     if (!act[q] || m >= nrows) continue;
-    for (int k1 = 0; k1 < nsw; k1++) jac[(long long)m * nparam + R.isw[k1]] = acc[m * nsw + k1];
+    for (int k1 = 0; k1 < nsw; k1++) {
+      const int k = R.isw[k1], kd = R.kdrow[k1];
+      double c = (double)dtcon[(long long)k * L + Lm[q]] * s1[q];
+      if (kd >= 0) c += s2[(kd * RL + q) * W + lane];
+      c *= (double)sc[q];
+      if (k == ngas) c += s3[q];
+      jac[(long long)m * nparam + k] = (T)c;
+    }
   }
 }
 }  // namespace
 
 // Emission as its own launch: one wavefront per (request, wavenumber, path)
 // item of a layers-per-lane class, reading the intermediate block in HBM.
-// LDS: acc[RL*64][nsw] and comp[RL*64][nsw] in T.
+// LDS: the FP64 g-sums s2[nks+1][RL][64].
 template <typename T, int RL, bool EXT>
 __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nblocks) {
   const long long item = xcd_item(nblocks);
@@ -2431,9 +2430,7 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const T *dkq = inter + (long long)NG * L;
   const T *tcon = dkq + (long long)NG * (R.nks + 1) * L;
   extern __shared__ double smem_d[];
-  T *acc = reinterpret_cast<T *>(smem_d);
-  emission_path<T, RL, EXT>(a, R, iw, p, inter, dkq, tcon, tcon + L, acc,
-                            acc + (long long)RL * W * R.nsw, lane);
+  emission_path<T, RL, EXT>(a, R, iw, p, inter, dkq, tcon, tcon + L, smem_d, lane);
 }
 
 // The whole wavenumber in one workgroup of RADG_FUSED_WAVES wavefronts:
@@ -2468,10 +2465,9 @@ __global__ __launch_bounds__(64 * RADG_FUSED_WAVES) void fused_kernel(KernelArgs
   for (int j = wv; j < L; j += RADG_FUSED_WAVES)
     layer_item<T, E, BF>(a, R, iw, j, hdr, ws, ktot, dkq, L, lane);
   __syncthreads();
-  T *acc = reinterpret_cast<T *>(ws);
+  double *s2 = reinterpret_cast<double *>(ws);
   for (int p = wv; p < R.npath; p += RADG_FUSED_WAVES)
-    emission_path<T, RL, EXT>(a, R, iw, p, ktot, dkq, tcon, dtcon, acc,
-                              acc + (long long)RL * W * R.nsw, lane);
+    emission_path<T, RL, EXT>(a, R, iw, p, ktot, dkq, tcon, dtcon, s2, lane);
 }
 
 // ======================================================= path quadrature
@@ -2560,9 +2556,9 @@ size_t radg_lds_layer_bytes(int ng, int ngas, int E, int fp64) {
          (size_t)radg_layer_waves(ng, ngas, E, fp64) * (size_t)layer_layout(ng, ngas, E, ts).total;
 }
 
-size_t radg_lds_emission_bytes(int RL, int nsw, int fp64) {
-  const size_t ts = fp64 ? 8 : 4;
-  return (size_t)2 * RL * W * nsw * ts + 16;
+size_t radg_lds_emission_bytes(int RL, int nkrow, int fp64) {
+  (void)fp64;  // the g-sums are FP64 in both forms
+  return (size_t)RL * W * nkrow * sizeof(double) + 16;
 }
 
 // The fused kernel's LDS for requests of up to maxL layers, maxgas gases,
@@ -2576,7 +2572,8 @@ size_t radg_lds_fused_bytes(int mode, int ng, int E, int RL, int maxgas, int max
   const int ts = mode == 0 ? 8 : 4;
   const int rl = RL <= 2 ? 2 : 4;
   int w = layer_layout(ng, maxgas, E, ts).total;
-  const int em = lds_round16(2 * rl * W * maxnsw * ts);
+  (void)maxnsw;
+  const int em = lds_round16(rl * W * (maxgas + 1) * (int)sizeof(double));
   if (w < em) w = em;
   *ws = w;
   return (size_t)layer_header_bytes(ng, E, ts) + (size_t)lds_round16(ts * (int)maxblock) +

@@ -102,7 +102,13 @@ struct TableSet {
   std::mutex form_mu;
   double *d_G = nullptr, *d_delg64 = nullptr;
   float *d_delg32 = nullptr;
+  // the layer kernel's LDS header for E = hdr_E slots per lane, as T = float
+  // (d_hdr[0]) and T = double (d_hdr[1])
+  char *d_hdr[2] = {nullptr, nullptr};
+  int hdr_E = 0;
   ~TableSet() {
+    for (char *h : d_hdr)
+      if (h) (void)hipFree(h);
     if (d_k64) (void)hipFree(d_k64);
     if (d_k32) (void)hipFree(d_k32);
     if (d_k16) (void)hipFree(d_k16);
@@ -1358,6 +1364,7 @@ void set_kernel_args(KernelArgs &a, char *m, const Plan &pl, int nreq, const Tab
   a.G = ts->d_G;
   a.delg64 = ts->d_delg64;
   a.delg32 = ts->d_delg32;
+  a.hdr_img = ts->d_hdr[mode == RADG_MODE_FP64 ? 1 : 0];
   a.kcia = cia->d_kcia;
   a.cia_temps = cia->d_temps;
   a.cia_dnu = cia->dnu;
@@ -1523,6 +1530,39 @@ int radg_set_mode(int mode) {
 int radg_get_mode(void) { return g_mode; }
 const char *radg_last_error(void) { return g_err.c_str(); }
 
+// The layer kernel's LDS header (header_layout): G[0..ng] then +inf to
+// ng + 5 (rankg.f:59-65 edges; the +inf pad lets searches read past ng), the
+// g weights in FP64 and in T, the FP64 weight delg_i delg_j of the element in
+// each sort slot (0 for padding) and the unforced cumulative weights summed
+// in order (rankg.f:59-62).  Every workgroup copies it into LDS.
+static std::vector<char> layer_header_image(int ng, int E, int ts, const double *G,
+                                            const double *delg) {
+  const HeaderLayout H = header_layout(ng, E, ts);
+  std::vector<char> img((size_t)H.total, 0);
+  double *g = reinterpret_cast<double *>(img.data() + H.G);
+  double *dl64 = reinterpret_cast<double *>(img.data() + H.dl64);
+  double *w64 = reinterpret_cast<double *>(img.data() + H.w64);
+  double *gw = reinterpret_cast<double *>(img.data() + H.Gw);
+  for (int i = 0; i < ng + 6; i++) g[i] = (i <= ng) ? G[i] : HUGE_VAL;
+  for (int i = 0; i < ng; i++) {
+    dl64[i] = delg[i];
+    if (ts == 8) reinterpret_cast<double *>(img.data() + H.dlg)[i] = delg[i];
+    else reinterpret_cast<float *>(img.data() + H.dlg)[i] = (float)delg[i];
+  }
+  double s = 0.0;
+  gw[0] = 0.0;
+  for (int i = 0; i < ng; i++) {
+    s += delg[i];
+    gw[i + 1] = s;
+  }
+  const int lpr = layer_lpr(ng, E);
+  for (int sl = 0; sl < RADG_WAVE * E; sl++) {
+    const int l = sl / E, i = l / lpr, jx = (l - i * lpr) * E + (sl & (E - 1));
+    w64[sl] = (i < ng && jx < ng) ? delg[i] * delg[jx] : 0.0;
+  }
+  return img;
+}
+
 // The k-table registry (lbl false) or the line-by-line one (lbl true: one
 // g-ordinate of weight 1, every table on one regular wavenumber grid).
 static int register_tables(bool lbl, int ntab, const int *idgas, const int *isogas, const int *npoint,
@@ -1618,6 +1658,12 @@ static int register_tables(bool lbl, int ntab, const int *idgas, const int *isog
   HIPCHK(hipMemcpy(ts->d_G, ts->G.data(), (ng + 1) * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ts->d_delg64, delg, ng * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ts->d_delg32, dg32.data(), ng * sizeof(float), hipMemcpyHostToDevice));
+  ts->hdr_E = radg_layer_E(ng);
+  for (int f = 0; f < 2; f++) {
+    const std::vector<char> img = layer_header_image(ng, ts->hdr_E, f ? 8 : 4, ts->G.data(), delg);
+    HIPCHK(hipMalloc(&ts->d_hdr[f], img.size()));
+    HIPCHK(hipMemcpy(ts->d_hdr[f], img.data(), img.size(), hipMemcpyHostToDevice));
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   if (lbl) g_lbl = ts;
   else g_tables = ts;

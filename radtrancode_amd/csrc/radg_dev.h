@@ -248,6 +248,7 @@ struct KernelArgs {
   const double *G;             // g boundaries [ng+1], G[ng] forced >= 1
   const double *delg64;
   const float *delg32;
+  const char *hdr_img;         // the layer kernel's LDS header, built once per table set and form
   // CIA
   const double *kcia;          // [1501][25][9]
   const double *cia_temps;     // [25]

@@ -949,39 +949,17 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Fills the workgroup header: G (g boundaries, G[NG] >= 1, rankg.f:59-65),
-// the g weights in FP64 and in T, and the FP64 weight delg_i delg_j of the
-// element in each sort slot (0 for padding).
+// Fills the workgroup header: G (g boundaries, G[NG] >= 1, rankg.f:59-65,
+// then +inf), the g weights in FP64 and in T, the FP64 weight delg_i delg_j
+// of the element in each sort slot (0 for padding) and the cumulative
+// weights (rankg.f:59-62).  The host builds it once per table set and form;
+// a workgroup copies it, 16 bytes per thread and step.
 template <typename T, int E>
 __device__ __forceinline__ void layer_header(const KernelArgs &a, char *hdr, int tid, int nthr) {
-  const int NG = a.ng;
-  const HeaderLayout H = header_layout(NG, E, (int)sizeof(T));
-  double *G = reinterpret_cast<double *>(hdr + H.G);
-  double *dl64 = reinterpret_cast<double *>(hdr + H.dl64);
-  T *dlg = reinterpret_cast<T *>(hdr + H.dlg);
-  double *w64 = reinterpret_cast<double *>(hdr + H.w64);
-  double *Gw = reinterpret_cast<double *>(hdr + H.Gw);
-  const T *delg = (sizeof(T) == 8) ? reinterpret_cast<const T *>(a.delg64)
-                                   : reinterpret_cast<const T *>(a.delg32);
-  for (int i = tid; i < NG + 6; i += nthr) G[i] = (i <= NG) ? a.G[i] : dinf<double>();
-  for (int i = tid; i < NG; i += nthr) {
-    dl64[i] = a.delg64[i];
-    dlg[i] = delg[i];
-  }
-  // unforced cumulative weights, summed in order as rankg.f:59-62 does
-  if (tid == 0) {
-    double s = 0.0;
-    Gw[0] = 0.0;
-    for (int i = 0; i < NG; i++) {
-      s += a.delg64[i];
-      Gw[i + 1] = s;
-    }
-  }
-  const int lpr = layer_lpr(NG, E);
-  for (int s = tid; s < W * E; s += nthr) {
-    const int l = s / E, i = l / lpr, jx = (l - i * lpr) * E + (s & (E - 1));
-    w64[s] = (i < NG && jx < NG) ? a.delg64[i] * a.delg64[jx] : 0.0;
-  }
+  const int n16 = layer_header_bytes(a.ng, E, (int)sizeof(T)) >> 4;
+  const uint4 *src = reinterpret_cast<const uint4 *>(a.hdr_img);
+  uint4 *dst = reinterpret_cast<uint4 *>(hdr);
+  for (int i = tid; i < n16; i += nthr) dst[i] = src[i];
 }
 
 // ---- row-count rebin of one overlap round --------------------------------

@@ -39,7 +39,14 @@ __device__ __forceinline__ long long xcd_item(long long nblocks) {
 // the 0.5/n margin, where an IEEE division would cost about ten instructions
 __device__ __forceinline__ float rcp_idx(int n) { return __builtin_amdgcn_rcpf((float)n); }
 
-__device__ __forceinline__ int find_req(const long long *start, int nreq, long long item) {
+// The request r of an item, start[r] <= item < start[r + 1], with
+// start[nreq] = total.  The requests of a launch class often hold equally
+// many items, so the proportional guess is tried first (one pair of loads
+// rather than a chain of dependent ones), then a binary search.
+__device__ __forceinline__ int find_req(const long long *start, int nreq, long long item,
+                                        long long total) {
+  const int g = min(max((int)((float)item * ((float)nreq / (float)total)), 0), nreq - 1);
+  if (start[g] <= item && item < start[g + 1]) return g;
   int lo = 0, hi = nreq;  // start[lo] <= item < start[hi]
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
@@ -47,6 +54,12 @@ __device__ __forceinline__ int find_req(const long long *start, int nreq, long l
     else hi = mid;
   }
   return lo;
+}
+
+// n / d for an item's offset n >= 0 within its request: a 32-bit division
+// where n fits (request sizes are bounded far below 2^31 items)
+__device__ __forceinline__ int div_item(long long n, int d) {
+  return (n <= 0x7fffffffLL) ? (int)((unsigned)n / (unsigned)d) : (int)(n / d);
 }
 
 template <typename T> __device__ __forceinline__ T dexp(T x);
@@ -1862,11 +1875,11 @@ __global__ __launch_bounds__(256, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArg
   __syncthreads();
   const long long item = xcd_item(nblocks) * a.lwaves + wv;
   if (item >= a.nA) return;
-  const int rqc = find_req(a.startA, a.nreqA, item);
+  const int rqc = find_req(a.startA, a.nreqA, item, a.nA);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, NG = a.ng;
   const long long loc = item - a.startA[rqc];
-  const int iw = (int)(loc / L);
+  const int iw = div_item(loc, L);
   const int j = (int)(loc - (long long)iw * L);
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
   layer_item<T, E, BF>(a, R, iw, j, hdr,
@@ -1880,11 +1893,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= a.nA) return;
-  const int rqc = find_req(a.startA, a.nreqA, item);
+  const int rqc = find_req(a.startA, a.nreqA, item, a.nA);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng;
   const long long loc = item - a.startA[rqc];
-  const int iw = (int)(loc / L);
+  const int iw = div_item(loc, L);
   const int j = (int)(loc - (long long)iw * L);
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
   T *tcon = inter + (long long)NG * L + (long long)NG * (R.nks + 1) * L;
@@ -1903,11 +1916,11 @@ template <typename T, int BF>
 __global__ __launch_bounds__(256) void lbl_kernel(KernelArgs a) {
   const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= a.nA) return;
-  const int rqc = find_req(a.startA, a.nreqA, item);
+  const int rqc = find_req(a.startA, a.nreqA, item, a.nA);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, ngas = R.ngas, nks = R.nks;
   const long long loc = item - a.startA[rqc];
-  const int iw = (int)(loc / L);
+  const int iw = div_item(loc, L);
   const int j = (int)(loc - (long long)iw * L);
   T *ok = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(1, L, nks, R.nparam);
   T *od = ok + L;
@@ -2397,11 +2410,11 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const long long item = xcd_item(nblocks);
   if (item >= a.nB) return;
   const int lane = threadIdx.x;
-  const int rqc = find_req(a.startB, a.nreqB, item);
+  const int rqc = find_req(a.startB, a.nreqB, item, a.nB);
   const DevReq &R = a.reqs[a.reqmapB[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath;
   const long long loc = item - a.startB[rqc];
-  const int iw = (int)(loc / npath);
+  const int iw = div_item(loc, npath);
   const int p = (int)(loc - (long long)iw * npath);
   const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
                    (long long)iw * inter_block(NG, L, R.nks, R.nparam);
@@ -2424,7 +2437,7 @@ __global__ __launch_bounds__(64 * RADG_FUSED_WAVES) void fused_kernel(KernelArgs
   if (item >= a.nB) return;
   const int tid = threadIdx.x;
   const int wv = tid / W, lane = tid - wv * W;
-  const int rqc = find_req(a.startB, a.nreqB, item);
+  const int rqc = find_req(a.startB, a.nreqB, item, a.nB);
   const DevReq &R = a.reqs[a.reqmapB[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng;
   const int iw = (int)(item - a.startB[rqc]);
@@ -2454,7 +2467,7 @@ __global__ __launch_bounds__(64) void quadrature_kernel(KernelArgs a) {
   const long long item = blockIdx.x;
   if (item >= a.nC) return;
   const int lane = threadIdx.x;
-  const int rq = find_req(a.startC, a.nreq, item);
+  const int rq = find_req(a.startC, a.nreq, item, a.nC);
   const DevReq &R = a.reqs[rq];
   const int iw = (int)(item - R.itemC0);
   const int L = R.L, npath = R.npath, nparam = R.nparam;

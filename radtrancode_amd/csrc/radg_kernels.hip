@@ -1478,19 +1478,6 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   }
   wave_sync();
 
-  // this lane's sort slots: row li of the running distribution, columns
-  // lj0 .. lj0 + E - 1 of the new gas (layer_lpr)
-  const int lpr = layer_lpr(NG, E);
-  const float rlpr = rcp_idx(lpr);
-  const int li = (int)(((float)lane + 0.5f) * rlpr);
-  const int lj0 = (lane - li * lpr) * E;
-  const bool lrow = li < NG;
-  // (i, j) of slot s
-  auto slot_ij = [&](int sl, int &i, int &jx) {
-    const int l = sl / E;
-    i = (int)(((float)l + 0.5f) * rlpr);
-    jx = (l - i * lpr) * E + (sl & (E - 1));
-  };
   const double *w64 = reinterpret_cast<const double *>(hdr + H.w64);
   const double *Gw = reinterpret_cast<const double *>(hdr + H.Gw);
 
@@ -1498,7 +1485,28 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
     if (a.rebin && ngas > 1 && counts_fit<E>(NG)) counts_init<T, E>(NG, ngas, dl64, uni, lane);
   }
   T *rows = bufA, *nxt = bufB;
-  for (int r = 1; r < ngas; r++) {
+  const int ngas_all = ngas;
+  for (int r = 1; r < ngas_all; r++) {
+    // Opaque per-round copies of NG, the gas count and the lane number: the
+    // round's uniform offsets and masks and its lane indices are formed from
+    // them inside the round (scalar and cheap) instead of being hoisted out
+    // of the loop and held, spilled, across it.
+    int ng_r = NG, ngas_r = ngas_all, lane_r = lane;
+    asm volatile("" : "+s"(ng_r), "+s"(ngas_r), "+v"(lane_r));
+    const int NG = ng_r, ngas = ngas_r, lane = lane_r, NS = ngas + 2;
+    // this lane's sort slots: row li of the running distribution, columns
+    // lj0 .. lj0 + E - 1 of the new gas (layer_lpr)
+    const int lpr = layer_lpr(NG, E);
+    const float rlpr = rcp_idx(lpr), rng = rcp_idx(NG);
+    const int li = (int)(((float)lane + 0.5f) * rlpr);
+    const int lj0 = (lane - li * lpr) * E;
+    const bool lrow = li < NG;
+    // (i, j) of slot s
+    auto slot_ij = [&](int sl, int &i, int &jx) {
+      const int l = sl / E;
+      i = (int)(((float)l + 0.5f) * rlpr);
+      jx = (l - i * lpr) * E + (sl & (E - 1));
+    };
     const T a2 = (T)amo[r];
     const T *k2 = kgn + r * NG;
     const T *d2 = dkgn + r * NG;

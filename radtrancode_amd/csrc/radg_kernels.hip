@@ -1871,8 +1871,14 @@ __device__ __forceinline__ void continuum_layer(const KernelArgs &a, const DevRe
 // The layer pass as its own launch: one wavefront per (request,
 // wavenumber, layer) item, a.lwaves consecutive items per workgroup sharing
 // the header, results to the intermediate block in HBM.
+// Occupancy: the FP32 forms with 8 slots per lane are held to 80 VGPRs, six
+// wavefronts per SIMD, which is also what their LDS (six 4-wavefront
+// workgroups per CU at 20 g-ordinates and 3 gases) allows: the few spills
+// this costs are cheaper than the latency that 4 or 5 wavefronts leave
+// exposed (measured 11.1, 10.5 and 10.4 ms per 16 spectra at 4, 5 and 6).
 template <typename T, int E, int BF>
-__global__ __launch_bounds__(256, (E >= 16 ? 1 : 4)) void layer_kernel(KernelArgs a, long long nblocks) {
+__global__ __launch_bounds__(256, (E >= 16 ? 1 : (E == 8 && sizeof(T) == 4) ? 6 : 4)) void layer_kernel(
+    KernelArgs a, long long nblocks) {
   const int tid = threadIdx.x;
   // the wavefront number is uniform: in a scalar register, every scratch
   // address of the item stays scalar

@@ -39,6 +39,16 @@ __device__ __forceinline__ long long xcd_item(long long nblocks) {
 // the 0.5/n margin, where an IEEE division would cost about ten instructions
 __device__ __forceinline__ float rcp_idx(int n) { return __builtin_amdgcn_rcpf((float)n); }
 
+// 1/x for x > 0 to within a few FP64 ulps: the hardware reciprocal and two
+// Newton steps (each doubles the correct bits), about half the instructions
+// of a correctly rounded division
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // The request r of an item, start[r] <= item < start[r + 1], with
 // start[nreq] = total.  The requests of a launch class often hold equally
 // many items, so the proportional guess is tried first (one pair of loads
@@ -1217,7 +1227,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
     is = (int)(((float)l + 0.5f) * rcp_idx(lpr));
     const int js = (l - (int)__umul24(is, lpr)) * E + (s & (E - 1));
     const double gs = recGs[bl];
-    f = (G[bl + 1] - gs) / (recGe[bl] - gs) * w64[s];
+    f = (G[bl + 1] - gs) * rcp_nr(recGe[bl] - gs) * w64[s];
     const T kv = rows[is] + k2[js] * a2;
     const T tv = rows[(NS - 1) * NG + is] + d2[js] * a2;
     Wtf += f;
@@ -1231,7 +1241,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   const bool reached = blane && (bl == 0 || bl <= gtot);
   const bool norm = (bl + 1 <= gtot) || bl == NG - 1;
   double inv = 1.0;
-  if (reached && norm) inv = 1.0 / (Wtf - pW);
+  if (reached && norm) inv = rcp_nr(Wtf - pW);
   if (blane) {
     nxt[bl] = reached ? (T)((QKf - pK) * inv) : (T)0;
     nxt[(1 + r) * NG + bl] = reached ? (T)((QNf - pN) * inv) : (T)0;

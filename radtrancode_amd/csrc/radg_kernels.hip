@@ -53,9 +53,14 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // start[nreq] = total.  The requests of a launch class often hold equally
 // many items, so the proportional guess is tried first (one pair of loads
 // rather than a chain of dependent ones), then a binary search.
+// UNI: item is the same in every lane (one item per wavefront), and the
+// guess is kept in a scalar register so that the loads below, and every
+// address derived from the request, stay scalar.
+template <bool UNI>
 __device__ __forceinline__ int find_req(const long long *start, int nreq, long long item,
                                         long long total) {
-  const int g = min(max((int)((float)item * ((float)nreq / (float)total)), 0), nreq - 1);
+  int g = min(max((int)((float)item * ((float)nreq / (float)total)), 0), nreq - 1);
+  if constexpr (UNI) g = __builtin_amdgcn_readfirstlane(g);
   if (start[g] <= item && item < start[g + 1]) return g;
   int lo = 0, hi = nreq;  // start[lo] <= item < start[hi]
   while (hi - lo > 1) {
@@ -68,8 +73,44 @@ __device__ __forceinline__ int find_req(const long long *start, int nreq, long l
 
 // n / d for an item's offset n >= 0 within its request: a 32-bit division
 // where n fits (request sizes are bounded far below 2^31 items)
+template <bool UNI>
 __device__ __forceinline__ int div_item(long long n, int d) {
-  return (n <= 0x7fffffffLL) ? (int)((unsigned)n / (unsigned)d) : (int)(n / d);
+  const int q = (n <= 0x7fffffffLL) ? (int)((unsigned)n / (unsigned)d) : (int)(n / d);
+  // UNI: n is the same in every lane, and the quotient stays scalar
+  if constexpr (UNI) return __builtin_amdgcn_readfirstlane(q);
+  return q;
+}
+
+// A uniform 64-bit value (a field read through a vector load) kept in
+// scalar registers, so addresses formed from it stay scalar.
+__device__ __forceinline__ long long rfl64(long long x) {
+  const unsigned long long u = (unsigned long long)x;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// The lane number formed where it is used (two mbcnt in a volatile asm, which
+// the compiler neither hoists nor merges with another), so that no copy of
+// it is held, and spilled, across the overlap-round loop.
+__device__ __forceinline__ int lane_fresh() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
+
+// Lane (lane ^ m)'s value by ds_bpermute, with the address formed from the
+// caller's lane number: a shuffle that forms it from its own lane id has
+// that id hoisted out of the overlap-round loop and spilled across it.
+__device__ __forceinline__ int bperm_xor(int x, int lane, int m) {
+  return __builtin_amdgcn_ds_bpermute((lane ^ m) << 2, x);
+}
+__device__ __forceinline__ double bperm_xor(double x, int lane, int m) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const int lo = bperm_xor((int)(unsigned)u, lane, m);
+  const int hi = bperm_xor((int)(unsigned)(u >> 32), lane, m);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
 }
 
 template <typename T> __device__ __forceinline__ T dexp(T x);
@@ -215,8 +256,8 @@ __device__ __forceinline__ void bitonic_sort_kd(double (&k)[E], int (&x)[E], int
       int ox[E];
 #pragma unroll
       for (int q = 0; q < E; q++) {
-        ok[q] = __shfl_xor(k[E - 1 - q], lm);
-        ox[q] = __shfl_xor(x[E - 1 - q], lm);
+        ok[q] = bperm_xor(k[E - 1 - q], lane, lm);
+        ox[q] = bperm_xor(x[E - 1 - q], lane, lm);
       }
 #pragma unroll
       for (int q = 0; q < E; q++) {
@@ -247,8 +288,8 @@ __device__ __forceinline__ void bitonic_sort_kd(double (&k)[E], int (&x)[E], int
         const bool lower = ((lane >> (js - LOGE)) & 1) == 0;
 #pragma unroll
         for (int q = 0; q < E; q++) {
-          const double ok = __shfl_xor(k[q], lm);
-          const int ox = __shfl_xor(x[q], lm);
+          const double ok = bperm_xor(k[q], lane, lm);
+          const int ox = bperm_xor(x[q], lane, lm);
           const bool take = kd_lt(ok, ox, k[q], x[q]) == lower;
           k[q] = take ? ok : k[q];
           x[q] = take ? ox : x[q];
@@ -290,7 +331,7 @@ __device__ __forceinline__ int dpp_mov(int x) {
   return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
 }
 template <int M>
-__device__ __forceinline__ unsigned xlane(unsigned v) {
+__device__ __forceinline__ unsigned xlane(unsigned v, int lane) {
   const int x = (int)v;
   if constexpr (M == 1) return (unsigned)dpp_mov<0xB1>(x);                  // quad_perm 1,0,3,2
   else if constexpr (M == 2) return (unsigned)dpp_mov<0x4E>(x);             // quad_perm 2,3,0,1
@@ -301,7 +342,7 @@ __device__ __forceinline__ unsigned xlane(unsigned v) {
   else if constexpr (M == 15) return (unsigned)dpp_mov<0x140>(x);           // row_mirror
   else if constexpr (M == 16 || M == 31)
     return (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x1F | (M << 10));
-  else return (unsigned)__shfl_xor(x, M);
+  else return (unsigned)bperm_xor(x, lane, M);
 }
 
 // Exchange across lanes: the lower lane of a pair keeps the smaller value,
@@ -327,7 +368,7 @@ __device__ __forceinline__ void u32_half_cleaners(unsigned (&v)[E], int lane) {
   if constexpr (JS >= LOGE) {
     const unsigned c = lane_bit_mask(lane, JS - LOGE);
 #pragma unroll
-    for (int q = 0; q < E; q++) v[q] = med3_u32(v[q], xlane<(1 << (JS - LOGE))>(v[q]), c);
+    for (int q = 0; q < E; q++) v[q] = med3_u32(v[q], xlane<(1 << (JS - LOGE))>(v[q], lane), c);
   } else {
 #pragma unroll
     for (int q = 0; q < E; q++) {
@@ -351,7 +392,7 @@ __device__ __forceinline__ void u32_merge_levels(unsigned (&v)[E], int lane) {
     const unsigned c = lane_bit_mask(lane, KS - 1 - LOGE);
     unsigned o[E];
 #pragma unroll
-    for (int q = 0; q < E; q++) o[q] = xlane<(((1 << KS) - 1) >> LOGE)>(v[E - 1 - q]);
+    for (int q = 0; q < E; q++) o[q] = xlane<(((1 << KS) - 1) >> LOGE)>(v[E - 1 - q], lane);
 #pragma unroll
     for (int q = 0; q < E; q++) v[q] = med3_u32(v[q], o[q], c);
   } else {
@@ -1114,7 +1155,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   }
   bad |= (cprev >= LA);
   if (__any(bad)) return false;
-  const int gtot = __shfl(b0 + cprev, W - 1);  // edges at or below the total weight
+  const int gtot = __builtin_amdgcn_readlane(b0 + cprev, W - 1);  // edges at or below the total weight
 
   // column prefix tables: the header's Gw and the new gas's two sums
   {
@@ -1473,7 +1514,7 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   for (int x = lane; x < NG; x += W) sE[x] = 0;  // bin record types, cleared per round
   // Row state: slot 0 = K, slot 1+s = gas s, slot 1+ngas = dK/dT.  Round 1
   // starts from gas 0 scaled by its amount.
-  const double *amo = a.amo + R.lg0 + (long long)j * ngas;
+  const double *amo = a.amo + rfl64(R.lg0) + (long long)j * ngas;
   {
     const T a1 = (T)amo[0];
     for (int x = lane; x < NS * NG; x += W) {
@@ -1501,9 +1542,9 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
     // round's uniform offsets and masks and its lane indices are formed from
     // them inside the round (scalar and cheap) instead of being hoisted out
     // of the loop and held, spilled, across it.
-    int ng_r = NG, ngas_r = ngas_all, lane_r = lane;
-    asm volatile("" : "+s"(ng_r), "+s"(ngas_r), "+v"(lane_r));
-    const int NG = ng_r, ngas = ngas_r, lane = lane_r, NS = ngas + 2;
+    int ng_r = NG, ngas_r = ngas_all;
+    asm volatile("" : "+s"(ng_r), "+s"(ngas_r));
+    const int NG = ng_r, ngas = ngas_r, lane = lane_fresh(), NS = ngas + 2;
     // this lane's sort slots: row li of the running distribution, columns
     // lj0 .. lj0 + E - 1 of the new gas (layer_lpr)
     const int lpr = layer_lpr(NG, E);
@@ -1802,9 +1843,13 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
   }
 
   // ---- the layer's k-distribution and derivatives ------------------------
+  // (the lane number and 1/NG formed again here, not held across the rounds)
   const int nks = R.nks;
-  for (int x = lane; x < NS * NG; x += W) {
-    const int sl = (int)(((float)x + 0.5f) * rng), i = x - sl * NG;
+  int ng_o = NG;
+  asm volatile("" : "+s"(ng_o));
+  const float rng_o = rcp_idx(ng_o);
+  for (int x = lane_fresh(); x < NS * NG; x += W) {
+    const int sl = (int)(((float)x + 0.5f) * rng_o), i = x - sl * NG;
     const T val = rows[x];
     const int ks = (sl == 0) ? -2 : (sl == NS - 1) ? nks : (int)R.kslot[sl - 1];
     if (ks == -2) ok[(long long)i * ostride + j] = val;
@@ -1899,11 +1944,11 @@ __global__ __launch_bounds__(256, (E >= 16 ? 1 : (E == 8 && sizeof(T) == 4) ? 6 
   __syncthreads();
   const long long item = xcd_item(nblocks) * a.lwaves + wv;
   if (item >= a.nA) return;
-  const int rqc = find_req(a.startA, a.nreqA, item, a.nA);
+  const int rqc = find_req<true>(a.startA, a.nreqA, item, a.nA);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, NG = a.ng;
   const long long loc = item - a.startA[rqc];
-  const int iw = div_item(loc, L);
+  const int iw = div_item<true>(loc, L);
   const int j = (int)(loc - (long long)iw * L);
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
   layer_item<T, E, BF>(a, R, iw, j, hdr,
@@ -1917,11 +1962,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void cia_kernel(KernelArgs a) {
   const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= a.nA) return;
-  const int rqc = find_req(a.startA, a.nreqA, item, a.nA);
+  const int rqc = find_req<false>(a.startA, a.nreqA, item, a.nA);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng;
   const long long loc = item - a.startA[rqc];
-  const int iw = div_item(loc, L);
+  const int iw = div_item<false>(loc, L);
   const int j = (int)(loc - (long long)iw * L);
   T *inter = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(NG, L, R.nks, R.nparam);
   T *tcon = inter + (long long)NG * L + (long long)NG * (R.nks + 1) * L;
@@ -1940,11 +1985,11 @@ template <typename T, int BF>
 __global__ __launch_bounds__(256) void lbl_kernel(KernelArgs a) {
   const long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (item >= a.nA) return;
-  const int rqc = find_req(a.startA, a.nreqA, item, a.nA);
+  const int rqc = find_req<false>(a.startA, a.nreqA, item, a.nA);
   const DevReq &R = a.reqs[a.reqmapA[rqc]];
   const int L = R.L, ngas = R.ngas, nks = R.nks;
   const long long loc = item - a.startA[rqc];
-  const int iw = div_item(loc, L);
+  const int iw = div_item<false>(loc, L);
   const int j = (int)(loc - (long long)iw * L);
   T *ok = reinterpret_cast<T *>(a.inter) + R.inter0 + (long long)iw * inter_block(1, L, nks, R.nparam);
   T *od = ok + L;
@@ -2518,11 +2563,11 @@ __global__ __launch_bounds__(64) void emission_kernel(KernelArgs a, long long nb
   const long long item = xcd_item(nblocks);
   if (item >= a.nB) return;
   const int lane = threadIdx.x;
-  const int rqc = find_req(a.startB, a.nreqB, item, a.nB);
+  const int rqc = find_req<true>(a.startB, a.nreqB, item, a.nB);
   const DevReq &R = a.reqs[a.reqmapB[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng, npath = R.npath;
   const long long loc = item - a.startB[rqc];
-  const int iw = div_item(loc, npath);
+  const int iw = div_item<true>(loc, npath);
   const int p = (int)(loc - (long long)iw * npath);
   const T *inter = reinterpret_cast<const T *>(a.inter) + R.inter0 +
                    (long long)iw * inter_block(NG, L, R.nks, R.nparam);
@@ -2545,7 +2590,7 @@ __global__ __launch_bounds__(64 * RADG_FUSED_WAVES) void fused_kernel(KernelArgs
   if (item >= a.nB) return;
   const int tid = threadIdx.x;
   const int wv = tid / W, lane = tid - wv * W;
-  const int rqc = find_req(a.startB, a.nreqB, item, a.nB);
+  const int rqc = find_req<true>(a.startB, a.nreqB, item, a.nB);
   const DevReq &R = a.reqs[a.reqmapB[rqc]];
   const int L = R.L, ngas = R.ngas, NG = a.ng;
   const int iw = (int)(item - a.startB[rqc]);
@@ -2575,7 +2620,7 @@ __global__ __launch_bounds__(64) void quadrature_kernel(KernelArgs a) {
   const long long item = blockIdx.x;
   if (item >= a.nC) return;
   const int lane = threadIdx.x;
-  const int rq = find_req(a.startC, a.nreq, item, a.nC);
+  const int rq = find_req<true>(a.startC, a.nreq, item, a.nC);
   const DevReq &R = a.reqs[rq];
   const int iw = (int)(item - R.itemC0);
   const int L = R.L, npath = R.npath, nparam = R.nparam;

@@ -99,6 +99,20 @@ __device__ __forceinline__ int lane_fresh() {
   return l;
 }
 
+// Zeros formed where they are stored or used (an asm the compiler does not
+// merge), rather than one zero register held across the overlap rounds,
+// which the register budget of the layer kernel otherwise spills.
+__device__ __forceinline__ double zero_d() {
+  double z;
+  asm volatile("v_mov_b64 %0, 0" : "=v"(z));
+  return z;
+}
+__device__ __forceinline__ int zero_i() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 // Lane (lane ^ m)'s value by ds_bpermute, with the address formed from the
 // caller's lane number: a shuffle that forms it from its own lane id has
 // that id hoisted out of the overlap-round loop and spilled across it.
@@ -1128,7 +1142,7 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   // most once with Gauss-Legendre weights).  Padding slots write gamma
   // bytes past a row's NG real ones, which the counts below mask.
   constexpr int LA = 2;
-  int b0 = 0;
+  int b0 = zero_i();
 #pragma unroll
   for (int st = 32; st > 0; st >>= 1)
     if (b0 + st <= NG && G[b0 + st] <= gprev) b0 += st;
@@ -1169,9 +1183,10 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
       ctab[3 * lane + 5] = id;
     }
     if (lane == 0) {
-      ctab[0] = 0.0;
-      ctab[1] = 0.0;
-      ctab[2] = 0.0;
+      const double z = zero_d();
+      ctab[0] = z;
+      ctab[1] = z;
+      ctab[2] = z;
     }
   }
   // lane = edge bl + 1 of row group grp, rows i0 + t, t < RPG: rows with
@@ -1287,7 +1302,10 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
     nxt[bl] = reached ? (T)((QKf - pK) * inv) : (T)0;
     nxt[(1 + r) * NG + bl] = reached ? (T)((QNf - pN) * inv) : (T)0;
     nxt[(NS - 1) * NG + bl] = reached ? (T)((QTf - pT) * inv) : (T)0;
-    for (int s4 = r + 1; s4 < ngas; s4++) nxt[(1 + s4) * NG + bl] = (T)0;
+    if (r + 1 < ngas) {
+      const T z = (T)zero_d();
+      for (int s4 = r + 1; s4 < ngas; s4++) nxt[(1 + s4) * NG + bl] = z;
+    }
   }
   // the payloads of the gases already mixed: row terms only
   for (int s = 0; s < r; s++) {

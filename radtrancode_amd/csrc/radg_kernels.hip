@@ -1135,41 +1135,34 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   }
   const double gprev = wave_shr1(gd[E - 1]);
   // gamma: b0 edges of G[1..NG] at or below the lane's start (a search),
-  // then for each slot b0 plus the number of the next LA edges at or below
-  // its end.  A slot whose count rises straddles the edge it reaches
-  // (rankg.f:120); a rise of two, or a lane reaching its LA-th edge, hands
-  // the round to the record rebin (a lane's 8 elements span a bin edge at
-  // most once with Gauss-Legendre weights).  Padding slots write gamma
-  // bytes past a row's NG real ones, which the counts below mask.
-  constexpr int LA = 2;
+  // then for each slot b0, plus 1 once its end reaches the next edge
+  // G[b0 + 1].  The slot where that first happens straddles the edge
+  // (rankg.f:120).  A lane whose end reaches G[b0 + 2] as well (two edges
+  // within a lane, or within one element) hands the round to the record
+  // rebin; with Gauss-Legendre weights a lane's 8 elements span a bin edge
+  // at most once.  Padding slots write gamma bytes past a row's NG real
+  // ones, which the counts below mask.
   int b0 = zero_i();
 #pragma unroll
   for (int st = 32; st > 0; st >>= 1)
     if (b0 + st <= NG && G[b0 + st] <= gprev) b0 += st;
-  double ge[LA];
-#pragma unroll
-  for (int t = 0; t < LA; t++) ge[t] = G[b0 + 1 + t];
-  int cprev = 0;
-  bool bad = false;
+  const double ge0 = G[b0 + 1], ge1 = G[b0 + 2];
+  const bool bad = ge1 <= gd[E - 1];
+  bool sprev = false;
 #pragma unroll
   for (int q = 0; q < E; q++) {
-    const double x = gd[q];
-    int c = 0;
-#pragma unroll
-    for (int t = 0; t < LA; t++) c += (ge[t] <= x) ? 1 : 0;
-    if (c > cprev) {
-      const int e = b0 + cprev;  // edge e + 1, recorded at e
-      recI[e] = idx[q];
-      recGs[e] = (q == 0) ? gprev : gd[q - 1];
-      recGe[e] = x;
+    const bool sq = ge0 <= gd[q];
+    if (sq && !sprev) {  // edge b0 + 1, recorded at b0
+      recI[b0] = idx[q];
+      recGs[b0] = (q == 0) ? gprev : gd[q - 1];
+      recGe[b0] = gd[q];
     }
-    bad |= (c > cprev + 1);
-    cprev = c;
-    gam[idx[q]] = (unsigned char)(b0 + c);
+    sprev = sq;
+    gam[idx[q]] = (unsigned char)(b0 + (sq ? 1 : 0));
   }
-  bad |= (cprev >= LA);
   if (__any(bad)) return false;
-  const int gtot = __builtin_amdgcn_readlane(b0 + cprev, W - 1);  // edges at or below the total weight
+  // edges at or below the total weight
+  const int gtot = __builtin_amdgcn_readlane(b0 + (sprev ? 1 : 0), W - 1);
 
   // column prefix tables: the header's Gw and the new gas's two sums
   {

@@ -435,60 +435,54 @@ __device__ __forceinline__ void bitonic_sort_u32_lanesorted(unsigned (&v)[E], in
   u32_merge_levels<E, ilog2c(E) + 1>(v, lane);
 }
 
-__device__ __forceinline__ bool kf_lt(float ka, int ia, float kb, int ib) {
-  return (ka < kb) | ((ka == kb) & (ia < ib));
-}
-
 // After the network has ordered the truncated words (key bits with the low
 // log2(64E) mantissa bits replaced by the element number), slots can only be
 // out of exact (key, element) order inside a run of equal truncated keys.
-// Odd-even transposition on the exact keys, until no adjacent pair is out of
-// order (a run of m slots needs at most m phases; the bound W*E is never
-// reached on a sorted input).
-// lane + 1's and lane - 1's value of a 32-bit quantity (DPP wave shifts;
-// 0 where there is no such lane)
+// Each slot's exact pair is one 64-bit word, key bits high and element low:
+// FP32 keys are >= 0, so the words order like the pairs, and a pair compare
+// is one 64-bit compare.  Odd-even transposition on the words, until no
+// adjacent pair is out of order (a run of m slots needs at most m phases;
+// the bound W*E is never reached on a sorted input).
+// lane + 1's and lane - 1's value (DPP wave shifts; 0 where there is no such
+// lane)
 __device__ __forceinline__ int lane_next(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xF, 0xF, true); }
 __device__ __forceinline__ int lane_prev(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xF, 0xF, true); }
-__device__ __forceinline__ float lane_next(float x) { return __int_as_float(lane_next(__float_as_int(x))); }
-__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(lane_prev(__float_as_int(x))); }
+__device__ __forceinline__ unsigned long long lane_next(unsigned long long x) {
+  return ((unsigned long long)(unsigned)lane_next((int)(unsigned)(x >> 32)) << 32) |
+         (unsigned)lane_next((int)(unsigned)x);
+}
+__device__ __forceinline__ unsigned long long lane_prev(unsigned long long x) {
+  return ((unsigned long long)(unsigned)lane_prev((int)(unsigned)(x >> 32)) << 32) |
+         (unsigned)lane_prev((int)(unsigned)x);
+}
+// (key, element) as one word
+__device__ __forceinline__ unsigned long long key_word(float k, unsigned e) {
+  return ((unsigned long long)__float_as_uint(k) << 32) | e;
+}
 
 template <int E>
-__device__ __forceinline__ void exact_fixup(float (&kx)[E], int (&ex)[E], int lane) {
+__device__ __forceinline__ void exact_fixup(unsigned long long (&w)[E], int lane) {
   auto unsorted = [&]() {
     bool bad = false;
 #pragma unroll
-    for (int q = 0; q + 1 < E; q++) bad |= kf_lt(kx[q + 1], ex[q + 1], kx[q], ex[q]);
-    const float nk = lane_next(kx[0]);
-    const int ne = lane_next(ex[0]);
-    bad |= (lane + 1 < W) && kf_lt(nk, ne, kx[E - 1], ex[E - 1]);
+    for (int q = 0; q + 1 < E; q++) bad |= w[q + 1] < w[q];
+    const unsigned long long nw = lane_next(w[0]);
+    bad |= (lane + 1 < W) && nw < w[E - 1];
     return __any(bad);
   };
   auto in_lane = [&](int q) {
-    if (kf_lt(kx[q + 1], ex[q + 1], kx[q], ex[q])) {
-      const float tk = kx[q];
-      const int te = ex[q];
-      kx[q] = kx[q + 1];
-      ex[q] = ex[q + 1];
-      kx[q + 1] = tk;
-      ex[q + 1] = te;
-    }
+    const unsigned long long a = w[q], b = w[q + 1];
+    const bool sw = b < a;
+    w[q] = sw ? b : a;
+    w[q + 1] = sw ? a : b;
   };
   // pair (slot E-1 of lane l, slot 0 of lane l+1) for every l with en(l)
   auto cross = [&](auto en) {
-    const float nk = lane_next(kx[0]);
-    const int ne = lane_next(ex[0]);
-    const float pk = lane_prev(kx[E - 1]);
-    const int pe = lane_prev(ex[E - 1]);
-    const bool lo = (lane + 1 < W) && en(lane) && kf_lt(nk, ne, kx[E - 1], ex[E - 1]);
-    const bool hi = (lane > 0) && en(lane - 1) && kf_lt(kx[0], ex[0], pk, pe);
-    if (lo) {
-      kx[E - 1] = nk;
-      ex[E - 1] = ne;
-    }
-    if (hi) {
-      kx[0] = pk;
-      ex[0] = pe;
-    }
+    const unsigned long long nw = lane_next(w[0]), pw = lane_prev(w[E - 1]);
+    const bool lo = (lane + 1 < W) && en(lane) && nw < w[E - 1];
+    const bool hi = (lane > 0) && en(lane - 1) && w[0] < pw;
+    if (lo) w[E - 1] = nw;
+    if (hi) w[0] = pw;
   };
   for (int it = 0; it < W * E && unsorted(); it++) {
     if constexpr (E == 1) {
@@ -1459,9 +1453,14 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
         unsigned pk[1] = {(__float_as_uint(kq) & ~(unsigned)(W - 1)) | (unsigned)lane};
         bitonic_sort_u32<1>(pk, lane);
         wave_sync();
-        idx[0] = (int)(pk[0] & (unsigned)(W - 1));
-        key[0] = keyM[idx[0]];
-        exact_fixup<1>(key, idx, lane);
+        unsigned long long w[1];
+        {
+          const unsigned e = pk[0] & (unsigned)(W - 1);
+          w[0] = key_word(keyM[e], e);
+        }
+        exact_fixup<1>(w, lane);
+        idx[0] = (int)(unsigned)w[0];
+        key[0] = __uint_as_float((unsigned)(w[0] >> 32));
       } else {
         key[0] = act ? x : dinf<T>();
         idx[0] = lane;
@@ -1620,12 +1619,15 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
           if (k2up) bitonic_sort_u32_lanesorted<E>(pk, lane);
           else bitonic_sort_u32<E>(pk, lane);
           wave_sync();
+          unsigned long long w[E];
 #pragma unroll
           for (int q = 0; q < E; q++) {
-            idx[q] = (int)(pk[q] & IM);
-            key[q] = keyE[idx[q]];
+            const unsigned e = pk[q] & IM;
+            w[q] = key_word(keyE[e], e);
           }
-          exact_fixup<E>(key, idx, lane);
+          exact_fixup<E>(w, lane);
+#pragma unroll
+          for (int q = 0; q < E; q++) idx[q] = (int)(unsigned)w[q];
         } else {
 #pragma unroll
           for (int q = 0; q < E; q++) {

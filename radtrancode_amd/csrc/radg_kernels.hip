@@ -1181,7 +1181,11 @@ __device__ __forceinline__ bool rebin_counts(const int NG, const int ngas, const
   // whatever follows (their counts are clamped to NG)
   const int ngrp = count_groups(NG);
   const int rpg = (NG + ngrp - 1) / ngrp;
-  const int grp = (int)(((float)lane + 0.5f) * rcp_idx(NG));
+  // (1/NG from an opaque copy of NG, so that it is formed here rather than
+  // merged with the round's and held across the sort)
+  int ng_g = NG;
+  asm volatile("" : "+s"(ng_g));
+  const int grp = (int)(((float)lane + 0.5f) * rcp_idx(ng_g));
   const int bl = lane - (int)__umul24(grp, NG);
   const bool act = grp < ngrp;
   const int i0 = act ? (int)__umul24(grp, rpg) : 0;

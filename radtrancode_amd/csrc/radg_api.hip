@@ -1036,8 +1036,11 @@ namespace {
 // Caching allocators.  Batches come and go at serving rates; a fresh
 // hipMalloc / hipHostMalloc of hundreds of MB per batch would dominate, so
 // freed blocks are kept (up to a cap) and handed out again for requests of
-// between half and all of their size.  The caches live until the process
-// ends.
+// between half and all of their size.  A block returned to a full cache
+// evicts cached blocks, largest first, to make room, so that the blocks of
+// an earlier, larger workload (which the current one cannot use) do not
+// hold the cache and send every batch back to hipHostMalloc.  The caches
+// live until the process ends.
 struct BlockCache {
   std::mutex mu;
   std::multimap<size_t, void *> freed;
@@ -1090,6 +1093,13 @@ struct BlockCache {
     if (it == live.end()) return;
     const size_t sz = it->second;
     live.erase(it);
+    while (cached + sz > cap && !freed.empty()) {
+      auto big = std::prev(freed.end());
+      if (pinned) (void)hipHostFree(big->second);
+      else (void)hipFree(big->second);
+      cached -= big->first;
+      freed.erase(big);
+    }
     if (cached + sz <= cap) {
       freed.insert({sz, p});
       cached += sz;

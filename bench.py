@@ -169,6 +169,25 @@ def kernel_source_hash():
     return h.hexdigest()[:16]
 
 
+def network_op_count(ng, nkgas):
+    """The overlap's sort arithmetic per (wavenumber, layer) item, from the
+    algorithm alone: each of the nkgas - 1 random-overlap rounds sorts 64 E
+    slots (E per lane, 64 E >= ng^2) with a bitonic network whose in-lane
+    merge levels a lane's already sorted row skips; every stage is
+    64 E / 2 compare-exchanges, each a min and a max."""
+    import math
+
+    E = 1
+    while 64 * E < ng * ng or ng * (-(-ng // E)) > 64:
+        E *= 2
+    levels, skip = int(math.log2(64 * E)), int(math.log2(E))
+    stages = levels * (levels + 1) // 2 - skip * (skip + 1) // 2
+    ce = stages * 64 * E // 2 * max(0, nkgas - 1)
+    return {"slots": 64 * E, "network_stages_per_round": stages,
+            "compare_exchanges_per_item": ce,
+            "minmax_wave_instructions_per_item": 2 * ce / 64}
+
+
 def pmc_figures(batch, t_layer_s):
     """HBM bytes per launch and VALU issue of the layer kernel from the
     committed rocprofv3 PMC passes (profiles/layer_kernel_traffic.json), when
@@ -187,6 +206,11 @@ def pmc_figures(batch, t_layer_s):
                                   "kernel sources (" + str(p.get("kernel_source_sha256")) + ")"}
     traffic = (p["fetch_kib"] * p["fetch_scale"] + p["write_kib"] * p["write_scale"]) * 1024
     rate = p["valu_wave_instructions"] / t_layer_s
+    # the operation count beside the issue rate: the sort's compare-exchange
+    # arithmetic (20 g-ordinates, 3 k-gases) against the issued instructions
+    ops = network_op_count(20, 3)
+    if p.get("valu_per_item"):
+        ops["frac_of_issued_valu"] = ops["minmax_wave_instructions_per_item"] / p["valu_per_item"]
     return {"traffic": traffic,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (" + p["source"] + ")",
             "kernel_source_sha256": p["kernel_source_sha256"],
@@ -194,7 +218,8 @@ def pmc_figures(batch, t_layer_s):
                            "instructions_per_item": p.get("valu_per_item"),
                            "per_item_breakdown": p.get("valu_per_item_breakdown"),
                            "achieved_per_s": rate, "peak_per_s": p["valu_issue_peak_per_s"],
-                           "frac": rate / p["valu_issue_peak_per_s"]}}
+                           "frac": rate / p["valu_issue_peak_per_s"]},
+            "op_count": ops}
 
 
 def layer_sweep():

@@ -435,6 +435,74 @@ __device__ __forceinline__ void bitonic_sort_u32_lanesorted(unsigned (&v)[E], in
   u32_merge_levels<E, ilog2c(E) + 1>(v, lane);
 }
 
+// The same network on 64-bit words (the FP64 form's truncated keys): the
+// lane moves of xlane on both halves, and one 64-bit compare per exchange.
+template <int M>
+__device__ __forceinline__ unsigned long long xlane64(unsigned long long v, int lane) {
+  const unsigned lo = xlane<M>((unsigned)v, lane), hi = xlane<M>((unsigned)(v >> 32), lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
+// the lower lane of a pair keeps the smaller word, the upper lane the larger
+__device__ __forceinline__ unsigned long long cx64(unsigned long long v, unsigned long long o,
+                                                   bool upper) {
+  return ((o < v) != upper) ? o : v;
+}
+template <int E, int JS>
+__device__ __forceinline__ void u64_half_cleaners(unsigned long long (&v)[E], int lane) {
+  constexpr int LOGE = ilog2c(E);
+  if constexpr (JS >= LOGE) {
+    const bool up = ((lane >> (JS - LOGE)) & 1) != 0;
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = cx64(v[q], xlane64<(1 << (JS - LOGE))>(v[q], lane), up);
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int q2 = q ^ (1 << JS);
+      if (q2 > q) {
+        const unsigned long long x = v[q], y = v[q2];
+        const bool sw = y < x;
+        v[q] = sw ? y : x;
+        v[q2] = sw ? x : y;
+      }
+    }
+  }
+  if constexpr (JS > 0) u64_half_cleaners<E, JS - 1>(v, lane);
+}
+template <int E, int KS>
+__device__ __forceinline__ void u64_merge_levels(unsigned long long (&v)[E], int lane) {
+  constexpr int LOGE = ilog2c(E);
+  constexpr int LOGP = 6 + LOGE;
+  if constexpr (KS - 1 >= LOGE) {
+    const bool up = ((lane >> (KS - 1 - LOGE)) & 1) != 0;
+    unsigned long long o[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) o[q] = xlane64<(((1 << KS) - 1) >> LOGE)>(v[E - 1 - q], lane);
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = cx64(v[q], o[q], up);
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int q2 = q ^ ((1 << KS) - 1);
+      if (q2 > q) {
+        const unsigned long long x = v[q], y = v[q2];
+        const bool sw = y < x;
+        v[q] = sw ? y : x;
+        v[q2] = sw ? x : y;
+      }
+    }
+  }
+  if constexpr (KS >= 2) u64_half_cleaners<E, KS - 2>(v, lane);
+  if constexpr (KS < LOGP) u64_merge_levels<E, KS + 1>(v, lane);
+}
+template <int E>
+__device__ __forceinline__ void bitonic_sort_u64(unsigned long long (&v)[E], int lane) {
+  u64_merge_levels<E, 1>(v, lane);
+}
+template <int E>
+__device__ __forceinline__ void bitonic_sort_u64_lanesorted(unsigned long long (&v)[E], int lane) {
+  u64_merge_levels<E, ilog2c(E) + 1>(v, lane);
+}
+
 // After the network has ordered the truncated words (key bits with the low
 // log2(64E) mantissa bits replaced by the element number), slots can only be
 // out of exact (key, element) order inside a run of equal truncated keys.
@@ -495,6 +563,61 @@ __device__ __forceinline__ void exact_fixup(unsigned long long (&w)[E], int lane
       for (int q = 1; q + 1 < E; q += 2) in_lane(q);
       cross([](int) { return true; });
     }
+  }
+}
+
+// The FP64 form's exact (key, slot) order inside runs of equal truncated
+// keys: odd-even transposition on (double key, slot) pairs, as above.
+__device__ __forceinline__ double lane_next(double x) {
+  return __longlong_as_double((long long)lane_next((unsigned long long)__double_as_longlong(x)));
+}
+__device__ __forceinline__ double lane_prev(double x) {
+  return __longlong_as_double((long long)lane_prev((unsigned long long)__double_as_longlong(x)));
+}
+__device__ __forceinline__ bool kd_pair_lt(double ka, int ia, double kb, int ib) {
+  return (ka < kb) | ((ka == kb) & (ia < ib));
+}
+template <int E>
+__device__ __forceinline__ void exact_fixup_d(double (&kx)[E], int (&ex)[E], int lane) {
+  auto unsorted = [&]() {
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q + 1 < E; q++) bad |= kd_pair_lt(kx[q + 1], ex[q + 1], kx[q], ex[q]);
+    const double nk = lane_next(kx[0]);
+    const int ne = lane_next(ex[0]);
+    bad |= (lane + 1 < W) && kd_pair_lt(nk, ne, kx[E - 1], ex[E - 1]);
+    return __any(bad);
+  };
+  auto in_lane = [&](int q) {
+    if (kd_pair_lt(kx[q + 1], ex[q + 1], kx[q], ex[q])) {
+      const double tk = kx[q];
+      const int te = ex[q];
+      kx[q] = kx[q + 1];
+      ex[q] = ex[q + 1];
+      kx[q + 1] = tk;
+      ex[q + 1] = te;
+    }
+  };
+  auto cross = [&]() {
+    const double nk = lane_next(kx[0]), pk = lane_prev(kx[E - 1]);
+    const int ne = lane_next(ex[0]), pe = lane_prev(ex[E - 1]);
+    const bool lo = (lane + 1 < W) && kd_pair_lt(nk, ne, kx[E - 1], ex[E - 1]);
+    const bool hi = (lane > 0) && kd_pair_lt(kx[0], ex[0], pk, pe);
+    if (lo) {
+      kx[E - 1] = nk;
+      ex[E - 1] = ne;
+    }
+    if (hi) {
+      kx[0] = pk;
+      ex[0] = pe;
+    }
+  };
+  for (int it = 0; it < W * E && unsorted(); it++) {
+#pragma unroll
+    for (int q = 0; q + 1 < E; q += 2) in_lane(q);
+#pragma unroll
+    for (int q = 1; q + 1 < E; q += 2) in_lane(q);
+    cross();
   }
 }
 
@@ -1632,6 +1755,52 @@ __device__ __forceinline__ void layer_item(const KernelArgs &a, const DevReq &R,
           exact_fixup<E>(w, lane);
 #pragma unroll
           for (int q = 0; q < E; q++) idx[q] = (int)(unsigned)w[q];
+        } else if constexpr (E >= 2) {
+          // FP64 keys are >= 0 too: one 64-bit word per slot, the key's bits
+          // with the low log2(64E) mantissa bits replaced by the slot, sorted
+          // by the same network.  Two distinct keys can tie in the truncated
+          // bits only within 2^-43 of each other; a wavefront that has such a
+          // pair forms the exact keys again from the slots and settles the
+          // (key, slot) order on them.
+          constexpr unsigned long long IM = (unsigned long long)(W * E - 1);
+          constexpr unsigned long long INFB = 0x7FF0000000000000ull & ~IM;
+          unsigned long long wd[E];
+#pragma unroll
+          for (int q = 0; q < E; q++) {
+            const int jx = lj0 + q;
+            double kq = dinf<double>();
+            if (lrow && jx < NG) {
+              kq = fma((double)k2[jx], (double)a2, (double)K);
+              kq = (kq > 0.0) ? kq : 0.0;  // -0 and 0 order alike
+            }
+            wd[q] = ((unsigned long long)__double_as_longlong(kq) & ~IM) | (unsigned long long)(lane * E + q);
+          }
+          if (k2up) bitonic_sort_u64_lanesorted<E>(wd, lane);
+          else bitonic_sort_u64<E>(wd, lane);
+          auto tie = [&](unsigned long long x, unsigned long long y) {
+            return ((x ^ y) & ~IM) == 0 && (x & ~IM) != INFB;
+          };
+          bool t = false;
+#pragma unroll
+          for (int q = 0; q + 1 < E; q++) t |= tie(wd[q], wd[q + 1]);
+          t |= (lane + 1 < W) && tie(wd[E - 1], lane_next(wd[0]));
+#pragma unroll
+          for (int q = 0; q < E; q++) idx[q] = (int)(wd[q] & IM);
+          if (__any(t)) {
+            double kx[E];
+#pragma unroll
+            for (int q = 0; q < E; q++) {
+              int i, jx;
+              slot_ij(idx[q], i, jx);
+              double kq = dinf<double>();
+              if (i < NG && jx < NG) {
+                kq = fma((double)k2[jx], (double)a2, (double)rows[i]);
+                kq = (kq > 0.0) ? kq : 0.0;
+              }
+              kx[q] = kq;
+            }
+            exact_fixup_d<E>(kx, idx, lane);
+          }
         } else {
 #pragma unroll
           for (int q = 0; q < E; q++) {

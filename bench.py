@@ -218,7 +218,10 @@ def pmc_figures(batch, t_layer_s):
                            "instructions_per_item": p.get("valu_per_item"),
                            "per_item_breakdown": p.get("valu_per_item_breakdown"),
                            "achieved_per_s": rate, "peak_per_s": p["valu_issue_peak_per_s"],
-                           "frac": rate / p["valu_issue_peak_per_s"]},
+                           "frac": rate / p["valu_issue_peak_per_s"],
+                           "peak_note": "nominal: one wave64 VALU instruction per 4 cycles per "
+                                        "SIMD at 2.4 GHz; not a ceiling for every mix (the FP64 "
+                                        "form's layer kernel issues 656 G/s at 2.33 GHz)"},
             "op_count": ops}
 
 
